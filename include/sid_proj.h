@@ -1,0 +1,141 @@
+/*
+ * sid_proj.h — C ABI of the MI355X (gfx950) light-field projection library
+ * (libsidproj.so).  Replaces SID's MATLAB projectors and ISRA loop:
+ *
+ *   forwardProjectGPU(H, realspace)          external/LFrecon_util/forwardProjectGPU.m:1
+ *   backwardProjectGPU_new(H, projection)    external/LFrecon_util/backwardProjectGPU_new.m:1
+ *   forwardProjectACC(H, realspace, CAindex) external/LFrecon_util/forwardProjectACC.m:1
+ *   backwardProjectACC_new(H, proj, CAindex) external/LFrecon_util/backwardProjectACC_new.m:1
+ *   deconvRL(F, B, Htf, maxIter, Xguess)     external/LFrecon_util/deconvRL.m:1 (basic, lambda 0)
+ *
+ * Layout contract: every array is MATLAB column-major, exactly as a MEX
+ * gateway receives it (mxGetData):
+ *   H        [M, M, Nnum, Nnum, Nz]  single   H(i,j,aa,bb,cc) at i + M*(j + M*(aa + Nnum*(bb + Nnum*cc)))
+ *   volume   [X, Y, nz]              single   V(x,y,c) at x + X*(y + Y*c)
+ *   image    [X, Y]                  single   O(x,y) at x + X*y
+ *   CAindex  [Nz, 2]                 int32    1-based inclusive crop, CAindex(c,k) at c + Nz*(k-1)
+ *
+ * Ownership: the caller owns every input and output buffer.  The library owns
+ * the device-resident PSF state behind an opaque sid_psf handle (H is uploaded
+ * once per handle, not once per call as forwardProjectGPU.m:19 does).
+ *
+ * Errors: every entry point returns a sid_status; sid_last_error() gives the
+ * message of the last failure on the calling thread.  A handle is used by one
+ * thread at a time.  All calls are synchronous: when one returns, its outputs
+ * are written (device outputs included).
+ *
+ * "stream" arguments take a hipStream_t cast to void* (NULL = the handle's
+ * own stream); device-pointer entry points read and write device memory.
+ */
+#ifndef SID_PROJ_H
+#define SID_PROJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sid_psf sid_psf;
+
+typedef enum {
+    SID_OK = 0,
+    SID_ERR_ARG = 1,        /* null pointer, bad option */
+    SID_ERR_DIMS = 2,       /* inconsistent or unsupported dimensions */
+    SID_ERR_NZ = 3,         /* volume has more planes than H (forwardProjectGPU.m:18-19) */
+    SID_ERR_OOM = 4,        /* device allocation failed */
+    SID_ERR_HIP = 5,        /* HIP runtime error */
+    SID_ERR_STATE = 6,      /* call not valid in the handle's state */
+    SID_ERR_NODEV = 7       /* no usable gfx950 device */
+} sid_status;
+
+/* Arithmetic of the projector GEMMs. */
+typedef enum {
+    SID_PREC_FP32 = 0,      /* plain method: FP32 operands, FP32 MFMA accumulate */
+    SID_PREC_BF16X2 = 1,    /* PSF in BF16, volume/image split hi+lo BF16, FP32 accumulate */
+    SID_PREC_BF16X3 = 2     /* PSF and volume/image both split hi+lo (3 products), FP32 accumulate */
+} sid_precision;
+
+/* Per-plane method of the projector. */
+typedef enum {
+    SID_METHOD_AUTO = 0,    /* per-plane choice from the autotuner */
+    SID_METHOD_DIRECT = 1,  /* implicit GEMM over lenslet taps on MFMA */
+    SID_METHOD_FFT = 2      /* sublattice-periodised spectra, radix-2/3/5 FFT */
+} sid_method;
+
+const char *sid_last_error(void);
+int sid_version(void);
+/* Number of visible devices that are gfx950; 0 when none (no error). */
+int sid_device_count(void);
+
+/* --- PSF handle ---------------------------------------------------------- */
+
+/* Upload H (host memory) and build the device-side projector state.
+ * CAindex: NULL reproduces forwardProjectGPU / backwardProjectGPU_new (the full
+ * M x M kernel, forwardProjectGPU.m:19); non-NULL reproduces forwardProjectACC /
+ * backwardProjectACC_new (kernel cropped to CAindex(cc,1):CAindex(cc,2),
+ * forwardProjectACC.m:22, backwardProjectACC_new.m:21). */
+int sid_psf_create(const float *H, const int64_t dims[5], const int32_t *CAindex, sid_psf **out);
+/* Same with H already in device memory (same layout); H is copied. */
+int sid_psf_create_device(const float *H_dev, const int64_t dims[5], const int32_t *CAindex,
+                          sid_psf **out);
+int sid_psf_destroy(sid_psf *psf);
+
+/* Select the GEMM precision (builds the BF16 weight copies on first use). */
+int sid_psf_set_precision(sid_psf *psf, int precision);
+/* Force a method for every plane (SID_METHOD_DIRECT / _FFT) or restore the
+ * autotuned per-plane choice (SID_METHOD_AUTO). */
+int sid_psf_set_method(sid_psf *psf, int method);
+
+typedef struct {
+    int64_t M, Nnum, Nz;
+    int64_t taps_max;          /* largest lenslet tap radius over planes */
+    int64_t weight_bytes;      /* device bytes held by projector weights */
+    int64_t support_lo[512];   /* per-plane inclusive kernel support used (0-based) */
+    int64_t support_hi[512];
+} sid_psf_info;
+int sid_psf_get_info(const sid_psf *psf, sid_psf_info *info);
+
+/* --- projectors (host buffers: the MEX path) ------------------------------ */
+
+/* forwardProjectGPU.m:1 / forwardProjectACC.m:1.  vdims = {X, Y, nz},
+ * nz <= Nz of H; O is X*Y. */
+int sid_forward(sid_psf *psf, const float *V, const int64_t vdims[3], float *O);
+/* backwardProjectGPU_new.m:1 / backwardProjectACC_new.m:1.  odims = {X, Y};
+ * B is X*Y*Nz (all planes of H, backwardProjectGPU_new.m:4-5). */
+int sid_backward(sid_psf *psf, const float *O, const int64_t odims[2], float *B);
+
+/* --- projectors (device buffers) ----------------------------------------- */
+int sid_forward_device(sid_psf *psf, const float *V_dev, const int64_t vdims[3], float *O_dev,
+                       void *stream);
+int sid_backward_device(sid_psf *psf, const float *O_dev, const int64_t odims[2], float *B_dev,
+                        void *stream);
+
+/* --- ISRA / Richardson-Lucy loop (deconvRL.m:46-93, basic, lambda = 0) ----
+ * Runs all iterations on the device.  X holds Xguess on entry and the
+ * result on exit.  hist (nullable) receives max_iter * X*Y*nz values:
+ * deconvRL.m:90-92 stores Xguess_history(i,:,:,:); here iteration i is the
+ * contiguous block hist + i*X*Y*nz.  vdims = {X, Y, Nz}. */
+int sid_deconv_rl(sid_psf *psf, const float *Htf, float *X, const int64_t vdims[3], int max_iter,
+                  float *hist);
+int sid_deconv_rl_device(sid_psf *psf, const float *Htf_dev, float *X_dev, const int64_t vdims[3],
+                         int max_iter, float *hist_dev, void *stream);
+
+/* Device time (ms, HIP events) of the last call's projector kernels:
+ * [0] forward GEMM, [1] forward reduce, [2] backward GEMM, [3] backward reduce
+ * + update, [4] whole call.  Summed over iterations for sid_deconv_rl. */
+int sid_last_timing(const sid_psf *psf, double ms[8]);
+
+/* Kernel launch counts and GEMM geometry of the direct path for the current
+ * sensor plan (for roofline accounting): {macs_fwd, macs_bwd, items_fwd,
+ * items_bwd, slabs_fwd, slabs_bwd, rows_padded, cols_padded}. */
+int sid_plan_stats(const sid_psf *psf, int64_t stats[8]);
+
+/* Built-in check of the MFMA lane layouts the kernels assume (one 16x16x32
+ * BF16 and one 16x16x4 FP32 product against a host loop).  0 = pass. */
+int sid_selftest(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SID_PROJ_H */

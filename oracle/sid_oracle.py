@@ -1,0 +1,505 @@
+"""FP64 numpy restatement of SID's light-field projection path (test infrastructure).
+
+ORACLE — test infrastructure only.  Imported by ``tests/``, by
+``__graft_entry__.smoke()`` and by ``bench.py``'s ``cpu_baseline`` leg as the
+checker; never by the product package ``sid_amd``.
+
+Every function restates one piece of the reference MATLAB code (paths below
+are relative to the reference repository).  Index conventions: arrays keep
+MATLAB's logical indexing with 0-based offsets, i.e. ``H[i, j, a, b, c]`` is
+``H(i+1, j+1, a+1, b+1, c+1)``; ``V[x, y, c]`` is ``realspace(x+1, y+1, c+1)``.
+The sublattice ``aa:Nnum:end`` is ``a::n`` here.
+
+How each function is pinned (tests/test_oracle.py):
+  conv2fft_same        scipy.signal.convolve2d(mode='same') and brute force
+  forward_project*     dense-matrix brute force on tiny cases, point-source
+                       closed form, linearity, unit-sum flux conservation
+  backward_project*    dense-matrix transpose, <F v, w> = <v, B w>
+  deconv_rl            dense-matrix ISRA, fixed point, monotone LS decrease,
+                       guard cases (0/0 and x/0 give 0)
+  quantile_matlab      Hyndman-Fan type 5 closed form on hand-worked cases
+  library placement    hand-worked integer cases, MATLAB round semantics
+  poisson_nll_grad     central finite differences
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+__all__ = [
+    "matlab_round", "fft_pad_size", "conv2fft_same", "conv2_same_direct",
+    "forward_project", "backward_project", "forward_project_acc",
+    "backward_project_acc", "deconv_rl", "quantile_matlab",
+    "reconstruct_preprocess", "reconstruct_isra", "library_gpu_placement",
+    "library_gpu_entry", "ball_offsets", "library_cpu_entry",
+    "project_forward_patched", "poisson_nll", "poisson_nll_grad_volume",
+    "poisson_nll_grad_psf", "ls_deconv", "dense_operator",
+]
+
+
+def matlab_round(x):
+    """MATLAB ``round``: half away from zero (numpy.round is half-to-even)."""
+    x = np.asarray(x, dtype=np.float64)
+    return np.sign(x) * np.floor(np.abs(x) + 0.5)
+
+
+def fft_pad_size(X: int, M: int) -> int:
+    """FFT pad rule of external/LFrecon_util/forwardProjectGPU.m:9-13.
+
+    ``N = min(2^ceil(log2(X + floor(M/2))), 256*ceil((X + floor(M/2))/256))``.
+    """
+    s = X + M // 2
+    return int(min(2 ** math.ceil(math.log2(s)), 256 * math.ceil(s / 256)))
+
+
+def conv2fft_same(x, m, N: int | None = None):
+    """external/LFrecon_util/conv2FFT_.m:1-17 with zeroPad.m:1-4 and the
+    'circular' rule of exindex.m:248-249.
+
+    Pads ``x`` and ``m`` into an N x N zero image, circularly shifts ``m`` by
+    ``-floor(M/2)`` (``exindex(m, mc:me, 'circular')`` with ``mc = 1+mmid``),
+    multiplies the spectra, keeps the real part and crops to size(x).  With
+    the pad rule above this equals MATLAB ``conv2(x, m, 'same')``.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    m = np.asarray(m, dtype=np.float64)
+    if N is None:
+        N = (fft_pad_size(x.shape[0], m.shape[0]), fft_pad_size(x.shape[1], m.shape[1]))
+    elif np.isscalar(N):
+        N = (int(N), int(N))
+    xp = np.zeros(N)
+    xp[: x.shape[0], : x.shape[1]] = x
+    mp = np.zeros(N)
+    mp[: m.shape[0], : m.shape[1]] = m
+    mid = (m.shape[0] // 2, m.shape[1] // 2)
+    # exindex(..., mc:me, 'circular'): element t (0-based) is mp[(t + mid) mod N]
+    mp = np.roll(mp, shift=(-mid[0], -mid[1]), axis=(0, 1))
+    y = np.real(np.fft.ifft2(np.fft.fft2(xp) * np.fft.fft2(mp)))
+    return y[: x.shape[0], : x.shape[1]]
+
+
+def conv2_same_direct(x, k):
+    """MATLAB ``conv2(x, k, 'same')`` by the defining sum (tiny inputs only).
+
+    ``out[i, j] = sum_{p,q} x[p, q] * k[i - p + floor(M/2), j - q + floor(N/2)]``.
+    """
+    x = np.asarray(x, dtype=np.float64)
+    k = np.asarray(k, dtype=np.float64)
+    X, Y = x.shape
+    M, N = k.shape
+    mr, mc = M // 2, N // 2
+    out = np.zeros((X, Y))
+    for i in range(X):
+        for j in range(Y):
+            s = 0.0
+            for p in range(max(0, i + mr - M + 1), min(X, i + mr + 1)):
+                for q in range(max(0, j + mc - N + 1), min(Y, j + mc + 1)):
+                    s += x[p, q] * k[i - p + mr, j - q + mc]
+            out[i, j] = s
+    return out
+
+
+def _sublattice_mask(X, Y, n, a, b):
+    mask = np.zeros((X, Y))
+    mask[a::n, b::n] = 1.0
+    return mask
+
+
+def forward_project(H, V):
+    """external/LFrecon_util/forwardProjectGPU.m:1-29 in FP64.
+
+    ``O = sum_{aa,bb,cc} conv2FFT_(mask_{aa,bb} .* V(:,:,cc), H(:,:,aa,bb,cc))``
+    over the full M x M kernel (no crop).  The per-(aa,bb,cc) inverse FFTs
+    are summed in the frequency domain before one inverse transform, which is
+    the same sum by linearity of the DFT.
+    """
+    H = np.asarray(H)
+    V = np.asarray(V, dtype=np.float64)
+    if V.ndim == 2:
+        V = V[:, :, None]
+    X, Y, nz = V.shape
+    M, _, n, _, _ = H.shape
+    N = (fft_pad_size(X, M), fft_pad_size(Y, M))
+    mid = M // 2
+    acc = np.zeros((N[0], N[1] // 2 + 1), dtype=np.complex128)
+    for a in range(n):
+        for b in range(n):
+            for c in range(nz):
+                t = np.zeros(N)
+                t[a:X:n, b:Y:n] = V[a::n, b::n, c]
+                mp = np.zeros(N)
+                mp[:M, :M] = H[:, :, a, b, c]
+                mp = np.roll(mp, shift=(-mid, -mid), axis=(0, 1))
+                acc += np.fft.rfft2(t) * np.fft.rfft2(mp)
+    y = np.fft.irfft2(acc, s=N)
+    return y[:X, :Y]
+
+
+def backward_project(H, O, nz: int | None = None):
+    """external/LFrecon_util/backwardProjectGPU_new.m:1-28 in FP64.
+
+    ``B(aa:n:end, bb:n:end, cc) = [conv2FFT_(O, rot180(H(:,:,aa,bb,cc)))](aa:n:end, bb:n:end)``.
+    ``imrotate(., 180)`` of a 2-D array is a flip of both axes.
+    """
+    H = np.asarray(H)
+    O = np.asarray(O, dtype=np.float64)
+    X, Y = O.shape
+    M, _, n, _, NzH = H.shape
+    nz = NzH if nz is None else nz
+    N = (fft_pad_size(X, M), fft_pad_size(Y, M))
+    mid = M // 2
+    op = np.zeros(N)
+    op[:X, :Y] = O
+    Of = np.fft.rfft2(op)
+    B = np.zeros((X, Y, nz))
+    for c in range(nz):
+        for a in range(n):
+            for b in range(n):
+                mp = np.zeros(N)
+                mp[:M, :M] = H[::-1, ::-1, a, b, c]
+                mp = np.roll(mp, shift=(-mid, -mid), axis=(0, 1))
+                t = np.fft.irfft2(Of * np.fft.rfft2(mp), s=N)
+                B[a:X:n, b:Y:n, c] = t[a:X:n, b:Y:n]
+    return B
+
+
+def _crop(H, CAindex, c):
+    lo, hi = int(CAindex[c][0]) - 1, int(CAindex[c][1])
+    return H[lo:hi, lo:hi]
+
+
+def forward_project_acc(H, V, CAindex):
+    """external/LFrecon_util/forwardProjectACC.m:1-33 in FP64.
+
+    Same operator as ``forward_project`` but each kernel is first cropped to
+    ``H(CAindex(cc,1):CAindex(cc,2), same, aa, bb, cc)`` (:22) and convolved
+    with ``conv2(., 'same')`` (:25), whose offset is ``floor(Mc/2)`` of the crop.
+    """
+    V = np.asarray(V, dtype=np.float64)
+    if V.ndim == 2:
+        V = V[:, :, None]
+    X, Y, nz = V.shape
+    n = H.shape[2]
+    out = np.zeros((X, Y))
+    for a in range(n):
+        for b in range(n):
+            for c in range(nz):
+                Hs = _crop(H[:, :, a, b, c], CAindex, c)
+                t = np.zeros((X, Y))
+                t[a::n, b::n] = V[a::n, b::n, c]
+                out += conv2fft_same(t, Hs)
+    return out
+
+
+def backward_project_acc(H, O, CAindex):
+    """external/LFrecon_util/backwardProjectACC_new.m:1-30 in FP64 (its
+    output is double, :6).  Kernel cropped as in :21, rotated by 180 degrees,
+    ``conv2(O, filter, 'same')`` (:22), sublattice kept (:23)."""
+    O = np.asarray(O, dtype=np.float64)
+    X, Y = O.shape
+    n, nz = H.shape[2], H.shape[4]
+    B = np.zeros((X, Y, nz))
+    for c in range(nz):
+        for a in range(n):
+            for b in range(n):
+                filt = _crop(H[:, :, a, b, c], CAindex, c)[::-1, ::-1]
+                t = conv2fft_same(O, filt)
+                B[a::n, b::n, c] = t[a::n, b::n]
+    return B
+
+
+def dense_operator(H, X, Y, nz=None, CAindex=None):
+    """Dense matrix A with vec(forward(V)) = A @ vec(V) (column-major vec),
+    built by pushing unit voxels through the defining sum of conv2 'same'.
+    Brute force for tiny cases only."""
+    M, _, n, _, NzH = H.shape
+    nz = NzH if nz is None else nz
+    A = np.zeros((X * Y, X * Y * nz))
+    for c in range(nz):
+        if CAindex is None:
+            lo, hi = 0, M
+        else:
+            lo, hi = int(CAindex[c][0]) - 1, int(CAindex[c][1])
+        Mc = hi - lo
+        off = Mc // 2
+        for qy in range(Y):
+            for qx in range(X):
+                a, b = qx % n, qy % n
+                K = H[lo:hi, lo:hi, a, b, c]
+                col = qx + X * (qy + Y * c)
+                for py in range(Y):
+                    j = py - qy + off
+                    if j < 0 or j >= Mc:
+                        continue
+                    for px in range(X):
+                        i = px - qx + off
+                        if 0 <= i < Mc:
+                            A[px + X * py, col] += K[i, j]
+    return A
+
+
+def deconv_rl(forward, backward, Htf, max_iter, X0, history=False):
+    """external/LFrecon_util/deconvRL.m:1-93, mode 'basic' with lambda = 0.
+
+    Per iteration (:46-52, :85-87):
+        HX = F(X);  D = B(HX);  R = Htf ./ D;  R(isinf(R)) = 0;
+        X = X .* R;  X(isnan(X)) = 0.
+    Returns X (and the per-iteration history, :9-11/:90-92, if asked).
+    """
+    X = np.array(X0, dtype=np.float64, copy=True)
+    Htf = np.asarray(Htf, dtype=np.float64)
+    hist = []
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(int(max_iter)):
+            HX = forward(X)
+            D = backward(HX)
+            R = Htf / D
+            R[np.isinf(R)] = 0.0
+            X = X * R
+            X[np.isnan(X)] = 0.0
+            if history:
+                hist.append(X.copy())
+    return (X, np.stack(hist)) if history else X
+
+
+def quantile_matlab(x, q):
+    """MATLAB Statistics Toolbox ``quantile(x, q)`` for a vector: Hyndman-Fan
+    type 5 (piecewise-linear through ((k-0.5)/n, x_(k)), clamped at the ends).
+    Used by external/LFrecon_util/reconstruct_S.m:132,169."""
+    x = np.sort(np.asarray(x, dtype=np.float64).ravel())
+    n = x.size
+    if n == 0:
+        return float("nan")
+    pos = n * float(q) + 0.5  # 1-based fractional order statistic
+    if pos <= 1:
+        return float(x[0])
+    if pos >= n:
+        return float(x[-1])
+    k = int(math.floor(pos))
+    f = pos - k
+    return float(x[k - 1] + f * (x[k] - x[k - 1]))
+
+
+def reconstruct_preprocess(img, microlenses=None, q=0.8):
+    """external/LFrecon_util/reconstruct_S.m:131-134 / :168-170,191.
+
+    ``img = max(img - quantile(img(microlenses==0), q), 0); img = img/max(img);
+    nrm = sqrt(sum(img.^2))``.  Returns ``(img / nrm, nrm)``, the back-projected
+    input and the scale the reconstruction is multiplied by (:205)."""
+    img = np.asarray(img, dtype=np.float64)
+    if microlenses is None:
+        sel = img.ravel(order="F")
+    else:
+        sel = img.ravel(order="F")[np.asarray(microlenses).ravel(order="F") == 0]
+    thr = quantile_matlab(sel, q)
+    img = np.maximum(img - thr, 0.0)
+    img = img / img.max()
+    nrm = math.sqrt(float(np.sum(img * img)))
+    return img / nrm, nrm
+
+
+def reconstruct_isra(H, img, max_iter=8, microlenses=None, q=0.8, CAindex=None):
+    """Single-image ISRA wiring of reconstruct_S.m:168-205 (GPU branch with
+    whichSolver='ISRA'): Xguess = B(img/nrm); X = deconvRL(F, B, Xguess,
+    max_iter, Xguess); recon = nrm * X."""
+    y, nrm = reconstruct_preprocess(img, microlenses, q)
+    if CAindex is None:
+        F = lambda v: forward_project(H, v)
+        B = lambda o: backward_project(H, o)
+    else:
+        F = lambda v: forward_project_acc(H, v, CAindex)
+        B = lambda o: backward_project_acc(H, o, CAindex)
+    htf = B(y)
+    X = deconv_rl(F, B, htf, max_iter, htf)
+    return nrm * X
+
+
+# --------------------------------------------------------------------------
+# Library generation (src/generate_LFM_library_GPU.m, src/generate_LFM_library_CPU.m,
+# src/project_forward_patched.m)
+# --------------------------------------------------------------------------
+
+def library_gpu_placement(center, neur_rad, axial, image_size, nz):
+    """Integer placement of src/generate_LFM_library_GPU.m:68-88 (1-based
+    MATLAB indices returned as inclusive ranges).
+
+    x, y, z = round(centre) (MATLAB round); z range
+    [max(z-ceil(r/axial),1), min(z+ceil(r/axial), Nz)]; lateral box
+    [max(x-r,1), min(x+r,X)] x [max(y-r,1), min(y+r,Y)].
+    """
+    x, y, z = (int(v) for v in matlab_round(np.asarray(center, dtype=np.float64)))
+    dz = int(math.ceil(neur_rad / axial))
+    z0, z1 = max(z - dz, 1), min(z + dz, nz)
+    x0, x1 = max(x - neur_rad, 1), min(x + neur_rad, image_size[0])
+    y0, y1 = max(y - neur_rad, 1), min(y + neur_rad, image_size[1])
+    return dict(x=x, y=y, z=z, z0=z0, z1=z1, x0=x0, x1=x1, y0=y0, y1=y1)
+
+
+def library_gpu_entry(recon, ids, center, neur_rad, axial, H):
+    """One entry of src/generate_LFM_library_GPU.m:67-108: build the
+    full-sensor sub-volume (mean over the listed components of ``recon``,
+    :76-85), slice H in z (:87-88) and forward-project it (:95,101)."""
+    X, Y, nz = recon[0].shape
+    p = library_gpu_placement(center, neur_rad, axial, (X, Y), H.shape[4])
+    vol = np.zeros((X, Y, p["z1"] - p["z0"] + 1))
+    sx = slice(p["x0"] - 1, p["x1"])
+    sy = slice(p["y0"] - 1, p["y1"])
+    sz = slice(p["z0"] - 1, p["z1"])
+    for k in ids:
+        vol[sx, sy, :] += recon[k][sx, sy, sz]
+    vol /= len(ids)
+    Hs = H[:, :, :, :, sz]
+    return forward_project(Hs, vol), p
+
+
+def ball_offsets(r, rr):
+    """Voxel list of the ball kernel of src/generate_LFM_library_CPU.m:34-39:
+    ``meshgrid(-r:r,-r:r,-rr:rr)``, ``X^2 + Y^2 + (Z*r/rr)^2 <= r^2`` and
+    ``ind2sub`` (column-major order).  Returns 1-based (i, j, k) rows; note
+    meshgrid swaps the first two axes (i runs over Y, j over X)."""
+    g = np.arange(-r, r + 1, dtype=np.float64)
+    gz = np.arange(-rr, rr + 1, dtype=np.float64)
+    # meshgrid(x, y, z) returns arrays indexed [y, x, z]
+    Xg = g[None, :, None] + 0 * g[:, None, None] + 0 * gz[None, None, :]
+    Yg = g[:, None, None] + 0 * g[None, :, None] + 0 * gz[None, None, :]
+    Zg = gz[None, None, :] + 0 * g[:, None, None] + 0 * g[None, :, None]
+    W = (Xg ** 2 + Yg ** 2 + (Zg * r / rr) ** 2) <= r ** 2
+    idx = np.flatnonzero(W.ravel(order="F"))
+    i, j, k = np.unravel_index(idx, W.shape, order="F")
+    return np.stack([i + 1, j + 1, k + 1], axis=1), W.astype(np.float64)
+
+
+def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
+    """src/project_forward_patched.m:1-76 in FP64 (1-based pixel triples).
+
+    Patch origin ``floor((px_min-1)/n)*n`` (:8-12), lenslet count forced odd
+    (:13-20), ``conv2(., 'full')`` per (i, j, k) (:37-49), centre
+    ``offset + ceil((n_ml/2)*n)`` (:56-57), clipped paste (:61-73)."""
+    pixels = np.asarray(pixels, dtype=np.int64)
+    px_min = pixels.min(axis=0)
+    px_max = pixels.max(axis=0)
+    off_ml = (px_min[:2] - 1) // n
+    off_px = off_ml * n
+    n_ml = -(-px_max[:2] // n) - off_ml
+    n_ml = np.where(n_ml % 2 == 0, n_ml + 1, n_ml)
+    off_z = px_min[2] - 1
+    n_z = px_max[2] - off_z
+    vol = np.zeros((int(n_ml[0] * n), int(n_ml[1] * n), int(n_z)))
+    for t in range(pixels.shape[0]):
+        vol[pixels[t, 0] - off_px[0] - 1, pixels[t, 1] - off_px[1] - 1, pixels[t, 2] - off_z - 1] = \
+            colv[idx[t, 0] - 1, idx[t, 1] - 1, idx[t, 2] - 1]
+    M = H.shape[0]
+    tot = np.zeros((vol.shape[0] + M - 1, vol.shape[1] + M - 1))
+    from scipy.signal import fftconvolve
+    for i in range(n):
+        for j in range(n):
+            for k in range(vol.shape[2]):
+                t = np.zeros(vol.shape[:2])
+                t[i::n, j::n] = vol[i::n, j::n, k]
+                if np.any(t):
+                    tot += fftconvolve(t, H[:, :, i, j, off_z + k].astype(np.float64), mode="full")
+    pjx, pjy = tot.shape
+    cx = off_px[0] + math.ceil((n_ml[0] / 2) * n)
+    cy = off_px[1] + math.ceil((n_ml[1] / 2) * n)
+    x0 = max(1, cx - pjx // 2)
+    x1 = min(volume_n_px[0], cx + pjx // 2)
+    y0 = max(1, cy - pjy // 2)
+    y1 = min(volume_n_px[1], cy + pjy // 2)
+    p0 = x0 - (cx - pjx // 2 - 1)
+    p1 = x1 - (cx - pjx // 2 - 1)
+    q0 = y0 - (cy - pjy // 2 - 1)
+    q1 = y1 - (cy - pjy // 2 - 1)
+    img = np.zeros((int(volume_n_px[0]), int(volume_n_px[1])))
+    img[x0 - 1:x1, y0 - 1:y1] = tot[p0 - 1:p1, q0 - 1:q1]
+    return img, dict(x0=x0, x1=x1, y0=y0, y1=y1, p0=p0, p1=p1, q0=q0, q1=q1,
+                     off_px=off_px, n_ml=n_ml, off_z=off_z, n_z=n_z)
+
+
+def library_cpu_entry(center, r, dimz, m, H, n):
+    """One entry of src/generate_LFM_library_CPU.m:18-57: ball voxels of
+    radius r (axial radius r/dim(3)) shifted to the centre with MATLAB round,
+    filtered to the volume (:43-50), projected by project_forward_patched and
+    L2-normalised (:51-53)."""
+    rr = r / dimz
+    BWW, W = ball_offsets(r, rr)
+    shift = np.array([(r + 1), (r + 1), (rr + 1)], dtype=np.float64)
+    bbb = matlab_round(BWW - shift + np.asarray(center, dtype=np.float64)).astype(np.int64)
+    keep = ((bbb[:, 0] <= m[0]) & (bbb[:, 0] > 0) & (bbb[:, 1] <= m[1]) & (bbb[:, 1] > 0)
+            & (bbb[:, 2] <= H.shape[4]) & (bbb[:, 2] > 0))
+    B = bbb[keep]
+    B_ = matlab_round(BWW[keep]).astype(np.int64)
+    img, info = project_forward_patched(B, m[:2], n, H, W, B_)
+    Q = img.ravel(order="F")
+    return Q / np.linalg.norm(Q), info
+
+
+# --------------------------------------------------------------------------
+# Differentiation (not in the reference: the Poisson log-likelihood of the
+# forward model and its exact gradients, used as the oracle for the adjoint
+# kernels).
+# --------------------------------------------------------------------------
+
+def poisson_nll(forward, V, y):
+    """NLL(V) = sum(F(V) - y .* log F(V)) (constant log(y!) dropped)."""
+    f = forward(V)
+    return float(np.sum(f - y * np.log(f)))
+
+
+def poisson_nll_grad_volume(forward, backward, V, y):
+    """dNLL/dV = B(1 - y ./ F(V)); B is the exact adjoint of F."""
+    f = forward(V)
+    return backward(1.0 - y / f)
+
+
+def poisson_nll_grad_psf(H, V, y, CAindex=None):
+    """dNLL/dH(i,j,a,b,c): sum over sublattice voxels q of plane c and pixels
+    p with p - q = (i,j) - floor(M/2) of r[p] V[q, c], r = 1 - y/F(V).
+    Computed here as a direct correlation per (a, b, c)."""
+    V = np.asarray(V, dtype=np.float64)
+    X, Y, nz = V.shape
+    M, _, n, _, _ = H.shape
+    mid = M // 2
+    f = forward_project(H, V)
+    r = 1.0 - y / f
+    G = np.zeros(H.shape[:4] + (nz,))
+    from scipy.signal import correlate2d
+    for c in range(nz):
+        for a in range(n):
+            for b in range(n):
+                t = np.zeros((X, Y))
+                t[a::n, b::n] = V[a::n, b::n, c]
+                # full cross-correlation: cc[s] = sum_q t[q] r[q + s - (X-1)]
+                cc = correlate2d(r, t, mode="full")
+                # G[i, j] = sum_q t[q] r[q + (i - mid, j - mid)]
+                for i in range(M):
+                    si = i - mid + X - 1
+                    if si < 0 or si >= cc.shape[0]:
+                        continue
+                    row = cc[si]
+                    js = np.arange(M) - mid + Y - 1
+                    ok = (js >= 0) & (js < cc.shape[1])
+                    G[i, ok, a, b, c] = row[js[ok]]
+    return G
+
+
+def ls_deconv(forward, backward, df, max_iter=8, lamb_L1=0.0):
+    """external/LFrecon_util/LS_deconv.m:1-97 without TV/L2 terms:
+    projected gradient descent with exact line search for
+    argmin_{x>=0} ||y - A x||^2 + lamb_L1 ||x||_1, started at df = A'y."""
+    Q = lambda X: backward(forward(X))
+    X = np.array(df, dtype=np.float64, copy=True)
+    g = Q(df) - df + lamb_L1
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(int(max_iter)):
+            passive = np.maximum(X > 0, g < 0)
+            d = passive * g
+            v = forward(d)
+            alpha = float(np.sum(d * d)) / float(np.sum(v * v))
+            if not np.isfinite(alpha):
+                alpha = 0.0
+            Xn = X - alpha * d
+            Xn[Xn < 0] = 0.0
+            g = g + Q(Xn - X)
+            X = Xn
+    return X

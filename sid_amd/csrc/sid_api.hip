@@ -1,0 +1,478 @@
+// C ABI of libsidproj: handle management, the MEX-shaped projector entry
+// points and the on-device ISRA loop (external/LFrecon_util/deconvRL.m).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "sid_internal.h"
+
+namespace sid {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+static int supported_np(int64_t nn2)
+{
+    static const int nts[] = {1, 2, 4, 8, 11, 15, 16};
+    const int64_t need = (nn2 + 15) / 16;
+    for (int nt : nts)
+        if (nt >= need) return nt * 16;
+    return 0;
+}
+
+static bool is_gfx950(int dev)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+// Event pool used for the per-phase timing of the last call.
+struct Timer {
+    sid_psf *p;
+    std::vector<std::pair<int, int>> spans;  // (phase, event index start); end = start + 1
+    std::vector<hipEvent_t> evs;
+    hipStream_t st;
+    explicit Timer(sid_psf *p_, hipStream_t s) : p(p_), st(s) {}
+    ~Timer()
+    {
+        for (auto e : evs) (void)hipEventDestroy(e);
+    }
+    int mark()
+    {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        evs.push_back(e);
+        (void)hipEventRecord(e, st);
+        return (int)evs.size() - 1;
+    }
+    void span(int phase, int e0) { spans.push_back({phase, e0}); }
+    void finish(int e_start, int e_end)
+    {
+        for (int i = 0; i < 8; ++i) p->timing[i] = 0.0;
+        for (auto &s : spans) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, evs[s.second], evs[s.second + 1]) == hipSuccess) p->timing[s.first] += ms;
+        }
+        float ms = 0.f;
+        if (e_start >= 0 && e_end >= 0 && hipEventElapsedTime(&ms, evs[e_start], evs[e_end]) == hipSuccess)
+            p->timing[4] = ms;
+    }
+};
+
+static int create_common(sid_psf *p, const int64_t dims[5], const int32_t *CA)
+{
+    const int64_t M = dims[0], M2 = dims[1], n = dims[2], n2 = dims[3], nz = dims[4];
+    if (M < 1 || M2 != M || n < 1 || n2 != n || nz < 1) {
+        set_error("H must be [M, M, Nnum, Nnum, Nz] with square kernels and square lenslets (got %lld x %lld x %lld x %lld x %lld)",
+                  (long long)M, (long long)M2, (long long)n, (long long)n2, (long long)nz);
+        return SID_ERR_DIMS;
+    }
+    if (nz > 512) {
+        set_error("at most 512 planes are supported (got %lld)", (long long)nz);
+        return SID_ERR_DIMS;
+    }
+    const int np = supported_np(n * n);
+    if (np == 0) {
+        set_error("Nnum = %lld exceeds the supported lenslet size (<= 16)", (long long)n);
+        return SID_ERR_DIMS;
+    }
+    p->M = M; p->n = n; p->nzH = nz;
+    p->S = (n * n + 7) / 8 * 8;
+    p->NP = np;
+    p->planes.assign(nz, Plane{});
+    if (CA) {
+        for (int64_t c = 0; c < nz; ++c) {
+            const int lo = CA[c] - 1, hi = CA[c + nz] - 1;
+            if (lo < 0 || hi >= M || hi < lo) {
+                set_error("CAindex(%lld,:) = [%d %d] is not a crop inside 1..%lld", (long long)c + 1, lo + 1, hi + 1,
+                          (long long)M);
+                return SID_ERR_DIMS;
+            }
+        }
+        p->cropped = true;
+    }
+    SID_TRY(support_scan(p));
+    for (int64_t c = 0; c < nz; ++c) {
+        Plane &pl = p->planes[c];
+        if (CA) {
+            const int clo = CA[c] - 1, chi = CA[c + nz] - 1, Mc = chi - clo + 1;
+            pl.off_f = clo + Mc / 2;            // conv2 'same' offset of the crop (forwardProjectACC.m:25)
+            pl.off_b = clo + (Mc - 1 - Mc / 2);  // after imrotate(., 180) (backwardProjectACC_new.m:21)
+            if (!pl.empty) {
+                pl.lo = std::max(pl.lo, clo);
+                pl.hi = std::min(pl.hi, chi);
+                if (pl.hi < pl.lo) pl.empty = true;
+            }
+        } else {
+            pl.off_f = (int)(M / 2);                 // conv2FFT_.m:6,10-12
+            pl.off_b = (int)(M - 1 - M / 2);         // rot180 kernel, backwardProjectGPU_new.m:19
+        }
+    }
+    return build_weights(p, 0);
+}
+
+static void destroy(sid_psf *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    SensorPlan &sp = p->plan;
+    for (void *q : {(void *)sp.vol, (void *)sp.img, (void *)sp.htf, (void *)sp.part, (void *)sp.vol_hi,
+                    (void *)sp.vol_lo, (void *)sp.img_hi, (void *)sp.img_lo, (void *)sp.fwd.d_segs,
+                    (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
+                    (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
+                    (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo})
+        if (q) (void)hipFree(q);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+static int check_device()
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("no HIP device visible");
+        return SID_ERR_NODEV;
+    }
+    int dev = 0;
+    SID_TRY_HIP(hipGetDevice(&dev));
+    if (!is_gfx950(dev)) {
+        set_error("device %d is not gfx950 (MI355X); this library is built for gfx950 only", dev);
+        return SID_ERR_NODEV;
+    }
+    return SID_OK;
+}
+
+static hipStream_t pick_stream(sid_psf *p, void *stream) { return stream ? (hipStream_t)stream : p->stream; }
+
+}  // namespace sid
+
+using namespace sid;
+
+extern "C" {
+
+const char *sid_last_error(void) { return g_err; }
+int sid_version(void) { return 1; }
+
+int sid_device_count(void)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) return 0;
+    int k = 0;
+    for (int d = 0; d < ndev; ++d) k += is_gfx950(d) ? 1 : 0;
+    return k;
+}
+
+static int psf_create_impl(const float *H, bool host, const int64_t dims[5], const int32_t *CA, sid_psf **out)
+{
+    if (!H || !dims || !out) {
+        set_error("sid_psf_create: null argument");
+        return SID_ERR_ARG;
+    }
+    *out = nullptr;
+    SID_TRY(check_device());
+    sid_psf *p = new (std::nothrow) sid_psf();
+    if (!p) {
+        set_error("host allocation failed");
+        return SID_ERR_OOM;
+    }
+    int rc = SID_OK;
+    do {
+        if (hipGetDevice(&p->device) != hipSuccess) { rc = SID_ERR_HIP; set_error("hipGetDevice failed"); break; }
+        if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = SID_ERR_HIP;
+            set_error("hipStreamCreate failed");
+            break;
+        }
+        for (int i = 0; i < 5; ++i)
+            if (dims[i] < 1) { rc = SID_ERR_DIMS; set_error("H has an empty dimension"); }
+        if (rc) break;
+        const int64_t elems = dims[0] * dims[1] * dims[2] * dims[3] * dims[4];
+        hipError_t e = hipMalloc(&p->H, elems * sizeof(float));
+        if (e != hipSuccess) { rc = SID_ERR_OOM; set_error("hipMalloc(H, %lld B) failed", (long long)(elems * 4)); break; }
+        e = hipMemcpy(p->H, H, elems * sizeof(float), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) { rc = SID_ERR_HIP; set_error("H upload failed: %s", hipGetErrorString(e)); break; }
+        rc = create_common(p, dims, CA);
+    } while (0);
+    if (rc != SID_OK) {
+        destroy(p);
+        return rc;
+    }
+    *out = p;
+    return SID_OK;
+}
+
+int sid_psf_create(const float *H, const int64_t dims[5], const int32_t *CAindex, sid_psf **out)
+{
+    return psf_create_impl(H, true, dims, CAindex, out);
+}
+
+int sid_psf_create_device(const float *H_dev, const int64_t dims[5], const int32_t *CAindex, sid_psf **out)
+{
+    return psf_create_impl(H_dev, false, dims, CAindex, out);
+}
+
+int sid_psf_destroy(sid_psf *psf)
+{
+    destroy(psf);
+    return SID_OK;
+}
+
+int sid_psf_set_precision(sid_psf *p, int precision)
+{
+    if (!p) { set_error("null handle"); return SID_ERR_ARG; }
+    if (precision < SID_PREC_FP32 || precision > SID_PREC_BF16X3) {
+        set_error("unknown precision %d", precision);
+        return SID_ERR_ARG;
+    }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    p->precision = precision;
+    if (precision != SID_PREC_FP32) {
+        SID_TRY(build_weights(p, 1));
+        if (precision == SID_PREC_BF16X3) SID_TRY(build_weights(p, 2));
+    }
+    if (p->plan.vol) SID_TRY(ensure_plan(p, p->plan.X, p->plan.Y, p->plan.nz));
+    return SID_OK;
+}
+
+int sid_psf_set_method(sid_psf *p, int method)
+{
+    if (!p) { set_error("null handle"); return SID_ERR_ARG; }
+    if (method == SID_METHOD_AUTO || method == SID_METHOD_DIRECT) {
+        p->method = method;
+        return SID_OK;
+    }
+    set_error("method %d is not available in this build", method);
+    return SID_ERR_STATE;
+}
+
+int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
+{
+    if (!p || !info) { set_error("null argument"); return SID_ERR_ARG; }
+    std::memset(info, 0, sizeof(*info));
+    info->M = p->M; info->Nnum = p->n; info->Nz = p->nzH; info->taps_max = p->T;
+    int64_t wb = (p->wf_elems + p->wb_elems) * 4;
+    if (p->wf_hi) wb += (p->wf_elems + p->wb_elems) * 2;
+    if (p->wf_lo) wb += (p->wf_elems + p->wb_elems) * 2;
+    info->weight_bytes = wb;
+    for (int64_t c = 0; c < p->nzH && c < 512; ++c) {
+        info->support_lo[c] = p->planes[c].empty ? 0 : p->planes[c].lo;
+        info->support_hi[c] = p->planes[c].empty ? -1 : p->planes[c].hi;
+    }
+    return SID_OK;
+}
+
+static int check_vdims(sid_psf *p, const int64_t vdims[3], bool rl)
+{
+    if (!vdims || vdims[0] < 1 || vdims[1] < 1 || vdims[2] < 1) {
+        set_error("volume dimensions must be positive");
+        return SID_ERR_DIMS;
+    }
+    if (vdims[2] > p->nzH) {
+        set_error("volume has %lld planes but H has %lld (forwardProjectGPU uses H(:,:,:,:,cc) for every plane)",
+                  (long long)vdims[2], (long long)p->nzH);
+        return SID_ERR_NZ;
+    }
+    if (rl && vdims[2] != p->nzH) {
+        set_error("deconvRL volume must have size(H,5) = %lld planes (got %lld)", (long long)p->nzH,
+                  (long long)vdims[2]);
+        return SID_ERR_NZ;
+    }
+    return SID_OK;
+}
+
+static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
+{
+    SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
+    Timer tm(p, st);
+    const int e0 = tm.mark();
+    SID_TRY(launch_vol_in(p, V_dev, vdims[2], p->plan.vol, 1, st));
+    const int e1 = tm.mark();
+    SID_TRY(launch_gemm(p, 0, st));
+    const int e2 = tm.mark();
+    SID_TRY(launch_fwd_reduce(p, O_dev, st));
+    const int e3 = tm.mark();
+    tm.span(0, e1);
+    tm.span(1, e2);
+    (void)e3;
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    tm.finish(e0, e3);
+    return SID_OK;
+}
+
+static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st)
+{
+    // the backward volume always has all planes of H (backwardProjectGPU_new.m:4-5); the
+    // sensor plan keeps the nz of the last forward call.
+    const int64_t nz = (p->plan.vol && p->plan.X == odims[0] && p->plan.Y == odims[1]) ? p->plan.nz : p->nzH;
+    SID_TRY(ensure_plan(p, odims[0], odims[1], nz));
+    Timer tm(p, st);
+    const int e0 = tm.mark();
+    SID_TRY(launch_img_in(p, O_dev, st));
+    const int e1 = tm.mark();
+    SID_TRY(launch_gemm(p, 1, st));
+    const int e2 = tm.mark();
+    SID_TRY(launch_bwd_reduce(p, B_dev, 0, st));
+    const int e3 = tm.mark();
+    tm.span(2, e1);
+    tm.span(3, e2);
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    tm.finish(e0, e3);
+    return SID_OK;
+}
+
+int sid_forward_device(sid_psf *p, const float *V, const int64_t vdims[3], float *O, void *stream)
+{
+    if (!p || !V || !O) { set_error("sid_forward_device: null argument"); return SID_ERR_ARG; }
+    SID_TRY(check_vdims(p, vdims, false));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return forward_impl(p, V, vdims, O, pick_stream(p, stream));
+}
+
+int sid_backward_device(sid_psf *p, const float *O, const int64_t odims[2], float *B, void *stream)
+{
+    if (!p || !O || !B || !odims) { set_error("sid_backward_device: null argument"); return SID_ERR_ARG; }
+    if (odims[0] < 1 || odims[1] < 1) { set_error("image dimensions must be positive"); return SID_ERR_DIMS; }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return backward_impl(p, O, odims, B, pick_stream(p, stream));
+}
+
+// Host-buffer variants: stage through temporary device buffers.
+struct DevBuf {
+    void *ptr = nullptr;
+    ~DevBuf()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+int sid_forward(sid_psf *p, const float *V, const int64_t vdims[3], float *O)
+{
+    if (!p || !V || !O) { set_error("sid_forward: null argument"); return SID_ERR_ARG; }
+    SID_TRY(check_vdims(p, vdims, false));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    const int64_t nv = vdims[0] * vdims[1] * vdims[2], no = vdims[0] * vdims[1];
+    DevBuf dv, dO;
+    SID_TRY_HIP(hipMalloc(&dv.ptr, nv * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&dO.ptr, no * sizeof(float)));
+    SID_TRY_HIP(hipMemcpyAsync(dv.ptr, V, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    SID_TRY(forward_impl(p, (const float *)dv.ptr, vdims, (float *)dO.ptr, p->stream));
+    SID_TRY_HIP(hipMemcpyAsync(O, dO.ptr, no * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return SID_OK;
+}
+
+int sid_backward(sid_psf *p, const float *O, const int64_t odims[2], float *B)
+{
+    if (!p || !O || !B || !odims) { set_error("sid_backward: null argument"); return SID_ERR_ARG; }
+    if (odims[0] < 1 || odims[1] < 1) { set_error("image dimensions must be positive"); return SID_ERR_DIMS; }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    const int64_t no = odims[0] * odims[1], nb = no * p->nzH;
+    DevBuf dO, dB;
+    SID_TRY_HIP(hipMalloc(&dO.ptr, no * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&dB.ptr, nb * sizeof(float)));
+    SID_TRY_HIP(hipMemcpyAsync(dO.ptr, O, no * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    SID_TRY(backward_impl(p, (const float *)dO.ptr, odims, (float *)dB.ptr, p->stream));
+    SID_TRY_HIP(hipMemcpyAsync(B, dB.ptr, nb * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return SID_OK;
+}
+
+static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist,
+                   hipStream_t st)
+{
+    SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
+    SensorPlan &sp = p->plan;
+    const int64_t nv = vdims[0] * vdims[1] * vdims[2];
+    Timer tm(p, st);
+    const int e0 = tm.mark();
+    SID_TRY(launch_vol_in(p, Htf, vdims[2], sp.htf, 0, st));
+    SID_TRY(launch_vol_in(p, X, vdims[2], sp.vol, 1, st));
+    for (int it = 0; it < max_iter; ++it) {
+        const int a = tm.mark();
+        SID_TRY(launch_gemm(p, 0, st));
+        const int b = tm.mark();
+        SID_TRY(launch_fwd_reduce(p, nullptr, st));
+        const int c = tm.mark();
+        SID_TRY(launch_gemm(p, 1, st));
+        const int d = tm.mark();
+        SID_TRY(launch_bwd_reduce(p, hist ? hist + (int64_t)it * nv : nullptr, 1, st));
+        const int e = tm.mark();
+        tm.span(0, a); tm.span(1, b); tm.span(2, c); tm.span(3, d);
+        (void)e;
+    }
+    SID_TRY(launch_vol_out(p, sp.vol, X, vdims[2], st));
+    const int e1 = tm.mark();
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    tm.finish(e0, e1);
+    return SID_OK;
+}
+
+int sid_deconv_rl_device(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist,
+                         void *stream)
+{
+    if (!p || !Htf || !X) { set_error("sid_deconv_rl_device: null argument"); return SID_ERR_ARG; }
+    if (max_iter < 0) { set_error("max_iter must be >= 0"); return SID_ERR_ARG; }
+    SID_TRY(check_vdims(p, vdims, true));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return rl_impl(p, Htf, X, vdims, max_iter, hist, pick_stream(p, stream));
+}
+
+int sid_deconv_rl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist)
+{
+    if (!p || !Htf || !X) { set_error("sid_deconv_rl: null argument"); return SID_ERR_ARG; }
+    if (max_iter < 0) { set_error("max_iter must be >= 0"); return SID_ERR_ARG; }
+    SID_TRY(check_vdims(p, vdims, true));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    const int64_t nv = vdims[0] * vdims[1] * vdims[2];
+    DevBuf dh, dx, dhist;
+    SID_TRY_HIP(hipMalloc(&dh.ptr, nv * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&dx.ptr, nv * sizeof(float)));
+    if (hist && max_iter > 0) SID_TRY_HIP(hipMalloc(&dhist.ptr, nv * max_iter * sizeof(float)));
+    SID_TRY_HIP(hipMemcpyAsync(dh.ptr, Htf, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    SID_TRY_HIP(hipMemcpyAsync(dx.ptr, X, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    SID_TRY(rl_impl(p, (const float *)dh.ptr, (float *)dx.ptr, vdims, max_iter, (float *)dhist.ptr, p->stream));
+    SID_TRY_HIP(hipMemcpyAsync(X, dx.ptr, nv * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    if (dhist.ptr)
+        SID_TRY_HIP(hipMemcpyAsync(hist, dhist.ptr, nv * max_iter * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return SID_OK;
+}
+
+int sid_last_timing(const sid_psf *p, double ms[8])
+{
+    if (!p || !ms) { set_error("null argument"); return SID_ERR_ARG; }
+    for (int i = 0; i < 8; ++i) ms[i] = p->timing[i];
+    return SID_OK;
+}
+
+int sid_plan_stats(const sid_psf *p, int64_t st[8])
+{
+    if (!p || !st) { set_error("null argument"); return SID_ERR_ARG; }
+    const SensorPlan &sp = p->plan;
+    st[0] = sp.fwd.macs; st[1] = sp.bwd.macs;
+    st[2] = (int64_t)sp.fwd.items.size(); st[3] = (int64_t)sp.bwd.items.size();
+    st[4] = sp.fwd.nslabs; st[5] = sp.bwd.nslabs;
+    st[6] = sp.mpad; st[7] = p->NP;
+    return SID_OK;
+}
+
+int sid_selftest(void)
+{
+    SID_TRY(check_device());
+    return run_selftest();
+}
+
+}  // extern "C"

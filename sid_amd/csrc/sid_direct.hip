@@ -1,0 +1,866 @@
+// Direct light-field projector on MFMA: the lenslet-sublattice PSF
+// convolution of external/LFrecon_util/forwardProjectGPU.m:16-28 and
+// backwardProjectGPU_new.m:16-26 written as implicit GEMMs.
+//
+// Forward.  With p = n*P + (u,v) (sensor pixel in lenslet P) and
+// q = n*Q + (a,b) (voxel), the reference sum
+//     O[p] = sum_c sum_q V[q,c] * H_{a,b,c}[p - q + off]
+// becomes, for every lenslet tap D = P - Q,
+//     O[P,uv] += sum_{c,ab} V[P - D, ab, c] * W_c[D][ab][uv],
+//     W_c[D][ab][uv] = H_{a,b,c}[n*D + (u - a, v - b) + off].
+// For a fixed plane c and tap row dx the sum over dy and ab is one GEMM
+// whose A operand is Toeplitz along y: the K index runs over the input
+// lenslets Qy = Py - dy in ascending order and ab inside each, so for output
+// lenslet P the A row is a contiguous run of the padded volume row Px - dx.
+// M = output lenslets, N = uv (padded to 16*NT), K = (#dy taps) * S.
+//
+// Backward (the adjoint, backwardProjectGPU_new.m:19-23):
+//     B[Q,ab,c] = sum_D sum_uv O[Q + D, uv] * W_c[D][ab][uv]
+// with the same structure: A = padded image rows Qx + dx, K = (dy, uv)
+// ascending, N = ab.
+//
+// Every (plane, dx) pair is a segment; a block computes one M tile over a
+// contiguous range of segments (split-K) into its own partial-sum slab; a
+// reduce kernel sums the slabs in a fixed order (deterministic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "sid_internal.h"
+
+namespace sid {
+
+__device__ __forceinline__ u16 f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                          // round to nearest even
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+// K chunk of the staged B operand (elements of K per LDS stage).
+template <int MODE> struct KChunk { static constexpr int value = 64; };
+template <> struct KChunk<0> { static constexpr int value = 32; };
+
+// ---------------------------------------------------------------------------
+// Support scan: smallest index box holding every nonzero of a plane.
+// ---------------------------------------------------------------------------
+__global__ void k_support_scan(const float *__restrict__ H, int64_t M, int64_t n, int64_t nz,
+                               int *__restrict__ box /* nz x 4: ilo, ihi, jlo, jhi */)
+{
+    const int64_t per_plane = M * M * n * n;
+    const int c = blockIdx.y;
+    int ilo = INT32_MAX, ihi = -1, jlo = INT32_MAX, jhi = -1;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per_plane;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        float v = H[(int64_t)c * per_plane + e];
+        if (v != 0.0f) {
+            int i = (int)(e % M), j = (int)((e / M) % M);
+            ilo = min(ilo, i); ihi = max(ihi, i); jlo = min(jlo, j); jhi = max(jhi, j);
+        }
+    }
+    // wave reduction, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) {
+        ilo = min(ilo, __shfl_xor(ilo, o)); ihi = max(ihi, __shfl_xor(ihi, o));
+        jlo = min(jlo, __shfl_xor(jlo, o)); jhi = max(jhi, __shfl_xor(jhi, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&box[c * 4 + 0], ilo); atomicMax(&box[c * 4 + 1], ihi);
+        atomicMin(&box[c * 4 + 2], jlo); atomicMax(&box[c * 4 + 3], jhi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight re-layout: H (MATLAB layout) -> per (plane, dx) GEMM B operands,
+// stored chunk-transposed: [K / KC][NP][KC] so one LDS stage is one
+// contiguous copy with K innermost.
+// ---------------------------------------------------------------------------
+template <int PART>  // 0: fp32, 1: bf16 hi, 2: bf16 lo
+__global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, int64_t M, int n,
+                          int c, int dir, int lo, int hi, int off, int t_lo, int ntap, int64_t K,
+                          int S, int NP, int KC)
+{
+    const int64_t per_tap = K * NP;
+    const int64_t total = (int64_t)ntap * per_tap;
+    const int nn2 = n * n;
+    const int t_hi = t_lo + ntap - 1;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int tx = (int)(idx / per_tap);
+        const int64_t rem = idx - (int64_t)tx * per_tap;
+        const int64_t chunk = rem / ((int64_t)NP * KC);
+        const int64_t r2 = rem - chunk * NP * KC;
+        const int col = (int)(r2 / KC);
+        const int64_t k = chunk * KC + (r2 - (int64_t)col * KC);
+        const int dx = t_lo + tx;
+        float v = 0.0f;
+        const int ky = (int)(k / S), kk = (int)(k - (int64_t)ky * S);
+        if (ky < ntap && kk < nn2 && col < nn2) {
+            int dy, ab, uv;
+            if (dir == 0) { dy = t_hi - ky; ab = kk; uv = col; }
+            else          { dy = t_lo + ky; uv = kk; ab = col; }
+            const int a = ab % n, b = ab / n, u = uv % n, w = uv / n;
+            const int i = n * dx + u - a + off, j = n * dy + w - b + off;
+            if (i >= lo && i <= hi && j >= lo && j <= hi)
+                v = H[(int64_t)i + M * ((int64_t)j + M * ((int64_t)a + n * ((int64_t)b + (int64_t)n * c)))];
+        }
+        if (PART == 0) {
+            ((float *)Wout)[idx] = v;
+        } else {
+            u16 h = f2bf(v);
+            ((u16 *)Wout)[idx] = PART == 1 ? h : f2bf(v - bf2f(h));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The projector GEMM.  256 threads = 4 waves; each wave owns RT*16 rows of the
+// M tile and all NT*16 columns.  B chunks are staged in LDS and shared by the
+// 4 waves; A is read straight from global memory (L2-resident rows, each row
+// a contiguous Toeplitz window).
+//
+// MODE 0 (FP32): v_mfma_f32_16x16x4_f32.  Lane (l16, g) = (lane & 15,
+//   lane >> 4) feeds A[row l16][k = 8g + s] and B[k = 8g + s][col l16] at step
+//   s = 0..7 of a 32-wide K chunk, so a lane's 8 A values are contiguous.
+// MODE 1/2 (BF16): v_mfma_f32_16x16x32_bf16 with A[row l16][8g..8g+7],
+//   B[8g..8g+7][col l16].  MODE 1: C += Ahi*Bhi + Alo*Bhi.  MODE 2 adds
+//   Ahi*Blo.
+// C layout (both): lane holds C[4g + i][l16], i = 0..3.
+// ---------------------------------------------------------------------------
+template <int NT, int RT, int MODE>
+__global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
+    const Item *__restrict__ items, const Seg *__restrict__ segs, const void *__restrict__ A0,
+    const void *__restrict__ A1, const void *__restrict__ W0, const void *__restrict__ W1,
+    float *__restrict__ part, int nrows, int Ly, int YP, int S, int64_t slab_stride)
+{
+    constexpr int NP = NT * 16;
+    constexpr int MT = 4 * RT * 16;
+    constexpr int KC = KChunk<MODE>::value;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const Item it = items[blockIdx.x];
+
+    int64_t rowoff[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        int r = it.mtile * MT + wave * RT * 16 + rt * 16 + l16;
+        r = r < nrows ? r : nrows - 1;
+        const int Px = r / Ly, Py = r - Px * Ly;
+        rowoff[rt] = ((int64_t)Px * YP + Py) * S;
+    }
+    f4 acc[RT][NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+
+    if constexpr (MODE == 0) {
+        constexpr int LDK = KC + 4;  // floats per staged column
+        __shared__ float Bs[NP * LDK];
+        for (int s = it.seg0; s < it.seg1; ++s) {
+            const Seg sg = segs[s];
+            const float *A = (const float *)A0 + sg.a_off;
+            const float *W = (const float *)W0 + sg.w_off;
+            for (int k0 = 0; k0 < sg.K; k0 += KC) {
+                float4 a0[RT], a1[RT];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    const float *pa = A + rowoff[rt] + k0 + 8 * g;
+                    a0[rt] = *(const float4 *)pa;
+                    a1[rt] = *(const float4 *)(pa + 4);
+                }
+                __syncthreads();
+                const float *wc = W + (int64_t)k0 * NP;  // chunk: [NP][KC]
+                for (int e = threadIdx.x; e < NP * KC / 4; e += NTHREADS) {
+                    const int col = e / (KC / 4), kq = e - col * (KC / 4);
+                    *(float4 *)(Bs + col * LDK + 4 * kq) = *(const float4 *)(wc + 4 * e);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct) {
+                    const float *pb = Bs + (ct * 16 + l16) * LDK + 8 * g;
+                    const float4 b0 = *(const float4 *)pb, b1 = *(const float4 *)(pb + 4);
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) {
+                        f4 c = acc[rt][ct];
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].x, b0.x, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].y, b0.y, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].z, b0.z, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].w, b0.w, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].x, b1.x, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].y, b1.y, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].z, b1.z, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].w, b1.w, c, 0, 0, 0);
+                        acc[rt][ct] = c;
+                    }
+                }
+            }
+        }
+    } else {
+        constexpr int LDK = KC + 8;  // bf16 elements per staged column
+        constexpr int NB = MODE == 2 ? 2 : 1;
+        __shared__ u16 Bs[NB * NP * LDK];
+        for (int s = it.seg0; s < it.seg1; ++s) {
+            const Seg sg = segs[s];
+            const u16 *Ah = (const u16 *)A0 + sg.a_off;
+            const u16 *Al = (const u16 *)A1 + sg.a_off;
+            const u16 *Wh = (const u16 *)W0 + sg.w_off;
+            const u16 *Wl = (const u16 *)W1 + sg.w_off;
+            for (int k0 = 0; k0 < sg.K; k0 += KC) {
+                bf8 ah[RT][KC / 32], al[RT][KC / 32];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                    for (int ks = 0; ks < KC / 32; ++ks) {
+                        const int64_t o = rowoff[rt] + k0 + 32 * ks + 8 * g;
+                        ah[rt][ks] = *(const bf8 *)(Ah + o);
+                        al[rt][ks] = *(const bf8 *)(Al + o);
+                    }
+                __syncthreads();
+                const int64_t wo = (int64_t)k0 * NP;
+                for (int e = threadIdx.x; e < NP * KC / 8; e += NTHREADS) {
+                    const int col = e / (KC / 8), kq = e - col * (KC / 8);
+                    *(uint4 *)(Bs + col * LDK + 8 * kq) = *(const uint4 *)(Wh + wo + 8 * e);
+                    if constexpr (MODE == 2)
+                        *(uint4 *)(Bs + NP * LDK + col * LDK + 8 * kq) = *(const uint4 *)(Wl + wo + 8 * e);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct) {
+#pragma unroll
+                    for (int ks = 0; ks < KC / 32; ++ks) {
+                        const bf8 bh = *(const bf8 *)(Bs + (ct * 16 + l16) * LDK + 32 * ks + 8 * g);
+                        bf8 bl;
+                        if constexpr (MODE == 2)
+                            bl = *(const bf8 *)(Bs + NP * LDK + (ct * 16 + l16) * LDK + 32 * ks + 8 * g);
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt) {
+                            f4 c = acc[rt][ct];
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][ks], bh, c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt][ks], bh, c, 0, 0, 0);
+                            if constexpr (MODE == 2)
+                                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][ks], bl, c, 0, 0, 0);
+                            acc[rt][ct] = c;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    float *out = part + (int64_t)it.slab * slab_stride;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int rbase = it.mtile * MT + wave * RT * 16 + rt * 16 + 4 * g;
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + i) * NP + ct * 16 + l16] = acc[rt][ct][i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions and layout conversions
+// ---------------------------------------------------------------------------
+
+// Sum the forward slabs; write the padded lenslet-major image (FP32 + BF16
+// hi/lo shadows when hi != nullptr) and, if O_col != nullptr, the MATLAB
+// column-major image.  Pixels beyond X, Y (the last partial lenslet) are 0.
+__global__ void k_fwd_reduce(const float *__restrict__ part, int nslabs, int64_t slab_stride,
+                             int NP, int nrows, int n, int Ly, int YP, int T, int S, int X, int Y,
+                             float *__restrict__ img, u16 *__restrict__ hi, u16 *__restrict__ lo,
+                             float *__restrict__ O_col)
+{
+    const int nn2 = n * n;
+    const int64_t total = (int64_t)nrows * nn2;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / nn2), uv = (int)(e - (int64_t)r * nn2);
+        float s = 0.0f;
+        const float *p = part + (int64_t)r * NP + uv;
+        for (int k = 0; k < nslabs; ++k) s += p[(int64_t)k * slab_stride];
+        const int Px = r / Ly, Py = r - Px * Ly;
+        const int x = Px * n + uv % n, y = Py * n + uv / n;
+        if (x >= X || y >= Y) s = 0.0f;
+        const int64_t o = ((int64_t)(Px + T) * YP + (Py + T)) * S + uv;
+        img[o] = s;
+        if (hi) {
+            const u16 h = f2bf(s);
+            hi[o] = h;
+            lo[o] = f2bf(s - bf2f(h));
+        }
+        if (O_col && x < X && y < Y) O_col[(int64_t)x + (int64_t)X * y] = s;
+    }
+}
+
+// Sum the backward slabs of each plane.  Without rl_update: write the MATLAB
+// column-major volume B_col.  With rl_update (deconvRL.m:52,85-87):
+//   r = Htf / D; r(isinf) = 0; X = X .* r; X(isnan) = 0
+// on the padded volume (and its BF16 shadows), and copy X to B_col if given.
+__global__ void k_bwd_reduce(const float *__restrict__ part, const int *__restrict__ plane_slab0,
+                             int64_t slab_stride, int NP, int nrows, int nz, int n, int Ly,
+                             int64_t XP, int64_t YP, int T, int S, int X, int Y,
+                             float *__restrict__ B_col, int rl_update,
+                             const float *__restrict__ htf, float *__restrict__ vol,
+                             u16 *__restrict__ hi, u16 *__restrict__ lo)
+{
+    const int nn2 = n * n;
+    const int64_t per_plane = (int64_t)nrows * nn2;
+    const int64_t total = per_plane * nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(e / per_plane);
+        const int64_t rem = e - (int64_t)c * per_plane;
+        const int r = (int)(rem / nn2), ab = (int)(rem - (int64_t)r * nn2);
+        float d = 0.0f;
+        const float *p = part + (int64_t)r * NP + ab;
+        const int s0 = plane_slab0[c], s1 = plane_slab0[c + 1];
+        for (int k = s0; k < s1; ++k) d += p[(int64_t)k * slab_stride];
+        const int Px = r / Ly, Py = r - Px * Ly;
+        const int x = Px * n + ab % n, y = Py * n + ab / n;
+        const bool real = x < X && y < Y;
+        if (!rl_update) {
+            if (real) B_col[(int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c)] = d;
+            continue;
+        }
+        const int64_t o = (((int64_t)c * XP + Px + T) * YP + (Py + T)) * S + ab;
+        float v = 0.0f;
+        if (real) {
+            float ratio = htf[o] / d;
+            if (isinf(ratio)) ratio = 0.0f;
+            v = vol[o] * ratio;
+            if (isnan(v)) v = 0.0f;
+        }
+        vol[o] = v;
+        if (hi) {
+            const u16 h = f2bf(v);
+            hi[o] = h;
+            lo[o] = f2bf(v - bf2f(h));
+        }
+        if (B_col && real) B_col[(int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c)] = v;
+    }
+}
+
+// MATLAB column-major volume -> padded lenslet-major volume (+ BF16 shadows).
+__global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int n, int64_t XP,
+                         int64_t YP, int T, int S, float *__restrict__ dst, u16 *__restrict__ hi,
+                         u16 *__restrict__ lo)
+{
+    const int64_t total = (int64_t)X * Y * nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % X);
+        const int64_t t = e / X;
+        const int y = (int)(t % Y), c = (int)(t / Y);
+        const int Px = x / n, Py = y / n, ab = (x - Px * n) + n * (y - Py * n);
+        const int64_t o = (((int64_t)c * XP + Px + T) * YP + (Py + T)) * S + ab;
+        const float v = V[e];
+        if (dst) dst[o] = v;
+        if (hi) {
+            const u16 h = f2bf(v);
+            hi[o] = h;
+            lo[o] = f2bf(v - bf2f(h));
+        }
+    }
+}
+
+// Padded lenslet-major volume -> MATLAB column-major volume.
+__global__ void k_vol_out(const float *__restrict__ src, int X, int Y, int nz, int n, int64_t XP,
+                          int64_t YP, int T, int S, float *__restrict__ V)
+{
+    const int64_t total = (int64_t)X * Y * nz;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % X);
+        const int64_t t = e / X;
+        const int y = (int)(t % Y), c = (int)(t / Y);
+        const int Px = x / n, Py = y / n, ab = (x - Px * n) + n * (y - Py * n);
+        V[e] = src[(((int64_t)c * XP + Px + T) * YP + (Py + T)) * S + ab];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+static inline int64_t roundup(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+static int grid_for(int64_t total, int threads = 256, int cap = 65536)
+{
+    int64_t g = (total + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+int support_scan(sid_psf *p)
+{
+    const int64_t nz = p->nzH;
+    std::vector<int> box(nz * 4);
+    for (int64_t c = 0; c < nz; ++c) {
+        box[c * 4 + 0] = INT32_MAX; box[c * 4 + 1] = -1;
+        box[c * 4 + 2] = INT32_MAX; box[c * 4 + 3] = -1;
+    }
+    int *d_box = nullptr;
+    SID_TRY_HIP(hipMalloc(&d_box, box.size() * sizeof(int)));
+    SID_TRY_HIP(hipMemcpyAsync(d_box, box.data(), box.size() * sizeof(int), hipMemcpyHostToDevice, p->stream));
+    const int64_t per_plane = p->M * p->M * p->n * p->n;
+    dim3 grid((unsigned)std::min<int64_t>((per_plane + 255) / 256, 1024), (unsigned)nz);
+    hipLaunchKernelGGL(k_support_scan, grid, dim3(256), 0, p->stream, p->H, p->M, p->n, nz, d_box);
+    SID_TRY_HIP(hipGetLastError());
+    SID_TRY_HIP(hipMemcpyAsync(box.data(), d_box, box.size() * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    SID_TRY_HIP(hipFree(d_box));
+    for (int64_t c = 0; c < nz; ++c) {
+        Plane &pl = p->planes[c];
+        if (box[c * 4 + 1] < 0) {
+            pl.empty = true;
+            pl.lo = 0;
+            pl.hi = -1;
+        } else {
+            pl.lo = std::min(box[c * 4 + 0], box[c * 4 + 2]);
+            pl.hi = std::max(box[c * 4 + 1], box[c * 4 + 3]);
+            pl.empty = false;
+        }
+    }
+    return SID_OK;
+}
+
+// Tap range for offsets d = i - off, i in [lo, hi]: n*D + (u - a) = d with
+// |u - a| <= n - 1.
+static void tap_range(int lo, int hi, int off, int n, int *t_lo, int *t_hi)
+{
+    *t_lo = -floordiv(-(lo - off - (n - 1)), n);  // ceil
+    *t_hi = floordiv(hi - off + (n - 1), n);
+}
+
+int build_weights(sid_psf *p, int which)
+{
+    const int n = (int)p->n;
+    // geometry (once)
+    if (p->wf_elems == 0) {
+        int64_t offf = 0, offb = 0;
+        p->T = 0;
+        for (auto &pl : p->planes) {
+            if (pl.empty) {
+                pl.tf_lo = pl.tb_lo = 0;
+                pl.tf_hi = pl.tb_hi = -1;
+                pl.Kf = pl.Kb = 0;
+                pl.wf_off = offf;
+                pl.wb_off = offb;
+                continue;
+            }
+            tap_range(pl.lo, pl.hi, pl.off_f, n, &pl.tf_lo, &pl.tf_hi);
+            tap_range(pl.lo, pl.hi, pl.off_b, n, &pl.tb_lo, &pl.tb_hi);
+            const int ntf = pl.tf_hi - pl.tf_lo + 1, ntb = pl.tb_hi - pl.tb_lo + 1;
+            pl.Kf = roundup((int64_t)ntf * p->S, KPAD);
+            pl.Kb = roundup((int64_t)ntb * p->S, KPAD);
+            pl.wf_off = offf;
+            pl.wb_off = offb;
+            offf += (int64_t)ntf * pl.Kf * p->NP;
+            offb += (int64_t)ntb * pl.Kb * p->NP;
+            p->T = std::max<int64_t>(p->T, std::max(std::max(std::abs(pl.tf_lo), std::abs(pl.tf_hi)),
+                                                    std::max(std::abs(pl.tb_lo), std::abs(pl.tb_hi))));
+        }
+        p->wf_elems = offf;
+        p->wb_elems = offb;
+    }
+    const size_t esz = which == 0 ? 4 : 2;
+    void **dstf = which == 0 ? (void **)&p->wf : which == 1 ? (void **)&p->wf_hi : (void **)&p->wf_lo;
+    void **dstb = which == 0 ? (void **)&p->wb : which == 1 ? (void **)&p->wb_hi : (void **)&p->wb_lo;
+    if (*dstf) return SID_OK;
+    const int KC = which == 0 ? KChunk<0>::value : KChunk<1>::value;
+    SID_TRY_HIP(hipMalloc(dstf, std::max<int64_t>(p->wf_elems, 1) * esz));
+    SID_TRY_HIP(hipMalloc(dstb, std::max<int64_t>(p->wb_elems, 1) * esz));
+    for (int64_t c = 0; c < p->nzH; ++c) {
+        const Plane &pl = p->planes[c];
+        if (pl.empty) continue;
+        for (int dir = 0; dir < 2; ++dir) {
+            const int t_lo = dir == 0 ? pl.tf_lo : pl.tb_lo;
+            const int ntap = (dir == 0 ? pl.tf_hi : pl.tb_hi) - t_lo + 1;
+            const int64_t K = dir == 0 ? pl.Kf : pl.Kb;
+            const int off = dir == 0 ? pl.off_f : pl.off_b;
+            char *base = (char *)(dir == 0 ? *dstf : *dstb) + (dir == 0 ? pl.wf_off : pl.wb_off) * esz;
+            const int64_t total = (int64_t)ntap * K * p->NP;
+            const int grid = grid_for(total, 256, 16384);
+            if (which == 0)
+                hipLaunchKernelGGL(k_build_w<0>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+            else if (which == 1)
+                hipLaunchKernelGGL(k_build_w<1>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+            else
+                hipLaunchKernelGGL(k_build_w<2>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+            SID_TRY_HIP(hipGetLastError());
+        }
+    }
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return SID_OK;
+}
+
+static int row_tiles_for(int prec) { return prec == SID_PREC_FP32 ? 2 : 2; }
+
+// Split a list of segments (all costs = K) into `nsplit` contiguous groups of
+// near-equal total K.
+static std::vector<std::pair<int, int>> split_segments(const std::vector<Seg> &segs, int s0, int s1, int nsplit)
+{
+    std::vector<std::pair<int, int>> out;
+    int64_t total = 0;
+    for (int s = s0; s < s1; ++s) total += segs[s].K;
+    if (s1 <= s0) return out;
+    nsplit = std::max(1, std::min(nsplit, s1 - s0));
+    int start = s0;
+    int64_t acc = 0;
+    for (int s = s0; s < s1; ++s) {
+        acc += segs[s].K;
+        const int g = (int)out.size();
+        const int64_t target = total * (g + 1) / nsplit;
+        const int remaining_groups = nsplit - g - 1;
+        const int remaining_segs = s1 - s - 1;
+        if ((acc >= target && remaining_segs >= remaining_groups) || remaining_segs == remaining_groups) {
+            if (g < nsplit - 1) {
+                out.push_back({start, s + 1});
+                start = s + 1;
+            }
+        }
+    }
+    if (start < s1) out.push_back({start, s1});
+    return out;
+}
+
+static int upload_plan(GemmPlan &g)
+{
+    if (g.d_segs) { (void)hipFree(g.d_segs); g.d_segs = nullptr; }
+    if (g.d_items) { (void)hipFree(g.d_items); g.d_items = nullptr; }
+    if (g.d_plane_slab0) { (void)hipFree(g.d_plane_slab0); g.d_plane_slab0 = nullptr; }
+    SID_TRY_HIP(hipMalloc(&g.d_segs, std::max<size_t>(g.segs.size(), 1) * sizeof(Seg)));
+    SID_TRY_HIP(hipMalloc(&g.d_items, std::max<size_t>(g.items.size(), 1) * sizeof(Item)));
+    if (!g.segs.empty())
+        SID_TRY_HIP(hipMemcpy(g.d_segs, g.segs.data(), g.segs.size() * sizeof(Seg), hipMemcpyHostToDevice));
+    if (!g.items.empty())
+        SID_TRY_HIP(hipMemcpy(g.d_items, g.items.data(), g.items.size() * sizeof(Item), hipMemcpyHostToDevice));
+    if (!g.plane_slab0.empty()) {
+        SID_TRY_HIP(hipMalloc(&g.d_plane_slab0, g.plane_slab0.size() * sizeof(int)));
+        SID_TRY_HIP(hipMemcpy(g.d_plane_slab0, g.plane_slab0.data(), g.plane_slab0.size() * sizeof(int),
+                              hipMemcpyHostToDevice));
+    }
+    return SID_OK;
+}
+
+static void free_plan_buffers(SensorPlan &sp)
+{
+    for (void *q : {(void *)sp.vol, (void *)sp.img, (void *)sp.htf, (void *)sp.part, (void *)sp.vol_hi,
+                    (void *)sp.vol_lo, (void *)sp.img_hi, (void *)sp.img_lo})
+        if (q) (void)hipFree(q);
+    sp.vol = sp.img = sp.htf = sp.part = nullptr;
+    sp.vol_hi = sp.vol_lo = sp.img_hi = sp.img_lo = nullptr;
+    sp.prec_built = -1;
+}
+
+int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
+{
+    SensorPlan &sp = p->plan;
+    const int RT = row_tiles_for(p->precision);
+    const int64_t mt = 4 * RT * 16;
+    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol) return ensure_precision_buffers(p);
+    free_plan_buffers(sp);
+    const int64_t n = p->n, S = p->S, NP = p->NP, T = p->T;
+    sp.X = X; sp.Y = Y; sp.nz = nz;
+    sp.Lx = (X + n - 1) / n; sp.Ly = (Y + n - 1) / n;
+    sp.XP = sp.Lx + 2 * T; sp.YP = sp.Ly + 2 * T + 1;
+    sp.nrows = sp.Lx * sp.Ly;
+    sp.mt = mt;
+    sp.ntiles = (sp.nrows + mt - 1) / mt;
+    sp.mpad = sp.ntiles * mt;
+    int dev_cus = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
+    }
+    const int target_blocks = 4 * dev_cus;
+
+    // ---- forward: segments over planes 0..nz-1 and their dx rows
+    GemmPlan &F = sp.fwd;
+    F.segs.clear(); F.items.clear(); F.macs = 0;
+    for (int64_t c = 0; c < nz; ++c) {
+        const Plane &pl = p->planes[c];
+        if (pl.empty) continue;
+        const int ntap = pl.tf_hi - pl.tf_lo + 1;
+        for (int tx = 0; tx < ntap; ++tx) {
+            const int dx = pl.tf_lo + tx;
+            Seg sg;
+            sg.a_off = ((c * sp.XP + (T - dx)) * sp.YP + (T - pl.tf_hi)) * S;
+            sg.w_off = pl.wf_off + (int64_t)tx * pl.Kf * NP;
+            sg.K = (int32_t)pl.Kf;
+            sg.plane = (int32_t)c;
+            F.segs.push_back(sg);
+        }
+    }
+    {
+        const int nsplit = (int)std::max<int64_t>(1, (target_blocks + sp.ntiles - 1) / sp.ntiles);
+        auto groups = split_segments(F.segs, 0, (int)F.segs.size(), nsplit);
+        F.nslabs = (int)groups.size();
+        for (int gi = 0; gi < (int)groups.size(); ++gi)
+            for (int t = 0; t < sp.ntiles; ++t) F.items.push_back(Item{groups[gi].first, groups[gi].second, t, gi});
+        for (auto &sg : F.segs) F.macs += (int64_t)sg.K * NP * sp.mpad;
+    }
+
+    // ---- backward: every plane of H, split per plane in proportion to cost
+    GemmPlan &B = sp.bwd;
+    B.segs.clear(); B.items.clear(); B.plane_slab0.assign(p->nzH + 1, 0); B.macs = 0;
+    std::vector<int> pseg0(p->nzH + 1, 0);
+    int64_t btotal = 0;
+    for (int64_t c = 0; c < p->nzH; ++c) {
+        pseg0[c] = (int)B.segs.size();
+        const Plane &pl = p->planes[c];
+        if (pl.empty) continue;
+        const int ntap = pl.tb_hi - pl.tb_lo + 1;
+        for (int tx = 0; tx < ntap; ++tx) {
+            const int dx = pl.tb_lo + tx;
+            Seg sg;
+            sg.a_off = ((T + dx) * sp.YP + (T + pl.tb_lo)) * S;
+            sg.w_off = pl.wb_off + (int64_t)tx * pl.Kb * NP;
+            sg.K = (int32_t)pl.Kb;
+            sg.plane = (int32_t)c;
+            B.segs.push_back(sg);
+            btotal += sg.K;
+        }
+    }
+    pseg0[p->nzH] = (int)B.segs.size();
+    {
+        const int64_t per_split = std::max<int64_t>(1, btotal * sp.ntiles / target_blocks);
+        int slab = 0;
+        struct Tmp { Item it; int64_t cost; };
+        std::vector<Tmp> tmp;
+        for (int64_t c = 0; c < p->nzH; ++c) {
+            B.plane_slab0[c] = slab;
+            int64_t cost = 0;
+            for (int s = pseg0[c]; s < pseg0[c + 1]; ++s) cost += B.segs[s].K;
+            if (cost == 0) continue;
+            const int nsplit = (int)std::max<int64_t>(1, (cost + per_split - 1) / per_split);
+            auto groups = split_segments(B.segs, pseg0[c], pseg0[c + 1], nsplit);
+            for (auto &gr : groups) {
+                int64_t gc = 0;
+                for (int s = gr.first; s < gr.second; ++s) gc += B.segs[s].K;
+                for (int t = 0; t < sp.ntiles; ++t) tmp.push_back({Item{gr.first, gr.second, t, slab}, gc});
+                ++slab;
+            }
+        }
+        B.plane_slab0[p->nzH] = slab;
+        B.nslabs = slab;
+        std::stable_sort(tmp.begin(), tmp.end(), [](const Tmp &a, const Tmp &b) { return a.cost > b.cost; });
+        for (auto &t : tmp) B.items.push_back(t.it);
+        for (auto &sg : B.segs) B.macs += (int64_t)sg.K * NP * sp.mpad;
+    }
+    SID_TRY(upload_plan(F));
+    SID_TRY(upload_plan(B));
+
+    // ---- buffers
+    sp.vol_elems = p->nzH * sp.XP * sp.YP * S;
+    sp.img_elems = sp.XP * sp.YP * S;
+    sp.part_elems = (int64_t)std::max(std::max(F.nslabs, B.nslabs), 1) * sp.mpad * NP;
+    SID_TRY_HIP(hipMalloc(&sp.vol, sp.vol_elems * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&sp.htf, sp.vol_elems * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&sp.img, sp.img_elems * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&sp.part, sp.part_elems * sizeof(float)));
+    SID_TRY_HIP(hipMemsetAsync(sp.vol, 0, sp.vol_elems * sizeof(float), p->stream));
+    SID_TRY_HIP(hipMemsetAsync(sp.htf, 0, sp.vol_elems * sizeof(float), p->stream));
+    SID_TRY_HIP(hipMemsetAsync(sp.img, 0, sp.img_elems * sizeof(float), p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return ensure_precision_buffers(p);
+}
+
+int ensure_precision_buffers(sid_psf *p)
+{
+    SensorPlan &sp = p->plan;
+    if (p->precision == SID_PREC_FP32) return SID_OK;
+    SID_TRY(build_weights(p, 1));
+    if (p->precision == SID_PREC_BF16X3) SID_TRY(build_weights(p, 2));
+    if (!sp.vol_hi) {
+        SID_TRY_HIP(hipMalloc(&sp.vol_hi, sp.vol_elems * sizeof(u16)));
+        SID_TRY_HIP(hipMalloc(&sp.vol_lo, sp.vol_elems * sizeof(u16)));
+        SID_TRY_HIP(hipMalloc(&sp.img_hi, sp.img_elems * sizeof(u16)));
+        SID_TRY_HIP(hipMalloc(&sp.img_lo, sp.img_elems * sizeof(u16)));
+        SID_TRY_HIP(hipMemsetAsync(sp.vol_hi, 0, sp.vol_elems * sizeof(u16), p->stream));
+        SID_TRY_HIP(hipMemsetAsync(sp.vol_lo, 0, sp.vol_elems * sizeof(u16), p->stream));
+        SID_TRY_HIP(hipMemsetAsync(sp.img_hi, 0, sp.img_elems * sizeof(u16), p->stream));
+        SID_TRY_HIP(hipMemsetAsync(sp.img_lo, 0, sp.img_elems * sizeof(u16), p->stream));
+        SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    }
+    return SID_OK;
+}
+
+template <int NT, int RT, int MODE>
+static void launch_one(const GemmPlan &g, const void *A0, const void *A1, const void *W0, const void *W1,
+                       const SensorPlan &sp, int S, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_proj_gemm<NT, RT, MODE>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
+                       g.d_items, g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                       sp.mpad * (int64_t)(NT * 16));
+}
+
+template <int NT>
+static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st)
+{
+    const SensorPlan &sp = p->plan;
+    const int S = (int)p->S;
+    const bool fwd = dir == 0;
+    if (p->precision == SID_PREC_FP32) {
+        launch_one<NT, 2, 0>(g, fwd ? (const void *)sp.vol : (const void *)sp.img, nullptr,
+                             fwd ? (const void *)p->wf : (const void *)p->wb, nullptr, sp, S, st);
+    } else if (p->precision == SID_PREC_BF16X2) {
+        launch_one<NT, 2, 1>(g, fwd ? (const void *)sp.vol_hi : (const void *)sp.img_hi,
+                             fwd ? (const void *)sp.vol_lo : (const void *)sp.img_lo,
+                             fwd ? (const void *)p->wf_hi : (const void *)p->wb_hi, nullptr, sp, S, st);
+    } else {
+        launch_one<NT, 2, 2>(g, fwd ? (const void *)sp.vol_hi : (const void *)sp.img_hi,
+                             fwd ? (const void *)sp.vol_lo : (const void *)sp.img_lo,
+                             fwd ? (const void *)p->wf_hi : (const void *)p->wb_hi,
+                             fwd ? (const void *)p->wf_lo : (const void *)p->wb_lo, sp, S, st);
+    }
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int launch_gemm(sid_psf *p, int dir, hipStream_t st)
+{
+    const GemmPlan &g = dir == 0 ? p->plan.fwd : p->plan.bwd;
+    if (g.items.empty()) return SID_OK;
+    switch (p->NP / 16) {
+        case 1: return launch_nt<1>(p, g, dir, st);
+        case 2: return launch_nt<2>(p, g, dir, st);
+        case 4: return launch_nt<4>(p, g, dir, st);
+        case 8: return launch_nt<8>(p, g, dir, st);
+        case 11: return launch_nt<11>(p, g, dir, st);
+        case 15: return launch_nt<15>(p, g, dir, st);
+        case 16: return launch_nt<16>(p, g, dir, st);
+        default:
+            set_error("unsupported padded lenslet size NP=%lld", (long long)p->NP);
+            return SID_ERR_DIMS;
+    }
+}
+
+int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    const bool sh = p->precision != SID_PREC_FP32;
+    const int64_t total = sp.nrows * p->n * p->n;
+    if (sp.fwd.nslabs == 0) {
+        // no contributing plane: the image is zero
+        SID_TRY_HIP(hipMemsetAsync(sp.part, 0, sp.mpad * p->NP * sizeof(float), st));
+    }
+    hipLaunchKernelGGL(k_fwd_reduce, dim3(grid_for(total)), dim3(256), 0, st, sp.part, std::max(sp.fwd.nslabs, 1),
+                       sp.mpad * p->NP, (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T,
+                       (int)p->S, (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr,
+                       O_col);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    const bool sh = p->precision != SID_PREC_FP32;
+    const int64_t total = sp.nrows * p->n * p->n * p->nzH;
+    hipLaunchKernelGGL(k_bwd_reduce, dim3(grid_for(total)), dim3(256), 0, st, sp.part, sp.bwd.d_plane_slab0,
+                       sp.mpad * p->NP, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n, (int)sp.Ly, sp.XP, sp.YP,
+                       (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update, sp.htf, sp.vol,
+                       sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, float *dst, int shadow, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    const bool sh = shadow && p->precision != SID_PREC_FP32;
+    const int64_t total = sp.X * sp.Y * nz;
+    if (total == 0) return SID_OK;
+    hipLaunchKernelGGL(k_vol_in, dim3(grid_for(total)), dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y, (int)nz,
+                       (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, dst, sh ? sp.vol_hi : nullptr,
+                       sh ? sp.vol_lo : nullptr);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    const bool sh = p->precision != SID_PREC_FP32;
+    const int64_t total = sp.X * sp.Y;
+    hipLaunchKernelGGL(k_vol_in, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1, (int)p->n,
+                       sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img, sh ? sp.img_hi : nullptr,
+                       sh ? sp.img_lo : nullptr);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int launch_vol_out(sid_psf *p, const float *src, float *V_col, int64_t nz, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    const int64_t total = sp.X * sp.Y * nz;
+    if (total == 0) return SID_OK;
+    hipLaunchKernelGGL(k_vol_out, dim3(grid_for(total)), dim3(256), 0, st, src, (int)sp.X, (int)sp.Y, (int)nz,
+                       (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, V_col);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Self-test of the MFMA lane layouts assumed above.
+// ---------------------------------------------------------------------------
+__global__ void k_selftest(const float *A, const float *B, float *C32, float *C16)
+{
+    // A: 16 x 32 row-major, B: 32 x 16 row-major
+    const int lane = threadIdx.x, l16 = lane & 15, g = lane >> 4;
+    f4 c = {0, 0, 0, 0};
+    for (int s = 0; s < 8; ++s)
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(A[l16 * 32 + 8 * g + s], B[(8 * g + s) * 16 + l16], c, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) C32[(4 * g + i) * 16 + l16] = c[i];
+    bf8 a, b;
+    for (int j = 0; j < 8; ++j) {
+        a[j] = (__bf16)A[l16 * 32 + 8 * g + j];
+        b[j] = (__bf16)B[(8 * g + j) * 16 + l16];
+    }
+    f4 d = {0, 0, 0, 0};
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, d, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) C16[(4 * g + i) * 16 + l16] = d[i];
+}
+
+int run_selftest()
+{
+    std::vector<float> A(16 * 32), B(32 * 16), C32(256), C16(256);
+    // small integers: exact in bf16 and in fp32 accumulation
+    for (int i = 0; i < 16 * 32; ++i) A[i] = (float)((i * 7 + 3) % 11) - 5.0f;
+    for (int i = 0; i < 32 * 16; ++i) B[i] = (float)((i * 5 + 1) % 13) - 6.0f;
+    float *dA, *dB, *dC, *dD;
+    SID_TRY_HIP(hipMalloc(&dA, A.size() * 4));
+    SID_TRY_HIP(hipMalloc(&dB, B.size() * 4));
+    SID_TRY_HIP(hipMalloc(&dC, 256 * 4));
+    SID_TRY_HIP(hipMalloc(&dD, 256 * 4));
+    SID_TRY_HIP(hipMemcpy(dA, A.data(), A.size() * 4, hipMemcpyHostToDevice));
+    SID_TRY_HIP(hipMemcpy(dB, B.data(), B.size() * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA, dB, dC, dD);
+    SID_TRY_HIP(hipGetLastError());
+    SID_TRY_HIP(hipMemcpy(C32.data(), dC, 256 * 4, hipMemcpyDeviceToHost));
+    SID_TRY_HIP(hipMemcpy(C16.data(), dD, 256 * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD);
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) {
+            double s = 0;
+            for (int k = 0; k < 32; ++k) s += (double)A[i * 32 + k] * B[k * 16 + j];
+            if (C32[i * 16 + j] != (float)s || C16[i * 16 + j] != (float)s) {
+                set_error("MFMA layout self-test failed at (%d,%d): fp32 %g bf16 %g expected %g", i, j,
+                          C32[i * 16 + j], C16[i * 16 + j], s);
+                return SID_ERR_HIP;
+            }
+        }
+    return SID_OK;
+}
+
+}  // namespace sid

@@ -1,0 +1,133 @@
+// Internal declarations shared by the libsidproj translation units.
+//
+// Device layout of the projector (lenslet-major, zero-padded):
+//   n       pixels per lenslet side (Nnum); Lx = ceil(X/n), Ly = ceil(Y/n)
+//   S       per-lenslet stride: n*n entries rounded up to a multiple of 8
+//   T       lenslet padding on every side (largest tap radius over planes)
+//   XP, YP  padded lenslet grid: XP = Lx + 2T, YP = Ly + 2T + 1
+//   volume  vol[c][xp][yp][ab]   ab = a + n*b  (a: row offset, b: column offset in the lenslet)
+//   image   img[xp][yp][uv]      uv = u + n*v
+// The projector is a sum of GEMMs over lenslet tap rows dx (see sid_direct.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sid_proj.h"
+
+namespace sid {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+void set_error(const char *fmt, ...);
+
+#define SID_TRY_HIP(call)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            ::sid::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                      \
+            return e_ == hipErrorOutOfMemory ? SID_ERR_OOM : SID_ERR_HIP;                    \
+        }                                                                                    \
+    } while (0)
+
+#define SID_TRY(expr)               \
+    do {                            \
+        int s_ = (expr);            \
+        if (s_ != SID_OK) return s_; \
+    } while (0)
+
+// One GEMM segment: a (plane, tap row) pair.  For output row r (lenslet
+// (Px, Py) = (r / Ly, r % Ly)) the A operand row starts at
+// a_off + (Px*YP + Py)*S and runs K contiguous elements (Toeplitz along y);
+// the B operand is K x NP contiguous at w_off.
+struct Seg {
+    int64_t a_off;
+    int64_t w_off;
+    int32_t K;
+    int32_t plane;
+};
+
+// One block of work: segments [seg0, seg1) for M tile `mtile`, written to
+// partial-sum slab `slab`.
+struct Item {
+    int32_t seg0, seg1, mtile, slab;
+};
+
+// Per-plane kernel support and lenslet tap ranges.
+struct Plane {
+    int lo, hi;          // inclusive kernel index range used (both axes)
+    int off_f, off_b;    // p - q = i - off (forward / backward)
+    int tf_lo, tf_hi;    // forward taps dx, dy in [tf_lo, tf_hi]
+    int tb_lo, tb_hi;    // backward taps
+    int64_t Kf, Kb;      // padded GEMM K per tap row
+    int64_t wf_off, wb_off;  // weight offsets (elements)
+    bool empty;
+};
+
+constexpr int KPAD = 64;     // K is padded to a multiple of this
+constexpr int NTHREADS = 256;
+
+struct GemmPlan {             // one direction on one sensor geometry
+    std::vector<Seg> segs;
+    std::vector<Item> items;
+    std::vector<int> slab_plane;      // backward: plane of each slab
+    std::vector<int> plane_slab0;     // backward: first slab of plane c (size nz+1)
+    Seg *d_segs = nullptr;
+    Item *d_items = nullptr;
+    int *d_plane_slab0 = nullptr;
+    int nslabs = 0;
+    int64_t macs = 0;                 // MFMA multiply-adds issued (padding included)
+};
+
+struct SensorPlan {
+    int64_t X = 0, Y = 0, nz = 0;     // nz of the forward input
+    int64_t Lx = 0, Ly = 0, XP = 0, YP = 0;
+    int64_t nrows = 0, ntiles = 0, mpad = 0, mt = 0;
+    GemmPlan fwd, bwd;
+    // device buffers (fp32 master + bf16 hi/lo shadows)
+    float *vol = nullptr, *img = nullptr, *htf = nullptr, *part = nullptr;
+    u16 *vol_hi = nullptr, *vol_lo = nullptr, *img_hi = nullptr, *img_lo = nullptr;
+    int64_t vol_elems = 0, img_elems = 0, part_elems = 0;
+    int prec_built = -1;
+};
+
+}  // namespace sid
+
+struct sid_psf {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t M = 0, n = 0, nzH = 0;
+    int64_t S = 0, NP = 0, T = 0;
+    bool cropped = false;
+    std::vector<sid::Plane> planes;
+    float *H = nullptr;               // device copy, MATLAB layout
+    float *wf = nullptr, *wb = nullptr;           // fp32 weights (forward / backward)
+    sid::u16 *wf_hi = nullptr, *wb_hi = nullptr;  // bf16 hi weights
+    sid::u16 *wf_lo = nullptr, *wb_lo = nullptr;  // bf16 lo weights (x3 mode)
+    int64_t wf_elems = 0, wb_elems = 0;
+    int precision = SID_PREC_FP32;
+    int method = SID_METHOD_AUTO;
+    sid::SensorPlan plan;
+    hipEvent_t ev[10] = {};
+    double timing[8] = {0};
+};
+
+namespace sid {
+// sid_direct.hip
+int build_weights(sid_psf *p, int which /*0 fp32, 1 bf16 hi, 2 bf16 lo*/);
+int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz);
+int ensure_precision_buffers(sid_psf *p);
+int launch_gemm(sid_psf *p, int dir, hipStream_t st);
+int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st);
+int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st);
+int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, float *dst, int shadow, hipStream_t st);
+int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st);
+int launch_vol_out(sid_psf *p, const float *src, float *V_col, int64_t nz, hipStream_t st);
+int support_scan(sid_psf *p);
+int run_selftest();
+}  // namespace sid
