@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Benchmark: ISRA/RL reconstruction throughput of the light-field projector.
+
+Workload (default ``rl_full``): 1005 x 1005 px sensor, Nnum = 15 (67 x 67
+lenslets), 51 planes at z = -25..25, 345 px PSF array with per-plane
+footprint up to 345 px, 3000 seeded ball neurons, Poisson image at a
+1000-photon peak, reconstruct_S pre-processing, 8 ISRA iterations
+(deconvRL basic, lambda = 0) entirely on one MI355X.
+
+A "step" is one 8-iteration reconstruction (sid_deconv_rl_device); ``value``
+is RL iterations per second of the BF16 hi/lo split MFMA path, with the
+plain FP32 path timed in the same run.  Inputs are resident in HBM before
+the timed region.  Errors are rel-L2 against the FP64 device path after
+each of the 8 iterations.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+PEAK = {"fp32": 157.3e12, "bf16x2": 2.5e15, "bf16x3": 2.5e15, "fp64": 78.6e12}
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, V, y_pre, sample_planes=None):
+    """Reference CPU path restated in C (oracle/liboracle.so: forwardProjectACC
+    in single precision, backwardProjectACC_new stored double, CAindex crop)
+    timed on the host cores over a sample of planes; scaled to one RL
+    iteration by the algorithmic work ratio sum(M_c^2)."""
+    from oracle.build_oracle import build as build_oracle
+    from sid_amd import synth
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "liboracle.so")
+    if not os.path.exists(path):
+        build_oracle()
+    L = ctypes.CDLL(path)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nz = wl.nz
+    if sample_planes is None:
+        sample_planes = sorted({0, nz // 4, nz // 2, (3 * nz) // 4})
+    zs = [wl.zs[c] for c in sample_planes]
+    Hs, CAs = synth.make_psf(wl.n, wl.M, zs, dtype=np.float32)
+    Vs = np.asfortranarray(V[:, :, sample_planes].astype(np.float64))
+    X, Y = wl.X, wl.Y
+    O = np.zeros((X, Y), dtype=np.float32, order="F")
+    B = np.zeros((X, Y, len(zs)), dtype=np.float64, order="F")
+    yf = np.asfortranarray(y_pre.astype(np.float32))
+    hd = (ctypes.c_int64 * 5)(*Hs.shape)
+    ca = np.asfortranarray(CAs.astype(np.int32))
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    t0 = time.perf_counter()
+    L.orc_forward_f32(P(Hs), hd, P(ca), P(Vs), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(len(zs)), P(O),
+                      ctypes.c_int(threads))
+    t1 = time.perf_counter()
+    L.orc_backward_f32(P(Hs), hd, P(ca), P(yf), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(len(zs)), P(B),
+                       ctypes.c_int(threads))
+    t2 = time.perf_counter()
+    msum_all = sum((hi - lo + 1) ** 2 for lo, hi in synth.ca_index(wl.zs, wl.M))
+    msum_s = sum((hi - lo + 1) ** 2 for lo, hi in CAs)
+    t_iter = (t2 - t0) * msum_all / msum_s
+    return {"value": 1.0 / t_iter, "unit": "RL iter/s", "cores": threads, "kind": "port",
+            "sample": (f"forwardProjectACC (single) + backwardProjectACC_new restated in C (OpenMP, -O3), "
+                       f"planes z={zs} of the same workload, {t2 - t0:.2f} s measured (fwd {t1 - t0:.2f} s, "
+                       f"bwd {t2 - t1:.2f} s), scaled by sum(M_c^2) all/sample = {msum_all}/{msum_s}")}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--iters", type=int, default=8, help="RL iterations per step")
+    ap.add_argument("--workload", default="rl_full")
+    ap.add_argument("--precisions", default="fp32,bf16x2", help="first is the baseline, last is the headline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-ref", action="store_true", help="skip the FP64 error reference")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the projector is measured on one "
+              f"MI355X; multi-GPU sharding of this path is a design note, not a measured run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import sid_amd
+    from sid_amd import recon, synth
+
+    if sid_amd.device_count() < 1:
+        print("no gfx950 device visible", file=sys.stderr)
+        sys.exit(1)
+    wl = synth.WORKLOADS[args.workload]
+    dev = torch.device("cuda:0")
+    log(f"workload {wl.name}: X={wl.X} Y={wl.Y} n={wl.n} Nz={wl.nz} M={wl.M}")
+    t = time.perf_counter()
+    H, CA = synth.make_psf(wl.n, wl.M, wl.zs, xp=torch, device=dev)
+    torch.cuda.synchronize()
+    log(f"PSF generated on device in {time.perf_counter() - t:.1f} s ({H.numel() * 4 / 1e9:.2f} GB)")
+    t = time.perf_counter()
+    psf = sid_amd.PSF(H)
+    del H
+    torch.cuda.empty_cache()
+    info = psf.info()
+    log(f"PSF handle built in {time.perf_counter() - t:.1f} s; taps_max={info['taps_max']} "
+        f"weights {info['weight_bytes'] / 1e9:.2f} GB")
+    V = synth.make_volume(wl.X, wl.Y, wl.nz, wl.n_balls, wl.r_lo, wl.r_hi, wl.seed, wl.background)
+    Vt = torch.from_numpy(np.ascontiguousarray(V.transpose(2, 1, 0))).to(dev)
+    clean = psf.forward(Vt)
+    img = synth.poisson_image(clean.cpu().numpy().T, wl.peak_photons, wl.seed)
+    y, nrm = recon.preprocess(img)
+    yt = torch.from_numpy(np.ascontiguousarray(y.T.astype(np.float32))).to(dev)
+    htf = psf.backward(yt)
+    torch.cuda.synchronize()
+    log("inputs ready (volume, Poisson image, Htf = B(y/nrm))")
+
+    # algorithmic work per projection: X*Y*sum_c M_c^2 MACs (M_c = plane support side)
+    msum = sum((hi - lo + 1) ** 2 for lo, hi in info["support"] if hi >= lo)
+    flops_proj = 2.0 * wl.X * wl.Y * msum
+
+    results = {}
+    htfs = {}
+    for prec in args.precisions.split(","):
+        psf.set_precision(prec)
+        # Xguess = Htf = B(img/nrm) in the same arithmetic as the iterations (reconstruct_S.m:192-196)
+        htfs[prec] = psf.backward(yt)
+        for _ in range(args.warmup):
+            psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
+        torch.cuda.synchronize()
+        tim = np.zeros(8)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
+            tim += np.array(psf.timing())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms_step = (t1 - t0) * 1e3 / args.steps
+        n_launch = args.steps * args.iters
+        t_f, t_b = tim[0] / n_launch, tim[2] / n_launch
+        avg_launch_ms = 0.5 * (t_f + t_b)
+        achieved = flops_proj / (avg_launch_ms * 1e-3)
+        results[prec] = dict(iter_per_s=args.steps * args.iters / (t1 - t0), ms_per_step=ms_step,
+                             fwd_gemm_ms=t_f, bwd_gemm_ms=t_b, fwd_reduce_ms=tim[1] / n_launch,
+                             bwd_reduce_ms=tim[3] / n_launch, achieved_tflops=achieved / 1e12,
+                             frac_of_peak=achieved / PEAK[prec],
+                             proj_per_s=2.0 / ((t_f + t_b + (tim[1] + tim[3]) / n_launch) * 1e-3))
+        log(f"{prec}: {results[prec]['iter_per_s']:.2f} RL iter/s, {ms_step:.1f} ms/step, fwd GEMM {t_f:.2f} ms, "
+            f"bwd GEMM {t_b:.2f} ms, {achieved / 1e12:.1f} TFLOP/s algorithmic")
+
+    # errors against the FP64 device path (FP64 MFMA, float I/O), per output and per iteration
+    rel = {}
+    ref_name = None
+    if not args.no_ref:
+        psf.set_precision("fp64")
+        ref_name = "fp64"
+        t = time.perf_counter()
+        O64 = psf.forward(Vt).double()
+        h64 = psf.backward(yt)
+        _, hist64 = psf.deconv_rl(h64, args.iters, h64, history=True)
+        torch.cuda.synchronize()
+        log(f"FP64 reference done in {time.perf_counter() - t:.1f} s")
+        h64d = h64.double()
+        for prec in htfs:
+            psf.set_precision(prec)
+            Op = psf.forward(Vt).double()
+            _, h = psf.deconv_rl(htfs[prec], args.iters, htfs[prec], history=True)
+            per_it = []
+            max_abs = []
+            for i in range(args.iters):
+                a, b = h[i].double(), hist64[i].double()
+                per_it.append(float((a - b).norm() / b.norm()))
+                max_abs.append(float((a - b).abs().max() / b.abs().max()))
+            rel[prec] = {"forward": float((Op - O64).norm() / O64.norm()),
+                         "backward": float((htfs[prec].double() - h64d).norm() / h64d.norm()),
+                         "rl_per_iter": per_it, "rl_max_abs_rel_per_iter": max_abs, "final": per_it[-1]}
+            log(f"{prec} rel-L2 vs FP64: forward {rel[prec]['forward']:.2e} backward {rel[prec]['backward']:.2e} "
+                f"RL per iteration " + " ".join(f"{e:.2e}" for e in per_it))
+            del h
+        del hist64
+
+    head = args.precisions.split(",")[-1]
+    base = args.precisions.split(",")[0]
+    hr = results[head]
+    peak = PEAK[head]
+    out = {
+        "metric": "RL iter/s @1005²×51z: BF16-split MFMA vs plain FP32; rel-L2 err vs FP64",
+        "value": round(hr["iter_per_s"], 4),
+        "unit": "RL iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if head.startswith("bf16") else head,
+        "data": "synthetic (seeded PSF, ball volume, Poisson image)",
+        "config": {"workload": f"{wl.name}: {wl.X}x{wl.Y} px, Nnum={wl.n}, {wl.nz} planes z={wl.zs[0]}..{wl.zs[-1]}, "
+                               f"M={wl.M}, {wl.n_balls} balls, {args.iters} ISRA iterations per step",
+                   "method": head, "baseline_method": base, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": round(hr["achieved_tflops"], 3), "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": round(hr["achieved_tflops"] * 1e12 / peak, 4), "traffic": None,
+                     "kernel": "k_proj_gemm (forward and backward launches)",
+                     "flops_per_launch": flops_proj},
+        "methods": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in results.items()},
+        "speedup_vs_fp32": round(hr["iter_per_s"] / results[base]["iter_per_s"], 3) if base in results else None,
+        "rel_l2_vs": ref_name,
+        "rel_l2": rel,
+    }
+    if not args.no_cpu:
+        log("CPU baseline (bounded sample) ...")
+        out["cpu_baseline"] = cpu_baseline(wl, V, y)
+        log(f"CPU baseline {out['cpu_baseline']['value']:.5f} RL iter/s on {out['cpu_baseline']['cores']} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -53,7 +53,8 @@ typedef enum {
 typedef enum {
     SID_PREC_FP32 = 0,      /* plain method: FP32 operands, FP32 MFMA accumulate */
     SID_PREC_BF16X2 = 1,    /* PSF in BF16, volume/image split hi+lo BF16, FP32 accumulate */
-    SID_PREC_BF16X3 = 2     /* PSF and volume/image both split hi+lo (3 products), FP32 accumulate */
+    SID_PREC_BF16X3 = 2,    /* PSF and volume/image both split hi+lo (3 products), FP32 accumulate */
+    SID_PREC_FP64 = 3       /* FP64 operands and MFMA (reference path for error studies); float I/O */
 } sid_precision;
 
 /* Per-plane method of the projector. */

@@ -111,6 +111,7 @@ static void crop_of(const int32_t *CA, int64_t nzH, int64_t M, int64_t c, int64_
                         int64_t i0 = 0, i1 = Mc - 1;                                                  \
                         if (qx - offb < 0) i0 = offb - qx;                                            \
                         if (qx - offb + Mc - 1 > X - 1) i1 = X - 1 - qx + offb;                       \
+                        _Pragma("omp simd reduction(+ : s)")                                          \
                         for (int64_t i = i0; i <= i1; ++i)                                            \
                             s += (ACC)Oc[qx + i - offb] * (ACC)Kc[i];                                 \
                     }                                                                                 \

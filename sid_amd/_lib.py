@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libsidproj.so")
 SID_OK = 0
 STATUS_NAMES = {0: "SID_OK", 1: "SID_ERR_ARG", 2: "SID_ERR_DIMS", 3: "SID_ERR_NZ", 4: "SID_ERR_OOM",
                 5: "SID_ERR_HIP", 6: "SID_ERR_STATE", 7: "SID_ERR_NODEV"}
-PREC = {"fp32": 0, "bf16x2": 1, "bf16x3": 2}
+PREC = {"fp32": 0, "bf16x2": 1, "bf16x3": 2, "fp64": 3}
 METHOD = {"auto": 0, "direct": 1, "fft": 2}
 
 # Every exported symbol with (restype, argtypes); tests check this list

@@ -129,11 +129,11 @@ static void destroy(sid_psf *p)
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     SensorPlan &sp = p->plan;
-    for (void *q : {(void *)sp.vol, (void *)sp.img, (void *)sp.htf, (void *)sp.part, (void *)sp.vol_hi,
-                    (void *)sp.vol_lo, (void *)sp.img_hi, (void *)sp.img_lo, (void *)sp.fwd.d_segs,
-                    (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
+    free_plan_buffers(sp);
+    for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
                     (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
-                    (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo})
+                    (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
+                    (void *)p->wf_d, (void *)p->wb_d})
         if (q) (void)hipFree(q);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -233,16 +233,15 @@ int sid_psf_destroy(sid_psf *psf)
 int sid_psf_set_precision(sid_psf *p, int precision)
 {
     if (!p) { set_error("null handle"); return SID_ERR_ARG; }
-    if (precision < SID_PREC_FP32 || precision > SID_PREC_BF16X3) {
+    if (precision < SID_PREC_FP32 || precision > SID_PREC_FP64) {
         set_error("unknown precision %d", precision);
         return SID_ERR_ARG;
     }
     SID_TRY_HIP(hipSetDevice(p->device));
     p->precision = precision;
-    if (precision != SID_PREC_FP32) {
-        SID_TRY(build_weights(p, 1));
-        if (precision == SID_PREC_BF16X3) SID_TRY(build_weights(p, 2));
-    }
+    if (precision == SID_PREC_BF16X2 || precision == SID_PREC_BF16X3) SID_TRY(build_weights(p, 1));
+    if (precision == SID_PREC_BF16X3) SID_TRY(build_weights(p, 2));
+    if (precision == SID_PREC_FP64) SID_TRY(build_weights(p, 3));
     if (p->plan.vol) SID_TRY(ensure_plan(p, p->plan.X, p->plan.Y, p->plan.nz));
     return SID_OK;
 }
@@ -266,6 +265,7 @@ int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
     int64_t wb = (p->wf_elems + p->wb_elems) * 4;
     if (p->wf_hi) wb += (p->wf_elems + p->wb_elems) * 2;
     if (p->wf_lo) wb += (p->wf_elems + p->wb_elems) * 2;
+    if (p->wf_d) wb += (p->wf_elems + p->wb_elems) * 8;
     info->weight_bytes = wb;
     for (int64_t c = 0; c < p->nzH && c < 512; ++c) {
         info->support_lo[c] = p->planes[c].empty ? 0 : p->planes[c].lo;
@@ -298,7 +298,7 @@ static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], 
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     Timer tm(p, st);
     const int e0 = tm.mark();
-    SID_TRY(launch_vol_in(p, V_dev, vdims[2], p->plan.vol, 1, st));
+    SID_TRY(launch_vol_in(p, V_dev, vdims[2], 0, st));
     const int e1 = tm.mark();
     SID_TRY(launch_gemm(p, 0, st));
     const int e2 = tm.mark();
@@ -398,8 +398,8 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
     const int64_t nv = vdims[0] * vdims[1] * vdims[2];
     Timer tm(p, st);
     const int e0 = tm.mark();
-    SID_TRY(launch_vol_in(p, Htf, vdims[2], sp.htf, 0, st));
-    SID_TRY(launch_vol_in(p, X, vdims[2], sp.vol, 1, st));
+    SID_TRY(launch_vol_in(p, Htf, vdims[2], 1, st));
+    SID_TRY(launch_vol_in(p, X, vdims[2], 0, st));
     for (int it = 0; it < max_iter; ++it) {
         const int a = tm.mark();
         SID_TRY(launch_gemm(p, 0, st));
@@ -413,7 +413,7 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
         tm.span(0, a); tm.span(1, b); tm.span(2, c); tm.span(3, d);
         (void)e;
     }
-    SID_TRY(launch_vol_out(p, sp.vol, X, vdims[2], st));
+    SID_TRY(launch_vol_out(p, X, vdims[2], st));
     const int e1 = tm.mark();
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e1);

@@ -78,7 +78,7 @@ __global__ void k_support_scan(const float *__restrict__ H, int64_t M, int64_t n
 // stored chunk-transposed: [K / KC][NP][KC] so one LDS stage is one
 // contiguous copy with K innermost.
 // ---------------------------------------------------------------------------
-template <int PART>  // 0: fp32, 1: bf16 hi, 2: bf16 lo
+template <int PART>  // 0: fp32, 1: bf16 hi, 2: bf16 lo, 3: fp64
 __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, int64_t M, int n,
                           int c, int dir, int lo, int hi, int off, int t_lo, int ntap, int64_t K,
                           int S, int NP, int KC)
@@ -109,6 +109,8 @@ __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, 
         }
         if (PART == 0) {
             ((float *)Wout)[idx] = v;
+        } else if (PART == 3) {
+            ((double *)Wout)[idx] = (double)v;
         } else {
             u16 h = f2bf(v);
             ((u16 *)Wout)[idx] = PART == 1 ? h : f2bf(v - bf2f(h));
@@ -262,37 +264,118 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
     }
 }
 
+
+// FP64 variant (reference path for error studies): v_mfma_f64_16x16x4_f64
+// with the same K permutation as MODE 0 (lane group g supplies k = 8g + s at
+// step s of a 32-wide chunk).
+template <int NT, int RT>
+__global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
+    const Item *__restrict__ items, const Seg *__restrict__ segs, const double *__restrict__ A0,
+    const double *__restrict__ W0, double *__restrict__ part, int nrows, int Ly, int YP, int S,
+    int64_t slab_stride)
+{
+    constexpr int NP = NT * 16;
+    constexpr int MT = 4 * RT * 16;
+    constexpr int KC = 32;
+    constexpr int LDK = KC + 2;
+    __shared__ double Bs[NP * LDK];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const Item it = items[blockIdx.x];
+    int64_t rowoff[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        int r = it.mtile * MT + wave * RT * 16 + rt * 16 + l16;
+        r = r < nrows ? r : nrows - 1;
+        const int Px = r / Ly, Py = r - Px * Ly;
+        rowoff[rt] = ((int64_t)Px * YP + Py) * S;
+    }
+    d4 acc[RT][NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = d4{0., 0., 0., 0.};
+    for (int s = it.seg0; s < it.seg1; ++s) {
+        const Seg sg = segs[s];
+        const double *A = A0 + sg.a_off;
+        const double *W = W0 + sg.w_off;
+        for (int k0 = 0; k0 < sg.K; k0 += KC) {
+            double2 a[RT][4];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[rt][q] = *(const double2 *)(A + rowoff[rt] + k0 + 8 * g + 2 * q);
+            __syncthreads();
+            const double *wc = W + (int64_t)k0 * NP;
+            for (int e = threadIdx.x; e < NP * KC / 2; e += NTHREADS) {
+                const int col = e / (KC / 2), kq = e - col * (KC / 2);
+                *(double2 *)(Bs + col * LDK + 2 * kq) = *(const double2 *)(wc + 2 * e);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                double2 b[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) b[q] = *(const double2 *)(Bs + (ct * 16 + l16) * LDK + 8 * g + 2 * q);
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    d4 c = acc[rt][ct];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][q].x, b[q].x, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][q].y, b[q].y, c, 0, 0, 0);
+                    }
+                    acc[rt][ct] = c;
+                }
+            }
+        }
+    }
+    double *out = part + (int64_t)it.slab * slab_stride;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int rbase = it.mtile * MT + wave * RT * 16 + rt * 16 + 4 * g;
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + i) * NP + ct * 16 + l16] = acc[rt][ct][i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Reductions and layout conversions
 // ---------------------------------------------------------------------------
 
-// Sum the forward slabs; write the padded lenslet-major image (FP32 + BF16
-// hi/lo shadows when hi != nullptr) and, if O_col != nullptr, the MATLAB
-// column-major image.  Pixels beyond X, Y (the last partial lenslet) are 0.
-__global__ void k_fwd_reduce(const float *__restrict__ part, int nslabs, int64_t slab_stride,
-                             int NP, int nrows, int n, int Ly, int YP, int T, int S, int X, int Y,
-                             float *__restrict__ img, u16 *__restrict__ hi, u16 *__restrict__ lo,
-                             float *__restrict__ O_col)
+__device__ __forceinline__ void store_shadow(u16 *hi, u16 *lo, int64_t o, float v)
+{
+    const u16 h = f2bf(v);
+    hi[o] = h;
+    lo[o] = f2bf(v - bf2f(h));
+}
+
+// Sum the forward slabs; write the padded lenslet-major image (T = float:
+// plus BF16 hi/lo shadows when hi != nullptr) and, if O_col != nullptr, the
+// MATLAB column-major image.  Pixels beyond X, Y (the last partial lenslet)
+// are 0.
+template <typename T>
+__global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t slab_stride, int NP, int nrows,
+                             int n, int Ly, int YP, int T_, int S, int X, int Y, T *__restrict__ img,
+                             u16 *__restrict__ hi, u16 *__restrict__ lo, float *__restrict__ O_col)
 {
     const int nn2 = n * n;
     const int64_t total = (int64_t)nrows * nn2;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(e / nn2), uv = (int)(e - (int64_t)r * nn2);
-        float s = 0.0f;
-        const float *p = part + (int64_t)r * NP + uv;
+        T s = 0;
+        const T *p = part + (int64_t)r * NP + uv;
         for (int k = 0; k < nslabs; ++k) s += p[(int64_t)k * slab_stride];
         const int Px = r / Ly, Py = r - Px * Ly;
         const int x = Px * n + uv % n, y = Py * n + uv / n;
-        if (x >= X || y >= Y) s = 0.0f;
-        const int64_t o = ((int64_t)(Px + T) * YP + (Py + T)) * S + uv;
+        if (x >= X || y >= Y) s = 0;
+        const int64_t o = ((int64_t)(Px + T_) * YP + (Py + T_)) * S + uv;
         img[o] = s;
-        if (hi) {
-            const u16 h = f2bf(s);
-            hi[o] = h;
-            lo[o] = f2bf(s - bf2f(h));
-        }
-        if (O_col && x < X && y < Y) O_col[(int64_t)x + (int64_t)X * y] = s;
+        if (hi) store_shadow(hi, lo, o, (float)s);
+        if (O_col && x < X && y < Y) O_col[(int64_t)x + (int64_t)X * y] = (float)s;
     }
 }
 
@@ -300,12 +383,11 @@ __global__ void k_fwd_reduce(const float *__restrict__ part, int nslabs, int64_t
 // column-major volume B_col.  With rl_update (deconvRL.m:52,85-87):
 //   r = Htf / D; r(isinf) = 0; X = X .* r; X(isnan) = 0
 // on the padded volume (and its BF16 shadows), and copy X to B_col if given.
-__global__ void k_bwd_reduce(const float *__restrict__ part, const int *__restrict__ plane_slab0,
-                             int64_t slab_stride, int NP, int nrows, int nz, int n, int Ly,
-                             int64_t XP, int64_t YP, int T, int S, int X, int Y,
-                             float *__restrict__ B_col, int rl_update,
-                             const float *__restrict__ htf, float *__restrict__ vol,
-                             u16 *__restrict__ hi, u16 *__restrict__ lo)
+template <typename T>
+__global__ void k_bwd_reduce(const T *__restrict__ part, const int *__restrict__ plane_slab0, int64_t slab_stride,
+                             int NP, int nrows, int nz, int n, int Ly, int64_t XP, int64_t YP, int T_, int S, int X,
+                             int Y, float *__restrict__ B_col, int rl_update, const T *__restrict__ htf,
+                             T *__restrict__ vol, u16 *__restrict__ hi, u16 *__restrict__ lo)
 {
     const int nn2 = n * n;
     const int64_t per_plane = (int64_t)nrows * nn2;
@@ -315,39 +397,36 @@ __global__ void k_bwd_reduce(const float *__restrict__ part, const int *__restri
         const int c = (int)(e / per_plane);
         const int64_t rem = e - (int64_t)c * per_plane;
         const int r = (int)(rem / nn2), ab = (int)(rem - (int64_t)r * nn2);
-        float d = 0.0f;
-        const float *p = part + (int64_t)r * NP + ab;
+        T d = 0;
+        const T *p = part + (int64_t)r * NP + ab;
         const int s0 = plane_slab0[c], s1 = plane_slab0[c + 1];
         for (int k = s0; k < s1; ++k) d += p[(int64_t)k * slab_stride];
         const int Px = r / Ly, Py = r - Px * Ly;
         const int x = Px * n + ab % n, y = Py * n + ab / n;
         const bool real = x < X && y < Y;
+        const int64_t col = (int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c);
         if (!rl_update) {
-            if (real) B_col[(int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c)] = d;
+            if (real) B_col[col] = (float)d;
             continue;
         }
-        const int64_t o = (((int64_t)c * XP + Px + T) * YP + (Py + T)) * S + ab;
-        float v = 0.0f;
+        const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
+        T v = 0;
         if (real) {
-            float ratio = htf[o] / d;
-            if (isinf(ratio)) ratio = 0.0f;
+            T ratio = htf[o] / d;
+            if (isinf(ratio)) ratio = 0;
             v = vol[o] * ratio;
-            if (isnan(v)) v = 0.0f;
+            if (isnan(v)) v = 0;
         }
         vol[o] = v;
-        if (hi) {
-            const u16 h = f2bf(v);
-            hi[o] = h;
-            lo[o] = f2bf(v - bf2f(h));
-        }
-        if (B_col && real) B_col[(int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c)] = v;
+        if (hi) store_shadow(hi, lo, o, (float)v);
+        if (B_col && real) B_col[col] = (float)v;
     }
 }
 
 // MATLAB column-major volume -> padded lenslet-major volume (+ BF16 shadows).
-__global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int n, int64_t XP,
-                         int64_t YP, int T, int S, float *__restrict__ dst, u16 *__restrict__ hi,
-                         u16 *__restrict__ lo)
+template <typename T>
+__global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int n, int64_t XP, int64_t YP, int T_,
+                         int S, T *__restrict__ dst, u16 *__restrict__ hi, u16 *__restrict__ lo)
 {
     const int64_t total = (int64_t)X * Y * nz;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -356,20 +435,17 @@ __global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int 
         const int64_t t = e / X;
         const int y = (int)(t % Y), c = (int)(t / Y);
         const int Px = x / n, Py = y / n, ab = (x - Px * n) + n * (y - Py * n);
-        const int64_t o = (((int64_t)c * XP + Px + T) * YP + (Py + T)) * S + ab;
+        const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
         const float v = V[e];
-        if (dst) dst[o] = v;
-        if (hi) {
-            const u16 h = f2bf(v);
-            hi[o] = h;
-            lo[o] = f2bf(v - bf2f(h));
-        }
+        if (dst) dst[o] = (T)v;
+        if (hi) store_shadow(hi, lo, o, v);
     }
 }
 
 // Padded lenslet-major volume -> MATLAB column-major volume.
-__global__ void k_vol_out(const float *__restrict__ src, int X, int Y, int nz, int n, int64_t XP,
-                          int64_t YP, int T, int S, float *__restrict__ V)
+template <typename T>
+__global__ void k_vol_out(const T *__restrict__ src, int X, int Y, int nz, int n, int64_t XP, int64_t YP, int T_,
+                          int S, float *__restrict__ V)
 {
     const int64_t total = (int64_t)X * Y * nz;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -378,7 +454,7 @@ __global__ void k_vol_out(const float *__restrict__ src, int X, int Y, int nz, i
         const int64_t t = e / X;
         const int y = (int)(t % Y), c = (int)(t / Y);
         const int Px = x / n, Py = y / n, ab = (x - Px * n) + n * (y - Py * n);
-        V[e] = src[(((int64_t)c * XP + Px + T) * YP + (Py + T)) * S + ab];
+        V[e] = (float)src[(((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab];
     }
 }
 
@@ -469,11 +545,13 @@ int build_weights(sid_psf *p, int which)
         p->wf_elems = offf;
         p->wb_elems = offb;
     }
-    const size_t esz = which == 0 ? 4 : 2;
-    void **dstf = which == 0 ? (void **)&p->wf : which == 1 ? (void **)&p->wf_hi : (void **)&p->wf_lo;
-    void **dstb = which == 0 ? (void **)&p->wb : which == 1 ? (void **)&p->wb_hi : (void **)&p->wb_lo;
+    const size_t esz = which == 0 ? 4 : which == 3 ? 8 : 2;
+    void **dstf = which == 0 ? (void **)&p->wf : which == 1 ? (void **)&p->wf_hi
+                : which == 2 ? (void **)&p->wf_lo : (void **)&p->wf_d;
+    void **dstb = which == 0 ? (void **)&p->wb : which == 1 ? (void **)&p->wb_hi
+                : which == 2 ? (void **)&p->wb_lo : (void **)&p->wb_d;
     if (*dstf) return SID_OK;
-    const int KC = which == 0 ? KChunk<0>::value : KChunk<1>::value;
+    const int KC = (which == 0 || which == 3) ? KChunk<0>::value : KChunk<1>::value;
     SID_TRY_HIP(hipMalloc(dstf, std::max<int64_t>(p->wf_elems, 1) * esz));
     SID_TRY_HIP(hipMalloc(dstb, std::max<int64_t>(p->wb_elems, 1) * esz));
     for (int64_t c = 0; c < p->nzH; ++c) {
@@ -493,8 +571,11 @@ int build_weights(sid_psf *p, int which)
             else if (which == 1)
                 hipLaunchKernelGGL(k_build_w<1>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
                                    (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
-            else
+            else if (which == 2)
                 hipLaunchKernelGGL(k_build_w<2>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+            else
+                hipLaunchKernelGGL(k_build_w<3>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
                                    (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
             SID_TRY_HIP(hipGetLastError());
         }
@@ -552,13 +633,15 @@ static int upload_plan(GemmPlan &g)
     return SID_OK;
 }
 
-static void free_plan_buffers(SensorPlan &sp)
+void free_plan_buffers(SensorPlan &sp)
 {
     for (void *q : {(void *)sp.vol, (void *)sp.img, (void *)sp.htf, (void *)sp.part, (void *)sp.vol_hi,
-                    (void *)sp.vol_lo, (void *)sp.img_hi, (void *)sp.img_lo})
+                    (void *)sp.vol_lo, (void *)sp.img_hi, (void *)sp.img_lo, (void *)sp.vol_d, (void *)sp.img_d,
+                    (void *)sp.htf_d, (void *)sp.part_d})
         if (q) (void)hipFree(q);
     sp.vol = sp.img = sp.htf = sp.part = nullptr;
     sp.vol_hi = sp.vol_lo = sp.img_hi = sp.img_lo = nullptr;
+    sp.vol_d = sp.img_d = sp.htf_d = sp.part_d = nullptr;
     sp.prec_built = -1;
 }
 
@@ -679,6 +762,20 @@ int ensure_precision_buffers(sid_psf *p)
 {
     SensorPlan &sp = p->plan;
     if (p->precision == SID_PREC_FP32) return SID_OK;
+    if (p->precision == SID_PREC_FP64) {
+        SID_TRY(build_weights(p, 3));
+        if (!sp.vol_d) {
+            SID_TRY_HIP(hipMalloc(&sp.vol_d, sp.vol_elems * sizeof(double)));
+            SID_TRY_HIP(hipMalloc(&sp.htf_d, sp.vol_elems * sizeof(double)));
+            SID_TRY_HIP(hipMalloc(&sp.img_d, sp.img_elems * sizeof(double)));
+            SID_TRY_HIP(hipMalloc(&sp.part_d, sp.part_elems * sizeof(double)));
+            SID_TRY_HIP(hipMemsetAsync(sp.vol_d, 0, sp.vol_elems * sizeof(double), p->stream));
+            SID_TRY_HIP(hipMemsetAsync(sp.htf_d, 0, sp.vol_elems * sizeof(double), p->stream));
+            SID_TRY_HIP(hipMemsetAsync(sp.img_d, 0, sp.img_elems * sizeof(double), p->stream));
+            SID_TRY_HIP(hipStreamSynchronize(p->stream));
+        }
+        return SID_OK;
+    }
     SID_TRY(build_weights(p, 1));
     if (p->precision == SID_PREC_BF16X3) SID_TRY(build_weights(p, 2));
     if (!sp.vol_hi) {
@@ -710,7 +807,11 @@ static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st)
     const SensorPlan &sp = p->plan;
     const int S = (int)p->S;
     const bool fwd = dir == 0;
-    if (p->precision == SID_PREC_FP32) {
+    if (p->precision == SID_PREC_FP64) {
+        hipLaunchKernelGGL((k_proj_gemm_f64<NT, 2>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
+                           g.d_items, g.d_segs, fwd ? sp.vol_d : sp.img_d, fwd ? p->wf_d : p->wb_d, sp.part_d,
+                           (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S, sp.mpad * (int64_t)(NT * 16));
+    } else if (p->precision == SID_PREC_FP32) {
         launch_one<NT, 2, 0>(g, fwd ? (const void *)sp.vol : (const void *)sp.img, nullptr,
                              fwd ? (const void *)p->wf : (const void *)p->wb, nullptr, sp, S, st);
     } else if (p->precision == SID_PREC_BF16X2) {
@@ -748,16 +849,23 @@ int launch_gemm(sid_psf *p, int dir, hipStream_t st)
 int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
-    const bool sh = p->precision != SID_PREC_FP32;
+    const bool f64 = p->precision == SID_PREC_FP64;
+    const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
     const int64_t total = sp.nrows * p->n * p->n;
-    if (sp.fwd.nslabs == 0) {
-        // no contributing plane: the image is zero
-        SID_TRY_HIP(hipMemsetAsync(sp.part, 0, sp.mpad * p->NP * sizeof(float), st));
+    const int64_t stride = sp.mpad * p->NP;
+    if (sp.fwd.nslabs == 0) {  // no contributing plane: the image is zero
+        if (f64) SID_TRY_HIP(hipMemsetAsync(sp.part_d, 0, stride * sizeof(double), st));
+        else SID_TRY_HIP(hipMemsetAsync(sp.part, 0, stride * sizeof(float), st));
     }
-    hipLaunchKernelGGL(k_fwd_reduce, dim3(grid_for(total)), dim3(256), 0, st, sp.part, std::max(sp.fwd.nslabs, 1),
-                       sp.mpad * p->NP, (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T,
-                       (int)p->S, (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr,
-                       O_col);
+    const int ns = std::max(sp.fwd.nslabs, 1);
+    if (f64)
+        hipLaunchKernelGGL(k_fwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d, ns, stride,
+                           (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
+                           (int)sp.X, (int)sp.Y, sp.img_d, (u16 *)nullptr, (u16 *)nullptr, O_col);
+    else
+        hipLaunchKernelGGL(k_fwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part, ns, stride,
+                           (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
+                           (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr, O_col);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -765,25 +873,40 @@ int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st)
 int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
-    const bool sh = p->precision != SID_PREC_FP32;
+    const bool f64 = p->precision == SID_PREC_FP64;
+    const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
     const int64_t total = sp.nrows * p->n * p->n * p->nzH;
-    hipLaunchKernelGGL(k_bwd_reduce, dim3(grid_for(total)), dim3(256), 0, st, sp.part, sp.bwd.d_plane_slab0,
-                       sp.mpad * p->NP, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n, (int)sp.Ly, sp.XP, sp.YP,
-                       (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update, sp.htf, sp.vol,
-                       sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
+    const int64_t stride = sp.mpad * p->NP;
+    if (f64)
+        hipLaunchKernelGGL(k_bwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d,
+                           sp.bwd.d_plane_slab0, stride, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n,
+                           (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update,
+                           sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr);
+    else
+        hipLaunchKernelGGL(k_bwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part,
+                           sp.bwd.d_plane_slab0, stride, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n,
+                           (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update,
+                           sp.htf, sp.vol, sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
-int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, float *dst, int shadow, hipStream_t st)
+// which: 0 = the iterate / forward input (with BF16 shadows), 1 = Htf.
+int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
-    const bool sh = shadow && p->precision != SID_PREC_FP32;
+    const bool f64 = p->precision == SID_PREC_FP64;
+    const bool sh = which == 0 && (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3);
     const int64_t total = sp.X * sp.Y * nz;
     if (total == 0) return SID_OK;
-    hipLaunchKernelGGL(k_vol_in, dim3(grid_for(total)), dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y, (int)nz,
-                       (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, dst, sh ? sp.vol_hi : nullptr,
-                       sh ? sp.vol_lo : nullptr);
+    if (f64)
+        hipLaunchKernelGGL(k_vol_in<double>, dim3(grid_for(total)), dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y,
+                           (int)nz, (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol_d : sp.htf_d,
+                           (u16 *)nullptr, (u16 *)nullptr);
+    else
+        hipLaunchKernelGGL(k_vol_in<float>, dim3(grid_for(total)), dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y,
+                           (int)nz, (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol : sp.htf,
+                           sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -791,22 +914,31 @@ int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, float *dst, int sh
 int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
-    const bool sh = p->precision != SID_PREC_FP32;
+    const bool f64 = p->precision == SID_PREC_FP64;
+    const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
     const int64_t total = sp.X * sp.Y;
-    hipLaunchKernelGGL(k_vol_in, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1, (int)p->n,
-                       sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img, sh ? sp.img_hi : nullptr,
-                       sh ? sp.img_lo : nullptr);
+    if (f64)
+        hipLaunchKernelGGL(k_vol_in<double>, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1,
+                           (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img_d, (u16 *)nullptr, (u16 *)nullptr);
+    else
+        hipLaunchKernelGGL(k_vol_in<float>, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1,
+                           (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img, sh ? sp.img_hi : nullptr,
+                           sh ? sp.img_lo : nullptr);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
-int launch_vol_out(sid_psf *p, const float *src, float *V_col, int64_t nz, hipStream_t st)
+int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
     const int64_t total = sp.X * sp.Y * nz;
     if (total == 0) return SID_OK;
-    hipLaunchKernelGGL(k_vol_out, dim3(grid_for(total)), dim3(256), 0, st, src, (int)sp.X, (int)sp.Y, (int)nz,
-                       (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, V_col);
+    if (p->precision == SID_PREC_FP64)
+        hipLaunchKernelGGL(k_vol_out<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.vol_d, (int)sp.X,
+                           (int)sp.Y, (int)nz, (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, V_col);
+    else
+        hipLaunchKernelGGL(k_vol_out<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.vol, (int)sp.X, (int)sp.Y,
+                           (int)nz, (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, V_col);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -814,7 +946,7 @@ int launch_vol_out(sid_psf *p, const float *src, float *V_col, int64_t nz, hipSt
 // ---------------------------------------------------------------------------
 // Self-test of the MFMA lane layouts assumed above.
 // ---------------------------------------------------------------------------
-__global__ void k_selftest(const float *A, const float *B, float *C32, float *C16)
+__global__ void k_selftest(const float *A, const float *B, float *C32, float *C16, float *C64)
 {
     // A: 16 x 32 row-major, B: 32 x 16 row-major
     const int lane = threadIdx.x, l16 = lane & 15, g = lane >> 4;
@@ -830,33 +962,40 @@ __global__ void k_selftest(const float *A, const float *B, float *C32, float *C1
     f4 d = {0, 0, 0, 0};
     d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, d, 0, 0, 0);
     for (int i = 0; i < 4; ++i) C16[(4 * g + i) * 16 + l16] = d[i];
+    d4 e = {0, 0, 0, 0};
+    for (int s = 0; s < 8; ++s)
+        e = __builtin_amdgcn_mfma_f64_16x16x4f64((double)A[l16 * 32 + 8 * g + s], (double)B[(8 * g + s) * 16 + l16], e,
+                                                0, 0, 0);
+    for (int i = 0; i < 4; ++i) C64[(4 * g + i) * 16 + l16] = (float)e[i];
 }
 
 int run_selftest()
 {
-    std::vector<float> A(16 * 32), B(32 * 16), C32(256), C16(256);
+    std::vector<float> A(16 * 32), B(32 * 16), C32(256), C16(256), C64(256);
     // small integers: exact in bf16 and in fp32 accumulation
     for (int i = 0; i < 16 * 32; ++i) A[i] = (float)((i * 7 + 3) % 11) - 5.0f;
     for (int i = 0; i < 32 * 16; ++i) B[i] = (float)((i * 5 + 1) % 13) - 6.0f;
-    float *dA, *dB, *dC, *dD;
+    float *dA, *dB, *dC, *dD, *dE;
     SID_TRY_HIP(hipMalloc(&dA, A.size() * 4));
     SID_TRY_HIP(hipMalloc(&dB, B.size() * 4));
     SID_TRY_HIP(hipMalloc(&dC, 256 * 4));
     SID_TRY_HIP(hipMalloc(&dD, 256 * 4));
+    SID_TRY_HIP(hipMalloc(&dE, 256 * 4));
     SID_TRY_HIP(hipMemcpy(dA, A.data(), A.size() * 4, hipMemcpyHostToDevice));
     SID_TRY_HIP(hipMemcpy(dB, B.data(), B.size() * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA, dB, dC, dD);
+    hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA, dB, dC, dD, dE);
     SID_TRY_HIP(hipGetLastError());
     SID_TRY_HIP(hipMemcpy(C32.data(), dC, 256 * 4, hipMemcpyDeviceToHost));
     SID_TRY_HIP(hipMemcpy(C16.data(), dD, 256 * 4, hipMemcpyDeviceToHost));
-    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD);
+    SID_TRY_HIP(hipMemcpy(C64.data(), dE, 256 * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD); (void)hipFree(dE);
     for (int i = 0; i < 16; ++i)
         for (int j = 0; j < 16; ++j) {
             double s = 0;
             for (int k = 0; k < 32; ++k) s += (double)A[i * 32 + k] * B[k * 16 + j];
-            if (C32[i * 16 + j] != (float)s || C16[i * 16 + j] != (float)s) {
-                set_error("MFMA layout self-test failed at (%d,%d): fp32 %g bf16 %g expected %g", i, j,
-                          C32[i * 16 + j], C16[i * 16 + j], s);
+            if (C32[i * 16 + j] != (float)s || C16[i * 16 + j] != (float)s || C64[i * 16 + j] != (float)s) {
+                set_error("MFMA layout self-test failed at (%d,%d): fp32 %g bf16 %g fp64 %g expected %g", i, j,
+                          C32[i * 16 + j], C16[i * 16 + j], C64[i * 16 + j], s);
                 return SID_ERR_HIP;
             }
         }

@@ -21,6 +21,7 @@ namespace sid {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
 void set_error(const char *fmt, ...);
@@ -92,6 +93,7 @@ struct SensorPlan {
     // device buffers (fp32 master + bf16 hi/lo shadows)
     float *vol = nullptr, *img = nullptr, *htf = nullptr, *part = nullptr;
     u16 *vol_hi = nullptr, *vol_lo = nullptr, *img_hi = nullptr, *img_lo = nullptr;
+    double *vol_d = nullptr, *img_d = nullptr, *htf_d = nullptr, *part_d = nullptr;  // FP64 mode
     int64_t vol_elems = 0, img_elems = 0, part_elems = 0;
     int prec_built = -1;
 };
@@ -109,6 +111,7 @@ struct sid_psf {
     float *wf = nullptr, *wb = nullptr;           // fp32 weights (forward / backward)
     sid::u16 *wf_hi = nullptr, *wb_hi = nullptr;  // bf16 hi weights
     sid::u16 *wf_lo = nullptr, *wb_lo = nullptr;  // bf16 lo weights (x3 mode)
+    double *wf_d = nullptr, *wb_d = nullptr;      // fp64 weights
     int64_t wf_elems = 0, wb_elems = 0;
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
@@ -119,15 +122,16 @@ struct sid_psf {
 
 namespace sid {
 // sid_direct.hip
-int build_weights(sid_psf *p, int which /*0 fp32, 1 bf16 hi, 2 bf16 lo*/);
+int build_weights(sid_psf *p, int which /*0 fp32, 1 bf16 hi, 2 bf16 lo, 3 fp64*/);
 int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz);
 int ensure_precision_buffers(sid_psf *p);
 int launch_gemm(sid_psf *p, int dir, hipStream_t st);
 int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st);
 int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st);
-int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, float *dst, int shadow, hipStream_t st);
+int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which /*0 iterate, 1 Htf*/, hipStream_t st);
 int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st);
-int launch_vol_out(sid_psf *p, const float *src, float *V_col, int64_t nz, hipStream_t st);
+int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st);
+void free_plan_buffers(SensorPlan &sp);
 int support_scan(sid_psf *p);
 int run_selftest();
 }  // namespace sid

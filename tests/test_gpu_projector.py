@@ -101,3 +101,40 @@ def test_errors(sid):
         psf.forward(np.zeros((9, 9, 3), np.float32))
     with pytest.raises(ValueError):
         sid.PSF(np.zeros((7, 7, 3), np.float32))
+
+
+def test_fp64_mode_vs_oracle(sid):
+    """The FP64 device path agrees with the C oracle to FP64 round-off (float I/O)."""
+    print("case: fp64 mode n5 X40 Y35 M15 Nz3", flush=True)
+    rng = np.random.default_rng(11)
+    H = random_psf(rng, 15, 5, 3).astype(np.float32).astype(np.float64)
+    V = rng.random((40, 35, 3)).astype(np.float32).astype(np.float64)
+    psf = sid.PSF(H.astype(np.float32), precision="fp64")
+    e = rel_l2(psf.forward(V.astype(np.float32)), c_forward(H, V))
+    assert e < 1e-7, e   # bounded by the float output rounding
+    y = c_forward(H, V)
+    htf = c_backward(H, y).astype(np.float32).astype(np.float64)
+    X = psf.deconv_rl(htf.astype(np.float32), 6, htf.astype(np.float32))
+    e = rel_l2(X, c_isra(H, htf, htf, 6))
+    print("fp64 isra rel-L2", e, flush=True)
+    assert e < 1e-6, e
+
+
+@pytest.mark.parametrize("prec,tol_fwd", [("bf16x2", 3e-3), ("bf16x3", 2e-5)])
+def test_bf16_split_error_budget(sid, prec, tol_fwd):
+    """BF16 split products: H rounded to BF16 (2^-9 relative) dominates bf16x2;
+    bf16x3 adds the H lo part and is near FP32."""
+    print("case:", prec, "n15 X90 Y75 M61 Nz3", flush=True)
+    from sid_amd import synth
+    H, _ = synth.make_psf(15, 61, (-3, 0, 3), dtype=np.float64)
+    rng = np.random.default_rng(5)
+    V = rng.random((90, 75, 3))
+    O_ref = c_forward(H, V)
+    psf = sid.PSF(H.astype(np.float32), precision=prec)
+    e = rel_l2(psf.forward(V.astype(np.float32)), O_ref)
+    print(prec, "forward rel-L2", e, flush=True)
+    assert e < tol_fwd
+    B_ref = c_backward(H, O_ref)
+    e = rel_l2(psf.backward(O_ref.astype(np.float32)), B_ref)
+    print(prec, "backward rel-L2", e, flush=True)
+    assert e < tol_fwd

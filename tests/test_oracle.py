@@ -1,0 +1,368 @@
+"""Pins the oracle (CPU, no GPU): library routines, brute force, closed forms
+and invariants, chosen so that a dropped term, a wrong sign or offset, or a
+transposed operand fails at least one check."""
+import math
+
+import numpy as np
+import pytest
+from scipy.signal import convolve2d
+
+from oracle import sid_oracle as O
+from sid_amd import synth
+from tests._helpers import c_backward, c_forward, c_isra, rel_l2
+
+
+# ---------------------------------------------------------------- conv2 'same'
+
+@pytest.mark.parametrize("shape_x,shape_k", [((13, 11), (7, 5)), ((9, 9), (9, 9)), ((5, 17), (3, 11)), ((12, 10), (1, 1))])
+def test_conv2fft_equals_scipy_same_for_odd_kernels(shape_x, shape_k):
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(shape_x)
+    k = rng.standard_normal(shape_k)
+    # for odd kernels MATLAB's and scipy's 'same' windows coincide
+    assert np.abs(O.conv2fft_same(x, k) - convolve2d(x, k, mode="same")).max() < 1e-12
+
+
+def test_conv2_same_matlab_window_for_even_kernels():
+    # MATLAB documentation example: conv([1 2 3], [1 1], 'same') = [3 5 3]
+    x = np.array([[1.0, 2.0, 3.0]])
+    k = np.array([[1.0, 1.0]])
+    assert np.allclose(O.conv2_same_direct(x, k), [[3.0, 5.0, 3.0]])
+    assert np.allclose(O.conv2fft_same(x, k), [[3.0, 5.0, 3.0]])
+    # general even kernels: the 'full' convolution cropped at floor(M/2)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((11, 9))
+    k = rng.standard_normal((6, 4))
+    full = convolve2d(x, k, mode="full")
+    want = full[3:3 + 11, 2:2 + 9]
+    assert np.abs(O.conv2fft_same(x, k) - want).max() < 1e-12
+    assert np.abs(O.conv2_same_direct(x, k) - want).max() < 1e-12
+
+
+def test_fft_pad_rule_values():
+    # forwardProjectGPU.m:12-13: min(2^ceil(log2(s)), 256*ceil(s/256)), s = X + floor(M/2)
+    assert O.fft_pad_size(255, 105) == 512          # s = 307
+    assert O.fft_pad_size(1005, 345) == 1280        # s = 1177
+    assert O.fft_pad_size(495, 225) == 768          # s = 607
+    assert O.fft_pad_size(100, 9) == 128            # s = 104: power of two wins
+    for X, M in [(255, 105), (1005, 345), (495, 225), (37, 11)]:
+        assert O.fft_pad_size(X, M) >= X + M // 2   # no wrap-around inside the 'same' window
+
+
+def test_conv2fft_no_aliasing_at_pad_size():
+    rng = np.random.default_rng(2)
+    x = rng.random((40, 33))
+    k = rng.random((21, 21))
+    assert np.abs(O.conv2fft_same(x, k) - O.conv2_same_direct(x, k)).max() < 1e-12
+
+
+# ---------------------------------------------------------------- projectors
+
+def _tiny(rng, n=3, X=9, Y=12, M=7, nz=3):
+    H = rng.random((M, M, n, n, nz))
+    V = rng.random((X, Y, nz))
+    return H, V
+
+
+def test_forward_equals_dense_brute_force():
+    rng = np.random.default_rng(3)
+    H, V = _tiny(rng)
+    A = O.dense_operator(H, 9, 12)
+    ref = (A @ V.ravel(order="F")).reshape((9, 12), order="F")
+    assert np.abs(O.forward_project(H, V) - ref).max() < 1e-12
+    assert np.abs(c_forward(H, V) - ref).max() < 1e-12
+
+
+def test_backward_is_dense_transpose():
+    rng = np.random.default_rng(4)
+    H, _ = _tiny(rng)
+    w = rng.random((9, 12))
+    A = O.dense_operator(H, 9, 12)
+    ref = (A.T @ w.ravel(order="F")).reshape((9, 12, 3), order="F")
+    assert np.abs(O.backward_project(H, w) - ref).max() < 1e-12
+    assert np.abs(c_backward(H, w) - ref).max() < 1e-12
+
+
+def test_even_kernel_forward_and_adjoint():
+    rng = np.random.default_rng(5)
+    H, V = _tiny(rng, M=6)
+    A = O.dense_operator(H, 9, 12)
+    f = O.forward_project(H, V)
+    assert np.abs(f - (A @ V.ravel(order="F")).reshape((9, 12), order="F")).max() < 1e-12
+    # the reference backward rotates the even kernel by 180 degrees; that is
+    # the exact transpose of the forward only for odd M.  The oracle follows
+    # the reference (rot180 + conv2 'same'), checked against the defining sum.
+    w = rng.random((9, 12))
+    B = O.backward_project(H, w)
+    for c in range(3):
+        for a in range(3):
+            for b in range(3):
+                t = O.conv2_same_direct(w, H[::-1, ::-1, a, b, c])
+                assert np.abs(B[a::3, b::3, c] - t[a::3, b::3]).max() < 1e-12
+    assert np.abs(c_backward(H, w) - B).max() < 1e-12
+
+
+def test_point_source_closed_form():
+    """A unit voxel at q gives H_{a(q),b(q),c} centred at q (conv2 'same')."""
+    n, M = 5, 11
+    H, _ = synth.make_psf(n, M, (-1, 0, 2), dtype=np.float64)
+    X = Y = 40
+    q = (17, 23)
+    for c in range(3):
+        V = np.zeros((X, Y, 3))
+        V[q[0], q[1], c] = 1.0
+        img = O.forward_project(H, V)
+        a, b = q[0] % n, q[1] % n
+        want = np.zeros((X, Y))
+        m = M // 2
+        want[q[0] - m:q[0] + m + 1, q[1] - m:q[1] + m + 1] = H[:, :, a, b, c]
+        assert np.abs(img - want).max() < 1e-13
+        assert np.abs(c_forward(H, V) - want).max() < 1e-13
+
+
+def test_linearity_and_flux_conservation():
+    n, M = 5, 15
+    H, _ = synth.make_psf(n, M, (-2, 0, 2), dtype=np.float64)
+    rng = np.random.default_rng(6)
+    X = Y = 50
+    V1 = np.zeros((X, Y, 3))
+    V1[15:35, 15:35, :] = rng.random((20, 20, 3))   # interior: the whole footprint lands on the sensor
+    V2 = rng.random((X, Y, 3))
+    f1, f2 = O.forward_project(H, V1), O.forward_project(H, V2)
+    assert np.abs(O.forward_project(H, 2.0 * V1 - 3.0 * V2) - (2.0 * f1 - 3.0 * f2)).max() < 1e-11
+    # unit-sum kernels conserve flux for interior sources
+    assert abs(f1.sum() - V1.sum()) < 1e-10 * V1.sum()
+
+
+def test_lenslet_shift_equivariance():
+    """Shifting the volume by a whole lenslet shifts the image by n pixels."""
+    n, M = 3, 9
+    rng = np.random.default_rng(7)
+    H = rng.random((M, M, n, n, 2))
+    V = np.zeros((30, 30, 2))
+    V[9:18, 9:18, :] = rng.random((9, 9, 2))
+    Vs = np.roll(V, shift=(n, 2 * n), axis=(0, 1))
+    f, fs = O.forward_project(H, V), O.forward_project(H, Vs)
+    assert np.abs(np.roll(f, shift=(n, 2 * n), axis=(0, 1)) - fs).max() < 1e-12
+
+
+def test_adjoint_identity_random():
+    rng = np.random.default_rng(8)
+    H = rng.random((13, 13, 5, 5, 3))
+    v = rng.random((33, 28, 3))
+    w = rng.random((33, 28))
+    lhs = np.sum(c_forward(H, v) * w)
+    rhs = np.sum(v * c_backward(H, w))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_acc_crop_semantics():
+    """forwardProjectACC crops each kernel to CAindex and uses the crop's own
+    'same' offset; with zeros outside the crop it equals the full-kernel path."""
+    rng = np.random.default_rng(9)
+    M, n = 11, 3
+    H = rng.random((M, M, n, n, 2))
+    CA = np.array([[3, 9], [2, 10]], dtype=np.int32)    # symmetric crops of odd size
+    Hz = np.zeros_like(H)
+    for c in range(2):
+        lo, hi = CA[c, 0] - 1, CA[c, 1]
+        Hz[lo:hi, lo:hi, :, :, c] = H[lo:hi, lo:hi, :, :, c]
+    V = rng.random((15, 12, 2))
+    fa = O.forward_project_acc(H, V, CA)
+    assert np.abs(fa - O.forward_project(Hz, V)).max() < 1e-12
+    assert np.abs(c_forward(H, V, CA) - fa).max() < 1e-12
+    w = rng.random((15, 12))
+    ba = O.backward_project_acc(H, w, CA)
+    assert np.abs(ba - O.backward_project(Hz, w)).max() < 1e-12
+    assert np.abs(c_backward(H, w, CA) - ba).max() < 1e-12
+    # an asymmetric / even crop uses floor(Mc/2) of the crop
+    CA2 = np.array([[2, 7], [4, 11]], dtype=np.int32)
+    fa2 = O.forward_project_acc(H, V, CA2)
+    want = np.zeros((15, 12))
+    for a in range(n):
+        for b in range(n):
+            for c in range(2):
+                t = np.zeros((15, 12))
+                t[a::n, b::n] = V[a::n, b::n, c]
+                K = H[CA2[c, 0] - 1:CA2[c, 1], CA2[c, 0] - 1:CA2[c, 1], a, b, c]
+                want += O.conv2_same_direct(t, K)
+    assert np.abs(fa2 - want).max() < 1e-12
+    assert np.abs(c_forward(H, V, CA2) - want).max() < 1e-12
+
+
+def test_c_and_numpy_oracles_agree_at_small_workload():
+    n, M = 15, 45
+    H, CA = synth.make_psf(n, M, (-2, 0, 2), dtype=np.float64)
+    rng = np.random.default_rng(10)
+    V = rng.random((75, 60, 3))
+    assert rel_l2(c_forward(H, V), O.forward_project(H, V)) < 1e-13
+    w = rng.random((75, 60))
+    assert rel_l2(c_backward(H, w), O.backward_project(H, w)) < 1e-13
+
+
+# ---------------------------------------------------------------- ISRA
+
+def test_isra_equals_dense_isra_and_guards():
+    rng = np.random.default_rng(11)
+    H, V = _tiny(rng, nz=2)
+    A = O.dense_operator(H, 9, 12)
+    y = A @ V.ravel(order="F")
+    htf = A.T @ y
+    x = htf.copy()
+    for _ in range(5):
+        x = x * htf / (A.T @ (A @ x))
+    F = lambda v: O.forward_project(H, v)
+    B = lambda o: O.backward_project(H, o)
+    htf3 = htf.reshape((9, 12, 2), order="F")
+    X = O.deconv_rl(F, B, htf3, 5, htf3)
+    assert np.abs(X.ravel(order="F") - x).max() < 1e-10 * x.max()
+    assert np.abs(c_isra(H, htf3, htf3, 5) - X).max() < 1e-10 * x.max()
+
+
+def test_isra_fixed_point_and_monotone_decrease():
+    rng = np.random.default_rng(12)
+    H = rng.random((9, 9, 3, 3, 2))
+    Vt = rng.random((12, 15, 2)) + 0.1
+    F = lambda v: O.forward_project(H, v)
+    B = lambda o: O.backward_project(H, o)
+    y = F(Vt)
+    htf = B(y)
+    # the true volume is a fixed point
+    X = O.deconv_rl(F, B, htf, 3, Vt)
+    assert np.abs(X - Vt).max() < 1e-10
+    # from Htf the least-squares residual never increases (ISRA, De Pierro 1987)
+    X = htf.copy()
+    res = [np.linalg.norm(y - F(X))]
+    for _ in range(6):
+        X = O.deconv_rl(F, B, htf, 1, X)
+        res.append(np.linalg.norm(y - F(X)))
+    assert all(res[i + 1] <= res[i] * (1 + 1e-12) for i in range(len(res) - 1))
+
+
+def test_isra_zero_guards():
+    """deconvRL.m:85-87: x/0 -> Inf -> 0; 0/0 -> NaN -> X set to 0."""
+    F = lambda v: np.zeros((2, 2))
+    B = lambda o: np.zeros((2, 2, 1))
+    htf = np.array([[1.0, 0.0], [2.0, 0.0]])[:, :, None]
+    X = O.deconv_rl(F, B, htf, 1, np.ones((2, 2, 1)))
+    assert np.all(X == 0.0)
+
+
+# ---------------------------------------------------------------- reconstruct_S pieces
+
+def test_quantile_hazen_hand_worked():
+    x = np.array([3.0, 1.0, 4.0, 1.0, 5.0])   # sorted 1 1 3 4 5, p_k = (k - 0.5)/5
+    assert O.quantile_matlab(x, 0.5) == 3.0
+    assert O.quantile_matlab(x, 0.1) == 1.0                   # p_1
+    assert O.quantile_matlab(x, 0.05) == 1.0                  # clamped below p_1
+    assert O.quantile_matlab(x, 0.99) == 5.0                  # clamped above p_5
+    assert abs(O.quantile_matlab(x, 0.8) - 4.5) < 1e-15       # halfway between p_4 = 0.7 and p_5 = 0.9
+    xs = np.random.default_rng(0).random(101)
+    for q in (0.2, 0.8, 0.95):
+        assert abs(O.quantile_matlab(xs, q) - np.quantile(xs, q, method="hazen")) < 1e-15
+
+
+def test_preprocess_scales():
+    img = np.arange(1.0, 13.0).reshape(3, 4)
+    y, nrm = O.reconstruct_preprocess(img, None, 0.5)
+    thr = O.quantile_matlab(img.ravel(order="F"), 0.5)
+    z = np.maximum(img - thr, 0)
+    z = z / z.max()
+    assert abs(nrm - math.sqrt((z ** 2).sum())) < 1e-14
+    assert abs(np.linalg.norm(y) - 1.0) < 1e-14
+
+
+def test_matlab_round_half_away_from_zero():
+    assert list(O.matlab_round([0.5, 1.5, 2.5, -0.5, -2.5, 2.4999])) == [1, 2, 3, -1, -3, 2]
+
+
+# ---------------------------------------------------------------- library placement
+
+def test_library_gpu_placement_hand_worked():
+    p = O.library_gpu_placement((10.5, 3.2, 2.5), neur_rad=4, axial=2, image_size=(20, 20), nz=5)
+    # round: x = 11, y = 3, z = 3; dz = ceil(4/2) = 2
+    assert (p["x"], p["y"], p["z"]) == (11, 3, 3)
+    assert (p["z0"], p["z1"]) == (1, 5)
+    assert (p["x0"], p["x1"], p["y0"], p["y1"]) == (7, 15, 1, 7)
+    p = O.library_gpu_placement((19.6, 19.4, 0.6), 3, 1.5, (20, 20), 4)
+    assert (p["x"], p["y"], p["z"]) == (20, 19, 1)
+    assert (p["z0"], p["z1"]) == (1, 3)
+    assert (p["x0"], p["x1"], p["y0"], p["y1"]) == (17, 20, 16, 20)
+
+
+def test_project_forward_patched_matches_full_sensor_projection():
+    """The patched CPU library projection equals the full-sensor forward
+    projection of the same voxels wherever the patch is pasted, for a patch
+    well inside the sensor (project_forward_patched.m:54-73)."""
+    n, M = 3, 7
+    rng = np.random.default_rng(13)
+    H = rng.random((M, M, n, n, 3))
+    pixels = np.array([[10, 11, 2], [11, 11, 2], [12, 13, 3], [10, 12, 1]])
+    colv = rng.random((2, 2, 2)) + 0.5
+    idx = np.array([[1, 1, 1], [2, 1, 1], [2, 2, 2], [1, 2, 1]])
+    img, info = O.project_forward_patched(pixels, (30, 30), n, H, colv, idx)
+    V = np.zeros((30, 30, 3))
+    for t in range(4):
+        V[pixels[t, 0] - 1, pixels[t, 1] - 1, pixels[t, 2] - 1] = colv[idx[t, 0] - 1, idx[t, 1] - 1, idx[t, 2] - 1]
+    full = np.zeros((30, 30))
+    for c in range(3):
+        Vc = np.zeros((30, 30, 3))
+        Vc[:, :, c] = V[:, :, c]
+        full += O.forward_project(H, Vc)
+    # offsets: px_min = (10, 11, 1) -> off_ml = (3, 3), off_px = (9, 9); n_ml = ceil(max/3) - 3 = (1, 2) -> odd (1, 3)
+    assert list(info["off_px"]) == [9, 9] and list(info["n_ml"]) == [1, 3]
+    # the 'full' patch is pasted centred at off + ceil(n_ml/2 * n); compare on the pasted window
+    x0, x1, y0, y1 = info["x0"], info["x1"], info["y0"], info["y1"]
+    win = img[x0 - 1:x1, y0 - 1:y1]
+    assert np.abs(win).sum() > 0
+    # for odd n and M the paste centre off + ceil(n_ml*n/2) lands the 'full'
+    # result exactly on the 'same'-aligned sensor position (zero shift)
+    assert np.abs(full[x0 - 1:x1, y0 - 1:y1] - win).max() < 1e-12
+    # outside the pasted rectangle the patched image is empty
+    mask = np.ones_like(img, dtype=bool)
+    mask[x0 - 1:x1, y0 - 1:y1] = False
+    assert np.all(img[mask] == 0)
+
+
+# ---------------------------------------------------------------- gradients
+
+def test_poisson_nll_gradients_finite_differences():
+    n, M = 3, 7
+    rng = np.random.default_rng(14)
+    H = rng.random((M, M, n, n, 2)) + 0.05
+    V = rng.random((9, 9, 2)) + 0.2
+    F = lambda v: O.forward_project(H, v)
+    B = lambda o: O.backward_project(H, o)
+    y = rng.poisson(F(V) * 3.0).astype(np.float64) / 3.0
+    gV = O.poisson_nll_grad_volume(F, B, V, y)
+    eps = 1e-6
+    for (i, j, c) in [(0, 0, 0), (4, 5, 1), (8, 3, 0), (2, 7, 1)]:
+        Vp, Vm = V.copy(), V.copy()
+        Vp[i, j, c] += eps
+        Vm[i, j, c] -= eps
+        fd = (O.poisson_nll(F, Vp, y) - O.poisson_nll(F, Vm, y)) / (2 * eps)
+        assert abs(fd - gV[i, j, c]) < 1e-6 * max(1.0, abs(fd))
+    gH = O.poisson_nll_grad_psf(H, V, y)
+    for (i, j, a, b, c) in [(3, 3, 0, 0, 0), (0, 6, 1, 2, 1), (5, 1, 2, 1, 0), (6, 6, 2, 2, 1)]:
+        Hp, Hm = H.copy(), H.copy()
+        Hp[i, j, a, b, c] += eps
+        Hm[i, j, a, b, c] -= eps
+        fd = (O.poisson_nll(lambda v: O.forward_project(Hp, v), V, y)
+              - O.poisson_nll(lambda v: O.forward_project(Hm, v), V, y)) / (2 * eps)
+        assert abs(fd - gH[i, j, a, b, c]) < 1e-6 * max(1.0, abs(fd))
+
+
+def test_ls_deconv_decreases_objective():
+    rng = np.random.default_rng(15)
+    H = rng.random((7, 7, 3, 3, 2))
+    V = rng.random((9, 12, 2))
+    F = lambda v: O.forward_project(H, v)
+    B = lambda o: O.backward_project(H, o)
+    y = F(V)
+    htf = B(y)
+    objs = []
+    for it in range(1, 6):
+        X = O.ls_deconv(F, B, htf, max_iter=it)
+        objs.append(np.linalg.norm(y - F(X)) ** 2)
+        assert X.min() >= 0
+    assert all(objs[i + 1] <= objs[i] * (1 + 1e-10) for i in range(len(objs) - 1))
