@@ -82,6 +82,20 @@ int sid_psf_create_device(const float *H_dev, const int64_t dims[5], const int32
                           sid_psf **out);
 int sid_psf_destroy(sid_psf *psf);
 
+/* Creation flags for sid_psf_create_ex. */
+enum {
+    SID_FLAG_H_ON_DEVICE = 1,   /* H points to device memory */
+    SID_FLAG_FULL_SUPPORT = 2   /* keep every kernel entry of the M x M (or CAindex) box instead of
+                                   shrinking each plane to its nonzero support; needed when H is
+                                   later updated with a wider support (PSF refinement) */
+};
+int sid_psf_create_ex(const float *H, const int64_t dims[5], const int32_t *CAindex, int flags,
+                      sid_psf **out);
+/* Replace the values of H (same dims; host or device memory per `on_device`)
+ * and rebuild every weight copy the handle holds.  The per-plane support
+ * chosen at creation is kept: entries outside it are ignored. */
+int sid_psf_update(sid_psf *psf, const float *H, int on_device);
+
 /* Select the GEMM precision (builds the BF16 weight copies on first use). */
 int sid_psf_set_precision(sid_psf *psf, int precision);
 /* Force a method for every plane (SID_METHOD_DIRECT / _FFT) or restore the
@@ -121,6 +135,20 @@ int sid_deconv_rl(sid_psf *psf, const float *Htf, float *X, const int64_t vdims[
                   float *hist);
 int sid_deconv_rl_device(sid_psf *psf, const float *Htf_dev, float *X_dev, const int64_t vdims[3],
                          int max_iter, float *hist_dev, void *stream);
+
+/* --- Poisson likelihood and gradients (no reference counterpart) ---------
+ * dims = {X, Y, Nz, nb}: V holds nb volumes [X,Y,Nz] back to back, y nb images
+ * [X,Y] (photon counts).  nll receives sum_f sum_p F_f - y_f log F_f (FP64,
+ * y log F taken as 0 where y = 0), gV (nullable) nb volumes of dNLL/dV,
+ * gH (nullable) dNLL/dH in the layout of H.  All device pointers; FP32
+ * projector precision only. */
+int sid_poisson_nll_grad_device(sid_psf *psf, const float *V_dev, const float *y_dev, const int64_t dims[4],
+                                double *nll, float *gV_dev, float *gH_dev, void *stream);
+/* Vector-Jacobian product of O_f = forward(H, V_f) for upstream image
+ * gradients G (nb images [X,Y]): gV = backward(G) (nullable), gH = dNLL/dH of
+ * sum_f <G_f, O_f> (nullable; needs V). */
+int sid_project_vjp_device(sid_psf *psf, const float *V_dev, const float *G_dev, const int64_t dims[4],
+                           float *gV_dev, float *gH_dev, void *stream);
 
 /* Device time (ms, HIP events) of the last call's projector kernels:
  * [0] forward GEMM, [1] forward reduce, [2] backward GEMM, [3] backward reduce

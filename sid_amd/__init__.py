@@ -62,8 +62,9 @@ class PSF:
     path.  ``precision`` is 'fp32', 'bf16x2' or 'bf16x3'.
     """
 
-    def __init__(self, H, CAindex=None, precision: str = "fp32", method: str = "auto"):
+    def __init__(self, H, CAindex=None, precision: str = "fp32", method: str = "auto", full_support: bool = False):
         self._h = ctypes.c_void_p()
+        flags = 2 if full_support else 0      # SID_FLAG_FULL_SUPPORT
         ca = None
         if CAindex is not None:
             ca_arr = np.asfortranarray(np.asarray(CAindex, dtype=np.int32))
@@ -77,13 +78,13 @@ class PSF:
             nz, n, n2, M, M2 = Hc.shape
             dims = i64(M2, M, n2, n, nz)
             torch.cuda.synchronize()
-            check(lib().sid_psf_create_device(ctypes.c_void_p(Hc.data_ptr()), dims, ca, ctypes.byref(self._h)))
+            check(lib().sid_psf_create_ex(ctypes.c_void_p(Hc.data_ptr()), dims, ca, flags | 1, ctypes.byref(self._h)))
         else:
             Ha = np.asfortranarray(np.asarray(H, dtype=np.float32))
             if Ha.ndim != 5:
                 raise ValueError("H must have 5 dimensions [M, M, Nnum, Nnum, Nz]")
             dims = i64(*Ha.shape)
-            check(lib().sid_psf_create(Ha.ctypes.data_as(ctypes.c_void_p), dims, ca, ctypes.byref(self._h)))
+            check(lib().sid_psf_create_ex(Ha.ctypes.data_as(ctypes.c_void_p), dims, ca, flags, ctypes.byref(self._h)))
         self.dims = tuple(int(d) for d in dims)
         self.M, _, self.Nnum, _, self.Nz = self.dims
         self.precision = "fp32"
@@ -92,6 +93,17 @@ class PSF:
             self.set_method(method)
 
     # -- configuration -----------------------------------------------------
+    def update(self, H):
+        """Replace the values of H (same shape) and rebuild the weights."""
+        if _is_torch(H):
+            torch = _torch()
+            Hc = H.detach().to(torch.float32).contiguous()
+            torch.cuda.synchronize()
+            check(lib().sid_psf_update(self._h, ctypes.c_void_p(Hc.data_ptr()), 1))
+        else:
+            Ha = np.asfortranarray(np.asarray(H, dtype=np.float32))
+            check(lib().sid_psf_update(self._h, Ha.ctypes.data_as(ctypes.c_void_p), 0))
+
     def set_precision(self, precision: str):
         check(lib().sid_psf_set_precision(self._h, PREC[precision]))
         self.precision = precision

@@ -29,6 +29,10 @@ SIGNATURES = {
     "sid_psf_create": (_I, [_P, _I64P, _I32P, ctypes.POINTER(_P)]),
     "sid_psf_create_device": (_I, [_P, _I64P, _I32P, ctypes.POINTER(_P)]),
     "sid_psf_destroy": (_I, [_P]),
+    "sid_psf_create_ex": (_I, [_P, _I64P, _I32P, _I, ctypes.POINTER(_P)]),
+    "sid_psf_update": (_I, [_P, _P, _I]),
+    "sid_poisson_nll_grad_device": (_I, [_P, _P, _P, _I64P, ctypes.POINTER(ctypes.c_double), _P, _P, _P]),
+    "sid_project_vjp_device": (_I, [_P, _P, _P, _I64P, _P, _P, _P]),
     "sid_psf_set_precision": (_I, [_P, _I]),
     "sid_psf_set_method": (_I, [_P, _I]),
     "sid_psf_get_info": (_I, [_P, _P]),
@@ -66,6 +70,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with "
                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        # One HIP runtime per process: when PyTorch is installed, load its
+        # libamdhip64 first so this library binds to the same runtime
+        # (two runtimes in one process cannot share the device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -71,7 +71,7 @@ struct Timer {
     }
 };
 
-static int create_common(sid_psf *p, const int64_t dims[5], const int32_t *CA)
+static int create_common(sid_psf *p, const int64_t dims[5], const int32_t *CA, bool full)
 {
     const int64_t M = dims[0], M2 = dims[1], n = dims[2], n2 = dims[3], nz = dims[4];
     if (M < 1 || M2 != M || n < 1 || n2 != n || nz < 1) {
@@ -103,7 +103,12 @@ static int create_common(sid_psf *p, const int64_t dims[5], const int32_t *CA)
         }
         p->cropped = true;
     }
-    SID_TRY(support_scan(p));
+    p->full_support = full;
+    if (full) {
+        for (auto &pl : p->planes) { pl.lo = 0; pl.hi = (int)M - 1; pl.empty = false; }
+    } else {
+        SID_TRY(support_scan(p));
+    }
     for (int64_t c = 0; c < nz; ++c) {
         Plane &pl = p->planes[c];
         if (CA) {
@@ -175,7 +180,8 @@ int sid_device_count(void)
     return k;
 }
 
-static int psf_create_impl(const float *H, bool host, const int64_t dims[5], const int32_t *CA, sid_psf **out)
+static int psf_create_impl(const float *H, bool host, const int64_t dims[5], const int32_t *CA, bool full,
+                           sid_psf **out)
 {
     if (!H || !dims || !out) {
         set_error("sid_psf_create: null argument");
@@ -204,7 +210,7 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
         if (e != hipSuccess) { rc = SID_ERR_OOM; set_error("hipMalloc(H, %lld B) failed", (long long)(elems * 4)); break; }
         e = hipMemcpy(p->H, H, elems * sizeof(float), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice);
         if (e != hipSuccess) { rc = SID_ERR_HIP; set_error("H upload failed: %s", hipGetErrorString(e)); break; }
-        rc = create_common(p, dims, CA);
+        rc = create_common(p, dims, CA, full);
     } while (0);
     if (rc != SID_OK) {
         destroy(p);
@@ -216,12 +222,65 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
 
 int sid_psf_create(const float *H, const int64_t dims[5], const int32_t *CAindex, sid_psf **out)
 {
-    return psf_create_impl(H, true, dims, CAindex, out);
+    return psf_create_impl(H, true, dims, CAindex, false, out);
 }
 
 int sid_psf_create_device(const float *H_dev, const int64_t dims[5], const int32_t *CAindex, sid_psf **out)
 {
-    return psf_create_impl(H_dev, false, dims, CAindex, out);
+    return psf_create_impl(H_dev, false, dims, CAindex, false, out);
+}
+
+int sid_psf_create_ex(const float *H, const int64_t dims[5], const int32_t *CAindex, int flags, sid_psf **out)
+{
+    return psf_create_impl(H, (flags & SID_FLAG_H_ON_DEVICE) == 0, dims, CAindex, (flags & SID_FLAG_FULL_SUPPORT) != 0,
+                           out);
+}
+
+int sid_psf_update(sid_psf *p, const float *H, int on_device)
+{
+    if (!p || !H) { set_error("sid_psf_update: null argument"); return SID_ERR_ARG; }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    const int64_t elems = p->M * p->M * p->n * p->n * p->nzH;
+    SID_TRY_HIP(hipMemcpyAsync(p->H, H, elems * sizeof(float), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                               p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    const bool had[4] = {p->wf != nullptr, p->wf_hi != nullptr, p->wf_lo != nullptr, p->wf_d != nullptr};
+    free_weights(p);
+    for (int w = 0; w < 4; ++w)
+        if (had[w] || w == 0) SID_TRY(build_weights(p, w));
+    return SID_OK;
+}
+
+int sid_poisson_nll_grad_device(sid_psf *p, const float *V, const float *y, const int64_t dims[4], double *nll,
+                                float *gV, float *gH, void *stream)
+{
+    if (!p || !V || !y || !dims) { set_error("sid_poisson_nll_grad_device: null argument"); return SID_ERR_ARG; }
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[3] < 1) {
+        set_error("dimensions must be positive");
+        return SID_ERR_DIMS;
+    }
+    if (dims[2] != p->nzH) {
+        set_error("the likelihood needs volumes with size(H,5) = %lld planes", (long long)p->nzH);
+        return SID_ERR_NZ;
+    }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return grad_core(p, V, y, dims, 0, nll, gV, gH, pick_stream(p, stream));
+}
+
+int sid_project_vjp_device(sid_psf *p, const float *V, const float *G, const int64_t dims[4], float *gV, float *gH,
+                           void *stream)
+{
+    if (!p || !G || !dims || (gH && !V)) { set_error("sid_project_vjp_device: null argument"); return SID_ERR_ARG; }
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[3] < 1) {
+        set_error("dimensions must be positive");
+        return SID_ERR_DIMS;
+    }
+    if (dims[2] != p->nzH) {
+        set_error("the projector gradient needs volumes with size(H,5) = %lld planes", (long long)p->nzH);
+        return SID_ERR_NZ;
+    }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return grad_core(p, V, G, dims, 1, nullptr, gV, gH, pick_stream(p, stream));
 }
 
 int sid_psf_destroy(sid_psf *psf)

@@ -267,7 +267,8 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
 
 // FP64 variant (reference path for error studies): v_mfma_f64_16x16x4_f64
 // with the same K permutation as MODE 0 (lane group g supplies k = 8g + s at
-// step s of a 32-wide chunk).
+// step s of a 32-wide chunk).  Its output layout differs from the FP32 one:
+// item i of lane (l16, g) is C[4i + g][l16].
 template <int NT, int RT>
 __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
     const Item *__restrict__ items, const Seg *__restrict__ segs, const double *__restrict__ A0,
@@ -330,14 +331,15 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
             }
         }
     }
+    // v_mfma_f64_16x16x4_f64 output layout: lane (l16, g), item i holds C[4i + g][l16]
     double *out = part + (int64_t)it.slab * slab_stride;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
-        const int rbase = it.mtile * MT + wave * RT * 16 + rt * 16 + 4 * g;
+        const int rbase = it.mtile * MT + wave * RT * 16 + rt * 16 + g;
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + i) * NP + ct * 16 + l16] = acc[rt][ct][i];
+            for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + 4 * i) * NP + ct * 16 + l16] = acc[rt][ct][i];
     }
 }
 
@@ -512,6 +514,15 @@ static void tap_range(int lo, int hi, int off, int n, int *t_lo, int *t_hi)
 {
     *t_lo = -floordiv(-(lo - off - (n - 1)), n);  // ceil
     *t_hi = floordiv(hi - off + (n - 1), n);
+}
+
+void free_weights(sid_psf *p)
+{
+    for (void **q : {(void **)&p->wf, (void **)&p->wb, (void **)&p->wf_hi, (void **)&p->wb_hi, (void **)&p->wf_lo,
+                     (void **)&p->wb_lo, (void **)&p->wf_d, (void **)&p->wb_d}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
 }
 
 int build_weights(sid_psf *p, int which)
@@ -966,7 +977,7 @@ __global__ void k_selftest(const float *A, const float *B, float *C32, float *C1
     for (int s = 0; s < 8; ++s)
         e = __builtin_amdgcn_mfma_f64_16x16x4f64((double)A[l16 * 32 + 8 * g + s], (double)B[(8 * g + s) * 16 + l16], e,
                                                 0, 0, 0);
-    for (int i = 0; i < 4; ++i) C64[(4 * g + i) * 16 + l16] = (float)e[i];
+    for (int i = 0; i < 4; ++i) C64[(4 * i + g) * 16 + l16] = (float)e[i];
 }
 
 int run_selftest()

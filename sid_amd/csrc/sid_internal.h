@@ -106,6 +106,7 @@ struct sid_psf {
     int64_t M = 0, n = 0, nzH = 0;
     int64_t S = 0, NP = 0, T = 0;
     bool cropped = false;
+    bool full_support = false;
     std::vector<sid::Plane> planes;
     float *H = nullptr;               // device copy, MATLAB layout
     float *wf = nullptr, *wb = nullptr;           // fp32 weights (forward / backward)
@@ -133,5 +134,9 @@ int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st);
 int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st);
 void free_plan_buffers(SensorPlan &sp);
 int support_scan(sid_psf *p);
+void free_weights(sid_psf *p);
+// sid_grad.hip
+int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4], int mode, double *nll, float *gV,
+              float *gH, hipStream_t st);
 int run_selftest();
 }  // namespace sid

@@ -1,0 +1,156 @@
+"""Adjoint kernels: Poisson NLL gradients w.r.t. volume and PSF against the
+FP64 oracle, central finite differences, the autograd wrappers, and
+bit-exact checkpoint/resume of the refinement loop."""
+import numpy as np
+import pytest
+
+from oracle import sid_oracle as O
+from tests._helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+    import sid_amd
+    from sid_amd import grad
+    if sid_amd.device_count() < 1:
+        pytest.fail("no gfx950 device visible")
+    return torch, sid_amd, grad
+
+
+def _case(seed=0, n=5, X=30, Y=25, M=11, nz=3, nb=2):
+    rng = np.random.default_rng(seed)
+    H = rng.random((M, M, n, n, nz)) + 0.05
+    H /= H.sum(axis=(0, 1), keepdims=True)
+    V = rng.random((nb, X, Y, nz)) + 0.2
+    y = np.stack([rng.poisson(O.forward_project(H, V[f]) * 50.0) / 50.0 for f in range(nb)])
+    return H, V, y
+
+
+def _to_dev(torch, H, V, y):
+    Ht = torch.from_numpy(np.ascontiguousarray(H.transpose(4, 3, 2, 1, 0)).astype(np.float32)).cuda()
+    Vt = torch.from_numpy(np.ascontiguousarray(V.transpose(0, 3, 2, 1)).astype(np.float32)).cuda()
+    yt = torch.from_numpy(np.ascontiguousarray(y.transpose(0, 2, 1)).astype(np.float32)).cuda()
+    return Ht, Vt, yt
+
+
+def _oracle_grads(H, V, y):
+    H32 = H.astype(np.float32).astype(np.float64)
+    V32 = V.astype(np.float32).astype(np.float64)
+    y32 = y.astype(np.float32).astype(np.float64)
+    F = lambda v: O.forward_project(H32, v)
+    B = lambda o: O.backward_project(H32, o)
+    nll = sum(O.poisson_nll(F, V32[f], y32[f]) for f in range(len(V)))
+    gV = np.stack([O.poisson_nll_grad_volume(F, B, V32[f], y32[f]) for f in range(len(V))])
+    gH = sum(O.poisson_nll_grad_psf(H32, V32[f], y32[f]) for f in range(len(V)))
+    return nll, gV, gH
+
+
+def test_nll_and_gradients_vs_oracle(env):
+    torch, sid_amd, grad = env
+    print("case: nll grads n5 X30 Y25 M11 Nz3 nb2", flush=True)
+    H, V, y = _case()
+    Ht, Vt, yt = _to_dev(torch, H, V, y)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    nll, gV, gH = grad.poisson_nll_grad(psf, Vt, yt)
+    nll_o, gV_o, gH_o = _oracle_grads(H, V, y)
+    assert abs(nll - nll_o) < 1e-5 * abs(nll_o)
+    e_v = rel_l2(gV.cpu().numpy().transpose(0, 3, 2, 1), gV_o)
+    e_h = rel_l2(gH.cpu().numpy().transpose(4, 3, 2, 1, 0), gH_o)
+    print("rel-L2 gV", e_v, "gH", e_h, flush=True)
+    assert e_v < 1e-5 and e_h < 1e-5
+
+
+def test_gradients_vs_central_finite_differences(env):
+    """eps = 1e-3 central differences of the FP64 oracle NLL on random
+    coordinates of V and H against the device gradients (tolerance 1e-4)."""
+    torch, sid_amd, grad = env
+    print("case: finite differences n5 X30 Y25 M11 Nz3 nb1", flush=True)
+    H, V, y = _case(seed=1, nb=1)
+    H = H.astype(np.float32).astype(np.float64)
+    V = V.astype(np.float32).astype(np.float64)
+    y = y.astype(np.float32).astype(np.float64)
+    Ht, Vt, yt = _to_dev(torch, H, V, y)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    _, gV, gH = grad.poisson_nll_grad(psf, Vt, yt)
+    gV = gV.cpu().numpy().transpose(0, 3, 2, 1)[0].astype(np.float64)
+    gH = gH.cpu().numpy().transpose(4, 3, 2, 1, 0).astype(np.float64)
+    rng = np.random.default_rng(2)
+    eps = 1e-3
+    nll = lambda HH, VV: O.poisson_nll(lambda v: O.forward_project(HH, v), VV, y[0])
+    worst = 0.0
+    for _ in range(16):
+        i, j, c = rng.integers(0, 30), rng.integers(0, 25), rng.integers(0, 3)
+        Vp, Vm = V[0].copy(), V[0].copy()
+        Vp[i, j, c] += eps
+        Vm[i, j, c] -= eps
+        fd = (nll(H, Vp) - nll(H, Vm)) / (2 * eps)
+        worst = max(worst, abs(fd - gV[i, j, c]) / max(1.0, abs(fd)))
+    for _ in range(16):
+        idx = tuple(int(rng.integers(0, s)) for s in H.shape)
+        Hp, Hm = H.copy(), H.copy()
+        Hp[idx] += eps
+        Hm[idx] -= eps
+        fd = (nll(Hp, V[0]) - nll(Hm, V[0])) / (2 * eps)
+        worst = max(worst, abs(fd - gH[idx]) / max(1.0, abs(fd)))
+    print("worst FD relative error", worst, flush=True)
+    assert worst < 1e-4
+
+
+def test_volume_gradient_is_backward_projection(env):
+    """dNLL/dV equals the backward projection (backwardProjectGPU_new) of the
+    residual image 1 - y/F."""
+    torch, sid_amd, grad = env
+    print("case: gV == B(1 - y/F)", flush=True)
+    H, V, y = _case(seed=3, nb=1)
+    Ht, Vt, yt = _to_dev(torch, H, V, y)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    _, gV, _ = grad.poisson_nll_grad(psf, Vt, yt, True, False)
+    F = psf.forward(Vt[0])
+    r = 1.0 - yt[0] / F
+    B = psf.backward(r)
+    assert rel_l2(gV[0].cpu().numpy(), B.cpu().numpy()) < 1e-6
+
+
+def test_autograd_functions(env):
+    torch, sid_amd, grad = env
+    print("case: autograd Project / PoissonNLL", flush=True)
+    Project, PoissonNLL = grad.functions()
+    H, V, y = _case(seed=4)
+    Ht, Vt, yt = _to_dev(torch, H, V, y)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    Vr = Vt.clone().requires_grad_(True)
+    Hr = Ht.clone().requires_grad_(True)
+    loss = PoissonNLL.apply(Vr, Hr, yt, psf)
+    loss.backward()
+    nll, gV, gH = grad.poisson_nll_grad(psf, Vt, yt)
+    assert abs(loss.item() - nll) < 1e-9 * abs(nll)
+    assert torch.equal(Vr.grad, gV) and torch.equal(Hr.grad, gH)
+    # Project: d<G, F(V)>/dV = B(G), /dH = project_vjp
+    Vr2 = Vt.clone().requires_grad_(True)
+    Hr2 = Ht.clone().requires_grad_(True)
+    G = torch.rand((Vt.shape[0], Vt.shape[2], Vt.shape[3]), device="cuda")
+    out = Project.apply(Vr2, Hr2, psf)
+    (out * G).sum().backward()
+    gV_o = np.stack([O.backward_project(H, G[f].cpu().numpy().T.astype(np.float64)) for f in range(len(V))])
+    assert rel_l2(Vr2.grad.cpu().numpy().transpose(0, 3, 2, 1), gV_o) < 1e-5
+    _, gH2 = grad.project_vjp(psf, Vt, G, False, True)
+    assert torch.equal(Hr2.grad, gH2)
+
+
+def test_refinement_checkpoint_resume_bit_exact(env, tmp_path):
+    torch, sid_amd, grad = env
+    print("case: refinement 4 steps, resume after 2", flush=True)
+    H, V, y = _case(seed=5)
+    Ht, Vt, yt = _to_dev(torch, H, V, y)
+    V0 = torch.full_like(Vt, 0.5)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    Va, Ha, ha = grad.refine(psf, Ht, V0, yt, 4, checkpoint=str(tmp_path / "a.st"))
+    assert all(ha[i + 1] <= ha[i] for i in range(3)), ha
+    ck = str(tmp_path / "b.st")
+    grad.refine(psf, Ht, V0, yt, 4, checkpoint=ck, stop_after=2)
+    Vb, Hb, hb = grad.refine(psf, Ht, V0, yt, 4, checkpoint=ck)
+    assert hb == ha
+    assert torch.equal(Va, Vb) and torch.equal(Ha, Hb)
