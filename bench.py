@@ -76,6 +76,47 @@ def cpu_baseline(wl, V, y_pre, sample_planes=None):
                        f"bwd {t2 - t1:.2f} s), scaled by sum(M_c^2) all/sample = {msum_all}/{msum_s}")}
 
 
+def library_bench(psf, wl_lib=None, steps=3):
+    """Batched library workload: 2500 seeds (lateral box 7-21 px, depth 3-9
+    planes, Gaussian blob sigma 2 px) in the 1005 x 1005 x 51 geometry, one
+    persistent hipGraph-captured launch per batch; a few seeds are checked
+    against the full-sensor forward projection of their sub-volume."""
+    import torch
+    from sid_amd import library, synth
+    wl = synth.WORKLOADS["library"]
+    psf.set_precision("fp32")
+    lib = library.Library(psf, (wl.X, wl.Y))
+    t = time.perf_counter()
+    boxes, vals = library.workload_seeds(wl)
+    b, v, voff, out, rects, offs = lib.prepare(boxes, vals)
+    lib.project(b, v, voff, out, use_graph=True)          # plan + capture + first launch
+    torch.cuda.synchronize()
+    log(f"library: {len(boxes)} seeds planned and captured in {time.perf_counter() - t:.1f} s")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lib.project(b, v, voff, out, use_graph=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = lib.stats()
+    errs = []
+    for s in (0, len(boxes) // 2, len(boxes) - 1):
+        x0, x1, y0, y1, z0, z1 = (int(q) for q in boxes[s])
+        V = torch.zeros((wl.nz, wl.Y, wl.X), dtype=torch.float32, device="cuda")
+        V[z0 - 1:z1, y0 - 1:y1, x0 - 1:x1] = torch.from_numpy(np.ascontiguousarray(vals[s].transpose(2, 1, 0))).cuda()
+        full = psf.forward(V).cpu().numpy().T.astype(np.float64)
+        r = rects[s]
+        ref = full[r[0] - 1:r[1], r[2] - 1:r[3]]
+        got = library.Library.patch(out, rects, offs, s).astype(np.float64)
+        errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    res = {"seeds": int(len(boxes)), "ms_per_batch": round(dt * 1e3, 3), "seeds_per_s": round(len(boxes) / dt, 1),
+           "mfma_tflops": round(2.0 * st["macs"] / dt / 1e12, 2), "work_items": st["items"],
+           "patch_pixels": int(offs[-1]), "spot_rel_l2_vs_full_sensor_fp32": errs}
+    log(f"library: {res['seeds_per_s']:.0f} seeds/s ({res['ms_per_batch']:.2f} ms per batch of {len(boxes)}), "
+        f"spot-check rel-L2 {max(errs):.2e}")
+    del out, v
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +127,7 @@ def main():
     ap.add_argument("--precisions", default="fp32,bf16x2", help="first is the baseline, last is the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 error reference")
+    ap.add_argument("--no-library", action="store_true", help="skip the batched-library measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the projector is measured on one "
@@ -188,6 +230,8 @@ def main():
             del h
         del hist64
 
+    if not args.no_library:
+        out_lib = library_bench(psf, wl_lib=None, steps=max(args.steps, 3))
     head = args.precisions.split(",")[-1]
     base = args.precisions.split(",")[0]
     hr = results[head]
@@ -218,6 +262,8 @@ def main():
         "rel_l2_vs": ref_name,
         "rel_l2": rel,
     }
+    if not args.no_library:
+        out["library"] = out_lib
     if not args.no_cpu:
         log("CPU baseline (bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline(wl, V, y)

@@ -150,6 +150,30 @@ int sid_poisson_nll_grad_device(sid_psf *psf, const float *V_dev, const float *y
 int sid_project_vjp_device(sid_psf *psf, const float *V_dev, const float *G_dev, const int64_t dims[4],
                            float *gV_dev, float *gH_dev, void *stream);
 
+/* --- Batched library projection (src/generate_LFM_library_GPU.m:63-108) ---
+ * Seed s is a sub-volume box {x0, x1, y0, y1, z0, z1} (1-based inclusive
+ * sensor pixels and planes of H; boxes is n x 6 row-major), its values a
+ * column-major (x1-x0+1) x (y1-y0+1) x (z1-z0+1) array.  Each seed is
+ * forward-projected with planes z0..z1 of H (generate_LFM_library_GPU.m:87-88,95)
+ * and its sensor image is returned on the rectangle that
+ * src/project_forward_patched.m:8-20,54-59 pastes: sid_library_rects gives
+ * {sx0, sx1, sy0, sy1} (1-based inclusive, n x 4) and the element offsets of
+ * each column-major patch in the concatenated output (n + 1). */
+int sid_library_rects(const sid_psf *psf, int64_t n_seeds, const int64_t *boxes, const int64_t sensor[2],
+                      int64_t *rects, int64_t *offsets);
+/* The same rectangle rule for one box without a handle (host only, no device
+ * needed): n = Nnum, M = kernel side, sensor = {X, Y}. */
+int sid_patched_rect(int64_t n, int64_t M, const int64_t sensor[2], const int64_t box[6], int64_t rect[4]);
+/* vals_dev: the seeds' values back to back (val_offsets: n + 1 host offsets);
+ * out_dev: concatenated patches.  flags bit 0: capture the launch sequence
+ * in a hipGraph, cached on the handle and replayed while the seeds and the
+ * buffer addresses stay the same.  One persistent kernel does the whole batch. */
+int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *boxes, const float *vals_dev,
+                               const int64_t *val_offsets, const int64_t sensor[2], float *out_dev, int flags,
+                               void *stream);
+/* {rows, work items, MFMA multiply-adds, accumulator elements} of the cached library plan. */
+int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
+
 /* Device time (ms, HIP events) of the last call's projector kernels:
  * [0] forward GEMM, [1] forward reduce, [2] backward GEMM, [3] backward reduce
  * + update, [4] whole call.  Summed over iterations for sid_deconv_rl. */

@@ -42,6 +42,10 @@ SIGNATURES = {
     "sid_backward_device": (_I, [_P, _P, _I64P, _P, _P]),
     "sid_deconv_rl": (_I, [_P, _P, _P, _I64P, _I, _P]),
     "sid_deconv_rl_device": (_I, [_P, _P, _P, _I64P, _I, _P, _P]),
+    "sid_library_rects": (_I, [_P, ctypes.c_int64, _I64P, _I64P, _I64P, _I64P]),
+    "sid_library_forward_device": (_I, [_P, ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P, _I, _P]),
+    "sid_library_stats": (_I, [_P, _I64P]),
+    "sid_patched_rect": (_I, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]),
     "sid_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "sid_plan_stats": (_I, [_P, _I64P]),
     "sid_selftest": (_I, []),

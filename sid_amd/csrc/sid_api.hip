@@ -134,6 +134,7 @@ static void destroy(sid_psf *p)
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     SensorPlan &sp = p->plan;
+    lib_plan_free(p);
     free_plan_buffers(sp);
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
                     (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
@@ -245,6 +246,7 @@ int sid_psf_update(sid_psf *p, const float *H, int on_device)
                                p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
     const bool had[4] = {p->wf != nullptr, p->wf_hi != nullptr, p->wf_lo != nullptr, p->wf_d != nullptr};
+    lib_plan_free(p);  // its captured graph holds the old weight pointers
     free_weights(p);
     for (int w = 0; w < 4; ++w)
         if (had[w] || w == 0) SID_TRY(build_weights(p, w));
@@ -508,6 +510,79 @@ int sid_deconv_rl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3]
         SID_TRY_HIP(hipMemcpyAsync(hist, dhist.ptr, nv * max_iter * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
     return SID_OK;
+}
+
+static int check_boxes(const sid_psf *p, int64_t ns, const int64_t *boxes, const int64_t sensor[2])
+{
+    for (int64_t s = 0; s < ns; ++s) {
+        const int64_t *b = boxes + 6 * s;
+        if (b[0] < 1 || b[1] < b[0] || b[1] > sensor[0] || b[2] < 1 || b[3] < b[2] || b[3] > sensor[1] || b[4] < 1 ||
+            b[5] < b[4] || b[5] > p->nzH) {
+            set_error("seed %lld: box [%lld %lld %lld %lld %lld %lld] is not inside %lld x %lld x %lld",
+                      (long long)s + 1, (long long)b[0], (long long)b[1], (long long)b[2], (long long)b[3],
+                      (long long)b[4], (long long)b[5], (long long)sensor[0], (long long)sensor[1],
+                      (long long)p->nzH);
+            return SID_ERR_DIMS;
+        }
+    }
+    return SID_OK;
+}
+
+int sid_library_rects(const sid_psf *p, int64_t ns, const int64_t *boxes, const int64_t sensor[2], int64_t *rects,
+                      int64_t *offsets)
+{
+    if (!p || (ns > 0 && (!boxes || !rects)) || !sensor || !offsets || ns < 0) {
+        set_error("sid_library_rects: null argument");
+        return SID_ERR_ARG;
+    }
+    SID_TRY(check_boxes(p, ns, boxes, sensor));
+    offsets[0] = 0;
+    for (int64_t s = 0; s < ns; ++s) {
+        patched_rect(p->n, p->M, sensor[0], sensor[1], boxes + 6 * s, rects + 4 * s);
+        const int64_t *r = rects + 4 * s;
+        offsets[s + 1] = offsets[s] + (r[1] - r[0] + 1) * (r[3] - r[2] + 1);
+    }
+    return SID_OK;
+}
+
+int sid_library_forward_device(sid_psf *p, int64_t ns, const int64_t *boxes, const float *vals,
+                               const int64_t *val_offsets, const int64_t sensor[2], float *out, int flags, void *stream)
+{
+    if (!p || ns < 0 || (ns > 0 && (!boxes || !vals || !val_offsets || !out)) || !sensor) {
+        set_error("sid_library_forward_device: null argument");
+        return SID_ERR_ARG;
+    }
+    if (ns == 0) return SID_OK;
+    SID_TRY(check_boxes(p, ns, boxes, sensor));
+    for (int64_t s = 0; s < ns; ++s) {
+        const int64_t *b = boxes + 6 * s;
+        const int64_t need = (b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1);
+        if (val_offsets[s + 1] - val_offsets[s] != need) {
+            set_error("seed %lld: %lld values given, box holds %lld", (long long)s + 1,
+                      (long long)(val_offsets[s + 1] - val_offsets[s]), (long long)need);
+            return SID_ERR_DIMS;
+        }
+    }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return library_forward(p, ns, boxes, vals, val_offsets, sensor[0], sensor[1], out, pick_stream(p, stream),
+                           flags & 1);
+}
+
+int sid_patched_rect(int64_t n, int64_t M, const int64_t sensor[2], const int64_t box[6], int64_t rect[4])
+{
+    if (!sensor || !box || !rect || n < 1 || M < 1) { set_error("sid_patched_rect: bad argument"); return SID_ERR_ARG; }
+    if (box[0] < 1 || box[1] < box[0] || box[2] < 1 || box[3] < box[2]) {
+        set_error("sid_patched_rect: empty or non-positive box");
+        return SID_ERR_DIMS;
+    }
+    patched_rect(n, M, sensor[0], sensor[1], box, rect);
+    return SID_OK;
+}
+
+int sid_library_stats(const sid_psf *p, int64_t st[4])
+{
+    if (!p || !st) { set_error("null argument"); return SID_ERR_ARG; }
+    return library_stats(p, st);
 }
 
 int sid_last_timing(const sid_psf *p, double ms[8])

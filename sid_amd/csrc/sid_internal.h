@@ -117,6 +117,7 @@ struct sid_psf {
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
     sid::SensorPlan plan;
+    void *lib = nullptr;              // cached library plan (sid_library.hip)
     hipEvent_t ev[10] = {};
     double timing[8] = {0};
 };
@@ -135,6 +136,12 @@ int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st);
 void free_plan_buffers(SensorPlan &sp);
 int support_scan(sid_psf *p);
 void free_weights(sid_psf *p);
+// sid_library.hip
+void patched_rect(int64_t n, int64_t M, int64_t X, int64_t Y, const int64_t box[6], int64_t rect[4]);
+int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *vals, const int64_t *val_off, int64_t X,
+                    int64_t Y, float *out, hipStream_t st, int use_graph);
+void lib_plan_free(sid_psf *p);
+int library_stats(const sid_psf *p, int64_t st[4]);
 // sid_grad.hip
 int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4], int mode, double *nll, float *gV,
               float *gH, hipStream_t st);

@@ -1,0 +1,121 @@
+"""Library generation around the batched HIP projector.
+
+``placement`` restates the integer seed placement of
+src/generate_LFM_library_GPU.m:68-88 (MATLAB ``round`` is half away from
+zero); ``patched_rect`` calls the library's own rectangle rule of
+src/project_forward_patched.m:8-20,54-59; ``Library`` projects thousands of
+seed sub-volumes in one persistent, hipGraph-captured launch
+(sid_library_forward_device).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from ._lib import check, i64, lib
+
+
+def matlab_round(x: float) -> int:
+    return int(math.copysign(math.floor(abs(x) + 0.5), x))
+
+
+def placement(center, neur_rad: int, dz: int, image_size, nz: int):
+    """Sub-volume box {x0, x1, y0, y1, z0, z1} (1-based inclusive) of a seed:
+    x, y, z = round(centre); z in [max(z - dz, 1), min(z + dz, Nz)] with
+    dz = ceil(neur_rad / axial) in the reference; lateral box
+    [max(x - r, 1), min(x + r, X)] x [max(y - r, 1), min(y + r, Y)]."""
+    x, y, z = (matlab_round(float(c)) for c in center)
+    return (max(x - neur_rad, 1), min(x + neur_rad, image_size[0]),
+            max(y - neur_rad, 1), min(y + neur_rad, image_size[1]),
+            max(z - dz, 1), min(z + dz, nz))
+
+
+def patched_rect(n: int, M: int, sensor, box):
+    rect = (ctypes.c_int64 * 4)()
+    check(lib().sid_patched_rect(int(n), int(M), i64(*sensor), i64(*box), rect))
+    return tuple(rect)
+
+
+def gaussian_blob(box, center, sigma: float):
+    """FP32 Gaussian blob on the box (column-major), centred at the float
+    centre (1-based coordinates)."""
+    x = np.arange(box[0], box[1] + 1, dtype=np.float64) - center[0]
+    y = np.arange(box[2], box[3] + 1, dtype=np.float64) - center[1]
+    z = np.arange(box[4], box[5] + 1, dtype=np.float64) - center[2]
+    g = (np.exp(-x * x / (2 * sigma * sigma))[:, None, None] * np.exp(-y * y / (2 * sigma * sigma))[None, :, None]
+         * np.exp(-z * z / (2 * sigma * sigma))[None, None, :])
+    return np.asfortranarray(g.astype(np.float32))
+
+
+class Library:
+    """Batched projection of seed sub-volumes with one PSF handle."""
+
+    def __init__(self, psf, sensor):
+        self.psf = psf
+        self.sensor = (int(sensor[0]), int(sensor[1]))
+
+    def rects(self, boxes):
+        boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int64).reshape(-1, 6))
+        ns = boxes.shape[0]
+        rects = np.zeros((ns, 4), dtype=np.int64)
+        offs = np.zeros(ns + 1, dtype=np.int64)
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        check(lib().sid_library_rects(self.psf._h, ns, P(boxes), i64(*self.sensor), P(rects), P(offs)))
+        return rects, offs
+
+    def prepare(self, boxes, values):
+        """Upload the seeds' values; returns (vals_dev, val_offsets, out_dev, rects, offsets)."""
+        import torch
+        boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int64).reshape(-1, 6))
+        flat = [np.asarray(v, dtype=np.float32).ravel(order="F") for v in values]
+        voff = np.zeros(len(flat) + 1, dtype=np.int64)
+        voff[1:] = np.cumsum([f.size for f in flat])
+        vals = torch.from_numpy(np.concatenate(flat) if flat else np.zeros(1, np.float32)).cuda()
+        rects, offs = self.rects(boxes)
+        out = torch.empty(int(max(offs[-1], 1)), dtype=torch.float32, device="cuda")
+        return boxes, vals, voff, out, rects, offs
+
+    def project(self, boxes, vals, voff, out, use_graph: bool = True):
+        """Run the batched projection into ``out`` (device tensor)."""
+        import torch
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        torch.cuda.synchronize()
+        check(lib().sid_library_forward_device(self.psf._h, boxes.shape[0], P(boxes), ctypes.c_void_p(vals.data_ptr()),
+                                               P(voff), i64(*self.sensor), ctypes.c_void_p(out.data_ptr()),
+                                               1 if use_graph else 0,
+                                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return out
+
+    def stats(self):
+        buf = (ctypes.c_int64 * 4)()
+        check(lib().sid_library_stats(self.psf._h, buf))
+        return dict(zip(["rows", "items", "macs", "acc_elems"], list(buf)))
+
+    @staticmethod
+    def patch(out, rects, offs, s):
+        """Seed s's patch as an (sx1-sx0+1, sy1-sy0+1) array (MATLAB indexing)."""
+        r = rects[s]
+        w, h = int(r[1] - r[0] + 1), int(r[3] - r[2] + 1)
+        flat = out[int(offs[s]):int(offs[s + 1])]
+        if hasattr(flat, "cpu"):
+            flat = flat.cpu().numpy()
+        return flat.reshape((w, h), order="F")
+
+
+def workload_seeds(wl):
+    """Boxes and Gaussian-blob values of the batched-library workload: seeds
+    with lateral box side in [box_lo, box_hi] (radius side // 2) and depth in
+    [dz_lo, dz_hi] planes (half-depth depth // 2), placed by ``placement``."""
+    from .synth import make_seeds
+    sd = make_seeds(wl)
+    boxes, vals = [], []
+    for k in range(len(sd["box"])):
+        r = int(sd["box"][k]) // 2
+        dz = int(sd["depth"][k]) // 2
+        c = sd["center"][k]
+        b = placement(c, r, dz, (wl.X, wl.Y), wl.nz)
+        boxes.append(b)
+        vals.append(gaussian_blob(b, c, sd["sigma"]))
+    return np.asarray(boxes, dtype=np.int64), vals

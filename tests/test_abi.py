@@ -58,3 +58,40 @@ def test_null_arguments_are_rejected():
     assert L.sid_forward(None, None, None, None) == 1
     assert b"null" in L.sid_last_error()
     assert L.sid_version() >= 1
+
+
+def test_patched_rect_matches_oracle_project_forward_patched():
+    """The library's rectangle rule is integer-identical to
+    project_forward_patched (oracle) for random boxes, including clipped ones."""
+    from oracle import sid_oracle as O
+    from sid_amd import library
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        n = int(rng.choice([3, 5, 15]))
+        M = int(rng.choice([7, 9, 21, 45]))
+        X, Y = int(rng.integers(20, 120)), int(rng.integers(20, 120))
+        x0 = int(rng.integers(1, X + 1)); x1 = int(rng.integers(x0, min(X, x0 + 25) + 1))
+        y0 = int(rng.integers(1, Y + 1)); y1 = int(rng.integers(y0, min(Y, y0 + 25) + 1))
+        box = (x0, x1, y0, y1, 1, 1)
+        rect = library.patched_rect(n, M, (X, Y), box)
+        # oracle: the rectangle that project_forward_patched pastes
+        pixels = np.array([[x0, y0, 1], [x1, y1, 1]])
+        H = np.zeros((M, M, n, n, 1))
+        _, info = O.project_forward_patched(pixels, (X, Y), n, H, np.ones((1, 1, 1)),
+                                            np.ones((2, 3), dtype=np.int64))
+        assert rect == (info["x0"], info["x1"], info["y0"], info["y1"]), (n, M, X, Y, box)
+
+
+def test_library_placement_matches_oracle():
+    from oracle import sid_oracle as O
+    from sid_amd import library
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        c = (rng.uniform(-1, 60), rng.uniform(-1, 60), rng.uniform(0, 12))
+        r = int(rng.integers(1, 8))
+        axial = float(rng.choice([1.0, 2.0, 2.5, 4.0]))
+        p = O.library_gpu_placement(c, r, axial, (50, 55), 10)
+        dz = int(np.ceil(r / axial))
+        assert library.placement(c, r, dz, (50, 55), 10) == (p["x0"], p["x1"], p["y0"], p["y1"], p["z0"], p["z1"])
+    # half-integers round away from zero
+    assert library.placement((2.5, -0.5, 3.5), 1, 0, (10, 10), 9)[0:2] == (2, 4)
