@@ -118,6 +118,18 @@ __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, 
     }
 }
 
+// Work items are listed K-group major, M tile minor.  Blocks are dispatched
+// round-robin over the 8 XCDs (block b runs on XCD b % 8), so remap block b
+// to a contiguous run of items per XCD: consecutive M tiles of one K group
+// then share their Toeplitz A rows and the weight chunks in that XCD's L2.
+__device__ __forceinline__ int xcd_item(int b, int nb)
+{
+    constexpr int NX = 8;
+    const int per = nb / NX, rem = nb - per * NX;
+    const int x = b % NX, local = b / NX;
+    return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
 // ---------------------------------------------------------------------------
 // The projector GEMM.  256 threads = 4 waves; each wave owns RT*16 rows of the
 // M tile and all NT*16 columns.  B chunks are staged in LDS and shared by the
@@ -143,7 +155,7 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
     constexpr int KC = KChunk<MODE>::value;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
-    const Item it = items[blockIdx.x];
+    const Item it = items[xcd_item(blockIdx.x, gridDim.x)];
 
     int64_t rowoff[RT];
 #pragma unroll
@@ -282,7 +294,7 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
     __shared__ double Bs[NP * LDK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
-    const Item it = items[blockIdx.x];
+    const Item it = items[xcd_item(blockIdx.x, gridDim.x)];
     int64_t rowoff[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
