@@ -128,6 +128,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 error reference")
     ap.add_argument("--no-library", action="store_true", help="skip the batched-library measurement")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="run one 1 GiB device copy first (the reference kernel for scaling PMC byte counters)")
+    ap.add_argument("--method", default="auto", choices=["auto", "direct", "fft"],
+                    help="projector method per plane (auto = autotuned direct-vs-FFT choice)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the projector is measured on one "
@@ -144,13 +148,20 @@ def main():
         sys.exit(1)
     wl = synth.WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
+    if args.profile_copy:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("reference copy: 1 GiB read + 1 GiB written")
     log(f"workload {wl.name}: X={wl.X} Y={wl.Y} n={wl.n} Nz={wl.nz} M={wl.M}")
     t = time.perf_counter()
     H, CA = synth.make_psf(wl.n, wl.M, wl.zs, xp=torch, device=dev)
     torch.cuda.synchronize()
     log(f"PSF generated on device in {time.perf_counter() - t:.1f} s ({H.numel() * 4 / 1e9:.2f} GB)")
     t = time.perf_counter()
-    psf = sid_amd.PSF(H)
+    psf = sid_amd.PSF(H, method=args.method)
     del H
     torch.cuda.empty_cache()
     info = psf.info()
@@ -179,7 +190,7 @@ def main():
         for _ in range(args.warmup):
             psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
         torch.cuda.synchronize()
-        tim = np.zeros(8)
+        tim = np.zeros(12)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
@@ -188,16 +199,31 @@ def main():
         t1 = time.perf_counter()
         ms_step = (t1 - t0) * 1e3 / args.steps
         n_launch = args.steps * args.iters
-        t_f, t_b = tim[0] / n_launch, tim[2] / n_launch
-        avg_launch_ms = 0.5 * (t_f + t_b)
-        achieved = flops_proj / (avg_launch_ms * 1e-3)
-        results[prec] = dict(iter_per_s=args.steps * args.iters / (t1 - t0), ms_per_step=ms_step,
-                             fwd_gemm_ms=t_f, bwd_gemm_ms=t_b, fwd_reduce_ms=tim[1] / n_launch,
-                             bwd_reduce_ms=tim[3] / n_launch, achieved_tflops=achieved / 1e12,
-                             frac_of_peak=achieved / PEAK[prec],
-                             proj_per_s=2.0 / ((t_f + t_b + (tim[1] + tim[3]) / n_launch) * 1e-3))
-        log(f"{prec}: {results[prec]['iter_per_s']:.2f} RL iter/s, {ms_step:.1f} ms/step, fwd GEMM {t_f:.2f} ms, "
-            f"bwd GEMM {t_b:.2f} ms, {achieved / 1e12:.1f} TFLOP/s algorithmic")
+        t_f, t_b = tim[0] / n_launch, tim[2] / n_launch          # direct GEMM per launch
+        t_ff, t_fb = tim[5] / n_launch, tim[6] / n_launch        # FFT path per projection
+        t_gf, t_gb = tim[7] / n_launch, tim[8] / n_launch        # spectra GEMVs inside it
+        inf = psf.info()
+        meth = inf["plane_method"]
+        sup = inf["support"]
+        flops_direct = 2.0 * wl.X * wl.Y * sum((hi - lo + 1) ** 2 for (lo, hi), m in zip(sup, meth) if m == "direct")
+        res = dict(iter_per_s=args.steps * args.iters / (t1 - t0), ms_per_step=ms_step,
+                   fwd_gemm_ms=t_f, bwd_gemm_ms=t_b, fwd_fft_ms=t_ff, bwd_fft_ms=t_fb,
+                   fwd_gemv_ms=t_gf, bwd_gemv_ms=t_gb,
+                   fwd_reduce_ms=tim[1] / n_launch, bwd_reduce_ms=tim[3] / n_launch,
+                   planes_direct=sum(m == "direct" for m in meth), planes_fft=sum(m == "fft" for m in meth),
+                   spectra_gb=inf["spectra_bytes"] / 1e9, autotune_ms=inf["autotune_ms"],
+                   proj_per_s=2.0 / ((t_f + t_b + t_ff + t_fb + (tim[1] + tim[3]) / n_launch) * 1e-3))
+        if t_f + t_b > 0:
+            res["gemm_achieved_tflops"] = flops_direct / (0.5 * (t_f + t_b) * 1e-3) / 1e12
+            res["gemm_frac_of_peak"] = res["gemm_achieved_tflops"] * 1e12 / PEAK[prec]
+        if t_gf + t_gb > 0:
+            # FFT path: each GEMV streams the spectra once (algorithmic bytes per launch)
+            res["gemv_stream_tbps"] = inf["spectra_bytes"] / (0.5 * (t_gf + t_gb) * 1e-3) / 1e12
+            res["gemv_frac_of_hbm"] = res["gemv_stream_tbps"] * 1e12 / HBM_PEAK
+        results[prec] = res
+        log(f"{prec}: {res['iter_per_s']:.2f} RL iter/s, {ms_step:.1f} ms/step; per projection: direct GEMM "
+            f"fwd {t_f:.2f} / bwd {t_b:.2f} ms on {res['planes_direct']} planes, FFT path fwd {t_ff:.2f} / "
+            f"bwd {t_fb:.2f} ms on {res['planes_fft']} planes ({res['spectra_gb']:.1f} GB spectra)")
 
     # errors against the FP64 device path (FP64 MFMA, float I/O), per output and per iteration
     rel = {}
@@ -236,6 +262,33 @@ def main():
     base = args.precisions.split(",")[0]
     hr = results[head]
     peak = PEAK[head]
+    # roofline of the dominant kernel family of the headline method
+    t_gemm = hr["fwd_gemm_ms"] + hr["bwd_gemm_ms"]
+    t_fftp = hr["fwd_fft_ms"] + hr["bwd_fft_ms"]
+    tj = None
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_rl_full.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        if not (tj.get("precision") == head and tj.get("method") == args.method and wl.name == tj.get("workload")):
+            tj = None
+    if t_gemm >= t_fftp:
+        gk = f"sid::k_proj_gemm<15, 2, {({'fp32': 0, 'bf16x2': 1, 'bf16x3': 2}).get(head, 0)}>"
+        traffic = (tj["kernels"][gk]["read_bytes"] + tj["kernels"][gk]["write_bytes"]) if tj and gk in tj["kernels"] else None
+        roof = {"bound": "mfma", "achieved": round(hr["gemm_achieved_tflops"], 3), "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": round(hr["gemm_frac_of_peak"], 4), "traffic": traffic,
+                "kernel": "k_proj_gemm (forward and backward launches, direct planes)",
+                "algorithmic_per_launch": "2*X*Y*sum(M_c^2) over direct planes"}
+    else:
+        traffic = None
+        if tj:
+            kk = tj["kernels"]
+            traffic = 0.5 * sum(kk[k]["read_bytes"] + kk[k]["write_bytes"]
+                                for k in ("sid::k_fft_gemv_fwd", "sid::k_fft_gemv_bwd"))
+        roof = {"bound": "hbm", "achieved": round(hr["gemv_stream_tbps"] * 1e3, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(hr["gemv_frac_of_hbm"], 4),
+                "traffic": traffic, "traffic_source": "profiles/pmc_traffic_rl_full_bf16x2.md" if traffic else None,
+                "kernel": "k_fft_gemv_fwd / k_fft_gemv_bwd (spectra GEMVs of the FFT planes)",
+                "algorithmic_per_launch": f"{hr['spectra_gb']:.2f} GB of spectra streamed once per launch"}
     out = {
         "metric": "RL iter/s @1005²×51z: BF16-split MFMA vs plain FP32; rel-L2 err vs FP64",
         "value": round(hr["iter_per_s"], 4),
@@ -251,11 +304,9 @@ def main():
         "data": "synthetic (seeded PSF, ball volume, Poisson image)",
         "config": {"workload": f"{wl.name}: {wl.X}x{wl.Y} px, Nnum={wl.n}, {wl.nz} planes z={wl.zs[0]}..{wl.zs[-1]}, "
                                f"M={wl.M}, {wl.n_balls} balls, {args.iters} ISRA iterations per step",
-                   "method": head, "baseline_method": base, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": round(hr["achieved_tflops"], 3), "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": round(hr["achieved_tflops"] * 1e12 / peak, 4), "traffic": None,
-                     "kernel": "k_proj_gemm (forward and backward launches)",
-                     "flops_per_launch": flops_proj},
+                   "method": head, "baseline_method": base, "projector": args.method,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
         "methods": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
         "speedup_vs_fp32": round(hr["iter_per_s"] / results[base]["iter_per_s"], 3) if base in results else None,

@@ -108,6 +108,9 @@ typedef struct {
     int64_t weight_bytes;      /* device bytes held by projector weights */
     int64_t support_lo[512];   /* per-plane inclusive kernel support used (0-based) */
     int64_t support_hi[512];
+    int32_t plane_method[512]; /* per plane in the current sensor plan: SID_METHOD_DIRECT / _FFT, 0 = empty */
+    double autotune_ms[4];     /* last autotune: direct ms, FFT ms (all eligible planes), #eligible, #chosen */
+    int64_t spectra_bytes;     /* device bytes of FFT-path spectra in the current plan */
 } sid_psf_info;
 int sid_psf_get_info(const sid_psf *psf, sid_psf_info *info);
 
@@ -175,14 +178,21 @@ int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *box
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
 
 /* Device time (ms, HIP events) of the last call's projector kernels:
- * [0] forward GEMM, [1] forward reduce, [2] backward GEMM, [3] backward reduce
- * + update, [4] whole call.  Summed over iterations for sid_deconv_rl. */
-int sid_last_timing(const sid_psf *psf, double ms[8]);
+ * [0] forward direct GEMM, [1] forward reduce, [2] backward direct GEMM,
+ * [3] backward reduce + update, [4] whole call, [5] forward FFT path,
+ * [6] backward FFT path, [7] forward spectra GEMV, [8] backward spectra GEMV,
+ * [9..11] 0.  Summed over iterations for sid_deconv_rl. */
+int sid_last_timing(const sid_psf *psf, double ms[12]);
 
 /* Kernel launch counts and GEMM geometry of the direct path for the current
  * sensor plan (for roofline accounting): {macs_fwd, macs_bwd, items_fwd,
  * items_bwd, slabs_fwd, slabs_bwd, rows_padded, cols_padded}. */
 int sid_plan_stats(const sid_psf *psf, int64_t stats[8]);
+
+/* The FFT primitive of the FFT path: nseq contiguous complex sequences
+ * (interleaved re, im floats) of length L (2/3/5-smooth, <= 2048) transformed
+ * in place on the device; direction -1 forward, +1 inverse (unnormalised). */
+int sid_fft_device(int L, int64_t nseq, float *data_dev, int direction, void *stream);
 
 /* Built-in check of the MFMA lane layouts the kernels assume (one 16x16x32
  * BF16 and one 16x16x4 FP32 product against a host loop).  0 = pass. */

@@ -114,11 +114,14 @@ class PSF:
     def info(self) -> dict:
         inf = _lib.PsfInfo()
         check(lib().sid_psf_get_info(self._h, ctypes.byref(inf)))
+        names = {0: "empty", 1: "direct", 2: "fft"}
         return dict(M=inf.M, Nnum=inf.Nnum, Nz=inf.Nz, taps_max=inf.taps_max, weight_bytes=inf.weight_bytes,
-                    support=[(inf.support_lo[c], inf.support_hi[c]) for c in range(inf.Nz)])
+                    support=[(inf.support_lo[c], inf.support_hi[c]) for c in range(inf.Nz)],
+                    plane_method=[names.get(inf.plane_method[c], "?") for c in range(inf.Nz)],
+                    autotune_ms=list(inf.autotune_ms), spectra_bytes=inf.spectra_bytes)
 
     def timing(self) -> list:
-        buf = (ctypes.c_double * 8)()
+        buf = (ctypes.c_double * 12)()
         check(lib().sid_last_timing(self._h, buf))
         return list(buf)
 

@@ -49,6 +49,7 @@ SIGNATURES = {
     "sid_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "sid_plan_stats": (_I, [_P, _I64P]),
     "sid_selftest": (_I, []),
+    "sid_fft_device": (_I, [_I, ctypes.c_int64, _P, _I, _P]),
 }
 
 
@@ -61,7 +62,9 @@ class SidError(RuntimeError):
 class PsfInfo(ctypes.Structure):
     _fields_ = [("M", ctypes.c_int64), ("Nnum", ctypes.c_int64), ("Nz", ctypes.c_int64),
                 ("taps_max", ctypes.c_int64), ("weight_bytes", ctypes.c_int64),
-                ("support_lo", ctypes.c_int64 * 512), ("support_hi", ctypes.c_int64 * 512)]
+                ("support_lo", ctypes.c_int64 * 512), ("support_hi", ctypes.c_int64 * 512),
+                ("plane_method", ctypes.c_int32 * 512), ("autotune_ms", ctypes.c_double * 4),
+                ("spectra_bytes", ctypes.c_int64)]
 
 
 _lib = None

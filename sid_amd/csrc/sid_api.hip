@@ -60,7 +60,8 @@ struct Timer {
     void span(int phase, int e0) { spans.push_back({phase, e0}); }
     void finish(int e_start, int e_end)
     {
-        for (int i = 0; i < 8; ++i) p->timing[i] = 0.0;
+        for (int i = 0; i < 12; ++i) p->timing[i] = 0.0;
+        fft_timing_collect(p, &p->timing[7], &p->timing[8]);
         for (auto &s : spans) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, evs[s.second], evs[s.second + 1]) == hipSuccess) p->timing[s.first] += ms;
@@ -135,6 +136,7 @@ static void destroy(sid_psf *p)
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     SensorPlan &sp = p->plan;
     lib_plan_free(p);
+    fft_state_free(sp.fft);
     free_plan_buffers(sp);
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
                     (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
@@ -310,12 +312,12 @@ int sid_psf_set_precision(sid_psf *p, int precision)
 int sid_psf_set_method(sid_psf *p, int method)
 {
     if (!p) { set_error("null handle"); return SID_ERR_ARG; }
-    if (method == SID_METHOD_AUTO || method == SID_METHOD_DIRECT) {
-        p->method = method;
-        return SID_OK;
+    if (method != SID_METHOD_AUTO && method != SID_METHOD_DIRECT && method != SID_METHOD_FFT) {
+        set_error("unknown method %d", method);
+        return SID_ERR_ARG;
     }
-    set_error("method %d is not available in this build", method);
-    return SID_ERR_STATE;
+    p->method = method;
+    return SID_OK;
 }
 
 int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
@@ -331,7 +333,11 @@ int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
     for (int64_t c = 0; c < p->nzH && c < 512; ++c) {
         info->support_lo[c] = p->planes[c].empty ? 0 : p->planes[c].lo;
         info->support_hi[c] = p->planes[c].empty ? -1 : p->planes[c].hi;
+        const bool fft = (size_t)c < p->plan.fft_use.size() && p->plan.fft_use[c];
+        info->plane_method[c] = p->planes[c].empty ? 0 : (fft ? SID_METHOD_FFT : SID_METHOD_DIRECT);
     }
+    for (int i = 0; i < 4; ++i) info->autotune_ms[i] = p->auto_ms[i];
+    info->spectra_bytes = p->plan.fft.g_bytes;
     return SID_OK;
 }
 
@@ -356,16 +362,20 @@ static int check_vdims(sid_psf *p, const int64_t vdims[3], bool rl)
 
 static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
 {
+    SID_TRY(autotune_methods(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     Timer tm(p, st);
     const int e0 = tm.mark();
     SID_TRY(launch_vol_in(p, V_dev, vdims[2], 0, st));
     const int e1 = tm.mark();
     SID_TRY(launch_gemm(p, 0, st));
+    const int e1b = tm.mark();
+    SID_TRY(fft_forward(p, st));
     const int e2 = tm.mark();
     SID_TRY(launch_fwd_reduce(p, O_dev, st));
     const int e3 = tm.mark();
     tm.span(0, e1);
+    tm.span(5, e1b);
     tm.span(1, e2);
     (void)e3;
     SID_TRY_HIP(hipStreamSynchronize(st));
@@ -378,16 +388,20 @@ static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2],
     // the backward volume always has all planes of H (backwardProjectGPU_new.m:4-5); the
     // sensor plan keeps the nz of the last forward call.
     const int64_t nz = (p->plan.vol && p->plan.X == odims[0] && p->plan.Y == odims[1]) ? p->plan.nz : p->nzH;
+    SID_TRY(autotune_methods(p, odims[0], odims[1], nz, st));
     SID_TRY(ensure_plan(p, odims[0], odims[1], nz));
     Timer tm(p, st);
     const int e0 = tm.mark();
     SID_TRY(launch_img_in(p, O_dev, st));
     const int e1 = tm.mark();
     SID_TRY(launch_gemm(p, 1, st));
+    const int e1b = tm.mark();
+    SID_TRY(fft_backward(p, st));
     const int e2 = tm.mark();
     SID_TRY(launch_bwd_reduce(p, B_dev, 0, st));
     const int e3 = tm.mark();
     tm.span(2, e1);
+    tm.span(6, e1b);
     tm.span(3, e2);
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e3);
@@ -454,6 +468,7 @@ int sid_backward(sid_psf *p, const float *O, const int64_t odims[2], float *B)
 static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist,
                    hipStream_t st)
 {
+    SID_TRY(autotune_methods(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     SensorPlan &sp = p->plan;
     const int64_t nv = vdims[0] * vdims[1] * vdims[2];
@@ -464,14 +479,18 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
     for (int it = 0; it < max_iter; ++it) {
         const int a = tm.mark();
         SID_TRY(launch_gemm(p, 0, st));
+        const int a2 = tm.mark();
+        SID_TRY(fft_forward(p, st));
         const int b = tm.mark();
         SID_TRY(launch_fwd_reduce(p, nullptr, st));
         const int c = tm.mark();
         SID_TRY(launch_gemm(p, 1, st));
+        const int c2 = tm.mark();
+        SID_TRY(fft_backward(p, st));
         const int d = tm.mark();
         SID_TRY(launch_bwd_reduce(p, hist ? hist + (int64_t)it * nv : nullptr, 1, st));
         const int e = tm.mark();
-        tm.span(0, a); tm.span(1, b); tm.span(2, c); tm.span(3, d);
+        tm.span(0, a); tm.span(5, a2); tm.span(1, b); tm.span(2, c); tm.span(6, c2); tm.span(3, d);
         (void)e;
     }
     SID_TRY(launch_vol_out(p, X, vdims[2], st));
@@ -585,10 +604,10 @@ int sid_library_stats(const sid_psf *p, int64_t st[4])
     return library_stats(p, st);
 }
 
-int sid_last_timing(const sid_psf *p, double ms[8])
+int sid_last_timing(const sid_psf *p, double ms[12])
 {
     if (!p || !ms) { set_error("null argument"); return SID_ERR_ARG; }
-    for (int i = 0; i < 8; ++i) ms[i] = p->timing[i];
+    for (int i = 0; i < 12; ++i) ms[i] = p->timing[i];
     return SID_OK;
 }
 
@@ -601,6 +620,13 @@ int sid_plan_stats(const sid_psf *p, int64_t st[8])
     st[4] = sp.fwd.nslabs; st[5] = sp.bwd.nslabs;
     st[6] = sp.mpad; st[7] = p->NP;
     return SID_OK;
+}
+
+int sid_fft_device(int L, int64_t nseq, float *data, int direction, void *stream)
+{
+    if (!data || nseq < 0 || (direction != -1 && direction != 1)) { set_error("sid_fft_device: bad argument"); return SID_ERR_ARG; }
+    SID_TRY(check_device());
+    return fft_test(L, nseq, (float2 *)data, direction, (hipStream_t)stream);
 }
 
 int sid_selftest(void)

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sid_internal.h"
@@ -118,18 +119,6 @@ __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, 
     }
 }
 
-// Work items are listed K-group major, M tile minor.  Blocks are dispatched
-// round-robin over the 8 XCDs (block b runs on XCD b % 8), so remap block b
-// to a contiguous run of items per XCD: consecutive M tiles of one K group
-// then share their Toeplitz A rows and the weight chunks in that XCD's L2.
-__device__ __forceinline__ int xcd_item(int b, int nb)
-{
-    constexpr int NX = 8;
-    const int per = nb / NX, rem = nb - per * NX;
-    const int x = b % NX, local = b / NX;
-    return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
-}
-
 // ---------------------------------------------------------------------------
 // The projector GEMM.  256 threads = 4 waves; each wave owns RT*16 rows of the
 // M tile and all NT*16 columns.  B chunks are staged in LDS and shared by the
@@ -155,7 +144,7 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
     constexpr int KC = KChunk<MODE>::value;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
-    const Item it = items[xcd_item(blockIdx.x, gridDim.x)];
+    const Item it = items[blockIdx.x];
 
     int64_t rowoff[RT];
 #pragma unroll
@@ -294,7 +283,7 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
     __shared__ double Bs[NP * LDK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
-    const Item it = items[xcd_item(blockIdx.x, gridDim.x)];
+    const Item it = items[blockIdx.x];
     int64_t rowoff[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
@@ -373,7 +362,8 @@ __device__ __forceinline__ void store_shadow(u16 *hi, u16 *lo, int64_t o, float 
 template <typename T>
 __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t slab_stride, int NP, int nrows,
                              int n, int Ly, int YP, int T_, int S, int X, int Y, T *__restrict__ img,
-                             u16 *__restrict__ hi, u16 *__restrict__ lo, float *__restrict__ O_col)
+                             u16 *__restrict__ hi, u16 *__restrict__ lo, float *__restrict__ O_col,
+                             const float2 *__restrict__ fft_img, int fft_ny)
 {
     const int nn2 = n * n;
     const int64_t total = (int64_t)nrows * nn2;
@@ -385,6 +375,7 @@ __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t sla
         for (int k = 0; k < nslabs; ++k) s += p[(int64_t)k * slab_stride];
         const int Px = r / Ly, Py = r - Px * Ly;
         const int x = Px * n + uv % n, y = Py * n + uv / n;
+        if (fft_img) s += (T)fft_img[(int64_t)x * fft_ny + y].x;   // planes on the FFT path
         if (x >= X || y >= Y) s = 0;
         const int64_t o = ((int64_t)(Px + T_) * YP + (Py + T_)) * S + uv;
         img[o] = s;
@@ -401,7 +392,8 @@ template <typename T>
 __global__ void k_bwd_reduce(const T *__restrict__ part, const int *__restrict__ plane_slab0, int64_t slab_stride,
                              int NP, int nrows, int nz, int n, int Ly, int64_t XP, int64_t YP, int T_, int S, int X,
                              int Y, float *__restrict__ B_col, int rl_update, const T *__restrict__ htf,
-                             T *__restrict__ vol, u16 *__restrict__ hi, u16 *__restrict__ lo)
+                             T *__restrict__ vol, u16 *__restrict__ hi, u16 *__restrict__ lo,
+                             const int *__restrict__ plane_fft, const float *__restrict__ dfft)
 {
     const int nn2 = n * n;
     const int64_t per_plane = (int64_t)nrows * nn2;
@@ -419,11 +411,12 @@ __global__ void k_bwd_reduce(const T *__restrict__ part, const int *__restrict__
         const int x = Px * n + ab % n, y = Py * n + ab / n;
         const bool real = x < X && y < Y;
         const int64_t col = (int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c);
+        const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
+        if (plane_fft && plane_fft[c]) d = (T)dfft[o];   // plane on the FFT path
         if (!rl_update) {
             if (real) B_col[col] = (float)d;
             continue;
         }
-        const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
         T v = 0;
         if (real) {
             T ratio = htf[o] / d;
@@ -673,8 +666,20 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     SensorPlan &sp = p->plan;
     const int RT = row_tiles_for(p->precision);
     const int64_t mt = 4 * RT * 16;
-    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol) return ensure_precision_buffers(p);
+    // planes on the FFT path (the FP64 reference path is direct only)
+    std::vector<char> use(p->nzH, 0);
+    if (p->precision != SID_PREC_FP64) {
+        for (int64_t c = 0; c < p->nzH; ++c) {
+            if (!fft_eligible(p, c)) continue;
+            if (p->method == SID_METHOD_FFT) use[c] = 1;
+            else if (p->method == SID_METHOD_AUTO && (size_t)c < p->auto_fft.size()) use[c] = p->auto_fft[c];
+        }
+    }
+    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol && use == sp.fft_use)
+        return ensure_precision_buffers(p);
     free_plan_buffers(sp);
+    fft_state_free(sp.fft);
+    sp.fft_use = use;
     const int64_t n = p->n, S = p->S, NP = p->NP, T = p->T;
     sp.X = X; sp.Y = Y; sp.nz = nz;
     sp.Lx = (X + n - 1) / n; sp.Ly = (Y + n - 1) / n;
@@ -695,7 +700,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     F.segs.clear(); F.items.clear(); F.macs = 0;
     for (int64_t c = 0; c < nz; ++c) {
         const Plane &pl = p->planes[c];
-        if (pl.empty) continue;
+        if (pl.empty || use[c]) continue;
         const int ntap = pl.tf_hi - pl.tf_lo + 1;
         for (int tx = 0; tx < ntap; ++tx) {
             const int dx = pl.tf_lo + tx;
@@ -724,7 +729,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     for (int64_t c = 0; c < p->nzH; ++c) {
         pseg0[c] = (int)B.segs.size();
         const Plane &pl = p->planes[c];
-        if (pl.empty) continue;
+        if (pl.empty || use[c]) continue;
         const int ntap = pl.tb_hi - pl.tb_lo + 1;
         for (int tx = 0; tx < ntap; ++tx) {
             const int dx = pl.tb_lo + tx;
@@ -778,6 +783,9 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     SID_TRY_HIP(hipMemsetAsync(sp.htf, 0, sp.vol_elems * sizeof(float), p->stream));
     SID_TRY_HIP(hipMemsetAsync(sp.img, 0, sp.img_elems * sizeof(float), p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    bool any = false;
+    for (char u : use) any = any || u;
+    if (any) SID_TRY(fft_state_build(p, use, p->stream));
     return ensure_precision_buffers(p);
 }
 
@@ -884,11 +892,13 @@ int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st)
     if (f64)
         hipLaunchKernelGGL(k_fwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
-                           (int)sp.X, (int)sp.Y, sp.img_d, (u16 *)nullptr, (u16 *)nullptr, O_col);
+                           (int)sp.X, (int)sp.Y, sp.img_d, (u16 *)nullptr, (u16 *)nullptr, O_col,
+                           (const float2 *)nullptr, 0);
     else
         hipLaunchKernelGGL(k_fwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
-                           (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr, O_col);
+                           (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr, O_col,
+                           sp.fft.active ? (const float2 *)sp.fft.grid : nullptr, sp.fft.Ny);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -904,12 +914,14 @@ int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st)
         hipLaunchKernelGGL(k_bwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d,
                            sp.bwd.d_plane_slab0, stride, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n,
                            (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update,
-                           sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr);
+                           sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr, (const int *)nullptr,
+                           (const float *)nullptr);
     else
         hipLaunchKernelGGL(k_bwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part,
                            sp.bwd.d_plane_slab0, stride, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n,
                            (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update,
-                           sp.htf, sp.vol, sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
+                           sp.htf, sp.vol, sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr,
+                           sp.fft.active ? sp.fft.d_plane_fft : nullptr, sp.fft.active ? sp.fft.dfft : nullptr);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }

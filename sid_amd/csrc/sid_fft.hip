@@ -1,0 +1,800 @@
+// FFT path of the light-field projector for wide (out-of-focus) planes.
+//
+// The reference evaluates every (aa, bb, cc) term with a full N x N FFT
+// convolution (external/LFrecon_util/conv2FFT_.m:8-15, N from
+// forwardProjectGPU.m:9-13).  Here the pad size is chosen as N = n*N' with N'
+// a 2/3/5-smooth length, N >= X + (kernel reach), so that the N-point DFT of a
+// plane masked to the sublattice (a, b) is a phase times the N'-point DFT of
+// the lenslet sub-image V_ab(Q) = V(a + nQ):
+//     DFT_N(mask_ab V)(k) = e^{-2 pi i (kx a + ky b)/N} * DFT_N'(V_ab)(k mod N').
+// With k = k0 + N' j (j = (jx, jy) in [0, n)^2) the forward projection is,
+// for every base frequency k0, a complex GEMV over q = (plane, ab):
+//     Ohat[k0][j] = sum_q G[k0][q][j] * Vt[k0][q],
+//     G[k0][(c,ab)][j] = DFT_N(kernel H_abc placed at (i - off + a, j' - off + b) mod N)(k0 + N' j),
+// followed by one inverse N x N transform.  The backward (adjoint) is
+//     Bt[k0][q] = sum_j conj(G[k0][q][j]) * What(k0 + N' j),  B_ab = IDFT_N'(Bt) / n^2.
+// Only k0y in [0, N'/2] is stored (real data: Hermitian symmetry gives the
+// rest).  G is precomputed per handle and streamed from HBM every projection
+// in two layouts (q-major for the forward, j-major for the backward) so both
+// GEMVs read coalesced rows: this path is HBM-bound.
+//
+// The FFTs are hand-written Stockham autosort transforms in LDS with
+// radix-4/2/3/5 butterflies; twiddles come from tables computed on the host
+// in double precision.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "sid_internal.h"
+
+namespace sid {
+
+// ---------------------------------------------------------------------------
+// 1-D complex FFT, batched: block b transforms sequences b*spb .. b*spb+spb-1,
+// each of length L (<= 2048), element t of sequence s at
+// data[s * seq_stride + t * elem_stride].  dir = -1 forward, +1 inverse
+// (unnormalised).  tw[t] = exp(dir * 2 pi i t / L) for t in [0, L).
+// ---------------------------------------------------------------------------
+struct FftDesc {
+    int L, nst, spb;
+    int rad[12];
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+template <int R>
+__device__ __forceinline__ void small_dft(float2 *v, float sgn)
+{
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = make_float2(a.x + b.x, a.y + b.y);
+        v[1] = make_float2(a.x - b.x, a.y - b.y);
+    } else if constexpr (R == 4) {
+        const float2 a = v[0], b = v[1], c = v[2], d = v[3];
+        const float2 s0 = make_float2(a.x + c.x, a.y + c.y), s1 = make_float2(a.x - c.x, a.y - c.y);
+        const float2 s2 = make_float2(b.x + d.x, b.y + d.y), s3 = make_float2(b.x - d.x, b.y - d.y);
+        // multiply s3 by (sgn * i)
+        const float2 s3i = make_float2(-sgn * s3.y, sgn * s3.x);
+        v[0] = make_float2(s0.x + s2.x, s0.y + s2.y);
+        v[2] = make_float2(s0.x - s2.x, s0.y - s2.y);
+        v[1] = make_float2(s1.x + s3i.x, s1.y + s3i.y);
+        v[3] = make_float2(s1.x - s3i.x, s1.y - s3i.y);
+    } else if constexpr (R == 3) {
+        // w = exp(sgn 2 pi i / 3) = -1/2 + sgn i sqrt(3)/2
+        const float c1 = -0.5f, s1 = sgn * 0.86602540378443865f;
+        const float2 a = v[0], b = v[1], c = v[2];
+        const float2 t = make_float2(b.x + c.x, b.y + c.y), u = make_float2(b.x - c.x, b.y - c.y);
+        v[0] = make_float2(a.x + t.x, a.y + t.y);
+        const float2 m = make_float2(a.x + c1 * t.x, a.y + c1 * t.y);
+        // v1 = m + i s1 u, v2 = m - i s1 u
+        v[1] = make_float2(m.x - s1 * u.y, m.y + s1 * u.x);
+        v[2] = make_float2(m.x + s1 * u.y, m.y - s1 * u.x);
+    } else {
+        // R == 5: w^e = cos(2 pi e / 5) + sgn i sin(2 pi e / 5)
+        const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+        const float s1 = sgn * 0.95105651629515357f, s2 = sgn * 0.58778525229247313f;
+        const float2 a = v[0];
+        const float2 t1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y), u1 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y);
+        const float2 t2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y), u2 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
+        v[0] = make_float2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+        const float2 m1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+        const float2 m2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+        // X1 = m1 + i (s1 u1 + s2 u2), X4 = m1 - i (...); X2 = m2 + i (s2 u1 - s1 u2), X3 = m2 - i (...)
+        const float2 n1 = make_float2(s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y);
+        const float2 n2 = make_float2(s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y);
+        v[1] = make_float2(m1.x - n1.y, m1.y + n1.x);
+        v[4] = make_float2(m1.x + n1.y, m1.y - n1.x);
+        v[2] = make_float2(m2.x - n2.y, m2.y + n2.x);
+        v[3] = make_float2(m2.x + n2.y, m2.y - n2.x);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void stockham_stage(const float2 *__restrict__ in, float2 *__restrict__ out, int L, int Ns,
+                                               const float2 *__restrict__ tw, float sgn, int spb, int tid, int nth)
+{
+    const int nb = L / R;
+    for (int e = tid; e < spb * nb; e += nth) {
+        const int s = e / nb, j = e - s * nb;
+        const float2 *src = in + s * L;
+        float2 *dst = out + s * L;
+        const int k = j % Ns;
+        const int tstep = (L / (Ns * R)) * k;   // twiddle index step: angle = 2 pi k r / (Ns R)
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float2 x = src[j + r * nb];
+            v[r] = r == 0 ? x : cmul(x, tw[(tstep * r) % L]);
+        }
+        small_dft<R>(v, sgn);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+    }
+}
+
+// Sequence s starts at (s / inner) * outer_stride + (s % inner) * seq_stride.
+__device__ __forceinline__ int64_t seq_off(int64_t s, int64_t inner, int64_t outer_stride, int64_t seq_stride)
+{
+    return (s / inner) * outer_stride + (s % inner) * seq_stride;
+}
+
+__global__ void k_fft1d(float2 *__restrict__ data, FftDesc d, int64_t s_begin, int64_t nseq, int64_t inner,
+                        int64_t outer_stride, int64_t seq_stride, int64_t elem_stride, const float2 *__restrict__ tw,
+                        float sgn, float scale)
+{
+    extern __shared__ float2 sm[];
+    float2 *b0 = sm, *b1 = sm + d.spb * d.L;
+    const int64_t s0 = s_begin + (int64_t)blockIdx.x * d.spb;
+    const int spb = (int)min((int64_t)d.spb, nseq - s0);
+    if (spb <= 0) return;
+    for (int e = threadIdx.x; e < spb * d.L; e += blockDim.x) {
+        const int s = e / d.L, t = e - s * d.L;
+        b0[e] = data[seq_off(s0 + s, inner, outer_stride, seq_stride) + (int64_t)t * elem_stride];
+    }
+    __syncthreads();
+    int Ns = 1;
+    float2 *in = b0, *out = b1;
+    for (int st = 0; st < d.nst; ++st) {
+        const int R = d.rad[st];
+        if (R == 4) stockham_stage<4>(in, out, d.L, Ns, tw, sgn, spb, threadIdx.x, blockDim.x);
+        else if (R == 2) stockham_stage<2>(in, out, d.L, Ns, tw, sgn, spb, threadIdx.x, blockDim.x);
+        else if (R == 3) stockham_stage<3>(in, out, d.L, Ns, tw, sgn, spb, threadIdx.x, blockDim.x);
+        else stockham_stage<5>(in, out, d.L, Ns, tw, sgn, spb, threadIdx.x, blockDim.x);
+        __syncthreads();
+        Ns *= R;
+        float2 *t = in; in = out; out = t;
+    }
+    for (int e = threadIdx.x; e < spb * d.L; e += blockDim.x) {
+        const int s = e / d.L, t = e - s * d.L;
+        const float2 v = in[e];
+        data[seq_off(s0 + s, inner, outer_stride, seq_stride) + (int64_t)t * elem_stride] =
+            make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+bool fft_factor(int L, std::vector<int> &rad)
+{
+    rad.clear();
+    int m = L;
+    while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+    while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+    while (m % 3 == 0) { rad.push_back(3); m /= 3; }
+    while (m % 5 == 0) { rad.push_back(5); m /= 5; }
+    return m == 1 && rad.size() <= 12;
+}
+
+bool is_smooth235(int L)
+{
+    std::vector<int> r;
+    return L >= 1 && fft_factor(L, r);
+}
+
+int next_smooth235(int L)
+{
+    while (!is_smooth235(L)) ++L;
+    return L;
+}
+
+int fft_plan_make(FftPlan1D &pl, int L)
+{
+    std::vector<int> rad;
+    if (L < 1 || L > 2048 || !fft_factor(L, rad)) {
+        set_error("FFT length %d is not a 2/3/5-smooth length <= 2048", L);
+        return SID_ERR_DIMS;
+    }
+    pl.L = L;
+    pl.nst = (int)rad.size();
+    for (int i = 0; i < pl.nst; ++i) pl.rad[i] = rad[i];
+    std::vector<float2> tf(L), ti(L);
+    for (int t = 0; t < L; ++t) {
+        const double a = 2.0 * M_PI * (double)t / (double)L;
+        tf[t] = make_float2((float)std::cos(a), (float)-std::sin(a));
+        ti[t] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    SID_TRY_HIP(hipMalloc(&pl.tw_fwd, sizeof(float2) * L));
+    SID_TRY_HIP(hipMalloc(&pl.tw_inv, sizeof(float2) * L));
+    SID_TRY_HIP(hipMemcpy(pl.tw_fwd, tf.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+    SID_TRY_HIP(hipMemcpy(pl.tw_inv, ti.data(), sizeof(float2) * L, hipMemcpyHostToDevice));
+    return SID_OK;
+}
+
+void fft_plan_free(FftPlan1D &pl)
+{
+    if (pl.tw_fwd) (void)hipFree(pl.tw_fwd);
+    if (pl.tw_inv) (void)hipFree(pl.tw_inv);
+    pl.tw_fwd = pl.tw_inv = nullptr;
+    pl.L = 0;
+}
+
+// Batched 1-D transforms; dir -1 forward, +1 inverse (scaled by `scale`).
+// Sequence s (0 <= s < nseq) starts at (s / inner) * outer_stride + (s % inner) * seq_stride.
+int fft_run(const FftPlan1D &pl, float2 *data, int64_t nseq, int64_t inner, int64_t outer_stride,
+            int64_t seq_stride, int64_t elem_stride, int dir, float scale, hipStream_t st)
+{
+    if (nseq <= 0) return SID_OK;
+    FftDesc d;
+    d.L = pl.L;
+    d.nst = pl.nst;
+    for (int i = 0; i < 12; ++i) d.rad[i] = i < pl.nst ? pl.rad[i] : 1;
+    d.spb = std::max(1, std::min(64, 1024 / pl.L));
+    const int64_t nblk = (nseq + d.spb - 1) / d.spb;
+    const size_t lds = sizeof(float2) * 2 * d.spb * pl.L;
+    for (int64_t b0 = 0; b0 < nblk; b0 += 65535) {
+        const int64_t nb = std::min<int64_t>(65535, nblk - b0);
+        hipLaunchKernelGGL(k_fft1d, dim3((unsigned)nb), dim3(256), lds, st, data, d, b0 * d.spb, nseq,
+                           std::max<int64_t>(inner, 1), outer_stride, seq_stride, elem_stride,
+                           dir < 0 ? pl.tw_fwd : pl.tw_inv, dir < 0 ? -1.0f : 1.0f, scale);
+        SID_TRY_HIP(hipGetLastError());
+    }
+    return SID_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FFT projector kernels
+// ---------------------------------------------------------------------------
+
+// Lenslet sub-images of the FFT planes: subs[q][Qx][Qy], q = slot * n2 + ab.
+__global__ void k_fft_sub_gather(const float *__restrict__ vol, const int *__restrict__ slot_plane, int nq, int n2,
+                                 int Nxp, int Nyp, int Lx, int Ly, int64_t XP, int64_t YP, int T, int S, int nz_in,
+                                 float2 *__restrict__ subs)
+{
+    const int64_t total = (int64_t)nq * Nxp * Nyp;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int Qy = (int)(e % Nyp);
+        const int64_t t = e / Nyp;
+        const int Qx = (int)(t % Nxp), q = (int)(t / Nxp);
+        const int c = slot_plane[q / n2], ab = q % n2;
+        float v = 0.0f;
+        if (Qx < Lx && Qy < Ly && c < nz_in) v = vol[(((int64_t)c * XP + Qx + T) * YP + (Qy + T)) * S + ab];
+        subs[e] = make_float2(v, 0.0f);
+    }
+}
+
+// vt[k0][q] = subs[q][k0x][k0y], k0 = k0x * Hy + k0y.
+__global__ void k_fft_pack_vt(const float2 *__restrict__ subs, int nq, int Nxp, int Nyp, int Hy,
+                              float2 *__restrict__ vt)
+{
+    const int64_t total = (int64_t)Nxp * Hy * nq;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int q = (int)(e % nq);
+        const int k0 = (int)(e / nq);
+        const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
+        vt[e] = subs[((int64_t)q * Nxp + k0x) * Nyp + k0y];
+    }
+}
+
+// Forward GEMV: oh[k0][j] = sum_q G[k0][q][j] * vt[k0][q]; one block per k0,
+// thread j.  vt[k0][q] is the same for every lane (a broadcast read through
+// the cache), so the block needs no LDS and many blocks share a CU to keep
+// enough HBM reads in flight.
+__global__ __launch_bounds__(256) void k_fft_gemv_fwd(const float2 *__restrict__ G, const float2 *__restrict__ vt,
+                                                      int nq, int n2, float2 *__restrict__ oh)
+{
+    const int k0 = blockIdx.x;
+    const int j = threadIdx.x;
+    if (j >= n2) return;
+    const float2 *g = G + (int64_t)k0 * nq * n2 + j;
+    const float2 *v = vt + (int64_t)k0 * nq;
+    float2 acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = make_float2(0.f, 0.f);
+    int q = 0;
+    for (; q + 8 <= nq; q += 8) {
+        float2 gg[8], vv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) gg[u] = g[(int64_t)(q + u) * n2];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) vv[u] = v[q + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc[u].x += gg[u].x * vv[u].x - gg[u].y * vv[u].y;
+            acc[u].y += gg[u].x * vv[u].y + gg[u].y * vv[u].x;
+        }
+    }
+    for (; q < nq; ++q) {
+        const float2 g0 = g[(int64_t)q * n2], v0 = v[q];
+        acc[0].x += g0.x * v0.x - g0.y * v0.y;
+        acc[0].y += g0.x * v0.y + g0.y * v0.x;
+    }
+    float2 r = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) { r.x += acc[u].x; r.y += acc[u].y; }
+    oh[(int64_t)k0 * n2 + j] = r;
+}
+
+// Full N x N spectrum of the forward output from oh (stored for k0y < Hy) by
+// Hermitian symmetry; j = jx + n * jy.
+__global__ void k_fft_assemble(const float2 *__restrict__ oh, int Nx, int Ny, int Nxp, int Nyp, int Hy, int n,
+                               float2 *__restrict__ grid)
+{
+    const int64_t total = (int64_t)Nx * Ny;
+    const int n2 = n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(e / Ny), ky = (int)(e % Ny);
+        int ax = kx, ay = ky;
+        bool cj = false;
+        if (ky % Nyp >= Hy) {
+            ax = (Nx - kx) % Nx;
+            ay = (Ny - ky) % Ny;
+            cj = true;
+        }
+        const int k0 = (ax % Nxp) * Hy + (ay % Nyp);
+        const int j = ax / Nxp + n * (ay / Nyp);
+        float2 v = oh[(int64_t)k0 * n2 + j];
+        if (cj) v.y = -v.y;
+        grid[e] = v;
+    }
+}
+
+// Sensor image (padded lenslet layout) -> complex N x N grid (zero padded).
+__global__ void k_fft_img_grid(const float *__restrict__ img, int Nx, int Ny, int Xn, int Yn, int n, int64_t YP,
+                               int T, int S, float2 *__restrict__ grid)
+{
+    const int64_t total = (int64_t)Nx * Ny;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e / Ny), y = (int)(e % Ny);
+        float v = 0.0f;
+        if (x < Xn && y < Yn)
+            v = img[((int64_t)(x / n + T) * YP + (y / n + T)) * S + (x % n) + n * (y % n)];
+        grid[e] = make_float2(v, 0.0f);
+    }
+}
+
+// Backward GEMV: bt[k0][q] = sum_j conj(G[k0][q][j]) * W(k0 + N' j).  One
+// block per k0.  A wave works on 8 rows q per iteration, 32 lanes per row
+// (lane jg covers j = jg + 32 t), so each load instruction reads two
+// contiguous 256-byte row segments and a row needs a 32-lane (5-step) shuffle
+// reduction; results for consecutive q are stored contiguously
+// (k_fft_unpack_bt moves them into the per-q sub-spectra).
+__global__ __launch_bounds__(256) void k_fft_gemv_bwd(const float2 *__restrict__ G, const float2 *__restrict__ grid,
+                                                      int nq, int n, int Nx, int Ny, int Nxp, int Nyp, int Hy,
+                                                      float2 *__restrict__ bt)
+{
+    constexpr int JT = 8;                  // j values per lane: covers n*n <= 256
+    __shared__ float2 w[32 * JT];
+    const int n2 = n * n;
+    const int k0 = blockIdx.x;
+    const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
+    for (int j = threadIdx.x; j < 32 * JT; j += blockDim.x) {
+        float2 v = make_float2(0.f, 0.f);
+        if (j < n2) {
+            const int jx = j % n, jy = j / n;
+            v = grid[(int64_t)(k0x + Nxp * jx) * Ny + (k0y + Nyp * jy)];
+        }
+        w[j] = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jg = lane & 31, qq = lane >> 5;
+    float2 wl[JT];
+#pragma unroll
+    for (int t = 0; t < JT; ++t) wl[t] = w[jg + 32 * t];
+    const float2 *gk = G + (int64_t)k0 * nq * n2;
+    float2 *out = bt + (int64_t)k0 * nq;
+    for (int q0 = 8 * wave; q0 < nq; q0 += 32) {
+        float re[4] = {0.f, 0.f, 0.f, 0.f}, im[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + 2 * u + qq;
+            if (q < nq) {
+                const float2 *g = gk + (int64_t)q * n2;
+#pragma unroll
+                for (int t = 0; t < JT; ++t) {
+                    const int j = jg + 32 * t;
+                    if (j < n2) {
+                        const float2 a = g[j], b = wl[t];
+                        re[u] += a.x * b.x + a.y * b.y;     // conj(a) * b
+                        im[u] += a.x * b.y - a.y * b.x;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float r = re[u], i = im[u];
+#pragma unroll
+            for (int o = 16; o > 0; o >>= 1) {
+                r += __shfl_xor(r, o);
+                i += __shfl_xor(i, o);
+            }
+            const int q = q0 + 2 * u + qq;
+            if (jg == 0 && q < nq) out[q] = make_float2(r, i);
+        }
+    }
+}
+
+// bt[k0][q] -> subs[q][k0x][k0y] (k0y < Hy).
+__global__ void k_fft_unpack_bt(const float2 *__restrict__ bt, int nq, int Nxp, int Nyp, int Hy,
+                                float2 *__restrict__ subs)
+{
+    const int64_t total = (int64_t)Nxp * Hy * nq;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int q = (int)(e / ((int64_t)Nxp * Hy));
+        const int k0 = (int)(e - (int64_t)q * Nxp * Hy);
+        const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
+        subs[((int64_t)q * Nxp + k0x) * Nyp + k0y] = bt[(int64_t)k0 * nq + q];
+    }
+}
+
+// Fill ky in [Hy, Nyp) of every sub-spectrum by Hermitian symmetry.
+__global__ void k_fft_herm_expand(float2 *__restrict__ subs, int nq, int Nxp, int Nyp, int Hy)
+{
+    const int nhi = Nyp - Hy;
+    const int64_t total = (int64_t)nq * Nxp * nhi;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int ky = Hy + (int)(e % nhi);
+        const int64_t t = e / nhi;
+        const int kx = (int)(t % Nxp), q = (int)(t / Nxp);
+        const float2 v = subs[((int64_t)q * Nxp + (Nxp - kx) % Nxp) * Nyp + (Nyp - ky)];
+        subs[((int64_t)q * Nxp + kx) * Nyp + ky] = make_float2(v.x, -v.y);
+    }
+}
+
+// Real parts of the inverse sub-transforms -> backward result in the padded
+// volume layout (FFT planes only).
+__global__ void k_fft_sub_scatter(const float2 *__restrict__ subs, const int *__restrict__ slot_plane, int nq, int n2,
+                                  int Nxp, int Nyp, int Lx, int Ly, int64_t XP, int64_t YP, int T, int S,
+                                  float *__restrict__ dfft)
+{
+    const int64_t total = (int64_t)nq * Lx * Ly;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int Qy = (int)(e % Ly);
+        const int64_t t = e / Ly;
+        const int Qx = (int)(t % Lx), q = (int)(t / Lx);
+        const int c = slot_plane[q / n2], ab = q % n2;
+        dfft[(((int64_t)c * XP + Qx + T) * YP + (Qy + T)) * S + ab] = subs[((int64_t)q * Nxp + Qx) * Nyp + Qy].x;
+    }
+}
+
+// Spectra build: kernel grids for a batch of (slot, ab) pairs.
+struct SpecJob {
+    int plane, ab, q, lo, hi, off;
+};
+
+__global__ void k_spec_fill(const float *__restrict__ H, const SpecJob *__restrict__ jobs, int64_t M, int n, int Nx,
+                            int Ny, float2 *__restrict__ grids, int nb)
+{
+    const int64_t per = (int64_t)Nx * Ny;
+    const int64_t total = per * nb;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int b = (int)(e / per);
+        const int64_t r = e - (int64_t)b * per;
+        const int x = (int)(r / Ny), y = (int)(r % Ny);
+        const SpecJob jb = jobs[b];
+        const int a = jb.ab % n, bb = jb.ab / n;
+        int i = (x + jb.off - a) % Nx;
+        if (i < 0) i += Nx;
+        int jj = (y + jb.off - bb) % Ny;
+        if (jj < 0) jj += Ny;
+        float v = 0.0f;
+        if (i >= jb.lo && i <= jb.hi && jj >= jb.lo && jj <= jb.hi)
+            v = H[(int64_t)i + M * ((int64_t)jj + M * ((int64_t)a + n * ((int64_t)bb + (int64_t)n * jb.plane)))];
+        grids[e] = make_float2(v, 0.0f);
+    }
+}
+
+__global__ void k_spec_scatter(const float2 *__restrict__ grids, const SpecJob *__restrict__ jobs, int nb, int Nx,
+                               int Ny, int Nxp, int Nyp, int Hy, int n, int nq, float2 *__restrict__ G)
+{
+    const int n2 = n * n;
+    const int64_t nk0 = (int64_t)Nxp * Hy;
+    const int64_t total = (int64_t)nb * nk0 * n2;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e % n2);
+        const int64_t t = e / n2;
+        const int k0 = (int)(t % nk0), b = (int)(t / nk0);
+        const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
+        const int kx = k0x + Nxp * (j % n), ky = k0y + Nyp * (j / n);
+        G[((int64_t)k0 * nq + jobs[b].q) * n2 + j] = grids[(int64_t)b * Nx * Ny + (int64_t)kx * Ny + ky];
+    }
+}
+
+static int grid_n(int64_t total) { return (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536)); }
+
+static void gemv_mark(FftState &f, hipStream_t st, int kind, bool begin)
+{
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, st);
+    f.ev.push_back(e);
+    if (begin) f.ev_kind.push_back(kind);
+}
+
+// Sum the GEMV times recorded since the last collection (call after a sync).
+void fft_timing_collect(sid_psf *p, double *fwd_ms, double *bwd_ms)
+{
+    FftState &f = p->plan.fft;
+    *fwd_ms = *bwd_ms = 0.0;
+    for (size_t i = 0; i + 1 < f.ev.size() && i / 2 < f.ev_kind.size(); i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, f.ev[i], f.ev[i + 1]) == hipSuccess) (f.ev_kind[i / 2] ? *bwd_ms : *fwd_ms) += ms;
+    }
+    for (auto e : f.ev) (void)hipEventDestroy(e);
+    f.ev.clear();
+    f.ev_kind.clear();
+}
+
+void fft_state_free(FftState &f)
+{
+    for (auto e : f.ev) (void)hipEventDestroy(e);
+    f.ev.clear();
+    fft_plan_free(f.fx); fft_plan_free(f.fy); fft_plan_free(f.fX); fft_plan_free(f.fY);
+    for (void *q : {(void *)f.G, (void *)f.subs, (void *)f.vt, (void *)f.oh, (void *)f.grid, (void *)f.dfft,
+                    (void *)f.d_slot_plane, (void *)f.d_plane_fft})
+        if (q) (void)hipFree(q);
+    f = FftState();
+}
+
+// Planes that can use the FFT path: nonempty with an odd support (the adjoint
+// of the FFT forward is then the reference backward, off_b == off_f).
+bool fft_eligible(const sid_psf *p, int64_t c)
+{
+    const Plane &pl = p->planes[c];
+    return !pl.empty && pl.off_b == pl.off_f;
+}
+
+// Build the FFT geometry and spectra for the planes marked in `use`.
+int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    FftState &f = sp.fft;
+    fft_state_free(f);
+    const int n = (int)p->n, n2 = n * n;
+    f.slot_of.assign(p->nzH, -1);
+    int reach = 0;
+    for (int64_t c = 0; c < p->nzH; ++c) {
+        if (!use[c]) continue;
+        const Plane &pl = p->planes[c];
+        f.slot_of[c] = (int)f.planes.size();
+        f.planes.push_back((int)c);
+        reach = std::max(reach, std::max(pl.hi - pl.off_f, pl.off_f - pl.lo));
+    }
+    if (f.planes.empty()) return SID_OK;
+    const int Xn = (int)(sp.Lx * n), Yn = (int)(sp.Ly * n);
+    f.Nxp = next_smooth235((Xn + reach + n - 1) / n);
+    f.Nyp = next_smooth235((Yn + reach + n - 1) / n);
+    f.Nx = n * f.Nxp;
+    f.Ny = n * f.Nyp;
+    if (f.Nx > 2048 || f.Ny > 2048) {
+        set_error("FFT grid %d x %d exceeds the 2048-point transforms of this build", f.Nx, f.Ny);
+        fft_state_free(f);
+        return SID_ERR_DIMS;
+    }
+    f.Hy = f.Nyp / 2 + 1;
+    f.nk0 = f.Nxp * f.Hy;
+    f.n2 = n2;
+    f.nq = (int)f.planes.size() * n2;
+    if ((int64_t)f.nq * 8 > 150 * 1024) {
+        set_error("too many FFT planes for the LDS-resident GEMV vector (%d)", (int)f.planes.size());
+        fft_state_free(f);
+        return SID_ERR_DIMS;
+    }
+    SID_TRY(fft_plan_make(f.fx, f.Nxp));
+    SID_TRY(fft_plan_make(f.fy, f.Nyp));
+    SID_TRY(fft_plan_make(f.fX, f.Nx));
+    SID_TRY(fft_plan_make(f.fY, f.Ny));
+    const int64_t gsz = (int64_t)f.nk0 * f.nq * n2;
+    f.g_bytes = gsz * (int64_t)sizeof(float2);
+    SID_TRY_HIP(hipMalloc(&f.G, f.g_bytes));
+    SID_TRY_HIP(hipMalloc(&f.subs, sizeof(float2) * (int64_t)f.nq * f.Nxp * f.Nyp));
+    SID_TRY_HIP(hipMalloc(&f.vt, sizeof(float2) * (int64_t)f.nk0 * f.nq));
+    SID_TRY_HIP(hipMalloc(&f.oh, sizeof(float2) * (int64_t)f.nk0 * n2));
+    SID_TRY_HIP(hipMalloc(&f.grid, sizeof(float2) * (int64_t)f.Nx * f.Ny));
+    SID_TRY_HIP(hipMalloc(&f.dfft, sizeof(float) * sp.vol_elems));
+    SID_TRY_HIP(hipMemsetAsync(f.dfft, 0, sizeof(float) * sp.vol_elems, st));
+    std::vector<int> pf(p->nzH, 0);
+    for (int c : f.planes) pf[c] = 1;
+    SID_TRY_HIP(hipMalloc(&f.d_slot_plane, sizeof(int) * f.planes.size()));
+    SID_TRY_HIP(hipMalloc(&f.d_plane_fft, sizeof(int) * p->nzH));
+    SID_TRY_HIP(hipMemcpy(f.d_slot_plane, f.planes.data(), sizeof(int) * f.planes.size(), hipMemcpyHostToDevice));
+    SID_TRY_HIP(hipMemcpy(f.d_plane_fft, pf.data(), sizeof(int) * p->nzH, hipMemcpyHostToDevice));
+
+    // spectra, in batches of kernels
+    std::vector<SpecJob> jobs;
+    for (size_t sl = 0; sl < f.planes.size(); ++sl) {
+        const Plane &pl = p->planes[f.planes[sl]];
+        for (int ab = 0; ab < n2; ++ab)
+            jobs.push_back(SpecJob{f.planes[sl], ab, (int)sl * n2 + ab, pl.lo, pl.hi, pl.off_f});
+    }
+    const int64_t per = (int64_t)f.Nx * f.Ny;
+    const int bmax = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)(512ll << 20) / (per * 8)));
+    float2 *grids = nullptr;
+    SpecJob *d_jobs = nullptr;
+    SID_TRY_HIP(hipMalloc(&grids, sizeof(float2) * per * bmax));
+    SID_TRY_HIP(hipMalloc(&d_jobs, sizeof(SpecJob) * jobs.size()));
+    SID_TRY_HIP(hipMemcpy(d_jobs, jobs.data(), sizeof(SpecJob) * jobs.size(), hipMemcpyHostToDevice));
+    int rc = SID_OK;
+    for (size_t b0 = 0; b0 < jobs.size() && rc == SID_OK; b0 += bmax) {
+        const int nb = (int)std::min<size_t>(bmax, jobs.size() - b0);
+        hipLaunchKernelGGL(k_spec_fill, dim3(grid_n(per * nb)), dim3(256), 0, st, p->H, d_jobs + b0, p->M, n, f.Nx,
+                           f.Ny, grids, nb);
+        // 2-D forward DFT of each grid: along y (contiguous), then along x
+        rc = fft_run(f.fY, grids, (int64_t)nb * f.Nx, (int64_t)nb * f.Nx, 0, f.Ny, 1, -1, 1.0f, st);
+        if (rc == SID_OK) rc = fft_run(f.fX, grids, (int64_t)nb * f.Ny, f.Ny, per, 1, f.Ny, -1, 1.0f, st);
+        if (rc == SID_OK)
+            hipLaunchKernelGGL(k_spec_scatter, dim3(grid_n((int64_t)nb * f.nk0 * n2)), dim3(256), 0, st, grids,
+                               d_jobs + b0, nb, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, n, f.nq, f.G);
+    }
+    const hipError_t e1 = hipStreamSynchronize(st);
+    (void)hipFree(grids);
+    (void)hipFree(d_jobs);
+    if (rc != SID_OK) return rc;
+    if (e1 != hipSuccess) {
+        set_error("FFT spectra build failed: %s", hipGetErrorString(e1));
+        return SID_ERR_HIP;
+    }
+    f.active = true;
+    return SID_OK;
+}
+
+// Forward FFT planes: leaves the real N x N image in f.grid (.x).
+int fft_forward(sid_psf *p, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    FftState &f = sp.fft;
+    if (!f.active) return SID_OK;
+    const int n2 = f.n2;
+    hipLaunchKernelGGL(k_fft_sub_gather, dim3(grid_n((int64_t)f.nq * f.Nxp * f.Nyp)), dim3(256), 0, st, sp.vol,
+                       f.d_slot_plane, f.nq, n2, f.Nxp, f.Nyp, (int)sp.Lx, (int)sp.Ly, sp.XP, sp.YP, (int)p->T,
+                       (int)p->S, (int)sp.nz, f.subs);
+    SID_TRY_HIP(hipGetLastError());
+    const int64_t sz = (int64_t)f.Nxp * f.Nyp;
+    SID_TRY(fft_run(f.fy, f.subs, (int64_t)f.nq * f.Nxp, (int64_t)f.nq * f.Nxp, 0, f.Nyp, 1, -1, 1.0f, st));
+    SID_TRY(fft_run(f.fx, f.subs, (int64_t)f.nq * f.Hy, f.Hy, sz, 1, f.Nyp, -1, 1.0f, st));
+    hipLaunchKernelGGL(k_fft_pack_vt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.subs, f.nq, f.Nxp,
+                       f.Nyp, f.Hy, f.vt);
+    gemv_mark(f, st, 0, true);
+    hipLaunchKernelGGL(k_fft_gemv_fwd, dim3(f.nk0), dim3(256), 0, st, f.G, f.vt, f.nq, n2, f.oh);
+    gemv_mark(f, st, 0, false);
+    hipLaunchKernelGGL(k_fft_assemble, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, f.oh, f.Nx, f.Ny, f.Nxp,
+                       f.Nyp, f.Hy, (int)p->n, f.grid);
+    SID_TRY_HIP(hipGetLastError());
+    SID_TRY(fft_run(f.fY, f.grid, f.Nx, f.Nx, 0, f.Ny, 1, +1, 1.0f, st));
+    SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, +1, 1.0f / ((float)f.Nx * (float)f.Ny), st));
+    return SID_OK;
+}
+
+// Backward FFT planes from the image in sp.img: writes f.dfft (padded volume layout).
+int fft_backward(sid_psf *p, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    FftState &f = sp.fft;
+    if (!f.active) return SID_OK;
+    const int n = (int)p->n;
+    hipLaunchKernelGGL(k_fft_img_grid, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, sp.img, f.Nx, f.Ny,
+                       (int)(sp.Lx * n), (int)(sp.Ly * n), n, sp.YP, (int)p->T, (int)p->S, f.grid);
+    SID_TRY_HIP(hipGetLastError());
+    SID_TRY(fft_run(f.fY, f.grid, f.Nx, f.Nx, 0, f.Ny, 1, -1, 1.0f, st));
+    SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, -1, 1.0f, st));
+    gemv_mark(f, st, 1, true);
+    hipLaunchKernelGGL(k_fft_gemv_bwd, dim3(f.nk0), dim3(256), 0, st, f.G, f.grid, f.nq, n, f.Nx, f.Ny, f.Nxp, f.Nyp,
+                       f.Hy, f.vt);
+    gemv_mark(f, st, 1, false);
+    hipLaunchKernelGGL(k_fft_unpack_bt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.vt, f.nq, f.Nxp,
+                       f.Nyp, f.Hy, f.subs);
+    const int nhi = f.Nyp - f.Hy;
+    if (nhi > 0)
+        hipLaunchKernelGGL(k_fft_herm_expand, dim3(grid_n((int64_t)f.nq * f.Nxp * nhi)), dim3(256), 0, st, f.subs,
+                           f.nq, f.Nxp, f.Nyp, f.Hy);
+    SID_TRY_HIP(hipGetLastError());
+    const int64_t sz = (int64_t)f.Nxp * f.Nyp;
+    // inverse 2-D sub transforms: along x (stride Nyp), then along y; scale 1 / (n^2 Nx' Ny')
+    SID_TRY(fft_run(f.fx, f.subs, (int64_t)f.nq * f.Nyp, f.Nyp, sz, 1, f.Nyp, +1, 1.0f, st));
+    SID_TRY(fft_run(f.fy, f.subs, (int64_t)f.nq * f.Nxp, (int64_t)f.nq * f.Nxp, 0, f.Nyp, 1, +1,
+                    1.0f / ((float)f.n2 * (float)f.Nxp * (float)f.Nyp), st));
+    hipLaunchKernelGGL(k_fft_sub_scatter, dim3(grid_n((int64_t)f.nq * sp.Lx * sp.Ly)), dim3(256), 0, st, f.subs,
+                       f.d_slot_plane, f.nq, f.n2, f.Nxp, f.Nyp, (int)sp.Lx, (int)sp.Ly, sp.XP, sp.YP, (int)p->T,
+                       (int)p->S, f.dfft);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+// Per-plane direct-vs-FFT choice from measurements on this device: time the
+// forward with every plane direct and with every eligible plane on the FFT
+// path, model a plane's direct cost as its share of the MFMA work and its FFT
+// cost as its share of the FFT-path time (the spectra stream dominates and
+// grows with the number of FFT planes), and put a plane on the FFT path when
+// that is cheaper.  Cached per (X, Y, nz, precision).
+static double time_ms(hipStream_t st, int reps, const std::function<int()> &fn, int *rc)
+{
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { *rc = SID_ERR_HIP; return 0; }
+    *rc = fn();  // warm-up
+    if (*rc == SID_OK) (void)hipEventRecord(a, st);
+    for (int r = 0; r < reps && *rc == SID_OK; ++r) *rc = fn();
+    float ms = 0.f;
+    if (*rc == SID_OK) {
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        (void)hipEventElapsedTime(&ms, a, b);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+}
+
+int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st)
+{
+    if (p->method != SID_METHOD_AUTO || p->precision == SID_PREC_FP64) return SID_OK;
+    const int64_t key[4] = {X, Y, nz, p->precision};
+    if (std::equal(key, key + 4, p->auto_key) && p->auto_fft.size() == (size_t)p->nzH) return SID_OK;
+    std::vector<char> elig(p->nzH, 0);
+    int ne = 0;
+    for (int64_t c = 0; c < p->nzH; ++c) {
+        elig[c] = fft_eligible(p, c) ? 1 : 0;
+        ne += elig[c];
+    }
+    p->auto_fft.assign(p->nzH, 0);
+    std::copy(key, key + 4, p->auto_key);
+    if (ne == 0) return SID_OK;
+    int rc = SID_OK;
+    // 1. every plane direct
+    p->method = SID_METHOD_DIRECT;
+    rc = ensure_plan(p, X, Y, nz);
+    double t_dir = 0;
+    std::vector<double> macs(p->nzH, 0.0);
+    double macs_all = 0;
+    if (rc == SID_OK) {
+        t_dir = time_ms(st, 2, [&]() { return launch_gemm(p, 0, st); }, &rc);
+        for (const Seg &sg : p->plan.fwd.segs) {
+            const double m = (double)sg.K * p->NP * p->plan.mpad;
+            macs[sg.plane] += m;
+            macs_all += m;
+        }
+    }
+    // 2. every eligible plane on the FFT path
+    double t_fft = 0;
+    if (rc == SID_OK) {
+        p->method = SID_METHOD_FFT;
+        rc = ensure_plan(p, X, Y, nz);
+        if (rc == SID_OK) t_fft = time_ms(st, 2, [&]() { return fft_forward(p, st); }, &rc);
+    }
+    p->method = SID_METHOD_AUTO;
+    if (rc != SID_OK) {
+        p->auto_key[0] = -1;
+        return rc;
+    }
+    const double fft_per_plane = t_fft / ne;
+    int chosen = 0;
+    for (int64_t c = 0; c < p->nzH; ++c) {
+        const double direct_ms = macs_all > 0 ? t_dir * macs[c] / macs_all : 0.0;
+        p->auto_fft[c] = (elig[c] && fft_per_plane < direct_ms) ? 1 : 0;
+        chosen += p->auto_fft[c];
+    }
+    p->auto_ms[0] = t_dir;
+    p->auto_ms[1] = t_fft;
+    p->auto_ms[2] = ne;
+    p->auto_ms[3] = chosen;
+    return SID_OK;
+}
+
+int fft_test(int L, int64_t nseq, float2 *data, int dir, hipStream_t st)
+{
+    FftPlan1D pl;
+    SID_TRY(fft_plan_make(pl, L));
+    const int rc = fft_run(pl, data, nseq, nseq, 0, L, 1, dir, 1.0f, st);
+    const hipError_t e = hipStreamSynchronize(st);
+    fft_plan_free(pl);
+    if (rc != SID_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("FFT test failed: %s", hipGetErrorString(e));
+        return SID_ERR_HIP;
+    }
+    return SID_OK;
+}
+
+}  // namespace sid

@@ -223,7 +223,13 @@ int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4],
         set_error("the likelihood gradient runs in FP32 (set precision fp32)");
         return SID_ERR_STATE;
     }
-    SID_TRY(ensure_plan(p, X, Y, nz));
+    // the PSF gradient is the weight gradient of the direct GEMM: run every
+    // plane on the direct path for this call
+    const int saved_method = p->method;
+    p->method = SID_METHOD_DIRECT;
+    const int prc = ensure_plan(p, X, Y, nz);
+    p->method = saved_method;
+    SID_TRY(prc);
     SensorPlan &sp = p->plan;
     GradBuffers gb;
     const int nblk = grid_of(X * Y);

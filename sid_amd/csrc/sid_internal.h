@@ -85,6 +85,32 @@ struct GemmPlan {             // one direction on one sensor geometry
     int64_t macs = 0;                 // MFMA multiply-adds issued (padding included)
 };
 
+struct FftPlan1D {
+    int L = 0, nst = 0;
+    int rad[12] = {};
+    float2 *tw_fwd = nullptr, *tw_inv = nullptr;
+};
+
+// FFT path state for one sensor geometry (sid_fft.hip).
+struct FftState {
+    bool active = false;
+    std::vector<int> planes;       // FFT planes, in slot order
+    std::vector<int> slot_of;      // per plane of H: slot or -1
+    int Nxp = 0, Nyp = 0, Nx = 0, Ny = 0, Hy = 0, nk0 = 0, n2 = 0, nq = 0;
+    FftPlan1D fx, fy, fX, fY;      // lengths Nx', Ny', Nx, Ny
+    float2 *G = nullptr;           // spectra [nk0][nq][n2]
+    float2 *subs = nullptr;        // sub-image transforms [nq][Nx'][Ny']
+    float2 *vt = nullptr;          // [nk0][nq]
+    float2 *oh = nullptr;          // [nk0][n2]
+    float2 *grid = nullptr;        // [Nx][Ny]
+    float *dfft = nullptr;         // backward result of the FFT planes, padded volume layout
+    int *d_slot_plane = nullptr;
+    int *d_plane_fft = nullptr;    // per plane of H: 1 when on the FFT path
+    int64_t g_bytes = 0;
+    std::vector<hipEvent_t> ev;     // event pairs bracketing the GEMVs of the last call
+    std::vector<int> ev_kind;        // 0 forward GEMV, 1 backward GEMV (one entry per pair)
+};
+
 struct SensorPlan {
     int64_t X = 0, Y = 0, nz = 0;     // nz of the forward input
     int64_t Lx = 0, Ly = 0, XP = 0, YP = 0;
@@ -96,6 +122,8 @@ struct SensorPlan {
     double *vol_d = nullptr, *img_d = nullptr, *htf_d = nullptr, *part_d = nullptr;  // FP64 mode
     int64_t vol_elems = 0, img_elems = 0, part_elems = 0;
     int prec_built = -1;
+    FftState fft;
+    std::vector<char> fft_use;       // per plane of H: on the FFT path in this plan
 };
 
 }  // namespace sid
@@ -116,10 +144,13 @@ struct sid_psf {
     int64_t wf_elems = 0, wb_elems = 0;
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
+    std::vector<char> auto_fft;       // autotuned per-plane FFT choice (SID_METHOD_AUTO)
+    int64_t auto_key[4] = {-1, -1, -1, -1};  // X, Y, nz, precision the choice was measured for
+    double auto_ms[4] = {0, 0, 0, 0};        // direct ms, fft ms (all eligible planes), n eligible, n chosen
     sid::SensorPlan plan;
     void *lib = nullptr;              // cached library plan (sid_library.hip)
     hipEvent_t ev[10] = {};
-    double timing[8] = {0};
+    double timing[12] = {0};
 };
 
 namespace sid {
@@ -136,6 +167,20 @@ int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st);
 void free_plan_buffers(SensorPlan &sp);
 int support_scan(sid_psf *p);
 void free_weights(sid_psf *p);
+// sid_fft.hip
+int fft_plan_make(FftPlan1D &pl, int L);
+void fft_plan_free(FftPlan1D &pl);
+int fft_run(const FftPlan1D &pl, float2 *data, int64_t nseq, int64_t inner, int64_t outer_stride, int64_t seq_stride,
+            int64_t elem_stride, int dir, float scale, hipStream_t st);
+void fft_state_free(FftState &f);
+bool fft_eligible(const sid_psf *p, int64_t c);
+int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st);
+int fft_forward(sid_psf *p, hipStream_t st);
+int fft_backward(sid_psf *p, hipStream_t st);
+void fft_timing_collect(sid_psf *p, double *fwd_ms, double *bwd_ms);
+int next_smooth235(int L);
+int fft_test(int L, int64_t nseq, float2 *data, int dir, hipStream_t st);
+int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st);
 // sid_library.hip
 void patched_rect(int64_t n, int64_t M, int64_t X, int64_t Y, const int64_t box[6], int64_t rect[4]);
 int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *vals, const int64_t *val_off, int64_t X,

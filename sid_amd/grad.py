@@ -190,6 +190,7 @@ def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf:
         V.copy_(Vc)
         H.copy_(Hc)
     psf.update(H)
+    psf.set_method("direct")   # the PSF gradient needs the direct projector; keep one plan
     nb, nz, Y, X = V.shape
     ones = torch.ones((nb, Y, X), dtype=torch.float32, device=V.device)
     B1, _ = project_vjp(psf, None, ones[:1], True, False)
