@@ -177,6 +177,26 @@ int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *box
 /* {rows, work items, MFMA multiply-adds, accumulator elements} of the cached library plan. */
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
 
+/* --- LS_deconv (external/LFrecon_util/LS_deconv.m:1-97) ------------------
+ * Projected gradient descent with exact line search for
+ *   argmin_{x >= 0} ||y - A x||^2 + TV_L2 + lamb_L2 ||x||^2 + lamb_L1 ||x||_1,
+ * started at Xguess = df (= A'y), with Q(x) = A'A x + tv(x) + lamb_L2 x where
+ * tv is the 7-point 'same' stencil of LS_deconv.m:54-58 (on when
+ * max(lamb_TV_L2) != 0).  The step is d'd / ||A d||^2, or d'd / d'Q(d) when
+ * lamb_TV_L1 * lamb_TV_L2 * lamb_L2 is nonzero in all three entries (:80-84).
+ * Dot products are FP64.  opts == NULL means the reference defaults
+ * (max_iter 8, all multipliers 0).  X receives the result; vdims = {X, Y, Nz}. */
+typedef struct sid_ls_opts {
+    int max_iter;          /* opts.max_iter */
+    double lamb_TV_L1;     /* opts.lamb_TV_L1 (only selects the step formula) */
+    double lamb_TV_L2[3];  /* opts.lamb_TV_L2, per dimension of the volume */
+    double lamb_L1;        /* opts.lamb_L1 */
+    double lamb_L2;        /* opts.lamb_L2 */
+} sid_ls_opts;
+int sid_ls_deconv(sid_psf *psf, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts *opts);
+int sid_ls_deconv_device(sid_psf *psf, const float *df_dev, float *X_dev, const int64_t vdims[3],
+                         const sid_ls_opts *opts, void *stream);
+
 /* Device time (ms, HIP events) of the last call's projector kernels:
  * [0] forward direct GEMM, [1] forward reduce, [2] backward direct GEMM,
  * [3] backward reduce + update, [4] whole call, [5] forward FFT path,

@@ -483,19 +483,60 @@ def poisson_nll_grad_psf(H, V, y, CAindex=None):
     return G
 
 
-def ls_deconv(forward, backward, df, max_iter=8, lamb_L1=0.0):
-    """external/LFrecon_util/LS_deconv.m:1-97 without TV/L2 terms:
-    projected gradient descent with exact line search for
-    argmin_{x>=0} ||y - A x||^2 + lamb_L1 ||x||_1, started at df = A'y."""
-    Q = lambda X: backward(forward(X))
+def tv_l2_stencil(X, lamb_TV_L2):
+    """convn(X, filter, 'same') with the 3x3x3 filter of
+    external/LFrecon_util/LS_deconv.m:54-58: lamb(1) on the two neighbours
+    along dimension 1, lamb(2) along dimension 2, lamb(3) along dimension 3,
+    -sum(lamb) at the centre; zeros outside the volume."""
+    X = np.asarray(X, dtype=np.float64)
+    l = [float(v) for v in lamb_TV_L2]
+    out = -sum(l) * X
+    for ax in range(3):
+        sh = np.zeros_like(X)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        sh[tuple(lo)] += X[tuple(hi)]
+        sh[tuple(hi)] += X[tuple(lo)]
+        out += l[ax] * sh
+    return out
+
+
+def ls_deconv(forward, backward, df, max_iter=8, lamb_L1=0.0, lamb_L2=0.0, lamb_TV_L2=(0.0, 0.0, 0.0),
+              lamb_TV_L1=0.0):
+    """external/LFrecon_util/LS_deconv.m:1-97: projected gradient descent with
+    exact line search for
+        argmin_{x>=0} ||y - A x||^2 + TV_L2(x) + lamb_L2 ||x||^2 + lamb_L1 ||x||_1,
+    started at df = A'y.  Q(x) = A'A x + tv(x) + lamb_L2 x (:67-73); the step
+    is d'd / ||A d||^2 unless lamb_TV_L1 * lamb_TV_L2 * lamb_L2 is nonzero in
+    every entry, then d'd / d'Q(d) (:80-84)."""
+    tvl = np.asarray(lamb_TV_L2, dtype=np.float64).reshape(-1)
+    if tvl.size != 3:
+        raise ValueError("lamb_TV_L2 needs one multiplier per dimension (LS_deconv.m:54-57 index 1..3)")
+    tv_on = float(tvl.max()) != 0.0
+    alt = bool(np.all(lamb_TV_L1 * tvl * lamb_L2 != 0))
+
+    def Q(X):
+        r = backward(forward(X))
+        if tv_on:
+            r = r + tv_l2_stencil(X, tvl)
+        if lamb_L2:
+            r = r + lamb_L2 * X
+        return r
+
     X = np.array(df, dtype=np.float64, copy=True)
     g = Q(df) - df + lamb_L1
     with np.errstate(divide="ignore", invalid="ignore"):
         for _ in range(int(max_iter)):
             passive = np.maximum(X > 0, g < 0)
             d = passive * g
-            v = forward(d)
-            alpha = float(np.sum(d * d)) / float(np.sum(v * v))
+            if alt:
+                den = float(np.sum(d * Q(d)))
+            else:
+                v = forward(d)
+                den = float(np.sum(v * v))
+            alpha = float(np.sum(d * d)) / den
             if not np.isfinite(alpha):
                 alpha = 0.0
             Xn = X - alpha * d

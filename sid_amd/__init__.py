@@ -11,6 +11,8 @@ every call raises.
     backwardProjectACC_new(H, proj, CAindex)   external/LFrecon_util/backwardProjectACC_new.m:1
     deconvRL(forwardFUN, backwardFUN, Htf, maxIter, Xguess, opts)
                                                external/LFrecon_util/deconvRL.m:1
+    LS_deconv(forwardFUN, backwardFUN, df, opts)
+                                               external/LFrecon_util/LS_deconv.m:1
 
 ``H`` may be a numpy array indexed like MATLAB (shape (M, M, Nnum, Nnum, Nz)),
 a device tensor of shape (Nz, Nnum, Nnum, M, M) (the same column-major
@@ -29,7 +31,7 @@ from . import _lib
 from ._lib import PREC, METHOD, SidError, check, i64, lib
 
 __all__ = ["PSF", "forwardProjectGPU", "backwardProjectGPU_new", "forwardProjectACC",
-           "backwardProjectACC_new", "deconvRL", "SidError", "device_count", "selftest"]
+           "backwardProjectACC_new", "deconvRL", "LS_deconv", "SidError", "device_count", "selftest"]
 
 
 def device_count() -> int:
@@ -208,6 +210,25 @@ class PSF:
             return X, hist
         return X
 
+    def ls_deconv(self, df, opts=None):
+        """LS_deconv (projected gradient descent, exact line search) with this
+        PSF's projectors, all on device.  ``opts`` keys as in LS_deconv.m:25-43."""
+        o = _ls_opts(opts)
+        if _is_torch(df):
+            torch = _torch()
+            h = df.detach().to(torch.float32).contiguous()
+            X = torch.empty_like(h)
+            nz, Y, Xs = h.shape
+            check(lib().sid_ls_deconv_device(self._h, ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(X.data_ptr()),
+                                             i64(Xs, Y, nz), ctypes.byref(o), _stream_ptr()))
+            return X
+        h = np.asfortranarray(np.asarray(df), dtype=np.float32)
+        X = np.zeros(h.shape, dtype=np.float32, order="F")
+        Xs, Y, nz = h.shape
+        check(lib().sid_ls_deconv(self._h, h.ctypes.data_as(ctypes.c_void_p), X.ctypes.data_as(ctypes.c_void_p),
+                                  i64(Xs, Y, nz), ctypes.byref(o)))
+        return X
+
     def forward_fun(self):
         f = lambda V: self.forward(V)
         f._sid_psf = self
@@ -219,6 +240,23 @@ class PSF:
         f._sid_psf = self
         f._sid_dir = "backward"
         return f
+
+
+def _ls_opts(opts) -> "_lib.LsOpts":
+    """Defaults of LS_deconv.m:25-43; lamb_TV_L2 needs all three entries, as
+    the filter construction at LS_deconv.m:54-57 indexes (1)..(3)."""
+    opts = dict(opts or {})
+    o = _lib.LsOpts()
+    o.max_iter = int(opts.get("max_iter", 8))
+    o.lamb_TV_L1 = float(opts.get("lamb_TV_L1", 0.0))
+    tv = np.asarray(opts.get("lamb_TV_L2", (0.0, 0.0, 0.0)), dtype=np.float64).reshape(-1)
+    if tv.size != 3:
+        raise ValueError("lamb_TV_L2 needs one multiplier per volume dimension")
+    for i in range(3):
+        o.lamb_TV_L2[i] = float(tv[i])
+    o.lamb_L1 = float(opts.get("lamb_L1", 0.0))
+    o.lamb_L2 = float(opts.get("lamb_L2", 0.0))
+    return o
 
 
 def _as_psf(H, CAindex=None) -> PSF:
@@ -265,3 +303,18 @@ def deconvRL(forwardFUN, backwardFUN, Htf, maxIter, Xguess, opts=None):
     if psf_f is None or psf_f is not psf_b:
         raise TypeError("forwardFUN and backwardFUN must be PSF.forward_fun()/backward_fun() of the same PSF")
     return psf_f.deconv_rl(Htf, int(maxIter), Xguess, history=bool(opts.get("history", False)))
+
+
+def LS_deconv(forwardFUN, backwardFUN, df, opts=None):
+    """external/LFrecon_util/LS_deconv.m:1 — projected gradient descent with
+    exact line search for the regularised non-negative least-squares problem
+    argmin_{x>=0} ||y - A x||^2 + TV_L2 + lamb_L2 ||x||^2 + lamb_L1 ||x||_1,
+    started at ``df`` = A'(y).  ``forwardFUN``/``backwardFUN`` must be the
+    projector handles of one ``PSF``; ``opts`` keys: max_iter, lamb_TV_L1,
+    lamb_TV_L2 (3 entries), lamb_L1, lamb_L2.
+    """
+    psf_f = getattr(forwardFUN, "_sid_psf", None)
+    psf_b = getattr(backwardFUN, "_sid_psf", None)
+    if psf_f is None or psf_f is not psf_b:
+        raise TypeError("forwardFUN and backwardFUN must be PSF.forward_fun()/backward_fun() of the same PSF")
+    return psf_f.ls_deconv(df, opts)

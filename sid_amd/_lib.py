@@ -46,6 +46,8 @@ SIGNATURES = {
     "sid_library_forward_device": (_I, [_P, ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P, _I, _P]),
     "sid_library_stats": (_I, [_P, _I64P]),
     "sid_patched_rect": (_I, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]),
+    "sid_ls_deconv": (_I, [_P, _P, _P, _I64P, _P]),
+    "sid_ls_deconv_device": (_I, [_P, _P, _P, _I64P, _P, _P]),
     "sid_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "sid_plan_stats": (_I, [_P, _I64P]),
     "sid_selftest": (_I, []),
@@ -65,6 +67,11 @@ class PsfInfo(ctypes.Structure):
                 ("support_lo", ctypes.c_int64 * 512), ("support_hi", ctypes.c_int64 * 512),
                 ("plane_method", ctypes.c_int32 * 512), ("autotune_ms", ctypes.c_double * 4),
                 ("spectra_bytes", ctypes.c_int64)]
+
+
+class LsOpts(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("lamb_TV_L1", ctypes.c_double), ("lamb_TV_L2", ctypes.c_double * 3),
+                ("lamb_L1", ctypes.c_double), ("lamb_L2", ctypes.c_double)]
 
 
 _lib = None

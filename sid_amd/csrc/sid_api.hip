@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -604,6 +605,48 @@ int sid_library_stats(const sid_psf *p, int64_t st[4])
     return library_stats(p, st);
 }
 
+static int ls_opts(const sid_ls_opts *o, sid_ls_opts *out)
+{
+    sid_ls_opts d{};
+    d.max_iter = 8;
+    if (o) d = *o;
+    if (d.max_iter < 0) { set_error("max_iter must be >= 0"); return SID_ERR_ARG; }
+    const double l[6] = {d.lamb_TV_L1, d.lamb_TV_L2[0], d.lamb_TV_L2[1], d.lamb_TV_L2[2], d.lamb_L1, d.lamb_L2};
+    for (double v : l)
+        if (!std::isfinite(v)) { set_error("LS_deconv multipliers must be finite"); return SID_ERR_ARG; }
+    *out = d;
+    return SID_OK;
+}
+
+int sid_ls_deconv_device(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts *opts,
+                         void *stream)
+{
+    if (!p || !df || !X) { set_error("sid_ls_deconv_device: null argument"); return SID_ERR_ARG; }
+    sid_ls_opts o;
+    SID_TRY(ls_opts(opts, &o));
+    SID_TRY(check_vdims(p, vdims, true));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    return ls_deconv(p, df, X, vdims, o, pick_stream(p, stream));
+}
+
+int sid_ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts *opts)
+{
+    if (!p || !df || !X) { set_error("sid_ls_deconv: null argument"); return SID_ERR_ARG; }
+    sid_ls_opts o;
+    SID_TRY(ls_opts(opts, &o));
+    SID_TRY(check_vdims(p, vdims, true));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    const int64_t nv = vdims[0] * vdims[1] * vdims[2];
+    DevBuf dh, dx;
+    SID_TRY_HIP(hipMalloc(&dh.ptr, nv * sizeof(float)));
+    SID_TRY_HIP(hipMalloc(&dx.ptr, nv * sizeof(float)));
+    SID_TRY_HIP(hipMemcpyAsync(dh.ptr, df, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    SID_TRY(ls_deconv(p, (const float *)dh.ptr, (float *)dx.ptr, vdims, o, p->stream));
+    SID_TRY_HIP(hipMemcpyAsync(X, dx.ptr, nv * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return SID_OK;
+}
+
 int sid_last_timing(const sid_psf *p, double ms[12])
 {
     if (!p || !ms) { set_error("null argument"); return SID_ERR_ARG; }
@@ -636,3 +679,14 @@ int sid_selftest(void)
 }
 
 }  // extern "C"
+
+// Projector passes on column-major device buffers for the solvers (checks done by the caller).
+int sid::forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st)
+{
+    return forward_impl(p, V, vdims, O, st);
+}
+
+int sid::backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st)
+{
+    return backward_impl(p, O, odims, B, st);
+}

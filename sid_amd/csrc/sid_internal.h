@@ -192,3 +192,11 @@ int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4],
               float *gH, hipStream_t st);
 int run_selftest();
 }  // namespace sid
+
+namespace sid {
+// sid_api.hip: projector passes on column-major device buffers (used by the solvers)
+int forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st);
+int backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st);
+// sid_solvers.hip
+int ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts &o, hipStream_t st);
+}  // namespace sid

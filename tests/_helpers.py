@@ -67,8 +67,12 @@ def c_isra(H, Htf, X0, iters, CA=None, threads=8):
 
 
 def rel_l2(a, ref):
-    a = np.asarray(a, dtype=np.float64)
-    ref = np.asarray(ref, dtype=np.float64)
+    """||a - ref|| / ||ref|| (real or complex)."""
+    a = np.asarray(a)
+    ref = np.asarray(ref)
+    cplx = np.iscomplexobj(a) or np.iscomplexobj(ref)
+    a = a.astype(np.complex128 if cplx else np.float64)
+    ref = ref.astype(np.complex128 if cplx else np.float64)
     return float(np.linalg.norm((a - ref).ravel()) / max(np.linalg.norm(ref.ravel()), 1e-300))
 
 

@@ -366,3 +366,96 @@ def test_ls_deconv_decreases_objective():
         objs.append(np.linalg.norm(y - F(X)) ** 2)
         assert X.min() >= 0
     assert all(objs[i + 1] <= objs[i] * (1 + 1e-10) for i in range(len(objs) - 1))
+
+
+def test_tv_l2_stencil_is_convn_same_of_the_reference_filter():
+    """LS_deconv.m:54-58 builds a 3x3x3 filter and applies convn(X, filter, 'same');
+    scipy's N-d 'same' convolution is the same operation for odd filters.
+    Distinct multipliers per dimension catch a transposed axis."""
+    from scipy.signal import convolve
+    rng = np.random.default_rng(21)
+    X = rng.random((7, 6, 5))
+    lam = (0.3, -1.7, 2.9)
+    filt = np.zeros((3, 3, 3))
+    filt[1, 1, [0, 2]] = lam[2]   # filter(2,2,[1 3]) = lamb(3)
+    filt[1, [0, 2], 1] = lam[1]   # filter(2,[1 3],2) = lamb(2)
+    filt[[0, 2], 1, 1] = lam[0]   # filter([1 3],2,2) = lamb(1)
+    filt[1, 1, 1] = -sum(lam)
+    np.testing.assert_allclose(O.tv_l2_stencil(X, lam), convolve(X, filt, mode="same"), rtol=1e-12, atol=1e-12)
+
+
+def _dense_problem(seed, rows=40, cols=15):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((rows, cols))
+    x_true = np.maximum(rng.standard_normal(cols), 0)
+    y = A @ x_true + 0.05 * rng.standard_normal(rows)
+    return A, y
+
+
+def test_ls_deconv_converges_to_nnls():
+    """With every multiplier 0 the fixed point is the non-negative least-squares
+    solution (scipy.optimize.nnls); with lamb_L2 it is NNLS of the stacked
+    system [A; sqrt(lamb_L2) I] x = [y; 0]."""
+    from scipy.optimize import nnls
+    A, y = _dense_problem(5)
+    F = lambda x: A @ x
+    B = lambda o: A.T @ o
+    X = O.ls_deconv(F, B, B(y), max_iter=3000)
+    x_nnls, _ = nnls(A, y)
+    assert np.abs(X - x_nnls).max() < 1e-7 * max(1.0, np.abs(x_nnls).max())
+    l2 = 7.5
+    X2 = O.ls_deconv(F, B, B(y), max_iter=3000, lamb_L2=l2)
+    x2, _ = nnls(np.vstack([A, math.sqrt(l2) * np.eye(A.shape[1])]), np.concatenate([y, np.zeros(A.shape[1])]))
+    assert np.abs(X2 - x2).max() < 1e-7 * max(1.0, np.abs(x2).max())
+    assert np.abs(X2 - X).max() > 1e-3   # the L2 term changes the answer
+
+
+def test_ls_deconv_l1_fixed_point_is_bound_constrained_minimum():
+    """With lamb_L1 the iteration minimises 1/2||Ax-y||^2 + lamb_L1 sum(x) over
+    x >= 0 (the gradient it follows is A'Ax - A'y + lamb_L1); compare with
+    scipy's L-BFGS-B on that objective."""
+    from scipy.optimize import minimize
+    A, y = _dense_problem(6)
+    lam = 2.0
+    F = lambda x: A @ x
+    B = lambda o: A.T @ o
+    X = O.ls_deconv(F, B, B(y), max_iter=4000, lamb_L1=lam)
+    f = lambda x: 0.5 * np.sum((A @ x - y) ** 2) + lam * np.sum(x)
+    gr = lambda x: A.T @ (A @ x - y) + lam
+    r = minimize(f, np.zeros(A.shape[1]), jac=gr, method="L-BFGS-B", bounds=[(0, None)] * A.shape[1],
+                 options={"ftol": 1e-15, "gtol": 1e-12, "maxiter": 10000})
+    assert abs(f(X) - r.fun) < 1e-9 * abs(r.fun)
+    assert np.abs(X - r.x).max() < 1e-4
+    assert (X == 0).sum() > 0   # the L1 term makes the solution sparse
+
+
+@pytest.mark.parametrize("terms", [{}, {"lamb_L2": 0.7, "lamb_TV_L2": (0.2, 0.1, 0.3), "lamb_TV_L1": 1.0}])
+def test_ls_deconv_step_is_exact_line_search(terms):
+    """One iteration moves to X1 = max(df - a d, 0) where a minimises the
+    quadratic model along -d.  With the default step the model is ||A x - y||^2
+    only (LS_deconv.m:83-84); when lamb_TV_L1*lamb_TV_L2*lamb_L2 is nonzero in
+    every entry it includes the penalties (:81).  A wrong denominator fails."""
+    rng = np.random.default_rng(8)
+    H = rng.random((7, 7, 3, 3, 2))
+    F = lambda v: O.forward_project(H, v)
+    B = lambda o: O.backward_project(H, o)
+    y = F(rng.random((9, 12, 2)))
+    df = B(y) * 1.3
+    X1 = O.ls_deconv(F, B, df, max_iter=1, **terms)
+    l2 = terms.get("lamb_L2", 0.0)
+    tv = terms.get("lamb_TV_L2", (0.0, 0.0, 0.0))
+    Qfull = lambda x: B(F(x)) + (O.tv_l2_stencil(x, tv) if max(tv) else 0) + l2 * x
+    g = Qfull(df) - df
+    d = np.where((df > 0) | (g < 0), g, 0.0)
+    if terms:
+        model = lambda x: 0.5 * np.sum(x * Qfull(x)) - np.sum((Qfull(df) - g) * x)
+    else:
+        # the default step's curvature along d is ||F(d)||^2 only
+        model = lambda x: np.sum(g * (x - df)) + 0.5 * np.sum(F(x - df) ** 2)
+    # the model is quadratic in a: fit it through three points and take its vertex
+    s_ = 1e-3
+    m0, m1, m2 = (model(df - k * s_ * d) for k in (0, 1, 2))
+    c2 = (m2 - 2 * m1 + m0) / (2 * s_ * s_)
+    c1 = (m1 - m0) / s_ - c2 * s_
+    a = -c1 / (2 * c2)
+    np.testing.assert_allclose(X1, np.maximum(df - a * d, 0), rtol=1e-6, atol=1e-9 * np.abs(df).max())
