@@ -269,7 +269,10 @@ def main():
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_rl_full.json")
     if os.path.exists(tpath):
         tj = json.load(open(tpath))
-        if not (tj.get("precision") == head and tj.get("method") == args.method and wl.name == tj.get("workload")):
+        # the counters belong to one plan: same precision, method, workload and spectra size
+        same_plan = abs(float(tj.get("spectra_gb", -1)) - hr["spectra_gb"]) < 0.01
+        if not (tj.get("precision") == head and tj.get("method") == args.method and wl.name == tj.get("workload")
+                and same_plan):
             tj = None
     if t_gemm >= t_fftp:
         gk = f"sid::k_proj_gemm<15, 2, {({'fp32': 0, 'bf16x2': 1, 'bf16x3': 2}).get(head, 0)}>"

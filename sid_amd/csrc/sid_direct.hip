@@ -675,7 +675,9 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             else if (p->method == SID_METHOD_AUTO && (size_t)c < p->auto_fft.size()) use[c] = p->auto_fft[c];
         }
     }
-    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol && use == sp.fft_use)
+    // the FFT state stores FP16 spectra in the BF16X2 mode and FP32 otherwise
+    const bool fft_ok = !sp.fft.active || sp.fft.half == (p->precision == SID_PREC_BF16X2);
+    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol && use == sp.fft_use && fft_ok)
         return ensure_precision_buffers(p);
     free_plan_buffers(sp);
     fft_state_free(sp.fft);

@@ -14,17 +14,23 @@
 // followed by one inverse N x N transform.  The backward (adjoint) is
 //     Bt[k0][q] = sum_j conj(G[k0][q][j]) * What(k0 + N' j),  B_ab = IDFT_N'(Bt) / n^2.
 // Only k0y in [0, N'/2] is stored (real data: Hermitian symmetry gives the
-// rest).  G is precomputed per handle and streamed from HBM every projection
-// in two layouts (q-major for the forward, j-major for the backward) so both
-// GEMVs read coalesced rows: this path is HBM-bound.
+// rest).  G is precomputed per handle and streamed from HBM once per
+// projection in each direction: this path is HBM-bound.  In the BF16X2
+// precision mode G is stored as FP16 pairs, row q scaled by 1/s_q with s_q the
+// L1 norm of its kernel (the exact maximum of |G| for a non-negative kernel),
+// which halves the stream; s_q is folded into the packed input (forward) and
+// the unpacked output (backward).
 //
 // The FFTs are hand-written Stockham autosort transforms in LDS with
 // radix-4/2/3/5 butterflies; twiddles come from tables computed on the host
 // in double precision.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <vector>
@@ -256,9 +262,9 @@ __global__ void k_fft_sub_gather(const float *__restrict__ vol, const int *__res
     }
 }
 
-// vt[k0][q] = subs[q][k0x][k0y], k0 = k0x * Hy + k0y.
-__global__ void k_fft_pack_vt(const float2 *__restrict__ subs, int nq, int Nxp, int Nyp, int Hy,
-                              float2 *__restrict__ vt)
+// vt[k0][q] = s_q * subs[q][k0x][k0y], k0 = k0x * Hy + k0y (s_q = 1 without qscale).
+__global__ void k_fft_pack_vt(const float2 *__restrict__ subs, const float *__restrict__ qscale, int nq, int Nxp,
+                              int Nyp, int Hy, float2 *__restrict__ vt)
 {
     const int64_t total = (int64_t)Nxp * Hy * nq;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -266,30 +272,47 @@ __global__ void k_fft_pack_vt(const float2 *__restrict__ subs, int nq, int Nxp, 
         const int q = (int)(e % nq);
         const int k0 = (int)(e / nq);
         const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
-        vt[e] = subs[((int64_t)q * Nxp + k0x) * Nyp + k0y];
+        float2 v = subs[((int64_t)q * Nxp + k0x) * Nyp + k0y];
+        if (qscale) {
+            const float sq = qscale[q];
+            v.x *= sq;
+            v.y *= sq;
+        }
+        vt[e] = v;
     }
 }
 
-// Forward GEMV: oh[k0][j] = sum_q G[k0][q][j] * vt[k0][q]; one block per k0,
-// thread j.  vt[k0][q] is the same for every lane (a broadcast read through
-// the cache), so the block needs no LDS and many blocks share a CU to keep
-// enough HBM reads in flight.
-__global__ __launch_bounds__(256) void k_fft_gemv_fwd(const float2 *__restrict__ G, const float2 *__restrict__ vt,
-                                                      int nq, int n2, float2 *__restrict__ oh)
+__device__ __forceinline__ float2 spec_ld(const float2 *p) { return *p; }
+__device__ __forceinline__ float2 spec_ld(const __half2 *p) { return __half22float2(*p); }
+
+// Forward GEMV: oh[s][k0][j] = sum_{q in chunk s} G[k0][q][j] * vt[k0][q];
+// block (k0, s), thread j.  The q range of a k0 is split over gridDim.y
+// blocks so the grid is many times the resident capacity (no half-empty last
+// round); k_fft_assemble adds the chunks in a fixed order.  vt[k0][q] is the
+// same for every lane (a broadcast read through the cache), so the block needs
+// no LDS and many blocks share a CU to keep enough HBM reads in flight.  A
+// wave reads 64 consecutive elements of a row per load (512 B for FP32, 256 B
+// for FP16 pairs), which measures within a few percent of a plain streaming
+// read on this device.
+template <typename GT>
+__global__ __launch_bounds__(256) void k_fft_gemv_fwd(const GT *__restrict__ G, const float2 *__restrict__ vt,
+                                                      int nq, int n2, int qc, int nk0, int64_t kstride,
+                                                      float2 *__restrict__ oh)
 {
     const int k0 = blockIdx.x;
     const int j = threadIdx.x;
     if (j >= n2) return;
-    const float2 *g = G + (int64_t)k0 * nq * n2 + j;
+    const int qb = blockIdx.y * qc, qe = min(nq, qb + qc);
+    const GT *g = G + (int64_t)k0 * kstride + j;
     const float2 *v = vt + (int64_t)k0 * nq;
     float2 acc[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = make_float2(0.f, 0.f);
-    int q = 0;
-    for (; q + 8 <= nq; q += 8) {
+    int q = qb;
+    for (; q + 8 <= qe; q += 8) {
         float2 gg[8], vv[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) gg[u] = g[(int64_t)(q + u) * n2];
+        for (int u = 0; u < 8; ++u) gg[u] = spec_ld(g + (int64_t)(q + u) * n2);
 #pragma unroll
         for (int u = 0; u < 8; ++u) vv[u] = v[q + u];
 #pragma unroll
@@ -298,21 +321,21 @@ __global__ __launch_bounds__(256) void k_fft_gemv_fwd(const float2 *__restrict__
             acc[u].y += gg[u].x * vv[u].y + gg[u].y * vv[u].x;
         }
     }
-    for (; q < nq; ++q) {
-        const float2 g0 = g[(int64_t)q * n2], v0 = v[q];
+    for (; q < qe; ++q) {
+        const float2 g0 = spec_ld(g + (int64_t)q * n2), v0 = v[q];
         acc[0].x += g0.x * v0.x - g0.y * v0.y;
         acc[0].y += g0.x * v0.y + g0.y * v0.x;
     }
     float2 r = make_float2(0.f, 0.f);
 #pragma unroll
     for (int u = 0; u < 8; ++u) { r.x += acc[u].x; r.y += acc[u].y; }
-    oh[(int64_t)k0 * n2 + j] = r;
+    oh[((int64_t)blockIdx.y * nk0 + k0) * n2 + j] = r;
 }
 
-// Full N x N spectrum of the forward output from oh (stored for k0y < Hy) by
-// Hermitian symmetry; j = jx + n * jy.
-__global__ void k_fft_assemble(const float2 *__restrict__ oh, int Nx, int Ny, int Nxp, int Nyp, int Hy, int n,
-                               float2 *__restrict__ grid)
+// Full N x N spectrum of the forward output from the chunk sums oh[s][k0][j]
+// (stored for k0y < Hy) by Hermitian symmetry; j = jx + n * jy.
+__global__ void k_fft_assemble(const float2 *__restrict__ oh, int nsplit, int nk0, int Nx, int Ny, int Nxp, int Nyp,
+                               int Hy, int n, float2 *__restrict__ grid)
 {
     const int64_t total = (int64_t)Nx * Ny;
     const int n2 = n * n;
@@ -329,6 +352,11 @@ __global__ void k_fft_assemble(const float2 *__restrict__ oh, int Nx, int Ny, in
         const int k0 = (ax % Nxp) * Hy + (ay % Nyp);
         const int j = ax / Nxp + n * (ay / Nyp);
         float2 v = oh[(int64_t)k0 * n2 + j];
+        for (int sp = 1; sp < nsplit; ++sp) {
+            const float2 u = oh[((int64_t)sp * nk0 + k0) * n2 + j];
+            v.x += u.x;
+            v.y += u.y;
+        }
         if (cj) v.y = -v.y;
         grid[e] = v;
     }
@@ -349,22 +377,28 @@ __global__ void k_fft_img_grid(const float *__restrict__ img, int Nx, int Ny, in
     }
 }
 
-// Backward GEMV: bt[k0][q] = sum_j conj(G[k0][q][j]) * W(k0 + N' j).  One
-// block per k0.  A wave works on 8 rows q per iteration, 32 lanes per row
-// (lane jg covers j = jg + 32 t), so each load instruction reads two
-// contiguous 256-byte row segments and a row needs a 32-lane (5-step) shuffle
-// reduction; results for consecutive q are stored contiguously
-// (k_fft_unpack_bt moves them into the per-q sub-spectra).
-__global__ __launch_bounds__(256) void k_fft_gemv_bwd(const float2 *__restrict__ G, const float2 *__restrict__ grid,
-                                                      int nq, int n, int Nx, int Ny, int Nxp, int Nyp, int Hy,
-                                                      float2 *__restrict__ bt)
+// Backward GEMV: bt[k0][q] = sum_j conj(G[k0][q][j]) * W(k0 + N' j).  Block
+// (k0, s) takes rows q of chunk s (qc rows, a multiple of 16).  The chunk is
+// contiguous in HBM, so tiles of 16 rows are copied into LDS with 16-byte
+// loads (every tile starts 16-byte aligned: k0 blocks are padded to a multiple
+// of 16 bytes and a tile starts at a multiple of 16 rows); then 16 lanes sum
+// one row (lane l takes j = l + 16 t, conflict-free LDS reads) and reduce with
+// four shuffles.  Results for consecutive q are stored contiguously
+// (k_fft_unpack_bt moves them into the per-q sub-spectra).  Requires n*n <= 256.
+template <typename GT>
+__global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, const float2 *__restrict__ grid,
+                                                      int nq, int qc, int64_t kstride, int n, int Nx, int Ny, int Nxp,
+                                                      int Nyp, int Hy, float2 *__restrict__ bt)
 {
-    constexpr int JT = 8;                  // j values per lane: covers n*n <= 256
-    __shared__ float2 w[32 * JT];
+    constexpr int TQ = 16;                      // rows per tile
+    constexpr int EPV = 16 / (int)sizeof(GT);   // elements per 16-byte load
+    __shared__ uint4 tile[TQ * 256 / EPV];
+    __shared__ float2 w[256];
     const int n2 = n * n;
     const int k0 = blockIdx.x;
     const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
-    for (int j = threadIdx.x; j < 32 * JT; j += blockDim.x) {
+    {
+        const int j = threadIdx.x;
         float2 v = make_float2(0.f, 0.f);
         if (j < n2) {
             const int jx = j % n, jy = j / n;
@@ -372,49 +406,44 @@ __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const float2 *__restrict__
         }
         w[j] = v;
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int jg = lane & 31, qq = lane >> 5;
-    float2 wl[JT];
-#pragma unroll
-    for (int t = 0; t < JT; ++t) wl[t] = w[jg + 32 * t];
-    const float2 *gk = G + (int64_t)k0 * nq * n2;
-    float2 *out = bt + (int64_t)k0 * nq;
-    for (int q0 = 8 * wave; q0 < nq; q0 += 32) {
-        float re[4] = {0.f, 0.f, 0.f, 0.f}, im[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int q = q0 + 2 * u + qq;
-            if (q < nq) {
-                const float2 *g = gk + (int64_t)q * n2;
-#pragma unroll
-                for (int t = 0; t < JT; ++t) {
-                    const int j = jg + 32 * t;
-                    if (j < n2) {
-                        const float2 a = g[j], b = wl[t];
-                        re[u] += a.x * b.x + a.y * b.y;     // conj(a) * b
-                        im[u] += a.x * b.y - a.y * b.x;
-                    }
-                }
-            }
+    const int qb = blockIdx.y * qc, qe = min(nq, qb + qc);
+    const GT *gk = G + (int64_t)k0 * kstride;
+    const GT *ts = reinterpret_cast<const GT *>(tile);
+    const int r = threadIdx.x >> 4, sl = threadIdx.x & 15;
+    for (int q0 = qb; q0 < qe; q0 += TQ) {
+        const int nr = min(TQ, qe - q0);
+        const int nel = nr * n2;
+        const GT *src = gk + (int64_t)q0 * n2;
+        __syncthreads();   // previous tile consumed (and w written, first pass)
+        if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+            const int nv = nel / EPV;
+            const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+            for (int e = threadIdx.x; e < nv; e += 256) tile[e] = s4[e];
+            for (int e = nv * EPV + threadIdx.x; e < nel; e += 256) reinterpret_cast<GT *>(tile)[e] = src[e];
+        } else {
+            for (int e = threadIdx.x; e < nel; e += 256) reinterpret_cast<GT *>(tile)[e] = src[e];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            float r = re[u], i = im[u];
-#pragma unroll
-            for (int o = 16; o > 0; o >>= 1) {
-                r += __shfl_xor(r, o);
-                i += __shfl_xor(i, o);
+        __syncthreads();
+        if (r < nr) {
+            float re = 0.f, im = 0.f;
+            for (int j = sl; j < n2; j += 16) {
+                const float2 a = spec_ld(ts + r * n2 + j), b = w[j];
+                re += a.x * b.x + a.y * b.y;   // conj(a) * b
+                im += a.x * b.y - a.y * b.x;
             }
-            const int q = q0 + 2 * u + qq;
-            if (jg == 0 && q < nq) out[q] = make_float2(r, i);
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) {
+                re += __shfl_xor(re, o);
+                im += __shfl_xor(im, o);
+            }
+            if (sl == 0) bt[(int64_t)k0 * nq + q0 + r] = make_float2(re, im);
         }
     }
 }
 
-// bt[k0][q] -> subs[q][k0x][k0y] (k0y < Hy).
-__global__ void k_fft_unpack_bt(const float2 *__restrict__ bt, int nq, int Nxp, int Nyp, int Hy,
-                                float2 *__restrict__ subs)
+// s_q * bt[k0][q] -> subs[q][k0x][k0y] (k0y < Hy; s_q = 1 without qscale).
+__global__ void k_fft_unpack_bt(const float2 *__restrict__ bt, const float *__restrict__ qscale, int nq, int Nxp,
+                                int Nyp, int Hy, float2 *__restrict__ subs)
 {
     const int64_t total = (int64_t)Nxp * Hy * nq;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -422,7 +451,13 @@ __global__ void k_fft_unpack_bt(const float2 *__restrict__ bt, int nq, int Nxp, 
         const int q = (int)(e / ((int64_t)Nxp * Hy));
         const int k0 = (int)(e - (int64_t)q * Nxp * Hy);
         const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
-        subs[((int64_t)q * Nxp + k0x) * Nyp + k0y] = bt[(int64_t)k0 * nq + q];
+        float2 v = bt[(int64_t)k0 * nq + q];
+        if (qscale) {
+            const float sq = qscale[q];
+            v.x *= sq;
+            v.y *= sq;
+        }
+        subs[((int64_t)q * Nxp + k0x) * Nyp + k0y] = v;
     }
 }
 
@@ -486,8 +521,13 @@ __global__ void k_spec_fill(const float *__restrict__ H, const SpecJob *__restri
     }
 }
 
-__global__ void k_spec_scatter(const float2 *__restrict__ grids, const SpecJob *__restrict__ jobs, int nb, int Nx,
-                               int Ny, int Nxp, int Nyp, int Hy, int n, int nq, float2 *__restrict__ G)
+__device__ __forceinline__ void spec_st(float2 *p, float2 v, float) { *p = v; }
+__device__ __forceinline__ void spec_st(__half2 *p, float2 v, float inv) { *p = __floats2half2_rn(v.x * inv, v.y * inv); }
+
+template <typename GT>
+__global__ void k_spec_scatter(const float2 *__restrict__ grids, const SpecJob *__restrict__ jobs,
+                               const float *__restrict__ qscale, int nb, int Nx, int Ny, int Nxp, int Nyp, int Hy,
+                               int n, int64_t kstride, GT *__restrict__ G)
 {
     const int n2 = n * n;
     const int64_t nk0 = (int64_t)Nxp * Hy;
@@ -499,8 +539,35 @@ __global__ void k_spec_scatter(const float2 *__restrict__ grids, const SpecJob *
         const int k0 = (int)(t % nk0), b = (int)(t / nk0);
         const int k0x = k0 / Hy, k0y = k0 - k0x * Hy;
         const int kx = k0x + Nxp * (j % n), ky = k0y + Nyp * (j / n);
-        G[((int64_t)k0 * nq + jobs[b].q) * n2 + j] = grids[(int64_t)b * Nx * Ny + (int64_t)kx * Ny + ky];
+        const int q = jobs[b].q;
+        const float inv = qscale ? 1.0f / qscale[q] : 1.0f;
+        spec_st(G + (int64_t)k0 * kstride + (int64_t)q * n2 + j,
+                grids[(int64_t)b * Nx * Ny + (int64_t)kx * Ny + ky], inv);
     }
+}
+
+// qscale[q] = sum |H| over the kernel box of job q (1 for an all-zero kernel):
+// a bound on |G[.][q][.]|, attained at frequency 0 by a non-negative kernel.
+__global__ __launch_bounds__(256) void k_spec_l1(const float *__restrict__ H, const SpecJob *__restrict__ jobs,
+                                                 int64_t M, int n, float *__restrict__ qscale)
+{
+    __shared__ double red[256];
+    const SpecJob jb = jobs[blockIdx.x];
+    const int a = jb.ab % n, bb = jb.ab / n;
+    const int side = jb.hi - jb.lo + 1;
+    const float *h = H + M * M * ((int64_t)a + n * ((int64_t)bb + (int64_t)n * jb.plane));
+    double s = 0.0;
+    for (int e = threadIdx.x; e < side * side; e += 256) {
+        const int i = jb.lo + e % side, jj = jb.lo + e / side;
+        s += fabs((double)h[(int64_t)i + M * jj]);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) qscale[jb.q] = red[0] > 0.0 ? (float)red[0] : 1.0f;
 }
 
 static int grid_n(int64_t total) { return (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536)); }
@@ -534,7 +601,7 @@ void fft_state_free(FftState &f)
     f.ev.clear();
     fft_plan_free(f.fx); fft_plan_free(f.fy); fft_plan_free(f.fX); fft_plan_free(f.fY);
     for (void *q : {(void *)f.G, (void *)f.subs, (void *)f.vt, (void *)f.oh, (void *)f.grid, (void *)f.dfft,
-                    (void *)f.d_slot_plane, (void *)f.d_plane_fft})
+                    (void *)f.d_slot_plane, (void *)f.d_plane_fft, (void *)f.qscale})
         if (q) (void)hipFree(q);
     f = FftState();
 }
@@ -578,21 +645,34 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     f.nk0 = f.Nxp * f.Hy;
     f.n2 = n2;
     f.nq = (int)f.planes.size() * n2;
-    if ((int64_t)f.nq * 8 > 150 * 1024) {
-        set_error("too many FFT planes for the LDS-resident GEMV vector (%d)", (int)f.planes.size());
-        fft_state_free(f);
-        return SID_ERR_DIMS;
+    // GEMV row chunks: about 32K blocks per GEMV (many rounds of the resident
+    // 256-thread blocks on 256 CUs, so no half-empty last round), chunks a
+    // multiple of 32 rows.  SID_GEMV_SPLIT overrides the count (measurement).
+    {
+        int ns = (int)std::max<int64_t>(1, (32768 + f.nk0 - 1) / f.nk0);
+        if (const char *e = std::getenv("SID_GEMV_SPLIT")) ns = std::max(1, std::atoi(e));
+        ns = std::min(ns, std::max(1, f.nq / 32));
+        f.qc = ((f.nq + ns - 1) / ns + 31) / 32 * 32;
+        f.nsplit = (f.nq + f.qc - 1) / f.qc;
     }
     SID_TRY(fft_plan_make(f.fx, f.Nxp));
     SID_TRY(fft_plan_make(f.fy, f.Nyp));
     SID_TRY(fft_plan_make(f.fX, f.Nx));
     SID_TRY(fft_plan_make(f.fY, f.Ny));
-    const int64_t gsz = (int64_t)f.nk0 * f.nq * n2;
-    f.g_bytes = gsz * (int64_t)sizeof(float2);
+    if (n2 > 256) {
+        set_error("FFT path supports Nnum <= 16");
+        fft_state_free(f);
+        return SID_ERR_DIMS;
+    }
+    // FP16 spectra in the BF16X2 mode; k0 blocks padded to 16 bytes
+    f.half = p->precision == SID_PREC_BF16X2;
+    f.kstride = ((int64_t)f.nq * n2 + 3) / 4 * 4;
+    f.g_bytes = (int64_t)f.nk0 * f.kstride * (int64_t)(f.half ? sizeof(__half2) : sizeof(float2));
     SID_TRY_HIP(hipMalloc(&f.G, f.g_bytes));
+    if (f.half) SID_TRY_HIP(hipMalloc(&f.qscale, sizeof(float) * f.nq));
     SID_TRY_HIP(hipMalloc(&f.subs, sizeof(float2) * (int64_t)f.nq * f.Nxp * f.Nyp));
     SID_TRY_HIP(hipMalloc(&f.vt, sizeof(float2) * (int64_t)f.nk0 * f.nq));
-    SID_TRY_HIP(hipMalloc(&f.oh, sizeof(float2) * (int64_t)f.nk0 * n2));
+    SID_TRY_HIP(hipMalloc(&f.oh, sizeof(float2) * (int64_t)f.nsplit * f.nk0 * n2));
     SID_TRY_HIP(hipMalloc(&f.grid, sizeof(float2) * (int64_t)f.Nx * f.Ny));
     SID_TRY_HIP(hipMalloc(&f.dfft, sizeof(float) * sp.vol_elems));
     SID_TRY_HIP(hipMemsetAsync(f.dfft, 0, sizeof(float) * sp.vol_elems, st));
@@ -618,6 +698,10 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     SID_TRY_HIP(hipMalloc(&d_jobs, sizeof(SpecJob) * jobs.size()));
     SID_TRY_HIP(hipMemcpy(d_jobs, jobs.data(), sizeof(SpecJob) * jobs.size(), hipMemcpyHostToDevice));
     int rc = SID_OK;
+    if (f.half) {
+        hipLaunchKernelGGL(k_spec_l1, dim3((unsigned)jobs.size()), dim3(256), 0, st, p->H, d_jobs, p->M, n, f.qscale);
+        if (hipGetLastError() != hipSuccess) rc = SID_ERR_HIP;
+    }
     for (size_t b0 = 0; b0 < jobs.size() && rc == SID_OK; b0 += bmax) {
         const int nb = (int)std::min<size_t>(bmax, jobs.size() - b0);
         hipLaunchKernelGGL(k_spec_fill, dim3(grid_n(per * nb)), dim3(256), 0, st, p->H, d_jobs + b0, p->M, n, f.Nx,
@@ -625,9 +709,16 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
         // 2-D forward DFT of each grid: along y (contiguous), then along x
         rc = fft_run(f.fY, grids, (int64_t)nb * f.Nx, (int64_t)nb * f.Nx, 0, f.Ny, 1, -1, 1.0f, st);
         if (rc == SID_OK) rc = fft_run(f.fX, grids, (int64_t)nb * f.Ny, f.Ny, per, 1, f.Ny, -1, 1.0f, st);
-        if (rc == SID_OK)
-            hipLaunchKernelGGL(k_spec_scatter, dim3(grid_n((int64_t)nb * f.nk0 * n2)), dim3(256), 0, st, grids,
-                               d_jobs + b0, nb, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, n, f.nq, f.G);
+        if (rc == SID_OK) {
+            const dim3 gs(grid_n((int64_t)nb * f.nk0 * n2));
+            if (f.half)
+                hipLaunchKernelGGL(k_spec_scatter<__half2>, gs, dim3(256), 0, st, grids, d_jobs + b0, f.qscale, nb,
+                                   f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, n, f.kstride, (__half2 *)f.G);
+            else
+                hipLaunchKernelGGL(k_spec_scatter<float2>, gs, dim3(256), 0, st, grids, d_jobs + b0,
+                                   (const float *)nullptr, nb, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, n, f.kstride,
+                                   (float2 *)f.G);
+        }
     }
     const hipError_t e1 = hipStreamSynchronize(st);
     (void)hipFree(grids);
@@ -655,13 +746,18 @@ int fft_forward(sid_psf *p, hipStream_t st)
     const int64_t sz = (int64_t)f.Nxp * f.Nyp;
     SID_TRY(fft_run(f.fy, f.subs, (int64_t)f.nq * f.Nxp, (int64_t)f.nq * f.Nxp, 0, f.Nyp, 1, -1, 1.0f, st));
     SID_TRY(fft_run(f.fx, f.subs, (int64_t)f.nq * f.Hy, f.Hy, sz, 1, f.Nyp, -1, 1.0f, st));
-    hipLaunchKernelGGL(k_fft_pack_vt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.subs, f.nq, f.Nxp,
-                       f.Nyp, f.Hy, f.vt);
+    hipLaunchKernelGGL(k_fft_pack_vt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.subs, f.qscale, f.nq,
+                       f.Nxp, f.Nyp, f.Hy, f.vt);
     gemv_mark(f, st, 0, true);
-    hipLaunchKernelGGL(k_fft_gemv_fwd, dim3(f.nk0), dim3(256), 0, st, f.G, f.vt, f.nq, n2, f.oh);
+    if (f.half)
+        hipLaunchKernelGGL(k_fft_gemv_fwd<__half2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const __half2 *)f.G,
+                           f.vt, f.nq, n2, f.qc, f.nk0, f.kstride, f.oh);
+    else
+        hipLaunchKernelGGL(k_fft_gemv_fwd<float2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const float2 *)f.G,
+                           f.vt, f.nq, n2, f.qc, f.nk0, f.kstride, f.oh);
     gemv_mark(f, st, 0, false);
-    hipLaunchKernelGGL(k_fft_assemble, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, f.oh, f.Nx, f.Ny, f.Nxp,
-                       f.Nyp, f.Hy, (int)p->n, f.grid);
+    hipLaunchKernelGGL(k_fft_assemble, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, f.oh, f.nsplit, f.nk0,
+                       f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, (int)p->n, f.grid);
     SID_TRY_HIP(hipGetLastError());
     SID_TRY(fft_run(f.fY, f.grid, f.Nx, f.Nx, 0, f.Ny, 1, +1, 1.0f, st));
     SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, +1, 1.0f / ((float)f.Nx * (float)f.Ny), st));
@@ -681,11 +777,15 @@ int fft_backward(sid_psf *p, hipStream_t st)
     SID_TRY(fft_run(f.fY, f.grid, f.Nx, f.Nx, 0, f.Ny, 1, -1, 1.0f, st));
     SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, -1, 1.0f, st));
     gemv_mark(f, st, 1, true);
-    hipLaunchKernelGGL(k_fft_gemv_bwd, dim3(f.nk0), dim3(256), 0, st, f.G, f.grid, f.nq, n, f.Nx, f.Ny, f.Nxp, f.Nyp,
-                       f.Hy, f.vt);
+    if (f.half)
+        hipLaunchKernelGGL(k_fft_gemv_bwd<__half2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const __half2 *)f.G,
+                           f.grid, f.nq, f.qc, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
+    else
+        hipLaunchKernelGGL(k_fft_gemv_bwd<float2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const float2 *)f.G,
+                           f.grid, f.nq, f.qc, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
     gemv_mark(f, st, 1, false);
-    hipLaunchKernelGGL(k_fft_unpack_bt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.vt, f.nq, f.Nxp,
-                       f.Nyp, f.Hy, f.subs);
+    hipLaunchKernelGGL(k_fft_unpack_bt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.vt, f.qscale, f.nq,
+                       f.Nxp, f.Nyp, f.Hy, f.subs);
     const int nhi = f.Nyp - f.Hy;
     if (nhi > 0)
         hipLaunchKernelGGL(k_fft_herm_expand, dim3(grid_n((int64_t)f.nq * f.Nxp * nhi)), dim3(256), 0, st, f.subs,

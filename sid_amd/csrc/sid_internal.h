@@ -97,11 +97,16 @@ struct FftState {
     std::vector<int> planes;       // FFT planes, in slot order
     std::vector<int> slot_of;      // per plane of H: slot or -1
     int Nxp = 0, Nyp = 0, Nx = 0, Ny = 0, Hy = 0, nk0 = 0, n2 = 0, nq = 0;
+    int nsplit = 1, qc = 0;        // GEMV row chunks per k0 and rows per chunk
     FftPlan1D fx, fy, fX, fY;      // lengths Nx', Ny', Nx, Ny
-    float2 *G = nullptr;           // spectra [nk0][nq][n2]
+    void *G = nullptr;             // spectra [nk0][nq][n2] (k0 blocks kstride elements apart):
+                                   // float2, or __half2 scaled by 1/qscale[q] when `half`
+    bool half = false;
+    int64_t kstride = 0;
+    float *qscale = nullptr;       // per row q: max |G| bound (L1 norm of the kernel), half only
     float2 *subs = nullptr;        // sub-image transforms [nq][Nx'][Ny']
     float2 *vt = nullptr;          // [nk0][nq]
-    float2 *oh = nullptr;          // [nk0][n2]
+    float2 *oh = nullptr;          // forward GEMV chunk sums [nsplit][nk0][n2]
     float2 *grid = nullptr;        // [Nx][Ny]
     float *dfft = nullptr;         // backward result of the FFT planes, padded volume layout
     int *d_slot_plane = nullptr;

@@ -85,3 +85,36 @@ def test_fft_and_direct_agree_on_workload(env):
     Od, Of = pd.forward(V), pf.forward(V)
     assert rel_l2(Of, Od) < 1e-5
     assert rel_l2(pf.backward(Od), pd.backward(Od)) < 1e-5
+
+
+@pytest.mark.parametrize("case", [CASES[1], CASES[3], CASES[5]],
+                         ids=lambda c: "n%d_X%d_Y%d_M%d_Nz%d_%s" % (c[:5] + (c[5] or "full",)))
+def test_fft_fp16_spectra_vs_oracle(env, case):
+    """BF16X2 mode keeps the FFT planes' spectra in FP16, row q scaled by
+    1/sum|h_q|.  One stored value carries a relative rounding error of at most
+    2^-11 (4.9e-4); the bar is that bound, and every output sums 225 * Nz
+    such values so the measured error sits well below it."""
+    torch, sid_amd = env
+    n, X, Y, M, Nz, crop = case
+    print("case: fft fp16 spectra", case, flush=True)
+    rng = np.random.default_rng(abs(hash(case)) % 2**32 + 1)
+    H = random_psf(rng, M, n, Nz)
+    CA = None
+    if crop:
+        m = M // 2
+        CA = np.array([[max(1, m + 1 - r), min(M, m + 1 + r)] for r in range(1, Nz + 1)], dtype=np.int32)
+    V = rng.random((X, Y, Nz))
+    psf = sid_amd.PSF(H.astype(np.float32), CA, precision="bf16x2", method="fft")
+    e_f = rel_l2(psf.forward(V.astype(np.float32)), c_forward(H, V, CA))
+    b16 = psf.info()["spectra_bytes"]
+    assert b16 > 0
+    W = rng.random((X, Y))
+    e_b = rel_l2(psf.backward(W.astype(np.float32)), c_backward(H, W, CA))
+    print("fp16-spectra rel-L2 forward", e_f, "backward", e_b, flush=True)
+    assert e_f < 2.0 ** -11 and e_b < 2.0 ** -11
+    # switching to FP32 rebuilds FP32 spectra (twice the bytes) and tightens the result
+    psf.set_precision("fp32")
+    e32 = rel_l2(psf.forward(V.astype(np.float32)), c_forward(H, V, CA))
+    assert psf.info()["spectra_bytes"] == 2 * b16
+    assert e32 < 1e-5
+    psf.close()
