@@ -388,46 +388,51 @@ __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t sla
 // column-major volume B_col.  With rl_update (deconvRL.m:52,85-87):
 //   r = Htf / D; r(isinf) = 0; X = X .* r; X(isnan) = 0
 // on the padded volume (and its BF16 shadows), and copy X to B_col if given.
+// Backward reduction: block (r, c) = lenslet row r of plane c, thread ab
+// (n*n <= 256 phases, contiguous in the padded layout): sum of the row's
+// split-K slabs (direct planes) or the FFT-path result, then either the
+// column-major output or the ISRA update X <- X * Htf / B (deconvRL.m:51-57
+// guards: Inf ratio -> 0, NaN product -> 0).
 template <typename T>
-__global__ void k_bwd_reduce(const T *__restrict__ part, const int *__restrict__ plane_slab0, int64_t slab_stride,
-                             int NP, int nrows, int nz, int n, int Ly, int64_t XP, int64_t YP, int T_, int S, int X,
-                             int Y, float *__restrict__ B_col, int rl_update, const T *__restrict__ htf,
-                             T *__restrict__ vol, u16 *__restrict__ hi, u16 *__restrict__ lo,
-                             const int *__restrict__ plane_fft, const float *__restrict__ dfft)
+__global__ __launch_bounds__(256) void k_bwd_reduce(const T *__restrict__ part, const int *__restrict__ plane_slab0,
+                                                    int64_t slab_stride, int NP, int n, int Ly, int64_t XP,
+                                                    int64_t YP, int T_, int S, int X, int Y,
+                                                    float *__restrict__ B_col, int rl_update,
+                                                    const T *__restrict__ htf, T *__restrict__ vol,
+                                                    u16 *__restrict__ hi, u16 *__restrict__ lo,
+                                                    const int *__restrict__ plane_fft,
+                                                    const float *__restrict__ dfft)
 {
-    const int nn2 = n * n;
-    const int64_t per_plane = (int64_t)nrows * nn2;
-    const int64_t total = per_plane * nz;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(e / per_plane);
-        const int64_t rem = e - (int64_t)c * per_plane;
-        const int r = (int)(rem / nn2), ab = (int)(rem - (int64_t)r * nn2);
-        T d = 0;
+    const int ab = threadIdx.x;
+    if (ab >= n * n) return;
+    const int r = blockIdx.x, c = blockIdx.y;
+    const int Px = r / Ly, Py = r - Px * Ly;
+    const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
+    T d = 0;
+    if (plane_fft && plane_fft[c]) {
+        d = (T)dfft[o];   // plane on the FFT path
+    } else {
         const T *p = part + (int64_t)r * NP + ab;
         const int s0 = plane_slab0[c], s1 = plane_slab0[c + 1];
         for (int k = s0; k < s1; ++k) d += p[(int64_t)k * slab_stride];
-        const int Px = r / Ly, Py = r - Px * Ly;
-        const int x = Px * n + ab % n, y = Py * n + ab / n;
-        const bool real = x < X && y < Y;
-        const int64_t col = (int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c);
-        const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
-        if (plane_fft && plane_fft[c]) d = (T)dfft[o];   // plane on the FFT path
-        if (!rl_update) {
-            if (real) B_col[col] = (float)d;
-            continue;
-        }
-        T v = 0;
-        if (real) {
-            T ratio = htf[o] / d;
-            if (isinf(ratio)) ratio = 0;
-            v = vol[o] * ratio;
-            if (isnan(v)) v = 0;
-        }
-        vol[o] = v;
-        if (hi) store_shadow(hi, lo, o, (float)v);
-        if (B_col && real) B_col[col] = (float)v;
     }
+    const int x = Px * n + ab % n, y = Py * n + ab / n;
+    const bool real = x < X && y < Y;
+    const int64_t col = (int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c);
+    if (!rl_update) {
+        if (real) B_col[col] = (float)d;
+        return;
+    }
+    T v = 0;
+    if (real) {
+        T ratio = htf[o] / d;
+        if (isinf(ratio)) ratio = 0;
+        v = vol[o] * ratio;
+        if (isnan(v)) v = 0;
+    }
+    vol[o] = v;
+    if (hi) store_shadow(hi, lo, o, (float)v);
+    if (B_col && real) B_col[col] = (float)v;
 }
 
 // MATLAB column-major volume -> padded lenslet-major volume (+ BF16 shadows).
@@ -910,20 +915,19 @@ int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st)
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
     const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
-    const int64_t total = sp.nrows * p->n * p->n * p->nzH;
     const int64_t stride = sp.mpad * p->NP;
+    const dim3 grid((unsigned)sp.nrows, (unsigned)p->nzH);
     if (f64)
-        hipLaunchKernelGGL(k_bwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d,
-                           sp.bwd.d_plane_slab0, stride, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n,
-                           (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update,
-                           sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr, (const int *)nullptr,
-                           (const float *)nullptr);
+        hipLaunchKernelGGL(k_bwd_reduce<double>, grid, dim3(256), 0, st, sp.part_d, sp.bwd.d_plane_slab0, stride,
+                           (int)p->NP, (int)p->n, (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X,
+                           (int)sp.Y, B_col, rl_update, sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr,
+                           (const int *)nullptr, (const float *)nullptr);
     else
-        hipLaunchKernelGGL(k_bwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part,
-                           sp.bwd.d_plane_slab0, stride, (int)p->NP, (int)sp.nrows, (int)p->nzH, (int)p->n,
-                           (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X, (int)sp.Y, B_col, rl_update,
-                           sp.htf, sp.vol, sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr,
-                           sp.fft.active ? sp.fft.d_plane_fft : nullptr, sp.fft.active ? sp.fft.dfft : nullptr);
+        hipLaunchKernelGGL(k_bwd_reduce<float>, grid, dim3(256), 0, st, sp.part, sp.bwd.d_plane_slab0, stride,
+                           (int)p->NP, (int)p->n, (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X,
+                           (int)sp.Y, B_col, rl_update, sp.htf, sp.vol, sh ? sp.vol_hi : nullptr,
+                           sh ? sp.vol_lo : nullptr, sp.fft.active ? sp.fft.d_plane_fft : nullptr,
+                           sp.fft.active ? sp.fft.dfft : nullptr);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }

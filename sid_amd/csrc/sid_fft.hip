@@ -217,17 +217,23 @@ void fft_plan_free(FftPlan1D &pl)
     pl.L = 0;
 }
 
+static FftDesc fft_desc(const FftPlan1D &pl)
+{
+    FftDesc d;
+    d.L = pl.L;
+    d.nst = pl.nst;
+    for (int i = 0; i < 12; ++i) d.rad[i] = i < pl.nst ? pl.rad[i] : 1;
+    d.spb = std::max(1, std::min(64, 1024 / pl.L));
+    return d;
+}
+
 // Batched 1-D transforms; dir -1 forward, +1 inverse (scaled by `scale`).
 // Sequence s (0 <= s < nseq) starts at (s / inner) * outer_stride + (s % inner) * seq_stride.
 int fft_run(const FftPlan1D &pl, float2 *data, int64_t nseq, int64_t inner, int64_t outer_stride,
             int64_t seq_stride, int64_t elem_stride, int dir, float scale, hipStream_t st)
 {
     if (nseq <= 0) return SID_OK;
-    FftDesc d;
-    d.L = pl.L;
-    d.nst = pl.nst;
-    for (int i = 0; i < 12; ++i) d.rad[i] = i < pl.nst ? pl.rad[i] : 1;
-    d.spb = std::max(1, std::min(64, 1024 / pl.L));
+    FftDesc d = fft_desc(pl);
     const int64_t nblk = (nseq + d.spb - 1) / d.spb;
     const size_t lds = sizeof(float2) * 2 * d.spb * pl.L;
     for (int64_t b0 = 0; b0 < nblk; b0 += 65535) {
@@ -332,13 +338,40 @@ __global__ __launch_bounds__(256) void k_fft_gemv_fwd(const GT *__restrict__ G, 
     oh[((int64_t)blockIdx.y * nk0 + k0) * n2 + j] = r;
 }
 
-// Full N x N spectrum of the forward output from the chunk sums oh[s][k0][j]
-// (stored for k0y < Hy) by Hermitian symmetry; j = jx + n * jy.
-__global__ void k_fft_assemble(const float2 *__restrict__ oh, int nsplit, int nk0, int Nx, int Ny, int Nxp, int Nyp,
-                               int Hy, int n, float2 *__restrict__ grid)
+// Chunk sums of the forward GEMV, transposed: ohT[j][k0] = sum_s oh[s][k0][j].
+// 32 x 32 tiles through LDS so reads (along j) and writes (along k0) are both
+// contiguous; chunks are added in a fixed order.
+__global__ __launch_bounds__(256) void k_fft_oh_sum(const float2 *__restrict__ oh, int nsplit, int nk0, int n2,
+                                                    float2 *__restrict__ ohT)
+{
+    __shared__ float2 t[32][33];
+    const int k0b = blockIdx.x * 32, jb = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+    for (int r = ty; r < 32; r += 8) {
+        const int k0 = k0b + r, j = jb + tx;
+        float2 v = make_float2(0.f, 0.f);
+        if (k0 < nk0 && j < n2)
+            for (int sp = 0; sp < nsplit; ++sp) {
+                const float2 u = oh[((int64_t)sp * nk0 + k0) * n2 + j];
+                v.x += u.x;
+                v.y += u.y;
+            }
+        t[r][tx] = v;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = jb + r, k0 = k0b + tx;
+        if (k0 < nk0 && j < n2) ohT[(int64_t)j * nk0 + k0] = t[tx][r];
+    }
+}
+
+// Full N x N spectrum of the forward output from ohT[j][k0] (stored for
+// k0y < Hy) by Hermitian symmetry; j = jx + n * jy.  Consecutive ky read
+// consecutive k0.
+__global__ void k_fft_assemble(const float2 *__restrict__ ohT, int nk0, int Nx, int Ny, int Nxp, int Nyp, int Hy,
+                               int n, float2 *__restrict__ grid)
 {
     const int64_t total = (int64_t)Nx * Ny;
-    const int n2 = n * n;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int kx = (int)(e / Ny), ky = (int)(e % Ny);
@@ -351,12 +384,7 @@ __global__ void k_fft_assemble(const float2 *__restrict__ oh, int nsplit, int nk
         }
         const int k0 = (ax % Nxp) * Hy + (ay % Nyp);
         const int j = ax / Nxp + n * (ay / Nyp);
-        float2 v = oh[(int64_t)k0 * n2 + j];
-        for (int sp = 1; sp < nsplit; ++sp) {
-            const float2 u = oh[((int64_t)sp * nk0 + k0) * n2 + j];
-            v.x += u.x;
-            v.y += u.y;
-        }
+        float2 v = ohT[(int64_t)j * nk0 + k0];
         if (cj) v.y = -v.y;
         grid[e] = v;
     }
@@ -377,11 +405,33 @@ __global__ void k_fft_img_grid(const float *__restrict__ img, int Nx, int Ny, in
     }
 }
 
+// Loads of one tile of nel elements at src (16-byte aligned) into registers:
+// vector u of this thread is element range [(u * 256 + tid) * EPV, +EPV);
+// vectors past the tile re-read its first vector (always inside the spectra
+// buffer, whose k0 blocks hold at least 16 bytes) and are not stored.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename GT, int kVec>
+__device__ __forceinline__ void tile_fetch(u32x4 (&pre)[kVec], GT &tail, const GT *__restrict__ src, int nel)
+{
+    constexpr int EPV = 16 / (int)sizeof(GT);
+    const int nv = nel / EPV;
+    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+#pragma unroll
+    for (int u = 0; u < kVec; ++u) {
+        const int e = u * 256 + (int)threadIdx.x;
+        pre[u] = s4[e < nv ? e : 0];
+    }
+    const int et = nv * EPV + (int)threadIdx.x;
+    if (et < nel) tail = src[et];
+}
+
 // Backward GEMV: bt[k0][q] = sum_j conj(G[k0][q][j]) * W(k0 + N' j).  Block
 // (k0, s) takes rows q of chunk s (qc rows, a multiple of 16).  The chunk is
 // contiguous in HBM, so tiles of 16 rows are copied into LDS with 16-byte
 // loads (every tile starts 16-byte aligned: k0 blocks are padded to a multiple
-// of 16 bytes and a tile starts at a multiple of 16 rows); then 16 lanes sum
+// of 16 bytes, a tile starts at a multiple of 16 rows, and the host checks the
+// base alignment); then 16 lanes sum
 // one row (lane l takes j = l + 16 t, conflict-free LDS reads) and reduce with
 // four shuffles.  Results for consecutive q are stored contiguously
 // (k_fft_unpack_bt moves them into the per-q sub-spectra).  Requires n*n <= 256.
@@ -392,7 +442,7 @@ __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, 
 {
     constexpr int TQ = 16;                      // rows per tile
     constexpr int EPV = 16 / (int)sizeof(GT);   // elements per 16-byte load
-    __shared__ uint4 tile[TQ * 256 / EPV];
+    __shared__ u32x4 tile[TQ * 256 / EPV];
     __shared__ float2 w[256];
     const int n2 = n * n;
     const int k0 = blockIdx.x;
@@ -410,20 +460,29 @@ __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, 
     const GT *gk = G + (int64_t)k0 * kstride;
     const GT *ts = reinterpret_cast<const GT *>(tile);
     const int r = threadIdx.x >> 4, sl = threadIdx.x & 15;
+    // Tiles are staged through registers: the loads of tile t+1 are issued
+    // before tile t is summed, so HBM reads overlap the LDS work.  A tile is
+    // at most 16 * 256 elements = kVec 16-byte vectors per thread.
+    constexpr int kVec = TQ * 256 / EPV / 256;
+    u32x4 pre[kVec];
+    GT pre_tail = GT();
+    if (qb < qe) tile_fetch<GT, kVec>(pre, pre_tail, gk + (int64_t)qb * n2, min(TQ, qe - qb) * n2);
     for (int q0 = qb; q0 < qe; q0 += TQ) {
         const int nr = min(TQ, qe - q0);
-        const int nel = nr * n2;
-        const GT *src = gk + (int64_t)q0 * n2;
         __syncthreads();   // previous tile consumed (and w written, first pass)
-        if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
-            const int nv = nel / EPV;
-            const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-            for (int e = threadIdx.x; e < nv; e += 256) tile[e] = s4[e];
-            for (int e = nv * EPV + threadIdx.x; e < nel; e += 256) reinterpret_cast<GT *>(tile)[e] = src[e];
-        } else {
-            for (int e = threadIdx.x; e < nel; e += 256) reinterpret_cast<GT *>(tile)[e] = src[e];
+        {
+            const int nel = nr * n2, nv = nel / EPV;
+#pragma unroll
+            for (int u = 0; u < kVec; ++u) {
+                const int e = u * 256 + (int)threadIdx.x;
+                if (e < nv) tile[e] = pre[u];
+            }
+            const int et = nv * EPV + (int)threadIdx.x;
+            if (et < nel) reinterpret_cast<GT *>(tile)[et] = pre_tail;
         }
         __syncthreads();
+        if (q0 + TQ < qe)
+            tile_fetch<GT, kVec>(pre, pre_tail, gk + (int64_t)(q0 + TQ) * n2, min(TQ, qe - q0 - TQ) * n2);
         if (r < nr) {
             float re = 0.f, im = 0.f;
             for (int j = sl; j < n2; j += 16) {
@@ -490,6 +549,213 @@ __global__ void k_fft_sub_scatter(const float2 *__restrict__ subs, const int *__
         const int Qx = (int)(t % Lx), q = (int)(t / Lx);
         const int c = slot_plane[q / n2], ab = q % n2;
         dfft[(((int64_t)c * XP + Qx + T) * YP + (Qy + T)) * S + ab] = subs[((int64_t)q * Nxp + Qx) * Nyp + Qy].x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused sub-image transforms.  One block per pair of sub-images (q, q+1)
+// transformed together as z = a + i b (two real 2-D DFTs for the price of one
+// complex one); the Nx' x Ny' grid and its ping-pong copy live in LDS.
+// ---------------------------------------------------------------------------
+
+// One Stockham stage over nseq sequences of length L in LDS, element t of
+// sequence s at s * ss + t * es.  Consecutive threads take consecutive
+// sequences when the sequences are interleaved (es > 1), consecutive elements
+// otherwise, so LDS accesses stay on distinct banks.
+template <int R>
+__device__ __forceinline__ void stockham_stage_2d(const float2 *__restrict__ in, float2 *__restrict__ out, int L,
+                                                  int Ns, const float2 *__restrict__ tw, float sgn, int nseq, int ss,
+                                                  int es)
+{
+    const int nb = L / R;
+    const int tot = nseq * nb;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        int sq, j;
+        if (es > 1) { sq = e % nseq; j = e / nseq; }
+        else { sq = e / nb; j = e - sq * nb; }
+        const float2 *src = in + sq * ss;
+        float2 *dst = out + sq * ss;
+        const int k = j % Ns;
+        const int tstep = (L / (Ns * R)) * k;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float2 x = src[(j + r * nb) * es];
+            v[r] = r == 0 ? x : cmul(x, tw[(tstep * r) % L]);
+        }
+        small_dft<R>(v, sgn);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[(base + r * Ns) * es] = v[r];
+    }
+}
+
+// All stages of one direction; returns the buffer holding the result.
+__device__ float2 *fft_pass_2d(float2 *a, float2 *b, const FftDesc &d, const float2 *__restrict__ tw, float sgn,
+                               int nseq, int ss, int es)
+{
+    int Ns = 1;
+    float2 *in = a, *out = b;
+    for (int st = 0; st < d.nst; ++st) {
+        const int R = d.rad[st];
+        if (R == 4) stockham_stage_2d<4>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
+        else if (R == 2) stockham_stage_2d<2>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
+        else if (R == 3) stockham_stage_2d<3>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
+        else stockham_stage_2d<5>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
+        __syncthreads();
+        Ns *= R;
+        float2 *t = in; in = out; out = t;
+    }
+    return in;
+}
+
+struct SubGeom {
+    int nq, n2, Nxp, Nyp, Hy, Lx, Ly, T, S, nz_in;
+    int64_t XP, YP;
+};
+
+// Forward: volume -> 2-D DFT of the pair's sub-images -> vt[k0][q] = s_q * V_q(k0)
+// for the stored half spectrum.  Z = DFT(a + i b) gives
+//   A(k) = (Z(k) + conj(Z(-k))) / 2,   B(k) = (Z(k) - conj(Z(-k))) / (2 i).
+constexpr int kSubThreads = 1024;   // one block per CU (the two grids take most of the LDS)
+constexpr int kSubUnroll = 8;       // independent global accesses in flight per thread
+
+// Pair handled by this block.  Blocks are dealt to the 8 XCDs round robin and
+// each XCD has its own L2; neighbouring pairs read the same cache lines (the
+// volume's lenslet rows, the packed spectra), so each XCD takes one
+// contiguous range of pairs.  A bijection of [0, npairs).
+__device__ __forceinline__ int xcd_pair(int b, int npairs)
+{
+    constexpr int kXcd = 8;
+    const int q = npairs / kXcd, r = npairs % kXcd;
+    const int x = b % kXcd, i = b / kXcd;
+    return x * q + min(x, r) + i;
+}
+
+// Twiddle tables of both lengths into LDS (after the two grids).
+__device__ __forceinline__ void load_twiddles(float2 *tx, float2 *ty, const float2 *__restrict__ twx,
+                                              const float2 *__restrict__ twy, int Lx, int Ly)
+{
+    for (int t = threadIdx.x; t < Lx; t += blockDim.x) tx[t] = twx[t];
+    for (int t = threadIdx.x; t < Ly; t += blockDim.x) ty[t] = twy[t];
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__restrict__ vol,
+                                                             const int *__restrict__ slot_plane,
+                                                             const float *__restrict__ qscale, SubGeom g, FftDesc dx,
+                                                             FftDesc dy, const float2 *__restrict__ twx,
+                                                             const float2 *__restrict__ twy, float2 *__restrict__ vt)
+{
+    extern __shared__ float2 sm[];
+    const int npix = g.Nxp * g.Nyp;
+    float2 *bufa = sm, *bufb = sm + npix, *tx = sm + 2 * npix, *ty = tx + g.Nxp;
+    load_twiddles(tx, ty, twx, twy, g.Nxp, g.Nyp);
+    const int q0 = 2 * xcd_pair(blockIdx.x, gridDim.x);
+    const bool has1 = q0 + 1 < g.nq;
+    const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
+    const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
+    const bool in0 = c0 < g.nz_in, in1 = has1 && c1 < g.nz_in;
+    const float *v0p = vol + (int64_t)c0 * g.XP * g.YP * g.S + ab0;
+    const float *v1p = vol + (int64_t)c1 * g.XP * g.YP * g.S + ab1;
+    for (int base = 0; base < npix; base += kSubUnroll * kSubThreads) {
+        float a[kSubUnroll], b[kSubUnroll];
+#pragma unroll
+        for (int u = 0; u < kSubUnroll; ++u) {
+            const int e = base + u * kSubThreads + (int)threadIdx.x;
+            const int Qx = e / g.Nyp, Qy = e - Qx * g.Nyp;
+            a[u] = b[u] = 0.f;
+            if (e < npix && Qx < g.Lx && Qy < g.Ly) {
+                const int64_t cell = (((int64_t)Qx + g.T) * g.YP + (Qy + g.T)) * g.S;
+                if (in0) a[u] = v0p[cell];
+                if (in1) b[u] = v1p[cell];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSubUnroll; ++u) {
+            const int e = base + u * kSubThreads + (int)threadIdx.x;
+            if (e < npix) bufa[e] = make_float2(a[u], b[u]);
+        }
+    }
+    __syncthreads();
+    float2 *r1 = fft_pass_2d(bufa, bufb, dy, ty, -1.0f, g.Nxp, g.Nyp, 1);         // along y (rows)
+    float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dx, tx, -1.0f, g.Nyp, 1, g.Nyp);  // along x
+    const float s0 = qscale ? qscale[q0] : 1.0f;
+    const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
+    const int nk0 = g.Nxp * g.Hy;
+    for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
+        const int kx = k0 / g.Hy, ky = k0 - kx * g.Hy;
+        const float2 z = r2[kx * g.Nyp + ky];
+        const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
+        const float2 zm = r2[mx * g.Nyp + my];
+        const float2 a = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+        const float2 b = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+        float2 *o = vt + (int64_t)k0 * g.nq + q0;
+        o[0] = make_float2(a.x * s0, a.y * s0);
+        if (has1) o[1] = make_float2(b.x * s1, b.y * s1);
+    }
+}
+
+// Backward: s_q * bt[k0][q] (stored half spectrum, Hermitian completion) ->
+// inverse 2-D DFT of A + i B -> real parts a, b into the padded volume layout,
+// scaled by `scale`.
+__global__ __launch_bounds__(kSubThreads) void k_fft_sub_bwd(const float2 *__restrict__ bt,
+                                                             const int *__restrict__ slot_plane,
+                                                             const float *__restrict__ qscale, SubGeom g, FftDesc dx,
+                                                             FftDesc dy, const float2 *__restrict__ twx,
+                                                             const float2 *__restrict__ twy, float scale,
+                                                             float *__restrict__ dfft)
+{
+    extern __shared__ float2 sm[];
+    const int npix = g.Nxp * g.Nyp;
+    float2 *bufa = sm, *bufb = sm + npix, *tx = sm + 2 * npix, *ty = tx + g.Nxp;
+    load_twiddles(tx, ty, twx, twy, g.Nxp, g.Nyp);
+    const int q0 = 2 * xcd_pair(blockIdx.x, gridDim.x);
+    const bool has1 = q0 + 1 < g.nq;
+    const float s0 = qscale ? qscale[q0] : 1.0f;
+    const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
+    for (int base = 0; base < npix; base += kSubUnroll * kSubThreads) {
+        float2 a[kSubUnroll], b[kSubUnroll];
+        bool cj[kSubUnroll];
+#pragma unroll
+        for (int u = 0; u < kSubUnroll; ++u) {
+            const int e = base + u * kSubThreads + (int)threadIdx.x;
+            a[u] = b[u] = make_float2(0.f, 0.f);
+            cj[u] = false;
+            if (e < npix) {
+                const int kx = e / g.Nyp, ky = e - kx * g.Nyp;
+                int sx = kx, sy = ky;
+                if (ky >= g.Hy) {   // conj of the stored value at -k
+                    sx = kx == 0 ? 0 : g.Nxp - kx;
+                    sy = g.Nyp - ky;
+                    cj[u] = true;
+                }
+                const float2 *src = bt + (int64_t)(sx * g.Hy + sy) * g.nq + q0;
+                a[u] = src[0];
+                if (has1) b[u] = src[1];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSubUnroll; ++u) {
+            const int e = base + u * kSubThreads + (int)threadIdx.x;
+            if (e < npix) {
+                const float2 x = make_float2(a[u].x * s0, (cj[u] ? -a[u].y : a[u].y) * s0);
+                const float2 y = make_float2(b[u].x * s1, (cj[u] ? -b[u].y : b[u].y) * s1);
+                bufa[e] = make_float2(x.x - y.y, x.y + y.x);   // A + i B
+            }
+        }
+    }
+    __syncthreads();
+    float2 *r1 = fft_pass_2d(bufa, bufb, dx, tx, 1.0f, g.Nyp, 1, g.Nyp);          // along x
+    float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dy, ty, 1.0f, g.Nxp, g.Nyp, 1);  // along y
+    const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
+    const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
+    const int npx = g.Lx * g.Ly;
+    for (int e = threadIdx.x; e < npx; e += blockDim.x) {
+        const int Qx = e / g.Ly, Qy = e - Qx * g.Ly;
+        const float2 z = r2[Qx * g.Nyp + Qy];
+        const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
+        dfft[((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0] = z.x * scale;
+        if (has1) dfft[((int64_t)c1 * g.XP * g.YP + cell) * g.S + ab1] = z.y * scale;
     }
 }
 
@@ -600,7 +866,7 @@ void fft_state_free(FftState &f)
     for (auto e : f.ev) (void)hipEventDestroy(e);
     f.ev.clear();
     fft_plan_free(f.fx); fft_plan_free(f.fy); fft_plan_free(f.fX); fft_plan_free(f.fY);
-    for (void *q : {(void *)f.G, (void *)f.subs, (void *)f.vt, (void *)f.oh, (void *)f.grid, (void *)f.dfft,
+    for (void *q : {(void *)f.G, (void *)f.subs, (void *)f.vt, (void *)f.oh, (void *)f.ohT, (void *)f.grid, (void *)f.dfft,
                     (void *)f.d_slot_plane, (void *)f.d_plane_fft, (void *)f.qscale})
         if (q) (void)hipFree(q);
     f = FftState();
@@ -645,15 +911,19 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     f.nk0 = f.Nxp * f.Hy;
     f.n2 = n2;
     f.nq = (int)f.planes.size() * n2;
-    // GEMV row chunks: about 32K blocks per GEMV (many rounds of the resident
-    // 256-thread blocks on 256 CUs, so no half-empty last round), chunks a
-    // multiple of 32 rows.  SID_GEMV_SPLIT overrides the count (measurement).
+    // GEMV row chunks, multiples of 32 rows: about 32K blocks per GEMV (many
+    // rounds of the resident blocks, so no half-empty last round).
+    // SID_GEMV_SPLIT overrides both counts (measurement).
     {
-        int ns = (int)std::max<int64_t>(1, (32768 + f.nk0 - 1) / f.nk0);
-        if (const char *e = std::getenv("SID_GEMV_SPLIT")) ns = std::max(1, std::atoi(e));
-        ns = std::min(ns, std::max(1, f.nq / 32));
-        f.qc = ((f.nq + ns - 1) / ns + 31) / 32 * 32;
-        f.nsplit = (f.nq + f.qc - 1) / f.qc;
+        auto chunks = [&](int target, int &ns, int &qc) {
+            ns = (int)std::max<int64_t>(1, (target + f.nk0 - 1) / f.nk0);
+            if (const char *e = std::getenv("SID_GEMV_SPLIT")) ns = std::max(1, std::atoi(e));
+            ns = std::min(ns, std::max(1, f.nq / 32));
+            qc = ((f.nq + ns - 1) / ns + 31) / 32 * 32;
+            ns = (f.nq + qc - 1) / qc;
+        };
+        chunks(32768, f.nsplit, f.qc);
+        chunks(32768, f.nsplit_b, f.qc_b);
     }
     SID_TRY(fft_plan_make(f.fx, f.Nxp));
     SID_TRY(fft_plan_make(f.fy, f.Nyp));
@@ -669,10 +939,32 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     f.kstride = ((int64_t)f.nq * n2 + 3) / 4 * 4;
     f.g_bytes = (int64_t)f.nk0 * f.kstride * (int64_t)(f.half ? sizeof(__half2) : sizeof(float2));
     SID_TRY_HIP(hipMalloc(&f.G, f.g_bytes));
+    if ((reinterpret_cast<uintptr_t>(f.G) & 15) != 0) {   // the backward GEMV's 16-byte tile loads rely on it
+        set_error("spectra buffer is not 16-byte aligned");
+        fft_state_free(f);
+        return SID_ERR_HIP;
+    }
     if (f.half) SID_TRY_HIP(hipMalloc(&f.qscale, sizeof(float) * f.nq));
-    SID_TRY_HIP(hipMalloc(&f.subs, sizeof(float2) * (int64_t)f.nq * f.Nxp * f.Nyp));
+    // fused sub-image transforms when both Nx' x Ny' grids fit in a block's LDS
+    // (SID_FFT_UNFUSED=1 keeps the separate gather / 1-D FFT / pack kernels)
+    {
+        const size_t lds = sizeof(float2) * (2 * (size_t)f.Nxp * f.Nyp + f.Nxp + f.Nyp);
+        hipDeviceProp_t prop;
+        f.fused = std::getenv("SID_FFT_UNFUSED") == nullptr &&
+                  hipGetDeviceProperties(&prop, p->device) == hipSuccess && lds <= prop.sharedMemPerBlock;
+        if (f.fused && lds > 65536) {
+            f.fused = hipFuncSetAttribute((const void *)k_fft_sub_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds) == hipSuccess &&
+                      hipFuncSetAttribute((const void *)k_fft_sub_bwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds) == hipSuccess;
+            (void)hipGetLastError();
+        }
+        f.sub_lds = lds;
+    }
+    if (!f.fused) SID_TRY_HIP(hipMalloc(&f.subs, sizeof(float2) * (int64_t)f.nq * f.Nxp * f.Nyp));
     SID_TRY_HIP(hipMalloc(&f.vt, sizeof(float2) * (int64_t)f.nk0 * f.nq));
     SID_TRY_HIP(hipMalloc(&f.oh, sizeof(float2) * (int64_t)f.nsplit * f.nk0 * n2));
+    SID_TRY_HIP(hipMalloc(&f.ohT, sizeof(float2) * (int64_t)f.nk0 * n2));
     SID_TRY_HIP(hipMalloc(&f.grid, sizeof(float2) * (int64_t)f.Nx * f.Ny));
     SID_TRY_HIP(hipMalloc(&f.dfft, sizeof(float) * sp.vol_elems));
     SID_TRY_HIP(hipMemsetAsync(f.dfft, 0, sizeof(float) * sp.vol_elems, st));
@@ -732,6 +1024,25 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     return SID_OK;
 }
 
+static SubGeom sub_geom(const sid_psf *p, const FftState &f)
+{
+    const SensorPlan &sp = p->plan;
+    SubGeom g;
+    g.nq = f.nq;
+    g.n2 = f.n2;
+    g.Nxp = f.Nxp;
+    g.Nyp = f.Nyp;
+    g.Hy = f.Hy;
+    g.Lx = (int)sp.Lx;
+    g.Ly = (int)sp.Ly;
+    g.T = (int)p->T;
+    g.S = (int)p->S;
+    g.nz_in = (int)sp.nz;
+    g.XP = sp.XP;
+    g.YP = sp.YP;
+    return g;
+}
+
 // Forward FFT planes: leaves the real N x N image in f.grid (.x).
 int fft_forward(sid_psf *p, hipStream_t st)
 {
@@ -739,15 +1050,21 @@ int fft_forward(sid_psf *p, hipStream_t st)
     FftState &f = sp.fft;
     if (!f.active) return SID_OK;
     const int n2 = f.n2;
-    hipLaunchKernelGGL(k_fft_sub_gather, dim3(grid_n((int64_t)f.nq * f.Nxp * f.Nyp)), dim3(256), 0, st, sp.vol,
-                       f.d_slot_plane, f.nq, n2, f.Nxp, f.Nyp, (int)sp.Lx, (int)sp.Ly, sp.XP, sp.YP, (int)p->T,
-                       (int)p->S, (int)sp.nz, f.subs);
-    SID_TRY_HIP(hipGetLastError());
-    const int64_t sz = (int64_t)f.Nxp * f.Nyp;
-    SID_TRY(fft_run(f.fy, f.subs, (int64_t)f.nq * f.Nxp, (int64_t)f.nq * f.Nxp, 0, f.Nyp, 1, -1, 1.0f, st));
-    SID_TRY(fft_run(f.fx, f.subs, (int64_t)f.nq * f.Hy, f.Hy, sz, 1, f.Nyp, -1, 1.0f, st));
-    hipLaunchKernelGGL(k_fft_pack_vt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.subs, f.qscale, f.nq,
-                       f.Nxp, f.Nyp, f.Hy, f.vt);
+    if (f.fused) {
+        hipLaunchKernelGGL(k_fft_sub_fwd, dim3((f.nq + 1) / 2), dim3(kSubThreads), f.sub_lds, st, sp.vol, f.d_slot_plane,
+                           f.qscale, sub_geom(p, f), fft_desc(f.fx), fft_desc(f.fy), f.fx.tw_fwd, f.fy.tw_fwd, f.vt);
+        SID_TRY_HIP(hipGetLastError());
+    } else {
+        hipLaunchKernelGGL(k_fft_sub_gather, dim3(grid_n((int64_t)f.nq * f.Nxp * f.Nyp)), dim3(256), 0, st, sp.vol,
+                           f.d_slot_plane, f.nq, n2, f.Nxp, f.Nyp, (int)sp.Lx, (int)sp.Ly, sp.XP, sp.YP, (int)p->T,
+                           (int)p->S, (int)sp.nz, f.subs);
+        SID_TRY_HIP(hipGetLastError());
+        const int64_t sz = (int64_t)f.Nxp * f.Nyp;
+        SID_TRY(fft_run(f.fy, f.subs, (int64_t)f.nq * f.Nxp, (int64_t)f.nq * f.Nxp, 0, f.Nyp, 1, -1, 1.0f, st));
+        SID_TRY(fft_run(f.fx, f.subs, (int64_t)f.nq * f.Hy, f.Hy, sz, 1, f.Nyp, -1, 1.0f, st));
+        hipLaunchKernelGGL(k_fft_pack_vt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.subs, f.qscale, f.nq,
+                           f.Nxp, f.Nyp, f.Hy, f.vt);
+    }
     gemv_mark(f, st, 0, true);
     if (f.half)
         hipLaunchKernelGGL(k_fft_gemv_fwd<__half2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const __half2 *)f.G,
@@ -756,8 +1073,10 @@ int fft_forward(sid_psf *p, hipStream_t st)
         hipLaunchKernelGGL(k_fft_gemv_fwd<float2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const float2 *)f.G,
                            f.vt, f.nq, n2, f.qc, f.nk0, f.kstride, f.oh);
     gemv_mark(f, st, 0, false);
-    hipLaunchKernelGGL(k_fft_assemble, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, f.oh, f.nsplit, f.nk0,
-                       f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, (int)p->n, f.grid);
+    hipLaunchKernelGGL(k_fft_oh_sum, dim3((f.nk0 + 31) / 32, (n2 + 31) / 32), dim3(256), 0, st, f.oh, f.nsplit, f.nk0,
+                       n2, f.ohT);
+    hipLaunchKernelGGL(k_fft_assemble, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, f.ohT, f.nk0, f.Nx, f.Ny,
+                       f.Nxp, f.Nyp, f.Hy, (int)p->n, f.grid);
     SID_TRY_HIP(hipGetLastError());
     SID_TRY(fft_run(f.fY, f.grid, f.Nx, f.Nx, 0, f.Ny, 1, +1, 1.0f, st));
     SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, +1, 1.0f / ((float)f.Nx * (float)f.Ny), st));
@@ -778,12 +1097,19 @@ int fft_backward(sid_psf *p, hipStream_t st)
     SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, -1, 1.0f, st));
     gemv_mark(f, st, 1, true);
     if (f.half)
-        hipLaunchKernelGGL(k_fft_gemv_bwd<__half2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const __half2 *)f.G,
-                           f.grid, f.nq, f.qc, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
+        hipLaunchKernelGGL(k_fft_gemv_bwd<__half2>, dim3(f.nk0, f.nsplit_b), dim3(256), 0, st, (const __half2 *)f.G,
+                           f.grid, f.nq, f.qc_b, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
     else
-        hipLaunchKernelGGL(k_fft_gemv_bwd<float2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const float2 *)f.G,
-                           f.grid, f.nq, f.qc, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
+        hipLaunchKernelGGL(k_fft_gemv_bwd<float2>, dim3(f.nk0, f.nsplit_b), dim3(256), 0, st, (const float2 *)f.G,
+                           f.grid, f.nq, f.qc_b, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
     gemv_mark(f, st, 1, false);
+    if (f.fused) {
+        hipLaunchKernelGGL(k_fft_sub_bwd, dim3((f.nq + 1) / 2), dim3(kSubThreads), f.sub_lds, st, f.vt, f.d_slot_plane,
+                           f.qscale, sub_geom(p, f), fft_desc(f.fx), fft_desc(f.fy), f.fx.tw_inv, f.fy.tw_inv,
+                           1.0f / ((float)f.n2 * (float)f.Nxp * (float)f.Nyp), f.dfft);
+        SID_TRY_HIP(hipGetLastError());
+        return SID_OK;
+    }
     hipLaunchKernelGGL(k_fft_unpack_bt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.vt, f.qscale, f.nq,
                        f.Nxp, f.Nyp, f.Hy, f.subs);
     const int nhi = f.Nyp - f.Hy;

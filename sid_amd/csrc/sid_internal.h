@@ -97,16 +97,20 @@ struct FftState {
     std::vector<int> planes;       // FFT planes, in slot order
     std::vector<int> slot_of;      // per plane of H: slot or -1
     int Nxp = 0, Nyp = 0, Nx = 0, Ny = 0, Hy = 0, nk0 = 0, n2 = 0, nq = 0;
-    int nsplit = 1, qc = 0;        // GEMV row chunks per k0 and rows per chunk
+    int nsplit = 1, qc = 0;        // forward GEMV row chunks per k0 and rows per chunk
+    int nsplit_b = 1, qc_b = 0;    // the same for the backward GEMV
+    bool fused = false;            // sub-image transforms in one LDS-resident kernel per pair
+    size_t sub_lds = 0;
     FftPlan1D fx, fy, fX, fY;      // lengths Nx', Ny', Nx, Ny
     void *G = nullptr;             // spectra [nk0][nq][n2] (k0 blocks kstride elements apart):
                                    // float2, or __half2 scaled by 1/qscale[q] when `half`
     bool half = false;
     int64_t kstride = 0;
     float *qscale = nullptr;       // per row q: max |G| bound (L1 norm of the kernel), half only
-    float2 *subs = nullptr;        // sub-image transforms [nq][Nx'][Ny']
+    float2 *subs = nullptr;        // sub-image transforms [nq][Nx'][Ny'] (unfused path only)
     float2 *vt = nullptr;          // [nk0][nq]
     float2 *oh = nullptr;          // forward GEMV chunk sums [nsplit][nk0][n2]
+    float2 *ohT = nullptr;         // their sum, transposed [n2][nk0]
     float2 *grid = nullptr;        // [Nx][Ny]
     float *dfft = nullptr;         // backward result of the FFT planes, padded volume layout
     int *d_slot_plane = nullptr;

@@ -118,3 +118,31 @@ def test_fft_fp16_spectra_vs_oracle(env, case):
     assert psf.info()["spectra_bytes"] == 2 * b16
     assert e32 < 1e-5
     psf.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x2"])
+def test_fft_fused_and_unfused_sub_transforms_agree(env, precision, monkeypatch):
+    """The LDS-resident pair transform (two real sub-images as one complex
+    2-D DFT) and the separate gather / 1-D FFT / pack kernels compute the same
+    projections; both agree with the oracle."""
+    torch, sid_amd = env
+    n, X, Y, M, Nz = 5, 33, 41, 13, 3        # 15 sub-images: an unpaired last one
+    print("case: fused vs unfused", precision, flush=True)
+    rng = np.random.default_rng(77)
+    H = random_psf(rng, M, n, Nz)
+    V = rng.random((X, Y, Nz))
+    W = rng.random((X, Y))
+    out = {}
+    for mode in ("fused", "unfused"):
+        if mode == "unfused":
+            monkeypatch.setenv("SID_FFT_UNFUSED", "1")
+        psf = sid_amd.PSF(H.astype(np.float32), precision=precision, method="fft")
+        out[mode] = (psf.forward(V.astype(np.float32)), psf.backward(W.astype(np.float32)))
+        psf.close()
+    monkeypatch.delenv("SID_FFT_UNFUSED", raising=False)
+    tol = 1e-5 if precision == "fp32" else 2.0 ** -11
+    for i, ref in enumerate((c_forward(H, V), c_backward(H, W))):
+        e_fu, e_un = rel_l2(out["fused"][i], ref), rel_l2(out["unfused"][i], ref)
+        print(precision, "fused", e_fu, "unfused", e_un, flush=True)
+        assert e_fu < tol and e_un < tol
+        assert rel_l2(out["fused"][i], out["unfused"][i]) < 1e-5
