@@ -268,7 +268,12 @@ def main():
     tj = None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_rl_full.json")
     if os.path.exists(tpath):
-        tj = json.load(open(tpath))
+        try:
+            tj = json.load(open(tpath))
+        except (OSError, ValueError) as e:
+            log(f"traffic summary unreadable ({e}); roofline.traffic = null")
+            tj = None
+    if tj is not None:
         # the counters belong to one plan: same precision, method, workload and spectra size
         same_plan = abs(float(tj.get("spectra_gb", -1)) - hr["spectra_gb"]) < 0.01
         if not (tj.get("precision") == head and tj.get("method") == args.method and wl.name == tj.get("workload")
@@ -276,7 +281,8 @@ def main():
             tj = None
     if t_gemm >= t_fftp:
         gk = f"sid::k_proj_gemm<15, 2, {({'fp32': 0, 'bf16x2': 1, 'bf16x3': 2}).get(head, 0)}>"
-        traffic = (tj["kernels"][gk]["read_bytes"] + tj["kernels"][gk]["write_bytes"]) if tj and gk in tj["kernels"] else None
+        kk = tj.get("kernels", {}) if tj else {}
+        traffic = (kk[gk]["read_bytes"] + kk[gk]["write_bytes"]) if gk in kk else None
         roof = {"bound": "mfma", "achieved": round(hr["gemm_achieved_tflops"], 3), "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": round(hr["gemm_frac_of_peak"], 4), "traffic": traffic,
                 "kernel": "k_proj_gemm (forward and backward launches, direct planes)",
@@ -284,9 +290,12 @@ def main():
     else:
         traffic = None
         if tj:
-            kk = tj["kernels"]
-            traffic = 0.5 * sum(kk[k]["read_bytes"] + kk[k]["write_bytes"]
-                                for k in ("sid::k_fft_gemv_fwd", "sid::k_fft_gemv_bwd"))
+            # the GEMV instantiation of this precision's spectra (FP16 pairs in BF16X2, FP32 otherwise)
+            inst = "<__half2>" if head == "bf16x2" else "<HIP_vector_type<float, 2u> >"
+            kk = tj.get("kernels", {})
+            names = [f"sid::k_fft_gemv_{d}{inst}" for d in ("fwd", "bwd")]
+            if all(k in kk for k in names):
+                traffic = 0.5 * sum(kk[k]["read_bytes"] + kk[k]["write_bytes"] for k in names)
         roof = {"bound": "hbm", "achieved": round(hr["gemv_stream_tbps"] * 1e3, 1), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(hr["gemv_frac_of_hbm"], 4),
                 "traffic": traffic, "traffic_source": "profiles/pmc_traffic_rl_full_bf16x2.md" if traffic else None,

@@ -291,51 +291,92 @@ __global__ void k_fft_pack_vt(const float2 *__restrict__ subs, const float *__re
 __device__ __forceinline__ float2 spec_ld(const float2 *p) { return *p; }
 __device__ __forceinline__ float2 spec_ld(const __half2 *p) { return __half22float2(*p); }
 
+// Loads of one tile of nel elements at src (16-byte aligned) into registers:
+// vector u of this thread is element range [(u * 256 + tid) * EPV, +EPV);
+// vectors past the tile re-read its first vector (always inside the spectra
+// buffer, whose k0 blocks hold at least 16 bytes) and are not stored.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename GT, int kVec>
+__device__ __forceinline__ void tile_fetch(u32x4 (&pre)[kVec], GT &tail, const GT *__restrict__ src, int nel)
+{
+    constexpr int EPV = 16 / (int)sizeof(GT);
+    const int nv = nel / EPV;
+    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+#pragma unroll
+    for (int u = 0; u < kVec; ++u) {
+        const int e = u * 256 + (int)threadIdx.x;
+        pre[u] = s4[e < nv ? e : 0];
+    }
+    const int et = nv * EPV + (int)threadIdx.x;
+    if (et < nel) tail = src[et];
+}
+
 // Forward GEMV: oh[s][k0][j] = sum_{q in chunk s} G[k0][q][j] * vt[k0][q];
 // block (k0, s), thread j.  The q range of a k0 is split over gridDim.y
 // blocks so the grid is many times the resident capacity (no half-empty last
-// round); k_fft_assemble adds the chunks in a fixed order.  vt[k0][q] is the
-// same for every lane (a broadcast read through the cache), so the block needs
-// no LDS and many blocks share a CU to keep enough HBM reads in flight.  A
-// wave reads 64 consecutive elements of a row per load (512 B for FP32, 256 B
-// for FP16 pairs), which measures within a few percent of a plain streaming
-// read on this device.
+// round); k_fft_oh_sum adds the chunks in a fixed order.  The chunk is one
+// contiguous stretch of HBM, streamed in tiles of 16 rows with 16-byte loads
+// through registers (the next tile's loads are in flight while the current
+// one is summed from LDS); reading rows directly (64 consecutive elements per
+// wave) fetches every cache line that straddles two waves' pieces twice.
+// vt[k0][q] is the same for every lane (scalar loads).
 template <typename GT>
 __global__ __launch_bounds__(256) void k_fft_gemv_fwd(const GT *__restrict__ G, const float2 *__restrict__ vt,
                                                       int nq, int n2, int qc, int nk0, int64_t kstride,
                                                       float2 *__restrict__ oh)
 {
+    constexpr int TQ = 16;
+    constexpr int EPV = 16 / (int)sizeof(GT);
+    constexpr int kVec = TQ * 256 / EPV / 256;
+    __shared__ u32x4 tile[TQ * 256 / EPV];
     const int k0 = blockIdx.x;
     const int j = threadIdx.x;
-    if (j >= n2) return;
     const int qb = blockIdx.y * qc, qe = min(nq, qb + qc);
-    const GT *g = G + (int64_t)k0 * kstride + j;
+    const GT *gk = G + (int64_t)k0 * kstride;
     const float2 *v = vt + (int64_t)k0 * nq;
-    float2 acc[8];
+    const GT *ts = reinterpret_cast<const GT *>(tile);
+    u32x4 pre[kVec];
+    GT pre_tail = GT();
+    float2 acc0 = make_float2(0.f, 0.f), acc1 = make_float2(0.f, 0.f);
+    if (qb < qe) tile_fetch<GT, kVec>(pre, pre_tail, gk + (int64_t)qb * n2, min(TQ, qe - qb) * n2);
+    for (int q0 = qb; q0 < qe; q0 += TQ) {
+        const int nr = min(TQ, qe - q0);
+        __syncthreads();   // previous tile consumed
+        {
+            const int nel = nr * n2, nv = nel / EPV;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc[u] = make_float2(0.f, 0.f);
-    int q = qb;
-    for (; q + 8 <= qe; q += 8) {
-        float2 gg[8], vv[8];
+            for (int u = 0; u < kVec; ++u) {
+                const int e = u * 256 + (int)threadIdx.x;
+                if (e < nv) tile[e] = pre[u];
+            }
+            const int et = nv * EPV + (int)threadIdx.x;
+            if (et < nel) reinterpret_cast<GT *>(tile)[et] = pre_tail;
+        }
+        __syncthreads();
+        if (q0 + TQ < qe)
+            tile_fetch<GT, kVec>(pre, pre_tail, gk + (int64_t)(q0 + TQ) * n2, min(TQ, qe - q0 - TQ) * n2);
+        if (j < n2) {
+            if (nr == TQ) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) gg[u] = spec_ld(g + (int64_t)(q + u) * n2);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) vv[u] = v[q + u];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            acc[u].x += gg[u].x * vv[u].x - gg[u].y * vv[u].y;
-            acc[u].y += gg[u].x * vv[u].y + gg[u].y * vv[u].x;
+                for (int r = 0; r < TQ; r += 2) {
+                    const float2 g0 = spec_ld(ts + r * n2 + j), v0 = v[q0 + r];
+                    const float2 g1 = spec_ld(ts + (r + 1) * n2 + j), v1 = v[q0 + r + 1];
+                    acc0.x += g0.x * v0.x - g0.y * v0.y;
+                    acc0.y += g0.x * v0.y + g0.y * v0.x;
+                    acc1.x += g1.x * v1.x - g1.y * v1.y;
+                    acc1.y += g1.x * v1.y + g1.y * v1.x;
+                }
+            } else {
+                for (int r = 0; r < nr; ++r) {
+                    const float2 g0 = spec_ld(ts + r * n2 + j), v0 = v[q0 + r];
+                    acc0.x += g0.x * v0.x - g0.y * v0.y;
+                    acc0.y += g0.x * v0.y + g0.y * v0.x;
+                }
+            }
         }
     }
-    for (; q < qe; ++q) {
-        const float2 g0 = spec_ld(g + (int64_t)q * n2), v0 = v[q];
-        acc[0].x += g0.x * v0.x - g0.y * v0.y;
-        acc[0].y += g0.x * v0.y + g0.y * v0.x;
-    }
-    float2 r = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int u = 0; u < 8; ++u) { r.x += acc[u].x; r.y += acc[u].y; }
-    oh[((int64_t)blockIdx.y * nk0 + k0) * n2 + j] = r;
+    if (j < n2) oh[((int64_t)blockIdx.y * nk0 + k0) * n2 + j] = make_float2(acc0.x + acc1.x, acc0.y + acc1.y);
 }
 
 // Chunk sums of the forward GEMV, transposed: ohT[j][k0] = sum_s oh[s][k0][j].
@@ -403,27 +444,6 @@ __global__ void k_fft_img_grid(const float *__restrict__ img, int Nx, int Ny, in
             v = img[((int64_t)(x / n + T) * YP + (y / n + T)) * S + (x % n) + n * (y % n)];
         grid[e] = make_float2(v, 0.0f);
     }
-}
-
-// Loads of one tile of nel elements at src (16-byte aligned) into registers:
-// vector u of this thread is element range [(u * 256 + tid) * EPV, +EPV);
-// vectors past the tile re-read its first vector (always inside the spectra
-// buffer, whose k0 blocks hold at least 16 bytes) and are not stored.
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-template <typename GT, int kVec>
-__device__ __forceinline__ void tile_fetch(u32x4 (&pre)[kVec], GT &tail, const GT *__restrict__ src, int nel)
-{
-    constexpr int EPV = 16 / (int)sizeof(GT);
-    const int nv = nel / EPV;
-    const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
-#pragma unroll
-    for (int u = 0; u < kVec; ++u) {
-        const int e = u * 256 + (int)threadIdx.x;
-        pre[u] = s4[e < nv ? e : 0];
-    }
-    const int et = nv * EPV + (int)threadIdx.x;
-    if (et < nel) tail = src[et];
 }
 
 // Backward GEMV: bt[k0][q] = sum_j conj(G[k0][q][j]) * W(k0 + N' j).  Block
