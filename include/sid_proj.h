@@ -177,6 +177,22 @@ int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *box
 /* {rows, work items, MFMA multiply-adds, accumulator elements} of the cached library plan. */
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
 
+/* --- Neuron kernel (reconstruct_S.m:110-128,184-190; generate_kernel.m) ----
+ * With a kernel k set, the handle's projectors become
+ *   forwardFUN(V) = F(convn(V, k, 'same')),  backwardFUN(O) = convn(B(O), k, 'same')
+ * for sid_forward*, sid_backward*, sid_deconv_rl* and sid_ls_deconv* (the
+ * wrapping reconstruct_S builds around forwardProjectGPU / backwardProjectGPU_new).
+ * k: column-major [kx, ky, kz] host array, 1 <= kx, ky <= 65, 1 <= kz <= 255;
+ * k == NULL removes the kernel.  Gradients and the batched library reject a
+ * handle with a kernel (convolve seed volumes first with sid_convn_same_device). */
+int sid_psf_set_kernel(sid_psf *psf, const float *k, const int64_t kdims[3]);
+/* convn(V, k, 'same') in FP32 on column-major volumes (MATLAB's window: offset
+ * floor(size(k)/2) per dimension, zeros outside V).  _device: V, k, out device
+ * pointers on `stream`; the plain variant takes host arrays (device 0). */
+int sid_convn_same_device(const float *V_dev, const int64_t vdims[3], const float *k_dev, const int64_t kdims[3],
+                          float *out_dev, void *stream);
+int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out);
+
 /* --- LS_deconv (external/LFrecon_util/LS_deconv.m:1-97) ------------------
  * Projected gradient descent with exact line search for
  *   argmin_{x >= 0} ||y - A x||^2 + TV_L2 + lamb_L2 ||x||^2 + lamb_L1 ||x||_1,

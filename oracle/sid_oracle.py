@@ -483,6 +483,31 @@ def poisson_nll_grad_psf(H, V, y, CAindex=None):
     return G
 
 
+def convn_same(V, k):
+    """MATLAB convn(V, k, 'same') (the neuron-kernel wrapping of
+    external/LFrecon_util/reconstruct_S.m:126-127,188-189): the full
+    N-d convolution cropped to V's size starting at floor(size(k)/2) in every
+    dimension, zeros outside V.  Direct sum over the kernel taps (FP64)."""
+    V = np.asarray(V, dtype=np.float64)
+    k = np.asarray(k, dtype=np.float64)
+    if V.ndim == 2:
+        V = V[:, :, None]
+    if k.ndim == 2:
+        k = k[:, :, None]
+    off = [s // 2 for s in k.shape]
+    out = np.zeros_like(V)
+    # out[p] = sum_t V[p + off - t] k[t]
+    for t in np.ndindex(*k.shape):
+        w = k[t]
+        if w == 0:
+            continue
+        sh = [off[d] - t[d] for d in range(3)]
+        src = tuple(slice(max(0, s), V.shape[d] + min(0, s)) for d, s in enumerate(sh))
+        dst = tuple(slice(max(0, -s), V.shape[d] - max(0, s)) for d, s in enumerate(sh))
+        out[dst] += w * V[src]
+    return out
+
+
 def tv_l2_stencil(X, lamb_TV_L2):
     """convn(X, filter, 'same') with the 3x3x3 filter of
     external/LFrecon_util/LS_deconv.m:54-58: lamb(1) on the two neighbours

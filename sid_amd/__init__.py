@@ -31,7 +31,7 @@ from . import _lib
 from ._lib import PREC, METHOD, SidError, check, i64, lib
 
 __all__ = ["PSF", "forwardProjectGPU", "backwardProjectGPU_new", "forwardProjectACC",
-           "backwardProjectACC_new", "deconvRL", "LS_deconv", "SidError", "device_count", "selftest"]
+           "backwardProjectACC_new", "deconvRL", "LS_deconv", "convn_same", "SidError", "device_count", "selftest"]
 
 
 def device_count() -> int:
@@ -90,6 +90,7 @@ class PSF:
         self.dims = tuple(int(d) for d in dims)
         self.M, _, self.Nnum, _, self.Nz = self.dims
         self.precision = "fp32"
+        self.kernel = None
         self.set_precision(precision)
         if method != "auto":
             self.set_method(method)
@@ -229,6 +230,22 @@ class PSF:
                                   i64(Xs, Y, nz), ctypes.byref(o)))
         return X
 
+    def set_kernel(self, kernel):
+        """Neuron kernel of reconstruct_S.m:122-128,184-190: afterwards every
+        projection is F(convn(V, k, 'same')) / convn(B(O), k, 'same').
+        ``kernel`` is indexed like MATLAB, shape (kx, ky, kz) (a 2-D array is
+        one plane); None removes it."""
+        if kernel is None:
+            check(lib().sid_psf_set_kernel(self._h, None, None))
+            self.kernel = None
+            return
+        k = np.asarray(kernel, dtype=np.float32)
+        if k.ndim == 2:
+            k = k[:, :, None]
+        k = np.asfortranarray(k)
+        check(lib().sid_psf_set_kernel(self._h, k.ctypes.data_as(ctypes.c_void_p), i64(*k.shape)))
+        self.kernel = k
+
     def forward_fun(self):
         f = lambda V: self.forward(V)
         f._sid_psf = self
@@ -242,6 +259,38 @@ class PSF:
         return f
 
 
+def convn_same(V, k):
+    """convn(V, k, 'same') in FP32 on the device (MATLAB window: offset
+    floor(size(k)/2)).  numpy arrays indexed like MATLAB ((X, Y, Z) and
+    (kx, ky, kz)), or device tensors in the reversed shape ((Z, Y, X),
+    (kz, ky, kx)); results come back in the same kind."""
+    if _is_torch(V):
+        torch = _torch()
+        v = V.detach().to(torch.float32).contiguous()
+        kk = k.detach().to(device=v.device, dtype=torch.float32).contiguous() if _is_torch(k) else \
+            torch.from_numpy(np.ascontiguousarray(np.asarray(k, dtype=np.float32).T)).to(v.device)
+        v3 = v if v.dim() == 3 else v[None]
+        k3 = kk if kk.dim() == 3 else kk[None]
+        out = torch.empty_like(v3)
+        check(lib().sid_convn_same_device(ctypes.c_void_p(v3.data_ptr()), i64(*reversed(v3.shape)),
+                                          ctypes.c_void_p(k3.data_ptr()), i64(*reversed(k3.shape)),
+                                          ctypes.c_void_p(out.data_ptr()), _stream_ptr()))
+        return out if v.dim() == 3 else out[0]
+    v = np.asarray(V, dtype=np.float32)
+    squeeze = v.ndim == 2
+    if squeeze:
+        v = v[:, :, None]
+    kk = np.asarray(k, dtype=np.float32)
+    if kk.ndim == 2:
+        kk = kk[:, :, None]
+    v = np.asfortranarray(v)
+    kk = np.asfortranarray(kk)
+    out = np.zeros(v.shape, dtype=np.float32, order="F")
+    check(lib().sid_convn_same(v.ctypes.data_as(ctypes.c_void_p), i64(*v.shape), kk.ctypes.data_as(ctypes.c_void_p),
+                               i64(*kk.shape), out.ctypes.data_as(ctypes.c_void_p)))
+    return out[:, :, 0] if squeeze else out
+
+
 def _ls_opts(opts) -> "_lib.LsOpts":
     """Defaults of LS_deconv.m:25-43; lamb_TV_L2 needs all three entries, as
     the filter construction at LS_deconv.m:54-57 indexes (1)..(3)."""
@@ -250,6 +299,8 @@ def _ls_opts(opts) -> "_lib.LsOpts":
     o.max_iter = int(opts.get("max_iter", 8))
     o.lamb_TV_L1 = float(opts.get("lamb_TV_L1", 0.0))
     tv = np.asarray(opts.get("lamb_TV_L2", (0.0, 0.0, 0.0)), dtype=np.float64).reshape(-1)
+    if tv.size == 1 and tv[0] == 0:
+        tv = np.zeros(3)   # a scalar 0 (the config default) leaves the TV term off, as in MATLAB
     if tv.size != 3:
         raise ValueError("lamb_TV_L2 needs one multiplier per volume dimension")
     for i in range(3):

@@ -142,7 +142,7 @@ static void destroy(sid_psf *p)
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
                     (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
-                    (void *)p->wf_d, (void *)p->wb_d})
+                    (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork})
         if (q) (void)hipFree(q);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -268,6 +268,7 @@ int sid_poisson_nll_grad_device(sid_psf *p, const float *V, const float *y, cons
         set_error("the likelihood needs volumes with size(H,5) = %lld planes", (long long)p->nzH);
         return SID_ERR_NZ;
     }
+    if (p->kern) { set_error("gradients are not available on a handle with a neuron kernel"); return SID_ERR_ARG; }
     SID_TRY_HIP(hipSetDevice(p->device));
     return grad_core(p, V, y, dims, 0, nll, gV, gH, pick_stream(p, stream));
 }
@@ -284,6 +285,7 @@ int sid_project_vjp_device(sid_psf *p, const float *V, const float *G, const int
         set_error("the projector gradient needs volumes with size(H,5) = %lld planes", (long long)p->nzH);
         return SID_ERR_NZ;
     }
+    if (p->kern) { set_error("gradients are not available on a handle with a neuron kernel"); return SID_ERR_ARG; }
     SID_TRY_HIP(hipSetDevice(p->device));
     return grad_core(p, V, G, dims, 1, nullptr, gV, gH, pick_stream(p, stream));
 }
@@ -361,7 +363,7 @@ static int check_vdims(sid_psf *p, const int64_t vdims[3], bool rl)
     return SID_OK;
 }
 
-static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
+static int forward_raw(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
 {
     SID_TRY(autotune_methods(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
@@ -384,7 +386,7 @@ static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], 
     return SID_OK;
 }
 
-static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st)
+static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st)
 {
     // the backward volume always has all planes of H (backwardProjectGPU_new.m:4-5); the
     // sensor plan keeps the nz of the last forward call.
@@ -406,6 +408,39 @@ static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2],
     tm.span(3, e2);
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e3);
+    return SID_OK;
+}
+
+// Neuron-kernel staging buffer of at least n floats (kept on the handle).
+static int kwork_reserve(sid_psf *p, int64_t n)
+{
+    if (p->kwork_elems >= n) return SID_OK;
+    if (p->kwork) (void)hipFree(p->kwork);
+    p->kwork = nullptr;
+    p->kwork_elems = 0;
+    SID_TRY_HIP(hipMalloc(&p->kwork, sizeof(float) * n));
+    p->kwork_elems = n;
+    return SID_OK;
+}
+
+// forwardFUN = F(convn(V, k, 'same')) when the handle carries a neuron kernel (reconstruct_S.m:126,188)
+static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
+{
+    if (!p->kern) return forward_raw(p, V_dev, vdims, O_dev, st);
+    SID_TRY(kwork_reserve(p, vdims[0] * vdims[1] * vdims[2]));
+    SID_TRY(convn_same(V_dev, vdims, p->kern, p->kdims, p->kwork, st));
+    return forward_raw(p, p->kwork, vdims, O_dev, st);
+}
+
+// backwardFUN = convn(B(O), k, 'same') (reconstruct_S.m:127,189)
+static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st)
+{
+    if (!p->kern) return backward_raw(p, O_dev, odims, B_dev, st);
+    const int64_t vdims[3] = {odims[0], odims[1], p->nzH};
+    SID_TRY(kwork_reserve(p, vdims[0] * vdims[1] * vdims[2]));
+    SID_TRY(backward_raw(p, O_dev, odims, p->kwork, st));
+    SID_TRY(convn_same(p->kwork, vdims, p->kern, p->kdims, B_dev, st));
+    SID_TRY_HIP(hipStreamSynchronize(st));
     return SID_OK;
 }
 
@@ -469,6 +504,7 @@ int sid_backward(sid_psf *p, const float *O, const int64_t odims[2], float *B)
 static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist,
                    hipStream_t st)
 {
+    if (p->kern) return rl_wrapped(p, Htf, X, vdims, max_iter, hist, st);
     SID_TRY(autotune_methods(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     SensorPlan &sp = p->plan;
@@ -573,6 +609,11 @@ int sid_library_forward_device(sid_psf *p, int64_t ns, const int64_t *boxes, con
         return SID_ERR_ARG;
     }
     if (ns == 0) return SID_OK;
+    if (p->kern) {
+        set_error("the batched library takes seed volumes already convolved with the neuron kernel "
+                  "(convolve with sid_convn_same_device, then project on a handle without a kernel)");
+        return SID_ERR_ARG;
+    }
     SID_TRY(check_boxes(p, ns, boxes, sensor));
     for (int64_t s = 0; s < ns; ++s) {
         const int64_t *b = boxes + 6 * s;
@@ -644,6 +685,65 @@ int sid_ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3],
     SID_TRY(ls_deconv(p, (const float *)dh.ptr, (float *)dx.ptr, vdims, o, p->stream));
     SID_TRY_HIP(hipMemcpyAsync(X, dx.ptr, nv * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    return SID_OK;
+}
+
+static int check_kdims(const int64_t kd[3])
+{
+    if (kd[0] < 1 || kd[1] < 1 || kd[2] < 1 || kd[0] > kConvnMaxXY || kd[1] > kConvnMaxXY || kd[2] > kConvnMaxZ) {
+        set_error("neuron kernel %lld x %lld x %lld outside 1..%d x 1..%d x 1..%d", (long long)kd[0],
+                  (long long)kd[1], (long long)kd[2], kConvnMaxXY, kConvnMaxXY, kConvnMaxZ);
+        return SID_ERR_DIMS;
+    }
+    return SID_OK;
+}
+
+int sid_psf_set_kernel(sid_psf *p, const float *k, const int64_t kdims[3])
+{
+    if (!p) { set_error("null handle"); return SID_ERR_ARG; }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    if (p->stream) SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    if (p->kern) (void)hipFree(p->kern);
+    p->kern = nullptr;
+    p->kdims[0] = p->kdims[1] = p->kdims[2] = 0;
+    if (!k) return SID_OK;
+    if (!kdims) { set_error("sid_psf_set_kernel: null kdims"); return SID_ERR_ARG; }
+    SID_TRY(check_kdims(kdims));
+    const int64_t nk = kdims[0] * kdims[1] * kdims[2];
+    SID_TRY_HIP(hipMalloc(&p->kern, sizeof(float) * nk));
+    SID_TRY_HIP(hipMemcpy(p->kern, k, sizeof(float) * nk, hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; ++d) p->kdims[d] = (int)kdims[d];
+    return SID_OK;
+}
+
+int sid_convn_same_device(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out,
+                          void *stream)
+{
+    if (!V || !vdims || !k || !kdims || !out) { set_error("sid_convn_same_device: null argument"); return SID_ERR_ARG; }
+    SID_TRY(check_kdims(kdims));
+    SID_TRY(check_device());
+    const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
+    SID_TRY(convn_same(V, vdims, k, kd, out, (hipStream_t)stream));
+    SID_TRY_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return SID_OK;
+}
+
+int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out)
+{
+    if (!V || !vdims || !k || !kdims || !out) { set_error("sid_convn_same: null argument"); return SID_ERR_ARG; }
+    SID_TRY(check_kdims(kdims));
+    if (vdims[0] < 1 || vdims[1] < 1 || vdims[2] < 1) { set_error("convn: dimensions must be positive"); return SID_ERR_DIMS; }
+    SID_TRY(check_device());
+    const int64_t nv = vdims[0] * vdims[1] * vdims[2], nk = kdims[0] * kdims[1] * kdims[2];
+    DevBuf dv, dk, dout;
+    SID_TRY_HIP(hipMalloc(&dv.ptr, sizeof(float) * nv));
+    SID_TRY_HIP(hipMalloc(&dk.ptr, sizeof(float) * nk));
+    SID_TRY_HIP(hipMalloc(&dout.ptr, sizeof(float) * nv));
+    SID_TRY_HIP(hipMemcpy(dv.ptr, V, sizeof(float) * nv, hipMemcpyHostToDevice));
+    SID_TRY_HIP(hipMemcpy(dk.ptr, k, sizeof(float) * nk, hipMemcpyHostToDevice));
+    const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
+    SID_TRY(convn_same((const float *)dv.ptr, vdims, (const float *)dk.ptr, kd, (float *)dout.ptr, nullptr));
+    SID_TRY_HIP(hipMemcpy(out, dout.ptr, sizeof(float) * nv, hipMemcpyDeviceToHost));
     return SID_OK;
 }
 

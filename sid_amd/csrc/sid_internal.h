@@ -158,6 +158,10 @@ struct sid_psf {
     double auto_ms[4] = {0, 0, 0, 0};        // direct ms, fft ms (all eligible planes), n eligible, n chosen
     sid::SensorPlan plan;
     void *lib = nullptr;              // cached library plan (sid_library.hip)
+    float *kern = nullptr;            // neuron kernel (device, [kx][ky][kz]); null = none
+    int kdims[3] = {0, 0, 0};
+    float *kwork = nullptr;           // convn staging volume (column-major), kwork_elems floats
+    int64_t kwork_elems = 0;
     hipEvent_t ev[10] = {};
     double timing[12] = {0};
 };
@@ -206,6 +210,13 @@ namespace sid {
 // sid_api.hip: projector passes on column-major device buffers (used by the solvers)
 int forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st);
 int backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st);
+// sid_convn.hip: convn(in, k, 'same') on column-major device volumes (k on the device, [kx][ky][kz])
+constexpr int kConvnMaxXY = 65, kConvnMaxZ = 255;
+int convn_same(const float *in, const int64_t vdims[3], const float *k_dev, const int kd[3], float *out,
+               hipStream_t st);
 // sid_solvers.hip
 int ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts &o, hipStream_t st);
+// deconvRL basic on column-major device buffers through forward_dev / backward_dev (neuron-kernel handles)
+int rl_wrapped(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist,
+               hipStream_t st);
 }  // namespace sid

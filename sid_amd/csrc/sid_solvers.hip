@@ -219,4 +219,40 @@ int ls_deconv(sid_psf *p, const float *df, float *Xout, const int64_t vdims[3], 
     return SID_OK;
 }
 
+// ISRA update of deconvRL.m:85-87 on column-major buffers:
+// R = Htf ./ D; R(isinf(R)) = 0; X = X .* R; X(isnan(X)) = 0.
+__global__ void k_rl_update(float *__restrict__ X, const float *__restrict__ htf, const float *__restrict__ D,
+                            int64_t n)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        float r = htf[e] / D[e];
+        if (isinf(r)) r = 0.0f;
+        float v = X[e] * r;
+        if (isnan(v)) v = 0.0f;
+        X[e] = v;
+    }
+}
+
+// deconvRL basic (lambda = 0) with the handle's wrapped projectors
+// (forwardFUN = F(convn(X, k)), backwardFUN = convn(B(y), k)): deconvRL.m:46-52,85-92.
+int rl_wrapped(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist,
+               hipStream_t st)
+{
+    const int64_t nv = vdims[0] * vdims[1] * vdims[2], ni = vdims[0] * vdims[1];
+    const int64_t odims[2] = {vdims[0], vdims[1]};
+    Bufs b;
+    float *img, *D;
+    SID_TRY(b.alloc(&img, ni));
+    SID_TRY(b.alloc(&D, nv));
+    for (int it = 0; it < max_iter; ++it) {
+        SID_TRY(forward_dev(p, X, vdims, img, st));
+        SID_TRY(backward_dev(p, img, odims, D, st));
+        hipLaunchKernelGGL(k_rl_update, dim3(nblk(nv)), dim3(256), 0, st, X, Htf, D, nv);
+        SID_TRY_HIP(hipGetLastError());
+        if (hist) SID_TRY_HIP(hipMemcpyAsync(hist + (int64_t)it * nv, X, sizeof(float) * nv, hipMemcpyDeviceToDevice, st));
+    }
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    return SID_OK;
+}
+
 }  // namespace sid

@@ -1,0 +1,132 @@
+"""Neuron-kernel wrapping (reconstruct_S.m:122-128,184-190): the device
+convn(V, k, 'same') against the oracle, and the wrapped projectors, ISRA and
+LS_deconv against the oracle composed the same way
+(F(convn(X, k)), convn(B(y), k))."""
+import numpy as np
+import pytest
+
+from oracle import sid_oracle as O
+from tests._helpers import c_backward, c_forward, random_psf, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sid():
+    import sid_amd
+    if sid_amd.device_count() < 1:
+        pytest.fail("no gfx950 device visible")
+    return sid_amd
+
+
+CONV_CASES = [
+    ((40, 35, 6), (3, 3, 3)),
+    ((70, 19, 4), (17, 17, 5)),      # the find_kernel-sized neuron, wider than one 64-px tile edge
+    ((33, 41, 3), (4, 2, 6)),        # even sides, kernel deeper than the volume
+    ((20, 20, 1), (5, 7, 1)),        # one plane
+    ((131, 17, 2), (65, 1, 1)),      # the largest lateral side
+]
+
+
+@pytest.mark.parametrize("vd,kd", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_convn_same_vs_oracle(sid, vd, kd):
+    import torch
+    print("case: convn", vd, kd, flush=True)
+    rng = np.random.default_rng(sum(vd) * 7 + sum(kd))
+    V = rng.random(vd).astype(np.float32)
+    k = rng.random(kd).astype(np.float32)
+    ref = O.convn_same(V.astype(np.float64), k.astype(np.float64))
+    got = sid.convn_same(V, k)
+    e = rel_l2(got, ref)
+    print("convn rel-L2", e, flush=True)
+    assert e < 1e-6
+    # device tensors in the reversed shape give the same values
+    Vt = torch.from_numpy(np.ascontiguousarray(V.transpose(2, 1, 0))).cuda()
+    kt = torch.from_numpy(np.ascontiguousarray(k.transpose(2, 1, 0))).cuda()
+    gt = sid.convn_same(Vt, kt).cpu().numpy().transpose(2, 1, 0)
+    assert np.array_equal(gt, got)
+
+
+def _wrapped_case(seed=5, n=5, X=40, Y=35, M=15, nz=4, kd=(5, 5, 3)):
+    rng = np.random.default_rng(seed)
+    H = random_psf(rng, M, n, nz).astype(np.float32).astype(np.float64)
+    k = rng.random(kd)
+    k = (k / np.linalg.norm(k)).astype(np.float32).astype(np.float64)
+    F = lambda v: c_forward(H, O.convn_same(v, k))
+    B = lambda o: O.convn_same(c_backward(H, o), k)
+    return rng, H, k, F, B
+
+
+def test_wrapped_projectors_vs_oracle(sid):
+    print("case: wrapped projectors n5 X40 Y35 M15 Nz4 k5x5x3", flush=True)
+    rng, H, k, F, B = _wrapped_case()
+    psf = sid.PSF(H.astype(np.float32))
+    psf.set_kernel(k.astype(np.float32))
+    V = rng.random((40, 35, 4))
+    W = rng.random((40, 35))
+    e_f = rel_l2(psf.forward(V.astype(np.float32)), F(V))
+    e_b = rel_l2(psf.backward(W.astype(np.float32)), B(W))
+    print("wrapped forward", e_f, "backward", e_b, flush=True)
+    assert e_f < 5e-6 and e_b < 5e-6
+    # removing the kernel restores the plain projectors
+    psf.set_kernel(None)
+    assert rel_l2(psf.forward(V.astype(np.float32)), c_forward(H, V)) < 5e-6
+
+
+def test_wrapped_isra_and_ls_vs_oracle(sid):
+    print("case: wrapped ISRA / LS n5 X40 Y35 M15 Nz4 k5x5x3", flush=True)
+    rng, H, k, F, B = _wrapped_case(seed=8)
+    psf = sid.PSF(H.astype(np.float32))
+    psf.set_kernel(k.astype(np.float32))
+    y = F(rng.random((40, 35, 4)))
+    htf = B(y).astype(np.float32).astype(np.float64)
+    X_ref = O.deconv_rl(F, B, htf, 4, htf)
+    X, hist = psf.deconv_rl(htf.astype(np.float32), 4, htf.astype(np.float32), history=True)
+    e_r = rel_l2(X, X_ref)
+    assert np.array_equal(hist[-1], X)
+    X_ls_ref = O.ls_deconv(F, B, htf, max_iter=3, lamb_L1=0.01)
+    X_ls = psf.ls_deconv(htf.astype(np.float32), {"max_iter": 3, "lamb_L1": 0.01})
+    e_l = rel_l2(X_ls, X_ls_ref)
+    print("wrapped isra", e_r, "ls", e_l, flush=True)
+    assert e_r < 2e-5 and e_l < 1e-4
+
+
+def test_reconstruct_with_kernel_and_solvers(sid):
+    """reconstruct_S GPU branch: Htf = B(img/nrm) (wrapped), lamb_L1 clamp to
+    the 1 - 1e-4 quantile of Htf, ISRA or fast_nnls, times nrm."""
+    from sid_amd import recon
+    print("case: reconstruct ball kernel n5 X40 Y35 M15 Nz4", flush=True)
+    rng = np.random.default_rng(12)
+    H = random_psf(rng, 15, 5, 4).astype(np.float32).astype(np.float64)
+    kern = recon.generate_kernel("ball", (2, 1)).astype(np.float64)
+    F = lambda v: c_forward(H, O.convn_same(v, kern))
+    B = lambda o: O.convn_same(c_backward(H, o), kern)
+    img = F(rng.random((40, 35, 4)))
+    y, nrm = recon.preprocess(img)
+    htf = B(y.astype(np.float32).astype(np.float64))
+    psf = sid.PSF(H.astype(np.float32))
+    opts = {"ker_shape": "ball", "ker_param": (2, 1)}
+    out = recon.reconstruct(psf, img, max_iter=3, opts=dict(opts, whichSolver="ISRA"))
+    ref = nrm * O.deconv_rl(F, B, htf, 3, htf)
+    e1 = rel_l2(out, ref)
+    lam = min(recon.quantile_type5(htf, 1 - 1e-4), 0.1)
+    out2 = recon.reconstruct(psf, img, max_iter=3, opts=dict(opts, whichSolver="fast_nnls", lamb_L1=0.1))
+    ref2 = nrm * O.ls_deconv(F, B, htf, max_iter=3, lamb_L1=lam)
+    e2 = rel_l2(out2, ref2)
+    print("reconstruct isra", e1, "fast_nnls", e2, flush=True)
+    assert e1 < 2e-5 and e2 < 1e-4
+
+
+def test_kernel_errors(sid):
+    rng = np.random.default_rng(1)
+    H = random_psf(rng, 7, 3, 2).astype(np.float32)
+    psf = sid.PSF(H)
+    with pytest.raises(sid.SidError, match="SID_ERR_DIMS"):
+        psf.set_kernel(np.ones((66, 3, 1), np.float32))
+    psf.set_kernel(np.ones((3, 3, 1), np.float32))
+    import torch
+    from sid_amd import grad
+    V = torch.rand((1, 2, 12, 12), device="cuda")
+    y = torch.rand((1, 12, 12), device="cuda")
+    with pytest.raises(sid.SidError, match="SID_ERR_ARG"):
+        grad.poisson_nll_grad(psf, V, y)

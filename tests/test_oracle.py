@@ -459,3 +459,64 @@ def test_ls_deconv_step_is_exact_line_search(terms):
     c1 = (m1 - m0) / s_ - c2 * s_
     a = -c1 / (2 * c2)
     np.testing.assert_allclose(X1, np.maximum(df - a * d, 0), rtol=1e-6, atol=1e-9 * np.abs(df).max())
+
+
+def test_convn_same_is_matlab_convn():
+    """convn(V, k, 'same'): scipy's N-d 'same' convolution for odd kernels;
+    for even sides MATLAB crops the full convolution from floor(size(k)/2)
+    (its documented conv([1 2 3], [1 1], 'same') = [3 5 3])."""
+    from scipy.signal import convolve
+    rng = np.random.default_rng(31)
+    V = rng.random((9, 8, 5))
+    for ks in [(3, 3, 3), (5, 1, 3), (1, 1, 1), (3, 5, 7), (17, 17, 5)]:
+        k = rng.random(ks)
+        np.testing.assert_allclose(O.convn_same(V, k), convolve(V, k, mode="same"), rtol=1e-12, atol=1e-12)
+    e = O.convn_same(np.array([1.0, 2.0, 3.0]).reshape(3, 1, 1), np.array([1.0, 1.0]).reshape(2, 1, 1))
+    np.testing.assert_array_equal(e.ravel(), [3.0, 5.0, 3.0])
+    k = rng.random((4, 2, 6))
+    full = convolve(V, k, mode="full")
+    o = [s // 2 for s in k.shape]
+    np.testing.assert_allclose(O.convn_same(V, k), full[o[0]:o[0] + 9, o[1]:o[1] + 8, o[2]:o[2] + 5],
+                               rtol=1e-12, atol=1e-12)
+
+
+def test_generate_kernel_known_answers():
+    """src/generate_kernel.m: fspecial('gaussian', 3, 0.5) is MATLAB's documented
+    [0.0113 0.0838 0.0113; 0.0838 0.6193 0.0838; ...]; a unit ball is the
+    6-neighbour cross; every kernel has unit L2 norm; the Gaussian kernel is the
+    lateral fspecial times exp(-z^2 / (4 p2^2)) on z = -3p2:3p2."""
+    from sid_amd.recon import fspecial_gaussian, generate_kernel
+    g = fspecial_gaussian(3, 0.5)
+    np.testing.assert_allclose(g, [[0.0113, 0.0838, 0.0113], [0.0838, 0.6193, 0.0838], [0.0113, 0.0838, 0.0113]],
+                               atol=5e-5)
+    b = generate_kernel("ball", (1, 1))
+    assert b.shape == (3, 3, 3)
+    cross = np.zeros((3, 3, 3))
+    cross[1, 1, :] = cross[1, :, 1] = cross[:, 1, 1] = 1
+    np.testing.assert_allclose(b, cross / math.sqrt(7), rtol=1e-6)
+    b2 = generate_kernel("ball", (8, 2))          # the find_kernel-sized neuron (17 x 17 x 5)
+    assert b2.shape == (17, 17, 5) and abs(np.linalg.norm(b2) - 1) < 1e-6
+    ga = generate_kernel("gaussian", (1.2, 1.0))
+    assert ga.shape == (13, 13, 7)
+    zf = np.exp(-np.arange(-3.0, 4.0) ** 2 / 4.0)
+    ref = fspecial_gaussian(13, 1.2)[:, :, None] * zf[None, None, :]
+    np.testing.assert_allclose(ga, ref / np.linalg.norm(ref), rtol=1e-6)
+    lo = generate_kernel("lorentz", (0.5, 0.4))
+    assert lo.shape == (7, 7, 5)
+    c = (3, 3, 2)
+    assert abs(lo[c] * np.linalg.norm(1.0 / (1 + 0.5 * (np.arange(-3, 4)[:, None, None] ** 2
+                                                        + np.arange(-3, 4)[None, :, None] ** 2)
+                                             + 0.4 * np.arange(-2, 3)[None, None, :] ** 2)) - 1.0) < 1e-6
+    u = generate_kernel("user", np.ones((2, 3)))
+    assert u.shape == (2, 3, 1) and abs(float(u.sum()) - math.sqrt(6)) < 1e-6
+
+
+def test_ls_options_accept_the_config_default_scalar_tv():
+    """sid_config_manage.m sets recon_opts.lamb_TV_L2 = 0 (a scalar); LS_deconv.m
+    accepts it because max(0) leaves the TV term off, but a nonzero scalar fails
+    at filter(2,2,[1 3]) = lamb_TV_L2(3)."""
+    import sid_amd
+    o = sid_amd._ls_opts({"lamb_TV_L2": 0})
+    assert list(o.lamb_TV_L2) == [0.0, 0.0, 0.0] and o.max_iter == 8
+    with pytest.raises(ValueError):
+        sid_amd._ls_opts({"lamb_TV_L2": 0.3})
