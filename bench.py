@@ -117,6 +117,72 @@ def library_bench(psf, wl_lib=None, steps=3):
     return res
 
 
+def rows_bench(psf, wl, htf, prec, iters, dev):
+    """The solver / wrapper / producer rows around the projector, each timed at
+    the full workload in the headline precision:
+    LS_deconv (reconstruct_S 'fast_nnls'), the neuron-kernel wrapping (a
+    find_kernel-sized 17 x 17 x 5 ball) and the movie std image (1005^2 px x
+    1000 frames, bg_iter 2, max-normalised)."""
+    import torch
+    import sid_amd
+    from sid_amd import movie, recon
+    out = {}
+    psf.set_precision(prec)
+    lam = min(recon.quantile_type5(htf.cpu().numpy(), 1 - 1e-4), 0.1)
+    opts = {"max_iter": iters, "lamb_L1": lam}
+    psf.ls_deconv(htf, opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psf.ls_deconv(htf, opts)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["ls_deconv"] = {"iter_per_s": round(iters / dt, 3), "ms_per_solve": round(dt * 1e3, 2), "max_iter": iters,
+                        "projections_per_iter": 3, "lamb_L1": lam}
+    log(f"LS_deconv ({prec}): {iters / dt:.2f} iter/s ({dt * 1e3:.0f} ms for {iters} iterations)")
+    k = recon.generate_kernel("ball", (8, 2))
+    kt = torch.from_numpy(np.ascontiguousarray(k.T)).to(dev)
+    vol = torch.rand((wl.nz, wl.Y, wl.X), device=dev)
+    sid_amd.convn_same(vol, kt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        sid_amd.convn_same(vol, kt)
+    torch.cuda.synchronize()
+    tc = (time.perf_counter() - t0) / reps
+    taps = int(np.prod(k.shape))
+    fl = 2.0 * wl.X * wl.Y * wl.nz * taps
+    psf.set_kernel(k)
+    psf.deconv_rl(htf, 1, htf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psf.deconv_rl(htf, iters, htf)
+    torch.cuda.synchronize()
+    dtw = time.perf_counter() - t0
+    psf.set_kernel(None)
+    out["neuron_kernel"] = {"kernel": list(k.shape), "convn_ms": round(tc * 1e3, 3),
+                            "convn_tflops": round(fl / tc / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
+                            "rl_iter_per_s_wrapped": round(iters / dtw, 3)}
+    log(f"neuron kernel {k.shape}: convn {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s FP32); "
+        f"wrapped RL {iters / dtw:.2f} iter/s")
+    del vol
+    P, T = wl.X * wl.Y, 1000
+    Y = torch.rand((T, P), device=dev)
+    movie.background_and_std(Y, bg_iter=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    movie.background_and_std(Y, bg_iter=2)
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    passes = 5   # max, 2 column passes, 2 row passes
+    out["std_image"] = {"pixels": P, "frames": T, "bg_iter": 2, "ms": round(tm * 1e3, 2),
+                        "stream_gbps": round(passes * P * T * 4 / tm / 1e9, 1), "passes": passes}
+    log(f"std image {P} px x {T} frames: {tm * 1e3:.1f} ms ({passes * P * T * 4 / tm / 1e9:.0f} GB/s over {passes} passes)")
+    del Y
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 error reference")
     ap.add_argument("--no-library", action="store_true", help="skip the batched-library measurement")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the LS_deconv / neuron-kernel / std-image measurements")
     ap.add_argument("--profile-copy", action="store_true",
                     help="run one 1 GiB device copy first (the reference kernel for scaling PMC byte counters)")
     ap.add_argument("--method", default="auto", choices=["auto", "direct", "fft"],
@@ -258,6 +326,10 @@ def main():
 
     if not args.no_library:
         out_lib = library_bench(psf, wl_lib=None, steps=max(args.steps, 3))
+    out_rows = None
+    if not args.no_rows:
+        head_prec = args.precisions.split(",")[-1]
+        out_rows = rows_bench(psf, wl, htfs[head_prec], head_prec, args.iters, dev)
     head = args.precisions.split(",")[-1]
     base = args.precisions.split(",")[0]
     hr = results[head]
@@ -327,6 +399,8 @@ def main():
     }
     if not args.no_library:
         out["library"] = out_lib
+    if out_rows:
+        out["rows"] = out_rows
     if not args.no_cpu:
         log("CPU baseline (bounded sample) ...")
         out["cpu_baseline"] = cpu_baseline(wl, V, y)

@@ -193,6 +193,23 @@ int sid_convn_same_device(const float *V_dev, const int64_t vdims[3], const floa
                           float *out_dev, void *stream);
 int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out);
 
+/* --- Movie background and std image (sid_main.m:88-103) ------------------
+ * Y: column-major movie [P pixels, T frames] (one frame per column), read only.
+ *   flags & 1: work on Y / max(Y(:)) (sid_main.m:88-89); *ymax receives the
+ *              maximum (0 when not normalising; ymax may be NULL)
+ *   bg_iter > 0: [s, t] = rank_1_factorization(Y, bg_iter)
+ *              (src/rank_1_factorization.m:31-39); bg_iter == 0: s = 0, t = 0
+ *              (bg_sub off, sid_main.m:94-98)
+ *   std = compute_std_image(Y, s, t) (src/compute_std_image.m:22-26); a
+ *              negative radicand (rounding) gives 0 where MATLAB returns a complex value
+ * Outputs: s [P], t [T], std [P] (FP32; s and t may be NULL).  FP64
+ * accumulation, deterministic.  _device: device pointers on `stream`; the plain
+ * variant takes host arrays (device 0). */
+int sid_std_image_device(const float *Y_dev, int64_t P, int64_t T, int bg_iter, int flags, float *s_dev,
+                         float *t_dev, float *std_dev, double *ymax, void *stream);
+int sid_std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t, float *std_img,
+                  double *ymax);
+
 /* --- LS_deconv (external/LFrecon_util/LS_deconv.m:1-97) ------------------
  * Projected gradient descent with exact line search for
  *   argmin_{x >= 0} ||y - A x||^2 + TV_L2 + lamb_L2 ||x||^2 + lamb_L1 ||x||_1,

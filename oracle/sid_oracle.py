@@ -483,6 +483,35 @@ def poisson_nll_grad_psf(H, V, y, CAindex=None):
     return G
 
 
+def rank_1_factorization(Y, max_iter=1):
+    """src/rank_1_factorization.m:31-39: alternating updates of
+    Y ~ s t with s = ones(P,1)/sqrt(P); t = s'Y / ||s'Y||; s = Y t'
+    (normalised between iterations, not after the last).  FP64."""
+    Y = np.asarray(Y, dtype=np.float64)
+    s = np.ones(Y.shape[0]) / np.sqrt(Y.shape[0])
+    t = np.zeros(Y.shape[1])
+    for it in range(1, int(max_iter) + 1):
+        t = s @ Y
+        t = t / np.linalg.norm(t)
+        s = Y @ t
+        if it < max_iter:
+            s = s / np.linalg.norm(s)
+    return s, t
+
+
+def compute_std_image(Y, y1=None, y2=None):
+    """src/compute_std_image.m:22-26: sqrt((A - y1^2)/T - B^2 + 2 B y1 C - y1^2 C^2)
+    with A = sum(Y.^2, 2), B = mean(Y, 2), C = mean(y2).  FP64."""
+    Y = np.asarray(Y, dtype=np.float64)
+    if y1 is None:
+        y1 = np.zeros(Y.shape[0])
+        y2 = np.zeros(Y.shape[1])
+    A = np.sum(Y * Y, axis=1)
+    B = np.mean(Y, axis=1)
+    C = np.mean(y2)
+    return np.sqrt((A - y1 ** 2) / len(y2) - B ** 2 + 2 * B * y1 * C - y1 ** 2 * C ** 2)
+
+
 def convn_same(V, k):
     """MATLAB convn(V, k, 'same') (the neuron-kernel wrapping of
     external/LFrecon_util/reconstruct_S.m:126-127,188-189): the full

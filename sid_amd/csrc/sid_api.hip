@@ -747,6 +747,37 @@ int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const
     return SID_OK;
 }
 
+int sid_std_image_device(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t,
+                         float *std_img, double *ymax, void *stream)
+{
+    if (!Y || !std_img) { set_error("sid_std_image_device: null argument"); return SID_ERR_ARG; }
+    if (P < 1 || T < 1 || T > (int64_t)1 << 30) { set_error("movie dimensions out of range"); return SID_ERR_DIMS; }
+    if (bg_iter < 0) { set_error("bg_iter must be >= 0"); return SID_ERR_ARG; }
+    SID_TRY(check_device());
+    return std_image(Y, P, T, bg_iter, flags, s, t, std_img, ymax, (hipStream_t)stream);
+}
+
+int sid_std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t, float *std_img,
+                  double *ymax)
+{
+    if (!Y || !std_img) { set_error("sid_std_image: null argument"); return SID_ERR_ARG; }
+    if (P < 1 || T < 1 || T > (int64_t)1 << 30) { set_error("movie dimensions out of range"); return SID_ERR_DIMS; }
+    if (bg_iter < 0) { set_error("bg_iter must be >= 0"); return SID_ERR_ARG; }
+    SID_TRY(check_device());
+    DevBuf dy, ds, dt, dstd;
+    SID_TRY_HIP(hipMalloc(&dy.ptr, sizeof(float) * P * T));
+    SID_TRY_HIP(hipMalloc(&ds.ptr, sizeof(float) * P));
+    SID_TRY_HIP(hipMalloc(&dt.ptr, sizeof(float) * T));
+    SID_TRY_HIP(hipMalloc(&dstd.ptr, sizeof(float) * P));
+    SID_TRY_HIP(hipMemcpy(dy.ptr, Y, sizeof(float) * P * T, hipMemcpyHostToDevice));
+    SID_TRY(std_image((const float *)dy.ptr, P, T, bg_iter, flags, (float *)ds.ptr, (float *)dt.ptr,
+                      (float *)dstd.ptr, ymax, nullptr));
+    SID_TRY_HIP(hipMemcpy(std_img, dstd.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
+    if (s) SID_TRY_HIP(hipMemcpy(s, ds.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
+    if (t) SID_TRY_HIP(hipMemcpy(t, dt.ptr, sizeof(float) * T, hipMemcpyDeviceToHost));
+    return SID_OK;
+}
+
 int sid_last_timing(const sid_psf *p, double ms[12])
 {
     if (!p || !ms) { set_error("null argument"); return SID_ERR_ARG; }

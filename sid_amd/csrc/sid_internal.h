@@ -214,6 +214,9 @@ int backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, h
 constexpr int kConvnMaxXY = 65, kConvnMaxZ = 255;
 int convn_same(const float *in, const int64_t vdims[3], const float *k_dev, const int kd[3], float *out,
                hipStream_t st);
+// sid_movie.hip: movie background (rank 1) and std image
+int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,
+              float *std_img, double *ymax_out, hipStream_t st);
 // sid_solvers.hip
 int ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts &o, hipStream_t st);
 // deconvRL basic on column-major device buffers through forward_dev / backward_dev (neuron-kernel handles)

@@ -1,0 +1,274 @@
+// Background and standard-deviation image of a movie (the producer of the RL
+// input in sid_main.m:88-103): optional division by the global maximum
+// (sid_main.m:88-89), rank_1_factorization (src/rank_1_factorization.m:31-39)
+// and compute_std_image (src/compute_std_image.m:22-26).
+//
+// The movie Y is column-major [P pixels, T frames] in HBM (one frame per
+// column), read-only.  Every step is one streaming pass over Y:
+//   column pass  t_j = sum_i s_i Y_ij            (block per frame)
+//   row pass     s_i = sum_j Y_ij t_j            (thread per pixel; in the last
+//                A_i = sum_j Y_ij^2, Sum_i = sum_j Y_ij      iteration also these)
+// with FP64 accumulation and fixed-order reductions (deterministic).  Norms of
+// s and t are reduced on the host in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sid_internal.h"
+
+namespace sid {
+
+constexpr int kMvThreads = 256;
+
+__global__ __launch_bounds__(kMvThreads) void k_mv_max(const float *__restrict__ Y, int64_t n, float *__restrict__ part)
+{
+    __shared__ float red[kMvThreads];
+    float m = -INFINITY;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, Y[e]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kMvThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Column pass over frames j0..j0+3 and pixel chunk c:
+// part[c][j] = scale * sum_{i in chunk c} s_i * Y[i, j], s_i = sd[i] (or 1 when sd is null).
+// Four frames per block so s is read once per four columns; chunks keep the
+// grid many blocks deep; the host adds the chunks in order.
+constexpr int kColGroup = 4;
+__global__ __launch_bounds__(kMvThreads) void k_mv_col(const float *__restrict__ Y, int64_t P, int64_t T,
+                                                       const double *__restrict__ sd, double scale, int64_t pc,
+                                                       double *__restrict__ part)
+{
+    __shared__ double red[kColGroup][kMvThreads];
+    const int64_t j0 = (int64_t)blockIdx.x * kColGroup;
+    const int64_t i0 = (int64_t)blockIdx.y * pc, i1 = std::min(P, i0 + pc);
+    const int nc = (int)std::min<int64_t>(kColGroup, T - j0);
+    double acc[kColGroup] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kMvThreads) {
+        const double sv = sd ? sd[i] : 1.0;
+        float y[kColGroup];
+#pragma unroll
+        for (int c = 0; c < kColGroup; ++c) y[c] = c < nc ? Y[(j0 + c) * P + i] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < kColGroup; ++c) acc[c] += (double)y[c] * sv;
+    }
+#pragma unroll
+    for (int c = 0; c < kColGroup; ++c) red[c][threadIdx.x] = acc[c];
+    __syncthreads();
+    for (int o = kMvThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int c = 0; c < kColGroup; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < nc) part[(int64_t)blockIdx.y * T + j0 + threadIdx.x] = red[threadIdx.x][0] * scale;
+}
+
+// Row pass: sd[i] = scale * sum_j Y[i, j] * t[j] (when t), and, when stats,
+// A[i] = scale^2 * sum_j Y[i, j]^2, Sm[i] = scale * sum_j Y[i, j].
+__global__ __launch_bounds__(kMvThreads) void k_mv_row(const float *__restrict__ Y, int64_t P, int64_t T,
+                                                       const double *__restrict__ t, double scale,
+                                                       double *__restrict__ sd, double *__restrict__ A,
+                                                       double *__restrict__ Sm)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    double s = 0.0, a = 0.0, m = 0.0;
+    const float *p = Y + i;
+    int64_t j = 0;
+    constexpr int U = 8;
+    for (; j + U <= T; j += U) {
+        float y[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) y[u] = p[(j + u) * P];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double v = y[u];
+            if (t) s += v * t[j + u];
+            if (A) {
+                a += v * v;
+                m += v;
+            }
+        }
+    }
+    for (; j < T; ++j) {
+        const double v = p[j * P];
+        if (t) s += v * t[j];
+        if (A) {
+            a += v * v;
+            m += v;
+        }
+    }
+    if (sd) sd[i] = s * scale;
+    if (A) {
+        A[i] = a * scale * scale;
+        Sm[i] = m * scale;
+    }
+}
+
+__global__ __launch_bounds__(kMvThreads) void k_mv_sumsq(const double *__restrict__ x, int64_t n,
+                                                         double *__restrict__ part)
+{
+    __shared__ double red[kMvThreads];
+    double a = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        a += x[e] * x[e];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = kMvThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// compute_std_image.m:26 with y1 = s (sd * s_scale), C = mean(t):
+//   std = sqrt((A - y1^2) / T - B^2 + 2 B y1 C - y1^2 C^2),  B = Sm / T.
+// A negative radicand (rounding, constant pixels) gives 0.
+__global__ void k_mv_std(const double *__restrict__ A, const double *__restrict__ Sm, const double *__restrict__ sd,
+                         double s_scale, int64_t P, int64_t T, double C, float *__restrict__ std_img,
+                         float *__restrict__ s_out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P; i += (int64_t)gridDim.x * blockDim.x) {
+        const double y1 = sd ? sd[i] * s_scale : 0.0;
+        const double B = Sm[i] / (double)T;
+        const double v = (A[i] - y1 * y1) / (double)T - B * B + 2.0 * B * y1 * C - y1 * y1 * C * C;
+        std_img[i] = (float)sqrt(v > 0.0 ? v : 0.0);
+        if (s_out) s_out[i] = (float)y1;
+    }
+}
+
+static int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
+
+template <typename T> struct MvBuf {
+    T *p = nullptr;
+    ~MvBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(int64_t n)
+    {
+        const hipError_t e = hipMalloc(&p, sizeof(T) * std::max<int64_t>(n, 1));
+        if (e != hipSuccess) {
+            set_error("hipMalloc failed: %s", hipGetErrorString(e));
+            return SID_ERR_OOM;
+        }
+        return SID_OK;
+    }
+};
+
+static int host_sumsq(const double *x, int64_t n, double *part, hipStream_t st, double *out)
+{
+    const int nb = grid_for(n);
+    hipLaunchKernelGGL(k_mv_sumsq, dim3(nb), dim3(kMvThreads), 0, st, x, n, part);
+    SID_TRY_HIP(hipGetLastError());
+    std::vector<double> h(nb);
+    SID_TRY_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    double s = 0.0;
+    for (double v : h) s += v;
+    *out = s;
+    return SID_OK;
+}
+
+int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,
+              float *std_img, double *ymax_out, hipStream_t st)
+{
+    // 1. global maximum (sid_main.m:88-89), when normalising
+    double scale = 1.0, ymax = 0.0;
+    MvBuf<float> fpart;
+    MvBuf<double> dpart, sd, td, A, Sm;
+    SID_TRY(fpart.alloc(4096));
+    SID_TRY(dpart.alloc(4096));
+    if (flags & 1) {
+        const int nb = grid_for(P * T);
+        hipLaunchKernelGGL(k_mv_max, dim3(nb), dim3(kMvThreads), 0, st, Y, P * T, fpart.p);
+        SID_TRY_HIP(hipGetLastError());
+        std::vector<float> h(nb);
+        SID_TRY_HIP(hipMemcpyAsync(h.data(), fpart.p, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
+        SID_TRY_HIP(hipStreamSynchronize(st));
+        float m = -INFINITY;
+        for (float v : h) m = std::max(m, v);
+        ymax = m;
+        if (!(ymax > 0.0) || !std::isfinite(ymax)) {
+            set_error("movie maximum %g: cannot normalise", ymax);
+            return SID_ERR_ARG;
+        }
+        scale = 1.0 / ymax;
+    }
+    if (ymax_out) *ymax_out = ymax;
+    SID_TRY(A.alloc(P));
+    SID_TRY(Sm.alloc(P));
+    double s_scale = 0.0, C = 0.0;
+    std::vector<double> th(T, 0.0);
+    if (bg_iter > 0) {
+        // pixel chunks for the column pass: about 2048 blocks in all
+        const int64_t ngroups = (T + kColGroup - 1) / kColGroup;
+        const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>((2048 + ngroups - 1) / ngroups, (P + 4095) / 4096));
+        const int64_t pc = (P + nchunk - 1) / nchunk;
+        std::vector<double> ph(nchunk * T);
+        SID_TRY(sd.alloc(P));
+        SID_TRY(td.alloc(std::max<int64_t>(T, nchunk * T)));
+        // s = ones(P, 1) / sqrt(P) (rank_1_factorization.m:31)
+        double s_mul = 1.0 / std::sqrt((double)P);
+        bool s_const = true;
+        for (int it = 1; it <= bg_iter; ++it) {
+            // t = s' * Y; t = t / norm(t)   (:33-34)
+            hipLaunchKernelGGL(k_mv_col, dim3((unsigned)ngroups, (unsigned)nchunk), dim3(kMvThreads), 0, st, Y, P, T,
+                               s_const ? (const double *)nullptr : sd.p, s_mul * scale, pc, td.p);
+            SID_TRY_HIP(hipGetLastError());
+            SID_TRY_HIP(hipMemcpyAsync(ph.data(), td.p, sizeof(double) * nchunk * T, hipMemcpyDeviceToHost, st));
+            SID_TRY_HIP(hipStreamSynchronize(st));
+            double nt = 0.0;
+            for (int64_t j = 0; j < T; ++j) {
+                double v = 0.0;
+                for (int64_t c = 0; c < nchunk; ++c) v += ph[c * T + j];
+                th[j] = v;
+                nt += v * v;
+            }
+            nt = std::sqrt(nt);
+            for (double &v : th) v /= nt;   // 0 / 0 = NaN, as MATLAB
+            SID_TRY_HIP(hipMemcpyAsync(td.p, th.data(), sizeof(double) * T, hipMemcpyHostToDevice, st));
+            // s = Y * t' (:35), with the std-image sums in the last iteration
+            const bool last = it == bg_iter;
+            hipLaunchKernelGGL(k_mv_row, dim3((unsigned)((P + kMvThreads - 1) / kMvThreads)), dim3(kMvThreads), 0, st,
+                               Y, P, T, (const double *)td.p, scale, sd.p, last ? A.p : (double *)nullptr,
+                               last ? Sm.p : (double *)nullptr);
+            SID_TRY_HIP(hipGetLastError());
+            s_const = false;
+            if (!last) {   // s = s / norm(s) (:36-38), folded into the next column pass
+                double ns = 0.0;
+                SID_TRY(host_sumsq(sd.p, P, dpart.p, st, &ns));
+                s_mul = 1.0 / std::sqrt(ns);
+            }
+        }
+        s_scale = 1.0;
+        double sum_t = 0.0;
+        for (double v : th) sum_t += v;
+        C = sum_t / (double)T;
+    } else {
+        hipLaunchKernelGGL(k_mv_row, dim3((unsigned)((P + kMvThreads - 1) / kMvThreads)), dim3(kMvThreads), 0, st, Y,
+                           P, T, (const double *)nullptr, scale, (double *)nullptr, A.p, Sm.p);
+        SID_TRY_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mv_std, dim3(grid_for(P)), dim3(256), 0, st, A.p, Sm.p,
+                       bg_iter > 0 ? (const double *)sd.p : (const double *)nullptr, s_scale, P, T, C, std_img,
+                       s_out);
+    SID_TRY_HIP(hipGetLastError());
+    if (t_out) {
+        std::vector<float> tf(T);
+        for (int64_t j = 0; j < T; ++j) tf[j] = (float)th[j];
+        SID_TRY_HIP(hipMemcpyAsync(t_out, tf.data(), sizeof(float) * T, hipMemcpyHostToDevice, st));
+    }
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    return SID_OK;
+}
+
+}  // namespace sid

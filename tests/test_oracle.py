@@ -520,3 +520,29 @@ def test_ls_options_accept_the_config_default_scalar_tv():
     assert list(o.lamb_TV_L2) == [0.0, 0.0, 0.0] and o.max_iter == 8
     with pytest.raises(ValueError):
         sid_amd._ls_opts({"lamb_TV_L2": 0.3})
+
+
+def test_rank_1_factorization_converges_to_the_leading_singular_pair():
+    """src/rank_1_factorization.m is a power iteration on Y Y': after many
+    iterations s t equals sigma_1 u_1 v_1' (numpy SVD); one iteration gives
+    t = column sums / their norm."""
+    rng = np.random.default_rng(41)
+    Y = rng.random((60, 25)) + np.outer(rng.random(60), rng.random(25)) * 3
+    s, t = O.rank_1_factorization(Y, 300)
+    U, S, Vt = np.linalg.svd(Y, full_matrices=False)
+    np.testing.assert_allclose(np.outer(s, t), S[0] * np.outer(U[:, 0], Vt[0]), rtol=1e-10, atol=1e-10)
+    assert abs(np.linalg.norm(t) - 1) < 1e-12
+    s1, t1 = O.rank_1_factorization(Y, 1)
+    cs = Y.sum(axis=0)
+    np.testing.assert_allclose(t1, cs / np.linalg.norm(cs), rtol=1e-12)
+
+
+def test_compute_std_image_is_the_residual_std():
+    """src/compute_std_image.m expands the per-pixel standard deviation (over
+    frames, normalised by T) of Y - y1 y2 for the rank-1 factors (||y2|| = 1,
+    y1 = Y y2'); with zero factors it is the plain std of Y."""
+    rng = np.random.default_rng(42)
+    Y = rng.random((50, 40)) * 2
+    np.testing.assert_allclose(O.compute_std_image(Y), np.std(Y, axis=1), rtol=1e-10)
+    s, t = O.rank_1_factorization(Y, 2)
+    np.testing.assert_allclose(O.compute_std_image(Y, s, t), np.std(Y - np.outer(s, t), axis=1), rtol=1e-9)
