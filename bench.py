@@ -142,14 +142,22 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     k = recon.generate_kernel("ball", (8, 2))
     kt = torch.from_numpy(np.ascontiguousarray(k.T)).to(dev)
     vol = torch.rand((wl.nz, wl.Y, wl.X), device=dev)
-    sid_amd.convn_same(vol, kt)
+    cprec = prec if prec in ("bf16x2", "bf16x3") else "fp32"
+    sid_amd.convn_same(vol, kt, precision=cprec)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     reps = 5
     for _ in range(reps):
-        sid_amd.convn_same(vol, kt)
+        sid_amd.convn_same(vol, kt, precision=cprec)
     torch.cuda.synchronize()
     tc = (time.perf_counter() - t0) / reps
+    sid_amd.convn_same(vol, kt, precision="fp32")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sid_amd.convn_same(vol, kt, precision="fp32")
+    torch.cuda.synchronize()
+    tc32 = (time.perf_counter() - t0) / reps
     taps = int(np.prod(k.shape))
     fl = 2.0 * wl.X * wl.Y * wl.nz * taps
     psf.set_kernel(k)
@@ -160,11 +168,12 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     torch.cuda.synchronize()
     dtw = time.perf_counter() - t0
     psf.set_kernel(None)
-    out["neuron_kernel"] = {"kernel": list(k.shape), "convn_ms": round(tc * 1e3, 3),
-                            "convn_tflops": round(fl / tc / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
+    out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_ms": round(tc * 1e3, 3),
+                            "convn_tflops": round(fl / tc / 1e12, 2), "convn_fp32_ms": round(tc32 * 1e3, 3),
+                            "convn_fp32_tflops": round(fl / tc32 / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
                             "rl_iter_per_s_wrapped": round(iters / dtw, 3)}
-    log(f"neuron kernel {k.shape}: convn {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s FP32); "
-        f"wrapped RL {iters / dtw:.2f} iter/s")
+    log(f"neuron kernel {k.shape}: convn {cprec} {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s algorithmic), "
+        f"FP32 {tc32 * 1e3:.2f} ms; wrapped RL {iters / dtw:.2f} iter/s")
     del vol
     P, T = wl.X * wl.Y, 1000
     Y = torch.rand((T, P), device=dev)

