@@ -42,10 +42,20 @@ __global__ __launch_bounds__(256) void k_convn_same(const float *__restrict__ V,
         if (zi < 0 || zi >= Z) continue;   // block-uniform
         const float *src = V + (int64_t)zi * X * Y;
         __syncthreads();
-        for (int e = threadIdx.x; e < WX * WY; e += 256) {
-            const int w = e / WX, u = e - w * WX;
-            const int xs = bx + u, ys = by + w;
-            tile[e] = (xs >= 0 && xs < X && ys >= 0 && ys < Y) ? src[(int64_t)ys * X + xs] : 0.0f;
+        for (int base = 0; base < WX * WY; base += 8 * 256) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = base + q * 256 + (int)threadIdx.x;
+                const int w = e / WX, u = e - w * WX;
+                const int xs = bx + u, ys = by + w;
+                v[q] = (e < WX * WY && xs >= 0 && xs < X && ys >= 0 && ys < Y) ? src[(int64_t)ys * X + xs] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int e = base + q * 256 + (int)threadIdx.x;
+                if (e < WX * WY) tile[e] = v[q];
+            }
         }
         for (int e = threadIdx.x; e < kx * ky; e += 256) ks[e] = k[(int64_t)l * kx * ky + e];
         __syncthreads();
@@ -117,15 +127,33 @@ __global__ void k_convn_bfrag(const float *__restrict__ k, int kx, int ky, int k
 }
 
 constexpr int kCmTile = 64;   // output tile side (x and y) of k_convn_mfma
+constexpr int kStageU = 8;    // staging loads in flight per thread
 
-template <int MODE>
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// Lane r of each 16-lane row takes lane r - 1's value (DPP row_shr:1); lane 0 keeps its own.
+__device__ __forceinline__ bf8 shift_rows(bf8 v)
+{
+    i32x4 x = __builtin_bit_cast(i32x4, v);
+    x.x = __builtin_amdgcn_update_dpp(x.x, x.x, 0x111, 0xf, 0xf, false);
+    x.y = __builtin_amdgcn_update_dpp(x.y, x.y, 0x111, 0xf, 0xf, false);
+    x.z = __builtin_amdgcn_update_dpp(x.z, x.z, 0x111, 0xf, 0xf, false);
+    x.w = __builtin_amdgcn_update_dpp(x.w, x.w, 0x111, 0xf, 0xf, false);
+    return __builtin_bit_cast(bf8, x);
+}
+
+// The A operand slides with the kernel row: lane r (output row r of the wave)
+// reads input tile row w = 16 wave + r + ky - 1 - j, which at step j + 1 is the
+// row lane r - 1 held at step j.  So after the first row of a slice only lane 0
+// of each 16-lane row reads LDS; the others take their neighbour's registers.
+template <int MODE, int KS>
 __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In, int X, int Y, int Z,
-                                                    const u16 *__restrict__ frag, int kx, int ky, int kz, int KS,
-                                                    int JC, float *__restrict__ out)
+                                                    const u16 *__restrict__ frag, int kx, int ky, int kz, int JC,
+                                                    float *__restrict__ out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int W = 48 + 32 * KS;       // input tile width: 3 x-block offsets + the K window
-    const int PW = W + 8;             // pitch in BF16 elements (multiple of 8: 16-byte rows)
+    constexpr int W = 48 + 32 * KS;   // input tile width: 3 x-block offsets + the K window
+    constexpr int PW = W + 8;         // pitch in BF16 elements (multiple of 8: 16-byte rows)
     const int Hh = kCmTile + ky - 1;  // input tile height
     u16 *thi = reinterpret_cast<u16 *>(smem);
     u16 *tlo = thi + Hh * PW;
@@ -137,42 +165,98 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
     f4 acc[4];
 #pragma unroll
     for (int xb = 0; xb < 4; ++xb) acc[xb] = f4{0.f, 0.f, 0.f, 0.f};
+    bf8 ah[4][KS], al[4][KS];
     for (int l = 0; l < kz; ++l) {
         const int zi = z + oz - l;
         if (zi < 0 || zi >= Z) continue;   // block-uniform
         __syncthreads();                    // tiles of the previous slice consumed
         const float *src = In + (int64_t)zi * X * Y;
-        for (int e = threadIdx.x; e < Hh * W; e += 256) {
-            const int w = e / W, u = e - w * W;
-            const int xs = X0 + u, ys = Y0 + w;
-            const float v = (xs >= 0 && xs < X && ys >= 0 && ys < Y) ? src[(int64_t)ys * X + xs] : 0.0f;
-            const u16 h = f2bf(v);
-            thi[w * PW + u] = h;
-            tlo[w * PW + u] = f2bf(v - bf2f(h));
+        // staging: kStageU independent loads in flight per thread, then convert and store
+        for (int base = 0; base < Hh * W; base += kStageU * 256) {
+            float v[kStageU];
+#pragma unroll
+            for (int q = 0; q < kStageU; ++q) {
+                const int e = base + q * 256 + (int)threadIdx.x;
+                const int w = e / W, u = e - w * W;
+                const int xs = X0 + u, ys = Y0 + w;
+                v[q] = (e < Hh * W && xs >= 0 && xs < X && ys >= 0 && ys < Y) ? src[(int64_t)ys * X + xs] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < kStageU; ++q) {
+                const int e = base + q * 256 + (int)threadIdx.x;
+                if (e < Hh * W) {
+                    const int w = e / W, u = e - w * W;
+                    const u16 h = f2bf(v[q]);
+                    thi[w * PW + u] = h;
+                    tlo[w * PW + u] = f2bf(v[q] - bf2f(h));
+                }
+            }
         }
         for (int jc0 = 0; jc0 < ky; jc0 += JC) {
             const int jn = min(JC, ky - jc0);
-            if (jc0 > 0) __syncthreads();   // previous B chunk consumed
+            __syncthreads();   // tile staged (first chunk) / previous B chunk consumed
             const bf8 *fsrc = reinterpret_cast<const bf8 *>(frag) + ((int64_t)l * ky + jc0) * KS * 2 * 64;
-            for (int e = threadIdx.x; e < jn * KS * 2 * 64; e += 256) bs[e] = fsrc[e];
+            const int nb = jn * KS * 2 * 64;
+            for (int base = 0; base < nb; base += kStageU * 256) {
+                bf8 t[kStageU];
+#pragma unroll
+                for (int q = 0; q < kStageU; ++q) {
+                    const int e = base + q * 256 + (int)threadIdx.x;
+                    if (e < nb) t[q] = fsrc[e];
+                }
+#pragma unroll
+                for (int q = 0; q < kStageU; ++q) {
+                    const int e = base + q * 256 + (int)threadIdx.x;
+                    if (e < nb) bs[e] = t[q];
+                }
+            }
             __syncthreads();
+            if (jc0 == 0) {   // all lanes read their row for j = 0
+                const int w = wave * 16 + l16 + ky - 1;
+#pragma unroll
+                for (int xb = 0; xb < 4; ++xb)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks) {
+                        const int u = 16 * xb + 32 * ks + 8 * g;
+                        ah[xb][ks] = *reinterpret_cast<const bf8 *>(thi + w * PW + u);
+                        al[xb][ks] = *reinterpret_cast<const bf8 *>(tlo + w * PW + u);
+                    }
+            }
             for (int jj = 0; jj < jn; ++jj) {
-                const int w = wave * 16 + l16 + ky - 1 - (jc0 + jj);   // input row of this lane's output row
-                const u16 *rh = thi + w * PW + 8 * g, *rl = tlo + w * PW + 8 * g;
+                const int j = jc0 + jj;
+#pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
                     const bf8 bh = bs[((jj * KS + ks) * 2 + 0) * 64 + lane];
                     bf8 bl;
                     if constexpr (MODE == 2) bl = bs[((jj * KS + ks) * 2 + 1) * 64 + lane];
 #pragma unroll
                     for (int xb = 0; xb < 4; ++xb) {
-                        const int u = 16 * xb + 32 * ks;
-                        const bf8 ah = *reinterpret_cast<const bf8 *>(rh + u);
-                        const bf8 al = *reinterpret_cast<const bf8 *>(rl + u);
                         f4 c = acc[xb];
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
-                        if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[xb][ks], bh, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[xb][ks], bh, c, 0, 0, 0);
+                        if constexpr (MODE == 2)
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[xb][ks], bl, c, 0, 0, 0);
                         acc[xb] = c;
+                    }
+                }
+                if (j + 1 < ky) {   // slide to row j + 1
+                    const int w0 = wave * 16 + ky - 2 - j;   // lane 0's new row
+#pragma unroll
+                    for (int xb = 0; xb < 4; ++xb)
+#pragma unroll
+                        for (int ks = 0; ks < KS; ++ks) {
+                            ah[xb][ks] = shift_rows(ah[xb][ks]);
+                            al[xb][ks] = shift_rows(al[xb][ks]);
+                        }
+                    if (l16 == 0) {
+#pragma unroll
+                        for (int xb = 0; xb < 4; ++xb)
+#pragma unroll
+                            for (int ks = 0; ks < KS; ++ks) {
+                                const int u = 16 * xb + 32 * ks + 8 * g;
+                                ah[xb][ks] = *reinterpret_cast<const bf8 *>(thi + w0 * PW + u);
+                                al[xb][ks] = *reinterpret_cast<const bf8 *>(tlo + w0 * PW + u);
+                            }
                     }
                 }
             }
@@ -191,6 +275,17 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
 }
 
 int convn_ks(int kx) { return (15 + kx + 31) / 32; }
+
+template <int MODE, int KS>
+static int launch_convn_mfma(dim3 grid, size_t lds, hipStream_t st, const float *in, const int64_t vdims[3],
+                             const u16 *frag, const int kd[3], int JC, float *out)
+{
+    SID_TRY_HIP(hipFuncSetAttribute((const void *)k_convn_mfma<MODE, KS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    hipLaunchKernelGGL((k_convn_mfma<MODE, KS>), grid, dim3(256), lds, st, in, (int)vdims[0], (int)vdims[1],
+                       (int)vdims[2], frag, kd[0], kd[1], kd[2], JC, out);
+    return SID_OK;
+}
 
 int convn_build_frag(const float *k_dev, const int kd[3], u16 **frag, hipStream_t st)
 {
@@ -220,17 +315,13 @@ int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const i
     const size_t lds = tile + JC * per_row;
     const dim3 grid((unsigned)((vdims[0] + kCmTile - 1) / kCmTile), (unsigned)((vdims[1] + kCmTile - 1) / kCmTile),
                     (unsigned)vdims[2]);
-    if (precision == SID_PREC_BF16X3) {
-        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_convn_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
-        hipLaunchKernelGGL(k_convn_mfma<2>, grid, dim3(256), lds, st, in, (int)vdims[0], (int)vdims[1],
-                           (int)vdims[2], frag, kd[0], kd[1], kd[2], KS, JC, out);
-    } else {
-        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_convn_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)lds));
-        hipLaunchKernelGGL(k_convn_mfma<1>, grid, dim3(256), lds, st, in, (int)vdims[0], (int)vdims[1],
-                           (int)vdims[2], frag, kd[0], kd[1], kd[2], KS, JC, out);
-    }
+    const int mode = precision == SID_PREC_BF16X3 ? 2 : 1;
+    if (mode == 2 && KS == 1) SID_TRY((launch_convn_mfma<2, 1>(grid, lds, st, in, vdims, frag, kd, JC, out)));
+    else if (mode == 2 && KS == 2) SID_TRY((launch_convn_mfma<2, 2>(grid, lds, st, in, vdims, frag, kd, JC, out)));
+    else if (mode == 2) SID_TRY((launch_convn_mfma<2, 3>(grid, lds, st, in, vdims, frag, kd, JC, out)));
+    else if (KS == 1) SID_TRY((launch_convn_mfma<1, 1>(grid, lds, st, in, vdims, frag, kd, JC, out)));
+    else if (KS == 2) SID_TRY((launch_convn_mfma<1, 2>(grid, lds, st, in, vdims, frag, kd, JC, out)));
+    else SID_TRY((launch_convn_mfma<1, 3>(grid, lds, st, in, vdims, frag, kd, JC, out)));
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }

@@ -25,6 +25,8 @@ CONV_CASES = [
     ((33, 41, 3), (4, 2, 6)),        # even sides, kernel deeper than the volume
     ((20, 20, 1), (5, 7, 1)),        # one plane
     ((131, 17, 2), (65, 1, 1)),      # the largest lateral side
+    ((90, 50, 4), (33, 9, 3)),       # a two-step Toeplitz window
+    ((70, 80, 3), (3, 65, 2)),       # the tallest kernel: B fragments staged in several chunks
 ]
 
 
