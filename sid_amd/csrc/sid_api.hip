@@ -329,8 +329,8 @@ int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
     std::memset(info, 0, sizeof(*info));
     info->M = p->M; info->Nnum = p->n; info->Nz = p->nzH; info->taps_max = p->T;
     int64_t wb = (p->wf_elems + p->wb_elems) * 4;
-    if (p->wf_hi) wb += (p->wf_elems + p->wb_elems) * 2;
-    if (p->wf_lo) wb += (p->wf_elems + p->wb_elems) * 2;
+    if (p->wf_hi) wb += (p->wf2_elems + p->wb2_elems) * 2;
+    if (p->wf_lo) wb += (p->wf2_elems + p->wb2_elems) * 2;
     if (p->wf_d) wb += (p->wf_elems + p->wb_elems) * 8;
     info->weight_bytes = wb;
     for (int64_t c = 0; c < p->nzH && c < 512; ++c) {

@@ -257,6 +257,190 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
 }
 
 
+// ---------------------------------------------------------------------------
+// BF16 projector GEMM with register-sliding A (MODE 1: BF16X2, MODE 2: BF16X3).
+//
+// Within a segment the A row of output lenslet r at tap ky is the window of
+// lenslet (r + ky) of the padded volume row, so lane r's A fragment at tap
+// ky + 1 is lane r + 1's at tap ky whenever lenslets r and r + 1 are
+// neighbours in the same row (always except at the end of a volume row).  The
+// loop runs over K chunks of 32 (kc, outer) and taps (ky, inner): after each
+// tap the A registers move one lane down (DPP row_shl:1) and only lane 15 and
+// lanes at a row end load their next fragment from memory, a tap ahead of its
+// use.  Each tap is padded to KT = 32 * ceil(S / 32) in the weights (rows
+// beyond S are zero), so the fragments past S read the next lenslet's values
+// and multiply zeros.  B chunks [NP][32] are double buffered in LDS through
+// registers, one barrier per step.
+// ---------------------------------------------------------------------------
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// Lane i of each 16-lane row takes lane i + 1's value (DPP row_shl:1); lane 15 keeps its own.
+__device__ __forceinline__ bf8 dpp_row_shl1(bf8 v)
+{
+    i32x4 x = __builtin_bit_cast(i32x4, v);
+    x.x = __builtin_amdgcn_update_dpp(x.x, x.x, 0x101, 0xf, 0xf, false);
+    x.y = __builtin_amdgcn_update_dpp(x.y, x.y, 0x101, 0xf, 0xf, false);
+    x.z = __builtin_amdgcn_update_dpp(x.z, x.z, 0x101, 0xf, 0xf, false);
+    x.w = __builtin_amdgcn_update_dpp(x.w, x.w, 0x101, 0xf, 0xf, false);
+    return __builtin_bit_cast(bf8, x);
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// B chunk [NP][KW] (16-byte vectors e = col * KW / 8 + kq) of the hi (and lo) weights into registers.
+template <int NP, int KW, int NB, int VPT>
+__device__ __forceinline__ void sl_fetch_b(u32x4 (&pre)[NB][VPT], const u16 *__restrict__ Wh,
+                                           const u16 *__restrict__ Wl, int64_t o)
+{
+    constexpr int NV = NP * KW / 8;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        const int e = v * NTHREADS + (int)threadIdx.x;
+        if (e < NV) {
+            pre[0][v] = *(const u32x4 *)(Wh + o + 8 * e);
+            if constexpr (NB == 2) pre[1][v] = *(const u32x4 *)(Wl + o + 8 * e);
+        }
+    }
+}
+
+// ... and from registers into an LDS buffer [NB][NP * LDK] (column pitch LDK).
+template <int NP, int KW, int LDK, int NB, int VPT>
+__device__ __forceinline__ void sl_stash_b(const u32x4 (&pre)[NB][VPT], u16 *bs)
+{
+    constexpr int NV = NP * KW / 8;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+        const int e = v * NTHREADS + (int)threadIdx.x;
+        if (e < NV) {
+            const int col = e / (KW / 8), kq = e - col * (KW / 8);
+            *(u32x4 *)(bs + col * LDK + 8 * kq) = pre[0][v];
+            if constexpr (NB == 2) *(u32x4 *)(bs + NP * LDK + col * LDK + 8 * kq) = pre[1][v];
+        }
+    }
+}
+
+template <int NT, int RT, int MODE>
+__global__ __launch_bounds__(NTHREADS) void k_proj_gemm_sl(
+    const Item *__restrict__ items, const Seg *__restrict__ segs, const u16 *__restrict__ A0,
+    const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
+    int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
+{
+    constexpr int NP = NT * 16;
+    constexpr int MT = 4 * RT * 16;
+    constexpr int KW = 32;               // K per step
+    constexpr int LDK = KW + 8;          // staged column pitch (BF16 elements)
+    constexpr int NB = MODE == 2 ? 2 : 1;
+    constexpr int NV = NP * KW / 8;      // 16-byte vectors per staged part
+    constexpr int VPT = (NV + NTHREADS - 1) / NTHREADS;
+    __shared__ u16 Bs[2][NB][NP * LDK];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const Item it = items[blockIdx.x];
+
+    int64_t rowoff[RT];
+    bool fresh[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int r0 = it.mtile * MT + wave * RT * 16 + rt * 16 + l16;
+        const int r = r0 < nrows ? r0 : nrows - 1;
+        const int rn = r0 + 1 < nrows ? r0 + 1 : nrows - 1;
+        const int Px = r / Ly, Py = r - Px * Ly;
+        const int Pxn = rn / Ly, Pyn = rn - Pxn * Ly;
+        rowoff[rt] = ((int64_t)Px * YP + Py) * S;
+        const int64_t next = ((int64_t)Pxn * YP + Pyn) * S;
+        fresh[rt] = l16 == 15 || next != rowoff[rt] + S;
+    }
+    f4 acc[RT][NT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+
+    for (int s = it.seg0; s < it.seg1; ++s) {
+        const Seg sg = segs[s];
+        const u16 *Ah = A0 + sg.a_off, *Al = A1 + sg.a_off;
+        const u16 *Wh = W0 + sg.w_off2, *Wl = W1 + sg.w_off2;
+        const int ntap = sg.K2 / KT, nkc = KT / KW;
+        for (int kc = 0; kc < nkc; ++kc) {
+            // step t = ky: B chunk (ky, kc) at W + (ky * nkc + kc) * NP * KW
+            u32x4 pre[NB][VPT];
+            // A fragments for tap 0 (all lanes) and tap 1 (slid + fresh lanes)
+            bf8 ah[RT], al[RT], nh[RT], nl[RT];
+            const int ko = kc * KW + 8 * g;
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                ah[rt] = *(const bf8 *)(Ah + rowoff[rt] + ko);
+                al[rt] = *(const bf8 *)(Al + rowoff[rt] + ko);
+            }
+            sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)0 * nkc + kc) * NP * KW);
+            __syncthreads();   // previous chunk's buffers consumed
+            sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[0][0][0]);
+            if (ntap > 1) {
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    nh[rt] = dpp_row_shl1(ah[rt]);
+                    nl[rt] = dpp_row_shl1(al[rt]);
+                    if (fresh[rt]) {
+                        nh[rt] = *(const bf8 *)(Ah + rowoff[rt] + S + ko);
+                        nl[rt] = *(const bf8 *)(Al + rowoff[rt] + S + ko);
+                    }
+                }
+                sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)1 * nkc + kc) * NP * KW);
+            }
+            __syncthreads();
+            for (int ky = 0; ky < ntap; ++ky) {
+                const int buf = ky & 1;
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct) {
+                    const bf8 bh = *(const bf8 *)(&Bs[buf][0][(ct * 16 + l16) * LDK + 8 * g]);
+                    bf8 bl;
+                    if constexpr (MODE == 2) bl = *(const bf8 *)(&Bs[buf][1][(ct * 16 + l16) * LDK + 8 * g]);
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) {
+                        f4 c = acc[rt][ct];
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
+                        if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
+                        acc[rt][ct] = c;
+                    }
+                }
+                if (ky + 1 < ntap) {
+                    sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[buf ^ 1][0][0]);   // B of tap ky + 1
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) {
+                        ah[rt] = nh[rt];
+                        al[rt] = nl[rt];
+                    }
+                    if (ky + 2 < ntap) {   // A of tap ky + 2 and B of tap ky + 2
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt) {
+                            nh[rt] = dpp_row_shl1(ah[rt]);
+                            nl[rt] = dpp_row_shl1(al[rt]);
+                            if (fresh[rt]) {
+                                const int64_t o = rowoff[rt] + (int64_t)(ky + 2) * S + ko;
+                                nh[rt] = *(const bf8 *)(Ah + o);
+                                nl[rt] = *(const bf8 *)(Al + o);
+                            }
+                        }
+                        sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)(ky + 2) * nkc + kc) * NP * KW);
+                    }
+                    __syncthreads();
+                }
+            }
+        }
+    }
+
+    float *out = part + (int64_t)it.slab * slab_stride;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int rbase = it.mtile * MT + wave * RT * 16 + rt * 16 + 4 * g;
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + i) * NP + ct * 16 + l16] = acc[rt][ct][i];
+    }
+}
+
 // FP64 variant (reference path for error studies): v_mfma_f64_16x16x4_f64
 // with the same K permutation as MODE 0 (lane group g supplies k = 8g + s at
 // step s of a 32-wide chunk).  Its output layout differs from the FP32 one:
@@ -531,15 +715,18 @@ int build_weights(sid_psf *p, int which)
     const int n = (int)p->n;
     // geometry (once)
     if (p->wf_elems == 0) {
-        int64_t offf = 0, offb = 0;
+        int64_t offf = 0, offb = 0, off2f = 0, off2b = 0;
         p->T = 0;
+        p->KT = roundup(p->S, 32);
         for (auto &pl : p->planes) {
             if (pl.empty) {
                 pl.tf_lo = pl.tb_lo = 0;
                 pl.tf_hi = pl.tb_hi = -1;
-                pl.Kf = pl.Kb = 0;
+                pl.Kf = pl.Kb = pl.Kf2 = pl.Kb2 = 0;
                 pl.wf_off = offf;
                 pl.wb_off = offb;
+                pl.wf2_off = off2f;
+                pl.wb2_off = off2b;
                 continue;
             }
             tap_range(pl.lo, pl.hi, pl.off_f, n, &pl.tf_lo, &pl.tf_hi);
@@ -551,11 +738,19 @@ int build_weights(sid_psf *p, int which)
             pl.wb_off = offb;
             offf += (int64_t)ntf * pl.Kf * p->NP;
             offb += (int64_t)ntb * pl.Kb * p->NP;
+            pl.Kf2 = (int64_t)ntf * p->KT;
+            pl.Kb2 = (int64_t)ntb * p->KT;
+            pl.wf2_off = off2f;
+            pl.wb2_off = off2b;
+            off2f += (int64_t)ntf * pl.Kf2 * p->NP;
+            off2b += (int64_t)ntb * pl.Kb2 * p->NP;
             p->T = std::max<int64_t>(p->T, std::max(std::max(std::abs(pl.tf_lo), std::abs(pl.tf_hi)),
                                                     std::max(std::abs(pl.tb_lo), std::abs(pl.tb_hi))));
         }
         p->wf_elems = offf;
         p->wb_elems = offb;
+        p->wf2_elems = off2f;
+        p->wb2_elems = off2b;
     }
     const size_t esz = which == 0 ? 4 : which == 3 ? 8 : 2;
     void **dstf = which == 0 ? (void **)&p->wf : which == 1 ? (void **)&p->wf_hi
@@ -563,32 +758,37 @@ int build_weights(sid_psf *p, int which)
     void **dstb = which == 0 ? (void **)&p->wb : which == 1 ? (void **)&p->wb_hi
                 : which == 2 ? (void **)&p->wb_lo : (void **)&p->wb_d;
     if (*dstf) return SID_OK;
-    const int KC = (which == 0 || which == 3) ? KChunk<0>::value : KChunk<1>::value;
-    SID_TRY_HIP(hipMalloc(dstf, std::max<int64_t>(p->wf_elems, 1) * esz));
-    SID_TRY_HIP(hipMalloc(dstb, std::max<int64_t>(p->wb_elems, 1) * esz));
+    // FP32 / FP64: [K / 32][NP][32] with K = taps * S (padded); BF16: the same with each tap padded to KT
+    const bool bf = which == 1 || which == 2;
+    const int KC = bf ? 32 : KChunk<0>::value;
+    const int64_t ef = bf ? p->wf2_elems : p->wf_elems, eb = bf ? p->wb2_elems : p->wb_elems;
+    const int Sk = bf ? (int)p->KT : (int)p->S;
+    SID_TRY_HIP(hipMalloc(dstf, std::max<int64_t>(ef, 1) * esz));
+    SID_TRY_HIP(hipMalloc(dstb, std::max<int64_t>(eb, 1) * esz));
     for (int64_t c = 0; c < p->nzH; ++c) {
         const Plane &pl = p->planes[c];
         if (pl.empty) continue;
         for (int dir = 0; dir < 2; ++dir) {
             const int t_lo = dir == 0 ? pl.tf_lo : pl.tb_lo;
             const int ntap = (dir == 0 ? pl.tf_hi : pl.tb_hi) - t_lo + 1;
-            const int64_t K = dir == 0 ? pl.Kf : pl.Kb;
+            const int64_t K = bf ? (dir == 0 ? pl.Kf2 : pl.Kb2) : (dir == 0 ? pl.Kf : pl.Kb);
             const int off = dir == 0 ? pl.off_f : pl.off_b;
-            char *base = (char *)(dir == 0 ? *dstf : *dstb) + (dir == 0 ? pl.wf_off : pl.wb_off) * esz;
+            const int64_t woff = bf ? (dir == 0 ? pl.wf2_off : pl.wb2_off) : (dir == 0 ? pl.wf_off : pl.wb_off);
+            char *base = (char *)(dir == 0 ? *dstf : *dstb) + woff * esz;
             const int64_t total = (int64_t)ntap * K * p->NP;
             const int grid = grid_for(total, 256, 16384);
             if (which == 0)
                 hipLaunchKernelGGL(k_build_w<0>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
-                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, Sk, (int)p->NP, KC);
             else if (which == 1)
                 hipLaunchKernelGGL(k_build_w<1>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
-                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, Sk, (int)p->NP, KC);
             else if (which == 2)
                 hipLaunchKernelGGL(k_build_w<2>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
-                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, Sk, (int)p->NP, KC);
             else
                 hipLaunchKernelGGL(k_build_w<3>, dim3(grid), dim3(256), 0, p->stream, p->H, (void *)base, p->M, n,
-                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, (int)p->S, (int)p->NP, KC);
+                                   (int)c, dir, pl.lo, pl.hi, off, t_lo, ntap, K, Sk, (int)p->NP, KC);
             SID_TRY_HIP(hipGetLastError());
         }
     }
@@ -707,6 +907,9 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             sg.w_off = pl.wf_off + (int64_t)tx * pl.Kf * NP;
             sg.K = (int32_t)pl.Kf;
             sg.plane = (int32_t)c;
+            sg.w_off2 = pl.wf2_off + (int64_t)tx * pl.Kf2 * NP;
+            sg.K2 = (int32_t)pl.Kf2;
+            sg.pad_ = 0;
             F.segs.push_back(sg);
         }
     }
@@ -736,6 +939,9 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             sg.w_off = pl.wb_off + (int64_t)tx * pl.Kb * NP;
             sg.K = (int32_t)pl.Kb;
             sg.plane = (int32_t)c;
+            sg.w_off2 = pl.wb2_off + (int64_t)tx * pl.Kb2 * NP;
+            sg.K2 = (int32_t)pl.Kb2;
+            sg.pad_ = 0;
             B.segs.push_back(sg);
             btotal += sg.K;
         }
@@ -843,15 +1049,18 @@ static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st)
     } else if (p->precision == SID_PREC_FP32) {
         launch_one<NT, 2, 0>(g, fwd ? (const void *)sp.vol : (const void *)sp.img, nullptr,
                              fwd ? (const void *)p->wf : (const void *)p->wb, nullptr, sp, S, st);
-    } else if (p->precision == SID_PREC_BF16X2) {
-        launch_one<NT, 2, 1>(g, fwd ? (const void *)sp.vol_hi : (const void *)sp.img_hi,
-                             fwd ? (const void *)sp.vol_lo : (const void *)sp.img_lo,
-                             fwd ? (const void *)p->wf_hi : (const void *)p->wb_hi, nullptr, sp, S, st);
     } else {
-        launch_one<NT, 2, 2>(g, fwd ? (const void *)sp.vol_hi : (const void *)sp.img_hi,
-                             fwd ? (const void *)sp.vol_lo : (const void *)sp.img_lo,
-                             fwd ? (const void *)p->wf_hi : (const void *)p->wb_hi,
-                             fwd ? (const void *)p->wf_lo : (const void *)p->wb_lo, sp, S, st);
+        const u16 *A0 = fwd ? sp.vol_hi : sp.img_hi, *A1 = fwd ? sp.vol_lo : sp.img_lo;
+        const u16 *W0 = fwd ? p->wf_hi : p->wb_hi;
+        const u16 *W1 = p->precision == SID_PREC_BF16X3 ? (fwd ? p->wf_lo : p->wb_lo) : W0;
+        if (p->precision == SID_PREC_BF16X2)
+            hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 1>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
+                               g.d_items, g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               (int)p->KT, sp.mpad * (int64_t)(NT * 16));
+        else
+            hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 2>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
+                               g.d_items, g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               (int)p->KT, sp.mpad * (int64_t)(NT * 16));
     }
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;

@@ -52,6 +52,22 @@ struct FftDesc {
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
+// and meets the other waves, without waiting for outstanding global loads
+// (the register prefetch of the persistent kernels stays in flight).  The
+// memory clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// a / b for 0 <= a < 2^22, b >= 1, with rb = 1.0f / b: the float quotient is
+// off by at most one and corrected exactly (no integer division sequence).
+__device__ __forceinline__ int qdiv(int a, int b, float rb)
+{
+    int q = (int)((float)a * rb);
+    q -= (q * b > a);
+    q += ((q + 1) * b <= a);
+    return q;
+}
+
 template <int R>
 __device__ __forceinline__ void small_dft(float2 *v, float sgn)
 {
@@ -104,20 +120,22 @@ __device__ __forceinline__ void stockham_stage(const float2 *__restrict__ in, fl
                                                const float2 *__restrict__ tw, float sgn, int spb, int tid, int nth)
 {
     const int nb = L / R;
+    const int step = L / (Ns * R);   // twiddle index step: angle = 2 pi k r / (Ns R); step * k * r < L
+    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
     for (int e = tid; e < spb * nb; e += nth) {
-        const int s = e / nb, j = e - s * nb;
+        const int s = qdiv(e, nb, rnb), j = e - s * nb;
         const float2 *src = in + s * L;
         float2 *dst = out + s * L;
-        const int k = j % Ns;
-        const int tstep = (L / (Ns * R)) * k;   // twiddle index step: angle = 2 pi k r / (Ns R)
+        const int jq = qdiv(j, Ns, rns), k = j - jq * Ns;
+        const int tstep = step * k;
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float2 x = src[j + r * nb];
-            v[r] = r == 0 ? x : cmul(x, tw[(tstep * r) % L]);
+            v[r] = r == 0 ? x : cmul(x, tw[tstep * r]);
         }
         small_dft<R>(v, sgn);
-        const int base = (j / Ns) * Ns * R + k;
+        const int base = jq * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
     }
@@ -134,13 +152,17 @@ __global__ void k_fft1d(float2 *__restrict__ data, FftDesc d, int64_t s_begin, i
                         float sgn, float scale)
 {
     extern __shared__ float2 sm[];
+    __shared__ int64_t soff[64];   // start of each of the block's sequences (spb <= 64)
     float2 *b0 = sm, *b1 = sm + d.spb * d.L;
     const int64_t s0 = s_begin + (int64_t)blockIdx.x * d.spb;
     const int spb = (int)min((int64_t)d.spb, nseq - s0);
     if (spb <= 0) return;
+    if ((int)threadIdx.x < spb) soff[threadIdx.x] = seq_off(s0 + threadIdx.x, inner, outer_stride, seq_stride);
+    __syncthreads();
+    const float rL = 1.0f / (float)d.L;
     for (int e = threadIdx.x; e < spb * d.L; e += blockDim.x) {
-        const int s = e / d.L, t = e - s * d.L;
-        b0[e] = data[seq_off(s0 + s, inner, outer_stride, seq_stride) + (int64_t)t * elem_stride];
+        const int s = qdiv(e, d.L, rL), t = e - s * d.L;
+        b0[e] = data[soff[s] + (int64_t)t * elem_stride];
     }
     __syncthreads();
     int Ns = 1;
@@ -156,10 +178,9 @@ __global__ void k_fft1d(float2 *__restrict__ data, FftDesc d, int64_t s_begin, i
         float2 *t = in; in = out; out = t;
     }
     for (int e = threadIdx.x; e < spb * d.L; e += blockDim.x) {
-        const int s = e / d.L, t = e - s * d.L;
+        const int s = qdiv(e, d.L, rL), t = e - s * d.L;
         const float2 v = in[e];
-        data[seq_off(s0 + s, inner, outer_stride, seq_stride) + (int64_t)t * elem_stride] =
-            make_float2(v.x * scale, v.y * scale);
+        data[soff[s] + (int64_t)t * elem_stride] = make_float2(v.x * scale, v.y * scale);
     }
 }
 
@@ -591,22 +612,24 @@ __device__ __forceinline__ void stockham_stage_2d(const float2 *__restrict__ in,
 {
     const int nb = L / R;
     const int tot = nseq * nb;
+    const int step = L / (Ns * R);   // step * k * r < L: no wrap of the twiddle index
+    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns, rseq = 1.0f / (float)nseq;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         int sq, j;
-        if (es > 1) { sq = e % nseq; j = e / nseq; }
-        else { sq = e / nb; j = e - sq * nb; }
+        if (es > 1) { j = qdiv(e, nseq, rseq); sq = e - j * nseq; }
+        else { sq = qdiv(e, nb, rnb); j = e - sq * nb; }
         const float2 *src = in + sq * ss;
         float2 *dst = out + sq * ss;
-        const int k = j % Ns;
-        const int tstep = (L / (Ns * R)) * k;
+        const int jq = qdiv(j, Ns, rns), k = j - jq * Ns;
+        const int tstep = step * k;
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float2 x = src[(j + r * nb) * es];
-            v[r] = r == 0 ? x : cmul(x, tw[(tstep * r) % L]);
+            v[r] = r == 0 ? x : cmul(x, tw[tstep * r]);
         }
         small_dft<R>(v, sgn);
-        const int base = (j / Ns) * Ns * R + k;
+        const int base = jq * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[(base + r * Ns) * es] = v[r];
     }
@@ -624,7 +647,7 @@ __device__ float2 *fft_pass_2d(float2 *a, float2 *b, const FftDesc &d, const flo
         else if (R == 2) stockham_stage_2d<2>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
         else if (R == 3) stockham_stage_2d<3>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
         else stockham_stage_2d<5>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
-        __syncthreads();
+        lds_barrier();
         Ns *= R;
         float2 *t = in; in = out; out = t;
     }
@@ -640,18 +663,25 @@ struct SubGeom {
 // for the stored half spectrum.  Z = DFT(a + i b) gives
 //   A(k) = (Z(k) + conj(Z(-k))) / 2,   B(k) = (Z(k) - conj(Z(-k))) / (2 i).
 constexpr int kSubThreads = 1024;   // one block per CU (the two grids take most of the LDS)
-constexpr int kSubUnroll = 8;       // independent global accesses in flight per thread
+constexpr int kSubUnroll = 8;       // grid elements per thread (prefetch registers): grids up to 8192 points
 
-// Pair handled by this block.  Blocks are dealt to the 8 XCDs round robin and
-// each XCD has its own L2; neighbouring pairs read the same cache lines (the
-// volume's lenslet rows, the packed spectra), so each XCD takes one
-// contiguous range of pairs.  A bijection of [0, npairs).
-__device__ __forceinline__ int xcd_pair(int b, int npairs)
+// Pairs of one block.  The kernels are persistent (one block per CU); blocks
+// are dealt to the 8 XCDs round robin and each XCD has its own L2.
+// Neighbouring pairs read the same cache lines (the volume's lenslet rows, the
+// packed spectra), so each XCD takes one contiguous range of pairs, and that
+// range is shared by the XCD's blocks in an interleaved order.
+struct PairIter {
+    int next, end, step;
+};
+
+__device__ __forceinline__ PairIter pair_iter(int npairs)
 {
     constexpr int kXcd = 8;
+    const int x = blockIdx.x % kXcd, i = blockIdx.x / kXcd;
     const int q = npairs / kXcd, r = npairs % kXcd;
-    const int x = b % kXcd, i = b / kXcd;
-    return x * q + min(x, r) + i;
+    const int begin = x * q + min(x, r), count = q + (x < r ? 1 : 0);
+    const int nbx = ((int)gridDim.x - x + kXcd - 1) / kXcd;   // blocks on this XCD
+    return PairIter{begin + i, begin + count, nbx};
 }
 
 // Twiddle tables of both lengths into LDS (after the two grids).
@@ -660,6 +690,37 @@ __device__ __forceinline__ void load_twiddles(float2 *tx, float2 *ty, const floa
 {
     for (int t = threadIdx.x; t < Lx; t += blockDim.x) tx[t] = twx[t];
     for (int t = threadIdx.x; t < Ly; t += blockDim.x) ty[t] = twy[t];
+}
+
+// Forward: volume -> 2-D DFT of the pair's sub-images -> vt[k0][q] = s_q * V_q(k0)
+// for the stored half spectrum.  Z = DFT(a + i b) gives
+//   A(k) = (Z(k) + conj(Z(-k))) / 2,   B(k) = (Z(k) - conj(Z(-k))) / (2 i).
+// The next pair's volume values are loaded into registers while the current
+// pair is transformed and written out.
+__device__ __forceinline__ void sub_fwd_fetch(const float *__restrict__ vol, const int *__restrict__ slot_plane,
+                                              const SubGeom &g, int pair, float (&a)[kSubUnroll],
+                                              float (&b)[kSubUnroll])
+{
+    const int q0 = 2 * pair;
+    const bool has1 = q0 + 1 < g.nq;
+    const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
+    const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
+    const bool in0 = c0 < g.nz_in, in1 = has1 && c1 < g.nz_in;
+    const float *v0p = vol + (int64_t)c0 * g.XP * g.YP * g.S + ab0;
+    const float *v1p = vol + (int64_t)c1 * g.XP * g.YP * g.S + ab1;
+    const int npix = g.Nxp * g.Nyp;
+    const float rnyp = 1.0f / (float)g.Nyp;
+#pragma unroll
+    for (int u = 0; u < kSubUnroll; ++u) {
+        const int e = u * kSubThreads + (int)threadIdx.x;
+        const int Qx = qdiv(e, g.Nyp, rnyp), Qy = e - Qx * g.Nyp;
+        a[u] = b[u] = 0.f;
+        if (e < npix && Qx < g.Lx && Qy < g.Ly) {
+            const int64_t cell = (((int64_t)Qx + g.T) * g.YP + (Qy + g.T)) * g.S;
+            if (in0) a[u] = v0p[cell];
+            if (in1) b[u] = v1p[cell];
+        }
+    }
 }
 
 __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__restrict__ vol,
@@ -672,54 +733,73 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__rest
     const int npix = g.Nxp * g.Nyp;
     float2 *bufa = sm, *bufb = sm + npix, *tx = sm + 2 * npix, *ty = tx + g.Nxp;
     load_twiddles(tx, ty, twx, twy, g.Nxp, g.Nyp);
-    const int q0 = 2 * xcd_pair(blockIdx.x, gridDim.x);
-    const bool has1 = q0 + 1 < g.nq;
-    const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
-    const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
-    const bool in0 = c0 < g.nz_in, in1 = has1 && c1 < g.nz_in;
-    const float *v0p = vol + (int64_t)c0 * g.XP * g.YP * g.S + ab0;
-    const float *v1p = vol + (int64_t)c1 * g.XP * g.YP * g.S + ab1;
-    for (int base = 0; base < npix; base += kSubUnroll * kSubThreads) {
-        float a[kSubUnroll], b[kSubUnroll];
+    PairIter it = pair_iter((g.nq + 1) / 2);
+    if (it.next >= it.end) return;   // block-uniform
+    const float rhy = 1.0f / (float)g.Hy;
+    const int nk0 = g.Nxp * g.Hy;
+    float a[kSubUnroll], b[kSubUnroll];
+    sub_fwd_fetch(vol, slot_plane, g, it.next, a, b);
+    for (int pair = it.next; pair < it.end; pair += it.step) {
+        lds_barrier();   // the previous pair's grids are consumed (twiddles loaded, first pass)
 #pragma unroll
         for (int u = 0; u < kSubUnroll; ++u) {
-            const int e = base + u * kSubThreads + (int)threadIdx.x;
-            const int Qx = e / g.Nyp, Qy = e - Qx * g.Nyp;
-            a[u] = b[u] = 0.f;
-            if (e < npix && Qx < g.Lx && Qy < g.Ly) {
-                const int64_t cell = (((int64_t)Qx + g.T) * g.YP + (Qy + g.T)) * g.S;
-                if (in0) a[u] = v0p[cell];
-                if (in1) b[u] = v1p[cell];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kSubUnroll; ++u) {
-            const int e = base + u * kSubThreads + (int)threadIdx.x;
+            const int e = u * kSubThreads + (int)threadIdx.x;
             if (e < npix) bufa[e] = make_float2(a[u], b[u]);
         }
-    }
-    __syncthreads();
-    float2 *r1 = fft_pass_2d(bufa, bufb, dy, ty, -1.0f, g.Nxp, g.Nyp, 1);         // along y (rows)
-    float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dx, tx, -1.0f, g.Nyp, 1, g.Nyp);  // along x
-    const float s0 = qscale ? qscale[q0] : 1.0f;
-    const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
-    const int nk0 = g.Nxp * g.Hy;
-    for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
-        const int kx = k0 / g.Hy, ky = k0 - kx * g.Hy;
-        const float2 z = r2[kx * g.Nyp + ky];
-        const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
-        const float2 zm = r2[mx * g.Nyp + my];
-        const float2 a = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-        const float2 b = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
-        float2 *o = vt + (int64_t)k0 * g.nq + q0;
-        o[0] = make_float2(a.x * s0, a.y * s0);
-        if (has1) o[1] = make_float2(b.x * s1, b.y * s1);
+        lds_barrier();
+        if (pair + it.step < it.end) sub_fwd_fetch(vol, slot_plane, g, pair + it.step, a, b);
+        float2 *r1 = fft_pass_2d(bufa, bufb, dy, ty, -1.0f, g.Nxp, g.Nyp, 1);         // along y (rows)
+        float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dx, tx, -1.0f, g.Nyp, 1, g.Nyp);  // along x
+        const int q0 = 2 * pair;
+        const bool has1 = q0 + 1 < g.nq;
+        const float s0 = qscale ? qscale[q0] : 1.0f;
+        const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
+        for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
+            const int kx = qdiv(k0, g.Hy, rhy), ky = k0 - kx * g.Hy;
+            const float2 z = r2[kx * g.Nyp + ky];
+            const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
+            const float2 zm = r2[mx * g.Nyp + my];
+            const float2 av = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+            const float2 bv = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+            float2 *o = vt + (int64_t)k0 * g.nq + q0;
+            o[0] = make_float2(av.x * s0, av.y * s0);
+            if (has1) o[1] = make_float2(bv.x * s1, bv.y * s1);
+        }
     }
 }
 
 // Backward: s_q * bt[k0][q] (stored half spectrum, Hermitian completion) ->
 // inverse 2-D DFT of A + i B -> real parts a, b into the padded volume layout,
-// scaled by `scale`.
+// scaled by `scale`.  The next pair's spectra are loaded into registers while
+// the current pair is transformed and written out.
+__device__ __forceinline__ void sub_bwd_fetch(const float2 *__restrict__ bt, const SubGeom &g, int pair,
+                                              float2 (&a)[kSubUnroll], float2 (&b)[kSubUnroll],
+                                              unsigned &cjmask)
+{
+    const int q0 = 2 * pair;
+    const bool has1 = q0 + 1 < g.nq;
+    const int npix = g.Nxp * g.Nyp;
+    const float rnyp = 1.0f / (float)g.Nyp;
+    cjmask = 0u;
+#pragma unroll
+    for (int u = 0; u < kSubUnroll; ++u) {
+        const int e = u * kSubThreads + (int)threadIdx.x;
+        a[u] = b[u] = make_float2(0.f, 0.f);
+        if (e < npix) {
+            const int kx = qdiv(e, g.Nyp, rnyp), ky = e - kx * g.Nyp;
+            int sx = kx, sy = ky;
+            if (ky >= g.Hy) {   // conj of the stored value at -k
+                sx = kx == 0 ? 0 : g.Nxp - kx;
+                sy = g.Nyp - ky;
+                cjmask |= 1u << u;
+            }
+            const float2 *src = bt + (int64_t)(sx * g.Hy + sy) * g.nq + q0;
+            a[u] = src[0];
+            if (has1) b[u] = src[1];
+        }
+    }
+}
+
 __global__ __launch_bounds__(kSubThreads) void k_fft_sub_bwd(const float2 *__restrict__ bt,
                                                              const int *__restrict__ slot_plane,
                                                              const float *__restrict__ qscale, SubGeom g, FftDesc dx,
@@ -731,53 +811,42 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_bwd(const float2 *__res
     const int npix = g.Nxp * g.Nyp;
     float2 *bufa = sm, *bufb = sm + npix, *tx = sm + 2 * npix, *ty = tx + g.Nxp;
     load_twiddles(tx, ty, twx, twy, g.Nxp, g.Nyp);
-    const int q0 = 2 * xcd_pair(blockIdx.x, gridDim.x);
-    const bool has1 = q0 + 1 < g.nq;
-    const float s0 = qscale ? qscale[q0] : 1.0f;
-    const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
-    for (int base = 0; base < npix; base += kSubUnroll * kSubThreads) {
-        float2 a[kSubUnroll], b[kSubUnroll];
-        bool cj[kSubUnroll];
+    PairIter it = pair_iter((g.nq + 1) / 2);
+    if (it.next >= it.end) return;   // block-uniform
+    const float rly = 1.0f / (float)g.Ly;
+    const int npx = g.Lx * g.Ly;
+    float2 a[kSubUnroll], b[kSubUnroll];
+    unsigned cj = 0u;
+    sub_bwd_fetch(bt, g, it.next, a, b, cj);
+    for (int pair = it.next; pair < it.end; pair += it.step) {
+        const int q0 = 2 * pair;
+        const bool has1 = q0 + 1 < g.nq;
+        const float s0 = qscale ? qscale[q0] : 1.0f;
+        const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
+        lds_barrier();   // the previous pair's grids are consumed (twiddles loaded, first pass)
 #pragma unroll
         for (int u = 0; u < kSubUnroll; ++u) {
-            const int e = base + u * kSubThreads + (int)threadIdx.x;
-            a[u] = b[u] = make_float2(0.f, 0.f);
-            cj[u] = false;
+            const int e = u * kSubThreads + (int)threadIdx.x;
             if (e < npix) {
-                const int kx = e / g.Nyp, ky = e - kx * g.Nyp;
-                int sx = kx, sy = ky;
-                if (ky >= g.Hy) {   // conj of the stored value at -k
-                    sx = kx == 0 ? 0 : g.Nxp - kx;
-                    sy = g.Nyp - ky;
-                    cj[u] = true;
-                }
-                const float2 *src = bt + (int64_t)(sx * g.Hy + sy) * g.nq + q0;
-                a[u] = src[0];
-                if (has1) b[u] = src[1];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kSubUnroll; ++u) {
-            const int e = base + u * kSubThreads + (int)threadIdx.x;
-            if (e < npix) {
-                const float2 x = make_float2(a[u].x * s0, (cj[u] ? -a[u].y : a[u].y) * s0);
-                const float2 y = make_float2(b[u].x * s1, (cj[u] ? -b[u].y : b[u].y) * s1);
+                const bool c = (cj >> u) & 1u;
+                const float2 x = make_float2(a[u].x * s0, (c ? -a[u].y : a[u].y) * s0);
+                const float2 y = make_float2(b[u].x * s1, (c ? -b[u].y : b[u].y) * s1);
                 bufa[e] = make_float2(x.x - y.y, x.y + y.x);   // A + i B
             }
         }
-    }
-    __syncthreads();
-    float2 *r1 = fft_pass_2d(bufa, bufb, dx, tx, 1.0f, g.Nyp, 1, g.Nyp);          // along x
-    float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dy, ty, 1.0f, g.Nxp, g.Nyp, 1);  // along y
-    const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
-    const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
-    const int npx = g.Lx * g.Ly;
-    for (int e = threadIdx.x; e < npx; e += blockDim.x) {
-        const int Qx = e / g.Ly, Qy = e - Qx * g.Ly;
-        const float2 z = r2[Qx * g.Nyp + Qy];
-        const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
-        dfft[((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0] = z.x * scale;
-        if (has1) dfft[((int64_t)c1 * g.XP * g.YP + cell) * g.S + ab1] = z.y * scale;
+        lds_barrier();
+        if (pair + it.step < it.end) sub_bwd_fetch(bt, g, pair + it.step, a, b, cj);
+        float2 *r1 = fft_pass_2d(bufa, bufb, dx, tx, 1.0f, g.Nyp, 1, g.Nyp);          // along x
+        float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dy, ty, 1.0f, g.Nxp, g.Nyp, 1);  // along y
+        const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
+        const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
+        for (int e = threadIdx.x; e < npx; e += blockDim.x) {
+            const int Qx = qdiv(e, g.Ly, rly), Qy = e - Qx * g.Ly;
+            const float2 z = r2[Qx * g.Nyp + Qy];
+            const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
+            dfft[((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0] = z.x * scale;
+            if (has1) dfft[((int64_t)c1 * g.XP * g.YP + cell) * g.S + ab1] = z.y * scale;
+        }
     }
 }
 
@@ -973,7 +1042,9 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
         const size_t lds = sizeof(float2) * (2 * (size_t)f.Nxp * f.Nyp + f.Nxp + f.Nyp);
         hipDeviceProp_t prop;
         f.fused = std::getenv("SID_FFT_UNFUSED") == nullptr &&
+                  (int64_t)f.Nxp * f.Nyp <= (int64_t)kSubUnroll * kSubThreads &&
                   hipGetDeviceProperties(&prop, p->device) == hipSuccess && lds <= prop.sharedMemPerBlock;
+        f.sub_blocks = f.fused ? std::max(1, std::min(prop.multiProcessorCount, (f.nq + 1) / 2)) : 0;
         if (f.fused && lds > 65536) {
             f.fused = hipFuncSetAttribute((const void *)k_fft_sub_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds) == hipSuccess &&
@@ -1073,7 +1144,7 @@ int fft_forward(sid_psf *p, hipStream_t st)
     if (!f.active) return SID_OK;
     const int n2 = f.n2;
     if (f.fused) {
-        hipLaunchKernelGGL(k_fft_sub_fwd, dim3((f.nq + 1) / 2), dim3(kSubThreads), f.sub_lds, st, sp.vol, f.d_slot_plane,
+        hipLaunchKernelGGL(k_fft_sub_fwd, dim3(f.sub_blocks), dim3(kSubThreads), f.sub_lds, st, sp.vol, f.d_slot_plane,
                            f.qscale, sub_geom(p, f), fft_desc(f.fx), fft_desc(f.fy), f.fx.tw_fwd, f.fy.tw_fwd, f.vt);
         SID_TRY_HIP(hipGetLastError());
     } else {
@@ -1126,7 +1197,7 @@ int fft_backward(sid_psf *p, hipStream_t st)
                            f.grid, f.nq, f.qc_b, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
     gemv_mark(f, st, 1, false);
     if (f.fused) {
-        hipLaunchKernelGGL(k_fft_sub_bwd, dim3((f.nq + 1) / 2), dim3(kSubThreads), f.sub_lds, st, f.vt, f.d_slot_plane,
+        hipLaunchKernelGGL(k_fft_sub_bwd, dim3(f.sub_blocks), dim3(kSubThreads), f.sub_lds, st, f.vt, f.d_slot_plane,
                            f.qscale, sub_geom(p, f), fft_desc(f.fx), fft_desc(f.fy), f.fx.tw_inv, f.fy.tw_inv,
                            1.0f / ((float)f.n2 * (float)f.Nxp * (float)f.Nyp), f.dfft);
         SID_TRY_HIP(hipGetLastError());

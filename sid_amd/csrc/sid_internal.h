@@ -58,9 +58,12 @@ void set_error(const char *fmt, ...);
 // the B operand is K x NP contiguous at w_off.
 struct Seg {
     int64_t a_off;
-    int64_t w_off;
+    int64_t w_off;     // FP32 / FP64 weights: K = taps * S padded to KPAD
     int32_t K;
     int32_t plane;
+    int64_t w_off2;    // BF16 weights: K2 = taps * KT (each tap padded to KT = S rounded up to 32)
+    int32_t K2;
+    int32_t pad_;
 };
 
 // One block of work: segments [seg0, seg1) for M tile `mtile`, written to
@@ -77,6 +80,8 @@ struct Plane {
     int tb_lo, tb_hi;    // backward taps
     int64_t Kf, Kb;      // padded GEMM K per tap row
     int64_t wf_off, wb_off;  // weight offsets (elements)
+    int64_t Kf2, Kb2;        // BF16 layout: taps * KT
+    int64_t wf2_off, wb2_off;
     bool empty;
 };
 
@@ -111,6 +116,7 @@ struct FftState {
     int nsplit_b = 1, qc_b = 0;    // the same for the backward GEMV
     bool fused = false;            // sub-image transforms in one LDS-resident kernel per pair
     size_t sub_lds = 0;
+    int sub_blocks = 0;            // persistent grid of the fused kernels (one block per CU)
     FftPlan1D fx, fy, fX, fY;      // lengths Nx', Ny', Nx, Ny
     void *G = nullptr;             // spectra [nk0][nq][n2] (k0 blocks kstride elements apart):
                                    // float2, or __half2 scaled by 1/qscale[q] when `half`
@@ -161,6 +167,8 @@ struct sid_psf {
     sid::u16 *wf_lo = nullptr, *wb_lo = nullptr;  // bf16 lo weights (x3 mode)
     double *wf_d = nullptr, *wb_d = nullptr;      // fp64 weights
     int64_t wf_elems = 0, wb_elems = 0;
+    int64_t wf2_elems = 0, wb2_elems = 0;   // BF16 weights (tap-padded layout)
+    int64_t KT = 0;                         // BF16 K per tap (S rounded up to 32)
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
     std::vector<char> auto_fft;       // autotuned per-plane FFT choice (SID_METHOD_AUTO)
