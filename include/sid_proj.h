@@ -101,6 +101,14 @@ int sid_psf_set_precision(sid_psf *psf, int precision);
 /* Force a method for every plane (SID_METHOD_DIRECT / _FFT) or restore the
  * autotuned per-plane choice (SID_METHOD_AUTO). */
 int sid_psf_set_method(sid_psf *psf, int method);
+/* Pin the method of single planes under SID_METHOD_AUTO: methods[c] is
+ * SID_METHOD_DIRECT or SID_METHOD_FFT for plane c, or SID_METHOD_AUTO to leave
+ * that plane to the autotuner; n = size(H,5).  methods = NULL clears every pin.
+ * A plane the FFT path cannot take (empty, or an even kernel support) is
+ * rejected with SID_ERR_ARG.  No counterpart in the reference, whose
+ * forwardProjectGPU.m convolves every plane by FFT; for callers that fix the
+ * plan of a workload they have already tuned. */
+int sid_psf_set_plane_methods(sid_psf *psf, const int32_t *methods, int64_t n);
 
 typedef struct {
     int64_t M, Nnum, Nz;

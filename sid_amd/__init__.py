@@ -114,6 +114,15 @@ class PSF:
     def set_method(self, method: str):
         check(lib().sid_psf_set_method(self._h, METHOD[method]))
 
+    def set_plane_methods(self, methods=None):
+        """Pin planes to "direct" / "fft" ("auto" leaves a plane to the
+        autotuner); one entry per plane of H, None clears every pin."""
+        if methods is None:
+            check(lib().sid_psf_set_plane_methods(self._h, None, 0))
+            return
+        arr = (ctypes.c_int32 * len(methods))(*[METHOD[m] for m in methods])
+        check(lib().sid_psf_set_plane_methods(self._h, arr, len(methods)))
+
     def info(self) -> dict:
         inf = _lib.PsfInfo()
         check(lib().sid_psf_get_info(self._h, ctypes.byref(inf)))

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -42,7 +43,8 @@ static bool is_gfx950(int dev)
 // Event pool used for the per-phase timing of the last call.
 struct Timer {
     sid_psf *p;
-    std::vector<std::pair<int, int>> spans;  // (phase, event index start); end = start + 1
+    struct Span { int phase, e0, e1; };
+    std::vector<Span> spans;
     std::vector<hipEvent_t> evs;
     hipStream_t st;
     explicit Timer(sid_psf *p_, hipStream_t s) : p(p_), st(s) {}
@@ -50,22 +52,25 @@ struct Timer {
     {
         for (auto e : evs) (void)hipEventDestroy(e);
     }
-    int mark()
+    // event recorded on s (default: the timer's stream); -1 when no event could be made
+    int mark(hipStream_t s = nullptr)
     {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return -1;
         evs.push_back(e);
-        (void)hipEventRecord(e, st);
+        (void)hipEventRecord(e, s ? s : st);
         return (int)evs.size() - 1;
     }
-    void span(int phase, int e0) { spans.push_back({phase, e0}); }
+    void span(int phase, int e0) { spans.push_back({phase, e0, e0 + 1}); }
+    void span(int phase, int e0, int e1) { spans.push_back({phase, e0, e1}); }
     void finish(int e_start, int e_end)
     {
         for (int i = 0; i < 12; ++i) p->timing[i] = 0.0;
         fft_timing_collect(p, &p->timing[7], &p->timing[8]);
         for (auto &s : spans) {
             float ms = 0.f;
-            if (hipEventElapsedTime(&ms, evs[s.second], evs[s.second + 1]) == hipSuccess) p->timing[s.first] += ms;
+            if (s.e0 >= 0 && s.e1 >= 0 && hipEventElapsedTime(&ms, evs[s.e0], evs[s.e1]) == hipSuccess)
+                p->timing[s.phase] += ms;
         }
         float ms = 0.f;
         if (e_start >= 0 && e_end >= 0 && hipEventElapsedTime(&ms, evs[e_start], evs[e_end]) == hipSuccess)
@@ -144,6 +149,12 @@ static void destroy(sid_psf *p)
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
                     (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag})
         if (q) (void)hipFree(q);
+    if (p->aux) {
+        (void)hipStreamSynchronize(p->aux);
+        (void)hipStreamDestroy(p->aux);
+    }
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -205,6 +216,14 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
             rc = SID_ERR_HIP;
             set_error("hipStreamCreate failed");
             break;
+        }
+        const char *ser = std::getenv("SID_NO_OVERLAP");
+        p->overlap = !(ser && ser[0] && ser[0] != '0');
+        const char *gb = std::getenv("SID_GEMM_BLOCKS");
+        p->gemm_blocks = gb ? std::max(0, std::atoi(gb)) : 0;
+        if (p->gemm_blocks == 0) {
+            hipDeviceProp_t prop;
+            p->gemm_blocks = hipGetDeviceProperties(&prop, p->device) == hipSuccess ? prop.multiProcessorCount : 256;
         }
         for (int i = 0; i < 5; ++i)
             if (dims[i] < 1) { rc = SID_ERR_DIMS; set_error("H has an empty dimension"); }
@@ -323,6 +342,35 @@ int sid_psf_set_method(sid_psf *p, int method)
     return SID_OK;
 }
 
+int sid_psf_set_plane_methods(sid_psf *p, const int32_t *methods, int64_t n)
+{
+    if (!p) { set_error("null handle"); return SID_ERR_ARG; }
+    if (!methods) {
+        p->pin.clear();
+        return SID_OK;
+    }
+    if (n != p->nzH) {
+        set_error("sid_psf_set_plane_methods: %lld entries for %lld planes", (long long)n, (long long)p->nzH);
+        return SID_ERR_ARG;
+    }
+    std::vector<char> pin(n, 0);
+    for (int64_t c = 0; c < n; ++c) {
+        const int m = methods[c];
+        if (m != SID_METHOD_AUTO && m != SID_METHOD_DIRECT && m != SID_METHOD_FFT) {
+            set_error("plane %lld: unknown method %d", (long long)c + 1, m);
+            return SID_ERR_ARG;
+        }
+        if (m == SID_METHOD_FFT && !fft_eligible(p, c)) {
+            set_error("plane %lld cannot take the FFT path (empty, or an even support whose adjoint is not the "
+                      "reference backward)", (long long)c + 1);
+            return SID_ERR_ARG;
+        }
+        pin[c] = (char)m;
+    }
+    p->pin = pin;
+    return SID_OK;
+}
+
 int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
 {
     if (!p || !info) { set_error("null argument"); return SID_ERR_ARG; }
@@ -363,24 +411,156 @@ static int check_vdims(sid_psf *p, const int64_t vdims[3], bool rl)
     return SID_OK;
 }
 
+static int ensure_aux(sid_psf *p)
+{
+    if (p->aux) return SID_OK;
+    SID_TRY_HIP(hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking));
+    SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    return SID_OK;
+}
+
+// The two halves of one projection (dir 0 forward, 1 backward): the direct
+// GEMM over the direct planes (MFMA-bound) and the FFT path over the others
+// (bound by the spectra stream from HBM).  They read the same staged volume or
+// image and write disjoint buffers (split-K slabs; FFT grid and volume spectra),
+// so when both are present the GEMM runs on the handle's second stream, forked
+// from st after the staging and joined back before the reduction that sums the
+// two.  Phase times go to timing[ph_gemm] and timing[ph_fft].
+static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph_gemm, int ph_fft)
+{
+    const GemmPlan &g = dir == 0 ? p->plan.fwd : p->plan.bwd;
+    if (!(p->overlap && p->plan.fft.active && !g.items.empty())) {
+        const int a = tm.mark();
+        SID_TRY(launch_gemm(p, dir, st));
+        const int b = tm.mark();
+        SID_TRY(dir == 0 ? fft_forward(p, st) : fft_backward(p, st));
+        const int c = tm.mark();
+        tm.span(ph_gemm, a, b);
+        tm.span(ph_fft, b, c);
+        return SID_OK;
+    }
+    SID_TRY(ensure_aux(p));
+    SID_TRY_HIP(hipEventRecord(p->ev_fork, st));
+    SID_TRY_HIP(hipStreamWaitEvent(p->aux, p->ev_fork, 0));
+    const int a = tm.mark(p->aux);
+    const int rc_g = launch_gemm(p, dir, p->aux, p->gemm_blocks);
+    const int b = tm.mark(p->aux);
+    // join even after a failed launch so that st never runs ahead of work left on aux
+    SID_TRY_HIP(hipEventRecord(p->ev_join, p->aux));
+    const int c = tm.mark();
+    const int rc_f = rc_g == SID_OK ? (dir == 0 ? fft_forward(p, st) : fft_backward(p, st)) : SID_OK;
+    const int d = tm.mark();
+    SID_TRY_HIP(hipStreamWaitEvent(st, p->ev_join, 0));
+    SID_TRY(rc_g);
+    SID_TRY(rc_f);
+    tm.span(ph_gemm, a, b);
+    tm.span(ph_fft, c, d);
+    return SID_OK;
+}
+
+// Mean device time (ms) of one forward plus one backward projection, both
+// halves as the projector runs them, on the current plan.  The staged volume
+// and image hold whatever the plan buffers hold: no kernel's time depends on
+// the values.
+static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = 0.f;
+    *rc = SID_OK;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) *rc = SID_ERR_HIP;
+    {
+        Timer tm(p, st);
+        for (int r = 0; r < 3 && *rc == SID_OK; ++r) {  // r = 0 warms the plan up
+            if (r == 1) (void)hipEventRecord(a, st);
+            *rc = project_halves(p, 0, st, tm, 0, 5);
+            if (*rc == SID_OK) *rc = project_halves(p, 1, st, tm, 2, 6);
+        }
+        if (*rc == SID_OK && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                              hipEventElapsedTime(&ms, a, b) != hipSuccess))
+            *rc = SID_ERR_HIP;
+        if (*rc != SID_OK) (void)hipStreamSynchronize(st);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return ms / 2;
+}
+
+// Split of the planes between the GEMM and the FFT path when the two halves
+// overlap.  autotune_methods prices each plane alone (direct GEMM time against
+// FFT-path time), which minimises the serial sum; overlapped, a projection
+// costs about the longer half plus their interference, which favours giving
+// the GEMM more planes.  Starting from that choice (k FFT planes), time the
+// overlapped forward + backward pair with the narrowest FFT planes moved to the
+// GEMM s at a time (s about a ninth of k) while that helps, or with s more
+// FFT planes when the first move does not, and keep the fastest.  Candidate
+// plans take the FFT planes widest first.  Skipped when planes are pinned or
+// the overlap is off; cached with the per-plane choice.
+static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st)
+{
+    SID_TRY(autotune_methods(p, X, Y, nz, st));
+    if (p->method != SID_METHOD_AUTO || p->precision == SID_PREC_FP64 || p->auto_refined) return SID_OK;
+    p->auto_refined = true;
+    if (!p->overlap || !p->pin.empty() || p->auto_fft.size() != (size_t)p->nzH) return SID_OK;
+    std::vector<int> order;  // eligible planes, widest support first
+    for (int64_t c = 0; c < p->nzH; ++c)
+        if (fft_eligible(p, c)) order.push_back((int)c);
+    auto width = [&](int c) { return p->planes[c].hi - p->planes[c].lo; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return width(a) > width(b); });
+    int k0 = 0;
+    for (int c : order) k0 += p->auto_fft[c];
+    if (k0 == 0) return SID_OK;
+    const int s = std::max(1, (k0 + 4) / 9), ne = (int)order.size();
+    auto use_prefix = [&](int k) {
+        std::fill(p->auto_fft.begin(), p->auto_fft.end(), 0);
+        for (int i = 0; i < k; ++i) p->auto_fft[order[i]] = 1;
+    };
+    int rc = SID_OK;
+    auto eval = [&](int k) {
+        use_prefix(k);
+        rc = ensure_plan(p, X, Y, nz);
+        return rc == SID_OK ? time_projection_pair(p, st, &rc) : 0.0;
+    };
+    int best_k = k0;
+    double best_t = eval(k0);
+    bool moved = false;
+    for (int k = k0 - s; rc == SID_OK && k >= 0; k -= s) {
+        const double t = eval(k);
+        if (rc != SID_OK || t >= 0.995 * best_t) break;
+        best_t = t;
+        best_k = k;
+        moved = true;
+    }
+    if (rc == SID_OK && !moved && k0 + s <= ne) {
+        const double t = eval(k0 + s);
+        if (rc == SID_OK && t < 0.995 * best_t) best_k = k0 + s;
+    }
+    if (rc == SID_OK) {
+        use_prefix(best_k);
+        rc = ensure_plan(p, X, Y, nz);
+    }
+    double d0, d1;
+    fft_timing_collect(p, &d0, &d1);  // drop the GEMV events of the trial runs
+    if (rc != SID_OK) {
+        p->auto_key[0] = -1;  // measure again on the next call
+        return rc;
+    }
+    p->auto_ms[3] = best_k;
+    return SID_OK;
+}
+
 static int forward_raw(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
 {
-    SID_TRY(autotune_methods(p, vdims[0], vdims[1], vdims[2], st));
+    SID_TRY(autotune(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     Timer tm(p, st);
     const int e0 = tm.mark();
     SID_TRY(launch_vol_in(p, V_dev, vdims[2], 0, st));
-    const int e1 = tm.mark();
-    SID_TRY(launch_gemm(p, 0, st));
-    const int e1b = tm.mark();
-    SID_TRY(fft_forward(p, st));
+    SID_TRY(project_halves(p, 0, st, tm, 0, 5));
     const int e2 = tm.mark();
     SID_TRY(launch_fwd_reduce(p, O_dev, st));
     const int e3 = tm.mark();
-    tm.span(0, e1);
-    tm.span(5, e1b);
-    tm.span(1, e2);
-    (void)e3;
+    tm.span(1, e2, e3);
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e3);
     return SID_OK;
@@ -391,21 +571,16 @@ static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], 
     // the backward volume always has all planes of H (backwardProjectGPU_new.m:4-5); the
     // sensor plan keeps the nz of the last forward call.
     const int64_t nz = (p->plan.vol && p->plan.X == odims[0] && p->plan.Y == odims[1]) ? p->plan.nz : p->nzH;
-    SID_TRY(autotune_methods(p, odims[0], odims[1], nz, st));
+    SID_TRY(autotune(p, odims[0], odims[1], nz, st));
     SID_TRY(ensure_plan(p, odims[0], odims[1], nz));
     Timer tm(p, st);
     const int e0 = tm.mark();
     SID_TRY(launch_img_in(p, O_dev, st));
-    const int e1 = tm.mark();
-    SID_TRY(launch_gemm(p, 1, st));
-    const int e1b = tm.mark();
-    SID_TRY(fft_backward(p, st));
+    SID_TRY(project_halves(p, 1, st, tm, 2, 6));
     const int e2 = tm.mark();
     SID_TRY(launch_bwd_reduce(p, B_dev, 0, st));
     const int e3 = tm.mark();
-    tm.span(2, e1);
-    tm.span(6, e1b);
-    tm.span(3, e2);
+    tm.span(3, e2, e3);
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e3);
     return SID_OK;
@@ -513,7 +688,7 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
                    hipStream_t st)
 {
     if (p->kern) return rl_wrapped(p, Htf, X, vdims, max_iter, hist, st);
-    SID_TRY(autotune_methods(p, vdims[0], vdims[1], vdims[2], st));
+    SID_TRY(autotune(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     SensorPlan &sp = p->plan;
     const int64_t nv = vdims[0] * vdims[1] * vdims[2];
@@ -522,21 +697,16 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
     SID_TRY(launch_vol_in(p, Htf, vdims[2], 1, st));
     SID_TRY(launch_vol_in(p, X, vdims[2], 0, st));
     for (int it = 0; it < max_iter; ++it) {
-        const int a = tm.mark();
-        SID_TRY(launch_gemm(p, 0, st));
-        const int a2 = tm.mark();
-        SID_TRY(fft_forward(p, st));
+        SID_TRY(project_halves(p, 0, st, tm, 0, 5));
         const int b = tm.mark();
         SID_TRY(launch_fwd_reduce(p, nullptr, st));
         const int c = tm.mark();
-        SID_TRY(launch_gemm(p, 1, st));
-        const int c2 = tm.mark();
-        SID_TRY(fft_backward(p, st));
+        SID_TRY(project_halves(p, 1, st, tm, 2, 6));
         const int d = tm.mark();
         SID_TRY(launch_bwd_reduce(p, hist ? hist + (int64_t)it * nv : nullptr, 1, st));
         const int e = tm.mark();
-        tm.span(0, a); tm.span(5, a2); tm.span(1, b); tm.span(2, c); tm.span(6, c2); tm.span(3, d);
-        (void)e;
+        tm.span(1, b, c);
+        tm.span(3, d, e);
     }
     SID_TRY(launch_vol_out(p, X, vdims[2], st));
     const int e1 = tm.mark();

@@ -33,9 +33,8 @@
 
 namespace sid {
 
-// K chunk of the staged B operand (elements of K per LDS stage).
-template <int MODE> struct KChunk { static constexpr int value = 64; };
-template <> struct KChunk<0> { static constexpr int value = 32; };
+// K chunk of the staged FP32 B operand (elements of K per LDS stage).
+constexpr int kKcF32 = 32;
 
 // ---------------------------------------------------------------------------
 // Support scan: smallest index box holding every nonzero of a plane.
@@ -111,31 +110,37 @@ __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, 
 }
 
 // ---------------------------------------------------------------------------
-// The projector GEMM.  256 threads = 4 waves; each wave owns RT*16 rows of the
-// M tile and all NT*16 columns.  B chunks are staged in LDS and shared by the
-// 4 waves; A is read straight from global memory (L2-resident rows, each row
-// a contiguous Toeplitz window).
+// The FP32 projector GEMM.  256 threads = 4 waves; each wave owns RT*16 rows of
+// the M tile and all NT*16 columns.  B chunks [NP][32] are staged in LDS and
+// shared by the 4 waves; A is read straight from global memory (L2-resident
+// rows, each row a contiguous Toeplitz window).
 //
-// MODE 0 (FP32): v_mfma_f32_16x16x4_f32.  Lane (l16, g) = (lane & 15,
-//   lane >> 4) feeds A[row l16][k = 8g + s] and B[k = 8g + s][col l16] at step
-//   s = 0..7 of a 32-wide K chunk, so a lane's 8 A values are contiguous.
-// MODE 1/2 (BF16): v_mfma_f32_16x16x32_bf16 with A[row l16][8g..8g+7],
-//   B[8g..8g+7][col l16].  MODE 1: C += Ahi*Bhi + Alo*Bhi.  MODE 2 adds
-//   Ahi*Blo.
-// C layout (both): lane holds C[4g + i][l16], i = 0..3.
+// v_mfma_f32_16x16x4_f32: lane (l16, g) = (lane & 15, lane >> 4) feeds
+// A[row l16][k = 8g + s] and B[k = 8g + s][col l16] at step s = 0..7 of a
+// 32-wide K chunk, so a lane's 8 A values are contiguous.  C layout: lane
+// holds C[4g + i][l16], i = 0..3.
 // ---------------------------------------------------------------------------
-template <int NT, int RT, int MODE>
-__global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
-    const Item *__restrict__ items, const Seg *__restrict__ segs, const void *__restrict__ A0,
-    const void *__restrict__ A1, const void *__restrict__ W0, const void *__restrict__ W1,
-    float *__restrict__ part, int nrows, int Ly, int YP, int S, int64_t slab_stride)
+template <int NT>
+struct F32Shape {
+    static constexpr int NP = NT * 16;
+    static constexpr int KC = kKcF32;
+    static constexpr int LDK = KC + 4;  // floats per staged column
+};
+
+// One work item; the barrier before each stage also orders it after the
+// previous item's last LDS reads.
+template <int NT, int RT>
+__device__ __forceinline__ void f32_item(const Item it, float (&Bs)[F32Shape<NT>::NP * F32Shape<NT>::LDK],
+                                         const Seg *__restrict__ segs, const float *__restrict__ A0,
+                                         const float *__restrict__ W0, float *__restrict__ part, int nrows, int Ly,
+                                         int YP, int S, int64_t slab_stride)
 {
-    constexpr int NP = NT * 16;
+    constexpr int NP = F32Shape<NT>::NP;
     constexpr int MT = 4 * RT * 16;
-    constexpr int KC = KChunk<MODE>::value;
+    constexpr int KC = F32Shape<NT>::KC;
+    constexpr int LDK = F32Shape<NT>::LDK;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
-    const Item it = items[blockIdx.x];
 
     int64_t rowoff[RT];
 #pragma unroll
@@ -151,95 +156,41 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
 
-    if constexpr (MODE == 0) {
-        constexpr int LDK = KC + 4;  // floats per staged column
-        __shared__ float Bs[NP * LDK];
-        for (int s = it.seg0; s < it.seg1; ++s) {
-            const Seg sg = segs[s];
-            const float *A = (const float *)A0 + sg.a_off;
-            const float *W = (const float *)W0 + sg.w_off;
-            for (int k0 = 0; k0 < sg.K; k0 += KC) {
-                float4 a0[RT], a1[RT];
+    for (int s = it.seg0; s < it.seg1; ++s) {
+        const Seg sg = segs[s];
+        const float *A = A0 + sg.a_off;
+        const float *W = W0 + sg.w_off;
+        for (int k0 = 0; k0 < sg.K; k0 += KC) {
+            float4 a0[RT], a1[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const float *pa = A + rowoff[rt] + k0 + 8 * g;
+                a0[rt] = *(const float4 *)pa;
+                a1[rt] = *(const float4 *)(pa + 4);
+            }
+            __syncthreads();
+            const float *wc = W + (int64_t)k0 * NP;  // chunk: [NP][KC]
+            for (int e = threadIdx.x; e < NP * KC / 4; e += NTHREADS) {
+                const int col = e / (KC / 4), kq = e - col * (KC / 4);
+                *(float4 *)(Bs + col * LDK + 4 * kq) = *(const float4 *)(wc + 4 * e);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                const float *pb = Bs + (ct * 16 + l16) * LDK + 8 * g;
+                const float4 b0 = *(const float4 *)pb, b1 = *(const float4 *)(pb + 4);
 #pragma unroll
                 for (int rt = 0; rt < RT; ++rt) {
-                    const float *pa = A + rowoff[rt] + k0 + 8 * g;
-                    a0[rt] = *(const float4 *)pa;
-                    a1[rt] = *(const float4 *)(pa + 4);
-                }
-                __syncthreads();
-                const float *wc = W + (int64_t)k0 * NP;  // chunk: [NP][KC]
-                for (int e = threadIdx.x; e < NP * KC / 4; e += NTHREADS) {
-                    const int col = e / (KC / 4), kq = e - col * (KC / 4);
-                    *(float4 *)(Bs + col * LDK + 4 * kq) = *(const float4 *)(wc + 4 * e);
-                }
-                __syncthreads();
-#pragma unroll
-                for (int ct = 0; ct < NT; ++ct) {
-                    const float *pb = Bs + (ct * 16 + l16) * LDK + 8 * g;
-                    const float4 b0 = *(const float4 *)pb, b1 = *(const float4 *)(pb + 4);
-#pragma unroll
-                    for (int rt = 0; rt < RT; ++rt) {
-                        f4 c = acc[rt][ct];
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].x, b0.x, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].y, b0.y, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].z, b0.z, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].w, b0.w, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].x, b1.x, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].y, b1.y, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].z, b1.z, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].w, b1.w, c, 0, 0, 0);
-                        acc[rt][ct] = c;
-                    }
-                }
-            }
-        }
-    } else {
-        constexpr int LDK = KC + 8;  // bf16 elements per staged column
-        constexpr int NB = MODE == 2 ? 2 : 1;
-        __shared__ u16 Bs[NB * NP * LDK];
-        for (int s = it.seg0; s < it.seg1; ++s) {
-            const Seg sg = segs[s];
-            const u16 *Ah = (const u16 *)A0 + sg.a_off;
-            const u16 *Al = (const u16 *)A1 + sg.a_off;
-            const u16 *Wh = (const u16 *)W0 + sg.w_off;
-            const u16 *Wl = (const u16 *)W1 + sg.w_off;
-            for (int k0 = 0; k0 < sg.K; k0 += KC) {
-                bf8 ah[RT][KC / 32], al[RT][KC / 32];
-#pragma unroll
-                for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-                    for (int ks = 0; ks < KC / 32; ++ks) {
-                        const int64_t o = rowoff[rt] + k0 + 32 * ks + 8 * g;
-                        ah[rt][ks] = *(const bf8 *)(Ah + o);
-                        al[rt][ks] = *(const bf8 *)(Al + o);
-                    }
-                __syncthreads();
-                const int64_t wo = (int64_t)k0 * NP;
-                for (int e = threadIdx.x; e < NP * KC / 8; e += NTHREADS) {
-                    const int col = e / (KC / 8), kq = e - col * (KC / 8);
-                    *(uint4 *)(Bs + col * LDK + 8 * kq) = *(const uint4 *)(Wh + wo + 8 * e);
-                    if constexpr (MODE == 2)
-                        *(uint4 *)(Bs + NP * LDK + col * LDK + 8 * kq) = *(const uint4 *)(Wl + wo + 8 * e);
-                }
-                __syncthreads();
-#pragma unroll
-                for (int ct = 0; ct < NT; ++ct) {
-#pragma unroll
-                    for (int ks = 0; ks < KC / 32; ++ks) {
-                        const bf8 bh = *(const bf8 *)(Bs + (ct * 16 + l16) * LDK + 32 * ks + 8 * g);
-                        bf8 bl;
-                        if constexpr (MODE == 2)
-                            bl = *(const bf8 *)(Bs + NP * LDK + (ct * 16 + l16) * LDK + 32 * ks + 8 * g);
-#pragma unroll
-                        for (int rt = 0; rt < RT; ++rt) {
-                            f4 c = acc[rt][ct];
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][ks], bh, c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt][ks], bh, c, 0, 0, 0);
-                            if constexpr (MODE == 2)
-                                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt][ks], bl, c, 0, 0, 0);
-                            acc[rt][ct] = c;
-                        }
-                    }
+                    f4 c = acc[rt][ct];
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].x, b0.x, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].y, b0.y, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].z, b0.z, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[rt].w, b0.w, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].x, b1.x, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].y, b1.y, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].z, b1.z, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[rt].w, b1.w, c, 0, 0, 0);
+                    acc[rt][ct] = c;
                 }
             }
         }
@@ -254,6 +205,20 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm(
 #pragma unroll
             for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + i) * NP + ct * 16 + l16] = acc[rt][ct][i];
     }
+}
+
+// Blocks take items blockIdx.x, blockIdx.x + gridDim.x, ...: one item per
+// block with the full grid, or a capped grid that leaves CUs to a kernel
+// running beside it on another stream.
+template <int NT, int RT>
+__global__ __launch_bounds__(NTHREADS, 2) void k_proj_gemm(const Item *__restrict__ items, int nitems,
+                                                        const Seg *__restrict__ segs, const float *__restrict__ A0,
+                                                        const float *__restrict__ W0, float *__restrict__ part,
+                                                        int nrows, int Ly, int YP, int S, int64_t slab_stride)
+{
+    __shared__ float Bs[F32Shape<NT>::NP * F32Shape<NT>::LDK];
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x)
+        f32_item<NT, RT>(items[i], Bs, segs, A0, W0, part, nrows, Ly, YP, S, slab_stride);
 }
 
 
@@ -319,23 +284,34 @@ __device__ __forceinline__ void sl_stash_b(const u32x4 (&pre)[NB][VPT], u16 *bs)
     }
 }
 
+template <int NT, int MODE>
+struct SlShape {
+    static constexpr int NP = NT * 16;
+    static constexpr int KW = 32;            // K per step
+    static constexpr int LDK = KW + 8;       // staged column pitch (BF16 elements)
+    static constexpr int NB = MODE == 2 ? 2 : 1;
+};
+
+// One work item (an M tile over a run of segments) of the sliding GEMM.  Bs is
+// the block's LDS; the barrier before the first stash also orders this item's
+// staging after the previous item's last reads.
 template <int NT, int RT, int MODE>
-__global__ __launch_bounds__(NTHREADS) void k_proj_gemm_sl(
-    const Item *__restrict__ items, const Seg *__restrict__ segs, const u16 *__restrict__ A0,
-    const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
-    int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
+__device__ __forceinline__ void sl_item(
+    const Item it, u16 (&Bs)[2][SlShape<NT, MODE>::NB][SlShape<NT, MODE>::NP * SlShape<NT, MODE>::LDK],
+    const Seg *__restrict__ segs, const u16 *__restrict__ A0, const u16 *__restrict__ A1,
+    const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part, int nrows, int Ly, int YP,
+    int S, int KT, int64_t slab_stride)
 {
-    constexpr int NP = NT * 16;
+    using SH = SlShape<NT, MODE>;
+    constexpr int NP = SH::NP;
     constexpr int MT = 4 * RT * 16;
-    constexpr int KW = 32;               // K per step
-    constexpr int LDK = KW + 8;          // staged column pitch (BF16 elements)
-    constexpr int NB = MODE == 2 ? 2 : 1;
+    constexpr int KW = SH::KW;
+    constexpr int LDK = SH::LDK;
+    constexpr int NB = SH::NB;
     constexpr int NV = NP * KW / 8;      // 16-byte vectors per staged part
     constexpr int VPT = (NV + NTHREADS - 1) / NTHREADS;
-    __shared__ u16 Bs[2][NB][NP * LDK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
-    const Item it = items[blockIdx.x];
 
     int64_t rowoff[RT];
     bool fresh[RT];
@@ -439,6 +415,21 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_sl(
 #pragma unroll
             for (int i = 0; i < 4; ++i) out[(int64_t)(rbase + i) * NP + ct * 16 + l16] = acc[rt][ct][i];
     }
+}
+
+// Blocks take items blockIdx.x, blockIdx.x + gridDim.x, ...: one item per
+// block with the full grid, or a capped grid that leaves CUs to a kernel
+// running beside it on another stream.
+template <int NT, int RT, int MODE>
+__global__ __launch_bounds__(NTHREADS, MODE == 2 ? 1 : 2) void k_proj_gemm_sl(
+    const Item *__restrict__ items, int nitems, const Seg *__restrict__ segs, const u16 *__restrict__ A0,
+    const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
+    int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
+{
+    using SH = SlShape<NT, MODE>;
+    __shared__ u16 Bs[2][SH::NB][SH::NP * SH::LDK];
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x)
+        sl_item<NT, RT, MODE>(items[i], Bs, segs, A0, A1, W0, W1, part, nrows, Ly, YP, S, KT, slab_stride);
 }
 
 // FP64 variant (reference path for error studies): v_mfma_f64_16x16x4_f64
@@ -760,7 +751,7 @@ int build_weights(sid_psf *p, int which)
     if (*dstf) return SID_OK;
     // FP32 / FP64: [K / 32][NP][32] with K = taps * S (padded); BF16: the same with each tap padded to KT
     const bool bf = which == 1 || which == 2;
-    const int KC = bf ? 32 : KChunk<0>::value;
+    const int KC = bf ? 32 : kKcF32;
     const int64_t ef = bf ? p->wf2_elems : p->wf_elems, eb = bf ? p->wb2_elems : p->wb_elems;
     const int Sk = bf ? (int)p->KT : (int)p->S;
     SID_TRY_HIP(hipMalloc(dstf, std::max<int64_t>(ef, 1) * esz));
@@ -868,7 +859,9 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         for (int64_t c = 0; c < p->nzH; ++c) {
             if (!fft_eligible(p, c)) continue;
             if (p->method == SID_METHOD_FFT) use[c] = 1;
-            else if (p->method == SID_METHOD_AUTO && (size_t)c < p->auto_fft.size()) use[c] = p->auto_fft[c];
+            else if (p->method != SID_METHOD_AUTO) continue;
+            else if ((size_t)c < p->pin.size() && p->pin[c]) use[c] = p->pin[c] == SID_METHOD_FFT;
+            else if ((size_t)c < p->auto_fft.size()) use[c] = p->auto_fft[c];
         }
     }
     // the FFT state stores FP16 spectra in the BF16X2 mode and FP32 otherwise
@@ -1027,57 +1020,52 @@ int ensure_precision_buffers(sid_psf *p)
     return SID_OK;
 }
 
-template <int NT, int RT, int MODE>
-static void launch_one(const GemmPlan &g, const void *A0, const void *A1, const void *W0, const void *W1,
-                       const SensorPlan &sp, int S, hipStream_t st)
-{
-    hipLaunchKernelGGL((k_proj_gemm<NT, RT, MODE>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
-                       g.d_items, g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
-                       sp.mpad * (int64_t)(NT * 16));
-}
-
 template <int NT>
-static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st)
+static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st, int max_blocks)
 {
     const SensorPlan &sp = p->plan;
     const int S = (int)p->S;
     const bool fwd = dir == 0;
+    const int nitems = (int)g.items.size();
+    const unsigned grid = (unsigned)(max_blocks > 0 ? std::min(nitems, max_blocks) : nitems);
+    const int64_t slab = sp.mpad * (int64_t)(NT * 16);
     if (p->precision == SID_PREC_FP64) {
-        hipLaunchKernelGGL((k_proj_gemm_f64<NT, 2>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
-                           g.d_items, g.d_segs, fwd ? sp.vol_d : sp.img_d, fwd ? p->wf_d : p->wb_d, sp.part_d,
-                           (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S, sp.mpad * (int64_t)(NT * 16));
+        hipLaunchKernelGGL((k_proj_gemm_f64<NT, 2>), dim3((unsigned)nitems), dim3(NTHREADS), 0, st, g.d_items,
+                           g.d_segs, fwd ? sp.vol_d : sp.img_d, fwd ? p->wf_d : p->wb_d, sp.part_d, (int)sp.nrows,
+                           (int)sp.Ly, (int)sp.YP, S, slab);
     } else if (p->precision == SID_PREC_FP32) {
-        launch_one<NT, 2, 0>(g, fwd ? (const void *)sp.vol : (const void *)sp.img, nullptr,
-                             fwd ? (const void *)p->wf : (const void *)p->wb, nullptr, sp, S, st);
+        hipLaunchKernelGGL((k_proj_gemm<NT, 2>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems, g.d_segs,
+                           fwd ? sp.vol : sp.img, fwd ? p->wf : p->wb, sp.part, (int)sp.nrows, (int)sp.Ly,
+                           (int)sp.YP, S, slab);
     } else {
         const u16 *A0 = fwd ? sp.vol_hi : sp.img_hi, *A1 = fwd ? sp.vol_lo : sp.img_lo;
         const u16 *W0 = fwd ? p->wf_hi : p->wb_hi;
         const u16 *W1 = p->precision == SID_PREC_BF16X3 ? (fwd ? p->wf_lo : p->wb_lo) : W0;
         if (p->precision == SID_PREC_BF16X2)
-            hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 1>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
-                               g.d_items, g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
-                               (int)p->KT, sp.mpad * (int64_t)(NT * 16));
+            hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 1>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems,
+                               g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               (int)p->KT, slab);
         else
-            hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 2>), dim3((unsigned)g.items.size()), dim3(NTHREADS), 0, st,
-                               g.d_items, g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
-                               (int)p->KT, sp.mpad * (int64_t)(NT * 16));
+            hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 2>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems,
+                               g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               (int)p->KT, slab);
     }
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
-int launch_gemm(sid_psf *p, int dir, hipStream_t st)
+int launch_gemm(sid_psf *p, int dir, hipStream_t st, int max_blocks)
 {
     const GemmPlan &g = dir == 0 ? p->plan.fwd : p->plan.bwd;
     if (g.items.empty()) return SID_OK;
     switch (p->NP / 16) {
-        case 1: return launch_nt<1>(p, g, dir, st);
-        case 2: return launch_nt<2>(p, g, dir, st);
-        case 4: return launch_nt<4>(p, g, dir, st);
-        case 8: return launch_nt<8>(p, g, dir, st);
-        case 11: return launch_nt<11>(p, g, dir, st);
-        case 15: return launch_nt<15>(p, g, dir, st);
-        case 16: return launch_nt<16>(p, g, dir, st);
+        case 1: return launch_nt<1>(p, g, dir, st, max_blocks);
+        case 2: return launch_nt<2>(p, g, dir, st, max_blocks);
+        case 4: return launch_nt<4>(p, g, dir, st, max_blocks);
+        case 8: return launch_nt<8>(p, g, dir, st, max_blocks);
+        case 11: return launch_nt<11>(p, g, dir, st, max_blocks);
+        case 15: return launch_nt<15>(p, g, dir, st, max_blocks);
+        case 16: return launch_nt<16>(p, g, dir, st, max_blocks);
         default:
             set_error("unsupported padded lenslet size NP=%lld", (long long)p->NP);
             return SID_ERR_DIMS;

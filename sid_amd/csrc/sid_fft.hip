@@ -1258,6 +1258,7 @@ int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t s
         ne += elig[c];
     }
     p->auto_fft.assign(p->nzH, 0);
+    p->auto_refined = false;
     std::copy(key, key + 4, p->auto_key);
     if (ne == 0) return SID_OK;
     int rc = SID_OK;

@@ -171,9 +171,11 @@ struct sid_psf {
     int64_t KT = 0;                         // BF16 K per tap (S rounded up to 32)
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
+    std::vector<char> pin;            // per plane: 0 autotuned, SID_METHOD_DIRECT / _FFT pinned by the caller
     std::vector<char> auto_fft;       // autotuned per-plane FFT choice (SID_METHOD_AUTO)
     int64_t auto_key[4] = {-1, -1, -1, -1};  // X, Y, nz, precision the choice was measured for
     double auto_ms[4] = {0, 0, 0, 0};        // direct ms, fft ms (all eligible planes), n eligible, n chosen
+    bool auto_refined = false;               // the choice has been re-timed with the halves overlapped
     sid::SensorPlan plan;
     void *lib = nullptr;              // cached library plan (sid_library.hip)
     float *kern = nullptr;            // neuron kernel (device, [kx][ky][kz]); null = none
@@ -183,6 +185,12 @@ struct sid_psf {
     int64_t kwork_elems = 0;
     hipEvent_t ev[10] = {};
     double timing[12] = {0};
+    // second stream for the direct GEMM while the FFT path runs on the caller's
+    // stream (fork / join events); off when SID_NO_OVERLAP is set at creation
+    bool overlap = true;
+    int gemm_blocks = 0;              // grid cap of the GEMM beside the FFT path (0: one block per CU)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace sid {
@@ -190,7 +198,8 @@ namespace sid {
 int build_weights(sid_psf *p, int which /*0 fp32, 1 bf16 hi, 2 bf16 lo, 3 fp64*/);
 int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz);
 int ensure_precision_buffers(sid_psf *p);
-int launch_gemm(sid_psf *p, int dir, hipStream_t st);
+// max_blocks > 0 caps the grid (blocks then loop over the items)
+int launch_gemm(sid_psf *p, int dir, hipStream_t st, int max_blocks = 0);
 int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st);
 int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st);
 int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which /*0 iterate, 1 Htf*/, hipStream_t st);
