@@ -209,6 +209,9 @@ def main():
                     help="run one 1 GiB device copy first (the reference kernel for scaling PMC byte counters)")
     ap.add_argument("--method", default="auto", choices=["auto", "direct", "fft"],
                     help="projector method per plane (auto = autotuned direct-vs-FFT choice)")
+    ap.add_argument("--fft-planes", type=int, default=-1,
+                    help="pin the K widest planes to the FFT path and the rest to the direct GEMM (reproduces an "
+                         "autotuned split in profiler runs, whose serialised kernels would mislead the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the projector is measured on one "
@@ -242,6 +245,14 @@ def main():
     del H
     torch.cuda.empty_cache()
     info = psf.info()
+    if args.fft_planes >= 0:
+        widths = [hi - lo + 1 for lo, hi in info["support"]]
+        order = sorted(range(len(widths)), key=lambda c: -widths[c])
+        pins = ["direct"] * len(widths)
+        for c in order[:args.fft_planes]:
+            pins[c] = "fft"
+        psf.set_plane_methods(pins)
+        log(f"pinned {args.fft_planes} widest planes to the FFT path")
     log(f"PSF handle built in {time.perf_counter() - t:.1f} s; taps_max={info['taps_max']} "
         f"weights {info['weight_bytes'] / 1e9:.2f} GB")
     V = synth.make_volume(wl.X, wl.Y, wl.nz, wl.n_balls, wl.r_lo, wl.r_hi, wl.seed, wl.background)

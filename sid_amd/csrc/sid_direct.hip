@@ -217,8 +217,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_proj_gemm(const Item *__restric
                                                         int nrows, int Ly, int YP, int S, int64_t slab_stride)
 {
     __shared__ float Bs[F32Shape<NT>::NP * F32Shape<NT>::LDK];
-    for (int i = blockIdx.x; i < nitems; i += gridDim.x)
-        f32_item<NT, RT>(items[i], Bs, segs, A0, W0, part, nrows, Ly, YP, S, slab_stride);
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        const Item it = items[i];
+        if (it.mtile >= 0) f32_item<NT, RT>(it, Bs, segs, A0, W0, part, nrows, Ly, YP, S, slab_stride);
+    }
 }
 
 
@@ -428,8 +430,10 @@ __global__ __launch_bounds__(NTHREADS, MODE == 2 ? 1 : 2) void k_proj_gemm_sl(
 {
     using SH = SlShape<NT, MODE>;
     __shared__ u16 Bs[2][SH::NB][SH::NP * SH::LDK];
-    for (int i = blockIdx.x; i < nitems; i += gridDim.x)
-        sl_item<NT, RT, MODE>(items[i], Bs, segs, A0, A1, W0, W1, part, nrows, Ly, YP, S, KT, slab_stride);
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        const Item it = items[i];
+        if (it.mtile >= 0) sl_item<NT, RT, MODE>(it, Bs, segs, A0, A1, W0, W1, part, nrows, Ly, YP, S, KT, slab_stride);
+    }
 }
 
 // FP64 variant (reference path for error studies): v_mfma_f64_16x16x4_f64
@@ -450,6 +454,7 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
     const Item it = items[blockIdx.x];
+    if (it.mtile < 0) return;  // padding of the per-XCD queues (whole block)
     int64_t rowoff[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
@@ -817,6 +822,38 @@ static std::vector<std::pair<int, int>> split_segments(const std::vector<Seg> &s
     return out;
 }
 
+// gfx950 has 8 XCDs, each with its own L2; the dispatcher deals workgroup b to
+// XCD b % 8, and a grid-stride block of a grid that is a multiple of 8 keeps
+// taking items on its XCD.  So position p of an item list runs on XCD p % 8.
+constexpr int kXcds = 8;
+
+// Item list for the XCDs: each group (the M tiles of one run of segments,
+// which all read the same weight chunks) goes whole to one XCD, costliest
+// first onto the least-loaded XCD, so one L2 fetches a group's weights once;
+// the per-XCD queues (costliest groups first) are interleaved, the shorter
+// ones padded with empty items (mtile < 0) that the kernels skip.
+static std::vector<Item> xcd_order(const std::vector<std::vector<Item>> &groups, const std::vector<int64_t> &cost)
+{
+    std::vector<int> idx(groups.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+        return cost[a] * (int64_t)groups[a].size() > cost[b] * (int64_t)groups[b].size();
+    });
+    std::vector<std::vector<Item>> q(kXcds);
+    std::vector<int64_t> load(kXcds, 0);
+    for (int g : idx) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        q[x].insert(q[x].end(), groups[g].begin(), groups[g].end());
+        load[x] += cost[g] * (int64_t)groups[g].size();
+    }
+    size_t len = 0;
+    for (auto &v : q) len = std::max(len, v.size());
+    std::vector<Item> out(len * kXcds, Item{0, 0, -1, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t j = 0; j < q[x].size(); ++j) out[j * kXcds + x] = q[x][j];
+    return out;
+}
+
 static int upload_plan(GemmPlan &g)
 {
     if (g.d_segs) { (void)hipFree(g.d_segs); g.d_segs = nullptr; }
@@ -907,11 +944,18 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         }
     }
     {
-        const int nsplit = (int)std::max<int64_t>(1, (target_blocks + sp.ntiles - 1) / sp.ntiles);
+        // a multiple of the XCD count of equal-cost groups balances the XCDs
+        int nsplit = (int)std::max<int64_t>(1, (target_blocks + sp.ntiles - 1) / sp.ntiles);
+        nsplit = (nsplit + kXcds - 1) / kXcds * kXcds;
         auto groups = split_segments(F.segs, 0, (int)F.segs.size(), nsplit);
         F.nslabs = (int)groups.size();
-        for (int gi = 0; gi < (int)groups.size(); ++gi)
-            for (int t = 0; t < sp.ntiles; ++t) F.items.push_back(Item{groups[gi].first, groups[gi].second, t, gi});
+        std::vector<std::vector<Item>> gi_items(groups.size());
+        std::vector<int64_t> gi_cost(groups.size(), 0);
+        for (int gi = 0; gi < (int)groups.size(); ++gi) {
+            for (int t = 0; t < sp.ntiles; ++t) gi_items[gi].push_back(Item{groups[gi].first, groups[gi].second, t, gi});
+            for (int s = groups[gi].first; s < groups[gi].second; ++s) gi_cost[gi] += F.segs[s].K;
+        }
+        F.items = xcd_order(gi_items, gi_cost);
         for (auto &sg : F.segs) F.macs += (int64_t)sg.K * NP * sp.mpad;
     }
 
@@ -943,8 +987,8 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     {
         const int64_t per_split = std::max<int64_t>(1, btotal * sp.ntiles / target_blocks);
         int slab = 0;
-        struct Tmp { Item it; int64_t cost; };
-        std::vector<Tmp> tmp;
+        std::vector<std::vector<Item>> gr_items;
+        std::vector<int64_t> gr_cost;
         for (int64_t c = 0; c < p->nzH; ++c) {
             B.plane_slab0[c] = slab;
             int64_t cost = 0;
@@ -955,14 +999,15 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             for (auto &gr : groups) {
                 int64_t gc = 0;
                 for (int s = gr.first; s < gr.second; ++s) gc += B.segs[s].K;
-                for (int t = 0; t < sp.ntiles; ++t) tmp.push_back({Item{gr.first, gr.second, t, slab}, gc});
+                gr_items.emplace_back();
+                for (int t = 0; t < sp.ntiles; ++t) gr_items.back().push_back(Item{gr.first, gr.second, t, slab});
+                gr_cost.push_back(gc);
                 ++slab;
             }
         }
         B.plane_slab0[p->nzH] = slab;
         B.nslabs = slab;
-        std::stable_sort(tmp.begin(), tmp.end(), [](const Tmp &a, const Tmp &b) { return a.cost > b.cost; });
-        for (auto &t : tmp) B.items.push_back(t.it);
+        B.items = xcd_order(gr_items, gr_cost);
         for (auto &sg : B.segs) B.macs += (int64_t)sg.K * NP * sp.mpad;
     }
     SID_TRY(upload_plan(F));

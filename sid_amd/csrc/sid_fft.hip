@@ -1249,6 +1249,8 @@ static double time_ms(hipStream_t st, int reps, const std::function<int()> &fn, 
 int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st)
 {
     if (p->method != SID_METHOD_AUTO || p->precision == SID_PREC_FP64) return SID_OK;
+    if (p->pin.size() == (size_t)p->nzH && std::find(p->pin.begin(), p->pin.end(), 0) == p->pin.end())
+        return SID_OK;  // every plane pinned: nothing to choose
     const int64_t key[4] = {X, Y, nz, p->precision};
     if (std::equal(key, key + 4, p->auto_key) && p->auto_fft.size() == (size_t)p->nzH) return SID_OK;
     std::vector<char> elig(p->nzH, 0);
