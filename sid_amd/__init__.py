@@ -95,6 +95,18 @@ class PSF:
         if method != "auto":
             self.set_method(method)
 
+    @classmethod
+    def from_file(cls, path: str, precision: str = "fp32", method: str = "auto", use_caindex: bool = False):
+        """Handle for the PSF MAT-file at path (``H``, ``Nnum``, ``CAindex``;
+        -v7.3 HDF5 or older MAT formats, see sid_amd.psf_io).  The GPU path of
+        reconstruct_S.m:155 takes H only; ``use_caindex`` applies the file's
+        CAindex crops as the CPU path does (reconstruct_S.m:118-121)."""
+        from .psf_io import load_psf
+        H, _, ca = load_psf(path)
+        if use_caindex and ca is None:
+            raise ValueError(f"{path}: use_caindex asked for but the file has no CAindex")
+        return cls(H, ca if use_caindex else None, precision=precision, method=method)
+
     # -- configuration -----------------------------------------------------
     def update(self, H):
         """Replace the values of H (same shape) and rebuild the weights."""
