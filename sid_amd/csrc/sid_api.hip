@@ -471,7 +471,7 @@ static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) *rc = SID_ERR_HIP;
     {
         Timer tm(p, st);
-        for (int r = 0; r < 3 && *rc == SID_OK; ++r) {  // r = 0 warms the plan up
+        for (int r = 0; r < 5 && *rc == SID_OK; ++r) {  // r = 0 warms the plan up
             if (r == 1) (void)hipEventRecord(a, st);
             *rc = project_halves(p, 0, st, tm, 0, 5);
             if (*rc == SID_OK) *rc = project_halves(p, 1, st, tm, 2, 6);
@@ -483,7 +483,7 @@ static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
     }
     if (a) (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
-    return ms / 2;
+    return ms / 4;
 }
 
 // Split of the planes between the GEMM and the FFT path when the two halves

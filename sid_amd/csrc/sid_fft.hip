@@ -697,6 +697,29 @@ __device__ __forceinline__ void load_twiddles(float2 *tx, float2 *ty, const floa
 //   A(k) = (Z(k) + conj(Z(-k))) / 2,   B(k) = (Z(k) - conj(Z(-k))) / (2 i).
 // The next pair's volume values are loaded into registers while the current
 // pair is transformed and written out.
+// Two adjacent floats at a 4-byte aligned address as one 8-byte access
+// (multi-dword global accesses need only dword alignment).
+__device__ __forceinline__ float2 ld_f2_a4(const float *p)
+{
+    float2 v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+__device__ __forceinline__ void st_f2_a4(float *p, float2 v) { __builtin_memcpy(p, &v, 8); }
+// ... and two adjacent complex values (q0, q0 + 1) of vt as one 16-byte access
+__device__ __forceinline__ void ld_c2(const float2 *p, float2 &x, float2 &y)
+{
+    float4 v;
+    __builtin_memcpy(&v, p, 16);
+    x = make_float2(v.x, v.y);
+    y = make_float2(v.z, v.w);
+}
+__device__ __forceinline__ void st_c2(float2 *p, float2 x, float2 y)
+{
+    const float4 v = make_float4(x.x, x.y, y.x, y.y);
+    __builtin_memcpy(p, &v, 16);
+}
+
 __device__ __forceinline__ void sub_fwd_fetch(const float *__restrict__ vol, const int *__restrict__ slot_plane,
                                               const SubGeom &g, int pair, float (&a)[kSubUnroll],
                                               float (&b)[kSubUnroll])
@@ -708,8 +731,24 @@ __device__ __forceinline__ void sub_fwd_fetch(const float *__restrict__ vol, con
     const bool in0 = c0 < g.nz_in, in1 = has1 && c1 < g.nz_in;
     const float *v0p = vol + (int64_t)c0 * g.XP * g.YP * g.S + ab0;
     const float *v1p = vol + (int64_t)c1 * g.XP * g.YP * g.S + ab1;
+    // the two sub-images are neighbours in one lenslet record: one load for both
+    const bool adjacent = in0 && in1 && c1 == c0;
     const int npix = g.Nxp * g.Nyp;
     const float rnyp = 1.0f / (float)g.Nyp;
+    // adjacency is uniform per pair: separate loops keep the 8-byte loads whole
+    if (adjacent) {
+#pragma unroll
+        for (int u = 0; u < kSubUnroll; ++u) {
+            const int e = u * kSubThreads + (int)threadIdx.x;
+            const int Qx = qdiv(e, g.Nyp, rnyp), Qy = e - Qx * g.Nyp;
+            float2 v = make_float2(0.f, 0.f);
+            if (e < npix && Qx < g.Lx && Qy < g.Ly)
+                v = ld_f2_a4(v0p + (((int64_t)Qx + g.T) * g.YP + (Qy + g.T)) * g.S);
+            a[u] = v.x;
+            b[u] = v.y;
+        }
+        return;
+    }
 #pragma unroll
     for (int u = 0; u < kSubUnroll; ++u) {
         const int e = u * kSubThreads + (int)threadIdx.x;
@@ -754,16 +793,25 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__rest
         const bool has1 = q0 + 1 < g.nq;
         const float s0 = qscale ? qscale[q0] : 1.0f;
         const float s1 = (qscale && has1) ? qscale[q0 + 1] : 1.0f;
-        for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
-            const int kx = qdiv(k0, g.Hy, rhy), ky = k0 - kx * g.Hy;
-            const float2 z = r2[kx * g.Nyp + ky];
-            const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
-            const float2 zm = r2[mx * g.Nyp + my];
-            const float2 av = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-            const float2 bv = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
-            float2 *o = vt + (int64_t)k0 * g.nq + q0;
-            o[0] = make_float2(av.x * s0, av.y * s0);
-            if (has1) o[1] = make_float2(bv.x * s1, bv.y * s1);
+        if (has1) {   // uniform per pair: one 16-byte store of both spectra per k0
+            for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
+                const int kx = qdiv(k0, g.Hy, rhy), ky = k0 - kx * g.Hy;
+                const float2 z = r2[kx * g.Nyp + ky];
+                const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
+                const float2 zm = r2[mx * g.Nyp + my];
+                const float2 av = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+                const float2 bv = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+                st_c2(vt + (int64_t)k0 * g.nq + q0, make_float2(av.x * s0, av.y * s0),
+                      make_float2(bv.x * s1, bv.y * s1));
+            }
+        } else {
+            for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
+                const int kx = qdiv(k0, g.Hy, rhy), ky = k0 - kx * g.Hy;
+                const float2 z = r2[kx * g.Nyp + ky];
+                const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
+                const float2 zm = r2[mx * g.Nyp + my];
+                vt[(int64_t)k0 * g.nq + q0] = make_float2(0.5f * (z.x + zm.x) * s0, 0.5f * (z.y - zm.y) * s0);
+            }
         }
     }
 }
@@ -781,6 +829,25 @@ __device__ __forceinline__ void sub_bwd_fetch(const float2 *__restrict__ bt, con
     const int npix = g.Nxp * g.Nyp;
     const float rnyp = 1.0f / (float)g.Nyp;
     cjmask = 0u;
+    // has1 is uniform per pair: separate loops keep the 16-byte loads whole
+    if (has1) {
+#pragma unroll
+        for (int u = 0; u < kSubUnroll; ++u) {
+            const int e = u * kSubThreads + (int)threadIdx.x;
+            a[u] = b[u] = make_float2(0.f, 0.f);
+            if (e < npix) {
+                const int kx = qdiv(e, g.Nyp, rnyp), ky = e - kx * g.Nyp;
+                int sx = kx, sy = ky;
+                if (ky >= g.Hy) {   // conj of the stored value at -k
+                    sx = kx == 0 ? 0 : g.Nxp - kx;
+                    sy = g.Nyp - ky;
+                    cjmask |= 1u << u;
+                }
+                ld_c2(bt + (int64_t)(sx * g.Hy + sy) * g.nq + q0, a[u], b[u]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int u = 0; u < kSubUnroll; ++u) {
         const int e = u * kSubThreads + (int)threadIdx.x;
@@ -788,14 +855,12 @@ __device__ __forceinline__ void sub_bwd_fetch(const float2 *__restrict__ bt, con
         if (e < npix) {
             const int kx = qdiv(e, g.Nyp, rnyp), ky = e - kx * g.Nyp;
             int sx = kx, sy = ky;
-            if (ky >= g.Hy) {   // conj of the stored value at -k
+            if (ky >= g.Hy) {
                 sx = kx == 0 ? 0 : g.Nxp - kx;
                 sy = g.Nyp - ky;
                 cjmask |= 1u << u;
             }
-            const float2 *src = bt + (int64_t)(sx * g.Hy + sy) * g.nq + q0;
-            a[u] = src[0];
-            if (has1) b[u] = src[1];
+            a[u] = bt[(int64_t)(sx * g.Hy + sy) * g.nq + q0];
         }
     }
 }
@@ -840,12 +905,21 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_bwd(const float2 *__res
         float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dy, ty, 1.0f, g.Nxp, g.Nyp, 1);  // along y
         const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
         const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
-        for (int e = threadIdx.x; e < npx; e += blockDim.x) {
-            const int Qx = qdiv(e, g.Ly, rly), Qy = e - Qx * g.Ly;
-            const float2 z = r2[Qx * g.Nyp + Qy];
-            const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
-            dfft[((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0] = z.x * scale;
-            if (has1) dfft[((int64_t)c1 * g.XP * g.YP + cell) * g.S + ab1] = z.y * scale;
+        if (has1 && c1 == c0) {   // neighbours in one lenslet record (uniform per pair): one 8-byte store
+            for (int e = threadIdx.x; e < npx; e += blockDim.x) {
+                const int Qx = qdiv(e, g.Ly, rly), Qy = e - Qx * g.Ly;
+                const float2 z = r2[Qx * g.Nyp + Qy];
+                const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
+                st_f2_a4(dfft + ((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0, make_float2(z.x * scale, z.y * scale));
+            }
+        } else {
+            for (int e = threadIdx.x; e < npx; e += blockDim.x) {
+                const int Qx = qdiv(e, g.Ly, rly), Qy = e - Qx * g.Ly;
+                const float2 z = r2[Qx * g.Nyp + Qy];
+                const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
+                dfft[((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0] = z.x * scale;
+                if (has1) dfft[((int64_t)c1 * g.XP * g.YP + cell) * g.S + ab1] = z.y * scale;
+            }
         }
     }
 }
