@@ -368,18 +368,38 @@ __device__ __forceinline__ void sl_item(
             __syncthreads();
             for (int ky = 0; ky < ntap; ++ky) {
                 const int buf = ky & 1;
+                {
+                    // B fragments through a ring of kBR register slots (NB fragments each):
+                    // fragment ct + kBR is read right after the MFMAs of fragment ct are
+                    // issued, so each LDS read has kBR - 1 fragments' MFMAs to land under
+                    // (the scheduling groups keep that order).
+                    constexpr int kBR = 3;
+                    constexpr int kMf = (MODE == 2 ? 3 : 2) * RT;   // MFMAs per fragment
+                    const u16 *bb = &Bs[buf][0][l16 * LDK + 8 * g];
+                    bf8 ring[kBR][NB];
 #pragma unroll
-                for (int ct = 0; ct < NT; ++ct) {
-                    const bf8 bh = *(const bf8 *)(&Bs[buf][0][(ct * 16 + l16) * LDK + 8 * g]);
-                    bf8 bl;
-                    if constexpr (MODE == 2) bl = *(const bf8 *)(&Bs[buf][1][(ct * 16 + l16) * LDK + 8 * g]);
+                    for (int i = 0; i < kBR; ++i)
 #pragma unroll
-                    for (int rt = 0; rt < RT; ++rt) {
-                        f4 c = acc[rt][ct];
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
-                        if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
-                        acc[rt][ct] = c;
+                        for (int h = 0; h < NB; ++h) ring[i][h] = *(const bf8 *)(bb + h * NP * LDK + i * 16 * LDK);
+                    __builtin_amdgcn_sched_group_barrier(0x100, kBR * NB, 0);   // the first reads
+#pragma unroll
+                    for (int ct = 0; ct < NT; ++ct) {
+                        const bf8 bh = ring[ct % kBR][0], bl = ring[ct % kBR][NB - 1];
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt) {
+                            f4 c = acc[rt][ct];
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
+                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
+                            if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
+                            acc[rt][ct] = c;
+                        }
+                        __builtin_amdgcn_sched_group_barrier(0x008, kMf, 0);   // this fragment's MFMAs
+                        if (ct + kBR < NT) {
+#pragma unroll
+                            for (int h = 0; h < NB; ++h)
+                                ring[ct % kBR][h] = *(const bf8 *)(bb + h * NP * LDK + (ct + kBR) * 16 * LDK);
+                            __builtin_amdgcn_sched_group_barrier(0x100, NB, 0);   // then the next reads
+                        }
                     }
                 }
                 if (ky + 1 < ntap) {
