@@ -443,7 +443,7 @@ __device__ __forceinline__ void sl_item(
 // block with the full grid, or a capped grid that leaves CUs to a kernel
 // running beside it on another stream.
 template <int NT, int RT, int MODE>
-__global__ __launch_bounds__(NTHREADS, MODE == 2 ? 1 : 2) void k_proj_gemm_sl(
+__global__ __launch_bounds__(NTHREADS, (MODE == 2 || RT > 2) ? 1 : 2) void k_proj_gemm_sl(
     const Item *__restrict__ items, int nitems, const Seg *__restrict__ segs, const u16 *__restrict__ A0,
     const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
     int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
@@ -812,7 +812,10 @@ int build_weights(sid_psf *p, int which)
     return SID_OK;
 }
 
-static int row_tiles_for(int prec) { return prec == SID_PREC_FP32 ? 2 : 2; }
+// 16-row MFMA tiles per wave (the block's M tile is 4 * 16 * RT rows).  RT = 4
+// for BF16X2 (occupancy 1, 120 MFMAs per barrier) measured 23 % slower alone
+// and 16 % slower beside the FFT path than RT = 2.
+static int row_tiles_for(int /*precision*/) { return 2; }
 
 // Split a list of segments (all costs = K) into `nsplit` contiguous groups of
 // near-equal total K.
