@@ -146,6 +146,43 @@ __device__ __forceinline__ bf8 shift_rows(bf8 v)
 // reads input tile row w = 16 wave + r + ky - 1 - j, which at step j + 1 is the
 // row lane r - 1 held at step j.  So after the first row of a slice only lane 0
 // of each 16-lane row reads LDS; the others take their neighbour's registers.
+constexpr int kCmPre = 32;    // input values per thread a prefetched slice may hold (tile <= 8192)
+
+// Global loads of one staged input tile (Hh x W values at (X0, Y0) of the slice
+// src, zero outside the volume) into registers, kCmPre per thread.
+__device__ __forceinline__ void convn_tile_load(float (&v)[kCmPre], const float *__restrict__ src, int n, int W,
+                                                int X0, int Y0, int X, int Y)
+{
+    // an opaque copy of the thread index keeps the per-value index math here
+    // instead of hoisted out of the slice loop (where it would hold ~100 registers)
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int q = 0; q < kCmPre; ++q) {
+        const int e = q * 256 + tid;
+        const int w = e / W, u = e - w * W;
+        const int xs = X0 + u, ys = Y0 + w;
+        v[q] = (e < n && xs >= 0 && xs < X && ys >= 0 && ys < Y) ? src[(int64_t)ys * X + xs] : 0.0f;
+    }
+}
+
+// ... and from registers into the BF16 hi / lo tiles (pitch PW).
+__device__ __forceinline__ void convn_tile_store(const float (&v)[kCmPre], u16 *thi, u16 *tlo, int n, int W, int PW)
+{
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+#pragma unroll
+    for (int q = 0; q < kCmPre; ++q) {
+        const int e = q * 256 + tid;
+        if (e < n) {
+            const int w = e / W, u = e - w * W;
+            const u16 h = f2bf(v[q]);
+            thi[w * PW + u] = h;
+            tlo[w * PW + u] = f2bf(v[q] - bf2f(h));
+        }
+    }
+}
+
 template <int MODE, int KS>
 __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In, int X, int Y, int Z,
                                                     const u16 *__restrict__ frag, int kx, int ky, int kz, int JC,
@@ -154,10 +191,11 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int W = 48 + 32 * KS;   // input tile width: 3 x-block offsets + the K window
     constexpr int PW = W + 8;         // pitch in BF16 elements (multiple of 8: 16-byte rows)
+    constexpr int NBP = MODE == 2 ? 2 : 1;   // B parts staged: hi (and lo for BF16X3)
     const int Hh = kCmTile + ky - 1;  // input tile height
     u16 *thi = reinterpret_cast<u16 *>(smem);
     u16 *tlo = thi + Hh * PW;
-    bf8 *bs = reinterpret_cast<bf8 *>(tlo + Hh * PW);   // [jj][ks][part][64 lanes]
+    bf8 *bs = reinterpret_cast<bf8 *>(tlo + Hh * PW);   // [jj][ks][part < NBP][64 lanes]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
     const int xb0 = blockIdx.x * kCmTile, yb0 = blockIdx.y * kCmTile, z = blockIdx.z;
     const int ox = kx / 2, oy = ky / 2, oz = kz / 2;
@@ -166,13 +204,20 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
 #pragma unroll
     for (int xb = 0; xb < 4; ++xb) acc[xb] = f4{0.f, 0.f, 0.f, 0.f};
     bf8 ah[4][KS], al[4][KS];
-    for (int l = 0; l < kz; ++l) {
+    // slices l with z + oz - l inside the volume: one contiguous range
+    const int lb = max(0, z + oz - (Z - 1)), le = min(kz - 1, z + oz);
+    // tiles of at most kCmPre values per thread are prefetched into registers one
+    // slice ahead (their loads land under the previous slice's MFMAs)
+    const bool prefetch = Hh * W <= 256 * kCmPre;
+    float nxt[kCmPre];
+    if (prefetch && lb <= le) convn_tile_load(nxt, In + (int64_t)(z + oz - lb) * X * Y, Hh * W, W, X0, Y0, X, Y);
+    for (int l = lb; l <= le; ++l) {
         const int zi = z + oz - l;
-        if (zi < 0 || zi >= Z) continue;   // block-uniform
         __syncthreads();                    // tiles of the previous slice consumed
         const float *src = In + (int64_t)zi * X * Y;
+        if (prefetch) convn_tile_store(nxt, thi, tlo, Hh * W, W, PW);
         // staging: kStageU independent loads in flight per thread, then convert and store
-        for (int base = 0; base < Hh * W; base += kStageU * 256) {
+        for (int base = 0; !prefetch && base < Hh * W; base += kStageU * 256) {
             float v[kStageU];
 #pragma unroll
             for (int q = 0; q < kStageU; ++q) {
@@ -195,14 +240,15 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
         for (int jc0 = 0; jc0 < ky; jc0 += JC) {
             const int jn = min(JC, ky - jc0);
             __syncthreads();   // tile staged (first chunk) / previous B chunk consumed
+            // fragments are stored [l][j][ks][hi, lo][64 lanes]; BF16X2 stages the hi parts only
             const bf8 *fsrc = reinterpret_cast<const bf8 *>(frag) + ((int64_t)l * ky + jc0) * KS * 2 * 64;
-            const int nb = jn * KS * 2 * 64;
+            const int nb = jn * KS * NBP * 64;
             for (int base = 0; base < nb; base += kStageU * 256) {
                 bf8 t[kStageU];
 #pragma unroll
                 for (int q = 0; q < kStageU; ++q) {
                     const int e = base + q * 256 + (int)threadIdx.x;
-                    if (e < nb) t[q] = fsrc[e];
+                    if (e < nb) t[q] = NBP == 2 ? fsrc[e] : fsrc[(e >> 6) * 128 + (e & 63)];
                 }
 #pragma unroll
                 for (int q = 0; q < kStageU; ++q) {
@@ -210,6 +256,9 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
                     if (e < nb) bs[e] = t[q];
                 }
             }
+            // the next slice's input, behind this slice's fragment loads (waits are in issue order)
+            if (prefetch && jc0 == 0 && l < le)
+                convn_tile_load(nxt, In + (int64_t)(zi - 1) * X * Y, Hh * W, W, X0, Y0, X, Y);
             __syncthreads();
             if (jc0 == 0) {   // all lanes read their row for j = 0
                 const int w = wave * 16 + l16 + ky - 1;
@@ -226,9 +275,9 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
                 const int j = jc0 + jj;
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
-                    const bf8 bh = bs[((jj * KS + ks) * 2 + 0) * 64 + lane];
+                    const bf8 bh = bs[((jj * KS + ks) * NBP + 0) * 64 + lane];
                     bf8 bl;
-                    if constexpr (MODE == 2) bl = bs[((jj * KS + ks) * 2 + 1) * 64 + lane];
+                    if constexpr (MODE == 2) bl = bs[((jj * KS + ks) * NBP + 1) * 64 + lane];
 #pragma unroll
                     for (int xb = 0; xb < 4; ++xb) {
                         f4 c = acc[xb];
@@ -239,7 +288,8 @@ __global__ __launch_bounds__(256) void k_convn_mfma(const float *__restrict__ In
                         acc[xb] = c;
                     }
                 }
-                if (j + 1 < ky) {   // slide to row j + 1
+                // slide to row j + 1 (preparing it before row j's MFMAs measured 25 % slower)
+                if (j + 1 < ky) {
                     const int w0 = wave * 16 + ky - 2 - j;   // lane 0's new row
 #pragma unroll
                     for (int xb = 0; xb < 4; ++xb)
@@ -309,7 +359,7 @@ int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const i
     }
     const int KS = convn_ks(kd[0]);
     const size_t tile = 4 * (size_t)(kCmTile + kd[1] - 1) * (48 + 32 * KS + 8);
-    const size_t per_row = (size_t)KS * 2 * 64 * 16;
+    const size_t per_row = (size_t)KS * (precision == SID_PREC_BF16X3 ? 2 : 1) * 64 * 16;   // staged B parts
     const size_t budget = 150 * 1024;
     const int JC = (int)std::max<size_t>(1, std::min<size_t>(kd[1], (budget - tile) / per_row));
     const size_t lds = tile + JC * per_row;
