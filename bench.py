@@ -171,6 +171,8 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_ms": round(tc * 1e3, 3),
                             "convn_tflops": round(fl / tc / 1e12, 2), "convn_fp32_ms": round(tc32 * 1e3, 3),
                             "convn_fp32_tflops": round(fl / tc32 / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
+                            "convn_frac_of_peak": round(fl / tc / PEAK[cprec], 4),
+                            "convn_fp32_frac_of_vector_peak": round(fl / tc32 / 78.6e12, 4),
                             "rl_iter_per_s_wrapped": round(iters / dtw, 3)}
     log(f"neuron kernel {k.shape}: convn {cprec} {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s algorithmic), "
         f"FP32 {tc32 * 1e3:.2f} ms; wrapped RL {iters / dtw:.2f} iter/s")
@@ -185,7 +187,8 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     tm = time.perf_counter() - t0
     passes = 5   # max, 2 column passes, 2 row passes
     out["std_image"] = {"pixels": P, "frames": T, "bg_iter": 2, "ms": round(tm * 1e3, 2),
-                        "stream_gbps": round(passes * P * T * 4 / tm / 1e9, 1), "passes": passes}
+                        "stream_gbps": round(passes * P * T * 4 / tm / 1e9, 1), "passes": passes,
+                        "frac_of_hbm": round(passes * P * T * 4 / tm / HBM_PEAK, 4)}
     log(f"std image {P} px x {T} frames: {tm * 1e3:.1f} ms ({passes * P * T * 4 / tm / 1e9:.0f} GB/s over {passes} passes)")
     del Y
     torch.cuda.empty_cache()
