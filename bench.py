@@ -212,6 +212,8 @@ def main():
                     help="run one 1 GiB device copy first (the reference kernel for scaling PMC byte counters)")
     ap.add_argument("--method", default="auto", choices=["auto", "direct", "fft"],
                     help="projector method per plane (auto = autotuned direct-vs-FFT choice)")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the direct-only leg (every plane on the MFMA GEMM, one step per precision)")
     ap.add_argument("--fft-planes", type=int, default=-1,
                     help="pin the K widest planes to the FFT path and the rest to the direct GEMM (reproduces an "
                          "autotuned split in profiler runs, whose serialised kernels would mislead the autotuner)")
@@ -355,6 +357,22 @@ def main():
         out_rows = rows_bench(psf, wl, htfs[head_prec], head_prec, args.iters, dev)
     head = args.precisions.split(",")[-1]
     base = args.precisions.split(",")[0]
+    direct = None
+    if not args.no_direct and args.method == "auto":
+        # every plane on the implicit GEMM: the MFMA path alone, per precision (one timed step each)
+        direct = {}
+        psf.set_plane_methods(None)
+        psf.set_method("direct")
+        for prec in args.precisions.split(","):
+            psf.set_precision(prec)
+            psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
+            torch.cuda.synchronize()
+            direct[prec] = round(args.iters / (time.perf_counter() - t0), 3)
+        psf.set_method("auto")
+        log("direct-only (every plane on the MFMA GEMM): " + ", ".join(f"{k} {v:.2f} RL iter/s" for k, v in direct.items()))
     hr = results[head]
     peak = PEAK[head]
     # roofline of the dominant kernel family of the headline method
@@ -375,7 +393,8 @@ def main():
                 and same_plan):
             tj = None
     if t_gemm >= t_fftp:
-        gk = f"sid::k_proj_gemm<15, 2, {({'fp32': 0, 'bf16x2': 1, 'bf16x3': 2}).get(head, 0)}>"
+        gk = {"fp32": "sid::k_proj_gemm<15, 2>", "bf16x2": "sid::k_proj_gemm_sl<15, 2, 1>",
+              "bf16x3": "sid::k_proj_gemm_sl<15, 2, 2>"}.get(head, "")
         kk = tj.get("kernels", {}) if tj else {}
         traffic = (kk[gk]["read_bytes"] + kk[gk]["write_bytes"]) if gk in kk else None
         roof = {"bound": "mfma", "achieved": round(hr["gemm_achieved_tflops"], 3), "peak": peak / 1e12,
@@ -417,6 +436,9 @@ def main():
         "methods": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
         "speedup_vs_fp32": round(hr["iter_per_s"] / results[base]["iter_per_s"], 3) if base in results else None,
+        "direct_only_iter_per_s": direct,
+        "speedup_vs_fp32_direct_only": (round(direct[head] / direct[base], 3)
+                                        if direct and head in direct and base in direct and head != base else None),
         "rel_l2_vs": ref_name,
         "rel_l2": rel,
     }
