@@ -76,6 +76,54 @@ def cpu_baseline(wl, V, y_pre, sample_planes=None):
                        f"bwd {t2 - t1:.2f} s), scaled by sum(M_c^2) all/sample = {msum_all}/{msum_s}")}
 
 
+def grad_bench(dev, reps=3):
+    """Differentiation row: Poisson NLL of a batch of 16 frames (495 x 495 px,
+    31 planes z=-15..15, 225 px kernels, 40 balls each) and its gradients with
+    respect to the 16 volumes and the PSF, FP32 direct path.  Algorithmic work
+    per frame: forward, backward and PSF-weight-gradient GEMMs of
+    2 * X * Y * sum_c M_c^2 flop each."""
+    import torch
+    import sid_amd
+    from sid_amd import grad, synth
+    wl = synth.WORKLOADS["grad"]
+    H, _ = synth.make_psf(wl.n, wl.M, wl.zs, xp=torch, device=dev)
+    psf = sid_amd.PSF(H, method="direct")
+    del H
+    psf.set_precision("fp32")
+    msum = sum((hi - lo + 1) ** 2 for lo, hi in psf.info()["support"] if hi >= lo)
+    nb = wl.batch
+    vols, imgs = [], []
+    for f in range(nb):
+        V = synth.make_volume(wl.X, wl.Y, wl.nz, wl.n_balls, wl.r_lo, wl.r_hi, wl.seed + f, wl.background)
+        vols.append(torch.from_numpy(np.ascontiguousarray(V.transpose(2, 1, 0))).to(dev))
+        img = synth.poisson_image(psf.forward(vols[-1]).cpu().numpy().T, wl.peak_photons, wl.seed + f)
+        imgs.append(torch.from_numpy(np.ascontiguousarray(img.T.astype(np.float32))).to(dev))
+    V, y = torch.stack(vols), torch.stack(imgs)
+    del vols, imgs
+    times = {}
+    for name, gv, gh in (("nll_gV", True, False), ("nll_gV_gH", True, True)):
+        grad.poisson_nll_grad(psf, V, y, gv, gh)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            nll, _, _ = grad.poisson_nll_grad(psf, V, y, gv, gh)
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / reps
+    fl_gemm = 2.0 * wl.X * wl.Y * msum
+    t, tv = times["nll_gV_gH"], times["nll_gV"]
+    res = {"frames": nb, "X": wl.X, "Y": wl.Y, "Nz": wl.nz, "M": wl.M, "precision": "fp32",
+           "ms_nll_gV_gH": round(t * 1e3, 2), "ms_nll_gV": round(tv * 1e3, 2),
+           "batches_per_s": round(1.0 / t, 3), "frames_per_s": round(nb / t, 2),
+           "tflops": round(3 * nb * fl_gemm / t / 1e12, 2),
+           "frac_of_fp32_peak": round(3 * nb * fl_gemm / t / PEAK["fp32"], 4),
+           "psf_grad_tflops": round(nb * fl_gemm / max(t - tv, 1e-9) / 1e12, 2), "nll": nll}
+    log(f"gradients ({nb} frames {wl.X}x{wl.Y}x{wl.nz}, M={wl.M}): NLL+dV+dH {t * 1e3:.1f} ms "
+        f"({res['tflops']:.1f} TFLOP/s), NLL+dV {tv * 1e3:.1f} ms")
+    del psf, V, y
+    torch.cuda.empty_cache()
+    return res
+
+
 def library_bench(psf, wl_lib=None, steps=3):
     """Batched library workload: 2500 seeds (lateral box 7-21 px, depth 3-9
     planes, Gaussian blob sigma 2 px) in the 1005 x 1005 x 51 geometry, one
@@ -212,6 +260,8 @@ def main():
                     help="run one 1 GiB device copy first (the reference kernel for scaling PMC byte counters)")
     ap.add_argument("--method", default="auto", choices=["auto", "direct", "fft"],
                     help="projector method per plane (auto = autotuned direct-vs-FFT choice)")
+    ap.add_argument("--grad-only", action="store_true",
+                    help="run only the differentiation row (NLL + volume and PSF gradients, 16 frames 495^2 x 31)")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the direct-only leg (every plane on the MFMA GEMM, one step per precision)")
     ap.add_argument("--fft-planes", type=int, default=-1,
@@ -233,6 +283,9 @@ def main():
         sys.exit(1)
     wl = synth.WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
+    if args.grad_only:
+        print(json.dumps({"gradient": grad_bench(dev)}))
+        return
     if args.profile_copy:
         a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -355,6 +408,7 @@ def main():
     if not args.no_rows:
         head_prec = args.precisions.split(",")[-1]
         out_rows = rows_bench(psf, wl, htfs[head_prec], head_prec, args.iters, dev)
+        out_rows["gradient"] = grad_bench(dev)
     head = args.precisions.split(",")[-1]
     base = args.precisions.split(",")[0]
     direct = None

@@ -944,7 +944,14 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
     }
-    const int target_blocks = 4 * dev_cus;
+    // Work items per CU the segment split aims for.  Alone on the device the
+    // GEMM gets 8 (four waves of blocks at two blocks per CU: a short tail);
+    // beside the FFT path 4 (8 measured 11 % slower for the overlapped hybrid
+    // plan at 1005^2 x 51 planes).  SID_SPLIT_PER_CU overrides both.
+    const bool any_fft = std::find(use.begin(), use.end(), 1) != use.end();
+    int per_cu = any_fft ? 4 : 8;
+    if (const char *e = std::getenv("SID_SPLIT_PER_CU")) per_cu = std::max(1, std::atoi(e));
+    const int target_blocks = per_cu * dev_cus;
 
     // ---- forward: segments over planes 0..nz-1 and their dx rows
     GemmPlan &F = sp.fwd;

@@ -62,76 +62,86 @@ __global__ void k_nll_residual(float *__restrict__ img, const float *__restrict_
 // 256 threads = 4 waves, tile 128 K-rows x NP columns, the sum over (f, P) in
 // chunks of 32 lenslets staged in LDS.  v_mfma_f32_16x16x4_f32 with the
 // lenslet index as the MFMA K dimension: lane (l16, g) feeds A^T[row l16][P =
-// 8g + s] and R[P = 8g + s][col l16] at step s.
+// 4s + g] and R[P = 4s + g][col l16] at step s; with LDS row strides of 16
+// (mod 64) floats the four lane groups of a read hit disjoint banks.  The next
+// chunk's tiles are loaded into registers while the current one is multiplied
+// (one LDS buffer, two barriers per chunk).
 struct DwItem {
     int32_t seg, ktile;
 };
 
 template <int NT>
-__global__ __launch_bounds__(256) void k_dw_gemm(const Seg *__restrict__ segs, const DwItem *__restrict__ items,
-                                                 const float *__restrict__ vols, int64_t vol_stride,
-                                                 const float *__restrict__ res, int64_t img_stride, int nb, int nrows,
-                                                 int Ly, int64_t YP, int T, int S, float *__restrict__ dW,
-                                                 const int64_t *__restrict__ dw_off)
+__global__ __launch_bounds__(256, 2) void k_dw_gemm(const Seg *__restrict__ segs, const DwItem *__restrict__ items,
+                                                    const float *__restrict__ vols, int64_t vol_stride,
+                                                    const float *__restrict__ res, int64_t img_stride, int nb,
+                                                    int nrows, int Ly, int64_t YP, int T, int S,
+                                                    float *__restrict__ dW, const int64_t *__restrict__ dw_off)
 {
     constexpr int NP = NT * 16;
     constexpr int MT = 128, PC = 32;
-    constexpr int LDA = MT + 4, LDR = NP + 4;
+    constexpr int LDA = MT + 16, LDR = NP + (80 - NP % 64) % 64;
+    constexpr int A4 = MT / 4, R4 = NP / 4;               // float4 per lenslet row of each tile
+    constexpr int NA = A4 / 8, NR = (R4 + 7) / 8;         // 8 threads per lenslet row
+    static_assert(PC * 8 == 256 && A4 % 8 == 0, "one lenslet row per 8 threads");
+    static_assert(LDA % 64 == 16 && LDR % 64 == 16, "row strides of 16 (mod 64) floats");
     __shared__ float As[PC * LDA];
     __shared__ float Rs[PC * LDR];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, g = lane >> 4;
+    const int pl = tid >> 3, q = tid & 7;                 // this thread's lenslet row in a chunk and column phase
     const DwItem it = items[blockIdx.x];
     const Seg sg = segs[it.seg];
     const int k0 = it.ktile * MT;
+    const int cpf = (nrows + PC - 1) / PC;
+    const int nch = nb * cpf;
     f4 acc[2][NT];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
 
-    for (int f = 0; f < nb; ++f) {
-        const float *V = vols + (int64_t)f * vol_stride + sg.a_off + k0;
-        const float *R = res + (int64_t)f * img_stride;
-        for (int p0 = 0; p0 < nrows; p0 += PC) {
-            __syncthreads();
-            {   // A^T tile: 32 lenslets x 128 K-rows, 8 threads per lenslet, 16 floats each
-                const int pl = threadIdx.x >> 3, q = threadIdx.x & 7;
-                const int r = p0 + pl;
-                float4 v[4] = {};
-                if (r < nrows) {
-                    const int Px = r / Ly, Py = r - Px * Ly;
-                    const float *src = V + ((int64_t)Px * YP + Py) * S + 16 * q;
+    float4 ra[NA], rr[NR];
+    auto fetch = [&](int ci) {
+        const int f = ci / cpf, r = (ci - f * cpf) * PC + pl;
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) v[j] = *(const float4 *)(src + 4 * j);
-                }
+        for (int j = 0; j < NA; ++j) ra[j] = float4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) *(float4 *)(As + pl * LDA + 16 * q + 4 * j) = v[j];
+        for (int j = 0; j < NR; ++j) rr[j] = float4{0.f, 0.f, 0.f, 0.f};
+        if (r < nrows) {
+            const int Px = r / Ly, Py = r - Px * Ly;
+            const float *V = vols + (int64_t)f * vol_stride + sg.a_off + k0 + ((int64_t)Px * YP + Py) * S;
+            const float *R = res + (int64_t)f * img_stride + ((int64_t)(Px + T) * YP + (Py + T)) * S;
+#pragma unroll
+            for (int j = 0; j < NA; ++j) ra[j] = *(const float4 *)(V + 4 * (q + 8 * j));
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                const int c4 = q + 8 * j;
+                if (c4 < R4 && 4 * c4 < S) rr[j] = *(const float4 *)(R + 4 * c4);
             }
-            for (int e = threadIdx.x; e < PC * (NP / 4); e += 256) {   // R tile: 32 lenslets x NP
-                const int pl = e / (NP / 4), c4 = e - pl * (NP / 4);
-                const int r = p0 + pl;
-                float4 v = {0.f, 0.f, 0.f, 0.f};
-                if (r < nrows && 4 * c4 < S) {
-                    const int Px = r / Ly, Py = r - Px * Ly;
-                    v = *(const float4 *)(R + ((int64_t)(Px + T) * YP + (Py + T)) * S + 4 * c4);
-                }
-                *(float4 *)(Rs + pl * LDR + 4 * c4) = v;
-            }
-            __syncthreads();
+        }
+    };
+    fetch(0);
+    for (int ci = 0; ci < nch; ++ci) {
+        __syncthreads();
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int P = 8 * g + s;
-                float a[2];
+        for (int j = 0; j < NA; ++j) *(float4 *)(As + pl * LDA + 4 * (q + 8 * j)) = ra[j];
 #pragma unroll
-                for (int rt = 0; rt < 2; ++rt) a[rt] = As[P * LDA + wave * 32 + rt * 16 + l16];
+        for (int j = 0; j < NR; ++j)
+            if (q + 8 * j < R4) *(float4 *)(Rs + pl * LDR + 4 * (q + 8 * j)) = rr[j];
+        __syncthreads();
+        if (ci + 1 < nch) fetch(ci + 1);
 #pragma unroll
-                for (int ct = 0; ct < NT; ++ct) {
-                    const float b = Rs[P * LDR + ct * 16 + l16];
+        for (int s = 0; s < PC / 4; ++s) {
+            const int P = 4 * s + g;
+            float a[2];
 #pragma unroll
-                    for (int rt = 0; rt < 2; ++rt)
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b, acc[rt][ct], 0, 0, 0);
-                }
+            for (int rt = 0; rt < 2; ++rt) a[rt] = As[P * LDA + wave * 32 + rt * 16 + l16];
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                const float b = Rs[P * LDR + ct * 16 + l16];
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b, acc[rt][ct], 0, 0, 0);
             }
         }
     }

@@ -63,6 +63,25 @@ def test_nll_and_gradients_vs_oracle(env):
     assert e_v < 1e-5 and e_h < 1e-5
 
 
+@pytest.mark.parametrize("n,X,Y,M,nz,nb", [(3, 30, 27, 7, 2, 3), (15, 45, 30, 31, 2, 2)])
+def test_psf_gradient_tiles_vs_oracle(env, n, X, Y, M, nz, nb):
+    """The PSF-gradient GEMM at other tile widths: Nnum 3 (one 16-column
+    tile, 90 lenslets = three chunks of 32, the last partial, over 3 frames)
+    and Nnum 15 (the 15-tile kernel of full-size PSFs)."""
+    torch, sid_amd, grad = env
+    print(f"case: gH tiles n{n} X{X} Y{Y} M{M} Nz{nz} nb{nb}", flush=True)
+    H, V, y = _case(seed=7, n=n, X=X, Y=Y, M=M, nz=nz, nb=nb)
+    Ht, Vt, yt = _to_dev(torch, H, V, y)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    nll, gV, gH = grad.poisson_nll_grad(psf, Vt, yt)
+    nll_o, gV_o, gH_o = _oracle_grads(H, V, y)
+    assert abs(nll - nll_o) < 1e-5 * abs(nll_o)
+    e_v = rel_l2(gV.cpu().numpy().transpose(0, 3, 2, 1), gV_o)
+    e_h = rel_l2(gH.cpu().numpy().transpose(4, 3, 2, 1, 0), gH_o)
+    print("rel-L2 gV", e_v, "gH", e_h, flush=True)
+    assert e_v < 1e-5 and e_h < 1e-5
+
+
 def test_gradients_vs_central_finite_differences(env):
     """eps = 1e-3 central differences of the FP64 oracle NLL on random
     coordinates of V and H against the device gradients (tolerance 1e-4)."""
