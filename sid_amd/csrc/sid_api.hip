@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "sid_internal.h"
 
@@ -465,25 +466,36 @@ static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph
 // the values.
 static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
 {
-    hipEvent_t a = nullptr, b = nullptr;
-    float ms = 0.f;
+    // one warm-up pair, then the median of kRounds timed pairs (robust to a
+    // slow or fast outlier on a shared device)
+    constexpr int kRounds = 6;
+    hipEvent_t ev[kRounds + 1] = {};
     *rc = SID_OK;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) *rc = SID_ERR_HIP;
+    for (auto &e : ev)
+        if (*rc == SID_OK && hipEventCreate(&e) != hipSuccess) *rc = SID_ERR_HIP;
+    std::vector<float> ms;
     {
         Timer tm(p, st);
-        for (int r = 0; r < 5 && *rc == SID_OK; ++r) {  // r = 0 warms the plan up
-            if (r == 1) (void)hipEventRecord(a, st);
-            *rc = project_halves(p, 0, st, tm, 0, 5);
+        for (int r = 0; r <= kRounds && *rc == SID_OK; ++r) {
+            if (r > 0 && hipEventRecord(ev[r - 1], st) != hipSuccess) *rc = SID_ERR_HIP;
+            if (*rc == SID_OK) *rc = project_halves(p, 0, st, tm, 0, 5);
             if (*rc == SID_OK) *rc = project_halves(p, 1, st, tm, 2, 6);
         }
-        if (*rc == SID_OK && (hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-                              hipEventElapsedTime(&ms, a, b) != hipSuccess))
+        if (*rc == SID_OK && (hipEventRecord(ev[kRounds], st) != hipSuccess ||
+                              hipEventSynchronize(ev[kRounds]) != hipSuccess))
             *rc = SID_ERR_HIP;
+        for (int r = 0; r < kRounds && *rc == SID_OK; ++r) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[r], ev[r + 1]) != hipSuccess) *rc = SID_ERR_HIP;
+            ms.push_back(t);
+        }
         if (*rc != SID_OK) (void)hipStreamSynchronize(st);
     }
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
-    return ms / 4;
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (*rc != SID_OK || ms.empty()) return 0.0;
+    std::sort(ms.begin(), ms.end());
+    return 0.5 * (ms[(kRounds - 1) / 2] + ms[kRounds / 2]);
 }
 
 // Split of the planes between the GEMM and the FFT path when the two halves
@@ -492,10 +504,11 @@ static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
 // costs about the longer half plus their interference, which favours giving
 // the GEMM more planes.  Starting from that choice (k FFT planes), time the
 // overlapped forward + backward pair with the narrowest FFT planes moved to the
-// GEMM s at a time (s about a ninth of k) while that helps, or with s more
-// FFT planes when the first move does not, and keep the fastest.  Candidate
-// plans take the FFT planes widest first.  Skipped when planes are pinned or
-// the overlap is off; cached with the per-plane choice.
+// GEMM s at a time (s about a ninth of k) while that gains 1 % over the
+// incumbent timed before and after it, else with s more FFT planes at a time
+// on the same terms, and keep the fastest.
+// Candidate plans take the FFT planes widest first.  Skipped when planes are
+// pinned or the overlap is off; cached with the per-plane choice.
 static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st)
 {
     SID_TRY(autotune_methods(p, X, Y, nz, st));
@@ -516,24 +529,31 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
         for (int i = 0; i < k; ++i) p->auto_fft[order[i]] = 1;
     };
     int rc = SID_OK;
+    const bool log = std::getenv("SID_AUTOTUNE_LOG") != nullptr;
     auto eval = [&](int k) {
         use_prefix(k);
         rc = ensure_plan(p, X, Y, nz);
-        return rc == SID_OK ? time_projection_pair(p, st, &rc) : 0.0;
+        const double t = rc == SID_OK ? time_projection_pair(p, st, &rc) : 0.0;
+        if (log) std::fprintf(stderr, "sid autotune: %d FFT planes (start %d, step %d): %.3f ms per pair\n", k, k0, s, t);
+        return t;
     };
     int best_k = k0;
     double best_t = eval(k0);
-    bool moved = false;
-    for (int k = k0 - s; rc == SID_OK && k >= 0; k -= s) {
-        const double t = eval(k);
-        if (rc != SID_OK || t >= 0.995 * best_t) break;
-        best_t = t;
-        best_k = k;
-        moved = true;
-    }
-    if (rc == SID_OK && !moved && k0 + s <= ne) {
-        const double t = eval(k0 + s);
-        if (rc == SID_OK && t < 0.995 * best_t) best_k = k0 + s;
+    for (int dir : {-s, s}) {
+        bool moved = false;
+        for (int k = best_k + dir; rc == SID_OK && k >= 0 && k <= ne; k += dir) {
+            const double t = eval(k);
+            if (rc != SID_OK || t >= 0.99 * best_t) break;
+            // confirm against the incumbent timed again after the candidate: one
+            // plan's pair time was measured to shift by up to 10 % between
+            // builds seconds apart, so a gain must hold on both sides of it
+            const double t_again = eval(best_k);
+            if (rc != SID_OK || t >= 0.99 * t_again) break;
+            best_t = t;
+            best_k = k;
+            moved = true;
+        }
+        if (moved || rc != SID_OK) break;
     }
     if (rc == SID_OK) {
         use_prefix(best_k);
