@@ -83,8 +83,9 @@ def test_psf_gradient_tiles_vs_oracle(env, n, X, Y, M, nz, nb):
 
 
 def test_gradients_vs_central_finite_differences(env):
-    """eps = 1e-3 central differences of the FP64 oracle NLL on random
-    coordinates of V and H against the device gradients (tolerance 1e-4)."""
+    """eps = 1e-3 central differences of the FP64 oracle NLL on 64 random
+    coordinates (32 of V, 32 of H) against the device gradients (tolerance
+    1e-4, relative to max(1, |fd|))."""
     torch, sid_amd, grad = env
     print("case: finite differences n5 X30 Y25 M11 Nz3 nb1", flush=True)
     H, V, y = _case(seed=1, nb=1)
@@ -100,14 +101,14 @@ def test_gradients_vs_central_finite_differences(env):
     eps = 1e-3
     nll = lambda HH, VV: O.poisson_nll(lambda v: O.forward_project(HH, v), VV, y[0])
     worst = 0.0
-    for _ in range(16):
+    for _ in range(32):
         i, j, c = rng.integers(0, 30), rng.integers(0, 25), rng.integers(0, 3)
         Vp, Vm = V[0].copy(), V[0].copy()
         Vp[i, j, c] += eps
         Vm[i, j, c] -= eps
         fd = (nll(H, Vp) - nll(H, Vm)) / (2 * eps)
         worst = max(worst, abs(fd - gV[i, j, c]) / max(1.0, abs(fd)))
-    for _ in range(16):
+    for _ in range(32):
         idx = tuple(int(rng.integers(0, s)) for s in H.shape)
         Hp, Hm = H.copy(), H.copy()
         Hp[idx] += eps
@@ -161,15 +162,16 @@ def test_autograd_functions(env):
 
 def test_refinement_checkpoint_resume_bit_exact(env, tmp_path):
     torch, sid_amd, grad = env
-    print("case: refinement 4 steps, resume after 2", flush=True)
+    print("case: refinement 20 steps, resume after 10", flush=True)
     H, V, y = _case(seed=5)
     Ht, Vt, yt = _to_dev(torch, H, V, y)
     V0 = torch.full_like(Vt, 0.5)
     psf = sid_amd.PSF(Ht, full_support=True)
-    Va, Ha, ha = grad.refine(psf, Ht, V0, yt, 4, checkpoint=str(tmp_path / "a.st"))
-    assert all(ha[i + 1] <= ha[i] for i in range(3)), ha
+    Va, Ha, ha = grad.refine(psf, Ht, V0, yt, 20, checkpoint=str(tmp_path / "a.st"))
+    # EM never increases the NLL; 1e-6 relative room for FP32 rounding once it plateaus
+    assert len(ha) == 20 and all(ha[i + 1] <= ha[i] + 1e-6 * abs(ha[i]) for i in range(19)), ha
     ck = str(tmp_path / "b.st")
-    grad.refine(psf, Ht, V0, yt, 4, checkpoint=ck, stop_after=2)
-    Vb, Hb, hb = grad.refine(psf, Ht, V0, yt, 4, checkpoint=ck)
+    grad.refine(psf, Ht, V0, yt, 20, checkpoint=ck, stop_after=10)
+    Vb, Hb, hb = grad.refine(psf, Ht, V0, yt, 20, checkpoint=ck)
     assert hb == ha
     assert torch.equal(Va, Vb) and torch.equal(Ha, Hb)
