@@ -613,11 +613,11 @@ __device__ __forceinline__ void stockham_stage_2d(const float2 *__restrict__ in,
     const int nb = L / R;
     const int tot = nseq * nb;
     const int step = L / (Ns * R);   // step * k * r < L: no wrap of the twiddle index
-    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns, rseq = 1.0f / (float)nseq;
+    const float rns = 1.0f / (float)Ns, rseq = 1.0f / (float)nseq;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        int sq, j;
-        if (es > 1) { j = qdiv(e, nseq, rseq); sq = e - j * nseq; }
-        else { sq = qdiv(e, nb, rnb); j = e - sq * nb; }
+        // neighbouring lanes take neighbouring lines: with an odd line pitch (or
+        // unit line stride) every read and write of a stage is bank-conflict free
+        const int j = qdiv(e, nseq, rseq), sq = e - j * nseq;
         const float2 *src = in + sq * ss;
         float2 *dst = out + sq * ss;
         const int jq = qdiv(j, Ns, rns), k = j - jq * Ns;
@@ -658,6 +658,10 @@ struct SubGeom {
     int nq, n2, Nxp, Nyp, Hy, Lx, Ly, T, S, nz_in;
     int64_t XP, YP;
 };
+
+// Row pitch of a sub-image grid in LDS: Ny' rounded up to odd, so that lanes
+// reading neighbouring rows at one column hit different banks.
+__host__ __device__ __forceinline__ int sub_pitch(int Nyp) { return Nyp | 1; }
 
 // Forward: volume -> 2-D DFT of the pair's sub-images -> vt[k0][q] = s_q * V_q(k0)
 // for the stored half spectrum.  Z = DFT(a + i b) gives
@@ -769,12 +773,12 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__rest
                                                              const float2 *__restrict__ twy, float2 *__restrict__ vt)
 {
     extern __shared__ float2 sm[];
-    const int npix = g.Nxp * g.Nyp;
-    float2 *bufa = sm, *bufb = sm + npix, *tx = sm + 2 * npix, *ty = tx + g.Nxp;
+    const int npix = g.Nxp * g.Nyp, P = sub_pitch(g.Nyp), nlds = g.Nxp * P;
+    float2 *bufa = sm, *bufb = sm + nlds, *tx = sm + 2 * nlds, *ty = tx + g.Nxp;
     load_twiddles(tx, ty, twx, twy, g.Nxp, g.Nyp);
     PairIter it = pair_iter((g.nq + 1) / 2);
     if (it.next >= it.end) return;   // block-uniform
-    const float rhy = 1.0f / (float)g.Hy;
+    const float rhy = 1.0f / (float)g.Hy, rnyp = 1.0f / (float)g.Nyp;
     const int nk0 = g.Nxp * g.Hy;
     float a[kSubUnroll], b[kSubUnroll];
     sub_fwd_fetch(vol, slot_plane, g, it.next, a, b);
@@ -783,12 +787,15 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__rest
 #pragma unroll
         for (int u = 0; u < kSubUnroll; ++u) {
             const int e = u * kSubThreads + (int)threadIdx.x;
-            if (e < npix) bufa[e] = make_float2(a[u], b[u]);
+            if (e < npix) {
+                const int Qx = qdiv(e, g.Nyp, rnyp), Qy = e - Qx * g.Nyp;
+                bufa[Qx * P + Qy] = make_float2(a[u], b[u]);
+            }
         }
         lds_barrier();
         if (pair + it.step < it.end) sub_fwd_fetch(vol, slot_plane, g, pair + it.step, a, b);
-        float2 *r1 = fft_pass_2d(bufa, bufb, dy, ty, -1.0f, g.Nxp, g.Nyp, 1);         // along y (rows)
-        float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dx, tx, -1.0f, g.Nyp, 1, g.Nyp);  // along x
+        float2 *r1 = fft_pass_2d(bufa, bufb, dy, ty, -1.0f, g.Nxp, P, 1);               // along y (rows)
+        float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dx, tx, -1.0f, g.Nyp, 1, P);   // along x
         const int q0 = 2 * pair;
         const bool has1 = q0 + 1 < g.nq;
         const float s0 = qscale ? qscale[q0] : 1.0f;
@@ -796,9 +803,9 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__rest
         if (has1) {   // uniform per pair: one 16-byte store of both spectra per k0
             for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
                 const int kx = qdiv(k0, g.Hy, rhy), ky = k0 - kx * g.Hy;
-                const float2 z = r2[kx * g.Nyp + ky];
+                const float2 z = r2[kx * P + ky];
                 const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
-                const float2 zm = r2[mx * g.Nyp + my];
+                const float2 zm = r2[mx * P + my];
                 const float2 av = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
                 const float2 bv = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
                 st_c2(vt + (int64_t)k0 * g.nq + q0, make_float2(av.x * s0, av.y * s0),
@@ -807,9 +814,9 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_fwd(const float *__rest
         } else {
             for (int k0 = threadIdx.x; k0 < nk0; k0 += blockDim.x) {
                 const int kx = qdiv(k0, g.Hy, rhy), ky = k0 - kx * g.Hy;
-                const float2 z = r2[kx * g.Nyp + ky];
+                const float2 z = r2[kx * P + ky];
                 const int mx = kx == 0 ? 0 : g.Nxp - kx, my = ky == 0 ? 0 : g.Nyp - ky;
-                const float2 zm = r2[mx * g.Nyp + my];
+                const float2 zm = r2[mx * P + my];
                 vt[(int64_t)k0 * g.nq + q0] = make_float2(0.5f * (z.x + zm.x) * s0, 0.5f * (z.y - zm.y) * s0);
             }
         }
@@ -873,12 +880,12 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_bwd(const float2 *__res
                                                              float *__restrict__ dfft)
 {
     extern __shared__ float2 sm[];
-    const int npix = g.Nxp * g.Nyp;
-    float2 *bufa = sm, *bufb = sm + npix, *tx = sm + 2 * npix, *ty = tx + g.Nxp;
+    const int npix = g.Nxp * g.Nyp, P = sub_pitch(g.Nyp), nlds = g.Nxp * P;
+    float2 *bufa = sm, *bufb = sm + nlds, *tx = sm + 2 * nlds, *ty = tx + g.Nxp;
     load_twiddles(tx, ty, twx, twy, g.Nxp, g.Nyp);
     PairIter it = pair_iter((g.nq + 1) / 2);
     if (it.next >= it.end) return;   // block-uniform
-    const float rly = 1.0f / (float)g.Ly;
+    const float rly = 1.0f / (float)g.Ly, rnyp = 1.0f / (float)g.Nyp;
     const int npx = g.Lx * g.Ly;
     float2 a[kSubUnroll], b[kSubUnroll];
     unsigned cj = 0u;
@@ -896,26 +903,27 @@ __global__ __launch_bounds__(kSubThreads) void k_fft_sub_bwd(const float2 *__res
                 const bool c = (cj >> u) & 1u;
                 const float2 x = make_float2(a[u].x * s0, (c ? -a[u].y : a[u].y) * s0);
                 const float2 y = make_float2(b[u].x * s1, (c ? -b[u].y : b[u].y) * s1);
-                bufa[e] = make_float2(x.x - y.y, x.y + y.x);   // A + i B
+                const int kx = qdiv(e, g.Nyp, rnyp), ky = e - kx * g.Nyp;
+                bufa[kx * P + ky] = make_float2(x.x - y.y, x.y + y.x);   // A + i B
             }
         }
         lds_barrier();
         if (pair + it.step < it.end) sub_bwd_fetch(bt, g, pair + it.step, a, b, cj);
-        float2 *r1 = fft_pass_2d(bufa, bufb, dx, tx, 1.0f, g.Nyp, 1, g.Nyp);          // along x
-        float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dy, ty, 1.0f, g.Nxp, g.Nyp, 1);  // along y
+        float2 *r1 = fft_pass_2d(bufa, bufb, dx, tx, 1.0f, g.Nyp, 1, P);                // along x
+        float2 *r2 = fft_pass_2d(r1, r1 == bufa ? bufb : bufa, dy, ty, 1.0f, g.Nxp, P, 1);    // along y
         const int c0 = slot_plane[q0 / g.n2], ab0 = q0 % g.n2;
         const int c1 = has1 ? slot_plane[(q0 + 1) / g.n2] : 0, ab1 = has1 ? (q0 + 1) % g.n2 : 0;
         if (has1 && c1 == c0) {   // neighbours in one lenslet record (uniform per pair): one 8-byte store
             for (int e = threadIdx.x; e < npx; e += blockDim.x) {
                 const int Qx = qdiv(e, g.Ly, rly), Qy = e - Qx * g.Ly;
-                const float2 z = r2[Qx * g.Nyp + Qy];
+                const float2 z = r2[Qx * P + Qy];
                 const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
                 st_f2_a4(dfft + ((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0, make_float2(z.x * scale, z.y * scale));
             }
         } else {
             for (int e = threadIdx.x; e < npx; e += blockDim.x) {
                 const int Qx = qdiv(e, g.Ly, rly), Qy = e - Qx * g.Ly;
-                const float2 z = r2[Qx * g.Nyp + Qy];
+                const float2 z = r2[Qx * P + Qy];
                 const int64_t cell = ((int64_t)Qx + g.T) * g.YP + (Qy + g.T);
                 dfft[((int64_t)c0 * g.XP * g.YP + cell) * g.S + ab0] = z.x * scale;
                 if (has1) dfft[((int64_t)c1 * g.XP * g.YP + cell) * g.S + ab1] = z.y * scale;
@@ -1113,7 +1121,7 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     // fused sub-image transforms when both Nx' x Ny' grids fit in a block's LDS
     // (SID_FFT_UNFUSED=1 keeps the separate gather / 1-D FFT / pack kernels)
     {
-        const size_t lds = sizeof(float2) * (2 * (size_t)f.Nxp * f.Nyp + f.Nxp + f.Nyp);
+        const size_t lds = sizeof(float2) * (2 * (size_t)f.Nxp * sub_pitch(f.Nyp) + f.Nxp + f.Nyp);
         hipDeviceProp_t prop;
         f.fused = std::getenv("SID_FFT_UNFUSED") == nullptr &&
                   (int64_t)f.Nxp * f.Nyp <= (int64_t)kSubUnroll * kSubThreads &&
