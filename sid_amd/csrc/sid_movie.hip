@@ -45,9 +45,10 @@ __global__ __launch_bounds__(kMvThreads) void k_mv_max(const float *__restrict__
 
 // Column pass over frames j0..j0+3 and pixel chunk c:
 // part[c][j] = scale * sum_{i in chunk c} s_i * Y[i, j], s_i = sd[i] (or 1 when sd is null).
-// Four frames per block so s is read once per four columns (eight halve the
-// occupancy and measured 1.7x slower); chunks keep the grid many blocks deep;
-// the host adds the chunks in order.
+// Four frames per block so s is read once per four columns; 8 or 16 frames per
+// block (1-4 pixels per trip, shuffle reductions) measured 8-100 % slower for
+// the pass although they read s less often.  Chunks keep the grid many blocks
+// deep; the host adds the chunks in order.
 constexpr int kColGroup = 4;
 __global__ __launch_bounds__(kMvThreads) void k_mv_col(const float *__restrict__ Y, int64_t P, int64_t T,
                                                        const double *__restrict__ sd, double scale, int64_t pc,
