@@ -25,13 +25,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
 
 // FP32 -> BF16 bits (round to nearest even, NaN kept) and back.
-__device__ __forceinline__ u16 f2bf(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                          // round to nearest even
-    return (u16)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950; NaN
+// stays NaN, infinities and overflow to infinity as IEEE rounding gives)
+__device__ __forceinline__ u16 f2bf(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
 __device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 void set_error(const char *fmt, ...);
