@@ -569,7 +569,8 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
     return SID_OK;
 }
 
-static int forward_raw(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
+static int forward_raw(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st,
+                       bool sync = true)
 {
     SID_TRY(autotune(p, vdims[0], vdims[1], vdims[2], st));
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
@@ -581,12 +582,14 @@ static int forward_raw(sid_psf *p, const float *V_dev, const int64_t vdims[3], f
     SID_TRY(launch_fwd_reduce(p, O_dev, st));
     const int e3 = tm.mark();
     tm.span(1, e2, e3);
+    if (!sync) return SID_OK;
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e3);
     return SID_OK;
 }
 
-static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st)
+static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st,
+                        bool sync = true)
 {
     // the backward volume always has all planes of H (backwardProjectGPU_new.m:4-5); the
     // sensor plan keeps the nz of the last forward call.
@@ -601,6 +604,7 @@ static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], 
     SID_TRY(launch_bwd_reduce(p, B_dev, 0, st));
     const int e3 = tm.mark();
     tm.span(3, e2, e3);
+    if (!sync) return SID_OK;
     SID_TRY_HIP(hipStreamSynchronize(st));
     tm.finish(e0, e3);
     return SID_OK;
@@ -627,23 +631,25 @@ static int handle_convn(sid_psf *p, const float *in, const int64_t vdims[3], flo
 }
 
 // forwardFUN = F(convn(V, k, 'same')) when the handle carries a neuron kernel (reconstruct_S.m:126,188)
-static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st)
+static int forward_impl(sid_psf *p, const float *V_dev, const int64_t vdims[3], float *O_dev, hipStream_t st,
+                        bool sync = true)
 {
-    if (!p->kern) return forward_raw(p, V_dev, vdims, O_dev, st);
+    if (!p->kern) return forward_raw(p, V_dev, vdims, O_dev, st, sync);
     SID_TRY(kwork_reserve(p, vdims[0] * vdims[1] * vdims[2]));
     SID_TRY(handle_convn(p, V_dev, vdims, p->kwork, st));
-    return forward_raw(p, p->kwork, vdims, O_dev, st);
+    return forward_raw(p, p->kwork, vdims, O_dev, st, sync);
 }
 
 // backwardFUN = convn(B(O), k, 'same') (reconstruct_S.m:127,189)
-static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st)
+static int backward_impl(sid_psf *p, const float *O_dev, const int64_t odims[2], float *B_dev, hipStream_t st,
+                         bool sync = true)
 {
-    if (!p->kern) return backward_raw(p, O_dev, odims, B_dev, st);
+    if (!p->kern) return backward_raw(p, O_dev, odims, B_dev, st, sync);
     const int64_t vdims[3] = {odims[0], odims[1], p->nzH};
     SID_TRY(kwork_reserve(p, vdims[0] * vdims[1] * vdims[2]));
-    SID_TRY(backward_raw(p, O_dev, odims, p->kwork, st));
+    SID_TRY(backward_raw(p, O_dev, odims, p->kwork, st, sync));
     SID_TRY(handle_convn(p, p->kwork, vdims, B_dev, st));
-    SID_TRY_HIP(hipStreamSynchronize(st));
+    if (sync) SID_TRY_HIP(hipStreamSynchronize(st));
     return SID_OK;
 }
 
@@ -1044,12 +1050,12 @@ int sid_selftest(void)
 }  // extern "C"
 
 // Projector passes on column-major device buffers for the solvers (checks done by the caller).
-int sid::forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st)
+int sid::forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st, bool sync)
 {
-    return forward_impl(p, V, vdims, O, st);
+    return forward_impl(p, V, vdims, O, st, sync);
 }
 
-int sid::backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st)
+int sid::backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st, bool sync)
 {
-    return backward_impl(p, O, odims, B, st);
+    return backward_impl(p, O, odims, B, st, sync);
 }

@@ -232,8 +232,10 @@ int run_selftest();
 
 namespace sid {
 // sid_api.hip: projector passes on column-major device buffers (used by the solvers)
-int forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st);
-int backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st);
+// sync = false leaves the work queued on st (no host wait, no per-call timing;
+// the caller drains the FFT path's timing events with fft_timing_collect)
+int forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st, bool sync = true);
+int backward_dev(sid_psf *p, const float *O, const int64_t odims[2], float *B, hipStream_t st, bool sync = true);
 // sid_convn.hip: convn(in, k, 'same') on column-major device volumes (k on the device, [kx][ky][kz])
 constexpr int kConvnMaxXY = 65, kConvnMaxZ = 255;
 int convn_same(const float *in, const int64_t vdims[3], const float *k_dev, const int kd[3], float *out,

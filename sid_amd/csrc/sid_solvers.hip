@@ -81,9 +81,10 @@ __global__ void k_ls_dir(const float *__restrict__ X, const float *__restrict__ 
 }
 
 // Xn = max(X - alpha d, 0) written over d; t = Xn - X   (:89-91, 94)
-__global__ void k_ls_step(const float *__restrict__ X, float *__restrict__ d, float alpha, int64_t n,
-                          float *__restrict__ t)
+__global__ void k_ls_step(const float *__restrict__ X, float *__restrict__ d, const float *__restrict__ alpha_p,
+                          int64_t n, float *__restrict__ t)
 {
+    const float alpha = *alpha_p;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         float xn = X[e] - alpha * d[e];
         if (xn < 0.0f) xn = 0.0f;
@@ -113,6 +114,25 @@ __global__ void __launch_bounds__(kDotBlock) k_dot_partial(const float *__restri
 
 static int nblk(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, kDotMaxBlocks)); }
 
+// alpha = num / den (:84), 0 when NaN or Inf (:85-87), from the block partials
+// of the two dot products summed in block order (one thread, the same order a
+// host sum would take), so the step size never leaves the device.
+__global__ void __launch_bounds__(kDotBlock) k_ls_alpha(const double *__restrict__ pn, int nn,
+                                                        const double *__restrict__ pd, int nd, float *__restrict__ alpha)
+{
+    __shared__ double sn[kDotMaxBlocks], sd[kDotMaxBlocks];
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) sn[i] = pn[i];
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) sd[i] = pd[i];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < nn; ++i) num += sn[i];
+    for (int i = 0; i < nd; ++i) den += sd[i];
+    double a = num / den;
+    if (std::isnan(a) || std::isinf(a)) a = 0.0;
+    *alpha = (float)a;
+}
+
 struct Bufs {
     std::vector<void *> ptrs;
     ~Bufs()
@@ -134,19 +154,11 @@ struct Bufs {
     }
 };
 
-// a'b in FP64 (block partials summed on the host in block order)
-static int dot(const float *a, const float *b, int64_t n, double *partial, std::vector<double> &host, hipStream_t st,
-               double *out)
+// FP64 block partials of a'b (nblk(n) of them, into partial)
+static int dot_partials(const float *a, const float *b, int64_t n, double *partial, hipStream_t st)
 {
-    const int nb = nblk(n);
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kDotBlock), 0, st, a, b, n, partial);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nblk(n)), dim3(kDotBlock), 0, st, a, b, n, partial);
     SID_TRY_HIP(hipGetLastError());
-    host.resize(nb);
-    SID_TRY_HIP(hipMemcpyAsync(host.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
-    SID_TRY_HIP(hipStreamSynchronize(st));
-    double s = 0.0;
-    for (double v : host) s += v;
-    *out = s;
     return SID_OK;
 }
 
@@ -172,50 +184,57 @@ int ls_deconv(sid_psf *p, const float *df, float *Xout, const int64_t vdims[3], 
     for (int i = 0; i < 3; ++i) alt = alt && (o.lamb_TV_L1 * o.lamb_TV_L2[i] * o.lamb_L2 != 0.0);
 
     Bufs b;
-    float *X, *g, *d, *tt, *q, *img;
-    double *partial;
+    float *X, *g, *d, *tt, *q, *img, *alpha;
+    double *pnum, *pden;
     SID_TRY(b.alloc(&X, nv));
     SID_TRY(b.alloc(&g, nv));
     SID_TRY(b.alloc(&d, nv));
     SID_TRY(b.alloc(&tt, nv));
     SID_TRY(b.alloc(&q, nv));
     SID_TRY(b.alloc(&img, ni));
-    SID_TRY(b.alloc(&partial, kDotMaxBlocks));
-    std::vector<double> host;
+    SID_TRY(b.alloc(&pnum, kDotMaxBlocks));
+    SID_TRY(b.alloc(&pden, kDotMaxBlocks));
+    SID_TRY(b.alloc(&alpha, 1));
     const int nbv = nblk(nv);
+    // every pass below stays queued on st: the projections skip their host waits
+    // and the step size is computed on the device
     SID_TRY_HIP(hipMemcpyAsync(X, df, sizeof(float) * nv, hipMemcpyDeviceToDevice, st));
     // g = Q(df) - df + lamb_L1
-    SID_TRY(forward_dev(p, X, vdims, img, st));
-    SID_TRY(backward_dev(p, img, odims, q, st));
+    SID_TRY(forward_dev(p, X, vdims, img, st, false));
+    SID_TRY(backward_dev(p, img, odims, q, st, false));
     hipLaunchKernelGGL(k_ls_qcomb, dim3(nbv), dim3(256), 0, st, q, X, g, 0, (float)o.lamb_L1, t);
     SID_TRY_HIP(hipGetLastError());
     for (int it = 0; it < o.max_iter; ++it) {
         hipLaunchKernelGGL(k_ls_dir, dim3(nbv), dim3(256), 0, st, X, g, nv, d);
         SID_TRY_HIP(hipGetLastError());
-        SID_TRY(forward_dev(p, d, vdims, img, st));
-        double num = 0.0, den = 0.0;
+        SID_TRY(forward_dev(p, d, vdims, img, st, false));
+        int nden;
         if (alt) {
-            SID_TRY(backward_dev(p, img, odims, q, st));
+            SID_TRY(backward_dev(p, img, odims, q, st, false));
             hipLaunchKernelGGL(k_ls_qcomb, dim3(nbv), dim3(256), 0, st, q, d, q, 2, 0.0f, t);
             SID_TRY_HIP(hipGetLastError());
-            SID_TRY(dot(d, q, nv, partial, host, st, &den));
+            SID_TRY(dot_partials(d, q, nv, pden, st));
+            nden = nbv;
         } else {
-            SID_TRY(dot(img, img, ni, partial, host, st, &den));
+            SID_TRY(dot_partials(img, img, ni, pden, st));
+            nden = nblk(ni);
         }
-        SID_TRY(dot(d, d, nv, partial, host, st, &num));
-        double alpha = num / den;
-        if (std::isnan(alpha) || std::isinf(alpha)) alpha = 0.0;
-        hipLaunchKernelGGL(k_ls_step, dim3(nbv), dim3(256), 0, st, X, d, (float)alpha, nv, tt);
+        SID_TRY(dot_partials(d, d, nv, pnum, st));
+        hipLaunchKernelGGL(k_ls_alpha, dim3(1), dim3(kDotBlock), 0, st, pnum, nbv, pden, nden, alpha);
+        SID_TRY_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ls_step, dim3(nbv), dim3(256), 0, st, X, d, alpha, nv, tt);
         SID_TRY_HIP(hipGetLastError());
         // g = g + Q(Xn - X)
-        SID_TRY(forward_dev(p, tt, vdims, img, st));
-        SID_TRY(backward_dev(p, img, odims, q, st));
+        SID_TRY(forward_dev(p, tt, vdims, img, st, false));
+        SID_TRY(backward_dev(p, img, odims, q, st, false));
         hipLaunchKernelGGL(k_ls_qcomb, dim3(nbv), dim3(256), 0, st, q, tt, g, 1, 0.0f, t);
         SID_TRY_HIP(hipGetLastError());
         std::swap(X, d);  // X = Xn
     }
     SID_TRY_HIP(hipMemcpyAsync(Xout, X, sizeof(float) * nv, hipMemcpyDeviceToDevice, st));
     SID_TRY_HIP(hipStreamSynchronize(st));
+    double f_ms, b_ms;
+    fft_timing_collect(p, &f_ms, &b_ms);   // release the FFT path's timing events
     return SID_OK;
 }
 
@@ -244,14 +263,16 @@ int rl_wrapped(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], i
     float *img, *D;
     SID_TRY(b.alloc(&img, ni));
     SID_TRY(b.alloc(&D, nv));
-    for (int it = 0; it < max_iter; ++it) {
-        SID_TRY(forward_dev(p, X, vdims, img, st));
-        SID_TRY(backward_dev(p, img, odims, D, st));
+    for (int it = 0; it < max_iter; ++it) {   // queued on st without host waits
+        SID_TRY(forward_dev(p, X, vdims, img, st, false));
+        SID_TRY(backward_dev(p, img, odims, D, st, false));
         hipLaunchKernelGGL(k_rl_update, dim3(nblk(nv)), dim3(256), 0, st, X, Htf, D, nv);
         SID_TRY_HIP(hipGetLastError());
         if (hist) SID_TRY_HIP(hipMemcpyAsync(hist + (int64_t)it * nv, X, sizeof(float) * nv, hipMemcpyDeviceToDevice, st));
     }
     SID_TRY_HIP(hipStreamSynchronize(st));
+    double f_ms, b_ms;
+    fft_timing_collect(p, &f_ms, &b_ms);   // release the FFT path's timing events
     return SID_OK;
 }
 
