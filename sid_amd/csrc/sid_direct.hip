@@ -626,7 +626,40 @@ __global__ __launch_bounds__(256) void k_bwd_reduce(const T *__restrict__ part, 
     if (B_col && real) B_col[col] = (float)v;
 }
 
-// MATLAB column-major volume -> padded lenslet-major volume (+ BF16 shadows).
+// MATLAB column-major volume -> padded lenslet-major volume (+ BF16 shadows),
+// through LDS: a block takes one lenslet row Py of one plane and kVinPx
+// lenslets along x, reads the n sensor rows of that strip whole (n * kVinPx
+// contiguous values each) and writes each lenslet record (n * n contiguous
+// values) whole; a thread per element wrote n-value fragments of records.
+constexpr int kVinPx = 16;
+template <typename T>
+__global__ __launch_bounds__(256) void k_vol_in_t(const float *__restrict__ V, int X, int Y, int n, int Lx,
+                                                  int64_t XP, int64_t YP, int T_, int S, T *__restrict__ dst,
+                                                  u16 *__restrict__ hi, u16 *__restrict__ lo)
+{
+    __shared__ float tile[16 * 16 * kVinPx];   // n <= 16 rows of n * kVinPx values
+    const int Py = blockIdx.x, Px0 = blockIdx.y * kVinPx, c = blockIdx.z;
+    const int W = n * kVinPx, x0 = Px0 * n, y0 = Py * n;
+    const float *src = V + (int64_t)c * X * Y;
+    for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
+        const int r = e / W, col = e - r * W;
+        const int x = x0 + col, y = y0 + r;
+        tile[e] = (x < X && y < Y) ? src[(int64_t)y * X + x] : 0.0f;
+    }
+    __syncthreads();
+    const int n2 = n * n;
+    for (int e = threadIdx.x; e < kVinPx * n2; e += blockDim.x) {
+        const int k = e / n2, ab = e - k * n2;
+        const int a = ab % n, b = ab / n;
+        const int Px = Px0 + k;
+        if (Px >= Lx || Px * n + a >= X || y0 + b >= Y) continue;
+        const float v = tile[b * W + k * n + a];
+        const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
+        if (dst) dst[o] = (T)v;
+        if (hi) store_shadow(hi, lo, o, v);
+    }
+}
+
 template <typename T>
 __global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int n, int64_t XP, int64_t YP, int T_,
                          int S, T *__restrict__ dst, u16 *__restrict__ hi, u16 *__restrict__ lo)
@@ -1203,13 +1236,14 @@ int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which, hipStre
     const bool sh = which == 0 && (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3);
     const int64_t total = sp.X * sp.Y * nz;
     if (total == 0) return SID_OK;
+    const dim3 grid((unsigned)sp.Ly, (unsigned)((sp.Lx + kVinPx - 1) / kVinPx), (unsigned)nz);
     if (f64)
-        hipLaunchKernelGGL(k_vol_in<double>, dim3(grid_for(total)), dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y,
-                           (int)nz, (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol_d : sp.htf_d,
+        hipLaunchKernelGGL(k_vol_in_t<double>, grid, dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y, (int)p->n,
+                           (int)sp.Lx, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol_d : sp.htf_d,
                            (u16 *)nullptr, (u16 *)nullptr);
     else
-        hipLaunchKernelGGL(k_vol_in<float>, dim3(grid_for(total)), dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y,
-                           (int)nz, (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol : sp.htf,
+        hipLaunchKernelGGL(k_vol_in_t<float>, grid, dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y, (int)p->n,
+                           (int)sp.Lx, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol : sp.htf,
                            sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
