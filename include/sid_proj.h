@@ -246,7 +246,9 @@ int sid_ls_deconv_device(sid_psf *psf, const float *df_dev, float *X_dev, const 
  * [0] forward direct GEMM, [1] forward reduce, [2] backward direct GEMM,
  * [3] backward reduce + update, [4] whole call, [5] forward FFT path,
  * [6] backward FFT path, [7] forward spectra GEMV, [8] backward spectra GEMV,
- * [9..11] 0.  Summed over iterations for sid_deconv_rl. */
+ * [9..11] 0.  Summed over iterations for sid_deconv_rl.  sid_ls_deconv and
+ * the ISRA loop of a handle with a neuron kernel queue their projections
+ * without per-projection timing and leave these values as they were. */
 int sid_last_timing(const sid_psf *psf, double ms[12]);
 
 /* Kernel launch counts and GEMM geometry of the direct path for the current
