@@ -262,6 +262,8 @@ def main():
                     help="projector method per plane (auto = autotuned direct-vs-FFT choice)")
     ap.add_argument("--grad-only", action="store_true",
                     help="run only the differentiation row (NLL + volume and PSF gradients, 16 frames 495^2 x 31)")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer leg (sid_deconv_rl with PCIe transfers, one 8-iteration call)")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the direct-only leg (every plane on the MFMA GEMM, one step per precision)")
     ap.add_argument("--fft-planes", type=int, default=-1,
@@ -411,6 +413,18 @@ def main():
         out_rows["gradient"] = grad_bench(dev)
     head = args.precisions.split(",")[-1]
     base = args.precisions.split(",")[0]
+    host_rate = None
+    if not args.no_host:
+        # the host-buffer entry (sid_deconv_rl, what a MATLAB MEX caller uses): Htf and X cross PCIe
+        # from and to pageable host memory around the same 8 device iterations
+        psf.set_precision(head)
+        h_np = np.asfortranarray(htfs[head].cpu().numpy().transpose(2, 1, 0))
+        psf.deconv_rl(h_np, args.iters, h_np)
+        t0 = time.perf_counter()
+        psf.deconv_rl(h_np, args.iters, h_np)
+        host_rate = round(args.iters / (time.perf_counter() - t0), 3)
+        del h_np
+        log(f"host-buffer entry ({head}): {host_rate:.2f} RL iter/s including the PCIe transfers")
     direct = None
     if not args.no_direct and args.method == "auto":
         # every plane on the implicit GEMM: the MFMA path alone, per precision (one timed step each)
@@ -490,6 +504,7 @@ def main():
         "methods": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
         "speedup_vs_fp32": round(hr["iter_per_s"] / results[base]["iter_per_s"], 3) if base in results else None,
+        "host_entry_iter_per_s": host_rate,
         "direct_only_iter_per_s": direct,
         "speedup_vs_fp32_direct_only": (round(direct[head] / direct[base], 3)
                                         if direct and head in direct and base in direct and head != base else None),
