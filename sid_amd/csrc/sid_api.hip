@@ -148,7 +148,8 @@ static void destroy(sid_psf *p)
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
                     (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
-                    (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag})
+                    (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag,
+                    (void *)p->hbuf[0], (void *)p->hbuf[1]})
         if (q) (void)hipFree(q);
     if (p->aux) {
         (void)hipStreamSynchronize(p->aux);
@@ -232,8 +233,14 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
         const int64_t elems = dims[0] * dims[1] * dims[2] * dims[3] * dims[4];
         hipError_t e = hipMalloc(&p->H, elems * sizeof(float));
         if (e != hipSuccess) { rc = SID_ERR_OOM; set_error("hipMalloc(H, %lld B) failed", (long long)(elems * 4)); break; }
-        e = hipMemcpy(p->H, H, elems * sizeof(float), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice);
-        if (e != hipSuccess) { rc = SID_ERR_HIP; set_error("H upload failed: %s", hipGetErrorString(e)); break; }
+        if (host) {
+            rc = copy_h2d(p->H, H, elems * sizeof(float), nullptr);
+            if (rc == SID_OK && (e = hipStreamSynchronize(nullptr)) != hipSuccess) rc = SID_ERR_HIP;
+            if (rc) { set_error("H upload failed"); break; }
+        } else {
+            e = hipMemcpy(p->H, H, elems * sizeof(float), hipMemcpyDeviceToDevice);
+            if (e != hipSuccess) { rc = SID_ERR_HIP; set_error("H upload failed: %s", hipGetErrorString(e)); break; }
+        }
         rc = create_common(p, dims, CA, full);
     } while (0);
     if (rc != SID_OK) {
@@ -265,8 +272,10 @@ int sid_psf_update(sid_psf *p, const float *H, int on_device)
     if (!p || !H) { set_error("sid_psf_update: null argument"); return SID_ERR_ARG; }
     SID_TRY_HIP(hipSetDevice(p->device));
     const int64_t elems = p->M * p->M * p->n * p->n * p->nzH;
-    SID_TRY_HIP(hipMemcpyAsync(p->H, H, elems * sizeof(float), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                               p->stream));
+    if (on_device)
+        SID_TRY_HIP(hipMemcpyAsync(p->H, H, elems * sizeof(float), hipMemcpyDeviceToDevice, p->stream));
+    else
+        SID_TRY(copy_h2d(p->H, H, elems * sizeof(float), p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
     const bool had[4] = {p->wf != nullptr, p->wf_hi != nullptr, p->wf_lo != nullptr, p->wf_d != nullptr};
     lib_plan_free(p);  // its captured graph holds the old weight pointers
@@ -669,7 +678,9 @@ int sid_backward_device(sid_psf *p, const float *O, const int64_t odims[2], floa
     return backward_impl(p, O, odims, B, pick_stream(p, stream));
 }
 
-// Host-buffer variants: stage through temporary device buffers.
+// Host-buffer variants: stage through device buffers kept on the handle
+// (hbuf_reserve; a 206 MB hipMalloc + hipFree per call cost more than the
+// transfer), or temporary ones (DevBuf) for the optional history.
 struct DevBuf {
     void *ptr = nullptr;
     ~DevBuf()
@@ -678,19 +689,31 @@ struct DevBuf {
     }
 };
 
+static int hbuf_reserve(sid_psf *p, int i, int64_t n, float **out)
+{
+    if (p->hbuf_elems[i] < n) {
+        if (p->hbuf[i]) (void)hipFree(p->hbuf[i]);
+        p->hbuf[i] = nullptr;
+        p->hbuf_elems[i] = 0;
+        SID_TRY_HIP(hipMalloc(&p->hbuf[i], sizeof(float) * std::max<int64_t>(n, 1)));
+        p->hbuf_elems[i] = n;
+    }
+    *out = p->hbuf[i];
+    return SID_OK;
+}
+
 int sid_forward(sid_psf *p, const float *V, const int64_t vdims[3], float *O)
 {
     if (!p || !V || !O) { set_error("sid_forward: null argument"); return SID_ERR_ARG; }
     SID_TRY(check_vdims(p, vdims, false));
     SID_TRY_HIP(hipSetDevice(p->device));
     const int64_t nv = vdims[0] * vdims[1] * vdims[2], no = vdims[0] * vdims[1];
-    DevBuf dv, dO;
-    SID_TRY_HIP(hipMalloc(&dv.ptr, nv * sizeof(float)));
-    SID_TRY_HIP(hipMalloc(&dO.ptr, no * sizeof(float)));
-    SID_TRY_HIP(hipMemcpyAsync(dv.ptr, V, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
-    SID_TRY(forward_impl(p, (const float *)dv.ptr, vdims, (float *)dO.ptr, p->stream));
-    SID_TRY_HIP(hipMemcpyAsync(O, dO.ptr, no * sizeof(float), hipMemcpyDeviceToHost, p->stream));
-    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    float *dv, *dO;
+    SID_TRY(hbuf_reserve(p, 0, nv, &dv));
+    SID_TRY(hbuf_reserve(p, 1, no, &dO));
+    SID_TRY(copy_h2d(dv, V, nv * sizeof(float), p->stream));
+    SID_TRY(forward_impl(p, dv, vdims, dO, p->stream));
+    SID_TRY(copy_d2h(O, dO, no * sizeof(float), p->stream));
     return SID_OK;
 }
 
@@ -700,12 +723,12 @@ int sid_backward(sid_psf *p, const float *O, const int64_t odims[2], float *B)
     if (odims[0] < 1 || odims[1] < 1) { set_error("image dimensions must be positive"); return SID_ERR_DIMS; }
     SID_TRY_HIP(hipSetDevice(p->device));
     const int64_t no = odims[0] * odims[1], nb = no * p->nzH;
-    DevBuf dO, dB;
-    SID_TRY_HIP(hipMalloc(&dO.ptr, no * sizeof(float)));
-    SID_TRY_HIP(hipMalloc(&dB.ptr, nb * sizeof(float)));
-    SID_TRY_HIP(hipMemcpyAsync(dO.ptr, O, no * sizeof(float), hipMemcpyHostToDevice, p->stream));
-    SID_TRY(backward_impl(p, (const float *)dO.ptr, odims, (float *)dB.ptr, p->stream));
-    SID_TRY_HIP(hipMemcpyAsync(B, dB.ptr, nb * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    float *dO, *dB;
+    SID_TRY(hbuf_reserve(p, 1, no, &dO));
+    SID_TRY(hbuf_reserve(p, 0, nb, &dB));
+    SID_TRY(copy_h2d(dO, O, no * sizeof(float), p->stream));
+    SID_TRY(backward_impl(p, dO, odims, dB, p->stream));
+    SID_TRY(copy_d2h(B, dB, nb * sizeof(float), p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
     return SID_OK;
 }
@@ -758,17 +781,16 @@ int sid_deconv_rl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3]
     SID_TRY(check_vdims(p, vdims, true));
     SID_TRY_HIP(hipSetDevice(p->device));
     const int64_t nv = vdims[0] * vdims[1] * vdims[2];
-    DevBuf dh, dx, dhist;
-    SID_TRY_HIP(hipMalloc(&dh.ptr, nv * sizeof(float)));
-    SID_TRY_HIP(hipMalloc(&dx.ptr, nv * sizeof(float)));
+    DevBuf dhist;
+    float *dh, *dx;
+    SID_TRY(hbuf_reserve(p, 0, nv, &dh));
+    SID_TRY(hbuf_reserve(p, 1, nv, &dx));
     if (hist && max_iter > 0) SID_TRY_HIP(hipMalloc(&dhist.ptr, nv * max_iter * sizeof(float)));
-    SID_TRY_HIP(hipMemcpyAsync(dh.ptr, Htf, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
-    SID_TRY_HIP(hipMemcpyAsync(dx.ptr, X, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
-    SID_TRY(rl_impl(p, (const float *)dh.ptr, (float *)dx.ptr, vdims, max_iter, (float *)dhist.ptr, p->stream));
-    SID_TRY_HIP(hipMemcpyAsync(X, dx.ptr, nv * sizeof(float), hipMemcpyDeviceToHost, p->stream));
-    if (dhist.ptr)
-        SID_TRY_HIP(hipMemcpyAsync(hist, dhist.ptr, nv * max_iter * sizeof(float), hipMemcpyDeviceToHost, p->stream));
-    SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    SID_TRY(copy_h2d(dh, Htf, nv * sizeof(float), p->stream));
+    SID_TRY(copy_h2d(dx, X, nv * sizeof(float), p->stream));
+    SID_TRY(rl_impl(p, dh, dx, vdims, max_iter, (float *)dhist.ptr, p->stream));
+    SID_TRY(copy_d2h(X, dx, nv * sizeof(float), p->stream));
+    if (dhist.ptr) SID_TRY(copy_d2h(hist, dhist.ptr, nv * max_iter * sizeof(float), p->stream));
     return SID_OK;
 }
 
@@ -882,12 +904,12 @@ int sid_ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3],
     SID_TRY(check_vdims(p, vdims, true));
     SID_TRY_HIP(hipSetDevice(p->device));
     const int64_t nv = vdims[0] * vdims[1] * vdims[2];
-    DevBuf dh, dx;
-    SID_TRY_HIP(hipMalloc(&dh.ptr, nv * sizeof(float)));
-    SID_TRY_HIP(hipMalloc(&dx.ptr, nv * sizeof(float)));
-    SID_TRY_HIP(hipMemcpyAsync(dh.ptr, df, nv * sizeof(float), hipMemcpyHostToDevice, p->stream));
-    SID_TRY(ls_deconv(p, (const float *)dh.ptr, (float *)dx.ptr, vdims, o, p->stream));
-    SID_TRY_HIP(hipMemcpyAsync(X, dx.ptr, nv * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    float *dh, *dx;
+    SID_TRY(hbuf_reserve(p, 0, nv, &dh));
+    SID_TRY(hbuf_reserve(p, 1, nv, &dx));
+    SID_TRY(copy_h2d(dh, df, nv * sizeof(float), p->stream));
+    SID_TRY(ls_deconv(p, dh, dx, vdims, o, p->stream));
+    SID_TRY(copy_d2h(X, dx, nv * sizeof(float), p->stream));
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
     return SID_OK;
 }
@@ -976,12 +998,12 @@ int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const
     SID_TRY_HIP(hipMalloc(&dv.ptr, sizeof(float) * nv));
     SID_TRY_HIP(hipMalloc(&dk.ptr, sizeof(float) * nk));
     SID_TRY_HIP(hipMalloc(&dout.ptr, sizeof(float) * nv));
-    SID_TRY_HIP(hipMemcpy(dv.ptr, V, sizeof(float) * nv, hipMemcpyHostToDevice));
+    SID_TRY(copy_h2d(dv.ptr, V, sizeof(float) * nv, nullptr));
     SID_TRY_HIP(hipMemcpy(dk.ptr, k, sizeof(float) * nk, hipMemcpyHostToDevice));
     const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
     SID_TRY(convn_dispatch((const float *)dv.ptr, vdims, (const float *)dk.ptr, kd, precision, (float *)dout.ptr,
                            nullptr));
-    SID_TRY_HIP(hipMemcpy(out, dout.ptr, sizeof(float) * nv, hipMemcpyDeviceToHost));
+    SID_TRY(copy_d2h(out, dout.ptr, sizeof(float) * nv, nullptr));
     return SID_OK;
 }
 
@@ -1007,7 +1029,7 @@ int sid_std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, 
     SID_TRY_HIP(hipMalloc(&ds.ptr, sizeof(float) * P));
     SID_TRY_HIP(hipMalloc(&dt.ptr, sizeof(float) * T));
     SID_TRY_HIP(hipMalloc(&dstd.ptr, sizeof(float) * P));
-    SID_TRY_HIP(hipMemcpy(dy.ptr, Y, sizeof(float) * P * T, hipMemcpyHostToDevice));
+    SID_TRY(copy_h2d(dy.ptr, Y, sizeof(float) * P * T, nullptr));
     SID_TRY(std_image((const float *)dy.ptr, P, T, bg_iter, flags, (float *)ds.ptr, (float *)dt.ptr,
                       (float *)dstd.ptr, ymax, nullptr));
     SID_TRY_HIP(hipMemcpy(std_img, dstd.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));

@@ -178,6 +178,8 @@ struct sid_psf {
     sid::u16 *kfrag = nullptr;        // its Toeplitz B fragments for the BF16 MFMA convn
     int kdims[3] = {0, 0, 0};
     float *kwork = nullptr;           // convn staging volume (column-major), kwork_elems floats
+    float *hbuf[2] = {nullptr, nullptr};   // device copies of the host-buffer entries' arrays
+    int64_t hbuf_elems[2] = {0, 0};
     int64_t kwork_elems = 0;
     hipEvent_t ev[10] = {};
     double timing[12] = {0};
@@ -232,6 +234,11 @@ int run_selftest();
 
 namespace sid {
 // sid_api.hip: projector passes on column-major device buffers (used by the solvers)
+// sid_hostio.hip: large pageable-host <-> device copies through pinned slots on
+// several host threads.  copy_h2d returns once the source has been read (the
+// DMAs are ordered before later work on st); copy_d2h returns with dst filled.
+int copy_h2d(void *dst_dev, const void *src_host, size_t bytes, hipStream_t st);
+int copy_d2h(void *dst_host, const void *src_dev, size_t bytes, hipStream_t st);
 // sync = false leaves the work queued on st (no host wait, no per-call timing;
 // the caller drains the FFT path's timing events with fft_timing_collect)
 int forward_dev(sid_psf *p, const float *V, const int64_t vdims[3], float *O, hipStream_t st, bool sync = true);
