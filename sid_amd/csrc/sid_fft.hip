@@ -469,14 +469,28 @@ __global__ void k_fft_img_grid(const float *__restrict__ img, int Nx, int Ny, in
     }
 }
 
+// Lane l of each 16-lane row gets lane l - o's value (0 for l < o), o in {1, 2, 4, 8}.
+__device__ __forceinline__ float row_shr_f(float v, int o)
+{
+    const int x = __float_as_int(v);
+    int y;
+    switch (o) {
+        case 8: y = __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true); break;
+        case 4: y = __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true); break;
+        case 2: y = __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true); break;
+        default: y = __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true); break;
+    }
+    return __int_as_float(y);
+}
+
 // Backward GEMV: bt[k0][q] = sum_j conj(G[k0][q][j]) * W(k0 + N' j).  Block
 // (k0, s) takes rows q of chunk s (qc rows, a multiple of 16).  The chunk is
 // contiguous in HBM, so tiles of 16 rows are copied into LDS with 16-byte
 // loads (every tile starts 16-byte aligned: k0 blocks are padded to a multiple
 // of 16 bytes, a tile starts at a multiple of 16 rows, and the host checks the
 // base alignment); then 16 lanes sum
-// one row (lane l takes j = l + 16 t, conflict-free LDS reads) and reduce with
-// four shuffles.  Results for consecutive q are stored contiguously
+// one row (lane l takes j = l + 16 t, conflict-free LDS reads, its 16 image
+// values W held in registers) and reduce with four DPP row shifts.  Results for consecutive q are stored contiguously
 // (k_fft_unpack_bt moves them into the per-q sub-spectra).  Requires n*n <= 256.
 template <typename GT>
 __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, const float2 *__restrict__ grid,
@@ -503,6 +517,12 @@ __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, 
     const GT *gk = G + (int64_t)k0 * kstride;
     const GT *ts = reinterpret_cast<const GT *>(tile);
     const int r = threadIdx.x >> 4, sl = threadIdx.x & 15;
+    // lane sl's image values W(k0 + N' j), j = sl + 16 t, held in registers for
+    // every row of the block (only the spectra tile is read from LDS per row)
+    __syncthreads();
+    float2 wr[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) wr[t] = w[sl + 16 * t];
     // Tiles are staged through registers: the loads of tile t+1 are issued
     // before tile t is summed, so HBM reads overlap the LDS work.  A tile is
     // at most 16 * 256 elements = kVec 16-byte vectors per thread.
@@ -528,17 +548,23 @@ __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, 
             tile_fetch<GT, kVec>(pre, pre_tail, gk + (int64_t)(q0 + TQ) * n2, min(TQ, qe - q0 - TQ) * n2);
         if (r < nr) {
             float re = 0.f, im = 0.f;
-            for (int j = sl; j < n2; j += 16) {
-                const float2 a = spec_ld(ts + r * n2 + j), b = w[j];
-                re += a.x * b.x + a.y * b.y;   // conj(a) * b
-                im += a.x * b.y - a.y * b.x;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int j = sl + 16 * t;
+                if (j < n2) {
+                    const float2 a = spec_ld(ts + r * n2 + j), b = wr[t];
+                    re += a.x * b.x + a.y * b.y;   // conj(a) * b
+                    im += a.x * b.y - a.y * b.x;
+                }
             }
+            // sum over the row's 16 lanes with DPP row shifts (no LDS traffic):
+            // lane 15 of the row ends with the total
 #pragma unroll
             for (int o = 8; o > 0; o >>= 1) {
-                re += __shfl_xor(re, o);
-                im += __shfl_xor(im, o);
+                re += row_shr_f(re, o);
+                im += row_shr_f(im, o);
             }
-            if (sl == 0) bt[(int64_t)k0 * nq + q0 + r] = make_float2(re, im);
+            if (sl == 15) bt[(int64_t)k0 * nq + q0 + r] = make_float2(re, im);
         }
     }
 }
