@@ -490,8 +490,9 @@ __device__ __forceinline__ float row_shr_f(float v, int o)
 // of 16 bytes, a tile starts at a multiple of 16 rows, and the host checks the
 // base alignment); then 16 lanes sum
 // one row (lane l takes j = l + 16 t, conflict-free LDS reads, its 16 image
-// values W held in registers) and reduce with four DPP row shifts.  Results for consecutive q are stored contiguously
-// (k_fft_unpack_bt moves them into the per-q sub-spectra).  Requires n*n <= 256.
+// values W held in registers) and reduce with four DPP row shifts.  Results
+// for consecutive q are stored contiguously (k_fft_unpack_bt moves them into
+// the per-q sub-spectra).  Requires n*n <= 256.
 template <typename GT>
 __global__ __launch_bounds__(256) void k_fft_gemv_bwd(const GT *__restrict__ G, const float2 *__restrict__ grid,
                                                       int nq, int qc, int64_t kstride, int n, int Nx, int Ny, int Nxp,
