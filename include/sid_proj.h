@@ -92,7 +92,8 @@ enum {
 int sid_psf_create_ex(const float *H, const int64_t dims[5], const int32_t *CAindex, int flags,
                       sid_psf **out);
 /* Replace the values of H (same dims; host or device memory per `on_device`)
- * and rebuild every weight copy the handle holds.  The per-plane support
+ * and rebuild every weight copy the handle holds (the sensor plan, with its
+ * FFT spectra, is built again by the next projection).  The per-plane support
  * chosen at creation is kept: entries outside it are ignored. */
 int sid_psf_update(sid_psf *psf, const float *H, int on_device);
 

@@ -146,7 +146,7 @@ static void destroy(sid_psf *p)
     fft_state_free(sp.fft);
     free_plan_buffers(sp);
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
-                    (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
+                    (void *)sp.fwd.d_tmap, (void *)sp.bwd.d_tmap, (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
                     (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag,
                     (void *)p->hbuf[0], (void *)p->hbuf[1]})
@@ -282,6 +282,10 @@ int sid_psf_update(sid_psf *p, const float *H, int on_device)
     free_weights(p);
     for (int w = 0; w < 4; ++w)
         if (had[w] || w == 0) SID_TRY(build_weights(p, w));
+    // the sensor plan holds values derived from H (the FFT planes' spectra, the
+    // GEMM's zero-tile tables): the next projection builds it again
+    fft_state_free(p->plan.fft);
+    free_plan_buffers(p->plan);
     return SID_OK;
 }
 

@@ -69,35 +69,44 @@ __global__ void k_support_scan(const float *__restrict__ H, int64_t M, int64_t n
 // stored chunk-transposed: [K / KC][NP][KC] so one LDS stage is one
 // contiguous copy with K innermost.
 // ---------------------------------------------------------------------------
+// Entry idx of the weight block of plane c in direction dir: [tap row][K / KC][NP][KC],
+// K = taps * Sk (tap ky holds rows ky * Sk .. ky * Sk + n^2 - 1, the rest zero).
+__device__ __forceinline__ float w_value(const float *__restrict__ H, int64_t idx, int64_t M, int n, int c, int dir,
+                                         int lo, int hi, int off, int t_lo, int ntap, int64_t K, int Sk, int NP,
+                                         int KC)
+{
+    const int64_t per_tap = K * NP;
+    const int nn2 = n * n;
+    const int t_hi = t_lo + ntap - 1;
+    const int tx = (int)(idx / per_tap);
+    const int64_t rem = idx - (int64_t)tx * per_tap;
+    const int64_t chunk = rem / ((int64_t)NP * KC);
+    const int64_t r2 = rem - chunk * NP * KC;
+    const int col = (int)(r2 / KC);
+    const int64_t k = chunk * KC + (r2 - (int64_t)col * KC);
+    const int dx = t_lo + tx;
+    const int ky = (int)(k / Sk), kk = (int)(k - (int64_t)ky * Sk);
+    if (ky < ntap && kk < nn2 && col < nn2) {
+        int dy, ab, uv;
+        if (dir == 0) { dy = t_hi - ky; ab = kk; uv = col; }
+        else          { dy = t_lo + ky; uv = kk; ab = col; }
+        const int a = ab % n, b = ab / n, u = uv % n, w = uv / n;
+        const int i = n * dx + u - a + off, j = n * dy + w - b + off;
+        if (i >= lo && i <= hi && j >= lo && j <= hi)
+            return H[(int64_t)i + M * ((int64_t)j + M * ((int64_t)a + n * ((int64_t)b + (int64_t)n * c)))];
+    }
+    return 0.0f;
+}
+
 template <int PART>  // 0: fp32, 1: bf16 hi, 2: bf16 lo, 3: fp64
 __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, int64_t M, int n,
                           int c, int dir, int lo, int hi, int off, int t_lo, int ntap, int64_t K,
                           int S, int NP, int KC)
 {
-    const int64_t per_tap = K * NP;
-    const int64_t total = (int64_t)ntap * per_tap;
-    const int nn2 = n * n;
-    const int t_hi = t_lo + ntap - 1;
+    const int64_t total = (int64_t)ntap * K * NP;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int tx = (int)(idx / per_tap);
-        const int64_t rem = idx - (int64_t)tx * per_tap;
-        const int64_t chunk = rem / ((int64_t)NP * KC);
-        const int64_t r2 = rem - chunk * NP * KC;
-        const int col = (int)(r2 / KC);
-        const int64_t k = chunk * KC + (r2 - (int64_t)col * KC);
-        const int dx = t_lo + tx;
-        float v = 0.0f;
-        const int ky = (int)(k / S), kk = (int)(k - (int64_t)ky * S);
-        if (ky < ntap && kk < nn2 && col < nn2) {
-            int dy, ab, uv;
-            if (dir == 0) { dy = t_hi - ky; ab = kk; uv = col; }
-            else          { dy = t_lo + ky; uv = kk; ab = col; }
-            const int a = ab % n, b = ab / n, u = uv % n, w = uv / n;
-            const int i = n * dx + u - a + off, j = n * dy + w - b + off;
-            if (i >= lo && i <= hi && j >= lo && j <= hi)
-                v = H[(int64_t)i + M * ((int64_t)j + M * ((int64_t)a + n * ((int64_t)b + (int64_t)n * c)))];
-        }
+        const float v = w_value(H, idx, M, n, c, dir, lo, hi, off, t_lo, ntap, K, S, NP, KC);
         if (PART == 0) {
             ((float *)Wout)[idx] = v;
         } else if (PART == 3) {
@@ -107,6 +116,24 @@ __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, 
             ((u16 *)Wout)[idx] = PART == 1 ? h : f2bf(v - bf2f(h));
         }
     }
+}
+
+// Zero tiles of the BF16 layout: one wave per 32 x 16 tile (512 consecutive
+// entries: 16 columns of a 32-row chunk); flag[t] = 1 when any entry is
+// nonzero.  Both BF16 parts of an entry are zero exactly when its FP32 value is.
+__global__ void k_tile_flags(const float *__restrict__ H, uint8_t *__restrict__ flag, int64_t ntiles, int64_t M,
+                             int n, int c, int dir, int lo, int hi, int off, int t_lo, int ntap, int64_t K, int Sk,
+                             int NP)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (t >= ntiles) return;   // whole waves
+    bool nz = false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        nz = nz || w_value(H, t * 512 + lane * 8 + i, M, n, c, dir, lo, hi, off, t_lo, ntap, K, Sk, NP, 32) != 0.0f;
+    const unsigned long long b = __ballot(nz);
+    if (lane == 0) flag[t] = b != 0 ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -237,7 +264,10 @@ __global__ __launch_bounds__(NTHREADS, 2) void k_proj_gemm(const Item *__restric
 // use.  Each tap is padded to KT = 32 * ceil(S / 32) in the weights (rows
 // beyond S are zero), so the fragments past S read the next lenslet's values
 // and multiply zeros.  B chunks [NP][32] are double buffered in LDS through
-// registers, one barrier per step.
+// registers, one barrier per step.  Chunk kc runs over the taps of the
+// segment's tap table only (the hull of its nonzero weight tiles): at the edge
+// taps of a plane whole chunks of weights are zero, and skipping them changes
+// no sum of finite values.
 // ---------------------------------------------------------------------------
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 
@@ -300,9 +330,9 @@ struct SlShape {
 template <int NT, int RT, int MODE>
 __device__ __forceinline__ void sl_item(
     const Item it, u16 (&Bs)[2][SlShape<NT, MODE>::NB][SlShape<NT, MODE>::NP * SlShape<NT, MODE>::LDK],
-    const Seg *__restrict__ segs, const u16 *__restrict__ A0, const u16 *__restrict__ A1,
-    const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part, int nrows, int Ly, int YP,
-    int S, int KT, int64_t slab_stride)
+    const Seg *__restrict__ segs, const uint32_t *__restrict__ tmap, const u16 *__restrict__ A0,
+    const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
+    int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
 {
     using SH = SlShape<NT, MODE>;
     constexpr int NP = SH::NP;
@@ -338,36 +368,43 @@ __device__ __forceinline__ void sl_item(
         const Seg sg = segs[s];
         const u16 *Ah = A0 + sg.a_off, *Al = A1 + sg.a_off;
         const u16 *Wh = W0 + sg.w_off2, *Wl = W1 + sg.w_off2;
-        const int ntap = sg.K2 / KT, nkc = KT / KW;
+        const int nkc = KT / KW;
+        const uint32_t *tab = tmap + sg.tmap;
         for (int kc = 0; kc < nkc; ++kc) {
+            // taps [ky0, ky1) hold every nonzero weight tile of this chunk (uniform across the block)
+            const uint32_t hull = tab[kc];
+            const int ky0 = (int)(hull & 0xffffu), ky1 = (int)(hull >> 16);
+            if (ky0 >= ky1) continue;
             // step t = ky: B chunk (ky, kc) at W + (ky * nkc + kc) * NP * KW
             u32x4 pre[NB][VPT];
-            // A fragments for tap 0 (all lanes) and tap 1 (slid + fresh lanes)
+            // A fragments for tap ky0 (all lanes) and tap ky0 + 1 (slid + fresh lanes)
             bf8 ah[RT], al[RT], nh[RT], nl[RT];
             const int ko = kc * KW + 8 * g;
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
-                ah[rt] = *(const bf8 *)(Ah + rowoff[rt] + ko);
-                al[rt] = *(const bf8 *)(Al + rowoff[rt] + ko);
+                const int64_t o = rowoff[rt] + (int64_t)ky0 * S + ko;
+                ah[rt] = *(const bf8 *)(Ah + o);
+                al[rt] = *(const bf8 *)(Al + o);
             }
-            sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)0 * nkc + kc) * NP * KW);
+            sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)ky0 * nkc + kc) * NP * KW);
             __syncthreads();   // previous chunk's buffers consumed
             sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[0][0][0]);
-            if (ntap > 1) {
+            if (ky0 + 1 < ky1) {
 #pragma unroll
                 for (int rt = 0; rt < RT; ++rt) {
                     nh[rt] = dpp_row_shl1(ah[rt]);
                     nl[rt] = dpp_row_shl1(al[rt]);
                     if (fresh[rt]) {
-                        nh[rt] = *(const bf8 *)(Ah + rowoff[rt] + S + ko);
-                        nl[rt] = *(const bf8 *)(Al + rowoff[rt] + S + ko);
+                        const int64_t o = rowoff[rt] + (int64_t)(ky0 + 1) * S + ko;
+                        nh[rt] = *(const bf8 *)(Ah + o);
+                        nl[rt] = *(const bf8 *)(Al + o);
                     }
                 }
-                sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)1 * nkc + kc) * NP * KW);
+                sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)(ky0 + 1) * nkc + kc) * NP * KW);
             }
             __syncthreads();
-            for (int ky = 0; ky < ntap; ++ky) {
-                const int buf = ky & 1;
+            for (int ky = ky0; ky < ky1; ++ky) {
+                const int buf = (ky - ky0) & 1;
                 {
                     // B fragments through a ring of kBR register slots (NB fragments each):
                     // fragment ct + kBR is read right after the MFMAs of fragment ct are
@@ -402,14 +439,14 @@ __device__ __forceinline__ void sl_item(
                         }
                     }
                 }
-                if (ky + 1 < ntap) {
+                if (ky + 1 < ky1) {
                     sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[buf ^ 1][0][0]);   // B of tap ky + 1
 #pragma unroll
                     for (int rt = 0; rt < RT; ++rt) {
                         ah[rt] = nh[rt];
                         al[rt] = nl[rt];
                     }
-                    if (ky + 2 < ntap) {   // A of tap ky + 2 and B of tap ky + 2
+                    if (ky + 2 < ky1) {   // A of tap ky + 2 and B of tap ky + 2
 #pragma unroll
                         for (int rt = 0; rt < RT; ++rt) {
                             nh[rt] = dpp_row_shl1(ah[rt]);
@@ -444,15 +481,16 @@ __device__ __forceinline__ void sl_item(
 // running beside it on another stream.
 template <int NT, int RT, int MODE>
 __global__ __launch_bounds__(NTHREADS, (MODE == 2 || RT > 2) ? 1 : 2) void k_proj_gemm_sl(
-    const Item *__restrict__ items, int nitems, const Seg *__restrict__ segs, const u16 *__restrict__ A0,
-    const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
-    int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
+    const Item *__restrict__ items, int nitems, const Seg *__restrict__ segs, const uint32_t *__restrict__ tmap,
+    const u16 *__restrict__ A0, const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1,
+    float *__restrict__ part, int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
 {
     using SH = SlShape<NT, MODE>;
     __shared__ u16 Bs[2][SH::NB][SH::NP * SH::LDK];
     for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
         const Item it = items[i];
-        if (it.mtile >= 0) sl_item<NT, RT, MODE>(it, Bs, segs, A0, A1, W0, W1, part, nrows, Ly, YP, S, KT, slab_stride);
+        if (it.mtile >= 0)
+            sl_item<NT, RT, MODE>(it, Bs, segs, tmap, A0, A1, W0, W1, part, nrows, Ly, YP, S, KT, slab_stride);
     }
 }
 
@@ -759,6 +797,41 @@ void free_weights(sid_psf *p)
     }
 }
 
+// Zero-tile flags of both BF16 weight layouts (p->tflag_f / tflag_b), from H.
+static int scan_tiles(sid_psf *p)
+{
+    const int n = (int)p->n;
+    p->tflag_f.assign((size_t)(p->wf2_elems / 512), 0);
+    p->tflag_b.assign((size_t)(p->wb2_elems / 512), 0);
+    uint8_t *d_flag = nullptr;
+    const int64_t nmax = std::max<int64_t>(std::max(p->wf2_elems, p->wb2_elems) / 512, 1);
+    SID_TRY_HIP(hipMalloc(&d_flag, nmax));
+    int rc = SID_OK;
+    for (int dir = 0; dir < 2 && rc == SID_OK; ++dir) {
+        for (int64_t c = 0; c < p->nzH; ++c) {
+            const Plane &pl = p->planes[c];
+            if (pl.empty) continue;
+            const int t_lo = dir == 0 ? pl.tf_lo : pl.tb_lo;
+            const int ntap = (dir == 0 ? pl.tf_hi : pl.tb_hi) - t_lo + 1;
+            const int64_t K = dir == 0 ? pl.Kf2 : pl.Kb2;
+            const int64_t t0 = (dir == 0 ? pl.wf2_off : pl.wb2_off) / 512;
+            const int64_t nt = (int64_t)ntap * K * p->NP / 512;
+            hipLaunchKernelGGL(k_tile_flags, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, p->stream, p->H,
+                               d_flag + t0, nt, p->M, n, (int)c, dir, pl.lo, pl.hi, dir == 0 ? pl.off_f : pl.off_b,
+                               t_lo, ntap, K, (int)p->KT, (int)p->NP);
+            if (hipGetLastError() != hipSuccess) { rc = SID_ERR_HIP; break; }
+        }
+        std::vector<uint8_t> &f = dir == 0 ? p->tflag_f : p->tflag_b;
+        if (rc == SID_OK && !f.empty() &&
+            (hipMemcpyAsync(f.data(), d_flag, f.size(), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+             hipStreamSynchronize(p->stream) != hipSuccess))
+            rc = SID_ERR_HIP;
+    }
+    (void)hipFree(d_flag);
+    if (rc != SID_OK) set_error("zero-tile scan of the weights failed");
+    return rc;
+}
+
 int build_weights(sid_psf *p, int which)
 {
     const int n = (int)p->n;
@@ -842,6 +915,7 @@ int build_weights(sid_psf *p, int which)
         }
     }
     SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    if (which == 0) SID_TRY(scan_tiles(p));
     return SID_OK;
 }
 
@@ -850,19 +924,18 @@ int build_weights(sid_psf *p, int which)
 // and 16 % slower beside the FFT path than RT = 2.
 static int row_tiles_for(int /*precision*/) { return 2; }
 
-// Split a list of segments (all costs = K) into `nsplit` contiguous groups of
-// near-equal total K.
-static std::vector<std::pair<int, int>> split_segments(const std::vector<Seg> &segs, int s0, int s1, int nsplit)
+// Split a list of segments into `nsplit` contiguous groups of near-equal total cost.
+static std::vector<std::pair<int, int>> split_segments(const std::vector<int64_t> &cost, int s0, int s1, int nsplit)
 {
     std::vector<std::pair<int, int>> out;
     int64_t total = 0;
-    for (int s = s0; s < s1; ++s) total += segs[s].K;
+    for (int s = s0; s < s1; ++s) total += cost[s];
     if (s1 <= s0) return out;
     nsplit = std::max(1, std::min(nsplit, s1 - s0));
     int start = s0;
     int64_t acc = 0;
     for (int s = s0; s < s1; ++s) {
-        acc += segs[s].K;
+        acc += cost[s];
         const int g = (int)out.size();
         const int64_t target = total * (g + 1) / nsplit;
         const int remaining_groups = nsplit - g - 1;
@@ -910,11 +983,45 @@ static std::vector<Item> xcd_order(const std::vector<std::vector<Item>> &groups,
     return out;
 }
 
+// Tap table of a BF16 segment, appended to tab (returns its offset): word kc
+// (< nkc) holds ky0 | ky1 << 16, the hull [ky0, ky1) of the taps whose 32-row
+// chunk kc has a nonzero weight tile (0 when none has).  The GEMM runs chunk kc
+// over these taps only.  Skipping single 16-column tiles inside a tap as well
+// (a uniform branch per fragment) measured 10 % slower on 1005^2 x 51 planes:
+// the branches break up the interleaving of LDS reads and MFMAs.
+static int32_t seg_table(std::vector<uint32_t> &tab, const std::vector<uint8_t> &flag, int64_t w_off2, int ntap,
+                         int nkc, int NT)
+{
+    const size_t at = tab.size();
+    tab.resize(at + nkc, 0);
+    const int64_t t0 = w_off2 / 512;
+    for (int kc = 0; kc < nkc; ++kc) {
+        int ky0 = ntap, ky1 = 0;
+        for (int ky = 0; ky < ntap; ++ky) {
+            bool any = false;
+            for (int ct = 0; ct < NT && !any; ++ct) {
+                const int64_t t = t0 + (int64_t)(ky * nkc + kc) * NT + ct;
+                any = t >= (int64_t)flag.size() || flag[t];   // unknown: computed
+            }
+            if (any) {
+                ky0 = std::min(ky0, ky);
+                ky1 = ky + 1;
+            }
+        }
+        tab[at + kc] = ky0 < ky1 ? (uint32_t)ky0 | (uint32_t)ky1 << 16 : 0u;
+    }
+    return (int32_t)at;
+}
+
 static int upload_plan(GemmPlan &g)
 {
     if (g.d_segs) { (void)hipFree(g.d_segs); g.d_segs = nullptr; }
     if (g.d_items) { (void)hipFree(g.d_items); g.d_items = nullptr; }
     if (g.d_plane_slab0) { (void)hipFree(g.d_plane_slab0); g.d_plane_slab0 = nullptr; }
+    if (g.d_tmap) { (void)hipFree(g.d_tmap); g.d_tmap = nullptr; }
+    SID_TRY_HIP(hipMalloc(&g.d_tmap, std::max<size_t>(g.tmap.size(), 1) * sizeof(uint32_t)));
+    if (!g.tmap.empty())
+        SID_TRY_HIP(hipMemcpy(g.d_tmap, g.tmap.data(), g.tmap.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     SID_TRY_HIP(hipMalloc(&g.d_segs, std::max<size_t>(g.segs.size(), 1) * sizeof(Seg)));
     SID_TRY_HIP(hipMalloc(&g.d_items, std::max<size_t>(g.items.size(), 1) * sizeof(Item)));
     if (!g.segs.empty())
@@ -959,9 +1066,13 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     }
     // the FFT state stores FP16 spectra in the BF16X2 mode and FP32 otherwise
     const bool fft_ok = !sp.fft.active || sp.fft.half == (p->precision == SID_PREC_BF16X2);
-    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol && use == sp.fft_use && fft_ok)
+    // the GEMM work split follows the cost of the precision's GEMM
+    const int cost_bf16 = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
+    if (sp.X == X && sp.Y == Y && sp.nz == nz && sp.mt == mt && sp.vol && use == sp.fft_use && fft_ok &&
+        sp.cost_bf16 == cost_bf16)
         return ensure_precision_buffers(p);
     free_plan_buffers(sp);
+    sp.cost_bf16 = cost_bf16;
     fft_state_free(sp.fft);
     sp.fft_use = use;
     const int64_t n = p->n, S = p->S, NP = p->NP, T = p->T;
@@ -985,10 +1096,21 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     int per_cu = any_fft ? 4 : 8;
     if (const char *e = std::getenv("SID_SPLIT_PER_CU")) per_cu = std::max(1, std::atoi(e));
     const int target_blocks = per_cu * dev_cus;
+    // Segment cost: K for the FP32 / FP64 GEMMs, the K rows inside the tap hulls for the BF16 GEMM.
+    const int nkc = (int)(p->KT / 32);
+    auto seg_cost = [&](const GemmPlan &g, const Seg &sg) -> int64_t {
+        if (!cost_bf16) return sg.K;
+        int64_t k = 0;
+        for (int kc = 0; kc < nkc; ++kc) {
+            const uint32_t h = g.tmap[sg.tmap + kc];
+            k += 32 * (int64_t)((int)(h >> 16) - (int)(h & 0xffffu));
+        }
+        return k;
+    };
 
     // ---- forward: segments over planes 0..nz-1 and their dx rows
     GemmPlan &F = sp.fwd;
-    F.segs.clear(); F.items.clear(); F.macs = 0;
+    F.segs.clear(); F.items.clear(); F.tmap.clear(); F.cost.clear(); F.macs = 0;
     for (int64_t c = 0; c < nz; ++c) {
         const Plane &pl = p->planes[c];
         if (pl.empty || use[c]) continue;
@@ -1002,21 +1124,22 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             sg.plane = (int32_t)c;
             sg.w_off2 = pl.wf2_off + (int64_t)tx * pl.Kf2 * NP;
             sg.K2 = (int32_t)pl.Kf2;
-            sg.pad_ = 0;
+            sg.tmap = seg_table(F.tmap, p->tflag_f, sg.w_off2, ntap, nkc, (int)(NP / 16));
             F.segs.push_back(sg);
+            F.cost.push_back(seg_cost(F, sg));
         }
     }
     {
         // a multiple of the XCD count of equal-cost groups balances the XCDs
         int nsplit = (int)std::max<int64_t>(1, (target_blocks + sp.ntiles - 1) / sp.ntiles);
         nsplit = (nsplit + kXcds - 1) / kXcds * kXcds;
-        auto groups = split_segments(F.segs, 0, (int)F.segs.size(), nsplit);
+        auto groups = split_segments(F.cost, 0, (int)F.segs.size(), nsplit);
         F.nslabs = (int)groups.size();
         std::vector<std::vector<Item>> gi_items(groups.size());
         std::vector<int64_t> gi_cost(groups.size(), 0);
         for (int gi = 0; gi < (int)groups.size(); ++gi) {
             for (int t = 0; t < sp.ntiles; ++t) gi_items[gi].push_back(Item{groups[gi].first, groups[gi].second, t, gi});
-            for (int s = groups[gi].first; s < groups[gi].second; ++s) gi_cost[gi] += F.segs[s].K;
+            for (int s = groups[gi].first; s < groups[gi].second; ++s) gi_cost[gi] += F.cost[s];
         }
         F.items = xcd_order(gi_items, gi_cost);
         for (auto &sg : F.segs) F.macs += (int64_t)sg.K * NP * sp.mpad;
@@ -1024,7 +1147,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
 
     // ---- backward: every plane of H, split per plane in proportion to cost
     GemmPlan &B = sp.bwd;
-    B.segs.clear(); B.items.clear(); B.plane_slab0.assign(p->nzH + 1, 0); B.macs = 0;
+    B.segs.clear(); B.items.clear(); B.tmap.clear(); B.cost.clear(); B.plane_slab0.assign(p->nzH + 1, 0); B.macs = 0;
     std::vector<int> pseg0(p->nzH + 1, 0);
     int64_t btotal = 0;
     for (int64_t c = 0; c < p->nzH; ++c) {
@@ -1041,9 +1164,10 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             sg.plane = (int32_t)c;
             sg.w_off2 = pl.wb2_off + (int64_t)tx * pl.Kb2 * NP;
             sg.K2 = (int32_t)pl.Kb2;
-            sg.pad_ = 0;
+            sg.tmap = seg_table(B.tmap, p->tflag_b, sg.w_off2, ntap, nkc, (int)(NP / 16));
             B.segs.push_back(sg);
-            btotal += sg.K;
+            B.cost.push_back(seg_cost(B, sg));
+            btotal += B.cost.back();
         }
     }
     pseg0[p->nzH] = (int)B.segs.size();
@@ -1055,13 +1179,13 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         for (int64_t c = 0; c < p->nzH; ++c) {
             B.plane_slab0[c] = slab;
             int64_t cost = 0;
-            for (int s = pseg0[c]; s < pseg0[c + 1]; ++s) cost += B.segs[s].K;
+            for (int s = pseg0[c]; s < pseg0[c + 1]; ++s) cost += B.cost[s];
             if (cost == 0) continue;
             const int nsplit = (int)std::max<int64_t>(1, (cost + per_split - 1) / per_split);
-            auto groups = split_segments(B.segs, pseg0[c], pseg0[c + 1], nsplit);
+            auto groups = split_segments(B.cost, pseg0[c], pseg0[c + 1], nsplit);
             for (auto &gr : groups) {
                 int64_t gc = 0;
-                for (int s = gr.first; s < gr.second; ++s) gc += B.segs[s].K;
+                for (int s = gr.first; s < gr.second; ++s) gc += B.cost[s];
                 gr_items.emplace_back();
                 for (int t = 0; t < sp.ntiles; ++t) gr_items.back().push_back(Item{gr.first, gr.second, t, slab});
                 gr_cost.push_back(gc);
@@ -1151,11 +1275,11 @@ static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st, int
         const u16 *W1 = p->precision == SID_PREC_BF16X3 ? (fwd ? p->wf_lo : p->wb_lo) : W0;
         if (p->precision == SID_PREC_BF16X2)
             hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 1>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems,
-                               g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               g.d_segs, g.d_tmap, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
                                (int)p->KT, slab);
         else
             hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 2>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems,
-                               g.d_segs, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               g.d_segs, g.d_tmap, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
                                (int)p->KT, slab);
     }
     SID_TRY_HIP(hipGetLastError());

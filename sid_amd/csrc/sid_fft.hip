@@ -1381,9 +1381,10 @@ int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t s
     double macs_all = 0;
     if (rc == SID_OK) {
         t_dir = time_ms(st, 2, [&]() { return launch_gemm(p, 0, st); }, &rc);
-        for (const Seg &sg : p->plan.fwd.segs) {
-            const double m = (double)sg.K * p->NP * p->plan.mpad;
-            macs[sg.plane] += m;
+        const GemmPlan &F = p->plan.fwd;
+        for (size_t s = 0; s < F.segs.size(); ++s) {
+            const double m = (double)F.cost[s] * p->NP * p->plan.mpad;
+            macs[F.segs[s].plane] += m;
             macs_all += m;
         }
     }

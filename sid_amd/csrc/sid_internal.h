@@ -59,7 +59,7 @@ struct Seg {
     int32_t plane;
     int64_t w_off2;    // BF16 weights: K2 = taps * KT (each tap padded to KT = S rounded up to 32)
     int32_t K2;
-    int32_t pad_;
+    int32_t tmap;      // BF16: offset (32-bit words) of the segment's zero-tile table
 };
 
 // One block of work: segments [seg0, seg1) for M tile `mtile`, written to
@@ -92,6 +92,9 @@ struct GemmPlan {             // one direction on one sensor geometry
     Seg *d_segs = nullptr;
     Item *d_items = nullptr;
     int *d_plane_slab0 = nullptr;
+    std::vector<uint32_t> tmap;       // zero-tile tables of the BF16 segments (Seg::tmap)
+    std::vector<int64_t> cost;        // per segment: K rows the GEMM runs at the plan's precision
+    uint32_t *d_tmap = nullptr;
     int nslabs = 0;
     int64_t macs = 0;                 // MFMA multiply-adds issued (padding included)
 };
@@ -137,6 +140,7 @@ struct SensorPlan {
     int64_t Lx = 0, Ly = 0, XP = 0, YP = 0;
     int64_t nrows = 0, ntiles = 0, mpad = 0, mt = 0;
     GemmPlan fwd, bwd;
+    int cost_bf16 = -1;               // segment costs of the BF16 GEMM (tap hulls) or of K
     // device buffers (fp32 master + bf16 hi/lo shadows)
     float *vol = nullptr, *img = nullptr, *htf = nullptr, *part = nullptr;
     u16 *vol_hi = nullptr, *vol_lo = nullptr, *img_hi = nullptr, *img_lo = nullptr;
@@ -165,6 +169,9 @@ struct sid_psf {
     int64_t wf_elems = 0, wb_elems = 0;
     int64_t wf2_elems = 0, wb2_elems = 0;   // BF16 weights (tap-padded layout)
     int64_t KT = 0;                         // BF16 K per tap (S rounded up to 32)
+    // per 32 x 16 tile of the BF16 weight layout (512 consecutive entries):
+    // 1 when an entry is nonzero (forward / backward)
+    std::vector<uint8_t> tflag_f, tflag_b;
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
     std::vector<char> pin;            // per plane: 0 autotuned, SID_METHOD_DIRECT / _FFT pinned by the caller
