@@ -461,16 +461,20 @@ def main():
         except (OSError, ValueError) as e:
             log(f"traffic summary unreadable ({e}); roofline.traffic = null")
             tj = None
+    traffic_scale = None
     if tj is not None:
-        # the counters belong to one plan: same precision, method, workload and spectra size
-        same_plan = abs(float(tj.get("spectra_gb", -1)) - hr["spectra_gb"]) < 0.01
-        if not (tj.get("precision") == head and tj.get("method") == args.method and wl.name == tj.get("workload")
-                and same_plan):
+        # the counters belong to one plan: same precision, method and workload; a plan that
+        # puts another number of planes on the FFT path streams spectra of another size, and
+        # the GEMVs' counted bytes are scaled by the ratio of the spectra sizes (each GEMV
+        # streams its spectra once, so its traffic is proportional to them)
+        if not (tj.get("precision") == head and tj.get("method") == args.method and wl.name == tj.get("workload")):
             tj = None
+        elif abs(float(tj.get("spectra_gb", -1)) - hr["spectra_gb"]) >= 0.01:
+            traffic_scale = hr["spectra_gb"] / float(tj["spectra_gb"])
     if t_gemm >= t_fftp:
         gk = {"fp32": "sid::k_proj_gemm<15, 2>", "bf16x2": "sid::k_proj_gemm_sl<15, 2, 1>",
               "bf16x3": "sid::k_proj_gemm_sl<15, 2, 2>"}.get(head, "")
-        kk = tj.get("kernels", {}) if tj else {}
+        kk = tj.get("kernels", {}) if tj and traffic_scale is None else {}
         traffic = (kk[gk]["read_bytes"] + kk[gk]["write_bytes"]) if gk in kk else None
         roof = {"bound": "mfma", "achieved": round(hr["gemm_achieved_tflops"], 3), "peak": peak / 1e12,
                 "unit": "TFLOP/s", "frac": round(hr["gemm_frac_of_peak"], 4), "traffic": traffic,
@@ -485,9 +489,12 @@ def main():
             names = [f"sid::k_fft_gemv_{d}{inst}" for d in ("fwd", "bwd")]
             if all(k in kk for k in names):
                 traffic = 0.5 * sum(kk[k]["read_bytes"] + kk[k]["write_bytes"] for k in names)
+                if traffic_scale is not None:
+                    traffic *= traffic_scale
         roof = {"bound": "hbm", "achieved": round(hr["gemv_stream_tbps"] * 1e3, 1), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(hr["gemv_frac_of_hbm"], 4),
                 "traffic": traffic, "traffic_source": "profiles/pmc_traffic_rl_full_bf16x2.md" if traffic else None,
+                "traffic_scaled_from_spectra_gb": float(tj["spectra_gb"]) if traffic and traffic_scale else None,
                 "kernel": "k_fft_gemv_fwd / k_fft_gemv_bwd (spectra GEMVs of the FFT planes)",
                 "algorithmic_per_launch": f"{hr['spectra_gb']:.2f} GB of spectra streamed once per launch"}
     out = {
