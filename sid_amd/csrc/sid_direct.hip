@@ -118,10 +118,11 @@ __global__ void k_build_w(const float *__restrict__ H, void *__restrict__ Wout, 
     }
 }
 
-// Zero tiles of the BF16 layout: one wave per 32 x 16 tile (512 consecutive
-// entries: 16 columns of a 32-row chunk); flag[t] = 1 when any entry is
-// nonzero.  Both BF16 parts of an entry are zero exactly when its FP32 value is.
-__global__ void k_tile_flags(const float *__restrict__ H, uint8_t *__restrict__ flag, int64_t ntiles, int64_t M,
+// Zero tiles of a weight layout: one wave per tile of E consecutive entries
+// (BF16 layout: E = 512, 16 columns of a 32-row chunk; FP32 layout: E = NP * 32,
+// a whole 32-row chunk); flag[t] = 1 when any entry is nonzero.  Both BF16 parts
+// of an entry are zero exactly when its FP32 value is.
+__global__ void k_tile_flags(const float *__restrict__ H, uint8_t *__restrict__ flag, int64_t ntiles, int E, int64_t M,
                              int n, int c, int dir, int lo, int hi, int off, int t_lo, int ntap, int64_t K, int Sk,
                              int NP)
 {
@@ -129,9 +130,8 @@ __global__ void k_tile_flags(const float *__restrict__ H, uint8_t *__restrict__ 
     const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (t >= ntiles) return;   // whole waves
     bool nz = false;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-        nz = nz || w_value(H, t * 512 + lane * 8 + i, M, n, c, dir, lo, hi, off, t_lo, ntap, K, Sk, NP, 32) != 0.0f;
+    for (int e = lane; e < E && !nz; e += 64)
+        nz = w_value(H, t * E + e, M, n, c, dir, lo, hi, off, t_lo, ntap, K, Sk, NP, 32) != 0.0f;
     const unsigned long long b = __ballot(nz);
     if (lane == 0) flag[t] = b != 0 ? 1 : 0;
 }
@@ -140,7 +140,8 @@ __global__ void k_tile_flags(const float *__restrict__ H, uint8_t *__restrict__ 
 // The FP32 projector GEMM.  256 threads = 4 waves; each wave owns RT*16 rows of
 // the M tile and all NT*16 columns.  B chunks [NP][32] are staged in LDS and
 // shared by the 4 waves; A is read straight from global memory (L2-resident
-// rows, each row a contiguous Toeplitz window).
+// rows, each row a contiguous Toeplitz window).  Chunks whose weights are all
+// zero (marked in the segment's zero-tile table) are skipped.
 //
 // v_mfma_f32_16x16x4_f32: lane (l16, g) = (lane & 15, lane >> 4) feeds
 // A[row l16][k = 8g + s] and B[k = 8g + s][col l16] at step s = 0..7 of a
@@ -158,9 +159,10 @@ struct F32Shape {
 // previous item's last LDS reads.
 template <int NT, int RT>
 __device__ __forceinline__ void f32_item(const Item it, float (&Bs)[F32Shape<NT>::NP * F32Shape<NT>::LDK],
-                                         const Seg *__restrict__ segs, const float *__restrict__ A0,
-                                         const float *__restrict__ W0, float *__restrict__ part, int nrows, int Ly,
-                                         int YP, int S, int64_t slab_stride)
+                                         const Seg *__restrict__ segs, const uint32_t *__restrict__ tmap, int nkc,
+                                         const float *__restrict__ A0, const float *__restrict__ W0,
+                                         float *__restrict__ part, int nrows, int Ly, int YP, int S,
+                                         int64_t slab_stride)
 {
     constexpr int NP = F32Shape<NT>::NP;
     constexpr int MT = 4 * RT * 16;
@@ -187,7 +189,10 @@ __device__ __forceinline__ void f32_item(const Item it, float (&Bs)[F32Shape<NT>
         const Seg sg = segs[s];
         const float *A = A0 + sg.a_off;
         const float *W = W0 + sg.w_off;
+        const uint32_t *live = tmap + sg.tmap + nkc;   // nonzero 32-row chunks, one bit each
         for (int k0 = 0; k0 < sg.K; k0 += KC) {
+            const int j = k0 / KC;
+            if (!((live[j >> 5] >> (j & 31)) & 1u)) continue;   // all-zero weights (uniform)
             float4 a0[RT], a1[RT];
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
@@ -239,14 +244,17 @@ __device__ __forceinline__ void f32_item(const Item it, float (&Bs)[F32Shape<NT>
 // running beside it on another stream.
 template <int NT, int RT>
 __global__ __launch_bounds__(NTHREADS, 2) void k_proj_gemm(const Item *__restrict__ items, int nitems,
-                                                        const Seg *__restrict__ segs, const float *__restrict__ A0,
-                                                        const float *__restrict__ W0, float *__restrict__ part,
-                                                        int nrows, int Ly, int YP, int S, int64_t slab_stride)
+                                                        const Seg *__restrict__ segs,
+                                                        const uint32_t *__restrict__ tmap, int nkc,
+                                                        const float *__restrict__ A0, const float *__restrict__ W0,
+                                                        float *__restrict__ part, int nrows, int Ly, int YP, int S,
+                                                        int64_t slab_stride)
 {
     __shared__ float Bs[F32Shape<NT>::NP * F32Shape<NT>::LDK];
     for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
         const Item it = items[i];
-        if (it.mtile >= 0) f32_item<NT, RT>(it, Bs, segs, A0, W0, part, nrows, Ly, YP, S, slab_stride);
+        if (it.mtile >= 0)
+            f32_item<NT, RT>(it, Bs, segs, tmap, nkc, A0, W0, part, nrows, Ly, YP, S, slab_stride);
     }
 }
 
@@ -797,31 +805,39 @@ void free_weights(sid_psf *p)
     }
 }
 
-// Zero-tile flags of both BF16 weight layouts (p->tflag_f / tflag_b), from H.
+// Zero-tile flags, from H: p->tflag_f / tflag_b for the 32 x 16 tiles of the
+// BF16 layouts, p->cflag_f / cflag_b for the 32-row chunks of the FP32 layouts.
 static int scan_tiles(sid_psf *p)
 {
     const int n = (int)p->n;
+    const int64_t EC = p->NP * 32;
     p->tflag_f.assign((size_t)(p->wf2_elems / 512), 0);
     p->tflag_b.assign((size_t)(p->wb2_elems / 512), 0);
+    p->cflag_f.assign((size_t)(p->wf_elems / EC), 0);
+    p->cflag_b.assign((size_t)(p->wb_elems / EC), 0);
     uint8_t *d_flag = nullptr;
-    const int64_t nmax = std::max<int64_t>(std::max(p->wf2_elems, p->wb2_elems) / 512, 1);
+    const int64_t nmax = std::max<int64_t>({(int64_t)p->tflag_f.size(), (int64_t)p->tflag_b.size(),
+                                            (int64_t)p->cflag_f.size(), (int64_t)p->cflag_b.size(), (int64_t)1});
     SID_TRY_HIP(hipMalloc(&d_flag, nmax));
     int rc = SID_OK;
-    for (int dir = 0; dir < 2 && rc == SID_OK; ++dir) {
+    for (int lay = 0; lay < 4 && rc == SID_OK; ++lay) {   // BF16 fwd, BF16 bwd, FP32 fwd, FP32 bwd
+        const int dir = lay & 1;
+        const bool bf = lay < 2;
+        const int64_t E = bf ? 512 : EC;
         for (int64_t c = 0; c < p->nzH; ++c) {
             const Plane &pl = p->planes[c];
             if (pl.empty) continue;
             const int t_lo = dir == 0 ? pl.tf_lo : pl.tb_lo;
             const int ntap = (dir == 0 ? pl.tf_hi : pl.tb_hi) - t_lo + 1;
-            const int64_t K = dir == 0 ? pl.Kf2 : pl.Kb2;
-            const int64_t t0 = (dir == 0 ? pl.wf2_off : pl.wb2_off) / 512;
-            const int64_t nt = (int64_t)ntap * K * p->NP / 512;
+            const int64_t K = bf ? (dir == 0 ? pl.Kf2 : pl.Kb2) : (dir == 0 ? pl.Kf : pl.Kb);
+            const int64_t w0 = bf ? (dir == 0 ? pl.wf2_off : pl.wb2_off) : (dir == 0 ? pl.wf_off : pl.wb_off);
+            const int64_t nt = (int64_t)ntap * K * p->NP / E;
             hipLaunchKernelGGL(k_tile_flags, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, p->stream, p->H,
-                               d_flag + t0, nt, p->M, n, (int)c, dir, pl.lo, pl.hi, dir == 0 ? pl.off_f : pl.off_b,
-                               t_lo, ntap, K, (int)p->KT, (int)p->NP);
+                               d_flag + w0 / E, nt, (int)E, p->M, n, (int)c, dir, pl.lo, pl.hi,
+                               dir == 0 ? pl.off_f : pl.off_b, t_lo, ntap, K, bf ? (int)p->KT : (int)p->S, (int)p->NP);
             if (hipGetLastError() != hipSuccess) { rc = SID_ERR_HIP; break; }
         }
-        std::vector<uint8_t> &f = dir == 0 ? p->tflag_f : p->tflag_b;
+        std::vector<uint8_t> &f = lay == 0 ? p->tflag_f : lay == 1 ? p->tflag_b : lay == 2 ? p->cflag_f : p->cflag_b;
         if (rc == SID_OK && !f.empty() &&
             (hipMemcpyAsync(f.data(), d_flag, f.size(), hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
              hipStreamSynchronize(p->stream) != hipSuccess))
@@ -983,25 +999,29 @@ static std::vector<Item> xcd_order(const std::vector<std::vector<Item>> &groups,
     return out;
 }
 
-// Tap table of a BF16 segment, appended to tab (returns its offset): word kc
-// (< nkc) holds ky0 | ky1 << 16, the hull [ky0, ky1) of the taps whose 32-row
-// chunk kc has a nonzero weight tile (0 when none has).  The GEMM runs chunk kc
-// over these taps only.  Skipping single 16-column tiles inside a tap as well
-// (a uniform branch per fragment) measured 10 % slower on 1005^2 x 51 planes:
-// the branches break up the interleaving of LDS reads and MFMAs.
-static int32_t seg_table(std::vector<uint32_t> &tab, const std::vector<uint8_t> &flag, int64_t w_off2, int ntap,
-                         int nkc, int NT)
+// Zero-tile table of a segment, appended to tab (returns its offset).
+// BF16 GEMM: word kc (< nkc) holds ky0 | ky1 << 16, the hull [ky0, ky1) of the
+// taps whose 32-row chunk kc has a nonzero weight tile (0 when none has); the
+// GEMM runs chunk kc over these taps only.  Skipping single 16-column tiles
+// inside a tap as well (a uniform branch per fragment) measured 10 % slower on
+// 1005^2 x 51 planes: the branches break up the interleaving of LDS reads and
+// MFMAs.  FP32 GEMM: words nkc, nkc + 1, ... hold one bit per 32-row chunk of
+// the segment's K (chunk j: bit j % 32 of word nkc + j / 32), set when the
+// chunk has a nonzero weight; the GEMM skips the others.
+static int32_t seg_table(std::vector<uint32_t> &tab, const std::vector<uint8_t> &tflag,
+                         const std::vector<uint8_t> &cflag, const Seg &sg, int ntap, int nkc, int NT, int64_t NP)
 {
     const size_t at = tab.size();
-    tab.resize(at + nkc, 0);
-    const int64_t t0 = w_off2 / 512;
+    const int nch = sg.K / 32;
+    tab.resize(at + nkc + (nch + 31) / 32, 0);
+    const int64_t t0 = sg.w_off2 / 512;
     for (int kc = 0; kc < nkc; ++kc) {
         int ky0 = ntap, ky1 = 0;
         for (int ky = 0; ky < ntap; ++ky) {
             bool any = false;
             for (int ct = 0; ct < NT && !any; ++ct) {
                 const int64_t t = t0 + (int64_t)(ky * nkc + kc) * NT + ct;
-                any = t >= (int64_t)flag.size() || flag[t];   // unknown: computed
+                any = t >= (int64_t)tflag.size() || tflag[t];   // unknown: computed
             }
             if (any) {
                 ky0 = std::min(ky0, ky);
@@ -1009,6 +1029,11 @@ static int32_t seg_table(std::vector<uint32_t> &tab, const std::vector<uint8_t> 
             }
         }
         tab[at + kc] = ky0 < ky1 ? (uint32_t)ky0 | (uint32_t)ky1 << 16 : 0u;
+    }
+    const int64_t c0 = sg.w_off / (NP * 32);
+    for (int j = 0; j < nch; ++j) {
+        const int64_t t = c0 + j;
+        if (t >= (int64_t)cflag.size() || cflag[t]) tab[at + nkc + j / 32] |= 1u << (j % 32);
     }
     return (int32_t)at;
 }
@@ -1096,11 +1121,14 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     int per_cu = any_fft ? 4 : 8;
     if (const char *e = std::getenv("SID_SPLIT_PER_CU")) per_cu = std::max(1, std::atoi(e));
     const int target_blocks = per_cu * dev_cus;
-    // Segment cost: K for the FP32 / FP64 GEMMs, the K rows inside the tap hulls for the BF16 GEMM.
+    // Segment cost: the K rows the GEMM runs (the nonzero chunks for FP32, the tap hulls for BF16).
     const int nkc = (int)(p->KT / 32);
     auto seg_cost = [&](const GemmPlan &g, const Seg &sg) -> int64_t {
-        if (!cost_bf16) return sg.K;
         int64_t k = 0;
+        if (!cost_bf16) {
+            for (int w = 0; w < (sg.K / 32 + 31) / 32; ++w) k += 32 * (int64_t)__builtin_popcount(g.tmap[sg.tmap + nkc + w]);
+            return k;
+        }
         for (int kc = 0; kc < nkc; ++kc) {
             const uint32_t h = g.tmap[sg.tmap + kc];
             k += 32 * (int64_t)((int)(h >> 16) - (int)(h & 0xffffu));
@@ -1124,7 +1152,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             sg.plane = (int32_t)c;
             sg.w_off2 = pl.wf2_off + (int64_t)tx * pl.Kf2 * NP;
             sg.K2 = (int32_t)pl.Kf2;
-            sg.tmap = seg_table(F.tmap, p->tflag_f, sg.w_off2, ntap, nkc, (int)(NP / 16));
+            sg.tmap = seg_table(F.tmap, p->tflag_f, p->cflag_f, sg, ntap, nkc, (int)(NP / 16), NP);
             F.segs.push_back(sg);
             F.cost.push_back(seg_cost(F, sg));
         }
@@ -1164,7 +1192,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             sg.plane = (int32_t)c;
             sg.w_off2 = pl.wb2_off + (int64_t)tx * pl.Kb2 * NP;
             sg.K2 = (int32_t)pl.Kb2;
-            sg.tmap = seg_table(B.tmap, p->tflag_b, sg.w_off2, ntap, nkc, (int)(NP / 16));
+            sg.tmap = seg_table(B.tmap, p->tflag_b, p->cflag_b, sg, ntap, nkc, (int)(NP / 16), NP);
             B.segs.push_back(sg);
             B.cost.push_back(seg_cost(B, sg));
             btotal += B.cost.back();
@@ -1267,8 +1295,8 @@ static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st, int
                            (int)sp.Ly, (int)sp.YP, S, slab);
     } else if (p->precision == SID_PREC_FP32) {
         hipLaunchKernelGGL((k_proj_gemm<NT, 2>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems, g.d_segs,
-                           fwd ? sp.vol : sp.img, fwd ? p->wf : p->wb, sp.part, (int)sp.nrows, (int)sp.Ly,
-                           (int)sp.YP, S, slab);
+                           g.d_tmap, (int)(p->KT / 32), fwd ? sp.vol : sp.img, fwd ? p->wf : p->wb, sp.part,
+                           (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S, slab);
     } else {
         const u16 *A0 = fwd ? sp.vol_hi : sp.img_hi, *A1 = fwd ? sp.vol_lo : sp.img_lo;
         const u16 *W0 = fwd ? p->wf_hi : p->wb_hi;

@@ -172,6 +172,8 @@ struct sid_psf {
     // per 32 x 16 tile of the BF16 weight layout (512 consecutive entries):
     // 1 when an entry is nonzero (forward / backward)
     std::vector<uint8_t> tflag_f, tflag_b;
+    // per 32-row chunk of the FP32 weight layout (NP * 32 consecutive entries)
+    std::vector<uint8_t> cflag_f, cflag_b;
     int precision = SID_PREC_FP32;
     int method = SID_METHOD_AUTO;
     std::vector<char> pin;            // per plane: 0 autotuned, SID_METHOD_DIRECT / _FFT pinned by the caller
