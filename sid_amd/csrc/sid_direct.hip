@@ -505,12 +505,13 @@ __global__ __launch_bounds__(NTHREADS, (MODE == 2 || RT > 2) ? 1 : 2) void k_pro
 // FP64 variant (reference path for error studies): v_mfma_f64_16x16x4_f64
 // with the same K permutation as MODE 0 (lane group g supplies k = 8g + s at
 // step s of a 32-wide chunk).  Its output layout differs from the FP32 one:
-// item i of lane (l16, g) is C[4i + g][l16].
+// item i of lane (l16, g) is C[4i + g][l16].  It skips the all-zero chunks as
+// the FP32 GEMM does (same layout, same chunk masks).
 template <int NT, int RT>
 __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
-    const Item *__restrict__ items, const Seg *__restrict__ segs, const double *__restrict__ A0,
-    const double *__restrict__ W0, double *__restrict__ part, int nrows, int Ly, int YP, int S,
-    int64_t slab_stride)
+    const Item *__restrict__ items, const Seg *__restrict__ segs, const uint32_t *__restrict__ tmap, int nkc,
+    const double *__restrict__ A0, const double *__restrict__ W0, double *__restrict__ part, int nrows, int Ly,
+    int YP, int S, int64_t slab_stride)
 {
     constexpr int NP = NT * 16;
     constexpr int MT = 4 * RT * 16;
@@ -538,7 +539,10 @@ __global__ __launch_bounds__(NTHREADS) void k_proj_gemm_f64(
         const Seg sg = segs[s];
         const double *A = A0 + sg.a_off;
         const double *W = W0 + sg.w_off;
+        const uint32_t *live = tmap + sg.tmap + nkc;   // nonzero 32-row chunks, one bit each
         for (int k0 = 0; k0 < sg.K; k0 += KC) {
+            const int j = k0 / KC;
+            if (!((live[j >> 5] >> (j & 31)) & 1u)) continue;   // all-zero weights (uniform)
             double2 a[RT][4];
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt)
@@ -1291,8 +1295,8 @@ static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st, int
     const int64_t slab = sp.mpad * (int64_t)(NT * 16);
     if (p->precision == SID_PREC_FP64) {
         hipLaunchKernelGGL((k_proj_gemm_f64<NT, 2>), dim3((unsigned)nitems), dim3(NTHREADS), 0, st, g.d_items,
-                           g.d_segs, fwd ? sp.vol_d : sp.img_d, fwd ? p->wf_d : p->wb_d, sp.part_d, (int)sp.nrows,
-                           (int)sp.Ly, (int)sp.YP, S, slab);
+                           g.d_segs, g.d_tmap, (int)(p->KT / 32), fwd ? sp.vol_d : sp.img_d, fwd ? p->wf_d : p->wb_d,
+                           sp.part_d, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S, slab);
     } else if (p->precision == SID_PREC_FP32) {
         hipLaunchKernelGGL((k_proj_gemm<NT, 2>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems, g.d_segs,
                            g.d_tmap, (int)(p->KT / 32), fwd ? sp.vol : sp.img, fwd ? p->wf : p->wb, sp.part,
