@@ -183,7 +183,8 @@ int sid_patched_rect(int64_t n, int64_t M, const int64_t sensor[2], const int64_
 int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *boxes, const float *vals_dev,
                                const int64_t *val_offsets, const int64_t sensor[2], float *out_dev, int flags,
                                void *stream);
-/* {rows, work items, MFMA multiply-adds, accumulator elements} of the cached library plan. */
+/* {rows, work items, MFMA multiply-adds issued (chunks of all-zero weights are
+ * skipped), accumulator elements} of the cached library plan. */
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
 
 /* --- Neuron kernel (reconstruct_S.m:110-128,184-190; generate_kernel.m) ----

@@ -37,6 +37,8 @@ struct LibItem {
     int32_t row0, nrows, plane, dx;
     int64_t w_off;             // weights of (plane, dx): chunked [K/32][NP][32]
     int32_t t_lo, t_hi;
+    int32_t live;              // offset (words) of the (plane, dx) chunk table: bit 8 (dy - t_lo) + kc / 32
+    int32_t pad;               //   set when chunk kc of tap dy has a nonzero weight
 };
 
 struct LibSeed {
@@ -88,7 +90,8 @@ template <int NT>
 __global__ __launch_bounds__(256) void k_lib_gemm(const LibItem *__restrict__ items, int n_items,
                                                   int *__restrict__ counter, const LibRow *__restrict__ rows,
                                                   const float *__restrict__ A, const float *__restrict__ W, int S,
-                                                  int nn2, float *__restrict__ acc_buf)
+                                                  int nn2, const uint32_t *__restrict__ live_tab,
+                                                  float *__restrict__ acc_buf)
 {
     constexpr int NP = NT * 16, KC = 32, LDK = KC + 4;
     __shared__ float Bs[NP * LDK];
@@ -118,11 +121,16 @@ __global__ __launch_bounds__(256) void k_lib_gemm(const LibItem *__restrict__ it
         const float *Arow = A + (int64_t)arow * LIB_KROW;
         const float *Wp = W + it.w_off;
         for (int dy = it.t_lo; dy <= it.t_hi; ++dy) {
+            // chunks of this tap with a nonzero weight (uniform); a tap without any adds nothing
+            const int tb = 8 * (dy - it.t_lo);
+            const uint32_t live = (live_tab[it.live + (tb >> 5)] >> (tb & 31)) & 0xffu;
+            if (live == 0) continue;
             const int kb = (it.t_hi - dy) * S;
             f4 acc[NT];
 #pragma unroll
             for (int ct = 0; ct < NT; ++ct) acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
             for (int kc = 0; kc < LIB_KROW; kc += KC) {
+                if (!((live >> (kc / KC)) & 1u)) continue;
                 const float4 a0 = *(const float4 *)(Arow + kc + 8 * g);
                 const float4 a1 = *(const float4 *)(Arow + kc + 8 * g + 4);
                 __syncthreads();
@@ -227,6 +235,7 @@ struct LibPlan {
     LibSeed *d_seeds = nullptr;
     int64_t *d_pix_off = nullptr;
     int *d_counter = nullptr;
+    uint32_t *d_live = nullptr;
     float *d_A = nullptr, *d_acc = nullptr;
     int64_t n_rows = 0, n_items = 0, acc_elems = 0, n_seeds = 0;
     int grid = 0;
@@ -238,7 +247,7 @@ struct LibPlan {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void *q : {(void *)d_rows, (void *)d_items, (void *)d_seeds, (void *)d_pix_off, (void *)d_counter,
-                        (void *)d_A, (void *)d_acc})
+                        (void *)d_live, (void *)d_A, (void *)d_acc})
             if (q) (void)hipFree(q);
     }
 };
@@ -263,7 +272,7 @@ static int enqueue(sid_psf *p, LibPlan &L, hipStream_t st)
 #define SID_LIB_CASE(NTV)                                                                                        \
     case NTV:                                                                                                    \
         hipLaunchKernelGGL((k_lib_gemm<NTV>), dim3(L.grid), dim3(256), 0, st, L.d_items, (int)L.n_items,         \
-                           L.d_counter, L.d_rows, L.d_A, p->wf, (int)p->S, nn2, L.d_acc);                        \
+                           L.d_counter, L.d_rows, L.d_A, p->wf, (int)p->S, nn2, L.d_live, L.d_acc);              \
         break;
             SID_LIB_CASE(1) SID_LIB_CASE(2) SID_LIB_CASE(4) SID_LIB_CASE(8) SID_LIB_CASE(11) SID_LIB_CASE(15)
             SID_LIB_CASE(16)
@@ -315,6 +324,26 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
         // rows grouped by plane
         std::vector<LibRow> rows;
         std::vector<LibItem> items;
+        // chunk tables per (plane, dx) from the FP32 zero-chunk flags: local chunk kc of tap dy
+        // covers K rows (t_hi - dy) S + [32 kc, min(32 kc + 32, S)) of the tap row's weights
+        std::vector<uint32_t> live;
+        auto live_table = [&](const Plane &pl, int dx) -> int32_t {
+            const int ntap = pl.tf_hi - pl.tf_lo + 1;
+            const int32_t at = (int32_t)live.size();
+            live.resize(live.size() + (8 * ntap + 31) / 32, 0);
+            const int64_t c0 = (pl.wf_off + (int64_t)(dx - pl.tf_lo) * pl.Kf * NP) / (NP * 32);
+            for (int dy = pl.tf_lo; dy <= pl.tf_hi; ++dy)
+                for (int kc = 0; kc < LIB_KROW / 32 && 32 * kc < S; ++kc) {
+                    const int64_t k0 = (int64_t)(pl.tf_hi - dy) * S + 32 * kc;
+                    const int64_t k1 = (int64_t)(pl.tf_hi - dy) * S + std::min<int64_t>(32 * kc + 32, S);
+                    bool any = false;
+                    for (int64_t j = k0 / 32; j <= (k1 - 1) / 32 && !any; ++j)
+                        any = c0 + j >= (int64_t)p->cflag_f.size() || p->cflag_f[c0 + j];   // unknown: computed
+                    const int b = 8 * (dy - pl.tf_lo) + kc;
+                    if (any) live[at + b / 32] |= 1u << (b % 32);
+                }
+            return at;
+        };
         for (int64_t c = 0; c < p->nzH; ++c) {
             const Plane &pl = p->planes[c];
             const int64_t r0 = (int64_t)rows.size();
@@ -332,6 +361,8 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
             }
             const int64_t r1 = (int64_t)rows.size();
             if (pl.empty || r1 == r0) continue;
+            std::vector<int32_t> live_at;
+            for (int dx = pl.tf_lo; dx <= pl.tf_hi; ++dx) live_at.push_back(live_table(pl, dx));
             for (int64_t r = r0; r < r1; r += LIB_MT)
                 for (int dx = pl.tf_lo; dx <= pl.tf_hi; ++dx) {
                     LibItem it;
@@ -339,8 +370,13 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
                     it.plane = (int)c; it.dx = dx;
                     it.w_off = pl.wf_off + (int64_t)(dx - pl.tf_lo) * pl.Kf * NP;
                     it.t_lo = pl.tf_lo; it.t_hi = pl.tf_hi;
+                    it.live = live_at[dx - pl.tf_lo];
+                    it.pad = 0;
                     items.push_back(it);
-                    L->macs += (int64_t)LIB_MT * LIB_KROW * NP * (pl.tf_hi - pl.tf_lo + 1);
+                    int64_t nlive = 0;   // chunks the GEMM runs for this item
+                    for (int w = 0; w < (8 * (pl.tf_hi - pl.tf_lo + 1) + 31) / 32; ++w)
+                        nlive += __builtin_popcount(live[it.live + w]);
+                    L->macs += (int64_t)LIB_MT * 32 * NP * nlive;
                 }
         }
         L->n_rows = (int64_t)rows.size();
@@ -354,6 +390,9 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
         SID_TRY_HIP(hipMalloc(&L->d_seeds, sizeof(LibSeed) * std::max<int64_t>(ns, 1)));
         SID_TRY_HIP(hipMalloc(&L->d_pix_off, sizeof(int64_t) * (ns + 1)));
         SID_TRY_HIP(hipMalloc(&L->d_counter, sizeof(int)));
+        SID_TRY_HIP(hipMalloc(&L->d_live, sizeof(uint32_t) * std::max<size_t>(live.size(), 1)));
+        if (!live.empty())
+            SID_TRY_HIP(hipMemcpy(L->d_live, live.data(), sizeof(uint32_t) * live.size(), hipMemcpyHostToDevice));
         SID_TRY_HIP(hipMalloc(&L->d_A, sizeof(float) * std::max<int64_t>(L->n_rows, 1) * LIB_KROW));
         SID_TRY_HIP(hipMalloc(&L->d_acc, sizeof(float) * L->acc_elems));
         if (L->n_rows)
