@@ -1119,10 +1119,12 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     }
     // Work items per CU the segment split aims for.  Alone on the device the
     // GEMM gets 8 (four waves of blocks at two blocks per CU: a short tail);
-    // beside the FFT path 4 (8 measured 11 % slower for the overlapped hybrid
-    // plan at 1005^2 x 51 planes).  SID_SPLIT_PER_CU overrides both.
+    // beside the FFT path 6 (at 1005^2 x 51 planes, BF16X2, alternating runs on
+    // one device: 4 -> 113.9 / 115.1, 6 -> 118.9 / 119.5, 3 -> 118.3 / 117.3 RL
+    // iter/s; 8 measured 11 % slower than 4 before the tap hulls).  SID_SPLIT_PER_CU
+    // overrides both.
     const bool any_fft = std::find(use.begin(), use.end(), 1) != use.end();
-    int per_cu = any_fft ? 4 : 8;
+    int per_cu = any_fft ? 6 : 8;
     if (const char *e = std::getenv("SID_SPLIT_PER_CU")) per_cu = std::max(1, std::atoi(e));
     const int target_blocks = per_cu * dev_cus;
     // Segment cost: the K rows the GEMM runs (the nonzero chunks for FP32, the tap hulls for BF16).
