@@ -68,3 +68,18 @@ def test_movie_errors(movie):
         mv.background_and_std(np.zeros((10, 4), np.float32), bg_iter=1, normalize=True)
     with pytest.raises(sid_amd.SidError, match="SID_ERR_ARG"):
         mv.background_and_std(np.ones((10, 4), np.float32), bg_iter=-1)
+
+
+@pytest.mark.parametrize("where", ["first", "last", "middle"])
+def test_normalising_maximum_at_the_edges(movie, where):
+    """The maximum comes out of the first column pass when background
+    iterations run, out of a pass of its own otherwise: both see every value."""
+    sid_amd, mv = movie
+    print("case: movie maximum at", where, flush=True)
+    rng = np.random.default_rng(9)
+    P, T = 2053, 39                        # ragged against the 4-frame groups and the pixel chunks
+    Y = (rng.random((P, T)) * 0.5).astype(np.float32)
+    i, j = {"first": (0, 0), "last": (P - 1, T - 1), "middle": (P // 2, T // 2)}[where]
+    Y[i, j] = 7.0
+    for it in (0, 1, 2):
+        assert mv.background_and_std(Y, bg_iter=it, bg_sub=it > 0, normalize=True)[3] == 7.0
