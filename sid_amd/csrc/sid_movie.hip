@@ -12,7 +12,7 @@
 // s and t are reduced on the host in FP64.  The normalising maximum comes out of
 // the first column pass (the first t is normalised, so its scale is applied on
 // the host afterwards), or out of a pass of its own without background
-// iterations: 2 * bg_iter passes over Y in all (4.9 -> 3.4 ms at 1010025 px x
+// iterations: 2 * bg_iter passes over Y in all (4.9 -> 3.3 ms at 1010025 px x
 // 1000 frames with the maximum fused).
 #include <hip/hip_runtime.h>
 
