@@ -1118,13 +1118,14 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) dev_cus = prop.multiProcessorCount;
     }
     // Work items per CU the segment split aims for.  Alone on the device the
-    // GEMM gets 8 (four waves of blocks at two blocks per CU: a short tail);
-    // beside the FFT path 6 (at 1005^2 x 51 planes, BF16X2, alternating runs on
-    // one device: 4 -> 113.9 / 115.1, 6 -> 118.9 / 119.5, 3 -> 118.3 / 117.3 RL
-    // iter/s; 8 measured 11 % slower than 4 before the tap hulls).  SID_SPLIT_PER_CU
-    // overrides both.
+    // GEMM gets 16 (at 1005^2 x 51 planes, alternating runs: 8 -> 16 took FP32
+    // from 9.23 to 9.63 and BF16X2 from 53.2 to 54.7 RL iter/s, the 16-frame
+    // gradient unchanged; 4 had left a third wave of blocks 11-25 % full);
+    // beside the FFT path 6 (BF16X2, alternating runs on one device: 4 -> 113.9 /
+    // 115.1, 6 -> 118.9 / 119.5, 3 -> 118.3 / 117.3 RL iter/s; 8 measured 11 %
+    // slower than 4 before the tap hulls).  SID_SPLIT_PER_CU overrides both.
     const bool any_fft = std::find(use.begin(), use.end(), 1) != use.end();
-    int per_cu = any_fft ? 6 : 8;
+    int per_cu = any_fft ? 6 : 16;
     if (const char *e = std::getenv("SID_SPLIT_PER_CU")) per_cu = std::max(1, std::atoi(e));
     const int target_blocks = per_cu * dev_cus;
     // Segment cost: the K rows the GEMM runs (the nonzero chunks for FP32, the tap hulls for BF16).
