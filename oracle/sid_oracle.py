@@ -10,16 +10,31 @@ MATLAB's logical indexing with 0-based offsets, i.e. ``H[i, j, a, b, c]`` is
 ``H(i+1, j+1, a+1, b+1, c+1)``; ``V[x, y, c]`` is ``realspace(x+1, y+1, c+1)``.
 The sublattice ``aa:Nnum:end`` is ``a::n`` here.
 
-How each function is pinned (tests/test_oracle.py):
+How each function is pinned (tests/test_oracle.py; every name in __all__ has
+at least one test that does not retype its formula):
+  matlab_round, matlab_colon, matlab_ind2sub3   hand-worked MATLAB values
+  fft_pad_size         hand-worked values of the pad rule
   conv2fft_same        scipy.signal.convolve2d(mode='same') and brute force
+  conv2_same_direct    MATLAB's documented conv([1 2 3],[1 1],'same') window
+  dense_operator       the point-source closed form (a unit voxel images its kernel)
   forward_project*     dense-matrix brute force on tiny cases, point-source
                        closed form, linearity, unit-sum flux conservation
   backward_project*    dense-matrix transpose, <F v, w> = <v, B w>
   deconv_rl            dense-matrix ISRA, fixed point, monotone LS decrease,
                        guard cases (0/0 and x/0 give 0)
   quantile_matlab      Hyndman-Fan type 5 closed form on hand-worked cases
+  reconstruct_preprocess  hand-worked threshold / max / norm values
+  reconstruct_isra     dense-matrix ISRA of the hand-preprocessed image
   library placement    hand-worked integer cases, MATLAB round semantics
-  poisson_nll_grad     central finite differences
+  library_gpu_entry    box average and z slice built by hand, full-H projection
+  ball_offsets, library_cpu_voxels  hand-counted voxel lists (rr = 4/3, 1.5, 2)
+  project_forward_patched, library_cpu_entry, library_cpu   equal to the
+                       full-sensor projection of the same voxels on the pasted
+                       rectangle, unit norm, sparse rows = find(Q)
+  poisson_nll*         central finite differences
+  rank_1_factorization, compute_std_image   SVD and numpy.std
+  convn_same, tv_l2_stencil   scipy.signal.convolve(mode='same')
+  ls_deconv            NNLS / L-BFGS-B minima, exact line-search vertex
 """
 from __future__ import annotations
 
@@ -30,11 +45,13 @@ import numpy as np
 __all__ = [
     "matlab_round", "fft_pad_size", "conv2fft_same", "conv2_same_direct",
     "forward_project", "backward_project", "forward_project_acc",
-    "backward_project_acc", "deconv_rl", "quantile_matlab",
+    "backward_project_acc", "dense_operator", "deconv_rl", "quantile_matlab",
     "reconstruct_preprocess", "reconstruct_isra", "library_gpu_placement",
-    "library_gpu_entry", "ball_offsets", "library_cpu_entry",
+    "library_gpu_entry", "matlab_colon", "matlab_ind2sub3", "ball_offsets",
+    "library_cpu_voxels", "library_cpu_entry", "library_cpu",
     "project_forward_patched", "poisson_nll", "poisson_nll_grad_volume",
-    "poisson_nll_grad_psf", "ls_deconv", "dense_operator",
+    "poisson_nll_grad_psf", "rank_1_factorization", "compute_std_image",
+    "convn_same", "tv_l2_stencil", "ls_deconv",
 ]
 
 
@@ -338,8 +355,9 @@ def library_gpu_placement(center, neur_rad, axial, image_size, nz):
 
 def library_gpu_entry(recon, ids, center, neur_rad, axial, H):
     """One entry of src/generate_LFM_library_GPU.m:67-108: build the
-    full-sensor sub-volume (mean over the listed components of ``recon``,
-    :76-85), slice H in z (:87-88) and forward-project it (:95,101)."""
+    full-sensor sub-volume (sum over the listed components of ``recon`` on the
+    clipped box, divided by their count, :76-85), slice H in z (:87-88) and
+    forward-project it (:95,101)."""
     X, Y, nz = recon[0].shape
     p = library_gpu_placement(center, neur_rad, axial, (X, Y), H.shape[4])
     vol = np.zeros((X, Y, p["z1"] - p["z0"] + 1))
@@ -353,21 +371,54 @@ def library_gpu_entry(recon, ids, center, neur_rad, axial, H):
     return forward_project(Hs, vol), p
 
 
+def matlab_colon(a, b, d=1.0):
+    """MATLAB ``a:d:b`` for real scalars and d > 0: the elements a + k*d for
+    k = 0..m with m = fix((b - a)/d) (MATLAB's documented rule), where a
+    quotient within a few ulps below an integer counts as that integer (MATLAB
+    tolerates round-off in (b - a)/d).  ``-rr:rr`` therefore has floor(2 rr) + 1
+    elements, not ceil."""
+    a, b, d = float(a), float(b), float(d)
+    q = (b - a) / d
+    m = math.floor(q)
+    if q - m > 1.0 - 4.0 * np.finfo(np.float64).eps * max(1.0, abs(q)):
+        m += 1
+    if m < 0:
+        return np.zeros(0)
+    return a + d * np.arange(m + 1, dtype=np.float64)
+
+
+def matlab_ind2sub3(siz, ndx):
+    """MATLAB ``[i, j, k] = ind2sub(siz, ndx)`` with three outputs (1-based).
+
+    Only siz(1) and siz(1)*siz(2) enter: k = (ndx - rem(ndx-1, s12) - 1)/s12 + 1,
+    then j and i from the remainder the same way."""
+    s1 = float(siz[0])
+    s12 = float(siz[0]) * float(siz[1])
+    ndx = np.asarray(ndx, dtype=np.float64)
+    vi = np.fmod(ndx - 1, s12) + 1
+    k = (ndx - vi) / s12 + 1
+    ndx = vi
+    vi = np.fmod(ndx - 1, s1) + 1
+    j = (ndx - vi) / s1 + 1
+    return vi, j, k
+
+
 def ball_offsets(r, rr):
-    """Voxel list of the ball kernel of src/generate_LFM_library_CPU.m:34-39:
-    ``meshgrid(-r:r,-r:r,-rr:rr)``, ``X^2 + Y^2 + (Z*r/rr)^2 <= r^2`` and
-    ``ind2sub`` (column-major order).  Returns 1-based (i, j, k) rows; note
-    meshgrid swaps the first two axes (i runs over Y, j over X)."""
-    g = np.arange(-r, r + 1, dtype=np.float64)
-    gz = np.arange(-rr, rr + 1, dtype=np.float64)
-    # meshgrid(x, y, z) returns arrays indexed [y, x, z]
-    Xg = g[None, :, None] + 0 * g[:, None, None] + 0 * gz[None, None, :]
-    Yg = g[:, None, None] + 0 * g[None, :, None] + 0 * gz[None, None, :]
-    Zg = gz[None, None, :] + 0 * g[:, None, None] + 0 * g[None, :, None]
-    W = (Xg ** 2 + Yg ** 2 + (Zg * r / rr) ** 2) <= r ** 2
-    idx = np.flatnonzero(W.ravel(order="F"))
-    i, j, k = np.unravel_index(idx, W.shape, order="F")
-    return np.stack([i + 1, j + 1, k + 1], axis=1), W.astype(np.float64)
+    """Voxel list of the ball of src/generate_LFM_library_CPU.m:34-39.
+
+    ``[X,Y,Z] = meshgrid(-r:r,-r:r,-rr:rr)`` (arrays indexed [y, x, z]),
+    ``W = single(X.^2 + Y.^2 + (Z*r/rr).^2 <= r^2)``, ``BW = find(W)`` and
+    ``[BWW(:,1), BWW(:,2), BWW(:,3)] = ind2sub([2r+1, 2r+1, 2rr+1], BW)``.
+    Returns (BWW as 1-based float rows, W)."""
+    g = matlab_colon(-r, r)
+    gz = matlab_colon(-rr, rr)
+    Xg = np.broadcast_to(g[None, :, None], (g.size, g.size, gz.size))
+    Yg = np.broadcast_to(g[:, None, None], (g.size, g.size, gz.size))
+    Zg = np.broadcast_to(gz[None, None, :], (g.size, g.size, gz.size))
+    W = ((Xg ** 2 + Yg ** 2 + (Zg * r / rr) ** 2) <= r ** 2).astype(np.float32)
+    BW = np.flatnonzero(W.ravel(order="F")) + 1
+    i, j, k = matlab_ind2sub3((2 * r + 1, 2 * r + 1, 2 * rr + 1), BW)
+    return np.stack([i, j, k], axis=1), W
 
 
 def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
@@ -416,22 +467,64 @@ def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
                      off_px=off_px, n_ml=n_ml, off_z=off_z, n_z=n_z)
 
 
-def library_cpu_entry(center, r, dimz, m, H, n):
-    """One entry of src/generate_LFM_library_CPU.m:18-57: ball voxels of
-    radius r (axial radius r/dim(3)) shifted to the centre with MATLAB round,
-    filtered to the volume (:43-50), projected by project_forward_patched and
-    L2-normalised (:51-53)."""
-    rr = r / dimz
-    BWW, W = ball_offsets(r, rr)
-    shift = np.array([(r + 1), (r + 1), (rr + 1)], dtype=np.float64)
-    bbb = matlab_round(BWW - shift + np.asarray(center, dtype=np.float64)).astype(np.int64)
+def library_cpu_voxels(center, r, rr, BWW, m, nzH):
+    """Ball voxels of one centre, src/generate_LFM_library_CPU.m:43-50:
+    ``bbb = round(BWW(j,:) - [(r+1)*[1 1] (rr+1)] + centers(k,:))`` (MATLAB
+    round), kept when 0 < bbb <= (m(1), m(2), size(H,5)); ``bbb_ = round(BWW(j,:))``.
+    Returns (B, B_) as int64 rows in BWW order."""
+    shift = np.array([r + 1, r + 1, rr + 1], dtype=np.float64)
+    c = np.asarray(center, dtype=np.float64)
+    bbb = matlab_round((BWW - shift) + c).astype(np.int64)
     keep = ((bbb[:, 0] <= m[0]) & (bbb[:, 0] > 0) & (bbb[:, 1] <= m[1]) & (bbb[:, 1] > 0)
-            & (bbb[:, 2] <= H.shape[4]) & (bbb[:, 2] > 0))
-    B = bbb[keep]
-    B_ = matlab_round(BWW[keep]).astype(np.int64)
+            & (bbb[:, 2] <= nzH) & (bbb[:, 2] > 0))
+    return bbb[keep], matlab_round(BWW[keep]).astype(np.int64)
+
+
+def library_cpu_entry(center, r, dim, m, H, n, kernel=None):
+    """One entry of src/generate_LFM_library_CPU.m:18-57 in FP64.
+
+    Without ``kernel``: the ball of radius r with axial radius rr = r/dim(3)
+    (:18,34-35); with it: W = kernel, r = (size(W,1)-1)/2, rr = (size(W,3)-1)/2
+    (:29-32).  Voxels are placed at the centre (:43-50), projected by
+    project_forward_patched with colv = W (:51) and L2-normalised (:52-53).
+    Returns (Q as the dense column-major sensor vector, info).  A centre with no
+    voxel inside the volume makes the reference fail inside
+    project_forward_patched (min of an empty array); it raises ValueError here."""
+    if kernel is None:
+        rr = r / dim[2]
+        BWW, W = ball_offsets(r, rr)
+    else:
+        W = np.asarray(kernel)
+        r = (W.shape[0] - 1) / 2
+        rr = (W.shape[2] - 1) / 2
+        BW = np.flatnonzero(W.ravel(order="F")) + 1
+        i, j, k = matlab_ind2sub3((2 * r + 1, 2 * r + 1, 2 * rr + 1), BW)
+        BWW = np.stack([i, j, k], axis=1)
+    B, B_ = library_cpu_voxels(center, r, rr, BWW, m, H.shape[4])
+    if B.shape[0] == 0:
+        raise ValueError("no voxel of this centre lies inside the volume")
     img, info = project_forward_patched(B, m[:2], n, H, W, B_)
     Q = img.ravel(order="F")
+    info["voxels"] = B
     return Q / np.linalg.norm(Q), info
+
+
+def library_cpu(centers, r, dim, m, H, n, kernel=None):
+    """src/generate_LFM_library_CPU.m:1-70: every centre's normalised pattern,
+    assembled as ``sparse(I, J, S, N, prod(m(1:2)))`` (:54-70) with I the centre,
+    J = find(Q) and S = Q(J).  Returned in compressed-row form: row_ptr
+    (N + 1), 0-based column-major pixel indices and values, rows in centre
+    order and columns ascending (find's order)."""
+    row_ptr = [0]
+    cols, vals = [], []
+    for c in np.asarray(centers, dtype=np.float64):
+        Q, _ = library_cpu_entry(c, r, dim, m, H, n, kernel)
+        J = np.flatnonzero(Q)
+        cols.append(J)
+        vals.append(Q[J])
+        row_ptr.append(row_ptr[-1] + J.size)
+    return (np.asarray(row_ptr, dtype=np.int64), np.concatenate(cols) if cols else np.zeros(0, np.int64),
+            np.concatenate(vals) if vals else np.zeros(0))
 
 
 # --------------------------------------------------------------------------

@@ -118,6 +118,12 @@ def test_point_source_closed_form():
         want[q[0] - m:q[0] + m + 1, q[1] - m:q[1] + m + 1] = H[:, :, a, b, c]
         assert np.abs(img - want).max() < 1e-13
         assert np.abs(c_forward(H, V) - want).max() < 1e-13
+    # the brute-force matrix's column of voxel (q, c) is the same closed form
+    A = O.dense_operator(H[:, :, :, :, :1], 20, 20, 1)
+    q = (7, 12)
+    want = np.zeros((20, 20))
+    want[q[0] - M // 2:q[0] + M // 2 + 1, q[1] - M // 2:q[1] + M // 2 + 1] = H[:, :, q[0] % n, q[1] % n, 0]
+    assert np.abs(A[:, q[0] + 20 * q[1]].reshape(20, 20, order="F") - want).max() == 0.0
 
 
 def test_linearity_and_flux_conservation():
@@ -262,14 +268,178 @@ def test_quantile_hazen_hand_worked():
         assert abs(O.quantile_matlab(xs, q) - np.quantile(xs, q, method="hazen")) < 1e-15
 
 
-def test_preprocess_scales():
-    img = np.arange(1.0, 13.0).reshape(3, 4)
+def test_preprocess_hand_worked():
+    """reconstruct_S.m:131-134,168-170,191 on values worked by hand.
+
+    img(:) = 1..12: the Hazen median of 1..12 is 6.5; after the threshold the
+    values 7..12 become 0.5..5.5, divided by 5.5; the norm is sqrt(71.5)/5.5."""
+    img = np.arange(1.0, 13.0).reshape(3, 4, order="F")
     y, nrm = O.reconstruct_preprocess(img, None, 0.5)
-    thr = O.quantile_matlab(img.ravel(order="F"), 0.5)
+    assert abs(nrm - math.sqrt(71.5) / 5.5) < 1e-15
+    want = np.maximum(img - 6.5, 0) / math.sqrt(71.5)
+    assert np.abs(y - want).max() < 1e-15
+    # with a microlens mask the threshold comes from the unmasked pixels only:
+    # values 1..4 (mask == 0) have Hazen median 2.5
+    mask = np.ones(12)
+    mask[:4] = 0
+    y, nrm = O.reconstruct_preprocess(img, mask.reshape(3, 4, order="F"), 0.5)
+    z = np.maximum(np.arange(1.0, 13.0) - 2.5, 0) / 9.5
+    assert abs(nrm - math.sqrt(float((z ** 2).sum()))) < 1e-15
+    assert np.abs(y.ravel(order="F") - z / math.sqrt(float((z ** 2).sum()))).max() < 1e-15
+
+
+def test_reconstruct_isra_is_dense_isra_of_the_preprocessed_image():
+    """reconstruct_S.m:168-205 (ISRA): recon = nrm * deconvRL(F, B, B(y/nrm), it, B(y/nrm)),
+    recomposed here from the dense matrix and numpy's Hazen quantile."""
+    n, M, X, Y, nz = 3, 5, 6, 6, 2
+    rng = np.random.default_rng(21)
+    H = rng.random((M, M, n, n, nz))
+    img = rng.random((X, Y)) * 10
+    A = O.dense_operator(H, X, Y, nz)
+    thr = np.quantile(img.ravel(), 0.8, method="hazen")
     z = np.maximum(img - thr, 0)
     z = z / z.max()
-    assert abs(nrm - math.sqrt((z ** 2).sum())) < 1e-14
-    assert abs(np.linalg.norm(y) - 1.0) < 1e-14
+    nrm = np.linalg.norm(z)
+    yv = (z / nrm).ravel(order="F")
+    htf = A.T @ yv
+    x = htf.copy()
+    for _ in range(5):
+        x = x * (htf / (A.T @ (A @ x)))
+    got = O.reconstruct_isra(H, img, max_iter=5)
+    want = (nrm * x).reshape(X, Y, nz, order="F")
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_matlab_colon_counts_and_values():
+    """MATLAB a:b has fix(b - a) + 1 elements starting at a (colon documentation)."""
+    assert list(O.matlab_colon(-2, 2)) == [-2, -1, 0, 1, 2]
+    v = O.matlab_colon(-4 / 3, 4 / 3)              # -rr:rr with rr = 4/3: 3 elements, not 4
+    assert v.size == 3 and abs(v[-1] - 2 / 3) < 1e-15
+    assert O.matlab_colon(-1.5, 1.5).size == 4       # 2 rr = 3 exactly
+    assert O.matlab_colon(-1.2, 1.2).size == 3
+    assert O.matlab_colon(0.0, 0.3 * 3 / 0.3).size == 4   # round-off in (b - a) is tolerated
+    assert O.matlab_colon(1, 0).size == 0
+
+
+def test_matlab_ind2sub3_hand_worked():
+    # siz [3 3 x]: index 14 -> (2, 2, 2); index 5 -> (2, 2, 1); the third size does not enter
+    for siz in ((3, 3, 3), (3, 3, 2.5)):
+        i, j, k = O.matlab_ind2sub3(siz, [5, 14, 23, 1, 27])
+        assert list(i) == [2, 2, 2, 1, 3] and list(j) == [2, 2, 2, 1, 3] and list(k) == [1, 2, 3, 1, 3]
+
+
+def test_ball_offsets_hand_counted():
+    """generate_LFM_library_CPU.m:34-39 on balls counted by hand."""
+    BWW, W = O.ball_offsets(1, 1.0)
+    # X^2 + Y^2 + Z^2 <= 1: the centre and its 6 neighbours, in find() order
+    assert [tuple(int(v) for v in r) for r in BWW] == [
+        (2, 2, 1), (2, 1, 2), (1, 2, 2), (2, 2, 2), (3, 2, 2), (2, 3, 2), (2, 2, 3)]
+    # r = 4, rr = 4/3 (dim(3) = 3): -rr:rr = [-4/3, -1/3, 2/3] -> 3 planes; Z*r/rr = -4, -1, 2
+    # -> 1 + #{x^2+y^2 <= 15} + #{x^2+y^2 <= 11} = 1 + 45 + 37 voxels
+    BWW, W = O.ball_offsets(4, 4 / 3)
+    assert W.shape == (9, 9, 3)
+    assert [int((BWW[:, 2] == k).sum()) for k in (1, 2, 3)] == [1, 45, 37]
+    # r = 2, rr = 1.5: 4 planes with Z*r/rr = -2, -2/3, 2/3, 2 -> 1, 9, 9, 1
+    BWW, W = O.ball_offsets(2, 1.5)
+    assert W.shape == (5, 5, 4)
+    assert [int((BWW[:, 2] == k).sum()) for k in (1, 2, 3, 4)] == [1, 9, 9, 1]
+    # (i, j) = (row of Y, column of X): the voxel list is symmetric, and its mean is the centre
+    assert np.allclose(BWW[:, :2].mean(axis=0), [3, 3])
+
+
+def test_library_cpu_voxels_hand_worked():
+    """generate_LFM_library_CPU.m:43-50: round(BWW - (r+1, r+1, rr+1) + centre), clipped."""
+    BWW, _ = O.ball_offsets(1, 1.0)
+    B, B_ = O.library_cpu_voxels((1.4, 10.0, 2.0), 1, 1.0, BWW, (20, 20), 3)
+    # x = round(i - 0.6): i = 1 -> 0 (dropped), 2 -> 1, 3 -> 2; y = j + 8; z = k
+    assert [tuple(r) for r in B] == [(1, 10, 1), (1, 9, 2), (1, 10, 2), (2, 10, 2), (1, 11, 2), (1, 10, 3)]
+    assert [tuple(r) for r in B_] == [(2, 2, 1), (2, 1, 2), (2, 2, 2), (3, 2, 2), (2, 3, 2), (2, 2, 3)]
+    # rr = 1.5: z = round(k - 2.5 + 5.2) = k + 3 for k = 1..4; planes above size(H,5) = 6 dropped
+    BWW, _ = O.ball_offsets(2, 1.5)
+    B, _ = O.library_cpu_voxels((10.0, 10.0, 5.2), 2, 1.5, BWW, (20, 20), 6)
+    assert sorted(set(int(z) for z in B[:, 2])) == [4, 5, 6]
+    assert B.shape[0] == 19
+
+
+def _full_projection_of(H, vox, vals, X, Y):
+    V = np.zeros((X, Y, H.shape[4]))
+    for (x, y, z), v in zip(vox, vals):
+        V[x - 1, y - 1, z - 1] = v
+    return O.forward_project(H, V)
+
+
+@pytest.mark.parametrize("center", [(12.3, 14.6, 2.4), (2.2, 13.0, 1.0), (23.6, 24.5, 3.0)])
+def test_library_cpu_entry_is_the_normalised_patch_of_the_full_projection(center):
+    """generate_LFM_library_CPU.m:51-53: Q = patched projection / its norm.  On the
+    pasted rectangle it equals the full-sensor projection of the same voxels
+    (odd n and M), zero elsewhere; the centres include clipped borders."""
+    n, M, m = 3, 7, (24, 25, 3)
+    rng = np.random.default_rng(5)
+    H = rng.random((M, M, n, n, 3))
+    Q, info = O.library_cpu_entry(center, 2, (1.0, 1.0, 1.5), m, H, n)
+    assert abs(np.linalg.norm(Q) - 1.0) < 1e-14
+    full = _full_projection_of(H, info["voxels"], np.ones(len(info["voxels"])), m[0], m[1])
+    win = np.zeros_like(full)
+    x0, x1, y0, y1 = info["x0"], info["x1"], info["y0"], info["y1"]
+    win[x0 - 1:x1, y0 - 1:y1] = full[x0 - 1:x1, y0 - 1:y1]
+    want = win.ravel(order="F") / np.linalg.norm(win)
+    assert np.abs(Q - want).max() < 1e-12
+
+
+def test_library_cpu_kernel_path_and_sparse_rows():
+    """generate_LFM_library_CPU.m:29-32 (kernel given: W = kernel, voxels = find(W),
+    values W) and :54-70 (rows I = centre, J = find(Q), S = Q(J))."""
+    n, M, m = 3, 5, (18, 18, 4)
+    rng = np.random.default_rng(8)
+    H = rng.random((M, M, n, n, 4))
+    ker = rng.random((3, 3, 3)) + 0.1
+    ker[0, 0, :] = 0
+    centers = np.array([[9.0, 9.0, 2.0], [4.4, 15.5, 3.0]])
+    row_ptr, cols, vals = O.library_cpu(centers, None, None, m, H, n, kernel=ker)
+    for s, c in enumerate(centers):
+        Q, info = O.library_cpu_entry(c, None, None, m, H, n, kernel=ker)
+        J = np.flatnonzero(Q)
+        assert np.array_equal(cols[row_ptr[s]:row_ptr[s + 1]], J)
+        assert np.array_equal(vals[row_ptr[s]:row_ptr[s + 1]], Q[J])
+        # r = rr = 1: voxel (i, j, k) goes to round((i, j, k) - 2 + centre) with value ker(i, j, k)
+        vox, val = [], []
+        for k in range(3):
+            for j in range(3):
+                for i in range(3):
+                    if ker[i, j, k] == 0:
+                        continue
+                    p = O.matlab_round(np.array([i + 1, j + 1, k + 1]) - 2.0 + c).astype(int)
+                    if 0 < p[0] <= m[0] and 0 < p[1] <= m[1] and 0 < p[2] <= 4:
+                        vox.append(p)
+                        val.append(ker[i, j, k])
+        assert len(vox) == len(info["voxels"])
+        full = _full_projection_of(H, vox, val, m[0], m[1])
+        x0, x1, y0, y1 = info["x0"], info["x1"], info["y0"], info["y1"]
+        win = np.zeros_like(full)
+        win[x0 - 1:x1, y0 - 1:y1] = full[x0 - 1:x1, y0 - 1:y1]
+        assert np.abs(Q - win.ravel(order="F") / np.linalg.norm(win)).max() < 1e-12
+
+
+def test_library_cpu_entry_outside_the_volume_raises():
+    H = np.ones((5, 5, 3, 3, 2))
+    with pytest.raises(ValueError):
+        O.library_cpu_entry((-10.0, 5.0, 1.0), 1, (1, 1, 1), (12, 12, 2), H, 3)
+
+
+def test_library_gpu_entry_is_box_average_on_its_planes():
+    """generate_LFM_library_GPU.m:76-95: the clipped box of each listed component,
+    summed and divided by their count, projected with H(:,:,:,:,z0:z1) - equal to
+    the full-H projection of that box placed at planes z0..z1 of a zero volume."""
+    n, M, X, Y, Nz = 3, 5, 15, 14, 5
+    rng = np.random.default_rng(3)
+    H = rng.random((M, M, n, n, Nz))
+    recon = [rng.random((X, Y, Nz)) for _ in range(3)]
+    img, p = O.library_gpu_entry(recon, [0, 2], (13.6, 3.2, 4.4), 3, 2.0, H)
+    # round -> (14, 3, 4); dz = ceil(3/2) = 2 -> planes 2..5 (clipped at Nz); box x 11..15, y 1..6
+    assert (p["x0"], p["x1"], p["y0"], p["y1"], p["z0"], p["z1"]) == (11, 15, 1, 6, 2, 5)
+    V = np.zeros((X, Y, Nz))
+    V[10:15, 0:6, 1:5] = 0.5 * (recon[0][10:15, 0:6, 1:5] + recon[2][10:15, 0:6, 1:5])
+    assert np.abs(img - O.forward_project(H, V)).max() < 1e-12 * np.abs(img).max()
 
 
 def test_matlab_round_half_away_from_zero():
