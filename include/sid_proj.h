@@ -24,8 +24,10 @@
  * thread at a time.  All calls are synchronous: when one returns, its outputs
  * are written (device outputs included).
  *
- * "stream" arguments take a hipStream_t cast to void* (NULL = the handle's
- * own stream); device-pointer entry points read and write device memory.
+ * "stream" arguments take a hipStream_t cast to void*; device-pointer entry
+ * points read and write device memory in stream order.  NULL is the device's
+ * default (null) stream: the call's work is ordered after everything already
+ * queued there (the handle runs it on its own stream after an event wait).
  */
 #ifndef SID_PROJ_H
 #define SID_PROJ_H

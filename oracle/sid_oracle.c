@@ -2,7 +2,8 @@
  * ORACLE — test infrastructure only (the checker and the CPU baseline, never
  * the product).  Plain-C restatement of SID's CPU projectors and ISRA loop.
  *
- *   orc_forward_*   external/LFrecon_util/forwardProjectACC.m:14-29
+ *   orc_forward_*   external/LFrecon_util/forwardProjectACC.m:14-29 (f64: FP64 PSF;
+ *                   h32: FP32 PSF, FP64 arithmetic; f32: the reference's single precision)
  *                   (forwardProjectGPU.m:16-28 when CAindex is NULL: full M)
  *   orc_backward_*  external/LFrecon_util/backwardProjectACC_new.m:14-26
  *                   (backwardProjectGPU_new.m:16-26 when CAindex is NULL)
@@ -131,6 +132,9 @@ static void omp_set_num_threads_wrap(int n)
 
 /* FP64 oracle. */
 DEFINE_FORWARD(orc_forward_f64, double, double, double, double)
+/* FP64 oracle on an FP32 PSF (exact FP64 products of the FP32 kernel values):
+ * avoids an FP64 copy of multi-GB PSFs. */
+DEFINE_FORWARD(orc_forward_h32, float, double, double, double)
 DEFINE_BACKWARD(orc_backward_f64, double, double, double, double)
 /* Reference CPU semantics: single-precision forward (forwardProjectACC.m:5,25),
  * single-precision convolution stored as double (backwardProjectACC_new.m:6,22). */

@@ -28,6 +28,7 @@ at least one test that does not retype its formula):
   library placement    hand-worked integer cases, MATLAB round semantics
   library_gpu_entry    box average and z slice built by hand, full-H projection
   ball_offsets, library_cpu_voxels  hand-counted voxel lists (rr = 4/3, 1.5, 2)
+  patched_rect         hand-worked origin / odd lenslet count / clipped cases
   project_forward_patched, library_cpu_entry, library_cpu   equal to the
                        full-sensor projection of the same voxels on the pasted
                        rectangle, unit norm, sparse rows = find(Q)
@@ -49,7 +50,7 @@ __all__ = [
     "reconstruct_preprocess", "reconstruct_isra", "library_gpu_placement",
     "library_gpu_entry", "matlab_colon", "matlab_ind2sub3", "ball_offsets",
     "library_cpu_voxels", "library_cpu_entry", "library_cpu",
-    "project_forward_patched", "poisson_nll", "poisson_nll_grad_volume",
+    "patched_rect", "project_forward_patched", "poisson_nll", "poisson_nll_grad_volume",
     "poisson_nll_grad_psf", "rank_1_factorization", "compute_std_image",
     "convn_same", "tv_l2_stencil", "ls_deconv",
 ]
@@ -421,6 +422,33 @@ def ball_offsets(r, rr):
     return np.stack([i, j, k], axis=1), W
 
 
+def patched_rect(px_min, px_max, volume_n_px, n, M):
+    """The patch geometry of src/project_forward_patched.m:8-24,54-70 for a
+    pixel set with 1-based bounds px_min / px_max (x, y, z) and an M x M PSF:
+    origin ``floor((px_min-1)/n)*n`` (:11-13), lenslet count
+    ``ceil(px_max/n) - offset`` forced odd (:14-21), 'full' size
+    ``n_ml*n + M - 1``, centre ``offset + ceil((n_ml/2)*n)`` (:56-57), the
+    clipped sensor rectangle [x0, x1] x [y0, y1] (:61-64) and its patch
+    coordinates [p0, p1] x [q0, q1] (:67-70)."""
+    px_min = np.asarray(px_min, dtype=np.int64)
+    px_max = np.asarray(px_max, dtype=np.int64)
+    off_ml = (px_min[:2] - 1) // n
+    off_px = off_ml * n
+    n_ml = -(-px_max[:2] // n) - off_ml
+    n_ml = np.where(n_ml % 2 == 0, n_ml + 1, n_ml)
+    pjx, pjy = int(n_ml[0] * n + M - 1), int(n_ml[1] * n + M - 1)
+    cx = int(off_px[0]) + math.ceil((n_ml[0] / 2) * n)
+    cy = int(off_px[1]) + math.ceil((n_ml[1] / 2) * n)
+    x0 = max(1, cx - pjx // 2)
+    x1 = min(int(volume_n_px[0]), cx + pjx // 2)
+    y0 = max(1, cy - pjy // 2)
+    y1 = min(int(volume_n_px[1]), cy + pjy // 2)
+    return dict(x0=x0, x1=x1, y0=y0, y1=y1,
+                p0=x0 - (cx - pjx // 2 - 1), p1=x1 - (cx - pjx // 2 - 1),
+                q0=y0 - (cy - pjy // 2 - 1), q1=y1 - (cy - pjy // 2 - 1),
+                off_px=off_px, n_ml=n_ml, off_z=int(px_min[2]) - 1, n_z=int(px_max[2] - px_min[2] + 1))
+
+
 def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
     """src/project_forward_patched.m:1-76 in FP64 (1-based pixel triples).
 
@@ -428,14 +456,8 @@ def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
     (:13-20), ``conv2(., 'full')`` per (i, j, k) (:37-49), centre
     ``offset + ceil((n_ml/2)*n)`` (:56-57), clipped paste (:61-73)."""
     pixels = np.asarray(pixels, dtype=np.int64)
-    px_min = pixels.min(axis=0)
-    px_max = pixels.max(axis=0)
-    off_ml = (px_min[:2] - 1) // n
-    off_px = off_ml * n
-    n_ml = -(-px_max[:2] // n) - off_ml
-    n_ml = np.where(n_ml % 2 == 0, n_ml + 1, n_ml)
-    off_z = px_min[2] - 1
-    n_z = px_max[2] - off_z
+    g = patched_rect(pixels.min(axis=0), pixels.max(axis=0), volume_n_px, n, H.shape[0])
+    off_px, n_ml, off_z, n_z = g["off_px"], g["n_ml"], g["off_z"], g["n_z"]
     vol = np.zeros((int(n_ml[0] * n), int(n_ml[1] * n), int(n_z)))
     for t in range(pixels.shape[0]):
         vol[pixels[t, 0] - off_px[0] - 1, pixels[t, 1] - off_px[1] - 1, pixels[t, 2] - off_z - 1] = \
@@ -450,21 +472,9 @@ def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
                 t[i::n, j::n] = vol[i::n, j::n, k]
                 if np.any(t):
                     tot += fftconvolve(t, H[:, :, i, j, off_z + k].astype(np.float64), mode="full")
-    pjx, pjy = tot.shape
-    cx = off_px[0] + math.ceil((n_ml[0] / 2) * n)
-    cy = off_px[1] + math.ceil((n_ml[1] / 2) * n)
-    x0 = max(1, cx - pjx // 2)
-    x1 = min(volume_n_px[0], cx + pjx // 2)
-    y0 = max(1, cy - pjy // 2)
-    y1 = min(volume_n_px[1], cy + pjy // 2)
-    p0 = x0 - (cx - pjx // 2 - 1)
-    p1 = x1 - (cx - pjx // 2 - 1)
-    q0 = y0 - (cy - pjy // 2 - 1)
-    q1 = y1 - (cy - pjy // 2 - 1)
     img = np.zeros((int(volume_n_px[0]), int(volume_n_px[1])))
-    img[x0 - 1:x1, y0 - 1:y1] = tot[p0 - 1:p1, q0 - 1:q1]
-    return img, dict(x0=x0, x1=x1, y0=y0, y1=y1, p0=p0, p1=p1, q0=q0, q1=q1,
-                     off_px=off_px, n_ml=n_ml, off_z=off_z, n_z=n_z)
+    img[g["x0"] - 1:g["x1"], g["y0"] - 1:g["y1"]] = tot[g["p0"] - 1:g["p1"], g["q0"] - 1:g["q1"]]
+    return img, g
 
 
 def library_cpu_voxels(center, r, rr, BWW, m, nzH):

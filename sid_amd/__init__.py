@@ -209,6 +209,8 @@ class PSF:
 
     def deconv_rl(self, Htf, maxIter: int, Xguess, history: bool = False):
         """deconvRL basic (lambda = 0) with this PSF's projectors, all on device."""
+        if tuple(Xguess.shape) != tuple(Htf.shape):
+            raise ValueError("deconv_rl: Xguess has shape %s but Htf has %s" % (tuple(Xguess.shape), tuple(Htf.shape)))
         if _is_torch(Htf):
             torch = _torch()
             h = Htf.detach().to(torch.float32).contiguous()

@@ -149,13 +149,14 @@ static void destroy(sid_psf *p)
                     (void *)sp.fwd.d_tmap, (void *)sp.bwd.d_tmap, (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
                     (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag,
-                    (void *)p->hbuf[0], (void *)p->hbuf[1]})
+                    (void *)p->hbuf[0], (void *)p->hbuf[1], (void *)p->nf})
         if (q) (void)hipFree(q);
     if (p->aux) {
         (void)hipStreamSynchronize(p->aux);
         (void)hipStreamDestroy(p->aux);
     }
     if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_null) (void)hipEventDestroy(p->ev_null);
     if (p->ev_join) (void)hipEventDestroy(p->ev_join);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -177,7 +178,25 @@ static int check_device()
     return SID_OK;
 }
 
-static hipStream_t pick_stream(sid_psf *p, void *stream) { return stream ? (hipStream_t)stream : p->stream; }
+// The stream a device-pointer entry point runs on.  A caller's stream is used
+// as given.  NULL is the device's default (null) stream, as in the HIP runtime:
+// the work then runs on the handle's own stream (non-blocking, so it would not
+// wait for the default stream by itself) after an event that makes it wait for
+// everything already queued on the default stream (the caller's producers of
+// the inputs).  Every entry point is synchronous, so its outputs are complete
+// when it returns either way.
+static int pick_stream(sid_psf *p, void *stream, hipStream_t *st)
+{
+    if (stream) {
+        *st = (hipStream_t)stream;
+        return SID_OK;
+    }
+    if (!p->ev_null) SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_null, hipEventDisableTiming));
+    SID_TRY_HIP(hipEventRecord(p->ev_null, nullptr));
+    SID_TRY_HIP(hipStreamWaitEvent(p->stream, p->ev_null, 0));
+    *st = p->stream;
+    return SID_OK;
+}
 
 }  // namespace sid
 
@@ -230,6 +249,12 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
         for (int i = 0; i < 5; ++i)
             if (dims[i] < 1) { rc = SID_ERR_DIMS; set_error("H has an empty dimension"); }
         if (rc) break;
+        if (hipMalloc(&p->nf, 2 * sizeof(unsigned long long)) != hipSuccess ||
+            hipMemset(p->nf, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+            rc = SID_ERR_OOM;
+            set_error("hipMalloc of the non-finite flags failed");
+            break;
+        }
         const int64_t elems = dims[0] * dims[1] * dims[2] * dims[3] * dims[4];
         hipError_t e = hipMalloc(&p->H, elems * sizeof(float));
         if (e != hipSuccess) { rc = SID_ERR_OOM; set_error("hipMalloc(H, %lld B) failed", (long long)(elems * 4)); break; }
@@ -303,7 +328,9 @@ int sid_poisson_nll_grad_device(sid_psf *p, const float *V, const float *y, cons
     }
     if (p->kern) { set_error("gradients are not available on a handle with a neuron kernel"); return SID_ERR_ARG; }
     SID_TRY_HIP(hipSetDevice(p->device));
-    return grad_core(p, V, y, dims, 0, nll, gV, gH, pick_stream(p, stream));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return grad_core(p, V, y, dims, 0, nll, gV, gH, st);
 }
 
 int sid_project_vjp_device(sid_psf *p, const float *V, const float *G, const int64_t dims[4], float *gV, float *gH,
@@ -320,7 +347,9 @@ int sid_project_vjp_device(sid_psf *p, const float *V, const float *G, const int
     }
     if (p->kern) { set_error("gradients are not available on a handle with a neuron kernel"); return SID_ERR_ARG; }
     SID_TRY_HIP(hipSetDevice(p->device));
-    return grad_core(p, V, G, dims, 1, nullptr, gV, gH, pick_stream(p, stream));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return grad_core(p, V, G, dims, 1, nullptr, gV, gH, st);
 }
 
 int sid_psf_destroy(sid_psf *psf)
@@ -589,10 +618,11 @@ static int forward_raw(sid_psf *p, const float *V_dev, const int64_t vdims[3], f
     SID_TRY(ensure_plan(p, vdims[0], vdims[1], vdims[2]));
     Timer tm(p, st);
     const int e0 = tm.mark();
-    SID_TRY(launch_vol_in(p, V_dev, vdims[2], 0, st));
+    const unsigned long long stamp = next_stamp(p);
+    SID_TRY(launch_vol_in(p, V_dev, vdims[2], 0, st, stamp));
     SID_TRY(project_halves(p, 0, st, tm, 0, 5));
     const int e2 = tm.mark();
-    SID_TRY(launch_fwd_reduce(p, O_dev, st));
+    SID_TRY(launch_fwd_reduce(p, O_dev, st, stamp));
     const int e3 = tm.mark();
     tm.span(1, e2, e3);
     if (!sync) return SID_OK;
@@ -611,10 +641,11 @@ static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], 
     SID_TRY(ensure_plan(p, odims[0], odims[1], nz));
     Timer tm(p, st);
     const int e0 = tm.mark();
-    SID_TRY(launch_img_in(p, O_dev, st));
+    const unsigned long long stamp = next_stamp(p);
+    SID_TRY(launch_img_in(p, O_dev, st, stamp));
     SID_TRY(project_halves(p, 1, st, tm, 2, 6));
     const int e2 = tm.mark();
-    SID_TRY(launch_bwd_reduce(p, B_dev, 0, st));
+    SID_TRY(launch_bwd_reduce(p, B_dev, 0, st, stamp));
     const int e3 = tm.mark();
     tm.span(3, e2, e3);
     if (!sync) return SID_OK;
@@ -671,7 +702,9 @@ int sid_forward_device(sid_psf *p, const float *V, const int64_t vdims[3], float
     if (!p || !V || !O) { set_error("sid_forward_device: null argument"); return SID_ERR_ARG; }
     SID_TRY(check_vdims(p, vdims, false));
     SID_TRY_HIP(hipSetDevice(p->device));
-    return forward_impl(p, V, vdims, O, pick_stream(p, stream));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return forward_impl(p, V, vdims, O, st);
 }
 
 int sid_backward_device(sid_psf *p, const float *O, const int64_t odims[2], float *B, void *stream)
@@ -679,7 +712,9 @@ int sid_backward_device(sid_psf *p, const float *O, const int64_t odims[2], floa
     if (!p || !O || !B || !odims) { set_error("sid_backward_device: null argument"); return SID_ERR_ARG; }
     if (odims[0] < 1 || odims[1] < 1) { set_error("image dimensions must be positive"); return SID_ERR_DIMS; }
     SID_TRY_HIP(hipSetDevice(p->device));
-    return backward_impl(p, O, odims, B, pick_stream(p, stream));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return backward_impl(p, O, odims, B, st);
 }
 
 // Host-buffer variants: stage through device buffers kept on the handle
@@ -747,16 +782,21 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
     const int64_t nv = vdims[0] * vdims[1] * vdims[2];
     Timer tm(p, st);
     const int e0 = tm.mark();
+    // non-finite stamps: the iterate's (set by the staging, then by each update
+    // for the next forward) and the image's (set by the forward reduction)
+    unsigned long long s_vol = next_stamp(p);
     SID_TRY(launch_vol_in(p, Htf, vdims[2], 1, st));
-    SID_TRY(launch_vol_in(p, X, vdims[2], 0, st));
+    SID_TRY(launch_vol_in(p, X, vdims[2], 0, st, s_vol));
     for (int it = 0; it < max_iter; ++it) {
         SID_TRY(project_halves(p, 0, st, tm, 0, 5));
         const int b = tm.mark();
-        SID_TRY(launch_fwd_reduce(p, nullptr, st));
+        const unsigned long long s_img = next_stamp(p);
+        SID_TRY(launch_fwd_reduce(p, nullptr, st, s_vol, s_img));
         const int c = tm.mark();
         SID_TRY(project_halves(p, 1, st, tm, 2, 6));
         const int d = tm.mark();
-        SID_TRY(launch_bwd_reduce(p, hist ? hist + (int64_t)it * nv : nullptr, 1, st));
+        s_vol = next_stamp(p);
+        SID_TRY(launch_bwd_reduce(p, hist ? hist + (int64_t)it * nv : nullptr, 1, st, s_img, s_vol));
         const int e = tm.mark();
         tm.span(1, b, c);
         tm.span(3, d, e);
@@ -775,7 +815,9 @@ int sid_deconv_rl_device(sid_psf *p, const float *Htf, float *X, const int64_t v
     if (max_iter < 0) { set_error("max_iter must be >= 0"); return SID_ERR_ARG; }
     SID_TRY(check_vdims(p, vdims, true));
     SID_TRY_HIP(hipSetDevice(p->device));
-    return rl_impl(p, Htf, X, vdims, max_iter, hist, pick_stream(p, stream));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return rl_impl(p, Htf, X, vdims, max_iter, hist, st);
 }
 
 int sid_deconv_rl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3], int max_iter, float *hist)
@@ -855,7 +897,9 @@ int sid_library_forward_device(sid_psf *p, int64_t ns, const int64_t *boxes, con
         }
     }
     SID_TRY_HIP(hipSetDevice(p->device));
-    return library_forward(p, ns, boxes, vals, val_offsets, sensor[0], sensor[1], out, pick_stream(p, stream),
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return library_forward(p, ns, boxes, vals, val_offsets, sensor[0], sensor[1], out, st,
                            flags & 1);
 }
 
@@ -897,7 +941,9 @@ int sid_ls_deconv_device(sid_psf *p, const float *df, float *X, const int64_t vd
     SID_TRY(ls_opts(opts, &o));
     SID_TRY(check_vdims(p, vdims, true));
     SID_TRY_HIP(hipSetDevice(p->device));
-    return ls_deconv(p, df, X, vdims, o, pick_stream(p, stream));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return ls_deconv(p, df, X, vdims, o, st);
 }
 
 int sid_ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts *opts)

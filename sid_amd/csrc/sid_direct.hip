@@ -600,14 +600,21 @@ __device__ __forceinline__ void store_shadow(u16 *hi, u16 *lo, int64_t o, float 
 // plus BF16 hi/lo shadows when hi != nullptr) and, if O_col != nullptr, the
 // MATLAB column-major image.  Pixels beyond X, Y (the last partial lenslet)
 // are 0.
+// Non-finite input (stamp `check` in nf[0]): every sensor pixel is NaN, as one
+// NaN or Inf voxel makes forwardProjectGPU's FFT convolutions NaN everywhere
+// (conv2FFT_.m:14).  mark != 0: a non-finite output pixel stamps nf[1] (the
+// input of the backward that follows).
 template <typename T>
 __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t slab_stride, int NP, int nrows,
                              int n, int Ly, int YP, int T_, int S, int X, int Y, T *__restrict__ img,
                              u16 *__restrict__ hi, u16 *__restrict__ lo, float *__restrict__ O_col,
-                             const float2 *__restrict__ fft_img, int fft_ny)
+                             const float2 *__restrict__ fft_img, int fft_ny, unsigned long long *__restrict__ nf,
+                             unsigned long long check, unsigned long long mark)
 {
     const int nn2 = n * n;
     const int64_t total = (int64_t)nrows * nn2;
+    const bool poison = check != 0 && nf[0] == check;
+    bool bad = false;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(e / nn2), uv = (int)(e - (int64_t)r * nn2);
@@ -617,12 +624,15 @@ __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t sla
         const int Px = r / Ly, Py = r - Px * Ly;
         const int x = Px * n + uv % n, y = Py * n + uv / n;
         if (fft_img) s += (T)fft_img[(int64_t)x * fft_ny + y].x;   // planes on the FFT path
+        if (poison) s = (T)NAN;
         if (x >= X || y >= Y) s = 0;
+        bad = bad || !isfinite(s);
         const int64_t o = ((int64_t)(Px + T_) * YP + (Py + T_)) * S + uv;
         img[o] = s;
         if (hi) store_shadow(hi, lo, o, (float)s);
         if (O_col && x < X && y < Y) O_col[(int64_t)x + (int64_t)X * y] = (float)s;
     }
+    if (bad && mark) nf[1] = mark;
 }
 
 // Sum the backward slabs of each plane.  Without rl_update: write the MATLAB
@@ -642,7 +652,9 @@ __global__ __launch_bounds__(256) void k_bwd_reduce(const T *__restrict__ part, 
                                                     const T *__restrict__ htf, T *__restrict__ vol,
                                                     u16 *__restrict__ hi, u16 *__restrict__ lo,
                                                     const int *__restrict__ plane_fft,
-                                                    const float *__restrict__ dfft)
+                                                    const float *__restrict__ dfft,
+                                                    unsigned long long *__restrict__ nf,
+                                                    unsigned long long check, unsigned long long mark)
 {
     const int ab = threadIdx.x;
     if (ab >= n * n) return;
@@ -657,6 +669,7 @@ __global__ __launch_bounds__(256) void k_bwd_reduce(const T *__restrict__ part, 
         const int s0 = plane_slab0[c], s1 = plane_slab0[c + 1];
         for (int k = s0; k < s1; ++k) d += p[(int64_t)k * slab_stride];
     }
+    if (check != 0 && nf[1] == check) d = (T)NAN;   // non-finite image: NaN everywhere
     const int x = Px * n + ab % n, y = Py * n + ab / n;
     const bool real = x < X && y < Y;
     const int64_t col = (int64_t)x + (int64_t)X * ((int64_t)y + (int64_t)Y * c);
@@ -671,6 +684,7 @@ __global__ __launch_bounds__(256) void k_bwd_reduce(const T *__restrict__ part, 
         v = vol[o] * ratio;
         if (isnan(v)) v = 0;
     }
+    if (mark && !isfinite(v)) nf[0] = mark;   // an Inf iterate poisons the next forward
     vol[o] = v;
     if (hi) store_shadow(hi, lo, o, (float)v);
     if (B_col && real) B_col[col] = (float)v;
@@ -685,17 +699,22 @@ constexpr int kVinPx = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void k_vol_in_t(const float *__restrict__ V, int X, int Y, int n, int Lx,
                                                   int64_t XP, int64_t YP, int T_, int S, T *__restrict__ dst,
-                                                  u16 *__restrict__ hi, u16 *__restrict__ lo)
+                                                  u16 *__restrict__ hi, u16 *__restrict__ lo,
+                                                  unsigned long long *__restrict__ nf, unsigned long long mark)
 {
     __shared__ float tile[16 * 16 * kVinPx];   // n <= 16 rows of n * kVinPx values
     const int Py = blockIdx.x, Px0 = blockIdx.y * kVinPx, c = blockIdx.z;
     const int W = n * kVinPx, x0 = Px0 * n, y0 = Py * n;
     const float *src = V + (int64_t)c * X * Y;
+    bool bad = false;
     for (int e = threadIdx.x; e < n * W; e += blockDim.x) {
         const int r = e / W, col = e - r * W;
         const int x = x0 + col, y = y0 + r;
-        tile[e] = (x < X && y < Y) ? src[(int64_t)y * X + x] : 0.0f;
+        const float v = (x < X && y < Y) ? src[(int64_t)y * X + x] : 0.0f;
+        bad = bad || !isfinite(v);
+        tile[e] = v;
     }
+    if (bad && mark) nf[0] = mark;
     __syncthreads();
     const int n2 = n * n;
     for (int e = threadIdx.x; e < kVinPx * n2; e += blockDim.x) {
@@ -712,9 +731,11 @@ __global__ __launch_bounds__(256) void k_vol_in_t(const float *__restrict__ V, i
 
 template <typename T>
 __global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int n, int64_t XP, int64_t YP, int T_,
-                         int S, T *__restrict__ dst, u16 *__restrict__ hi, u16 *__restrict__ lo)
+                         int S, T *__restrict__ dst, u16 *__restrict__ hi, u16 *__restrict__ lo,
+                         unsigned long long *__restrict__ nf_slot, unsigned long long mark)
 {
     const int64_t total = (int64_t)X * Y * nz;
+    bool bad = false;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(e % X);
@@ -723,9 +744,11 @@ __global__ void k_vol_in(const float *__restrict__ V, int X, int Y, int nz, int 
         const int Px = x / n, Py = y / n, ab = (x - Px * n) + n * (y - Py * n);
         const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
         const float v = V[e];
+        bad = bad || !isfinite(v);
         if (dst) dst[o] = (T)v;
         if (hi) store_shadow(hi, lo, o, v);
     }
+    if (bad && mark) *nf_slot = mark;
 }
 
 // Padded lenslet-major volume -> MATLAB column-major volume.
@@ -1339,7 +1362,7 @@ int launch_gemm(sid_psf *p, int dir, hipStream_t st, int max_blocks)
     }
 }
 
-int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st)
+int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long long check, unsigned long long mark)
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
@@ -1355,17 +1378,18 @@ int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st)
         hipLaunchKernelGGL(k_fwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
                            (int)sp.X, (int)sp.Y, sp.img_d, (u16 *)nullptr, (u16 *)nullptr, O_col,
-                           (const float2 *)nullptr, 0);
+                           (const float2 *)nullptr, 0, p->nf, check, mark);
     else
         hipLaunchKernelGGL(k_fwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
                            (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr, O_col,
-                           sp.fft.active ? (const float2 *)sp.fft.grid : nullptr, sp.fft.Ny);
+                           sp.fft.active ? (const float2 *)sp.fft.grid : nullptr, sp.fft.Ny, p->nf, check, mark);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
-int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st)
+int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st, unsigned long long check,
+                      unsigned long long mark)
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
@@ -1376,19 +1400,19 @@ int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st)
         hipLaunchKernelGGL(k_bwd_reduce<double>, grid, dim3(256), 0, st, sp.part_d, sp.bwd.d_plane_slab0, stride,
                            (int)p->NP, (int)p->n, (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X,
                            (int)sp.Y, B_col, rl_update, sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr,
-                           (const int *)nullptr, (const float *)nullptr);
+                           (const int *)nullptr, (const float *)nullptr, p->nf, check, mark);
     else
         hipLaunchKernelGGL(k_bwd_reduce<float>, grid, dim3(256), 0, st, sp.part, sp.bwd.d_plane_slab0, stride,
                            (int)p->NP, (int)p->n, (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X,
                            (int)sp.Y, B_col, rl_update, sp.htf, sp.vol, sh ? sp.vol_hi : nullptr,
                            sh ? sp.vol_lo : nullptr, sp.fft.active ? sp.fft.d_plane_fft : nullptr,
-                           sp.fft.active ? sp.fft.dfft : nullptr);
+                           sp.fft.active ? sp.fft.dfft : nullptr, p->nf, check, mark);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
 // which: 0 = the iterate / forward input (with BF16 shadows), 1 = Htf.
-int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which, hipStream_t st)
+int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which, hipStream_t st, unsigned long long mark)
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
@@ -1399,16 +1423,16 @@ int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which, hipStre
     if (f64)
         hipLaunchKernelGGL(k_vol_in_t<double>, grid, dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y, (int)p->n,
                            (int)sp.Lx, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol_d : sp.htf_d,
-                           (u16 *)nullptr, (u16 *)nullptr);
+                           (u16 *)nullptr, (u16 *)nullptr, p->nf, mark);
     else
         hipLaunchKernelGGL(k_vol_in_t<float>, grid, dim3(256), 0, st, V_col, (int)sp.X, (int)sp.Y, (int)p->n,
                            (int)sp.Lx, sp.XP, sp.YP, (int)p->T, (int)p->S, which == 0 ? sp.vol : sp.htf,
-                           sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr);
+                           sh ? sp.vol_hi : nullptr, sh ? sp.vol_lo : nullptr, p->nf, mark);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
-int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st)
+int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st, unsigned long long mark)
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
@@ -1416,11 +1440,12 @@ int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st)
     const int64_t total = sp.X * sp.Y;
     if (f64)
         hipLaunchKernelGGL(k_vol_in<double>, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1,
-                           (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img_d, (u16 *)nullptr, (u16 *)nullptr);
+                           (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img_d, (u16 *)nullptr, (u16 *)nullptr,
+                           p->nf + 1, mark);
     else
         hipLaunchKernelGGL(k_vol_in<float>, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1,
                            (int)p->n, sp.XP, sp.YP, (int)p->T, (int)p->S, sp.img, sh ? sp.img_hi : nullptr,
-                           sh ? sp.img_lo : nullptr);
+                           sh ? sp.img_lo : nullptr, p->nf + 1, mark);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }

@@ -198,6 +198,14 @@ struct sid_psf {
     int gemm_blocks = 0;              // grid cap of the GEMM beside the FFT path (0: one block per CU)
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_null = nullptr;     // orders the handle's stream after the default stream (stream = NULL)
+    // Non-finite inputs (forwardProjectGPU's rule, conv2FFT_.m:14: one NaN or Inf
+    // anywhere in a projection's input makes the whole output NaN, for every
+    // plane method).  nf[0] / nf[1]: the stamp of the last forward / backward
+    // input that held a non-finite value.  Every staged input gets a fresh stamp
+    // from nf_epoch (never reused), so the flags are never cleared.
+    unsigned long long *nf = nullptr;
+    unsigned long long nf_epoch = 0;
 };
 
 namespace sid {
@@ -207,10 +215,18 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz);
 int ensure_precision_buffers(sid_psf *p);
 // max_blocks > 0 caps the grid (blocks then loop over the items)
 int launch_gemm(sid_psf *p, int dir, hipStream_t st, int max_blocks = 0);
-int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st);
-int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st);
-int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which /*0 iterate, 1 Htf*/, hipStream_t st);
-int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st);
+// Non-finite stamps (sid_psf::nf): `mark` != 0 stamps the flag the kernel's
+// output feeds when it sees a non-finite value (staging: its input); `check`
+// != 0 makes the reduction write NaN to every sensor pixel / voxel when the
+// input's flag carries that stamp.  0 turns either off.
+int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long long check = 0,
+                      unsigned long long mark = 0);
+int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st, unsigned long long check = 0,
+                      unsigned long long mark = 0);
+int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which /*0 iterate, 1 Htf*/, hipStream_t st,
+                  unsigned long long mark = 0);
+int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st, unsigned long long mark = 0);
+inline unsigned long long next_stamp(sid_psf *p) { return ++p->nf_epoch; }
 int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st);
 void free_plan_buffers(SensorPlan &sp);
 int support_scan(sid_psf *p);

@@ -19,6 +19,7 @@ import ctypes
 import hashlib
 import json
 import os
+import weakref
 
 import numpy as np
 
@@ -76,11 +77,18 @@ def project_vjp(psf, V, G, want_gV: bool = True, want_gH: bool = True):
 
 
 def _sync_psf(psf, H):
-    """Keep the handle's H equal to the tensor H (rebuild weights on change)."""
-    key = (H.data_ptr(), H._version)
-    if getattr(psf, "_h_key", None) != key:
+    """Keep the handle's H equal to the tensor H (rebuild weights on change).
+
+    The handle is current when H is the very tensor it was last built from
+    (held by a weak reference, so a new tensor that reuses a freed block's
+    address never matches) and H has not been modified in place since (its
+    version counter).  Writes through ``H.data`` bypass the version counter;
+    call ``psf.update(H)`` after them."""
+    ref = getattr(psf, "_h_ref", None)
+    if ref is None or ref() is not H or psf._h_version != H._version:
         psf.update(H)
-        psf._h_key = key
+        psf._h_ref = weakref.ref(H)
+        psf._h_version = H._version
 
 
 def _make_functions():
@@ -164,7 +172,7 @@ def load_checkpoint(path: str, device):
 
 
 def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf: bool = True,
-           resume: bool = True, stop_after: int | None = None):
+           resume: bool = True, stop_after: int | None = None, checkpoint_every: int = 1):
     """Poisson-EM (Richardson-Lucy) refinement of volumes and PSF.
 
     Each step is two EM half-steps, each of which never increases the NLL:
@@ -173,7 +181,8 @@ def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf:
                                  G1 = d(sum F)/dH at V_new      (if refine_psf)
     B(1) is the back-projected normalisation (recomputed when H changes).
     The NLL recorded for a step is the one at the start of the step.  After
-    each step the state is checkpointed atomically to ``checkpoint``;
+    each step (every ``checkpoint_every`` steps, and after the last or the
+    ``stop_after`` step) the state is checkpointed atomically to ``checkpoint``;
     ``stop_after`` ends the loop early (to test resumption).
     Returns (V, H, nll_history).
     """
@@ -208,8 +217,9 @@ def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf:
                 H = torch.clamp(H * (1.0 - rh), min=0.0).contiguous()
             psf.update(H)
             B1, _ = project_vjp(psf, None, ones[:1], True, False)
-        if checkpoint:
+        stop = stop_after is not None and step + 1 >= stop_after
+        if checkpoint and ((step + 1) % max(1, int(checkpoint_every)) == 0 or step + 1 == steps or stop):
             save_checkpoint(checkpoint, step + 1, V, H, hist, cfg)
-        if stop_after is not None and step + 1 >= stop_after:
+        if stop:
             break
     return V, H, hist

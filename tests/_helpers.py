@@ -42,6 +42,18 @@ def c_forward(H, V, CA=None, threads=8):
     return O
 
 
+def c_forward_h32(H32, V, CA=None, threads=8):
+    """C oracle forward with an FP32 PSF and FP64 arithmetic (no FP64 copy of H)."""
+    Hf = np.asfortranarray(H32, dtype=np.float32)
+    Vf = np.asfortranarray(np.asarray(V, dtype=np.float64).reshape(V.shape[0], V.shape[1], -1))
+    X, Y, nz = Vf.shape
+    O = np.zeros((X, Y), order="F")
+    ca = None if CA is None else np.asfortranarray(np.asarray(CA, dtype=np.int32))
+    orc().orc_forward_h32(_p(Hf), _i64(*Hf.shape), None if ca is None else _p(ca), _p(Vf), ctypes.c_int64(X),
+                          ctypes.c_int64(Y), ctypes.c_int64(nz), _p(O), ctypes.c_int(threads))
+    return O
+
+
 def c_backward(H, O, CA=None, threads=8):
     Hf = np.asfortranarray(H, dtype=np.float64)
     Of = np.asfortranarray(O, dtype=np.float64)

@@ -160,6 +160,31 @@ def test_autograd_functions(env):
     assert torch.equal(Hr2.grad, gH2)
 
 
+def test_autograd_follows_a_new_psf_tensor_at_a_reused_address(env):
+    """An update loop makes a new H each step (H = H - lr * g); the caching
+    allocator often hands it the freed block of an older H.  The handle must
+    follow the tensor, not its address."""
+    torch, sid_amd, grad = env
+    print("case: autograd with H tensors at a reused address", flush=True)
+    Project, _ = grad.functions()
+    H, V, y = _case(seed=6)
+    Ht, Vt, _ = _to_dev(torch, H, V, y)
+    psf = sid_amd.PSF(Ht, full_support=True)
+    H1 = Ht.clone()
+    out1 = Project.apply(Vt, H1, psf)
+    ptr = H1.data_ptr()
+    del H1
+    H2 = torch.empty_like(Ht)
+    H2.copy_(Ht * 1.5)
+    print("address reused:", H2.data_ptr() == ptr, flush=True)
+    out2 = Project.apply(Vt, H2, psf)
+    assert rel_l2(out2.cpu().numpy(), 1.5 * out1.cpu().numpy()) < 1e-6
+    # an in-place change of the same tensor is seen through its version counter
+    H2.mul_(2.0)
+    out3 = Project.apply(Vt, H2, psf)
+    assert rel_l2(out3.cpu().numpy(), 3.0 * out1.cpu().numpy()) < 1e-6
+
+
 def test_refinement_checkpoint_resume_bit_exact(env, tmp_path):
     torch, sid_amd, grad = env
     print("case: refinement 20 steps, resume after 10", flush=True)

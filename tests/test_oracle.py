@@ -460,6 +460,19 @@ def test_library_gpu_placement_hand_worked():
     assert (p["x0"], p["x1"], p["y0"], p["y1"]) == (17, 20, 16, 20)
 
 
+def test_patched_rect_hand_worked():
+    """project_forward_patched.m:8-24,54-70 worked by hand."""
+    # pixels (10..12, 11..13), n 3, M 7: origin 9; n_ml = ceil(max/3) - 3 = (1, 2) -> (1, 3);
+    # 'full' 9 x 15 centred at 9 + ceil(n_ml/2*3) = (11, 14) -> x 7..15, y 7..21, patch 1..9, 1..15
+    g = O.patched_rect((10, 11, 1), (12, 13, 3), (30, 30), 3, 7)
+    assert (g["x0"], g["x1"], g["y0"], g["y1"]) == (7, 15, 7, 21)
+    assert (g["p0"], g["p1"], g["q0"], g["q1"]) == (1, 9, 1, 15)
+    assert list(g["n_ml"]) == [1, 3] and (g["off_z"], g["n_z"]) == (0, 3)
+    # at the sensor corner: centre 2, 'full' 9 wide -> x 1..6 (clipped), patch rows 4..9
+    g = O.patched_rect((1, 2, 1), (2, 2, 1), (10, 10), 3, 7)
+    assert (g["x0"], g["x1"], g["p0"], g["p1"]) == (1, 6, 4, 9)
+
+
 def test_project_forward_patched_matches_full_sensor_projection():
     """The patched CPU library projection equals the full-sensor forward
     projection of the same voxels wherever the patch is pasted, for a patch
