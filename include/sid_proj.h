@@ -225,6 +225,17 @@ int sid_std_image_device(const float *Y_dev, int64_t P, int64_t T, int bg_iter, 
                          float *t_dev, float *std_dev, double *ymax, void *stream);
 int sid_std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t, float *std_img,
                   double *ymax);
+/* sid_std_image_device without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under stream capture,
+ * serving loops, harnesses that forbid hipMalloc).  The same kernels and
+ * results as sid_std_image_device; its work space (block partial sums, the FP64
+ * s and t vectors, per-pixel sums: about 24 B per pixel + 8 B per frame and
+ * pixel chunk) lives in a 32 MiB array the library holds on every device it
+ * runs on until the process ends (enough for 1005^2 px x 1000 frames).  Larger
+ * movies are rejected with SID_ERR_DIMS.  Every call on one device shares that
+ * array: calls that overlap on one device race. */
+int sid_std_image_device_noalloc(const float *Y_dev, int64_t P, int64_t T, int bg_iter, int flags, float *s_dev,
+                                 float *t_dev, float *std_dev, double *ymax, void *stream);
 
 /* --- LS_deconv (external/LFrecon_util/LS_deconv.m:1-97) ------------------
  * Projected gradient descent with exact line search for

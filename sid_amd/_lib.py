@@ -52,6 +52,7 @@ SIGNATURES = {
     "sid_convn_same": (_I, [_P, _I64P, _P, _I64P, _I, _P]),
     "sid_std_image_device": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _I, _I, _P, _P, _P, _P, _P]),
     "sid_std_image": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _I, _I, _P, _P, _P, _P]),
+    "sid_std_image_device_noalloc": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _I, _I, _P, _P, _P, _P, _P]),
     "sid_ls_deconv": (_I, [_P, _P, _P, _I64P, _P]),
     "sid_ls_deconv_device": (_I, [_P, _P, _P, _I64P, _P, _P]),
     "sid_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),

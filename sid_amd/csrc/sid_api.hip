@@ -1064,7 +1064,25 @@ int sid_std_image_device(const float *Y, int64_t P, int64_t T, int bg_iter, int 
     if (P < 1 || T < 1 || T > (int64_t)1 << 30) { set_error("movie dimensions out of range"); return SID_ERR_DIMS; }
     if (bg_iter < 0) { set_error("bg_iter must be >= 0"); return SID_ERR_ARG; }
     SID_TRY(check_device());
-    return std_image(Y, P, T, bg_iter, flags, s, t, std_img, ymax, (hipStream_t)stream);
+    DevBuf work;
+    SID_TRY_HIP(hipMalloc(&work.ptr, mv_work_bytes(P, T, nullptr, nullptr)));
+    return std_image(Y, P, T, bg_iter, flags, s, t, std_img, ymax, (hipStream_t)stream, work.ptr);
+}
+
+int sid_std_image_device_noalloc(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t,
+                                 float *std_img, double *ymax, void *stream)
+{
+    if (!Y || !std_img) { set_error("sid_std_image_device_noalloc: null argument"); return SID_ERR_ARG; }
+    if (P < 1 || T < 1 || T > (int64_t)1 << 30) { set_error("movie dimensions out of range"); return SID_ERR_DIMS; }
+    if (bg_iter < 0) { set_error("bg_iter must be >= 0"); return SID_ERR_ARG; }
+    const size_t need = mv_work_bytes(P, T, nullptr, nullptr);
+    if (need > mv_scratch_bytes()) {
+        set_error("movie of %lld px x %lld frames needs %zu bytes of work space; the allocation-free entry holds %zu",
+                  (long long)P, (long long)T, need, mv_scratch_bytes());
+        return SID_ERR_DIMS;
+    }
+    SID_TRY(check_device());
+    return std_image_noalloc(Y, P, T, bg_iter, flags, s, t, std_img, ymax, (hipStream_t)stream);
 }
 
 int sid_std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t, float *std_img,
@@ -1079,9 +1097,11 @@ int sid_std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, 
     SID_TRY_HIP(hipMalloc(&ds.ptr, sizeof(float) * P));
     SID_TRY_HIP(hipMalloc(&dt.ptr, sizeof(float) * T));
     SID_TRY_HIP(hipMalloc(&dstd.ptr, sizeof(float) * P));
+    DevBuf work;
+    SID_TRY_HIP(hipMalloc(&work.ptr, mv_work_bytes(P, T, nullptr, nullptr)));
     SID_TRY(copy_h2d(dy.ptr, Y, sizeof(float) * P * T, nullptr));
     SID_TRY(std_image((const float *)dy.ptr, P, T, bg_iter, flags, (float *)ds.ptr, (float *)dt.ptr,
-                      (float *)dstd.ptr, ymax, nullptr));
+                      (float *)dstd.ptr, ymax, nullptr, work.ptr));
     SID_TRY_HIP(hipMemcpy(std_img, dstd.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
     if (s) SID_TRY_HIP(hipMemcpy(s, ds.ptr, sizeof(float) * P, hipMemcpyDeviceToHost));
     if (t) SID_TRY_HIP(hipMemcpy(t, dt.ptr, sizeof(float) * T, hipMemcpyDeviceToHost));

@@ -277,8 +277,15 @@ int convn_build_frag(const float *k_dev, const int kd[3], u16 **frag, hipStream_
 int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const int kd[3], int precision, float *out,
                hipStream_t st);
 // sid_movie.hip: movie background (rank 1) and std image
+// work: mv_work_bytes(P, T, nullptr, nullptr) bytes of device memory
+struct MvWork;
+size_t mv_work_bytes(int64_t P, int64_t T, MvWork *w, void *base);
 int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,
-              float *std_img, double *ymax_out, hipStream_t st);
+              float *std_img, double *ymax_out, hipStream_t st, void *work);
+// the same with the work space in the library's per-device scratch array (no allocation)
+int std_image_noalloc(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,
+                      float *std_img, double *ymax_out, hipStream_t st);
+size_t mv_scratch_bytes();
 // sid_solvers.hip
 int ls_deconv(sid_psf *p, const float *df, float *X, const int64_t vdims[3], const sid_ls_opts &o, hipStream_t st);
 // deconvRL basic on column-major device buffers through forward_dev / backward_dev (neuron-kernel handles)

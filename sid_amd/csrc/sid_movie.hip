@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "sid_internal.h"
@@ -190,22 +191,45 @@ __global__ void k_mv_std(const double *__restrict__ A, const double *__restrict_
 
 static int grid_for(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
 
-template <typename T> struct MvBuf {
-    T *p = nullptr;
-    ~MvBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    int alloc(int64_t n)
-    {
-        const hipError_t e = hipMalloc(&p, sizeof(T) * std::max<int64_t>(n, 1));
-        if (e != hipSuccess) {
-            set_error("hipMalloc failed: %s", hipGetErrorString(e));
-            return SID_ERR_OOM;
-        }
-        return SID_OK;
-    }
+// Column-pass geometry: groups of kColGroup frames by pixel chunks, about
+// 16 K blocks in all (many rounds of the ~2 K resident ones, so no half-empty
+// last round).
+static void col_geometry(int64_t P, int64_t T, int64_t *ngroups, int64_t *nchunk, int64_t *pc)
+{
+    *ngroups = (T + kColGroup - 1) / kColGroup;
+    *nchunk = std::max<int64_t>(1, std::min<int64_t>((16384 + *ngroups - 1) / *ngroups, (P + 1023) / 1024));
+    *pc = (P + *nchunk - 1) / *nchunk;
+}
+
+// Device work space of one call (partial sums, the FP64 s and t vectors and
+// per-pixel sums), carved out of one buffer at 256-byte boundaries.
+struct MvWork {
+    float *fpart = nullptr, *mpart = nullptr;
+    double *dpart = nullptr, *A = nullptr, *Sm = nullptr, *sd = nullptr, *td = nullptr;
 };
+
+// Bytes of work space for (P, T); with base != nullptr also sets the pointers.
+size_t mv_work_bytes(int64_t P, int64_t T, MvWork *w, void *base)
+{
+    int64_t ngroups, nchunk, pc;
+    col_geometry(P, T, &ngroups, &nchunk, &pc);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return base ? (void *)((char *)base + o) : nullptr;
+    };
+    MvWork t;
+    t.fpart = (float *)take(sizeof(float) * 4096);
+    t.dpart = (double *)take(sizeof(double) * 4096);
+    t.A = (double *)take(sizeof(double) * P);
+    t.Sm = (double *)take(sizeof(double) * P);
+    t.sd = (double *)take(sizeof(double) * P);
+    t.td = (double *)take(sizeof(double) * std::max<int64_t>(T, nchunk * T));
+    t.mpart = (float *)take(sizeof(float) * ngroups * nchunk);
+    if (w) *w = t;
+    return off;
+}
 
 static int host_sumsq(const double *x, int64_t n, double *part, hipStream_t st, double *out)
 {
@@ -221,23 +245,23 @@ static int host_sumsq(const double *x, int64_t n, double *part, hipStream_t st, 
     return SID_OK;
 }
 
+// work: mv_work_bytes(P, T) bytes of device memory (every value in it is
+// written before it is read within the call).
 int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,
-              float *std_img, double *ymax_out, hipStream_t st)
+              float *std_img, double *ymax_out, hipStream_t st, void *work)
 {
+    MvWork w;
+    mv_work_bytes(P, T, &w, work);
     // 1. global maximum (sid_main.m:88-89), when normalising; with background
     // iterations it comes out of the first column pass (below)
     double scale = 1.0, ymax = 0.0;
-    MvBuf<float> fpart;
-    MvBuf<double> dpart, sd, td, A, Sm;
-    SID_TRY(fpart.alloc(4096));
-    SID_TRY(dpart.alloc(4096));
     const bool fuse_max = (flags & 1) && bg_iter > 0;
     if ((flags & 1) && !fuse_max) {
         const int nb = grid_for(P * T);
-        hipLaunchKernelGGL(k_mv_max, dim3(nb), dim3(kMvThreads), 0, st, Y, P * T, fpart.p);
+        hipLaunchKernelGGL(k_mv_max, dim3(nb), dim3(kMvThreads), 0, st, Y, P * T, w.fpart);
         SID_TRY_HIP(hipGetLastError());
         std::vector<float> h(nb);
-        SID_TRY_HIP(hipMemcpyAsync(h.data(), fpart.p, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
+        SID_TRY_HIP(hipMemcpyAsync(h.data(), w.fpart, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
         SID_TRY_HIP(hipStreamSynchronize(st));
         float m = -INFINITY;
         for (float v : h) m = std::max(m, v);
@@ -249,21 +273,12 @@ int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, floa
         scale = 1.0 / ymax;
     }
     if (ymax_out) *ymax_out = ymax;
-    SID_TRY(A.alloc(P));
-    SID_TRY(Sm.alloc(P));
     double s_scale = 0.0, C = 0.0;
     std::vector<double> th(T, 0.0);
     if (bg_iter > 0) {
-        // pixel chunks for the column pass: about 2048 blocks in all
-        const int64_t ngroups = (T + kColGroup - 1) / kColGroup;
-        // ~16 K blocks: many rounds of the ~2 K resident ones, so no half-empty last round
-        const int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>((16384 + ngroups - 1) / ngroups, (P + 1023) / 1024));
-        const int64_t pc = (P + nchunk - 1) / nchunk;
+        int64_t ngroups, nchunk, pc;
+        col_geometry(P, T, &ngroups, &nchunk, &pc);
         std::vector<double> ph(nchunk * T);
-        MvBuf<float> mpart;
-        if (fuse_max) SID_TRY(mpart.alloc(ngroups * nchunk));
-        SID_TRY(sd.alloc(P));
-        SID_TRY(td.alloc(std::max<int64_t>(T, nchunk * T)));
         // s = ones(P, 1) / sqrt(P) (rank_1_factorization.m:31)
         double s_mul = 1.0 / std::sqrt((double)P);
         bool s_const = true;
@@ -274,13 +289,13 @@ int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, floa
             // kernel would apply (identical products)
             const bool find_max = fuse_max && it == 1;
             hipLaunchKernelGGL(k_mv_col, dim3((unsigned)ngroups, (unsigned)nchunk), dim3(kMvThreads), 0, st, Y, P, T,
-                               s_const ? (const double *)nullptr : sd.p, find_max ? 1.0 : s_mul * scale, pc, td.p,
-                               find_max ? mpart.p : (float *)nullptr);
+                               s_const ? (const double *)nullptr : w.sd, find_max ? 1.0 : s_mul * scale, pc, w.td,
+                               find_max ? w.mpart : (float *)nullptr);
             SID_TRY_HIP(hipGetLastError());
-            SID_TRY_HIP(hipMemcpyAsync(ph.data(), td.p, sizeof(double) * nchunk * T, hipMemcpyDeviceToHost, st));
+            SID_TRY_HIP(hipMemcpyAsync(ph.data(), w.td, sizeof(double) * nchunk * T, hipMemcpyDeviceToHost, st));
             if (find_max) {
                 std::vector<float> h(ngroups * nchunk);
-                SID_TRY_HIP(hipMemcpyAsync(h.data(), mpart.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost, st));
+                SID_TRY_HIP(hipMemcpyAsync(h.data(), w.mpart, sizeof(float) * h.size(), hipMemcpyDeviceToHost, st));
                 SID_TRY_HIP(hipStreamSynchronize(st));
                 float m = -INFINITY;
                 for (float v : h) m = std::max(m, v);
@@ -304,17 +319,17 @@ int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, floa
             }
             nt = std::sqrt(nt);
             for (double &v : th) v /= nt;   // 0 / 0 = NaN, as MATLAB
-            SID_TRY_HIP(hipMemcpyAsync(td.p, th.data(), sizeof(double) * T, hipMemcpyHostToDevice, st));
+            SID_TRY_HIP(hipMemcpyAsync(w.td, th.data(), sizeof(double) * T, hipMemcpyHostToDevice, st));
             // s = Y * t' (:35), with the std-image sums in the last iteration
             const bool last = it == bg_iter;
             hipLaunchKernelGGL(k_mv_row, dim3((unsigned)((P + kMvThreads - 1) / kMvThreads)), dim3(kMvThreads), 0, st,
-                               Y, P, T, (const double *)td.p, scale, sd.p, last ? A.p : (double *)nullptr,
-                               last ? Sm.p : (double *)nullptr);
+                               Y, P, T, (const double *)w.td, scale, w.sd, last ? w.A : (double *)nullptr,
+                               last ? w.Sm : (double *)nullptr);
             SID_TRY_HIP(hipGetLastError());
             s_const = false;
             if (!last) {   // s = s / norm(s) (:36-38), folded into the next column pass
                 double ns = 0.0;
-                SID_TRY(host_sumsq(sd.p, P, dpart.p, st, &ns));
+                SID_TRY(host_sumsq(w.sd, P, w.dpart, st, &ns));
                 s_mul = 1.0 / std::sqrt(ns);
             }
         }
@@ -324,11 +339,11 @@ int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, floa
         C = sum_t / (double)T;
     } else {
         hipLaunchKernelGGL(k_mv_row, dim3((unsigned)((P + kMvThreads - 1) / kMvThreads)), dim3(kMvThreads), 0, st, Y,
-                           P, T, (const double *)nullptr, scale, (double *)nullptr, A.p, Sm.p);
+                           P, T, (const double *)nullptr, scale, (double *)nullptr, w.A, w.Sm);
         SID_TRY_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_mv_std, dim3(grid_for(P)), dim3(256), 0, st, A.p, Sm.p,
-                       bg_iter > 0 ? (const double *)sd.p : (const double *)nullptr, s_scale, P, T, C, std_img,
+    hipLaunchKernelGGL(k_mv_std, dim3(grid_for(P)), dim3(256), 0, st, w.A, w.Sm,
+                       bg_iter > 0 ? (const double *)w.sd : (const double *)nullptr, s_scale, P, T, C, std_img,
                        s_out);
     SID_TRY_HIP(hipGetLastError());
     if (t_out) {
@@ -338,6 +353,47 @@ int std_image(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, floa
     }
     SID_TRY_HIP(hipStreamSynchronize(st));
     return SID_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Work space without allocation (sid_std_image_device_noalloc): a file-scope
+// device array, one copy per device, looked up once per device.  Calls that
+// overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kMvScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kMvScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+size_t mv_scratch_bytes() { return kMvScratchBytes; }
+
+int std_image_noalloc(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s_out, float *t_out,
+                      float *std_img, double *ymax_out, hipStream_t st)
+{
+    const size_t need = mv_work_bytes(P, T, nullptr, nullptr);
+    if (need > kMvScratchBytes) {
+        set_error("movie of %lld px x %lld frames needs %zu bytes of work space; the allocation-free entry holds %zu",
+                  (long long)P, (long long)T, need, kMvScratchBytes);
+        return SID_ERR_DIMS;
+    }
+    int dev = 0;
+    SID_TRY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d beyond the %d devices the allocation-free entry supports", dev, kMaxDevices);
+        return SID_ERR_NODEV;
+    }
+    void *ptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[dev]) {
+            void *q = nullptr;
+            SID_TRY_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+            g_scratch[dev] = q;
+        }
+        ptr = g_scratch[dev];
+    }
+    return std_image(Y, P, T, bg_iter, flags, s_out, t_out, std_img, ymax_out, st, ptr);
 }
 
 }  // namespace sid
