@@ -189,6 +189,27 @@ int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *box
  * skipped), accumulator elements} of the cached library plan. */
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
 
+/* --- Library of normalised footprints (src/generate_LFM_library_CPU.m:1-70) ---
+ * forward_model = generate_LFM_library_CPU(centers, psf_ballistic, r, dim, m, kernel):
+ * centers: n_seeds x 3 column-major doubles (MATLAB centers, 1-based x, y, z);
+ * kernel == NULL: the ball of whole radius r with axial radius r / dim[2]
+ * (:18,34-35); kernel [k, k, kz] column-major (kdims): its nonzero entries and
+ * values (:29-32).  m = {m(1), m(2)} (the volume's lateral size).  Each centre's
+ * voxels inside the volume (:43-50) are projected with the handle's PSF as
+ * project_forward_patched does (:51) and divided by their L2 norm (:52-53).
+ * Output: the rows of sparse(I, J, S, n_seeds, m(1)*m(2)) (:59-70) in
+ * compressed-row form: row_ptr (n_seeds + 1), cols (0-based column-major pixel
+ * indices, ascending within a row) and vals (FP64), at most cap entries
+ * (sid_library_nsf_bound gives a sufficient cap).  A centre with no voxel inside
+ * the volume, or a zero projection, gives an empty row (the reference fails in
+ * project_forward_patched, or returns NaN entries).  The handle must be made
+ * without CAindex and have odd Nnum and PSF side. */
+int sid_library_nsf_bound(const sid_psf *psf, int64_t n_seeds, const double *centers, double r, const double dim[3],
+                          const int64_t m[2], const float *kernel, const int64_t kdims[3], int64_t *bound);
+int sid_library_nsf(sid_psf *psf, int64_t n_seeds, const double *centers, double r, const double dim[3],
+                    const int64_t m[2], const float *kernel, const int64_t kdims[3], int64_t *row_ptr, int64_t *cols,
+                    double *vals, int64_t cap);
+
 /* --- Neuron kernel (reconstruct_S.m:110-128,184-190; generate_kernel.m) ----
  * With a kernel k set, the handle's projectors become
  *   forwardFUN(V) = F(convn(V, k, 'same')),  backwardFUN(O) = convn(B(O), k, 'same')

@@ -453,8 +453,10 @@ def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
     """src/project_forward_patched.m:1-76 in FP64 (1-based pixel triples).
 
     Patch origin ``floor((px_min-1)/n)*n`` (:8-12), lenslet count forced odd
-    (:13-20), ``conv2(., 'full')`` per (i, j, k) (:37-49), centre
-    ``offset + ceil((n_ml/2)*n)`` (:56-57), clipped paste (:61-73)."""
+    (:13-20), ``conv2(., 'full')`` per (i, j, k) (:37-49) as the direct sum,
+    centre ``offset + ceil((n_ml/2)*n)`` (:56-57), clipped paste (:61-73).
+    The patch origin is a multiple of n, so patch row px has sublattice
+    phase px mod n (the tempspace(i:n:end, j:n:end) of :41)."""
     pixels = np.asarray(pixels, dtype=np.int64)
     g = patched_rect(pixels.min(axis=0), pixels.max(axis=0), volume_n_px, n, H.shape[0])
     off_px, n_ml, off_z, n_z = g["off_px"], g["n_ml"], g["off_z"], g["n_z"]
@@ -463,15 +465,12 @@ def project_forward_patched(pixels, volume_n_px, n, H, colv, idx):
         vol[pixels[t, 0] - off_px[0] - 1, pixels[t, 1] - off_px[1] - 1, pixels[t, 2] - off_z - 1] = \
             colv[idx[t, 0] - 1, idx[t, 1] - 1, idx[t, 2] - 1]
     M = H.shape[0]
+    # conv2(tempspace, H(:,:,i,j,offset_z+k), 'full') summed over the sublattices
+    # (i, j) and planes k: the patch value at (px, py) adds its kernel at
+    # [px, px + M) x [py, py + M) (MATLAB's direct sum: exact zeros elsewhere)
     tot = np.zeros((vol.shape[0] + M - 1, vol.shape[1] + M - 1))
-    from scipy.signal import fftconvolve
-    for i in range(n):
-        for j in range(n):
-            for k in range(vol.shape[2]):
-                t = np.zeros(vol.shape[:2])
-                t[i::n, j::n] = vol[i::n, j::n, k]
-                if np.any(t):
-                    tot += fftconvolve(t, H[:, :, i, j, off_z + k].astype(np.float64), mode="full")
+    for px, py, k in zip(*np.nonzero(vol)):
+        tot[px:px + M, py:py + M] += vol[px, py, k] * H[:, :, px % n, py % n, off_z + k].astype(np.float64)
     img = np.zeros((int(volume_n_px[0]), int(volume_n_px[1])))
     img[g["x0"] - 1:g["x1"], g["y0"] - 1:g["y1"]] = tot[g["p0"] - 1:g["p1"], g["q0"] - 1:g["q1"]]
     return img, g

@@ -47,6 +47,9 @@ SIGNATURES = {
     "sid_library_forward_device": (_I, [_P, ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P, _I, _P]),
     "sid_library_stats": (_I, [_P, _I64P]),
     "sid_patched_rect": (_I, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]),
+    "sid_library_nsf_bound": (_I, [_P, ctypes.c_int64, _P, ctypes.c_double, _P, _I64P, _P, _I64P, _I64P]),
+    "sid_library_nsf": (_I, [_P, ctypes.c_int64, _P, ctypes.c_double, _P, _I64P, _P, _I64P, _I64P, _I64P, _P,
+                             ctypes.c_int64]),
     "sid_psf_set_kernel": (_I, [_P, _P, _I64P]),
     "sid_convn_same_device": (_I, [_P, _I64P, _P, _I64P, _I, _P, _P]),
     "sid_convn_same": (_I, [_P, _I64P, _P, _I64P, _I, _P]),

@@ -225,7 +225,7 @@ void patched_rect(int64_t n, int64_t M, int64_t X, int64_t Y, const int64_t box[
 
 struct LibPlan {
     // key
-    std::vector<int64_t> boxes;
+    std::vector<int64_t> boxes, val_off;
     int64_t X = 0, Y = 0;
     const float *vals = nullptr;
     float *out = nullptr;
@@ -294,12 +294,14 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
     LibPlan *L = (LibPlan *)p->lib;
     const bool same = L && L->X == X && L->Y == Y && L->vals == vals && L->out == out && L->n_seeds == ns &&
                       L->boxes.size() == (size_t)ns * 6 &&
-                      std::memcmp(L->boxes.data(), boxes, sizeof(int64_t) * 6 * ns) == 0;
+                      std::memcmp(L->boxes.data(), boxes, sizeof(int64_t) * 6 * ns) == 0 &&
+                      std::memcmp(L->val_off.data(), val_off, sizeof(int64_t) * ns) == 0;
     if (!same) {
         lib_plan_free(p);
         L = new LibPlan();
         p->lib = L;
         L->boxes.assign(boxes, boxes + 6 * ns);
+        L->val_off.assign(val_off, val_off + ns);
         L->X = X; L->Y = Y; L->vals = vals; L->out = out; L->n_seeds = ns;
         std::vector<LibSeed> seeds(ns);
         std::vector<int64_t> pix_off(ns + 1, 0);

@@ -1,5 +1,7 @@
 """Library generation around the batched HIP projector.
 
+``generate_LFM_library_CPU`` builds the normalised ball / kernel footprints of
+src/generate_LFM_library_CPU.m on the same batched projection.
 ``placement`` restates the integer seed placement of
 src/generate_LFM_library_GPU.m:68-88 (MATLAB ``round`` is half away from
 zero); ``patched_rect`` calls the library's own rectangle rule of
@@ -102,6 +104,42 @@ class Library:
         if hasattr(flat, "cpu"):
             flat = flat.cpu().numpy()
         return flat.reshape((w, h), order="F")
+
+
+def generate_LFM_library_CPU(centers, psf, r, dim, m, kernel=None):
+    """src/generate_LFM_library_CPU.m:1 on the device: the L2-normalised
+    footprint of every centre (a ball of radius ``r`` with axial radius
+    ``r / dim[2]``, or the nonzeros of ``kernel``) in a volume of lateral size
+    ``m[:2]``, projected with ``psf`` (made without CAindex).
+
+    Returns (row_ptr, cols, vals): the rows of the reference's
+    ``sparse(I, J, S, N, prod(m(1:2)))`` in compressed-row form (0-based
+    column-major pixel indices, ascending per row; FP64 values).
+    ``scipy.sparse.csr_matrix((vals, cols, row_ptr), shape=(N, m[0]*m[1]))``
+    builds the matrix."""
+    c = np.asfortranarray(np.asarray(centers, dtype=np.float64).reshape(-1, 3))
+    ns = c.shape[0]
+    dm = (ctypes.c_double * 3)(*[float(v) for v in (dim if dim is not None else (1.0, 1.0, 1.0))])
+    mm = i64(int(m[0]), int(m[1]))
+    if kernel is not None:
+        k = np.asfortranarray(np.asarray(kernel, dtype=np.float32))
+        if k.ndim == 2:
+            k = k[:, :, None]
+        kp, kd = k.ctypes.data_as(ctypes.c_void_p), i64(*k.shape)
+    else:
+        kp, kd = None, None
+    rv = float(r) if r is not None else 0.0
+    bound = ctypes.c_int64(0)
+    cp = c.ctypes.data_as(ctypes.c_void_p)
+    check(lib().sid_library_nsf_bound(psf._h, ns, cp, rv, dm, mm, kp, kd, ctypes.byref(bound)))
+    row_ptr = np.zeros(ns + 1, dtype=np.int64)
+    cols = np.zeros(max(bound.value, 1), dtype=np.int64)
+    vals = np.zeros(max(bound.value, 1), dtype=np.float64)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    check(lib().sid_library_nsf(psf._h, ns, cp, rv, dm, mm, kp, kd, P(row_ptr), P(cols),
+                                vals.ctypes.data_as(ctypes.c_void_p), int(bound.value)))
+    nnz = int(row_ptr[-1])
+    return row_ptr, cols[:nnz].copy(), vals[:nnz].copy()
 
 
 def workload_seeds(wl):
