@@ -83,6 +83,10 @@ int sid_psf_create(const float *H, const int64_t dims[5], const int32_t *CAindex
 int sid_psf_create_device(const float *H_dev, const int64_t dims[5], const int32_t *CAindex,
                           sid_psf **out);
 int sid_psf_destroy(sid_psf *psf);
+/* SID_OK when psf is a handle created in this process and not yet destroyed,
+ * else SID_ERR_ARG.  No counterpart in the reference; for callers that carry
+ * handles through integers (a MEX gateway's uint64 handle argument). */
+int sid_psf_is_live(const sid_psf *psf);
 
 /* Creation flags for sid_psf_create_ex. */
 enum {
@@ -185,6 +189,10 @@ int sid_patched_rect(int64_t n, int64_t M, const int64_t sensor[2], const int64_
 int sid_library_forward_device(sid_psf *psf, int64_t n_seeds, const int64_t *boxes, const float *vals_dev,
                                const int64_t *val_offsets, const int64_t sensor[2], float *out_dev, int flags,
                                void *stream);
+/* The same on host buffers (vals and out in host memory; the MEX gateway of
+ * generate_LFM_library_GPU), one launch without graph capture. */
+int sid_library_forward(sid_psf *psf, int64_t n_seeds, const int64_t *boxes, const float *vals,
+                        const int64_t *val_offsets, const int64_t sensor[2], float *out);
 /* {rows, work items, MFMA multiply-adds issued (chunks of all-zero weights are
  * skipped), accumulator elements} of the cached library plan. */
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);

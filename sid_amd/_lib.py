@@ -29,6 +29,7 @@ SIGNATURES = {
     "sid_psf_create": (_I, [_P, _I64P, _I32P, ctypes.POINTER(_P)]),
     "sid_psf_create_device": (_I, [_P, _I64P, _I32P, ctypes.POINTER(_P)]),
     "sid_psf_destroy": (_I, [_P]),
+    "sid_psf_is_live": (_I, [_P]),
     "sid_psf_create_ex": (_I, [_P, _I64P, _I32P, _I, ctypes.POINTER(_P)]),
     "sid_psf_update": (_I, [_P, _P, _I]),
     "sid_poisson_nll_grad_device": (_I, [_P, _P, _P, _I64P, ctypes.POINTER(ctypes.c_double), _P, _P, _P]),
@@ -46,6 +47,7 @@ SIGNATURES = {
     "sid_library_rects": (_I, [_P, ctypes.c_int64, _I64P, _I64P, _I64P, _I64P]),
     "sid_library_forward_device": (_I, [_P, ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P, _I, _P]),
     "sid_library_stats": (_I, [_P, _I64P]),
+    "sid_library_forward": (_I, [_P, ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P]),
     "sid_patched_rect": (_I, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]),
     "sid_library_nsf_bound": (_I, [_P, ctypes.c_int64, _P, ctypes.c_double, _P, _I64P, _P, _I64P, _I64P]),
     "sid_library_nsf": (_I, [_P, ctypes.c_int64, _P, ctypes.c_double, _P, _I64P, _P, _I64P, _I64P, _I64P, _P,

@@ -9,6 +9,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <mutex>
+#include <set>
 #include <vector>
 
 #include "sid_internal.h"
@@ -216,6 +218,11 @@ int sid_device_count(void)
     return k;
 }
 
+// Handles alive in this process (for callers that pass them through an integer,
+// such as a MATLAB uint64, and need to check one before use).
+static std::mutex g_live_mu;
+static std::set<const sid_psf *> g_live;
+
 static int psf_create_impl(const float *H, bool host, const int64_t dims[5], const int32_t *CA, bool full,
                            sid_psf **out)
 {
@@ -271,6 +278,10 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
     if (rc != SID_OK) {
         destroy(p);
         return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        g_live.insert(p);
     }
     *out = p;
     return SID_OK;
@@ -354,8 +365,20 @@ int sid_project_vjp_device(sid_psf *p, const float *V, const float *G, const int
 
 int sid_psf_destroy(sid_psf *psf)
 {
+    if (psf) {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        g_live.erase(psf);
+    }
     destroy(psf);
     return SID_OK;
+}
+
+int sid_psf_is_live(const sid_psf *psf)
+{
+    std::lock_guard<std::mutex> lk(g_live_mu);
+    if (psf && g_live.count(psf)) return SID_OK;
+    set_error("not a live sid_psf handle");
+    return SID_ERR_ARG;
 }
 
 int sid_psf_set_precision(sid_psf *p, int precision)
@@ -901,6 +924,29 @@ int sid_library_forward_device(sid_psf *p, int64_t ns, const int64_t *boxes, con
     SID_TRY(pick_stream(p, stream, &st));
     return library_forward(p, ns, boxes, vals, val_offsets, sensor[0], sensor[1], out, st,
                            flags & 1);
+}
+
+int sid_library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *vals, const int64_t *val_offsets,
+                        const int64_t sensor[2], float *out)
+{
+    if (!p || ns < 0 || (ns > 0 && (!boxes || !vals || !val_offsets || !out)) || !sensor) {
+        set_error("sid_library_forward: null argument");
+        return SID_ERR_ARG;
+    }
+    if (ns == 0) return SID_OK;
+    SID_TRY(check_boxes(p, ns, boxes, sensor));
+    std::vector<int64_t> rects(4 * ns), offs(ns + 1);
+    SID_TRY(sid_library_rects(p, ns, boxes, sensor, rects.data(), offs.data()));
+    SID_TRY_HIP(hipSetDevice(p->device));
+    DevBuf dv, dout;
+    SID_TRY_HIP(hipMalloc(&dv.ptr, sizeof(float) * std::max<int64_t>(val_offsets[ns], 1)));
+    SID_TRY_HIP(hipMalloc(&dout.ptr, sizeof(float) * std::max<int64_t>(offs[ns], 1)));
+    SID_TRY(copy_h2d(dv.ptr, vals, sizeof(float) * val_offsets[ns], p->stream));
+    const int rc = sid_library_forward_device(p, ns, boxes, (const float *)dv.ptr, val_offsets, sensor,
+                                              (float *)dout.ptr, 0, p->stream);
+    lib_plan_free(p);   // its key holds the buffers freed on return
+    SID_TRY(rc);
+    return copy_d2h(out, dout.ptr, sizeof(float) * offs[ns], p->stream);
 }
 
 int sid_patched_rect(int64_t n, int64_t M, const int64_t sensor[2], const int64_t box[6], int64_t rect[4])
