@@ -162,6 +162,13 @@ int sid_deconv_rl_device(sid_psf *psf, const float *Htf_dev, float *X_dev, const
  * projector precision only. */
 int sid_poisson_nll_grad_device(sid_psf *psf, const float *V_dev, const float *y_dev, const int64_t dims[4],
                                 double *nll, float *gV_dev, float *gH_dev, void *stream);
+/* The FP64 normalisation of the Richardson-Lucy / EM update,
+ * B1 = backwardProjectGPU_new(H, ones(X, Y)) (backwardProjectACC_new with the
+ * handle's CAindex): at voxel q of plane c the exact FP64 sum of its kernel over
+ * the part that lands on the sensor (summed-area tables of every kernel, built
+ * plane by plane).  B1: device, column-major X*Y*Nz doubles.  No reference
+ * counterpart (the reference's deconvRL has no normalisation). */
+int sid_backward_ones_device(sid_psf *psf, int64_t X, int64_t Y, double *B1_dev, void *stream);
 /* Vector-Jacobian product of O_f = forward(H, V_f) for upstream image
  * gradients G (nb images [X,Y]): gV = backward(G) (nullable), gH = dNLL/dH of
  * sum_f <G_f, O_f> (nullable; needs V). */

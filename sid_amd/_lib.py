@@ -34,6 +34,7 @@ SIGNATURES = {
     "sid_psf_update": (_I, [_P, _P, _I]),
     "sid_poisson_nll_grad_device": (_I, [_P, _P, _P, _I64P, ctypes.POINTER(ctypes.c_double), _P, _P, _P]),
     "sid_project_vjp_device": (_I, [_P, _P, _P, _I64P, _P, _P, _P]),
+    "sid_backward_ones_device": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P]),
     "sid_psf_set_precision": (_I, [_P, _I]),
     "sid_psf_set_method": (_I, [_P, _I]),
     "sid_psf_set_plane_methods": (_I, [_P, _I32P, ctypes.c_int64]),

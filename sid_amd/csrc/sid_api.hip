@@ -344,6 +344,17 @@ int sid_poisson_nll_grad_device(sid_psf *p, const float *V, const float *y, cons
     return grad_core(p, V, y, dims, 0, nll, gV, gH, st);
 }
 
+int sid_backward_ones_device(sid_psf *p, int64_t X, int64_t Y, double *B1, void *stream)
+{
+    if (!p || !B1) { set_error("sid_backward_ones_device: null argument"); return SID_ERR_ARG; }
+    if (X < 1 || Y < 1 || X > INT32_MAX || Y > INT32_MAX) { set_error("sensor size out of range"); return SID_ERR_DIMS; }
+    if (p->kern) { set_error("B(1) is taken of the projector alone; remove the neuron kernel"); return SID_ERR_ARG; }
+    SID_TRY_HIP(hipSetDevice(p->device));
+    hipStream_t st;
+    SID_TRY(pick_stream(p, stream, &st));
+    return backward_ones_f64(p, X, Y, B1, st);
+}
+
 int sid_project_vjp_device(sid_psf *p, const float *V, const float *G, const int64_t dims[4], float *gV, float *gH,
                            void *stream)
 {

@@ -336,4 +336,100 @@ int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4],
     return SID_OK;
 }
 
+// ---------------------------------------------------------------------------
+// FP64 back-projection of a sensor of ones: the normalisation B(1) of the
+// Richardson-Lucy / EM updates.  backwardProjectGPU_new(H, ones(X, Y)) at voxel
+// q of plane c is the sum of the kernel H_{a(q),b(q),c} over the indices
+// i = p - q + off_b that land on the sensor, i.e. over a rectangle of the
+// kernel's support; each kernel's FP64 summed-area table gives that sum from
+// four lookups.  Computed plane by plane (the tables of one plane: n^2 kernels
+// of (Mc + 1)^2 doubles).
+// ---------------------------------------------------------------------------
+
+// Summed-area table of kernel ab of plane c over its support [lo, hi]^2:
+// sat[(j + 1) * W + (i + 1)] = sum of H[lo..lo+i][lo..lo+j], first row / column 0.
+__global__ __launch_bounds__(256) void k_b1_sat(const float *__restrict__ H, int64_t M, int n, int c, int lo, int Mc,
+                                                double *__restrict__ sat)
+{
+    const int ab = blockIdx.x, a = ab % n, b = ab / n;
+    const int W = Mc + 1;
+    const float *K = H + M * M * ((int64_t)a + (int64_t)n * ((int64_t)b + (int64_t)n * c));
+    double *T = sat + (int64_t)ab * W * W;
+    // rows: running sums along i for every column j
+    for (int j = threadIdx.x; j < W; j += blockDim.x) {
+        double acc = 0.0;
+        T[(int64_t)j * W] = 0.0;
+        for (int i = 0; i < Mc; ++i) {
+            if (j > 0) acc += (double)K[(int64_t)(lo + i) + M * (lo + j - 1)];
+            T[(int64_t)j * W + i + 1] = acc;
+        }
+    }
+    __syncthreads();
+    // columns: running sums along j
+    for (int i = threadIdx.x; i < W; i += blockDim.x) {
+        double acc = 0.0;
+        for (int j = 0; j < W; ++j) {
+            acc += T[(int64_t)j * W + i];
+            T[(int64_t)j * W + i] = acc;
+        }
+    }
+}
+
+// B1[x + X (y + Y c)] = sum of kernel (x mod n, y mod n, c) over i in
+// [max(lo, off - x), min(hi, X - 1 - x + off)] x the same for y.
+__global__ void k_b1_rect(const double *__restrict__ sat, int X, int Y, int n, int c, int lo, int hi, int off,
+                          double *__restrict__ B1)
+{
+    const int W = hi - lo + 2;
+    const int64_t total = (int64_t)X * Y;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % X), y = (int)(e / X);
+        const int i0 = max(lo, off - x), i1 = min(hi, X - 1 - x + off);
+        const int j0 = max(lo, off - y), j1 = min(hi, Y - 1 - y + off);
+        double v = 0.0;
+        if (i0 <= i1 && j0 <= j1) {
+            const double *T = sat + (int64_t)((x % n) + n * (y % n)) * W * W;
+            const int u0 = i0 - lo, u1 = i1 - lo + 1, w0 = j0 - lo, w1 = j1 - lo + 1;
+            v = T[(int64_t)w1 * W + u1] - T[(int64_t)w0 * W + u1] - T[(int64_t)w1 * W + u0] + T[(int64_t)w0 * W + u0];
+        }
+        B1[(int64_t)X * Y * c + e] = v;
+    }
+}
+
+int backward_ones_f64(sid_psf *p, int64_t X, int64_t Y, double *B1, hipStream_t st)
+{
+    const int n = (int)p->n;
+    int64_t tab = 1;
+    for (const Plane &pl : p->planes)
+        if (!pl.empty) tab = std::max<int64_t>(tab, (int64_t)(pl.hi - pl.lo + 2) * (pl.hi - pl.lo + 2));
+    double *sat = nullptr;
+    SID_TRY_HIP(hipMalloc(&sat, sizeof(double) * tab * n * n));
+    int rc = SID_OK;
+    for (int64_t c = 0; c < p->nzH && rc == SID_OK; ++c) {
+        const Plane &pl = p->planes[c];
+        if (pl.empty) {
+            hipLaunchKernelGGL(k_b1_rect, dim3(grid_of(X * Y)), dim3(256), 0, st, (const double *)nullptr, (int)X,
+                               (int)Y, n, (int)c, 1, 0, 0, B1);   // empty range: zeros
+        } else {
+            const int Mc = pl.hi - pl.lo + 1;
+            hipLaunchKernelGGL(k_b1_sat, dim3((unsigned)(n * n)), dim3(256), 0, st, p->H, p->M, n, (int)c, pl.lo,
+                               Mc, sat);
+            hipLaunchKernelGGL(k_b1_rect, dim3(grid_of(X * Y)), dim3(256), 0, st, (const double *)sat, (int)X,
+                               (int)Y, n, (int)c, pl.lo, pl.hi, pl.off_b, B1);
+        }
+        if (hipGetLastError() != hipSuccess) {
+            set_error("B(1) kernels failed to launch");
+            rc = SID_ERR_HIP;
+        }
+    }
+    const hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(sat);
+    if (rc == SID_OK && e != hipSuccess) {
+        set_error("B(1): %s", hipGetErrorString(e));
+        rc = SID_ERR_HIP;
+    }
+    return rc;
+}
+
 }  // namespace sid

@@ -255,6 +255,8 @@ int library_stats(const sid_psf *p, int64_t st[4]);
 int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4], int mode, double *nll, float *gV,
               float *gH, hipStream_t st);
 int run_selftest();
+// FP64 B(1) = backward(ones(X, Y)) on the handle's support crops, column-major [X, Y, Nz] (device)
+int backward_ones_f64(sid_psf *p, int64_t X, int64_t Y, double *B1, hipStream_t st);
 }  // namespace sid
 
 namespace sid {

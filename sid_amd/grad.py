@@ -76,6 +76,15 @@ def project_vjp(psf, V, G, want_gV: bool = True, want_gH: bool = True):
     return gV, gH
 
 
+def backward_ones(psf, X: int, Y: int):
+    """FP64 B(1) = backwardProjectGPU_new(H, ones(X, Y)) as a (Nz, Y, X) float64
+    tensor: exact sums of each kernel over its part that lands on the sensor."""
+    torch = _torch()
+    out = torch.empty((psf.Nz, Y, X), dtype=torch.float64, device="cuda")
+    check(lib().sid_backward_ones_device(psf._h, int(X), int(Y), _ptr(out), _stream()))
+    return out
+
+
 def _sync_psf(psf, H):
     """Keep the handle's H equal to the tensor H (rebuild weights on change).
 
@@ -179,7 +188,8 @@ def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf:
       V <- V * (1 - gV / B(1))   with gV = dNLL/dV at (V, H)   (= V * B(y/F) / B(1))
       H <- H * (1 - gH / G1)     with gH = dNLL/dH at (V_new, H) and
                                  G1 = d(sum F)/dH at V_new      (if refine_psf)
-    B(1) is the back-projected normalisation (recomputed when H changes).
+    B(1) is the back-projected normalisation, exact in FP64 (backward_ones;
+    recomputed when H changes); the V ratio is formed in FP64.
     The NLL recorded for a step is the one at the start of the step.  After
     each step (every ``checkpoint_every`` steps, and after the last or the
     ``stop_after`` step) the state is checkpointed atomically to ``checkpoint``;
@@ -202,12 +212,12 @@ def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf:
     psf.set_method("direct")   # the PSF gradient needs the direct projector; keep one plan
     nb, nz, Y, X = V.shape
     ones = torch.ones((nb, Y, X), dtype=torch.float32, device=V.device)
-    B1, _ = project_vjp(psf, None, ones[:1], True, False)
+    B1 = backward_ones(psf, X, Y).unsqueeze(0)
     for step in range(start, steps):
         nll, gV, _ = poisson_nll_grad(psf, V, y, True, False)
         hist.append(nll)
         with torch.no_grad():
-            ratio = torch.where(B1 > 0, gV / B1, torch.zeros_like(gV))
+            ratio = torch.where(B1 > 0, gV.double() / B1, torch.zeros_like(B1)).to(torch.float32)
             V = torch.clamp(V * (1.0 - ratio), min=0.0).contiguous()
         if refine_psf:
             _, _, gH = poisson_nll_grad(psf, V, y, False, True)
@@ -216,7 +226,7 @@ def refine(psf, H, V0, y, steps: int, checkpoint: str | None = None, refine_psf:
                 rh = torch.where(G1 > 0, gH / G1, torch.zeros_like(gH))
                 H = torch.clamp(H * (1.0 - rh), min=0.0).contiguous()
             psf.update(H)
-            B1, _ = project_vjp(psf, None, ones[:1], True, False)
+            B1 = backward_ones(psf, X, Y).unsqueeze(0)
         stop = stop_after is not None and step + 1 >= stop_after
         if checkpoint and ((step + 1) % max(1, int(checkpoint_every)) == 0 or step + 1 == steps or stop):
             save_checkpoint(checkpoint, step + 1, V, H, hist, cfg)

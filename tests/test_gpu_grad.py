@@ -200,3 +200,32 @@ def test_refinement_checkpoint_resume_bit_exact(env, tmp_path):
     Vb, Hb, hb = grad.refine(psf, Ht, V0, yt, 20, checkpoint=ck)
     assert hb == ha
     assert torch.equal(Va, Vb) and torch.equal(Ha, Hb)
+
+
+@pytest.mark.parametrize("n,X,Y,M,nz,crop", [(5, 33, 27, 11, 3, False), (3, 20, 17, 8, 2, False),
+                                             (5, 40, 35, 13, 3, True), (15, 60, 75, 31, 2, False)])
+def test_fp64_back_projected_normalisation(env, n, X, Y, M, nz, crop):
+    """B(1) = backwardProjectGPU_new(H, ones) in FP64 from summed-area tables, against
+    the FP64 oracle's backward projection of a sensor of ones (odd and even M,
+    cropped kernels, partial lenslets, an all-zero plane)."""
+    torch, sid_amd, grad = env
+    print(f"case: FP64 B(1) n{n} X{X} Y{Y} M{M} Nz{nz} crop {crop}", flush=True)
+    rng = np.random.default_rng(n * 100 + M)
+    H = rng.random((M, M, n, n, nz)) + 0.05
+    H[:, :, :, :, nz - 1] = 0.0
+    CA = None
+    if crop:
+        m = M // 2
+        CA = np.array([[m + 1 - 2 - c, m + 1 + 2 + c] for c in range(nz)], dtype=np.int32)
+    H32 = H.astype(np.float32)
+    psf = sid_amd.PSF(H32, CAindex=CA)
+    B1 = grad.backward_ones(psf, X, Y).cpu().numpy().transpose(2, 1, 0)
+    H64 = H32.astype(np.float64)
+    want = O.backward_project(H64, np.ones((X, Y))) if CA is None else O.backward_project_acc(H64, np.ones((X, Y)), CA)
+    assert np.all(B1[:, :, nz - 1] == 0.0)
+    assert rel_l2(B1, want) < 1e-13
+    # interior voxels: the kernel's whole sum (every entry lands on the sensor)
+    if CA is None and X > 2 * M and Y > 2 * M:
+        q = (X // 2, Y // 2)
+        assert abs(B1[q[0], q[1], 0] - H64[:, :, q[0] % n, q[1] % n, 0].sum()) <= 1e-12 * B1[q[0], q[1], 0]
+    psf.close()
