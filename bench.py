@@ -70,10 +70,96 @@ def cpu_baseline(wl, V, y_pre, sample_planes=None):
     msum_all = sum((hi - lo + 1) ** 2 for lo, hi in synth.ca_index(wl.zs, wl.M))
     msum_s = sum((hi - lo + 1) ** 2 for lo, hi in CAs)
     t_iter = (t2 - t0) * msum_all / msum_s
-    return {"value": 1.0 / t_iter, "unit": "RL iter/s", "cores": threads, "kind": "port",
-            "sample": (f"forwardProjectACC (single) + backwardProjectACC_new restated in C (OpenMP, -O3), "
-                       f"planes z={zs} of the same workload, {t2 - t0:.2f} s measured (fwd {t1 - t0:.2f} s, "
-                       f"bwd {t2 - t1:.2f} s), scaled by sum(M_c^2) all/sample = {msum_all}/{msum_s}")}
+    out = {"value": 1.0 / t_iter, "unit": "RL iter/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "sample": (f"forwardProjectACC (single) + backwardProjectACC_new restated in C (OpenMP, -O3), "
+                      f"planes z={zs} of the same workload, {t2 - t0:.2f} s measured (fwd {t1 - t0:.2f} s, "
+                      f"bwd {t2 - t1:.2f} s), scaled by sum(M_c^2) all/sample = {msum_all}/{msum_s}")}
+    out["rl_small"] = cpu_rl_small(L, threads)
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_rl_small(L, threads):
+    """The small workload in full on the host: 255 x 255 px, 11 planes, 105 px
+    kernels, reconstruct_S pre-processing, 8 ISRA iterations of the reference
+    CPU path's arithmetic (single-precision forward, double-stored backward,
+    CAindex crops), timed whole."""
+    from sid_amd import recon, synth
+    wl = synth.WORKLOADS["rl_small"]
+    H, CA = synth.make_psf(wl.n, wl.M, wl.zs, dtype=np.float32)
+    V = synth.make_volume(wl.X, wl.Y, wl.nz, wl.n_balls, wl.r_lo, wl.r_hi, wl.seed, wl.background)
+    X, Y, nz = wl.X, wl.Y, wl.nz
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    hd = (ctypes.c_int64 * 5)(*H.shape)
+    ca = np.asfortranarray(CA.astype(np.int32))
+    O = np.zeros((X, Y), dtype=np.float32, order="F")
+    L.orc_forward_f32(P(H), hd, P(ca), P(np.asfortranarray(V.astype(np.float64))), ctypes.c_int64(X),
+                      ctypes.c_int64(Y), ctypes.c_int64(nz), P(O), ctypes.c_int(threads))
+    img = synth.poisson_image(O, wl.peak_photons, wl.seed)
+    y, _ = recon.preprocess(img)
+    yf = np.asfortranarray(y.astype(np.float32))
+    t0 = time.perf_counter()
+    htf = np.zeros((X, Y, nz), dtype=np.float64, order="F")
+    L.orc_backward_f32(P(H), hd, P(ca), P(yf), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(nz), P(htf),
+                       ctypes.c_int(threads))
+    Xg = htf.copy(order="F")
+    img_w = np.zeros((X, Y), dtype=np.float32, order="F")
+    D = np.zeros((X, Y, nz), dtype=np.float64, order="F")
+    t1 = time.perf_counter()
+    L.orc_isra_f32(P(H), hd, P(ca), P(htf), P(Xg), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(nz),
+                   ctypes.c_int(wl.iters), P(img_w), P(D), ctypes.c_int(threads))
+    t2 = time.perf_counter()
+    return {"workload": f"rl_small: {X}x{Y} px, {nz} planes, M={wl.M}, {wl.iters} ISRA iterations",
+            "iter_per_s": round(wl.iters / (t2 - t1), 4), "s_isra": round(t2 - t1, 3),
+            "s_htf_backward": round(t1 - t0, 3), "proj_per_s": round(2 * wl.iters / (t2 - t1), 3)}
+
+
+def projection_rates(dev, names=("rl_small",), precisions=("fp32", "bf16x2"), reps=20):
+    """Forward and backward projections per second on device buffers (the
+    autotuned plan of each workload's handle)."""
+    import torch
+    import sid_amd
+    from sid_amd import synth
+    out = {}
+    for name in names:
+        wl = synth.WORKLOADS[name]
+        H, _ = synth.make_psf(wl.n, wl.M, wl.zs, xp=torch, device=dev)
+        psf = sid_amd.PSF(H)
+        del H
+        V = torch.rand((wl.nz, wl.Y, wl.X), device=dev)
+        O = torch.rand((wl.Y, wl.X), device=dev)
+        res = {}
+        for prec in precisions:
+            psf.set_precision(prec)
+            psf.forward(V), psf.backward(O)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                psf.forward(V)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(reps):
+                psf.backward(O)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            res[prec] = {"forward_per_s": round(reps / (t1 - t0), 1), "backward_per_s": round(reps / (t2 - t1), 1)}
+        out[name] = res
+        log(f"projections {name}: " + ", ".join(f"{k} fwd {v['forward_per_s']:.0f}/s bwd {v['backward_per_s']:.0f}/s"
+                                                   for k, v in res.items()))
+        psf.close()
+        del V, O
+        torch.cuda.empty_cache()
+    return out
 
 
 def grad_bench(dev, reps=3):
@@ -111,12 +197,26 @@ def grad_bench(dev, reps=3):
         times[name] = (time.perf_counter() - t0) / reps
     fl_gemm = 2.0 * wl.X * wl.Y * msum
     t, tv = times["nll_gV_gH"], times["nll_gV"]
+    # forward / backward projections per second at this workload (FP32 direct)
+    psf.forward(V[0]), psf.backward(y[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(nb):
+        psf.forward(V[f])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for f in range(nb):
+        psf.backward(y[f])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    proj = {"forward_per_s": round(nb / (t1 - t0), 1), "backward_per_s": round(nb / (t2 - t1), 1)}
     res = {"frames": nb, "X": wl.X, "Y": wl.Y, "Nz": wl.nz, "M": wl.M, "precision": "fp32",
            "ms_nll_gV_gH": round(t * 1e3, 2), "ms_nll_gV": round(tv * 1e3, 2),
            "batches_per_s": round(1.0 / t, 3), "frames_per_s": round(nb / t, 2),
            "tflops": round(3 * nb * fl_gemm / t / 1e12, 2),
            "frac_of_fp32_peak": round(3 * nb * fl_gemm / t / PEAK["fp32"], 4),
-           "psf_grad_tflops": round(nb * fl_gemm / max(t - tv, 1e-9) / 1e12, 2), "nll": nll}
+           "psf_grad_tflops": round(nb * fl_gemm / max(t - tv, 1e-9) / 1e12, 2), "nll": nll,
+           "projections_fp32_direct": proj}
     log(f"gradients ({nb} frames {wl.X}x{wl.Y}x{wl.nz}, M={wl.M}): NLL+dV+dH {t * 1e3:.1f} ms "
         f"({res['tflops']:.1f} TFLOP/s), NLL+dV {tv * 1e3:.1f} ms")
     del psf, V, y
@@ -368,6 +468,13 @@ def main():
             # FFT path: each GEMV streams the spectra once (algorithmic bytes per launch)
             res["gemv_stream_tbps"] = inf["spectra_bytes"] / (0.5 * (t_gf + t_gb) * 1e-3) / 1e12
             res["gemv_frac_of_hbm"] = res["gemv_stream_tbps"] * 1e12 / HBM_PEAK
+        # MFMA work of the direct planes: algorithmic (X*Y*M_c^2 per plane) against executed (K rows run)
+        st8 = psf.plan_stats()
+        pm_f, pm_b = psf.plane_macs()
+        res["gemm_macs_executed"] = 0.5 * (st8["macs_fwd"] + st8["macs_bwd"])
+        res["gemm_macs_algorithmic"] = flops_direct / 2.0
+        res["plane_exec_over_alg"] = [round(0.5 * (pm_f[c] + pm_b[c]) / (wl.X * wl.Y * (hi - lo + 1) ** 2), 3)
+                                      if m == "direct" else None for c, ((lo, hi), m) in enumerate(zip(sup, meth))]
         results[prec] = res
         log(f"{prec}: {res['iter_per_s']:.2f} RL iter/s, {ms_step:.1f} ms/step; per projection: direct GEMM "
             f"fwd {t_f:.2f} / bwd {t_b:.2f} ms on {res['planes_direct']} planes, FFT path fwd {t_ff:.2f} / "
@@ -406,6 +513,7 @@ def main():
 
     if not args.no_library:
         out_lib = library_bench(psf, wl_lib=None, steps=max(args.steps, 3))
+    proj_rates = projection_rates(dev) if not args.no_rows else None
     out_rows = None
     if not args.no_rows:
         head_prec = args.precisions.split(",")[-1]
@@ -435,7 +543,7 @@ def main():
     direct = None
     if not args.no_direct and args.method == "auto":
         # every plane on the implicit GEMM: the MFMA path alone, per precision (one timed step each)
-        direct = {}
+        direct, direct_roof = {}, {}
         psf.set_plane_methods(None)
         psf.set_method("direct")
         for prec in args.precisions.split(","):
@@ -446,6 +554,17 @@ def main():
             psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
             torch.cuda.synchronize()
             direct[prec] = round(args.iters / (time.perf_counter() - t0), 3)
+            # the GEMM alone over every plane: algorithmic and executed MFMA rate per launch
+            tm = psf.timing()
+            t_launch = 0.5 * (tm[0] + tm[2]) / args.iters * 1e-3
+            st8 = psf.plan_stats()
+            prods = {"bf16x2": 2, "bf16x3": 3}.get(prec, 1)
+            ex = 2.0 * 0.5 * (st8["macs_fwd"] + st8["macs_bwd"]) * prods
+            direct_roof[prec] = {"ms_per_launch": round(t_launch * 1e3, 3),
+                                 "achieved_tflops": round(flops_proj / t_launch / 1e12, 2),
+                                 "frac": round(flops_proj / t_launch / PEAK[prec], 4),
+                                 "executed_mfma_frac": round(ex / t_launch / PEAK[prec], 4),
+                                 "executed_over_algorithmic": round(ex / prods / flops_proj, 3)}
         psf.set_method("auto")
         log("direct-only (every plane on the MFMA GEMM): " + ", ".join(f"{k} {v:.2f} RL iter/s" for k, v in direct.items()))
     hr = results[head]
@@ -497,6 +616,25 @@ def main():
                 "traffic_scaled_from_spectra_gb": float(tj["spectra_gb"]) if traffic and traffic_scale else None,
                 "kernel": "k_fft_gemv_fwd / k_fft_gemv_bwd (spectra GEMVs of the FFT planes)",
                 "algorithmic_per_launch": f"{hr['spectra_gb']:.2f} GB of spectra streamed once per launch"}
+    # the MFMA half of the headline plan: the direct GEMM on its planes, beside the FFT path
+    roof_mfma = None
+    if hr["fwd_gemm_ms"] + hr["bwd_gemm_ms"] > 0:
+        prods = {"bf16x2": 2, "bf16x3": 3}.get(head, 1)
+        t_launch = 0.5 * (hr["fwd_gemm_ms"] + hr["bwd_gemm_ms"]) * 1e-3
+        alg = 2.0 * hr["gemm_macs_algorithmic"]
+        ex = 2.0 * hr["gemm_macs_executed"] * prods
+        roof_mfma = {"bound": "mfma", "achieved": round(alg / t_launch / 1e12, 2), "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": round(alg / t_launch / peak, 4),
+                     "executed_mfma_frac": round(ex / t_launch / peak, 4),
+                     "executed_over_algorithmic": round(ex / prods / alg, 3),
+                     "kernel": "k_proj_gemm (direct planes, forward and backward launches beside the FFT path)",
+                     "algorithmic_per_launch": f"2*X*Y*sum(M_c^2) over {hr['planes_direct']} direct planes = {alg:.3e} flop",
+                     "ms_per_launch": round(t_launch * 1e3, 3),
+                     "plane_exec_over_alg": hr["plane_exec_over_alg"]}
+    fft_planes = hr["planes_fft"]
+    dtype = head
+    if head == "bf16x2":
+        dtype = "bf16x2+fp16-spectra" if fft_planes else "bf16x2"
     out = {
         "metric": "RL iter/s @1005²×51z: BF16-split MFMA vs plain FP32; rel-L2 err vs FP64",
         "value": round(hr["iter_per_s"], 4),
@@ -508,13 +646,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if head.startswith("bf16") else head,
+        "dtype": dtype,
+        "plane_mix": {"direct": hr["planes_direct"], "fft": fft_planes,
+                      "direct_arithmetic": {"bf16x2": "bf16 hi/lo split x bf16 PSF, fp32 accumulate",
+                                            "bf16x3": "bf16 hi/lo split x bf16 hi/lo PSF, fp32 accumulate",
+                                            "fp32": "fp32", "fp64": "fp64"}.get(head, head),
+                      "fft_spectra": ("fp16 pairs, row-scaled" if head == "bf16x2" else "complex fp32")
+                      if fft_planes else None},
         "data": "synthetic (seeded PSF, ball volume, Poisson image)",
         "config": {"workload": f"{wl.name}: {wl.X}x{wl.Y} px, Nnum={wl.n}, {wl.nz} planes z={wl.zs[0]}..{wl.zs[-1]}, "
                                f"M={wl.M}, {wl.n_balls} balls, {args.iters} ISRA iterations per step",
                    "method": head, "baseline_method": base, "projector": args.method,
                    "parallelism": "single GPU"},
         "roofline": roof,
+        "roofline_mfma": roof_mfma,
+        "roofline_direct_only": direct_roof if direct else None,
         "methods": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in results.items()},
         "speedup_vs_fp32": round(hr["iter_per_s"] / results[base]["iter_per_s"], 3) if base in results else None,
@@ -529,10 +675,22 @@ def main():
         out["library"] = out_lib
     if out_rows:
         out["rows"] = out_rows
+    if proj_rates:
+        out["projections"] = proj_rates
+        out["projections"]["rl_full"] = {head: {"proj_per_s": round(hr["proj_per_s"], 1)},
+                                         base: {"proj_per_s": round(results[base]["proj_per_s"], 1)}}
+        if out_rows and "gradient" in out_rows:
+            out["projections"]["grad"] = {"fp32": out_rows["gradient"]["projections_fp32_direct"]}
+    # release the device state before the CPU leg and the exit
+    psf.close()
+    del Vt, yt, htfs, clean
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     if not args.no_cpu:
-        log("CPU baseline (bounded sample) ...")
+        log("CPU baseline (bounded sample, and the small workload in full) ...")
         out["cpu_baseline"] = cpu_baseline(wl, V, y)
-        log(f"CPU baseline {out['cpu_baseline']['value']:.5f} RL iter/s on {out['cpu_baseline']['cores']} threads")
+        log(f"CPU baseline {out['cpu_baseline']['value']:.5f} RL iter/s on {out['cpu_baseline']['cores']} threads "
+            f"({out['cpu_baseline']['cpu_model']}); rl_small {out['cpu_baseline']['rl_small']['iter_per_s']} RL iter/s")
     print(json.dumps(out), flush=True)
 
 

@@ -304,8 +304,14 @@ int sid_last_timing(const sid_psf *psf, double ms[12]);
 
 /* Kernel launch counts and GEMM geometry of the direct path for the current
  * sensor plan (for roofline accounting): {macs_fwd, macs_bwd, items_fwd,
- * items_bwd, slabs_fwd, slabs_bwd, rows_padded, cols_padded}. */
+ * items_bwd, slabs_fwd, slabs_bwd, rows_padded, cols_padded}.  macs = the MFMA
+ * multiply-adds one launch runs (K rows executed after the all-zero chunks are
+ * skipped, x padded N x padded M; one product per MAC, so BF16X2 issues twice
+ * as many MFMA operations and BF16X3 three times). */
 int sid_plan_stats(const sid_psf *psf, int64_t stats[8]);
+/* The same MFMA multiply-adds per plane of H (n = size(H,5); 0 for planes on
+ * the FFT path or empty) for the forward and backward launches. */
+int sid_plan_plane_macs(const sid_psf *psf, int64_t *fwd, int64_t *bwd, int64_t n);
 
 /* The FFT primitive of the FFT path: nseq contiguous complex sequences
  * (interleaved re, im floats) of length L (2/3/5-smooth, <= 2048) transformed

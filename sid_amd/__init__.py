@@ -156,6 +156,14 @@ class PSF:
                 "cols_padded"]
         return dict(zip(keys, list(buf)))
 
+    def plane_macs(self):
+        """MFMA multiply-adds per plane of the current plan's forward and
+        backward GEMM launches (executed K rows; 0 for FFT or empty planes)."""
+        f = (ctypes.c_int64 * self.Nz)()
+        b = (ctypes.c_int64 * self.Nz)()
+        check(lib().sid_plan_plane_macs(self._h, f, b, self.Nz))
+        return list(f), list(b)
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             lib().sid_psf_destroy(self._h)

@@ -63,6 +63,7 @@ SIGNATURES = {
     "sid_ls_deconv_device": (_I, [_P, _P, _P, _I64P, _P, _P]),
     "sid_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "sid_plan_stats": (_I, [_P, _I64P]),
+    "sid_plan_plane_macs": (_I, [_P, _I64P, _I64P, ctypes.c_int64]),
     "sid_selftest": (_I, []),
     "sid_fft_device": (_I, [_I, ctypes.c_int64, _P, _I, _P]),
 }

@@ -1183,6 +1183,18 @@ int sid_plan_stats(const sid_psf *p, int64_t st[8])
     return SID_OK;
 }
 
+int sid_plan_plane_macs(const sid_psf *p, int64_t *fwd, int64_t *bwd, int64_t n)
+{
+    if (!p || !fwd || !bwd) { set_error("sid_plan_plane_macs: null argument"); return SID_ERR_ARG; }
+    if (n != p->nzH) { set_error("sid_plan_plane_macs: n must be size(H,5) = %lld", (long long)p->nzH); return SID_ERR_DIMS; }
+    const SensorPlan &sp = p->plan;
+    for (int64_t c = 0; c < n; ++c) fwd[c] = bwd[c] = 0;
+    if (!sp.vol) { set_error("no sensor plan yet: project once first"); return SID_ERR_STATE; }
+    for (size_t i = 0; i < sp.fwd.segs.size(); ++i) fwd[sp.fwd.segs[i].plane] += sp.fwd.cost[i] * p->NP * sp.mpad;
+    for (size_t i = 0; i < sp.bwd.segs.size(); ++i) bwd[sp.bwd.segs[i].plane] += sp.bwd.cost[i] * p->NP * sp.mpad;
+    return SID_OK;
+}
+
 int sid_fft_device(int L, int64_t nseq, float *data, int direction, void *stream)
 {
     if (!data || nseq < 0 || (direction != -1 && direction != 1)) { set_error("sid_fft_device: bad argument"); return SID_ERR_ARG; }

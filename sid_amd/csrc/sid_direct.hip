@@ -1200,7 +1200,8 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
             for (int s = groups[gi].first; s < groups[gi].second; ++s) gi_cost[gi] += F.cost[s];
         }
         F.items = xcd_order(gi_items, gi_cost);
-        for (auto &sg : F.segs) F.macs += (int64_t)sg.K * NP * sp.mpad;
+        // MFMA multiply-adds the GEMM runs: the executed K rows (tap hulls / live chunks) x N x padded M
+        for (size_t i = 0; i < F.segs.size(); ++i) F.macs += F.cost[i] * NP * sp.mpad;
     }
 
     // ---- backward: every plane of H, split per plane in proportion to cost
@@ -1253,7 +1254,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         B.plane_slab0[p->nzH] = slab;
         B.nslabs = slab;
         B.items = xcd_order(gr_items, gr_cost);
-        for (auto &sg : B.segs) B.macs += (int64_t)sg.K * NP * sp.mpad;
+        for (size_t i = 0; i < B.segs.size(); ++i) B.macs += B.cost[i] * NP * sp.mpad;
     }
     SID_TRY(upload_plan(F));
     SID_TRY(upload_plan(B));
