@@ -148,7 +148,8 @@ static void destroy(sid_psf *p)
     fft_state_free(sp.fft);
     free_plan_buffers(sp);
     for (void *q : {(void *)sp.fwd.d_segs, (void *)sp.fwd.d_items, (void *)sp.bwd.d_segs, (void *)sp.bwd.d_items,
-                    (void *)sp.fwd.d_tmap, (void *)sp.bwd.d_tmap, (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
+                    (void *)sp.fwd.d_tmap, (void *)sp.bwd.d_tmap, (void *)sp.fwd.d_steps, (void *)sp.bwd.d_steps,
+                    (void *)sp.fwd.d_run0, (void *)sp.bwd.d_run0, (void *)sp.bwd.d_plane_slab0, (void *)sp.fwd.d_plane_slab0, (void *)p->H, (void *)p->wf,
                     (void *)p->wb, (void *)p->wf_hi, (void *)p->wb_hi, (void *)p->wf_lo, (void *)p->wb_lo,
                     (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag,
                     (void *)p->hbuf[0], (void *)p->hbuf[1], (void *)p->nf})

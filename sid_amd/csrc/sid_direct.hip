@@ -300,11 +300,11 @@ __device__ __forceinline__ void sl_fetch_b(u32x4 (&pre)[NB][VPT], const u16 *__r
     constexpr int NV = NP * KW / 8;
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
-        const int e = v * NTHREADS + (int)threadIdx.x;
-        if (e < NV) {
-            pre[0][v] = *(const u32x4 *)(Wh + o + 8 * e);
-            if constexpr (NB == 2) pre[1][v] = *(const u32x4 *)(Wl + o + 8 * e);
-        }
+        // threads past the chunk load its last vector again (the stash skips
+        // them): an unconditional load keeps pre out of the loop's phi copies
+        const int e = std::min(v * NTHREADS + (int)threadIdx.x, NV - 1);
+        pre[0][v] = *(const u32x4 *)(Wh + o + 8 * e);
+        if constexpr (NB == 2) pre[1][v] = *(const u32x4 *)(Wl + o + 8 * e);
     }
 }
 
@@ -315,12 +315,12 @@ __device__ __forceinline__ void sl_stash_b(const u32x4 (&pre)[NB][VPT], u16 *bs)
     constexpr int NV = NP * KW / 8;
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
-        const int e = v * NTHREADS + (int)threadIdx.x;
-        if (e < NV) {
-            const int col = e / (KW / 8), kq = e - col * (KW / 8);
-            *(u32x4 *)(bs + col * LDK + 8 * kq) = pre[0][v];
-            if constexpr (NB == 2) *(u32x4 *)(bs + NP * LDK + col * LDK + 8 * kq) = pre[1][v];
-        }
+        // threads past the chunk hold its last vector (sl_fetch_b) and store it
+        // again at the same place: the same bytes, no branch
+        const int e = std::min(v * NTHREADS + (int)threadIdx.x, NV - 1);
+        const int col = e / (KW / 8), kq = e - col * (KW / 8);
+        *(u32x4 *)(bs + col * LDK + 8 * kq) = pre[0][v];
+        if constexpr (NB == 2) *(u32x4 *)(bs + NP * LDK + col * LDK + 8 * kq) = pre[1][v];
     }
 }
 
@@ -332,15 +332,56 @@ struct SlShape {
     static constexpr int NB = MODE == 2 ? 2 : 1;
 };
 
+// Buffer resource over the A operand from `base` (gfx9 word 3): a lane whose
+// offset is kNoLoad reads zeros without a memory access.
+constexpr uint32_t kNoLoad = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t a_rsrc(const u16 *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
+}
+
+// Decoded step-table entry (uniform): A offset, B offset, and whether the step
+// continues the previous step's chunk at the next tap.
+struct SlStep {
+    int64_t a, w;
+    bool slide;
+};
+
+// Entry whose 32-bit word l % 4 lane l holds.
+__device__ __forceinline__ SlStep sl_decode(int v)
+{
+    const uint32_t a0 = __builtin_amdgcn_readlane(v, 0), a1 = __builtin_amdgcn_readlane(v, 1);
+    const uint32_t w0 = __builtin_amdgcn_readlane(v, 2), w1 = __builtin_amdgcn_readlane(v, 3);
+    const uint64_t wv = (uint64_t)w1 << 32 | w0;
+    return SlStep{(int64_t)((uint64_t)a1 << 32 | a0), (int64_t)(wv >> 1), (wv & 1u) != 0};
+}
+
 // One work item (an M tile over a run of segments) of the sliding GEMM.  Bs is
 // the block's LDS; the barrier before the first stash also orders this item's
-// staging after the previous item's last reads.
+// staging after the previous item's last reads.  The item's steps (one tap of
+// one K chunk each) come from the run's step table (A offset; B offset << 1 |
+// 1 when the step continues the previous step's chunk at the next tap).  All
+// steps of all chunks and segments form one software pipeline; while step t
+// runs its MFMAs,
+//   - the B chunk of step t + 1 waits in LDS and that of step t + 2 is in flight,
+//   - the A fragments of step t + 1 are in flight: every lane loads at the first
+//     tap of a chunk; at the next taps only lane 15 and the lanes at a volume
+//     row end load (the others pass an offset past the buffer resource's range,
+//     which reads zeros without a memory access) and the rest take the step's
+//     fragments slid one lane down (DPP row_shl:1), chosen per lane at the top
+//     of step t + 1,
+//   - the table entry of step t + 3 is in flight.
+// Every load is unconditional in the data flow and each loaded register is
+// consumed before it is loaded again, so no copy (and no wait for a load just
+// issued) sits in the loop.  Lane l reads 32-bit word l % 4 of a table entry
+// (a vector load, read out with readlane): a scalar-memory wait in the loop
+// would also drain the outstanding LDS reads.
 template <int NT, int RT, int MODE>
 __device__ __forceinline__ void sl_item(
     const Item it, u16 (&Bs)[2][SlShape<NT, MODE>::NB][SlShape<NT, MODE>::NP * SlShape<NT, MODE>::LDK],
-    const Seg *__restrict__ segs, const uint32_t *__restrict__ tmap, const u16 *__restrict__ A0,
+    const int64_t *__restrict__ steps, const int32_t *__restrict__ run0, const u16 *__restrict__ A0,
     const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1, float *__restrict__ part,
-    int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
+    int nrows, int Ly, int YP, int S, int64_t slab_stride)
 {
     using SH = SlShape<NT, MODE>;
     constexpr int NP = SH::NP;
@@ -353,7 +394,7 @@ __device__ __forceinline__ void sl_item(
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int l16 = lane & 15, g = lane >> 4;
 
-    int64_t rowoff[RT];
+    uint32_t aoff[RT];   // byte offset of the lane's A window from a step's A offset
     bool fresh[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
@@ -362,9 +403,10 @@ __device__ __forceinline__ void sl_item(
         const int rn = r0 + 1 < nrows ? r0 + 1 : nrows - 1;
         const int Px = r / Ly, Py = r - Px * Ly;
         const int Pxn = rn / Ly, Pyn = rn - Pxn * Ly;
-        rowoff[rt] = ((int64_t)Px * YP + Py) * S;
+        const int64_t ro = ((int64_t)Px * YP + Py) * S;
         const int64_t next = ((int64_t)Pxn * YP + Pyn) * S;
-        fresh[rt] = l16 == 15 || next != rowoff[rt] + S;
+        aoff[rt] = (uint32_t)((ro + 8 * g) * 2);
+        fresh[rt] = l16 == 15 || next != ro + S;
     }
     f4 acc[RT][NT];
 #pragma unroll
@@ -372,104 +414,91 @@ __device__ __forceinline__ void sl_item(
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
 
-    for (int s = it.seg0; s < it.seg1; ++s) {
-        const Seg sg = segs[s];
-        const u16 *Ah = A0 + sg.a_off, *Al = A1 + sg.a_off;
-        const u16 *Wh = W0 + sg.w_off2, *Wl = W1 + sg.w_off2;
-        const int nkc = KT / KW;
-        const uint32_t *tab = tmap + sg.tmap;
-        for (int kc = 0; kc < nkc; ++kc) {
-            // taps [ky0, ky1) hold every nonzero weight tile of this chunk (uniform across the block)
-            const uint32_t hull = tab[kc];
-            const int ky0 = (int)(hull & 0xffffu), ky1 = (int)(hull >> 16);
-            if (ky0 >= ky1) continue;
-            // step t = ky: B chunk (ky, kc) at W + (ky * nkc + kc) * NP * KW
-            u32x4 pre[NB][VPT];
-            // A fragments for tap ky0 (all lanes) and tap ky0 + 1 (slid + fresh lanes)
-            bf8 ah[RT], al[RT], nh[RT], nl[RT];
-            const int ko = kc * KW + 8 * g;
+    const int step0 = run0[it.slab], nst = run0[it.slab + 1] - step0;
+    if (nst > 0) {
+        const int *ent = (const int *)(steps + 2 * (int64_t)step0) + (lane & 3);
+        // A fragments of step e (all lanes, or only the fresh ones when it slides)
+        auto load_a = [&](const SlStep &e, bf8 (&h)[RT], bf8 (&l)[RT]) {
+            const __amdgpu_buffer_rsrc_t rh = a_rsrc(A0 + e.a), rl = a_rsrc(A1 + e.a);
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
-                const int64_t o = rowoff[rt] + (int64_t)ky0 * S + ko;
-                ah[rt] = *(const bf8 *)(Ah + o);
-                al[rt] = *(const bf8 *)(Al + o);
+                const uint32_t o = (!e.slide || fresh[rt]) ? aoff[rt] : kNoLoad;
+                h[rt] = __builtin_bit_cast(bf8, __builtin_amdgcn_raw_buffer_load_b128(rh, o, 0, 0));
+                l[rt] = __builtin_bit_cast(bf8, __builtin_amdgcn_raw_buffer_load_b128(rl, o, 0, 0));
             }
-            sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)ky0 * nkc + kc) * NP * KW);
-            __syncthreads();   // previous chunk's buffers consumed
-            sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[0][0][0]);
-            if (ky0 + 1 < ky1) {
+        };
+        // The loop body is one basic block (entries past the run are clamped to
+        // its last step, whose loads are then repeated and unused, and the last
+        // step stages an unused B chunk), so the compiler's wait counts stay
+        // exact across iterations and no load sinks below the MFMAs.
+        const int last = nst - 1;
+        u32x4 pre[NB][VPT];
+        bf8 ldh[RT], ldl[RT], slh[RT], sll[RT];
+        const int v1 = ent[4 * min(1, last)], v2 = ent[4 * min(2, last)];
+        const SlStep e0 = sl_decode(ent[0]);
+        load_a(e0, ldh, ldl);
+        sl_fetch_b<NP, KW, NB, VPT>(pre, W0, W1, e0.w);
+        bool slide_t = false;   // the first step of a run starts a chunk
+        SlStep e_1 = sl_decode(v1), e_2 = sl_decode(v2);   // entries of steps t + 1, t + 2
+        int vn = ent[4 * min(3, last)];                   // entry word of step t + 3, in flight
+        __syncthreads();   // previous item's buffers consumed
+        sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[0][0][0]);
+        sl_fetch_b<NP, KW, NB, VPT>(pre, W0, W1, e_1.w);
+        __syncthreads();
+        for (int t = 0, buf = 0; t < nst; ++t, buf ^= 1) {
+            // A fragments of step t: loaded, or the previous step's slid one lane down
+            bf8 ah[RT], al[RT];
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) {
-                    nh[rt] = dpp_row_shl1(ah[rt]);
-                    nl[rt] = dpp_row_shl1(al[rt]);
-                    if (fresh[rt]) {
-                        const int64_t o = rowoff[rt] + (int64_t)(ky0 + 1) * S + ko;
-                        nh[rt] = *(const bf8 *)(Ah + o);
-                        nl[rt] = *(const bf8 *)(Al + o);
-                    }
-                }
-                sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)(ky0 + 1) * nkc + kc) * NP * KW);
+            for (int rt = 0; rt < RT; ++rt) {
+                const bool ld = !slide_t || fresh[rt];
+                ah[rt] = ld ? ldh[rt] : slh[rt];
+                al[rt] = ld ? ldl[rt] : sll[rt];
+                slh[rt] = dpp_row_shl1(ah[rt]);
+                sll[rt] = dpp_row_shl1(al[rt]);
             }
-            __syncthreads();
-            for (int ky = ky0; ky < ky1; ++ky) {
-                const int buf = (ky - ky0) & 1;
-                {
-                    // B fragments through a ring of kBR register slots (NB fragments each):
-                    // fragment ct + kBR is read right after the MFMAs of fragment ct are
-                    // issued, so each LDS read has kBR - 1 fragments' MFMAs to land under
-                    // (the scheduling groups keep that order).
-                    constexpr int kBR = 3;
-                    constexpr int kMf = (MODE == 2 ? 3 : 2) * RT;   // MFMAs per fragment
-                    const u16 *bb = &Bs[buf][0][l16 * LDK + 8 * g];
-                    bf8 ring[kBR][NB];
+            load_a(e_1, ldh, ldl);   // A of step t + 1
+            {
+                // B fragments through a ring of kBR register slots (NB fragments each):
+                // fragment ct + kBR is read right after the MFMAs of fragment ct are
+                // issued, so each LDS read has kBR - 1 fragments' MFMAs to land under
+                // (the scheduling groups keep that order).
+                constexpr int kBR = 3;
+                constexpr int kMf = (MODE == 2 ? 3 : 2) * RT;   // MFMAs per fragment
+                const u16 *bb = &Bs[buf][0][l16 * LDK + 8 * g];
+                bf8 ring[kBR][NB];
 #pragma unroll
-                    for (int i = 0; i < kBR; ++i)
+                for (int i = 0; i < kBR; ++i)
 #pragma unroll
-                        for (int h = 0; h < NB; ++h) ring[i][h] = *(const bf8 *)(bb + h * NP * LDK + i * 16 * LDK);
-                    __builtin_amdgcn_sched_group_barrier(0x100, kBR * NB, 0);   // the first reads
+                    for (int h = 0; h < NB; ++h) ring[i][h] = *(const bf8 *)(bb + h * NP * LDK + i * 16 * LDK);
+                __builtin_amdgcn_sched_group_barrier(0x020, 2 * RT, 0);     // the A loads of step t + 1 first
+                __builtin_amdgcn_sched_group_barrier(0x100, kBR * NB, 0);   // the first reads
 #pragma unroll
-                    for (int ct = 0; ct < NT; ++ct) {
-                        const bf8 bh = ring[ct % kBR][0], bl = ring[ct % kBR][NB - 1];
-#pragma unroll
-                        for (int rt = 0; rt < RT; ++rt) {
-                            f4 c = acc[rt][ct];
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
-                            if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
-                            acc[rt][ct] = c;
-                        }
-                        __builtin_amdgcn_sched_group_barrier(0x008, kMf, 0);   // this fragment's MFMAs
-                        if (ct + kBR < NT) {
-#pragma unroll
-                            for (int h = 0; h < NB; ++h)
-                                ring[ct % kBR][h] = *(const bf8 *)(bb + h * NP * LDK + (ct + kBR) * 16 * LDK);
-                            __builtin_amdgcn_sched_group_barrier(0x100, NB, 0);   // then the next reads
-                        }
-                    }
-                }
-                if (ky + 1 < ky1) {
-                    sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[buf ^ 1][0][0]);   // B of tap ky + 1
+                for (int ct = 0; ct < NT; ++ct) {
+                    const bf8 bh = ring[ct % kBR][0], bl = ring[ct % kBR][NB - 1];
 #pragma unroll
                     for (int rt = 0; rt < RT; ++rt) {
-                        ah[rt] = nh[rt];
-                        al[rt] = nl[rt];
+                        f4 c = acc[rt][ct];
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bh, c, 0, 0, 0);
+                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rt], bh, c, 0, 0, 0);
+                        if constexpr (MODE == 2) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rt], bl, c, 0, 0, 0);
+                        acc[rt][ct] = c;
                     }
-                    if (ky + 2 < ky1) {   // A of tap ky + 2 and B of tap ky + 2
+                    __builtin_amdgcn_sched_group_barrier(0x008, kMf, 0);   // this fragment's MFMAs
+                    if (ct + kBR < NT) {
 #pragma unroll
-                        for (int rt = 0; rt < RT; ++rt) {
-                            nh[rt] = dpp_row_shl1(ah[rt]);
-                            nl[rt] = dpp_row_shl1(al[rt]);
-                            if (fresh[rt]) {
-                                const int64_t o = rowoff[rt] + (int64_t)(ky + 2) * S + ko;
-                                nh[rt] = *(const bf8 *)(Ah + o);
-                                nl[rt] = *(const bf8 *)(Al + o);
-                            }
-                        }
-                        sl_fetch_b<NP, KW, NB, VPT>(pre, Wh, Wl, ((int64_t)(ky + 2) * nkc + kc) * NP * KW);
+                        for (int h = 0; h < NB; ++h)
+                            ring[ct % kBR][h] = *(const bf8 *)(bb + h * NP * LDK + (ct + kBR) * 16 * LDK);
+                        __builtin_amdgcn_sched_group_barrier(0x100, NB, 0);   // then the next reads
                     }
-                    __syncthreads();
                 }
             }
+            sl_stash_b<NP, KW, LDK, NB, VPT>(pre, &Bs[buf ^ 1][0][0]);   // B of step t + 1
+            slide_t = e_1.slide;
+            sl_fetch_b<NP, KW, NB, VPT>(pre, W0, W1, e_2.w);               // B of step t + 2
+            e_1 = e_2;
+            e_2 = sl_decode(vn);
+            vn = ent[4 * min(t + 4, last)];
+            __syncthreads();
         }
     }
 
@@ -489,16 +518,16 @@ __device__ __forceinline__ void sl_item(
 // running beside it on another stream.
 template <int NT, int RT, int MODE>
 __global__ __launch_bounds__(NTHREADS, (MODE == 2 || RT > 2) ? 1 : 2) void k_proj_gemm_sl(
-    const Item *__restrict__ items, int nitems, const Seg *__restrict__ segs, const uint32_t *__restrict__ tmap,
+    const Item *__restrict__ items, int nitems, const int64_t *__restrict__ steps, const int32_t *__restrict__ run0,
     const u16 *__restrict__ A0, const u16 *__restrict__ A1, const u16 *__restrict__ W0, const u16 *__restrict__ W1,
-    float *__restrict__ part, int nrows, int Ly, int YP, int S, int KT, int64_t slab_stride)
+    float *__restrict__ part, int nrows, int Ly, int YP, int S, int64_t slab_stride)
 {
     using SH = SlShape<NT, MODE>;
     __shared__ u16 Bs[2][SH::NB][SH::NP * SH::LDK];
     for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
         const Item it = items[i];
         if (it.mtile >= 0)
-            sl_item<NT, RT, MODE>(it, Bs, segs, tmap, A0, A1, W0, W1, part, nrows, Ly, YP, S, KT, slab_stride);
+            sl_item<NT, RT, MODE>(it, Bs, steps, run0, A0, A1, W0, W1, part, nrows, Ly, YP, S, slab_stride);
     }
 }
 
@@ -1065,12 +1094,40 @@ static int32_t seg_table(std::vector<uint32_t> &tab, const std::vector<uint8_t> 
     return (int32_t)at;
 }
 
+// Step table of the BF16 GEMM for the run of segments [s0, s1): one entry per
+// (segment, chunk kc, tap ky of the chunk's hull), in the order the GEMM runs
+// them, appended to g.steps.  Entry = (A offset of the tap's window, B offset
+// of chunk (ky, kc) << 1 | 1 when ky continues the previous entry's chunk).
+static void append_steps(GemmPlan &g, int s0, int s1, int64_t S, int64_t NP, int nkc)
+{
+    for (int s = s0; s < s1; ++s) {
+        const Seg &sg = g.segs[s];
+        for (int kc = 0; kc < nkc; ++kc) {
+            const uint32_t h = g.tmap[sg.tmap + kc];
+            const int ky0 = (int)(h & 0xffffu), ky1 = (int)(h >> 16);
+            for (int ky = ky0; ky < ky1; ++ky) {
+                g.steps.push_back(sg.a_off + (int64_t)ky * S + (int64_t)kc * 32);
+                g.steps.push_back((sg.w_off2 + ((int64_t)ky * nkc + kc) * NP * 32) << 1 | (ky > ky0 ? 1 : 0));
+            }
+        }
+    }
+    g.run0.push_back((int32_t)(g.steps.size() / 2));
+}
+
 static int upload_plan(GemmPlan &g)
 {
     if (g.d_segs) { (void)hipFree(g.d_segs); g.d_segs = nullptr; }
     if (g.d_items) { (void)hipFree(g.d_items); g.d_items = nullptr; }
     if (g.d_plane_slab0) { (void)hipFree(g.d_plane_slab0); g.d_plane_slab0 = nullptr; }
     if (g.d_tmap) { (void)hipFree(g.d_tmap); g.d_tmap = nullptr; }
+    if (g.d_steps) { (void)hipFree(g.d_steps); g.d_steps = nullptr; }
+    if (g.d_run0) { (void)hipFree(g.d_run0); g.d_run0 = nullptr; }
+    SID_TRY_HIP(hipMalloc(&g.d_steps, std::max<size_t>(g.steps.size(), 2) * sizeof(int64_t)));
+    SID_TRY_HIP(hipMalloc(&g.d_run0, std::max<size_t>(g.run0.size(), 1) * sizeof(int32_t)));
+    if (!g.steps.empty())
+        SID_TRY_HIP(hipMemcpy(g.d_steps, g.steps.data(), g.steps.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (!g.run0.empty())
+        SID_TRY_HIP(hipMemcpy(g.d_run0, g.run0.data(), g.run0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     SID_TRY_HIP(hipMalloc(&g.d_tmap, std::max<size_t>(g.tmap.size(), 1) * sizeof(uint32_t)));
     if (!g.tmap.empty())
         SID_TRY_HIP(hipMemcpy(g.d_tmap, g.tmap.data(), g.tmap.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -1132,6 +1189,10 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     sp.Lx = (X + n - 1) / n; sp.Ly = (Y + n - 1) / n;
     sp.XP = sp.Lx + 2 * T; sp.YP = sp.Ly + 2 * T + 1;
     sp.nrows = sp.Lx * sp.Ly;
+    if (sp.XP * sp.YP * S * 2 >= (1LL << 31) - 64) {   // BF16 GEMM lane offsets are 32-bit byte offsets
+        set_error("sensor %lld x %lld too large for the projector's 32-bit plane offsets", (long long)X, (long long)Y);
+        return SID_ERR_DIMS;
+    }
     sp.mt = mt;
     sp.ntiles = (sp.nrows + mt - 1) / mt;
     sp.mpad = sp.ntiles * mt;
@@ -1168,7 +1229,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
 
     // ---- forward: segments over planes 0..nz-1 and their dx rows
     GemmPlan &F = sp.fwd;
-    F.segs.clear(); F.items.clear(); F.tmap.clear(); F.cost.clear(); F.macs = 0;
+    F.segs.clear(); F.items.clear(); F.tmap.clear(); F.cost.clear(); F.steps.clear(); F.run0.assign(1, 0); F.macs = 0;
     for (int64_t c = 0; c < nz; ++c) {
         const Plane &pl = p->planes[c];
         if (pl.empty || use[c]) continue;
@@ -1198,6 +1259,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
         for (int gi = 0; gi < (int)groups.size(); ++gi) {
             for (int t = 0; t < sp.ntiles; ++t) gi_items[gi].push_back(Item{groups[gi].first, groups[gi].second, t, gi});
             for (int s = groups[gi].first; s < groups[gi].second; ++s) gi_cost[gi] += F.cost[s];
+            append_steps(F, groups[gi].first, groups[gi].second, S, NP, nkc);   // run of slab gi
         }
         F.items = xcd_order(gi_items, gi_cost);
         // MFMA multiply-adds the GEMM runs: the executed K rows (tap hulls / live chunks) x N x padded M
@@ -1207,6 +1269,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
     // ---- backward: every plane of H, split per plane in proportion to cost
     GemmPlan &B = sp.bwd;
     B.segs.clear(); B.items.clear(); B.tmap.clear(); B.cost.clear(); B.plane_slab0.assign(p->nzH + 1, 0); B.macs = 0;
+    B.steps.clear(); B.run0.assign(1, 0);
     std::vector<int> pseg0(p->nzH + 1, 0);
     int64_t btotal = 0;
     for (int64_t c = 0; c < p->nzH; ++c) {
@@ -1248,6 +1311,7 @@ int ensure_plan(sid_psf *p, int64_t X, int64_t Y, int64_t nz)
                 gr_items.emplace_back();
                 for (int t = 0; t < sp.ntiles; ++t) gr_items.back().push_back(Item{gr.first, gr.second, t, slab});
                 gr_cost.push_back(gc);
+                append_steps(B, gr.first, gr.second, S, NP, nkc);   // run of this slab
                 ++slab;
             }
         }
@@ -1334,12 +1398,12 @@ static int launch_nt(sid_psf *p, const GemmPlan &g, int dir, hipStream_t st, int
         const u16 *W1 = p->precision == SID_PREC_BF16X3 ? (fwd ? p->wf_lo : p->wb_lo) : W0;
         if (p->precision == SID_PREC_BF16X2)
             hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 1>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems,
-                               g.d_segs, g.d_tmap, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
-                               (int)p->KT, slab);
+                               g.d_steps, g.d_run0, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               slab);
         else
             hipLaunchKernelGGL((k_proj_gemm_sl<NT, 2, 2>), dim3(grid), dim3(NTHREADS), 0, st, g.d_items, nitems,
-                               g.d_segs, g.d_tmap, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
-                               (int)p->KT, slab);
+                               g.d_steps, g.d_run0, A0, A1, W0, W1, sp.part, (int)sp.nrows, (int)sp.Ly, (int)sp.YP, S,
+                               slab);
     }
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;

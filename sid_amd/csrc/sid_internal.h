@@ -95,6 +95,10 @@ struct GemmPlan {             // one direction on one sensor geometry
     std::vector<uint32_t> tmap;       // zero-tile tables of the BF16 segments (Seg::tmap)
     std::vector<int64_t> cost;        // per segment: K rows the GEMM runs at the plan's precision
     uint32_t *d_tmap = nullptr;
+    std::vector<int64_t> steps;       // BF16 GEMM step table: per step (A offset, B offset << 1 | continues chunk)
+    std::vector<int32_t> run0;        // first step of slab i's run of segments (size nslabs + 1)
+    int64_t *d_steps = nullptr;
+    int32_t *d_run0 = nullptr;
     int nslabs = 0;
     int64_t macs = 0;                 // MFMA multiply-adds issued (padding included)
 };
