@@ -1098,18 +1098,27 @@ static int32_t seg_table(std::vector<uint32_t> &tab, const std::vector<uint8_t> 
 // (segment, chunk kc, tap ky of the chunk's hull), in the order the GEMM runs
 // them, appended to g.steps.  Entry = (A offset of the tap's window, B offset
 // of chunk (ky, kc) << 1 | 1 when ky continues the previous entry's chunk).
+// Within each plane of the run the chunk kc is the outer loop and the tap
+// rows (segments) the inner one: the A windows of tap rows dx and dx + 1 are
+// the same lenslet rows shifted by one, so the M tiles of a group that run
+// together on one XCD re-read a chunk's 32 elements of the plane's lenslet
+// records from L2 instead of fetching the plane once per tap row.
 static void append_steps(GemmPlan &g, int s0, int s1, int64_t S, int64_t NP, int nkc)
 {
-    for (int s = s0; s < s1; ++s) {
-        const Seg &sg = g.segs[s];
-        for (int kc = 0; kc < nkc; ++kc) {
-            const uint32_t h = g.tmap[sg.tmap + kc];
-            const int ky0 = (int)(h & 0xffffu), ky1 = (int)(h >> 16);
-            for (int ky = ky0; ky < ky1; ++ky) {
-                g.steps.push_back(sg.a_off + (int64_t)ky * S + (int64_t)kc * 32);
-                g.steps.push_back((sg.w_off2 + ((int64_t)ky * nkc + kc) * NP * 32) << 1 | (ky > ky0 ? 1 : 0));
+    for (int p0 = s0; p0 < s1;) {
+        int p1 = p0 + 1;
+        while (p1 < s1 && g.segs[p1].plane == g.segs[p0].plane) ++p1;
+        for (int kc = 0; kc < nkc; ++kc)
+            for (int s = p0; s < p1; ++s) {
+                const Seg &sg = g.segs[s];
+                const uint32_t h = g.tmap[sg.tmap + kc];
+                const int ky0 = (int)(h & 0xffffu), ky1 = (int)(h >> 16);
+                for (int ky = ky0; ky < ky1; ++ky) {
+                    g.steps.push_back(sg.a_off + (int64_t)ky * S + (int64_t)kc * 32);
+                    g.steps.push_back((sg.w_off2 + ((int64_t)ky * nkc + kc) * NP * 32) << 1 | (ky > ky0 ? 1 : 0));
+                }
             }
-        }
+        p0 = p1;
     }
     g.run0.push_back((int32_t)(g.steps.size() / 2));
 }
