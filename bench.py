@@ -35,11 +35,12 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, V, y_pre, sample_planes=None):
+def cpu_baseline(wl, H_host, htf_host):
     """Reference CPU path restated in C (oracle/liboracle.so: forwardProjectACC
-    in single precision, backwardProjectACC_new stored double, CAindex crop)
-    timed on the host cores over a sample of planes; scaled to one RL
-    iteration by the algorithmic work ratio sum(M_c^2)."""
+    in single precision, backwardProjectACC_new stored double, CAindex crop,
+    deconvRL's basic update), one whole RL iteration of the full workload on
+    all host cores, started from the device's Htf (the input of the timed GPU
+    iterations); plus the small workload end to end."""
     from oracle.build_oracle import build as build_oracle
     from sid_amd import synth
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "liboracle.so")
@@ -47,33 +48,23 @@ def cpu_baseline(wl, V, y_pre, sample_planes=None):
         build_oracle()
     L = ctypes.CDLL(path)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    nz = wl.nz
-    if sample_planes is None:
-        sample_planes = sorted({0, nz // 4, nz // 2, (3 * nz) // 4})
-    zs = [wl.zs[c] for c in sample_planes]
-    Hs, CAs = synth.make_psf(wl.n, wl.M, zs, dtype=np.float32)
-    Vs = np.asfortranarray(V[:, :, sample_planes].astype(np.float64))
-    X, Y = wl.X, wl.Y
-    O = np.zeros((X, Y), dtype=np.float32, order="F")
-    B = np.zeros((X, Y, len(zs)), dtype=np.float64, order="F")
-    yf = np.asfortranarray(y_pre.astype(np.float32))
-    hd = (ctypes.c_int64 * 5)(*Hs.shape)
-    ca = np.asfortranarray(CAs.astype(np.int32))
+    X, Y, nz = wl.X, wl.Y, wl.nz
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    # H_host holds MATLAB's column-major [M, M, n, n, Nz] memory; htf_host [X, Y, Nz]
+    hd = (ctypes.c_int64 * 5)(wl.M, wl.M, wl.n, wl.n, nz)
+    ca = np.asfortranarray(synth.ca_index(wl.zs, wl.M).astype(np.int32))
+    htf = np.ascontiguousarray(htf_host, dtype=np.float64)
+    Xg = htf.copy()
+    img = np.zeros((X, Y), dtype=np.float32, order="F")
+    D = np.zeros_like(htf)
     t0 = time.perf_counter()
-    L.orc_forward_f32(P(Hs), hd, P(ca), P(Vs), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(len(zs)), P(O),
-                      ctypes.c_int(threads))
+    L.orc_isra_f32(P(H_host), hd, P(ca), P(htf), P(Xg), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(nz),
+                   ctypes.c_int(1), P(img), P(D), ctypes.c_int(threads))
     t1 = time.perf_counter()
-    L.orc_backward_f32(P(Hs), hd, P(ca), P(yf), ctypes.c_int64(X), ctypes.c_int64(Y), ctypes.c_int64(len(zs)), P(B),
-                       ctypes.c_int(threads))
-    t2 = time.perf_counter()
-    msum_all = sum((hi - lo + 1) ** 2 for lo, hi in synth.ca_index(wl.zs, wl.M))
-    msum_s = sum((hi - lo + 1) ** 2 for lo, hi in CAs)
-    t_iter = (t2 - t0) * msum_all / msum_s
-    out = {"value": 1.0 / t_iter, "unit": "RL iter/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
-           "sample": (f"forwardProjectACC (single) + backwardProjectACC_new restated in C (OpenMP, -O3), "
-                      f"planes z={zs} of the same workload, {t2 - t0:.2f} s measured (fwd {t1 - t0:.2f} s, "
-                      f"bwd {t2 - t1:.2f} s), scaled by sum(M_c^2) all/sample = {msum_all}/{msum_s}")}
+    out = {"value": 1.0 / (t1 - t0), "unit": "RL iter/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "sample": (f"one whole RL iteration of the full workload ({X}x{Y} px, {nz} planes, CAindex crops): "
+                      f"forwardProjectACC (single) + backwardProjectACC_new (double) + the deconvRL update, "
+                      f"restated in C (OpenMP, -O3), {t1 - t0:.2f} s from the device's Htf")}
     out["rl_small"] = cpu_rl_small(L, threads)
     return out
 
@@ -682,13 +673,19 @@ def main():
         if out_rows and "gradient" in out_rows:
             out["projections"]["grad"] = {"fp32": out_rows["gradient"]["projections_fp32_direct"]}
     # release the device state before the CPU leg and the exit
+    htf_host = htfs[base].cpu().numpy() if not args.no_cpu else None
     psf.close()
     del Vt, yt, htfs, clean
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     if not args.no_cpu:
-        log("CPU baseline (bounded sample, and the small workload in full) ...")
-        out["cpu_baseline"] = cpu_baseline(wl, V, y)
+        log("CPU baseline (one RL iteration of the full workload, and the small workload in full) ...")
+        Hd, _ = synth.make_psf(wl.n, wl.M, wl.zs, xp=torch, device=dev)
+        H_host = Hd.cpu().numpy()   # (Nz, n, n, M, M) C order = MATLAB [M, M, n, n, Nz]
+        del Hd
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(wl, H_host, htf_host)
+        del H_host
         log(f"CPU baseline {out['cpu_baseline']['value']:.5f} RL iter/s on {out['cpu_baseline']['cores']} threads "
             f"({out['cpu_baseline']['cpu_model']}); rl_small {out['cpu_baseline']['rl_small']['iter_per_s']} RL iter/s")
     print(json.dumps(out), flush=True)
