@@ -250,6 +250,7 @@ static int psf_create_impl(const float *H, bool host, const int64_t dims[5], con
         p->overlap = !(ser && ser[0] && ser[0] != '0');
         const char *gb = std::getenv("SID_GEMM_BLOCKS");
         p->gemm_blocks = gb ? std::max(0, std::atoi(gb)) : 0;
+        p->gemm_blocks_env = p->gemm_blocks > 0;
         if (p->gemm_blocks == 0) {
             hipDeviceProp_t prop;
             p->gemm_blocks = hipGetDeviceProperties(&prop, p->device) == hipSuccess ? prop.multiProcessorCount : 256;
@@ -522,7 +523,7 @@ static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph
     SID_TRY_HIP(hipEventRecord(p->ev_fork, st));
     SID_TRY_HIP(hipStreamWaitEvent(p->aux, p->ev_fork, 0));
     const int a = tm.mark(p->aux);
-    const int rc_g = launch_gemm(p, dir, p->aux, p->gemm_blocks);
+    const int rc_g = launch_gemm(p, dir, p->aux, dir == 0 && p->gemm_blocks_f > 0 ? p->gemm_blocks_f : p->gemm_blocks);
     const int b = tm.mark(p->aux);
     // join even after a failed launch so that st never runs ahead of work left on aux
     SID_TRY_HIP(hipEventRecord(p->ev_join, p->aux));
@@ -591,6 +592,7 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
     SID_TRY(autotune_methods(p, X, Y, nz, st));
     if (p->method != SID_METHOD_AUTO || p->precision == SID_PREC_FP64 || p->auto_refined) return SID_OK;
     p->auto_refined = true;
+    p->gemm_blocks_f = 0;
     if (!p->overlap || !p->pin.empty() || p->auto_fft.size() != (size_t)p->nzH) return SID_OK;
     std::vector<int> order;  // eligible planes, widest support first
     for (int64_t c = 0; c < p->nzH; ++c)
@@ -635,6 +637,34 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
     if (rc == SID_OK) {
         use_prefix(best_k);
         rc = ensure_plan(p, X, Y, nz);
+    }
+    // The forward's GEMM half is the longer one at the chosen split (its A
+    // operand is a volume plane per plane, the backward's the one image), so a
+    // larger grid cap for the forward alone may shorten the pair: 1.25 and 1.5
+    // blocks per CU are tried on the same terms as the split (at 1005^2 x 51,
+    // BF16X2, 26 FFT planes: 1.25 per CU 115.9 RL iter/s against 114.1).
+    p->gemm_blocks_f = 0;
+    if (rc == SID_OK && !p->gemm_blocks_env && best_k > 0) {
+        auto timed = [&](int cap) {
+            p->gemm_blocks_f = cap;
+            const double t = time_projection_pair(p, st, &rc);
+            if (log) std::fprintf(stderr, "sid autotune: forward GEMM cap %d blocks: %.3f ms per pair\n",
+                                  cap ? cap : p->gemm_blocks, t);
+            return t;
+        };
+        int best_cap = 0;   // 0: the common cap
+        double t_inc = timed(best_cap);
+        for (int num : {5, 6}) {
+            if (rc != SID_OK) break;
+            const int cap = (p->gemm_blocks * num / 4 + 7) / 8 * 8;
+            const double t = timed(cap);
+            if (rc != SID_OK || t >= 0.99 * t_inc) break;
+            const double t_again = timed(best_cap);   // the incumbent on both sides, as above
+            if (rc != SID_OK || t >= 0.99 * t_again) break;
+            best_cap = cap;
+            t_inc = t;
+        }
+        p->gemm_blocks_f = best_cap;
     }
     double d0, d1;
     fft_timing_collect(p, &d0, &d1);  // drop the GEMV events of the trial runs

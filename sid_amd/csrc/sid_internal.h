@@ -200,6 +200,8 @@ struct sid_psf {
     // stream (fork / join events); off when SID_NO_OVERLAP is set at creation
     bool overlap = true;
     int gemm_blocks = 0;              // grid cap of the GEMM beside the FFT path (0: one block per CU)
+    int gemm_blocks_f = 0;            // the forward's cap when the autotuner chose another one (0: gemm_blocks)
+    bool gemm_blocks_env = false;     // the cap was set with SID_GEMM_BLOCKS (not tuned)
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_null = nullptr;     // orders the handle's stream after the default stream (stream = NULL)
