@@ -203,6 +203,24 @@ int sid_library_forward(sid_psf *psf, int64_t n_seeds, const int64_t *boxes, con
 /* {rows, work items, MFMA multiply-adds issued (chunks of all-zero weights are
  * skipped), accumulator elements} of the cached library plan. */
 int sid_library_stats(const sid_psf *psf, int64_t stats[4]);
+/* The kernel option of generate_LFM_library_GPU (src/generate_LFM_library_GPU.m:
+ * 54-56,95-101: forwardFUN = forwardFUN_raw(convn(Volume, kern, 'same')) on
+ * each seed's image_size x (z1-z0+1) volume).  The seed volume is zero outside
+ * its box, so its convolution lives on the box grown by the kernel's reach,
+ * x in [x0 - floor(kx/2), x1 + kx - 1 - floor(kx/2)] and likewise y, clipped to
+ * the sensor, on the box's own planes.  sid_library_kernel_boxes gives the
+ * grown boxes (n x 6, host only) and their value offsets (n + 1);
+ * sid_library_convn_seeds(_device) convolves every seed's box values (FP32, the
+ * library's arithmetic) into values on its grown box, which then go through
+ * sid_library_forward(_device) with the grown boxes.  kdims = {kx, ky, kz} up to
+ * 65 x 65 x 255, k column-major; val_offsets as for sid_library_forward. */
+int sid_library_kernel_boxes(int64_t n_seeds, const int64_t *boxes, const int64_t sensor[2], const int64_t kdims[3],
+                             int64_t *boxes_out, int64_t *val_offsets_out);
+int sid_library_convn_seeds_device(int64_t n_seeds, const int64_t *boxes, const float *vals_dev,
+                                   const int64_t *val_offsets, const int64_t sensor[2], const float *k_dev,
+                                   const int64_t kdims[3], float *vals_out_dev, void *stream);
+int sid_library_convn_seeds(int64_t n_seeds, const int64_t *boxes, const float *vals, const int64_t *val_offsets,
+                            const int64_t sensor[2], const float *k, const int64_t kdims[3], float *vals_out);
 
 /* --- Library of normalised footprints (src/generate_LFM_library_CPU.m:1-70) ---
  * forward_model = generate_LFM_library_CPU(centers, psf_ballistic, r, dim, m, kernel):

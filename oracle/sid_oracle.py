@@ -354,10 +354,11 @@ def library_gpu_placement(center, neur_rad, axial, image_size, nz):
     return dict(x=x, y=y, z=z, z0=z0, z1=z1, x0=x0, x1=x1, y0=y0, y1=y1)
 
 
-def library_gpu_entry(recon, ids, center, neur_rad, axial, H):
+def library_gpu_entry(recon, ids, center, neur_rad, axial, H, kernel=None):
     """One entry of src/generate_LFM_library_GPU.m:67-108: build the
     full-sensor sub-volume (sum over the listed components of ``recon`` on the
-    clipped box, divided by their count, :76-85), slice H in z (:87-88) and
+    clipped box, divided by their count, :76-85), slice H in z (:87-88),
+    convolve it with ``kernel`` when one is given (convn 'same', :96-99) and
     forward-project it (:95,101)."""
     X, Y, nz = recon[0].shape
     p = library_gpu_placement(center, neur_rad, axial, (X, Y), H.shape[4])
@@ -368,6 +369,8 @@ def library_gpu_entry(recon, ids, center, neur_rad, axial, H):
     for k in ids:
         vol[sx, sy, :] += recon[k][sx, sy, sz]
     vol /= len(ids)
+    if kernel is not None:
+        vol = convn_same(vol, kernel)
     Hs = H[:, :, :, :, sz]
     return forward_project(Hs, vol), p
 

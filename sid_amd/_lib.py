@@ -50,6 +50,9 @@ SIGNATURES = {
     "sid_library_stats": (_I, [_P, _I64P]),
     "sid_library_forward": (_I, [_P, ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P]),
     "sid_patched_rect": (_I, [ctypes.c_int64, ctypes.c_int64, _I64P, _I64P, _I64P]),
+    "sid_library_kernel_boxes": (_I, [ctypes.c_int64, _I64P, _I64P, _I64P, _I64P, _I64P]),
+    "sid_library_convn_seeds_device": (_I, [ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P, _I64P, _P, _P]),
+    "sid_library_convn_seeds": (_I, [ctypes.c_int64, _I64P, _P, _I64P, _I64P, _P, _I64P, _P]),
     "sid_library_nsf_bound": (_I, [_P, ctypes.c_int64, _P, ctypes.c_double, _P, _I64P, _P, _I64P, _I64P]),
     "sid_library_nsf": (_I, [_P, ctypes.c_int64, _P, ctypes.c_double, _P, _I64P, _P, _I64P, _I64P, _I64P, _P,
                              ctypes.c_int64]),

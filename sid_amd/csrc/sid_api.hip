@@ -1145,6 +1145,64 @@ int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const
     return SID_OK;
 }
 
+int sid_library_kernel_boxes(int64_t n_seeds, const int64_t *boxes, const int64_t sensor[2], const int64_t kdims[3],
+                             int64_t *boxes_out, int64_t *val_offsets_out)
+{
+    if (n_seeds < 0 || (n_seeds > 0 && (!boxes || !boxes_out)) || !sensor || !kdims || !val_offsets_out) {
+        set_error("sid_library_kernel_boxes: null argument");
+        return SID_ERR_ARG;
+    }
+    SID_TRY(check_kdims(kdims));
+    const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
+    return library_kernel_boxes(n_seeds, boxes, sensor[0], sensor[1], kd, boxes_out, val_offsets_out);
+}
+
+int sid_library_convn_seeds_device(int64_t n_seeds, const int64_t *boxes, const float *vals_dev,
+                                   const int64_t *val_offsets, const int64_t sensor[2], const float *k_dev,
+                                   const int64_t kdims[3], float *vals_out_dev, void *stream)
+{
+    if (n_seeds < 0 || (n_seeds > 0 && (!boxes || !vals_dev || !val_offsets || !vals_out_dev)) || !sensor || !k_dev ||
+        !kdims) {
+        set_error("sid_library_convn_seeds_device: null argument");
+        return SID_ERR_ARG;
+    }
+    SID_TRY(check_kdims(kdims));
+    SID_TRY(check_device());
+    const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
+    return library_convn_seeds(n_seeds, boxes, vals_dev, val_offsets, sensor[0], sensor[1], k_dev, kd, vals_out_dev,
+                               (hipStream_t)stream);
+}
+
+int sid_library_convn_seeds(int64_t n_seeds, const int64_t *boxes, const float *vals, const int64_t *val_offsets,
+                            const int64_t sensor[2], const float *k, const int64_t kdims[3], float *vals_out)
+{
+    if (n_seeds < 0 || (n_seeds > 0 && (!boxes || !vals || !val_offsets || !vals_out)) || !sensor || !k || !kdims) {
+        set_error("sid_library_convn_seeds: null argument");
+        return SID_ERR_ARG;
+    }
+    SID_TRY(check_kdims(kdims));
+    if (n_seeds == 0) return SID_OK;
+    std::vector<int64_t> grown(6 * n_seeds), goff(n_seeds + 1);
+    const int kd[3] = {(int)kdims[0], (int)kdims[1], (int)kdims[2]};
+    SID_TRY(library_kernel_boxes(n_seeds, boxes, sensor[0], sensor[1], kd, grown.data(), goff.data()));
+    if (val_offsets[0] != 0) {
+        set_error("sid_library_convn_seeds: val_offsets[0] must be 0");
+        return SID_ERR_ARG;
+    }
+    SID_TRY(check_device());
+    const int64_t nin = val_offsets[n_seeds], nout = goff[n_seeds], nk = kdims[0] * kdims[1] * kdims[2];
+    DevBuf dv, dk, dout;
+    SID_TRY_HIP(hipMalloc(&dv.ptr, sizeof(float) * std::max<int64_t>(nin, 1)));
+    SID_TRY_HIP(hipMalloc(&dk.ptr, sizeof(float) * nk));
+    SID_TRY_HIP(hipMalloc(&dout.ptr, sizeof(float) * std::max<int64_t>(nout, 1)));
+    SID_TRY(copy_h2d(dv.ptr, vals, sizeof(float) * nin, nullptr));
+    SID_TRY_HIP(hipMemcpy(dk.ptr, k, sizeof(float) * nk, hipMemcpyHostToDevice));
+    SID_TRY(library_convn_seeds(n_seeds, boxes, (const float *)dv.ptr, val_offsets, sensor[0], sensor[1],
+                                (const float *)dk.ptr, kd, (float *)dout.ptr, nullptr));
+    SID_TRY(copy_d2h(vals_out, dout.ptr, sizeof(float) * nout, nullptr));
+    return SID_OK;
+}
+
 int sid_std_image_device(const float *Y, int64_t P, int64_t T, int bg_iter, int flags, float *s, float *t,
                          float *std_img, double *ymax, void *stream)
 {

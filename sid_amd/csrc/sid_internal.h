@@ -257,6 +257,10 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
                     int64_t Y, float *out, hipStream_t st, int use_graph);
 void lib_plan_free(sid_psf *p);
 int library_stats(const sid_psf *p, int64_t st[4]);
+int library_kernel_boxes(int64_t ns, const int64_t *boxes, int64_t X, int64_t Y, const int kd[3], int64_t *out,
+                         int64_t *voff_out);
+int library_convn_seeds(int64_t ns, const int64_t *boxes, const float *vals, const int64_t *voff, int64_t X,
+                        int64_t Y, const float *k_dev, const int kd[3], float *out, hipStream_t st);
 // sid_grad.hip
 int grad_core(sid_psf *p, const float *V, const float *y, const int64_t dims[4], int mode, double *nll, float *gV,
               float *gH, hipStream_t st);

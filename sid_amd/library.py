@@ -106,6 +106,94 @@ class Library:
         return flat.reshape((w, h), order="F")
 
 
+def kernel_boxes(boxes, sensor, kdims):
+    """Boxes grown by a kernel's reach (sid_library_kernel_boxes): (n x 6 boxes,
+    n + 1 value offsets)."""
+    boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int64).reshape(-1, 6))
+    ns = boxes.shape[0]
+    out = np.zeros_like(boxes)
+    voff = np.zeros(ns + 1, dtype=np.int64)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    check(lib().sid_library_kernel_boxes(ns, P(boxes), i64(*sensor), i64(*kdims), P(out), P(voff)))
+    return out, voff
+
+
+def convn_seeds(boxes, values, sensor, kernel):
+    """convn(Volume, kernel, 'same') of each seed's volume (zero outside its
+    box, src/generate_LFM_library_GPU.m:70-85,98) on the grown boxes
+    (sid_library_convn_seeds, FP32): returns (grown boxes, per-seed values as
+    column-major arrays)."""
+    k = np.asfortranarray(np.asarray(kernel, dtype=np.float32))
+    if k.ndim == 2:
+        k = k[:, :, None]
+    kd = tuple(int(v) for v in k.shape)
+    boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int64).reshape(-1, 6))
+    flat = [np.asarray(v, dtype=np.float32).ravel(order="F") for v in values]
+    voff = np.zeros(len(flat) + 1, dtype=np.int64)
+    voff[1:] = np.cumsum([f.size for f in flat])
+    vin = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros(1, np.float32))
+    grown, goff = kernel_boxes(boxes, sensor, kd)
+    vout = np.zeros(max(int(goff[-1]), 1), dtype=np.float32)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    check(lib().sid_library_convn_seeds(boxes.shape[0], P(boxes), vin.ctypes.data_as(ctypes.c_void_p), P(voff),
+                                        i64(*sensor), k.ctypes.data_as(ctypes.c_void_p), i64(*kd),
+                                        vout.ctypes.data_as(ctypes.c_void_p)))
+    out = []
+    for s in range(boxes.shape[0]):
+        g = grown[s]
+        shape = (int(g[1] - g[0] + 1), int(g[3] - g[2] + 1), int(g[5] - g[4] + 1))
+        out.append(vout[int(goff[s]):int(goff[s + 1])].reshape(shape, order="F"))
+    return grown, out
+
+
+def generate_LFM_library_GPU(recon, centers, neur_id, psf, options):
+    """src/generate_LFM_library_GPU.m:1-113: row s of the single
+    N x prod(image_size) result is the sensor image of centre s's sub-volume —
+    the mean of the reconstructions listed in neur_id(s, :) on its clipped box
+    (:68-85), convolved with generate_kernel(ker_shape, ker_param) when
+    options has ker_param (:54-56,95-101) — projected with planes z0..z1 of the
+    handle's PSF (:87-95), all seeds in one batched launch.  recon: list of
+    [X, Y, Nz] arrays; centers: N x 3 (1-based); options: dict with image_size,
+    axial, neur_rad and optionally ker_shape (default 'ball') and ker_param."""
+    from .recon import generate_kernel
+    X, Y = (int(v) for v in np.asarray(options["image_size"]).ravel()[:2])
+    r = int(options["neur_rad"])
+    dz = int(math.ceil(r / float(options["axial"])))
+    nz = psf.Nz
+    centers = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    neur_id = np.asarray(neur_id).reshape(centers.shape[0], -1)
+    boxes, values = [], []
+    for s in range(centers.shape[0]):
+        b = placement(centers[s], r, dz, (X, Y), nz)
+        ids = np.flatnonzero(neur_id[s])
+        acc = np.zeros((b[1] - b[0] + 1, b[3] - b[2] + 1, b[5] - b[4] + 1))
+        for k in ids:
+            acc += np.asarray(recon[k])[b[0] - 1:b[1], b[2] - 1:b[3], b[4] - 1:b[5]]
+        boxes.append(b)
+        values.append((acc / len(ids)).astype(np.float32) if len(ids) else acc.astype(np.float32))
+    if "ker_param" in options:
+        kern = generate_kernel(options.get("ker_shape", "ball"), options["ker_param"])
+        boxes, values = convn_seeds(boxes, values, (X, Y), kern)
+    boxes = np.ascontiguousarray(np.asarray(boxes, dtype=np.int64).reshape(-1, 6))
+    lb = Library(psf, (X, Y))
+    rects, offs = lb.rects(boxes)
+    flat = [np.asarray(v, dtype=np.float32).ravel(order="F") for v in values]
+    voff = np.zeros(len(flat) + 1, dtype=np.int64)
+    voff[1:] = np.cumsum([f.size for f in flat])
+    vin = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros(1, np.float32))
+    patches = np.zeros(max(int(offs[-1]), 1), dtype=np.float32)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    check(lib().sid_library_forward(psf._h, boxes.shape[0], P(boxes), vin.ctypes.data_as(ctypes.c_void_p), P(voff),
+                                    i64(X, Y), patches.ctypes.data_as(ctypes.c_void_p)))
+    fm = np.zeros((boxes.shape[0], X * Y), dtype=np.float32)
+    for s in range(boxes.shape[0]):
+        rc = rects[s]
+        img = np.zeros((X, Y), dtype=np.float32)
+        img[rc[0] - 1:rc[1], rc[2] - 1:rc[3]] = Library.patch(patches, rects, offs, s)
+        fm[s] = img.ravel(order="F")
+    return fm
+
+
 def generate_LFM_library_CPU(centers, psf, r, dim, m, kernel=None):
     """src/generate_LFM_library_CPU.m:1 on the device: the L2-normalised
     footprint of every centre (a ball of radius ``r`` with axial radius

@@ -423,8 +423,6 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         if (!mxIsCell(recon) || mxGetNumberOfElements(recon) < 1) arg_error("recon must be a non-empty cell array");
         opts = prhs[4];
         if (!mxIsStruct(opts)) arg_error("options must be a struct");
-        if (mxGetField(opts, 0, "ker_shape") && mxGetNumberOfElements(mxGetField(opts, 0, "ker_shape")) > 0)
-            arg_error("options.ker_shape: convolve the reconstructions with the kernel first (sid_convn_same)");
         if (!mxGetField(opts, 0, "image_size") || !mxGetField(opts, 0, "axial") || !mxGetField(opts, 0, "neur_rad"))
             arg_error("options needs image_size, axial and neur_rad");
         {
@@ -503,6 +501,38 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
         mxFree(ids);
         sensor[0] = X;
         sensor[1] = Y;
+        /* kernel = generate_kernel(options.ker_shape, options.ker_param) when ker_param is set (:50-56; the
+         * reference's own generate_kernel.m, called back), forwardFUN = forwardFUN_raw(convn(V, kern,
+         * 'same')) (:95-101): each seed moves to its box grown by the kernel's reach, with the convolved
+         * values */
+        if (mxGetField(opts, 0, "ker_param") && mxGetNumberOfElements(mxGetField(opts, 0, "ker_param")) > 0) {
+            mxArray *kin[2], *kout[1] = {NULL};
+            mxArray *ks = mxGetField(opts, 0, "ker_shape");
+            int64_t kd[3], *gboxes, *goff;
+            float *kern, *gvals;
+            int kown = 0;
+            kin[0] = ks ? ks : mxCreateString("ball");
+            kin[1] = mxGetField(opts, 0, "ker_param");
+            mexCallMATLAB(1, kout, 2, kin, "generate_kernel");
+            if (!kout[0] || !is_real_numeric(kout[0]) || mxGetNumberOfDimensions(kout[0]) > 3)
+                arg_error("generate_kernel must return a real [kx, ky, kz] array");
+            dims_of(kout[0], kd, 3);
+            kern = as_single(kout[0], &kown);
+            gboxes = (int64_t *)mxMalloc(6 * (N ? N : 1) * sizeof(int64_t));
+            goff = (int64_t *)mxMalloc((N + 1) * sizeof(int64_t));
+            rc = sid_library_kernel_boxes(N, boxes, sensor, kd, gboxes, goff);
+            if (rc != SID_OK) fail(rc, "sid_library_kernel_boxes");
+            gvals = (float *)mxMalloc((goff[N] ? goff[N] : 1) * sizeof(float));
+            rc = sid_library_convn_seeds(N, boxes, vals, voff, sensor, kern, kd, gvals);
+            if (kown) mxFree(kern);
+            if (rc != SID_OK) fail(rc, "sid_library_convn_seeds");
+            mxFree(boxes);
+            mxFree(voff);
+            mxFree(vals);
+            boxes = gboxes;
+            voff = goff;
+            vals = gvals;
+        }
         rects = (int64_t *)mxMalloc(4 * (N ? N : 1) * sizeof(int64_t));
         offs = (int64_t *)mxMalloc((N + 1) * sizeof(int64_t));
         rc = sid_library_rects(p, N, boxes, sensor, rects, offs);

@@ -40,8 +40,36 @@ def _load(name):
         L.stub_err_msg.restype = ctypes.c_char_p
         L.stub_call.restype = ctypes.c_int
         L.stub_call.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.stub_set_matlab.restype = None
+        L.stub_set_matlab.argtypes = [_MATLAB_CB]
         _libs[name] = L
+        L.stub_set_matlab(_make_cb(L))
     return _libs[name]
+
+
+# MATLAB functions the gateways call back through mexCallMATLAB (name -> python
+# callable taking and returning what tests/_mex.Mex converts).
+MATLAB_FUNCTIONS = {}
+_MATLAB_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p)
+_callbacks = []   # keeps the ctypes callbacks alive
+
+
+def _make_cb(L):
+    def cb(nlhs, plhs, nrhs, prhs, name):
+        fn = MATLAB_FUNCTIONS.get(name.decode())
+        if fn is None:
+            return 1
+        m = Mex.__new__(Mex)
+        m.L = L
+        args = [m.out(prhs[i])[1] for i in range(nrhs)]
+        res = fn(*args)
+        if nlhs >= 1:
+            plhs[0] = m.arr(res)
+        return 0
+    c = _MATLAB_CB(cb)
+    _callbacks.append(c)
+    return c
 
 
 class MexError(Exception):
@@ -91,6 +119,10 @@ class Mex:
             ir = np.ctypeslib.as_array(ctypes.cast(L.stub_ir(p), ctypes.POINTER(ctypes.c_size_t)), (max(nnz, 1),))[:nnz]
             pr = np.ctypeslib.as_array(ctypes.cast(L.stub_data(p), ctypes.POINTER(ctypes.c_double)), (max(nnz, 1),))[:nnz]
             return "sparse", scipy.sparse.csc_matrix((pr.copy(), ir.copy(), jc), shape=(m, n))
+        if cls == 4:   # char: UTF-16 code units
+            n = int(np.prod(shape))
+            buf = np.ctypeslib.as_array(ctypes.cast(L.stub_data(p), ctypes.POINTER(ctypes.c_uint16)), (max(n, 1),))[:n]
+            return "char", "".join(chr(c) for c in buf)
         dt = {6: np.float64, 7: np.float32, 12: np.int32, 15: np.uint64}[cls]
         n = int(np.prod(shape))
         buf = np.ctypeslib.as_array(ctypes.cast(L.stub_data(p), ctypes.POINTER(np.ctypeslib.as_ctypes_type(dt))),

@@ -45,6 +45,8 @@ mxArray *mxCreateSparse(mwSize m, mwSize n, mwSize nzmax, mxComplexity cx);
 void *mxMalloc(size_t n);
 void mxFree(void *p);
 void mexErrMsgIdAndTxt(const char *id, const char *fmt, ...);
+mxArray *mxCreateString(const char *s);
+int mexCallMATLAB(int nlhs, mxArray *plhs[], int nrhs, mxArray *prhs[], const char *name);
 int mexAtExit(void (*fn)(void));
 
 void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]);

@@ -152,6 +152,19 @@ int mexAtExit(void (*fn)(void))
     return 0;
 }
 
+/* mexCallMATLAB: a MATLAB function the gateway calls back (generate_kernel);
+ * the harness registers one handler, which returns 0 and fills plhs, or
+ * nonzero when the function is unknown (MATLAB then raises its own error). */
+typedef int (*stub_matlab_fn)(int nlhs, mxArray **plhs, int nrhs, mxArray **prhs, const char *name);
+static stub_matlab_fn g_matlab = NULL;
+void stub_set_matlab(stub_matlab_fn fn) { g_matlab = fn; }
+int mexCallMATLAB(int nlhs, mxArray *plhs[], int nrhs, mxArray *prhs[], const char *name)
+{
+    if (!g_matlab || g_matlab(nlhs, plhs, nrhs, prhs, name) != 0)
+        mexErrMsgIdAndTxt("MATLAB:UndefinedFunction", "Undefined function '%s'.", name);
+    return 0;
+}
+
 /* ---- harness entry points (called from Python through ctypes) ---- */
 mxArray *stub_numeric(int cls, int ndim, const int64_t *dims, const void *data)
 {
@@ -170,6 +183,7 @@ mxArray *stub_char(const char *s)
     for (i = 0; i < d[1]; ++i) ((uint16_t *)a->data)[i] = (uint16_t)(unsigned char)s[i];
     return a;
 }
+mxArray *mxCreateString(const char *s) { return stub_char(s); }
 mxArray *stub_struct(int n, const char **names, mxArray **vals)
 {
     mwSize d[2] = {1, 1};

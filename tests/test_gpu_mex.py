@@ -122,6 +122,42 @@ def test_library_gpu_gateway(env):
         assert rel_l2(fm[s], want.ravel(order="F")) < 2e-6
 
 
+def test_library_gpu_gateway_with_kernel(env):
+    """options.ker_param: the gateway calls back MATLAB's generate_kernel
+    (here the test's stand-in, sid_amd.recon.generate_kernel) and convolves each
+    seed's volume before the projection (generate_LFM_library_GPU.m:50-56,95-101)."""
+    from sid_amd import recon as R
+    from tests._mex import MATLAB_FUNCTIONS
+    print("case: MEX generate_LFM_library_GPU with ker_param", flush=True)
+    calls = []
+
+    def gen(shape, param):
+        calls.append((shape, np.asarray(param).ravel().tolist()))
+        return R.generate_kernel(shape, np.asarray(param).ravel())
+
+    MATLAB_FUNCTIONS["generate_kernel"] = gen
+    try:
+        rng = np.random.default_rng(7)
+        n, M, X, Y, nz = 5, 21, 45, 40, 6
+        H = random_psf(rng, M, n, nz).astype(np.float32)
+        recon = [rng.random((X, Y, nz)).astype(np.float32) for _ in range(2)]
+        centers = np.array([[2.4, 20.5, 1.2], [44.5, 38.6, 6.0], [22.5, 17.4, 3.5]])
+        neur_id = np.array([[1.0, 1.0], [0.0, 1.0], [1.0, 0.0]])
+        opts = {"image_size": np.array([X, Y], dtype=np.float64), "axial": 2.0, "neur_rad": 3.0,
+                "ker_shape": "ball", "ker_param": np.array([2.0, 1.0])}
+        cls, fm = Mex("generate_LFM_library_GPU")(recon, centers, neur_id, {"H": H}, opts)
+    finally:
+        MATLAB_FUNCTIONS.pop("generate_kernel", None)
+    assert calls == [("ball", [2.0, 1.0])]
+    assert cls == "single" and fm.shape == (3, X * Y)
+    kern = R.generate_kernel("ball", [2.0, 1.0]).astype(np.float64)
+    for s in range(3):
+        ids = list(np.flatnonzero(neur_id[s]))
+        want, _ = O.library_gpu_entry([r.astype(np.float64) for r in recon], ids, centers[s], 3, 2.0,
+                                      H.astype(np.float64), kernel=kern)
+        assert rel_l2(fm[s], want.ravel(order="F")) < 2e-6
+
+
 def test_library_cpu_gateway(env):
     print("case: MEX generate_LFM_library_CPU (sparse)", flush=True)
     rng = np.random.default_rng(6)

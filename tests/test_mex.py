@@ -49,9 +49,6 @@ def test_argument_errors_before_any_device_call():
     e = _err(Mex("generate_LFM_library_GPU"), np.ones((9, 9, 2)), np.ones((1, 3)), np.ones((1, 1)), H,
              {"image_size": np.array([9.0, 9.0]), "axial": 1.0, "neur_rad": 2.0})
     assert e.ident == "sid:arg" and "cell" in e.msg
-    e = _err(Mex("generate_LFM_library_GPU"), [np.ones((9, 9, 2))], np.ones((1, 3)), np.ones((1, 1)), H,
-             {"image_size": np.array([9.0, 9.0]), "axial": 1.0, "neur_rad": 2.0, "ker_shape": "ball"})
-    assert e.ident == "sid:arg" and "kernel" in e.msg
     e = _err(Mex("generate_LFM_library_CPU"), np.ones((2, 2)), {"H": H}, 2.0, np.ones(3), np.array([9.0, 9.0, 2.0]))
     assert e.ident == "sid:arg" and "N x 3" in e.msg
     e = _err(Mex("sidPSF"), "destroy", np.array([12345], dtype=np.uint64))
@@ -69,6 +66,13 @@ def test_argument_errors_before_any_device_call():
     ("backwardProjectGPU_new", (H, np.ones((9, 9)))),
     ("sidDeconvRL", (H, np.ones((9, 9, 2)), 2.0, np.ones((9, 9, 2)))),
     ("sidPSF", ("create", H)),
+    # ker_shape alone selects no kernel (generate_LFM_library_GPU.m:50-58: only ker_param does)
+    ("generate_LFM_library_GPU", ([np.ones((9, 9, 2))], np.ones((1, 3)), np.ones((1, 1)), H,
+                                  {"image_size": np.array([9.0, 9.0]), "axial": 1.0, "neur_rad": 2.0,
+                                   "ker_shape": "ball"})),
+    ("generate_LFM_library_GPU", ([np.ones((9, 9, 2))], np.ones((1, 3)), np.ones((1, 1)), H,
+                                  {"image_size": np.array([9.0, 9.0]), "axial": 1.0, "neur_rad": 2.0,
+                                   "ker_shape": "ball", "ker_param": np.array([2.0, 1.0])})),
 ])
 def test_device_calls_report_nodev_without_a_gpu(name, args):
     import sid_amd

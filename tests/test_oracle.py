@@ -442,6 +442,43 @@ def test_library_gpu_entry_is_box_average_on_its_planes():
     assert np.abs(img - O.forward_project(H, V)).max() < 1e-12 * np.abs(img).max()
 
 
+def test_library_gpu_entry_with_kernel():
+    """generate_LFM_library_GPU.m:95-101 with a kernel: the seed volume (zero
+    outside the box, image_size x its planes) convolved by convn 'same' before
+    the projection.  Pinned by scipy's full convolution cropped at MATLAB's
+    'same' window (floor(size(k)/2)), by a delta kernel (the kernel-free entry),
+    and by the z crop: with a kernel that only shifts content down in z the
+    last plane of the sub-volume receives nothing."""
+    from scipy.signal import convolve
+    n, M, X, Y, Nz = 3, 5, 16, 14, 6
+    rng = np.random.default_rng(4)
+    H = rng.random((M, M, n, n, Nz))
+    recon = [rng.random((X, Y, Nz)) for _ in range(2)]
+    centre = (2.4, 7.6, 5.2)   # box touches the x = 1 edge; planes 3..6 touch the last plane
+    for k in (rng.random((5, 3, 3)), rng.random((4, 3, 2))):
+        img, p = O.library_gpu_entry(recon, [0, 1], centre, 3, 1.5, H, kernel=k)
+        V = np.zeros((X, Y, p["z1"] - p["z0"] + 1))
+        sx, sy, sz = slice(p["x0"] - 1, p["x1"]), slice(p["y0"] - 1, p["y1"]), slice(p["z0"] - 1, p["z1"])
+        V[sx, sy, :] = 0.5 * (recon[0][sx, sy, sz] + recon[1][sx, sy, sz])
+        full = convolve(V, k, mode="full")
+        o = [s // 2 for s in k.shape]
+        Vc = full[o[0]:o[0] + X, o[1]:o[1] + Y, o[2]:o[2] + V.shape[2]]
+        want = O.forward_project(H[:, :, :, :, sz], Vc)
+        assert np.abs(img - want).max() < 1e-12 * np.abs(want).max()
+    img0, _ = O.library_gpu_entry(recon, [0, 1], centre, 3, 1.5, H)
+    img1, _ = O.library_gpu_entry(recon, [0, 1], centre, 3, 1.5, H, kernel=np.ones((1, 1, 1)))
+    assert np.array_equal(img0, img1)
+    kz = np.zeros((1, 1, 3))
+    kz[0, 0, 0] = 1.0   # out[z] = V[z + 1]: the last plane of the sub-volume gets zero
+    img2, p = O.library_gpu_entry(recon, [0, 1], centre, 3, 1.5, H, kernel=kz)
+    sx, sy = slice(p["x0"] - 1, p["x1"]), slice(p["y0"] - 1, p["y1"])
+    V = np.zeros((X, Y, p["z1"] - p["z0"] + 1))
+    for l in range(V.shape[2] - 1):
+        V[sx, sy, l] = 0.5 * (recon[0][sx, sy, p["z0"] + l] + recon[1][sx, sy, p["z0"] + l])
+    want = O.forward_project(H[:, :, :, :, p["z0"] - 1:p["z1"]], V)
+    assert np.abs(img2 - want).max() < 1e-12 * np.abs(want).max()
+
+
 def test_matlab_round_half_away_from_zero():
     assert list(O.matlab_round([0.5, 1.5, 2.5, -0.5, -2.5, 2.4999])) == [1, 2, 3, -1, -3, 2]
 
