@@ -506,8 +506,20 @@ static int ensure_aux(sid_psf *p)
 // so when both are present the GEMM runs on the handle's second stream, forked
 // from st after the staging and joined back before the reduction that sums the
 // two.  Phase times go to timing[ph_gemm] and timing[ph_fft].
-static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph_gemm, int ph_fft)
+// The backward reduction that follows a backward projection (arguments of
+// launch_bwd_reduce).  With both halves present its direct planes are reduced
+// on the GEMM's stream as soon as the GEMM is done, beside the rest of the FFT
+// path; the caller then reduces the FFT planes only.
+struct BwdTail {
+    float *B_col;
+    int rl_update;
+    unsigned long long check, mark;
+};
+
+static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph_gemm, int ph_fft,
+                          const BwdTail *tail = nullptr, bool *tail_direct_done = nullptr)
 {
+    if (tail_direct_done) *tail_direct_done = false;
     const GemmPlan &g = dir == 0 ? p->plan.fwd : p->plan.bwd;
     if (!(p->overlap && p->plan.fft.active && !g.items.empty())) {
         const int a = tm.mark();
@@ -525,6 +537,11 @@ static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph
     const int a = tm.mark(p->aux);
     const int rc_g = launch_gemm(p, dir, p->aux, dir == 0 && p->gemm_blocks_f > 0 ? p->gemm_blocks_f : p->gemm_blocks);
     const int b = tm.mark(p->aux);
+    int rc_t = SID_OK;
+    if (rc_g == SID_OK && tail && dir == 1) {
+        rc_t = launch_bwd_reduce(p, tail->B_col, tail->rl_update, p->aux, tail->check, tail->mark, 1);
+        if (rc_t == SID_OK && tail_direct_done) *tail_direct_done = true;
+    }
     // join even after a failed launch so that st never runs ahead of work left on aux
     SID_TRY_HIP(hipEventRecord(p->ev_join, p->aux));
     const int c = tm.mark();
@@ -532,6 +549,7 @@ static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph
     const int d = tm.mark();
     SID_TRY_HIP(hipStreamWaitEvent(st, p->ev_join, 0));
     SID_TRY(rc_g);
+    SID_TRY(rc_t);
     SID_TRY(rc_f);
     tm.span(ph_gemm, a, b);
     tm.span(ph_fft, c, d);
@@ -708,9 +726,11 @@ static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], 
     const int e0 = tm.mark();
     const unsigned long long stamp = next_stamp(p);
     SID_TRY(launch_img_in(p, O_dev, st, stamp));
-    SID_TRY(project_halves(p, 1, st, tm, 2, 6));
+    const BwdTail tail{B_dev, 0, stamp, 0};
+    bool direct_done = false;
+    SID_TRY(project_halves(p, 1, st, tm, 2, 6, &tail, &direct_done));
     const int e2 = tm.mark();
-    SID_TRY(launch_bwd_reduce(p, B_dev, 0, st, stamp));
+    SID_TRY(launch_bwd_reduce(p, B_dev, 0, st, stamp, 0, direct_done ? 2 : 0));
     const int e3 = tm.mark();
     tm.span(3, e2, e3);
     if (!sync) return SID_OK;
@@ -858,10 +878,12 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
         const unsigned long long s_img = next_stamp(p);
         SID_TRY(launch_fwd_reduce(p, nullptr, st, s_vol, s_img));
         const int c = tm.mark();
-        SID_TRY(project_halves(p, 1, st, tm, 2, 6));
-        const int d = tm.mark();
         s_vol = next_stamp(p);
-        SID_TRY(launch_bwd_reduce(p, hist ? hist + (int64_t)it * nv : nullptr, 1, st, s_img, s_vol));
+        const BwdTail tail{hist ? hist + (int64_t)it * nv : nullptr, 1, s_img, s_vol};
+        bool direct_done = false;
+        SID_TRY(project_halves(p, 1, st, tm, 2, 6, &tail, &direct_done));
+        const int d = tm.mark();
+        SID_TRY(launch_bwd_reduce(p, tail.B_col, 1, st, s_img, s_vol, direct_done ? 2 : 0));
         const int e = tm.mark();
         tm.span(1, b, c);
         tm.span(3, d, e);

@@ -683,11 +683,12 @@ __global__ __launch_bounds__(256) void k_bwd_reduce(const T *__restrict__ part, 
                                                     const int *__restrict__ plane_fft,
                                                     const float *__restrict__ dfft,
                                                     unsigned long long *__restrict__ nf,
-                                                    unsigned long long check, unsigned long long mark)
+                                                    unsigned long long check, unsigned long long mark,
+                                                    const int *__restrict__ planes)
 {
     const int ab = threadIdx.x;
     if (ab >= n * n) return;
-    const int r = blockIdx.x, c = blockIdx.y;
+    const int r = blockIdx.x, c = planes ? planes[blockIdx.y] : (int)blockIdx.y;
     const int Px = r / Ly, Py = r - Px * Ly;
     const int64_t o = (((int64_t)c * XP + Px + T_) * YP + (Py + T_)) * S + ab;
     T d = 0;
@@ -1462,25 +1463,38 @@ int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long lo
     return SID_OK;
 }
 
+// subset: 0 every plane of H, 1 the planes on the direct GEMM, 2 those on the
+// FFT path (1 and 2 need an active FFT state; together they cover every plane).
 int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st, unsigned long long check,
-                      unsigned long long mark)
+                      unsigned long long mark, int subset)
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
     const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
     const int64_t stride = sp.mpad * p->NP;
-    const dim3 grid((unsigned)sp.nrows, (unsigned)p->nzH);
+    const int *planes = nullptr;
+    int64_t np = p->nzH;
+    if (subset != 0) {
+        if (!sp.fft.active || f64) {
+            set_error("internal: a plane subset of the backward reduction needs the FFT path");
+            return SID_ERR_ARG;
+        }
+        planes = sp.fft.d_plane_list + (subset == 2 ? sp.fft.n_list_direct : 0);
+        np = subset == 2 ? sp.fft.n_list_fft : sp.fft.n_list_direct;
+        if (np == 0) return SID_OK;
+    }
+    const dim3 grid((unsigned)sp.nrows, (unsigned)np);
     if (f64)
         hipLaunchKernelGGL(k_bwd_reduce<double>, grid, dim3(256), 0, st, sp.part_d, sp.bwd.d_plane_slab0, stride,
                            (int)p->NP, (int)p->n, (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X,
                            (int)sp.Y, B_col, rl_update, sp.htf_d, sp.vol_d, (u16 *)nullptr, (u16 *)nullptr,
-                           (const int *)nullptr, (const float *)nullptr, p->nf, check, mark);
+                           (const int *)nullptr, (const float *)nullptr, p->nf, check, mark, (const int *)nullptr);
     else
         hipLaunchKernelGGL(k_bwd_reduce<float>, grid, dim3(256), 0, st, sp.part, sp.bwd.d_plane_slab0, stride,
                            (int)p->NP, (int)p->n, (int)sp.Ly, sp.XP, sp.YP, (int)p->T, (int)p->S, (int)sp.X,
                            (int)sp.Y, B_col, rl_update, sp.htf, sp.vol, sh ? sp.vol_hi : nullptr,
                            sh ? sp.vol_lo : nullptr, sp.fft.active ? sp.fft.d_plane_fft : nullptr,
-                           sp.fft.active ? sp.fft.dfft : nullptr, p->nf, check, mark);
+                           sp.fft.active ? sp.fft.dfft : nullptr, p->nf, check, mark, planes);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }

@@ -1067,7 +1067,7 @@ void fft_state_free(FftState &f)
     f.ev.clear();
     fft_plan_free(f.fx); fft_plan_free(f.fy); fft_plan_free(f.fX); fft_plan_free(f.fY);
     for (void *q : {(void *)f.G, (void *)f.subs, (void *)f.vt, (void *)f.oh, (void *)f.ohT, (void *)f.grid, (void *)f.dfft,
-                    (void *)f.d_slot_plane, (void *)f.d_plane_fft, (void *)f.qscale})
+                    (void *)f.d_slot_plane, (void *)f.d_plane_fft, (void *)f.d_plane_list, (void *)f.qscale})
         if (q) (void)hipFree(q);
     f = FftState();
 }
@@ -1176,6 +1176,19 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     SID_TRY_HIP(hipMalloc(&f.d_plane_fft, sizeof(int) * p->nzH));
     SID_TRY_HIP(hipMemcpy(f.d_slot_plane, f.planes.data(), sizeof(int) * f.planes.size(), hipMemcpyHostToDevice));
     SID_TRY_HIP(hipMemcpy(f.d_plane_fft, pf.data(), sizeof(int) * p->nzH, hipMemcpyHostToDevice));
+    {
+        // the backward reduction of the direct planes runs on the GEMM's stream
+        // as soon as the GEMM is done, that of the FFT planes after the FFT path
+        std::vector<int> lst;
+        for (int64_t c = 0; c < p->nzH; ++c)
+            if (!pf[c]) lst.push_back((int)c);
+        f.n_list_direct = (int)lst.size();
+        for (int64_t c = 0; c < p->nzH; ++c)
+            if (pf[c]) lst.push_back((int)c);
+        f.n_list_fft = (int)lst.size() - f.n_list_direct;
+        SID_TRY_HIP(hipMalloc(&f.d_plane_list, sizeof(int) * std::max<size_t>(lst.size(), 1)));
+        SID_TRY_HIP(hipMemcpy(f.d_plane_list, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
+    }
 
     // spectra, in batches of kernels
     std::vector<SpecJob> jobs;

@@ -134,6 +134,8 @@ struct FftState {
     float *dfft = nullptr;         // backward result of the FFT planes, padded volume layout
     int *d_slot_plane = nullptr;
     int *d_plane_fft = nullptr;    // per plane of H: 1 when on the FFT path
+    int *d_plane_list = nullptr;   // planes of H on the direct GEMM, then those on the FFT path
+    int n_list_direct = 0, n_list_fft = 0;
     int64_t g_bytes = 0;
     std::vector<hipEvent_t> ev;     // event pairs bracketing the GEMVs of the last call
     std::vector<int> ev_kind;        // 0 forward GEMV, 1 backward GEMV (one entry per pair)
@@ -228,7 +230,7 @@ int launch_gemm(sid_psf *p, int dir, hipStream_t st, int max_blocks = 0);
 int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long long check = 0,
                       unsigned long long mark = 0);
 int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st, unsigned long long check = 0,
-                      unsigned long long mark = 0);
+                      unsigned long long mark = 0, int subset = 0);
 int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which /*0 iterate, 1 Htf*/, hipStream_t st,
                   unsigned long long mark = 0);
 int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st, unsigned long long mark = 0);
