@@ -142,7 +142,8 @@ class PSF:
         return dict(M=inf.M, Nnum=inf.Nnum, Nz=inf.Nz, taps_max=inf.taps_max, weight_bytes=inf.weight_bytes,
                     support=[(inf.support_lo[c], inf.support_hi[c]) for c in range(inf.Nz)],
                     plane_method=[names.get(inf.plane_method[c], "?") for c in range(inf.Nz)],
-                    autotune_ms=list(inf.autotune_ms), spectra_bytes=inf.spectra_bytes)
+                    autotune_ms=list(inf.autotune_ms), spectra_bytes=inf.spectra_bytes,
+                    fft_tile=list(inf.fft_tile))
 
     def timing(self) -> list:
         buf = (ctypes.c_double * 12)()

@@ -83,7 +83,8 @@ class PsfInfo(ctypes.Structure):
                 ("taps_max", ctypes.c_int64), ("weight_bytes", ctypes.c_int64),
                 ("support_lo", ctypes.c_int64 * 512), ("support_hi", ctypes.c_int64 * 512),
                 ("plane_method", ctypes.c_int32 * 512), ("autotune_ms", ctypes.c_double * 4),
-                ("spectra_bytes", ctypes.c_int64)]
+                ("spectra_bytes", ctypes.c_int64),
+                ("fft_tile", ctypes.c_int32 * 4)]
 
 
 class LsOpts(ctypes.Structure):

@@ -468,6 +468,14 @@ int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
     }
     for (int i = 0; i < 4; ++i) info->autotune_ms[i] = p->auto_ms[i];
     info->spectra_bytes = p->plan.fft.g_bytes;
+    {
+        const FftState &f = p->plan.fft;
+        const bool t = f.active && f.tiled;
+        info->fft_tile[0] = t ? f.tT : 0;
+        info->fft_tile[1] = t ? f.tB : 0;
+        info->fft_tile[2] = t ? f.ntile : 0;
+        info->fft_tile[3] = t ? f.tK : 0;
+    }
     return SID_OK;
 }
 

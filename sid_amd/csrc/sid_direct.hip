@@ -637,8 +637,8 @@ template <typename T>
 __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t slab_stride, int NP, int nrows,
                              int n, int Ly, int YP, int T_, int S, int X, int Y, T *__restrict__ img,
                              u16 *__restrict__ hi, u16 *__restrict__ lo, float *__restrict__ O_col,
-                             const float2 *__restrict__ fft_img, int fft_ny, unsigned long long *__restrict__ nf,
-                             unsigned long long check, unsigned long long mark)
+                             const float2 *__restrict__ fft_img, int fft_ny, const float *__restrict__ tout,
+                             unsigned long long *__restrict__ nf, unsigned long long check, unsigned long long mark)
 {
     const int nn2 = n * n;
     const int64_t total = (int64_t)nrows * nn2;
@@ -653,6 +653,7 @@ __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t sla
         const int Px = r / Ly, Py = r - Px * Ly;
         const int x = Px * n + uv % n, y = Py * n + uv / n;
         if (fft_img) s += (T)fft_img[(int64_t)x * fft_ny + y].x;   // planes on the FFT path
+        if (tout) s += (T)tout[(int64_t)r * NP + uv];                 // ... in the tiled form
         if (poison) s = (T)NAN;
         if (x >= X || y >= Y) s = 0;
         bad = bad || !isfinite(s);
@@ -1453,12 +1454,13 @@ int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long lo
         hipLaunchKernelGGL(k_fwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
                            (int)sp.X, (int)sp.Y, sp.img_d, (u16 *)nullptr, (u16 *)nullptr, O_col,
-                           (const float2 *)nullptr, 0, p->nf, check, mark);
+                           (const float2 *)nullptr, 0, (const float *)nullptr, p->nf, check, mark);
     else
         hipLaunchKernelGGL(k_fwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
                            (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr, O_col,
-                           sp.fft.active ? (const float2 *)sp.fft.grid : nullptr, sp.fft.Ny, p->nf, check, mark);
+                           sp.fft.active && !sp.fft.tiled ? (const float2 *)sp.fft.grid : nullptr, sp.fft.Ny,
+                           sp.fft.active && sp.fft.tiled ? sp.fft.tout : nullptr, p->nf, check, mark);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -1556,7 +1558,9 @@ int launch_vol_out(sid_psf *p, float *V_col, int64_t nz, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Self-test of the MFMA lane layouts assumed above.
 // ---------------------------------------------------------------------------
-__global__ void k_selftest(const float *A, const float *B, float *C32, float *C16, float *C64)
+typedef _Float16 st_h8 __attribute__((ext_vector_type(8)));
+
+__global__ void k_selftest(const float *A, const float *B, float *C32, float *C16, float *C64, float *C16h)
 {
     // A: 16 x 32 row-major, B: 32 x 16 row-major
     const int lane = threadIdx.x, l16 = lane & 15, g = lane >> 4;
@@ -1572,6 +1576,14 @@ __global__ void k_selftest(const float *A, const float *B, float *C32, float *C1
     f4 d = {0, 0, 0, 0};
     d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, d, 0, 0, 0);
     for (int i = 0; i < 4; ++i) C16[(4 * g + i) * 16 + l16] = d[i];
+    st_h8 ah, bh;   // v_mfma_f32_16x16x32_f16 (the tiled FFT path's GEMM): the same layout
+    for (int j = 0; j < 8; ++j) {
+        ah[j] = (_Float16)A[l16 * 32 + 8 * g + j];
+        bh[j] = (_Float16)B[(8 * g + j) * 16 + l16];
+    }
+    f4 h = {0, 0, 0, 0};
+    h = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, h, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) C16h[(4 * g + i) * 16 + l16] = h[i];
     d4 e = {0, 0, 0, 0};
     for (int s = 0; s < 8; ++s)
         e = __builtin_amdgcn_mfma_f64_16x16x4f64((double)A[l16 * 32 + 8 * g + s], (double)B[(8 * g + s) * 16 + l16], e,
@@ -1581,31 +1593,34 @@ __global__ void k_selftest(const float *A, const float *B, float *C32, float *C1
 
 int run_selftest()
 {
-    std::vector<float> A(16 * 32), B(32 * 16), C32(256), C16(256), C64(256);
+    std::vector<float> A(16 * 32), B(32 * 16), C32(256), C16(256), C64(256), C16h(256);
     // small integers: exact in bf16 and in fp32 accumulation
     for (int i = 0; i < 16 * 32; ++i) A[i] = (float)((i * 7 + 3) % 11) - 5.0f;
     for (int i = 0; i < 32 * 16; ++i) B[i] = (float)((i * 5 + 1) % 13) - 6.0f;
-    float *dA, *dB, *dC, *dD, *dE;
+    float *dA, *dB, *dC, *dD, *dE, *dF;
     SID_TRY_HIP(hipMalloc(&dA, A.size() * 4));
     SID_TRY_HIP(hipMalloc(&dB, B.size() * 4));
     SID_TRY_HIP(hipMalloc(&dC, 256 * 4));
     SID_TRY_HIP(hipMalloc(&dD, 256 * 4));
     SID_TRY_HIP(hipMalloc(&dE, 256 * 4));
+    SID_TRY_HIP(hipMalloc(&dF, 256 * 4));
     SID_TRY_HIP(hipMemcpy(dA, A.data(), A.size() * 4, hipMemcpyHostToDevice));
     SID_TRY_HIP(hipMemcpy(dB, B.data(), B.size() * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA, dB, dC, dD, dE);
+    hipLaunchKernelGGL(k_selftest, dim3(1), dim3(64), 0, 0, dA, dB, dC, dD, dE, dF);
     SID_TRY_HIP(hipGetLastError());
     SID_TRY_HIP(hipMemcpy(C32.data(), dC, 256 * 4, hipMemcpyDeviceToHost));
     SID_TRY_HIP(hipMemcpy(C16.data(), dD, 256 * 4, hipMemcpyDeviceToHost));
     SID_TRY_HIP(hipMemcpy(C64.data(), dE, 256 * 4, hipMemcpyDeviceToHost));
-    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD); (void)hipFree(dE);
+    SID_TRY_HIP(hipMemcpy(C16h.data(), dF, 256 * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD); (void)hipFree(dE); (void)hipFree(dF);
     for (int i = 0; i < 16; ++i)
         for (int j = 0; j < 16; ++j) {
             double s = 0;
             for (int k = 0; k < 32; ++k) s += (double)A[i * 32 + k] * B[k * 16 + j];
-            if (C32[i * 16 + j] != (float)s || C16[i * 16 + j] != (float)s || C64[i * 16 + j] != (float)s) {
-                set_error("MFMA layout self-test failed at (%d,%d): fp32 %g bf16 %g fp64 %g expected %g", i, j,
-                          C32[i * 16 + j], C16[i * 16 + j], C64[i * 16 + j], s);
+            if (C32[i * 16 + j] != (float)s || C16[i * 16 + j] != (float)s || C64[i * 16 + j] != (float)s ||
+                C16h[i * 16 + j] != (float)s) {
+                set_error("MFMA layout self-test failed at (%d,%d): fp32 %g bf16 %g fp64 %g fp16 %g expected %g", i,
+                          j, C32[i * 16 + j], C16[i * 16 + j], C64[i * 16 + j], C16h[i * 16 + j], s);
                 return SID_ERR_HIP;
             }
         }

@@ -35,6 +35,7 @@
 #include <functional>
 #include <vector>
 
+#include "sid_fft_core.h"
 #include "sid_internal.h"
 
 namespace sid {
@@ -49,71 +50,6 @@ struct FftDesc {
     int L, nst, spb;
     int rad[12];
 };
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-
-// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations
-// and meets the other waves, without waiting for outstanding global loads
-// (the register prefetch of the persistent kernels stays in flight).  The
-// memory clobber keeps the compiler from moving memory accesses across it.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
-// a / b for 0 <= a < 2^22, b >= 1, with rb = 1.0f / b: the float quotient is
-// off by at most one and corrected exactly (no integer division sequence).
-__device__ __forceinline__ int qdiv(int a, int b, float rb)
-{
-    int q = (int)((float)a * rb);
-    q -= (q * b > a);
-    q += ((q + 1) * b <= a);
-    return q;
-}
-
-template <int R>
-__device__ __forceinline__ void small_dft(float2 *v, float sgn)
-{
-    if constexpr (R == 2) {
-        const float2 a = v[0], b = v[1];
-        v[0] = make_float2(a.x + b.x, a.y + b.y);
-        v[1] = make_float2(a.x - b.x, a.y - b.y);
-    } else if constexpr (R == 4) {
-        const float2 a = v[0], b = v[1], c = v[2], d = v[3];
-        const float2 s0 = make_float2(a.x + c.x, a.y + c.y), s1 = make_float2(a.x - c.x, a.y - c.y);
-        const float2 s2 = make_float2(b.x + d.x, b.y + d.y), s3 = make_float2(b.x - d.x, b.y - d.y);
-        // multiply s3 by (sgn * i)
-        const float2 s3i = make_float2(-sgn * s3.y, sgn * s3.x);
-        v[0] = make_float2(s0.x + s2.x, s0.y + s2.y);
-        v[2] = make_float2(s0.x - s2.x, s0.y - s2.y);
-        v[1] = make_float2(s1.x + s3i.x, s1.y + s3i.y);
-        v[3] = make_float2(s1.x - s3i.x, s1.y - s3i.y);
-    } else if constexpr (R == 3) {
-        // w = exp(sgn 2 pi i / 3) = -1/2 + sgn i sqrt(3)/2
-        const float c1 = -0.5f, s1 = sgn * 0.86602540378443865f;
-        const float2 a = v[0], b = v[1], c = v[2];
-        const float2 t = make_float2(b.x + c.x, b.y + c.y), u = make_float2(b.x - c.x, b.y - c.y);
-        v[0] = make_float2(a.x + t.x, a.y + t.y);
-        const float2 m = make_float2(a.x + c1 * t.x, a.y + c1 * t.y);
-        // v1 = m + i s1 u, v2 = m - i s1 u
-        v[1] = make_float2(m.x - s1 * u.y, m.y + s1 * u.x);
-        v[2] = make_float2(m.x + s1 * u.y, m.y - s1 * u.x);
-    } else {
-        // R == 5: w^e = cos(2 pi e / 5) + sgn i sin(2 pi e / 5)
-        const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
-        const float s1 = sgn * 0.95105651629515357f, s2 = sgn * 0.58778525229247313f;
-        const float2 a = v[0];
-        const float2 t1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y), u1 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y);
-        const float2 t2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y), u2 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
-        v[0] = make_float2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
-        const float2 m1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
-        const float2 m2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
-        // X1 = m1 + i (s1 u1 + s2 u2), X4 = m1 - i (...); X2 = m2 + i (s2 u1 - s1 u2), X3 = m2 - i (...)
-        const float2 n1 = make_float2(s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y);
-        const float2 n2 = make_float2(s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y);
-        v[1] = make_float2(m1.x - n1.y, m1.y + n1.x);
-        v[4] = make_float2(m1.x + n1.y, m1.y - n1.x);
-        v[2] = make_float2(m2.x - n2.y, m2.y + n2.x);
-        v[3] = make_float2(m2.x + n2.y, m2.y - n2.x);
-    }
-}
 
 template <int R>
 __device__ __forceinline__ void stockham_stage(const float2 *__restrict__ in, float2 *__restrict__ out, int L, int Ns,
@@ -728,15 +664,6 @@ __device__ __forceinline__ void load_twiddles(float2 *tx, float2 *ty, const floa
 //   A(k) = (Z(k) + conj(Z(-k))) / 2,   B(k) = (Z(k) - conj(Z(-k))) / (2 i).
 // The next pair's volume values are loaded into registers while the current
 // pair is transformed and written out.
-// Two adjacent floats at a 4-byte aligned address as one 8-byte access
-// (multi-dword global accesses need only dword alignment).
-__device__ __forceinline__ float2 ld_f2_a4(const float *p)
-{
-    float2 v;
-    __builtin_memcpy(&v, p, 8);
-    return v;
-}
-__device__ __forceinline__ void st_f2_a4(float *p, float2 v) { __builtin_memcpy(p, &v, 8); }
 // ... and two adjacent complex values (q0, q0 + 1) of vt as one 16-byte access
 __device__ __forceinline__ void ld_c2(const float2 *p, float2 &x, float2 &y)
 {
@@ -1038,7 +965,7 @@ __global__ __launch_bounds__(256) void k_spec_l1(const float *__restrict__ H, co
 
 static int grid_n(int64_t total) { return (int)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536)); }
 
-static void gemv_mark(FftState &f, hipStream_t st, int kind, bool begin)
+void fft_mark(FftState &f, hipStream_t st, int kind, bool begin)
 {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return;
@@ -1063,6 +990,7 @@ void fft_timing_collect(sid_psf *p, double *fwd_ms, double *bwd_ms)
 
 void fft_state_free(FftState &f)
 {
+    tf_free(f);
     for (auto e : f.ev) (void)hipEventDestroy(e);
     f.ev.clear();
     fft_plan_free(f.fx); fft_plan_free(f.fy); fft_plan_free(f.fX); fft_plan_free(f.fY);
@@ -1097,20 +1025,82 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
         reach = std::max(reach, std::max(pl.hi - pl.off_f, pl.off_f - pl.lo));
     }
     if (f.planes.empty()) return SID_OK;
+    if (n2 > 256) {
+        set_error("FFT path supports Nnum <= 16");
+        fft_state_free(f);
+        return SID_ERR_DIMS;
+    }
+    f.nq = (int)f.planes.size() * n2;
+    f.n2 = n2;
+    // FP16 spectra in the BF16X2 mode, row q scaled by 1/qscale[q]
+    f.half = p->precision == SID_PREC_BF16X2;
+    std::vector<SpecJob> jobs;
+    for (size_t sl = 0; sl < f.planes.size(); ++sl) {
+        const Plane &pl = p->planes[f.planes[sl]];
+        for (int ab = 0; ab < n2; ++ab)
+            jobs.push_back(SpecJob{f.planes[sl], ab, (int)sl * n2 + ab, pl.lo, pl.hi, pl.off_f});
+    }
+    SpecJob *d_jobs = nullptr;
+    SID_TRY_HIP(hipMalloc(&d_jobs, sizeof(SpecJob) * jobs.size()));
+    if (hipMemcpy(d_jobs, jobs.data(), sizeof(SpecJob) * jobs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d_jobs);
+        set_error("copy of the spectra jobs failed");
+        return SID_ERR_HIP;
+    }
+    if (f.half) {
+        SID_TRY_HIP(hipMalloc(&f.qscale, sizeof(float) * f.nq));
+        hipLaunchKernelGGL(k_spec_l1, dim3((unsigned)jobs.size()), dim3(256), 0, st, p->H, d_jobs, p->M, n, f.qscale);
+        SID_TRY_HIP(hipGetLastError());
+    }
+    SID_TRY_HIP(hipMalloc(&f.dfft, sizeof(float) * sp.vol_elems));
+    SID_TRY_HIP(hipMemsetAsync(f.dfft, 0, sizeof(float) * sp.vol_elems, st));
+    std::vector<int> pf(p->nzH, 0);
+    for (int c : f.planes) pf[c] = 1;
+    SID_TRY_HIP(hipMalloc(&f.d_slot_plane, sizeof(int) * f.planes.size()));
+    SID_TRY_HIP(hipMalloc(&f.d_plane_fft, sizeof(int) * p->nzH));
+    SID_TRY_HIP(hipMemcpy(f.d_slot_plane, f.planes.data(), sizeof(int) * f.planes.size(), hipMemcpyHostToDevice));
+    SID_TRY_HIP(hipMemcpy(f.d_plane_fft, pf.data(), sizeof(int) * p->nzH, hipMemcpyHostToDevice));
+    {
+        // the backward reduction of the direct planes runs on the GEMM's stream
+        // as soon as the GEMM is done, that of the FFT planes after the FFT path
+        std::vector<int> lst;
+        for (int64_t c = 0; c < p->nzH; ++c)
+            if (!pf[c]) lst.push_back((int)c);
+        f.n_list_direct = (int)lst.size();
+        for (int64_t c = 0; c < p->nzH; ++c)
+            if (pf[c]) lst.push_back((int)c);
+        f.n_list_fft = (int)lst.size() - f.n_list_direct;
+        SID_TRY_HIP(hipMalloc(&f.d_plane_list, sizeof(int) * std::max<size_t>(lst.size(), 1)));
+        SID_TRY_HIP(hipMemcpy(f.d_plane_list, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
+    }
+    // BF16X2: the tiled form (SID_FFT_UNTILED=1 keeps the whole-sensor spectra)
+    if (f.half && std::getenv("SID_FFT_UNTILED") == nullptr) {
+        const int rc = tf_build(p, st);
+        if (rc == SID_OK) {
+            (void)hipFree(d_jobs);
+            f.active = true;
+            return SID_OK;
+        }
+        tf_free(f);
+        if (rc != kTfNotApplicable) {
+            (void)hipFree(d_jobs);
+            fft_state_free(f);
+            return rc;
+        }
+    }
     const int Xn = (int)(sp.Lx * n), Yn = (int)(sp.Ly * n);
     f.Nxp = next_smooth235((Xn + reach + n - 1) / n);
     f.Nyp = next_smooth235((Yn + reach + n - 1) / n);
     f.Nx = n * f.Nxp;
     f.Ny = n * f.Nyp;
     if (f.Nx > 2048 || f.Ny > 2048) {
+        (void)hipFree(d_jobs);
         set_error("FFT grid %d x %d exceeds the 2048-point transforms of this build", f.Nx, f.Ny);
         fft_state_free(f);
         return SID_ERR_DIMS;
     }
     f.Hy = f.Nyp / 2 + 1;
     f.nk0 = f.Nxp * f.Hy;
-    f.n2 = n2;
-    f.nq = (int)f.planes.size() * n2;
     // GEMV row chunks, multiples of 32 rows: about 32K blocks per GEMV (many
     // rounds of the resident blocks, so no half-empty last round).
     // SID_GEMV_SPLIT overrides both counts (measurement).
@@ -1129,13 +1119,7 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     SID_TRY(fft_plan_make(f.fy, f.Nyp));
     SID_TRY(fft_plan_make(f.fX, f.Nx));
     SID_TRY(fft_plan_make(f.fY, f.Ny));
-    if (n2 > 256) {
-        set_error("FFT path supports Nnum <= 16");
-        fft_state_free(f);
-        return SID_ERR_DIMS;
-    }
-    // FP16 spectra in the BF16X2 mode; k0 blocks padded to 16 bytes
-    f.half = p->precision == SID_PREC_BF16X2;
+    // k0 blocks padded to 16 bytes
     f.kstride = ((int64_t)f.nq * n2 + 3) / 4 * 4;
     f.g_bytes = (int64_t)f.nk0 * f.kstride * (int64_t)(f.half ? sizeof(__half2) : sizeof(float2));
     SID_TRY_HIP(hipMalloc(&f.G, f.g_bytes));
@@ -1144,7 +1128,6 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
         fft_state_free(f);
         return SID_ERR_HIP;
     }
-    if (f.half) SID_TRY_HIP(hipMalloc(&f.qscale, sizeof(float) * f.nq));
     // fused sub-image transforms when both Nx' x Ny' grids fit in a block's LDS
     // (SID_FFT_UNFUSED=1 keeps the separate gather / 1-D FFT / pack kernels)
     {
@@ -1168,47 +1151,13 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     SID_TRY_HIP(hipMalloc(&f.oh, sizeof(float2) * (int64_t)f.nsplit * f.nk0 * n2));
     SID_TRY_HIP(hipMalloc(&f.ohT, sizeof(float2) * (int64_t)f.nk0 * n2));
     SID_TRY_HIP(hipMalloc(&f.grid, sizeof(float2) * (int64_t)f.Nx * f.Ny));
-    SID_TRY_HIP(hipMalloc(&f.dfft, sizeof(float) * sp.vol_elems));
-    SID_TRY_HIP(hipMemsetAsync(f.dfft, 0, sizeof(float) * sp.vol_elems, st));
-    std::vector<int> pf(p->nzH, 0);
-    for (int c : f.planes) pf[c] = 1;
-    SID_TRY_HIP(hipMalloc(&f.d_slot_plane, sizeof(int) * f.planes.size()));
-    SID_TRY_HIP(hipMalloc(&f.d_plane_fft, sizeof(int) * p->nzH));
-    SID_TRY_HIP(hipMemcpy(f.d_slot_plane, f.planes.data(), sizeof(int) * f.planes.size(), hipMemcpyHostToDevice));
-    SID_TRY_HIP(hipMemcpy(f.d_plane_fft, pf.data(), sizeof(int) * p->nzH, hipMemcpyHostToDevice));
-    {
-        // the backward reduction of the direct planes runs on the GEMM's stream
-        // as soon as the GEMM is done, that of the FFT planes after the FFT path
-        std::vector<int> lst;
-        for (int64_t c = 0; c < p->nzH; ++c)
-            if (!pf[c]) lst.push_back((int)c);
-        f.n_list_direct = (int)lst.size();
-        for (int64_t c = 0; c < p->nzH; ++c)
-            if (pf[c]) lst.push_back((int)c);
-        f.n_list_fft = (int)lst.size() - f.n_list_direct;
-        SID_TRY_HIP(hipMalloc(&f.d_plane_list, sizeof(int) * std::max<size_t>(lst.size(), 1)));
-        SID_TRY_HIP(hipMemcpy(f.d_plane_list, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
-    }
 
     // spectra, in batches of kernels
-    std::vector<SpecJob> jobs;
-    for (size_t sl = 0; sl < f.planes.size(); ++sl) {
-        const Plane &pl = p->planes[f.planes[sl]];
-        for (int ab = 0; ab < n2; ++ab)
-            jobs.push_back(SpecJob{f.planes[sl], ab, (int)sl * n2 + ab, pl.lo, pl.hi, pl.off_f});
-    }
     const int64_t per = (int64_t)f.Nx * f.Ny;
     const int bmax = (int)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)(512ll << 20) / (per * 8)));
     float2 *grids = nullptr;
-    SpecJob *d_jobs = nullptr;
     SID_TRY_HIP(hipMalloc(&grids, sizeof(float2) * per * bmax));
-    SID_TRY_HIP(hipMalloc(&d_jobs, sizeof(SpecJob) * jobs.size()));
-    SID_TRY_HIP(hipMemcpy(d_jobs, jobs.data(), sizeof(SpecJob) * jobs.size(), hipMemcpyHostToDevice));
     int rc = SID_OK;
-    if (f.half) {
-        hipLaunchKernelGGL(k_spec_l1, dim3((unsigned)jobs.size()), dim3(256), 0, st, p->H, d_jobs, p->M, n, f.qscale);
-        if (hipGetLastError() != hipSuccess) rc = SID_ERR_HIP;
-    }
     for (size_t b0 = 0; b0 < jobs.size() && rc == SID_OK; b0 += bmax) {
         const int nb = (int)std::min<size_t>(bmax, jobs.size() - b0);
         hipLaunchKernelGGL(k_spec_fill, dim3(grid_n(per * nb)), dim3(256), 0, st, p->H, d_jobs + b0, p->M, n, f.Nx,
@@ -1264,6 +1213,7 @@ int fft_forward(sid_psf *p, hipStream_t st)
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
     if (!f.active) return SID_OK;
+    if (f.tiled) return tf_forward(p, st);
     const int n2 = f.n2;
     if (f.fused) {
         hipLaunchKernelGGL(k_fft_sub_fwd, dim3(f.sub_blocks), dim3(kSubThreads), f.sub_lds, st, sp.vol, f.d_slot_plane,
@@ -1280,14 +1230,14 @@ int fft_forward(sid_psf *p, hipStream_t st)
         hipLaunchKernelGGL(k_fft_pack_vt, dim3(grid_n((int64_t)f.nk0 * f.nq)), dim3(256), 0, st, f.subs, f.qscale, f.nq,
                            f.Nxp, f.Nyp, f.Hy, f.vt);
     }
-    gemv_mark(f, st, 0, true);
+    fft_mark(f, st, 0, true);
     if (f.half)
         hipLaunchKernelGGL(k_fft_gemv_fwd<__half2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const __half2 *)f.G,
                            f.vt, f.nq, n2, f.qc, f.nk0, f.kstride, f.oh);
     else
         hipLaunchKernelGGL(k_fft_gemv_fwd<float2>, dim3(f.nk0, f.nsplit), dim3(256), 0, st, (const float2 *)f.G,
                            f.vt, f.nq, n2, f.qc, f.nk0, f.kstride, f.oh);
-    gemv_mark(f, st, 0, false);
+    fft_mark(f, st, 0, false);
     hipLaunchKernelGGL(k_fft_oh_sum, dim3((f.nk0 + 31) / 32, (n2 + 31) / 32), dim3(256), 0, st, f.oh, f.nsplit, f.nk0,
                        n2, f.ohT);
     hipLaunchKernelGGL(k_fft_assemble, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, f.ohT, f.nk0, f.Nx, f.Ny,
@@ -1304,20 +1254,21 @@ int fft_backward(sid_psf *p, hipStream_t st)
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
     if (!f.active) return SID_OK;
+    if (f.tiled) return tf_backward(p, st);
     const int n = (int)p->n;
     hipLaunchKernelGGL(k_fft_img_grid, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, sp.img, f.Nx, f.Ny,
                        (int)(sp.Lx * n), (int)(sp.Ly * n), n, sp.YP, (int)p->T, (int)p->S, f.grid);
     SID_TRY_HIP(hipGetLastError());
     SID_TRY(fft_run(f.fY, f.grid, f.Nx, f.Nx, 0, f.Ny, 1, -1, 1.0f, st));
     SID_TRY(fft_run(f.fX, f.grid, f.Ny, f.Ny, 0, 1, f.Ny, -1, 1.0f, st));
-    gemv_mark(f, st, 1, true);
+    fft_mark(f, st, 1, true);
     if (f.half)
         hipLaunchKernelGGL(k_fft_gemv_bwd<__half2>, dim3(f.nk0, f.nsplit_b), dim3(256), 0, st, (const __half2 *)f.G,
                            f.grid, f.nq, f.qc_b, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
     else
         hipLaunchKernelGGL(k_fft_gemv_bwd<float2>, dim3(f.nk0, f.nsplit_b), dim3(256), 0, st, (const float2 *)f.G,
                            f.grid, f.nq, f.qc_b, f.kstride, n, f.Nx, f.Ny, f.Nxp, f.Nyp, f.Hy, f.vt);
-    gemv_mark(f, st, 1, false);
+    fft_mark(f, st, 1, false);
     if (f.fused) {
         hipLaunchKernelGGL(k_fft_sub_bwd, dim3(f.sub_blocks), dim3(kSubThreads), f.sub_lds, st, f.vt, f.d_slot_plane,
                            f.qscale, sub_geom(p, f), fft_desc(f.fx), fft_desc(f.fy), f.fx.tw_inv, f.fy.tw_inv,

@@ -137,6 +137,22 @@ struct FftState {
     int *d_plane_list = nullptr;   // planes of H on the direct GEMM, then those on the FFT path
     int n_list_direct = 0, n_list_fft = 0;
     int64_t g_bytes = 0;
+    // Tiled form (BF16X2, sid_tfft.hip): overlap-save tiles of tT x tT lenslets
+    // over the lenslet sub-images; per frequency bin one complex GEMM over all
+    // tiles (FP16 spectra x FP16 hi + lo inputs on MFMA, FP32 accumulate).
+    bool tiled = false;
+    int tT = 0, tH = 0, tB = 0;        // FFT length, stored half length tT/2 + 1, valid outputs per tile
+    int tDlo = 0, tDhi = 0, tK = 0;    // common lenslet tap range of the planes and its width
+    int ntx = 0, nty = 0, ntile = 0, nbin = 0, chp = 0, kslot = 0, kpad = 0, ngrp = 0;
+    FftPlan1D ft;                      // twiddles of length tT
+    void *Gb = nullptr;                // backward spectra (G holds the forward ones), fragment order
+    float2 *vin = nullptr;             // forward input spectra [nbin][ntile][kpad]
+    float2 *zin = nullptr;             // backward input (image) spectra [nbin][ntile][chp]
+    float2 *yout = nullptr;            // forward GEMM result [nbin][ntile][chp]
+    float2 *bout = nullptr;            // backward GEMM result [nbin][slot][ntile][chp]
+    float *vmax = nullptr;             // per tile and transform block: bound on its spectra
+    float *tmax = nullptr;             // per tile: bound over all its blocks
+    float *tout = nullptr;             // forward result of the FFT planes, lenslet image [nrows][NP]
     std::vector<hipEvent_t> ev;     // event pairs bracketing the GEMVs of the last call
     std::vector<int> ev_kind;        // 0 forward GEMV, 1 backward GEMV (one entry per pair)
 };
@@ -253,6 +269,14 @@ void fft_timing_collect(sid_psf *p, double *fwd_ms, double *bwd_ms);
 int next_smooth235(int L);
 int fft_test(int L, int64_t nseq, float2 *data, int dir, hipStream_t st);
 int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st);
+// sid_tfft.hip: the tiled form of the FFT path.  tf_build returns
+// kTfNotApplicable (nothing allocated) when the geometry has no tile size.
+constexpr int kTfNotApplicable = 1000;
+int tf_build(sid_psf *p, hipStream_t st);
+void tf_free(FftState &f);
+int tf_forward(sid_psf *p, hipStream_t st);    // FFT planes' forward image into f.tout
+int tf_backward(sid_psf *p, hipStream_t st);   // FFT planes' back-projection into f.dfft
+void fft_mark(FftState &f, hipStream_t st, int kind, bool begin);   // timing events (GEMV / GEMM)
 // sid_library.hip
 void patched_rect(int64_t n, int64_t M, int64_t X, int64_t Y, const int64_t box[6], int64_t rect[4]);
 int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *vals, const int64_t *val_off, int64_t X,

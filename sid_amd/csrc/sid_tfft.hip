@@ -1,0 +1,909 @@
+// Tiled FFT path of the light-field projector (BF16X2 precision mode).
+//
+// In lenslet coordinates the projection of plane c is a multichannel 2-D
+// convolution (external/LFrecon_util/forwardProjectGPU.m:16-28 in the form of
+// sid_direct.hip):
+//     O[P, uv] = sum_{c, ab} sum_D V_c[P - D, ab] W_c[D][ab][uv],
+//     W_c[D][ab][uv] = H_{a,b,c}[n D + (u - a, v - b) + off],
+// D over the plane's lenslet taps.  With the taps of every plane on this path
+// inside one range [Dlo, Dhi] (K = Dhi - Dlo + 1 taps per axis) the output is
+// cut into tiles of B x B lenslets, and each tile is one circular convolution
+// of length T = B + K - 1 (overlap-save): the input window of the tile at
+// origin P0 starts at P0 - Dhi, and output P0 + p is element p + K - 1 of
+//     IDFT_T( sum_{c,ab} G_c[f][ab][uv] * DFT_T(window_{c,ab})[f] ),
+//     G_c[f][ab][uv] = sum_{e in [0,K)^2} W_c[Dlo + e][ab][uv] e^{-2 pi i f.e / T}.
+// For one frequency bin f that sum is a complex GEMM, [uv] x [(c, ab)] times
+// [(c, ab)] x [tile].  The backward (backwardProjectGPU_new.m:16-26; for the
+// planes of this path it is the adjoint) correlates image windows starting at
+// Q0 + Dlo with the same kernels:
+//     B_c[Q0 + p, ab] = IDFT_T( sum_uv conj(G_c[f][ab][uv]) * DFT_T(image window_uv)[f] )[p].
+// The data are real, so only fy in [0, T/2] is stored.
+//
+// The GEMMs run on MFMA (v_mfma_f32_16x16x32_f16) with FP32 accumulation:
+// the spectra G in FP16, row (c, ab) scaled by 1/s_q (s_q the L1 norm of the
+// kernel, so |G| <= 1); the input spectra split into FP16 hi + lo (together
+// 21 bits), each tile's column scaled by a bound on its magnitudes (from the
+// L1 norms of its windows) so that neither half can overflow.  G is stored once
+// per direction in MFMA operand order: one wave's operand of a K step is one
+// contiguous 1 KB run.  Against one GEMV over spectra of the whole padded
+// sensor, the tiles shrink the spectra to (T / (L + K))^2 of their size and
+// apply every stored value to all tiles at once.
+//
+// The window transforms are register FFTs (one sequence per thread, length T
+// fixed at compile time, Stockham order with radices 4, 2, 3, 5) with one LDS
+// transpose between the two axes.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <vector>
+
+#include "sid_fft_core.h"
+#include "sid_internal.h"
+
+namespace sid {
+
+typedef unsigned int tu32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 th8 __attribute__((ext_vector_type(8)));
+
+constexpr int kTfCG = 8;   // channels per transform block (4 real pairs)
+
+// ---------------------------------------------------------------------------
+// Register FFT of compile-time length N (2/3/5-smooth): Stockham stages with
+// radices 4, then 2, 3, 5; tw[t] = exp(sgn 2 pi i t / N).  All indices are
+// compile-time constants after unrolling, so the sequence stays in registers.
+// ---------------------------------------------------------------------------
+template <int N, int Ns, int R>
+__device__ __forceinline__ void rf_stage(float2 (&x)[N], const float2 *__restrict__ tw, float sgn)
+{
+    float2 y[N];
+#pragma unroll
+    for (int j = 0; j < N / R; ++j) {
+        const int k = j % Ns;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const float2 a = x[j + r * (N / R)];
+            v[r] = (r == 0 || k == 0) ? a : cmul(a, tw[k * r * (N / (Ns * R))]);
+        }
+        small_dft<R>(v, sgn);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) y[base + r * Ns] = v[r];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = y[i];
+}
+
+template <int N, int Ns, int Rem>
+struct RegFft {
+    static __device__ __forceinline__ void run(float2 (&x)[N], const float2 *__restrict__ tw, float sgn)
+    {
+        constexpr int R = Rem % 4 == 0 ? 4 : Rem % 2 == 0 ? 2 : Rem % 3 == 0 ? 3 : 5;
+        static_assert(Rem % R == 0, "register FFT lengths are 2/3/5-smooth");
+        rf_stage<N, Ns, R>(x, tw, sgn);
+        RegFft<N, Ns * R, Rem / R>::run(x, tw, sgn);
+    }
+};
+template <int N, int Ns>
+struct RegFft<N, Ns, 1> {
+    static __device__ __forceinline__ void run(float2 (&)[N], const float2 *__restrict__, float) {}
+};
+template <int N>
+__device__ __forceinline__ void reg_fft(float2 (&x)[N], const float2 *__restrict__ tw, float sgn)
+{
+    RegFft<N, 1, N>::run(x, tw, sgn);
+}
+
+// Geometry of one transform launch.  Lenslet (x, y) of plane c of the source
+// (k_tf_in) or destination (k_tf_out) is at ((c XP + x + pad) YP + y + pad) S
+// + channel.  Blocks are (slot, tile, channel group).
+struct TfGeom {
+    int n2, chp, Lx, Ly, B, nty, ntile, ntp, nslot, ngrp, nz_in;
+    int pad, S;
+    int64_t XP, YP;
+    int off;            // in: window origin = tile origin + off; out: first valid element of a tile
+    int kslot, kdim;    // in: channel ch of slot s is column k = s kslot + ch of a row of kdim
+};
+
+// ---------------------------------------------------------------------------
+// Window transforms: per block one (slot, tile) and kTfCG channels.  The
+// channels are transformed in real pairs a + i b (Z = DFT(a + i b) gives
+// A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)): along y
+// in registers (one row per thread), the stored half spectrum through LDS,
+// then along x (one column per thread).  Output: out[bin][tile][k], scaled by
+// s_q (qscale, forward) and vmax[tile][slot ngrp + grp] = max over the block's
+// channels of s_q T max_x sum_y |w[x][y]|, a bound on every |out| it wrote (at
+// least the window's L1 norm; a maximum, so independent of thread timing).
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, const int *__restrict__ slot_plane,
+                                               const float *__restrict__ qscale, TfGeom g,
+                                               const float2 *__restrict__ tw, float2 *__restrict__ out,
+                                               float *__restrict__ vmax)
+{
+    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2;
+    extern __shared__ float2 tsm[];   // [CG][T][H]
+    __shared__ unsigned int rmax[CG];   // per channel: largest row sum of |w| (float bits, >= 0)
+    const int grp = blockIdx.x % g.ngrp, rest = blockIdx.x / g.ngrp;
+    const int tile = rest % g.ntile, slot = rest / g.ntile;
+    const int tx = tile / g.nty, ty = tile - tx * g.nty;
+    const int x0 = tx * g.B + g.off, y0 = ty * g.B + g.off;
+    const int c = slot_plane ? slot_plane[slot] : 0;
+    const bool live = c < g.nz_in;
+    const int ch0 = grp * CG;
+    if (threadIdx.x < CG) rmax[threadIdx.x] = 0u;
+    __syncthreads();
+    const float *base = src + (int64_t)c * g.XP * g.YP * g.S;
+    for (int it = threadIdx.x; it < NPAIR * T; it += blockDim.x) {
+        const int pair = it % NPAIR, x = it / NPAIR;
+        const int ch = ch0 + 2 * pair;
+        const int X = x0 + x;
+        const bool rowin = live && X >= 0 && X < g.Lx && ch < g.n2;
+        const bool two = ch + 1 < g.n2;
+        float2 z[T];
+        float sa = 0.f, sb = 0.f;
+#pragma unroll
+        for (int y = 0; y < T; ++y) {
+            const int Y = y0 + y;
+            float2 v = make_float2(0.f, 0.f);
+            if (rowin && Y >= 0 && Y < g.Ly) {
+                const float *p = base + (((int64_t)X + g.pad) * g.YP + (Y + g.pad)) * g.S + ch;
+                if (two) v = ld_f2_a4(p);
+                else v.x = p[0];
+            }
+            z[y] = v;
+            sa += fabsf(v.x);
+            sb += fabsf(v.y);
+        }
+        reg_fft<T>(z, tw, -1.0f);
+        float2 *dA = tsm + ((2 * pair) * T + x) * H, *dB = tsm + ((2 * pair + 1) * T + x) * H;
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const float2 zk = z[k], zm = z[(T - k) % T];
+            dA[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+            dB[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        }
+        atomicMax(&rmax[2 * pair], __float_as_uint(sa));
+        atomicMax(&rmax[2 * pair + 1], __float_as_uint(sb));
+    }
+    __syncthreads();
+    const int64_t bstride = (int64_t)g.ntile * g.kdim;
+    for (int it = threadIdx.x; it < CG * H; it += blockDim.x) {
+        const int ch = it % CG, k = it / CG;
+        const int chg = ch0 + ch;
+        float2 w[T];
+        const float2 *s = tsm + ch * T * H + k;
+#pragma unroll
+        for (int x = 0; x < T; ++x) w[x] = s[x * H];
+        reg_fft<T>(w, tw, -1.0f);
+        if (chg < g.n2) {
+            const float q = qscale ? qscale[slot * g.n2 + chg] : 1.0f;
+            float2 *o = out + (int64_t)tile * g.kdim + slot * g.kslot + chg;
+#pragma unroll
+            for (int fx = 0; fx < T; ++fx) o[(int64_t)(fx * H + k) * bstride] = make_float2(w[fx].x * q, w[fx].y * q);
+        }
+    }
+    if (threadIdx.x == 0) {
+        float m = 0.f;
+        for (int ch = 0; ch < CG; ++ch)
+            if (ch0 + ch < g.n2)
+                m = fmaxf(m, (float)T * __uint_as_float(rmax[ch]) * (qscale ? qscale[slot * g.n2 + ch0 + ch] : 1.0f));
+        vmax[(int64_t)tile * g.nslot * g.ngrp + slot * g.ngrp + grp] = m;
+    }
+}
+
+// Inverse: spec[bin][slot][ntp][chp] -> along x (one stored fy per thread;
+// only the tile's valid rows kept in LDS), then along y in real pairs with the
+// Hermitian completion; the valid B x B block, times scale * s_q (qscale, when
+// given), goes to the destination layout.
+template <int T>
+__global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec, const int *__restrict__ slot_plane,
+                                                const float *__restrict__ qscale, TfGeom g,
+                                                const float2 *__restrict__ tw, float scale, float *__restrict__ dst)
+{
+    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2;
+    extern __shared__ float2 tsm[];   // [CG][B][H]
+    const int B = g.B;
+    const int grp = blockIdx.x % g.ngrp, rest = blockIdx.x / g.ngrp;
+    const int tile = rest % g.ntile, slot = rest / g.ntile;
+    const int tx = tile / g.nty, ty = tile - tx * g.nty;
+    const int c = slot_plane ? slot_plane[slot] : 0;
+    const int ch0 = grp * CG;
+    const int64_t bstride = (int64_t)g.nslot * g.ntp * g.chp;
+    for (int it = threadIdx.x; it < CG * H; it += blockDim.x) {
+        const int ch = it % CG, k = it / CG;
+        const int chg = ch0 + ch;
+        float2 w[T];
+        const float2 *s = spec + ((int64_t)slot * g.ntp + tile) * g.chp + chg;
+#pragma unroll
+        for (int fx = 0; fx < T; ++fx)
+            w[fx] = chg < g.n2 ? s[(int64_t)(fx * H + k) * bstride] : make_float2(0.f, 0.f);
+        reg_fft<T>(w, tw, 1.0f);
+#pragma unroll
+        for (int x = 0; x < T; ++x) {
+            const int xr = x - g.off;
+            if (xr >= 0 && xr < B) tsm[(ch * B + xr) * H + k] = w[x];
+        }
+    }
+    __syncthreads();
+    for (int it = threadIdx.x; it < NPAIR * B; it += blockDim.x) {
+        const int pair = it % NPAIR, xr = it / NPAIR;
+        const int Px = tx * B + xr;
+        const int ch = ch0 + 2 * pair;
+        if (Px >= g.Lx || ch >= g.n2) continue;
+        const float2 *sA = tsm + ((2 * pair) * B + xr) * H, *sB = tsm + ((2 * pair + 1) * B + xr) * H;
+        float2 z[T];
+#pragma unroll
+        for (int fy = 0; fy < T; ++fy) {
+            float2 a, b;
+            if (fy < H) {
+                a = sA[fy];
+                b = sB[fy];
+            } else {
+                const float2 ta = sA[T - fy], tb = sB[T - fy];
+                a = make_float2(ta.x, -ta.y);
+                b = make_float2(tb.x, -tb.y);
+            }
+            z[fy] = make_float2(a.x - b.y, a.y + b.x);
+        }
+        reg_fft<T>(z, tw, 1.0f);
+        const bool two = ch + 1 < g.n2;
+        const float qa = scale * (qscale ? qscale[slot * g.n2 + ch] : 1.0f);
+        const float qb = two ? scale * (qscale ? qscale[slot * g.n2 + ch + 1] : 1.0f) : 0.f;
+        float *o = dst + (((int64_t)c * g.XP + Px + g.pad) * g.YP + g.pad) * g.S + ch;
+#pragma unroll
+        for (int y = 0; y < T; ++y) {
+            const int yr = y - g.off, Py = ty * B + yr;
+            if (yr >= 0 && yr < B && Py < g.Ly) {
+                float *q = o + (int64_t)Py * g.S;
+                if (two) st_f2_a4(q, make_float2(z[y].x * qa, z[y].y * qb));
+                else q[0] = z[y].x * qa;
+            }
+        }
+    }
+}
+
+// Per tile: the bound over its transform blocks (one wave per tile).
+__global__ __launch_bounds__(256) void k_tf_tmax(const float *__restrict__ vmax, int ntile, int rowlen,
+                                                 float *__restrict__ tmax)
+{
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    for (int t = blockIdx.x * wpb + (threadIdx.x >> 6); t < ntile; t += gridDim.x * wpb) {
+        float m = 0.f;
+        for (int j = lane; j < rowlen; j += 64) m = fmaxf(m, vmax[(int64_t)t * rowlen + j]);
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0) tmax[t] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectra GEMM.  Block = one frequency bin; over all tiles it computes
+//     C[row][tile] = sum_k A[row][k] * Bm[tile][k]        (complex)
+// forward: rows uv, K = (plane, ab), one slot; backward: for each of the nslot
+// planes in turn, rows ab, K = uv (16 K steps of 16 per slot, uv padded).
+// A: FP16 (re, im) pairs in fragment order, the 16 bytes of lane (l16, g) of
+// flattened K step s (slot * KS + ks) and row block rb at A[((bin nslot KS + s)
+// RB + rb) 64 + lane]: row rb*16 + l16, complex k = ks*16 + 4g .. 4g + 3.
+// Bm: FP32 complex [bin][tile][kB], scaled per tile by 1/tmax and split into
+// FP16 hi + lo; Re C = A . (Br, -Bi), Im C = A . (Bi, Br) over the interleaved
+// real K (v_mfma_f32_16x16x32_f16: lane (l16, g) feeds A[row l16][k 8g..8g+7],
+// B[k 8g..8g+7][col l16] and holds C[4g + i][l16]).
+// Pipeline: every wave keeps kTfRing K steps of its A fragments in flight (a
+// register ring, refilled right after use); each thread converts its 32 bytes
+// of Bm per step into the four operands in an LDS double buffer (one barrier
+// per step), loaded kTfRing steps ahead.  The loop body has no branch (row
+// blocks past RB repeat the last one, tiles past ntile read zeros and write to
+// the padding columns of `out`), so the compiler's waits count the ring
+// exactly.  Output out[bin][slot][ntp][chp] (ntp = tiles padded to 16) times
+// tmax[tile].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pk_rtz(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+__device__ __forceinline__ float h_lo(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu)); }
+__device__ __forceinline__ float h_hi(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16)); }
+
+// Four complex values (v0 = k, k+1; v1 = k+2, k+3) times s: the operands
+// (re, -im) and (im, re), each as FP16 hi and lo (round toward zero: the
+// halves of (im, re) are those of (re, -im), swapped and negated, exactly).
+__device__ __forceinline__ void tf_split(float4 v0, float4 v1, float s, tu32x4 &rh, tu32x4 &rl, tu32x4 &ih,
+                                         tu32x4 &il)
+{
+    const float re[4] = {v0.x * s, v0.z * s, v1.x * s, v1.z * s};
+    const float im[4] = {v0.y * s, v0.w * s, v1.y * s, v1.w * s};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t h = pk_rtz(re[j], -im[j]);
+        const float r0 = re[j] - h_lo(h), r1 = -im[j] - h_hi(h);
+        rh[j] = h;
+        rl[j] = pk_rtz(r0, r1);
+        ih[j] = pk_rtz(im[j], re[j]);
+        il[j] = pk_rtz(-r1, r0);
+    }
+}
+
+__device__ __forceinline__ f4 mfma16(tu32x4 a, tu32x4 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(th8, a), __builtin_bit_cast(th8, b), c, 0, 0, 0);
+}
+
+constexpr int kTfRing = 4;    // K steps in flight per wave
+constexpr int kTfKsB = 16;    // backward K steps per slot (uv padded to 256)
+
+template <int NTB, bool BWD>
+__global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A, const float2 *__restrict__ Bm,
+                                                    const float *__restrict__ tmax, int nbin, int nslot, int ntile,
+                                                    int ntp, int KS, int kB, int RB, int chp,
+                                                    float2 *__restrict__ out)
+{
+    constexpr int D = kTfRing;
+    constexpr int U = BWD ? kTfKsB : D;   // steps per loop trip (a backward trip is one slot)
+    static_assert(U % D == 0 && U % 2 == 0, "ring and LDS buffers repeat within a trip");
+    __shared__ tu32x4 bop[2][4][4][64];   // [buffer][operand rh, rl, ih, il][tile block][lane]
+    // blocks b, b + 8, ... run on one XCD: each XCD takes a contiguous range of bins
+    const int per = (nbin + 7) >> 3;
+    const int idx = blockIdx.x >> 3, bin = (blockIdx.x & 7) * per + idx;
+    if (idx >= per || bin >= nbin) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
+    // this thread's Bm chunk of every step: tile tt, complex 4 qt .. 4 qt + 3
+    const int tt = threadIdx.x >> 2, qt = threadIdx.x & 3;
+    const bool tvalid = tt < ntile;
+    const float mt = tvalid ? tmax[tt] : 0.f;
+    const float sc = mt > 0.f ? 1.0f / mt : 0.f;
+    const float2 *bsrc = Bm + ((int64_t)bin * ntile + (tvalid ? tt : 0)) * kB + 4 * qt;
+    tu32x4 *bdst = &bop[0][0][tt >> 4][(tt & 15) + 16 * qt];
+    const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
+    float mx[NTB];
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) mx[tb] = tb * 16 + l16 < ntile ? tmax[tb * 16 + l16] : 0.f;
+    int rbw[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rbw[i] = min(wave + 4 * i, RB - 1);
+    const int64_t astep = (int64_t)RB * 64;
+    const tu32x4 *abase = A + (int64_t)bin * nslot * KS * astep + lane;
+    const int nsteps = nslot * KS;
+    tu32x4 a[D][4];
+    float4 b0[D], b1[D];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+        const int s = min(r, nsteps - 1);
+        const tu32x4 *pa = abase + (int64_t)s * astep;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
+        const float2 *pb = bsrc + (BWD ? (s & (kTfKsB - 1)) : s) * 16;
+        b0[r] = *(const float4 *)pb;
+        b1[r] = *(const float4 *)(pb + 2);
+    }
+    f4 cr[4][NTB], ci[4][NTB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int tb = 0; tb < NTB; ++tb) cr[i][tb] = ci[i][tb] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int s0 = 0; s0 < nsteps; s0 += U) {
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int r = j % D;
+            const int sn = min(s0 + j + D, nsteps - 1);
+            {
+                tu32x4 rh, rl, ih, il;
+                tf_split(tvalid ? b0[r] : zf, tvalid ? b1[r] : zf, sc, rh, rl, ih, il);
+                tu32x4 *d = bdst + (j & 1) * (4 * 4 * 64);
+                d[0] = rh;
+                d[4 * 64] = rl;
+                d[2 * 4 * 64] = ih;
+                d[3 * 4 * 64] = il;
+            }
+            {
+                const float2 *pb = bsrc + (BWD ? (sn & (kTfKsB - 1)) : sn) * 16;
+                b0[r] = *(const float4 *)pb;
+                b1[r] = *(const float4 *)(pb + 2);
+            }
+            __syncthreads();
+            const tu32x4 *bs = &bop[j & 1][0][0][lane];
+#pragma unroll
+            for (int tb = 0; tb < NTB; ++tb) {
+                const tu32x4 rh = bs[tb * 64], rl = bs[(4 + tb) * 64], ih = bs[(8 + tb) * 64], il = bs[(12 + tb) * 64];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    cr[i][tb] = mfma16(a[r][i], rh, cr[i][tb]);
+                    ci[i][tb] = mfma16(a[r][i], ih, ci[i][tb]);
+                    cr[i][tb] = mfma16(a[r][i], rl, cr[i][tb]);
+                    ci[i][tb] = mfma16(a[r][i], il, ci[i][tb]);
+                }
+            }
+            const tu32x4 *pa = abase + (int64_t)sn * astep;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
+            __builtin_amdgcn_sched_barrier(0);   // keep the steps apart (register pressure)
+        }
+        if constexpr (BWD) {
+            // slot s0 / 16 is complete: write it (repeated row blocks store the same values twice)
+            float2 *ob = out + ((int64_t)bin * nslot + s0 / kTfKsB) * ntp * chp + 4 * g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int tb = 0; tb < NTB; ++tb) {
+                    float2 *o = ob + (int64_t)(tb * 16 + l16) * chp + rbw[i] * 16;
+                    const float m = mx[tb];
+                    *(float4 *)o = make_float4(cr[i][tb][0] * m, ci[i][tb][0] * m, cr[i][tb][1] * m, ci[i][tb][1] * m);
+                    *(float4 *)(o + 2) =
+                        make_float4(cr[i][tb][2] * m, ci[i][tb][2] * m, cr[i][tb][3] * m, ci[i][tb][3] * m);
+                    cr[i][tb] = ci[i][tb] = f4{0.f, 0.f, 0.f, 0.f};
+                }
+        }
+    }
+    if constexpr (!BWD) {
+        float2 *ob = out + (int64_t)bin * ntp * chp + 4 * g;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int tb = 0; tb < NTB; ++tb) {
+                float2 *o = ob + (int64_t)(tb * 16 + l16) * chp + rbw[i] * 16;
+                const float m = mx[tb];
+                *(float4 *)o = make_float4(cr[i][tb][0] * m, ci[i][tb][0] * m, cr[i][tb][1] * m, ci[i][tb][1] * m);
+                *(float4 *)(o + 2) = make_float4(cr[i][tb][2] * m, ci[i][tb][2] * m, cr[i][tb][3] * m, ci[i][tb][3] * m);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectra build, one slot (plane) at a time.
+//   k_tf_g1: X1[ab][ex][fy][uv] = sum_ey W[Dlo + ex, Dlo + ey][ab][uv] e^{-2 pi i fy ey / T}
+//   k_tf_g2: Gt[fx][fy][ab][uv] = sum_ex X1[ab][ex][fy][uv] e^{-2 pi i fx ex / T}
+//   k_tf_sf / k_tf_sb: Gt / s_q in FP16 into the forward / backward operand order
+// ---------------------------------------------------------------------------
+template <int T>
+__global__ __launch_bounds__(256) void k_tf_g1(const float *__restrict__ Hm, int64_t M, int n, int c, int lo, int hi,
+                                               int off, int Dlo, int K, int chp, const float2 *__restrict__ tw,
+                                               float2 *__restrict__ X1)
+{
+    constexpr int H = T / 2 + 1;
+    const int ex = blockIdx.x % K, ab = blockIdx.x / K;
+    const int uv = threadIdx.x;
+    const int n2 = n * n;
+    if (uv >= chp) return;
+    float2 acc[H];
+#pragma unroll
+    for (int f = 0; f < H; ++f) acc[f] = make_float2(0.f, 0.f);
+    const int a = ab % n, b = ab / n, u = uv % n, v = uv / n;
+    const int i = n * (Dlo + ex) + u - a + off;
+    if (uv < n2 && i >= lo && i <= hi) {
+        const float *h = Hm + (int64_t)i + M * M * ((int64_t)a + n * ((int64_t)b + (int64_t)n * c));
+#pragma unroll
+        for (int ey = 0; ey < T; ++ey) {
+            const int j = n * (Dlo + ey) + v - b + off;
+            const float w = (ey < K && j >= lo && j <= hi) ? h[M * (int64_t)j] : 0.f;
+#pragma unroll
+            for (int f = 0; f < H; ++f) {
+                const float2 t = tw[(f * ey) % T];
+                acc[f].x += w * t.x;
+                acc[f].y += w * t.y;
+            }
+        }
+    }
+    float2 *o = X1 + (((int64_t)ab * K + ex) * H) * chp + uv;
+#pragma unroll
+    for (int f = 0; f < H; ++f) o[(int64_t)f * chp] = acc[f];
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void k_tf_g2(const float2 *__restrict__ X1, int n2, int K, int chp,
+                                               const float2 *__restrict__ tw, float2 *__restrict__ Gt)
+{
+    constexpr int H = T / 2 + 1;
+    const int fy = blockIdx.x % H, ab = blockIdx.x / H;
+    const int uv = threadIdx.x;
+    if (uv >= chp) return;
+    float2 x[T];
+#pragma unroll
+    for (int ex = 0; ex < T; ++ex)
+        x[ex] = ex < K ? X1[(((int64_t)ab * K + ex) * H + fy) * chp + uv] : make_float2(0.f, 0.f);
+#pragma unroll 1
+    for (int fx = 0; fx < T; ++fx) {
+        float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int ex = 0; ex < T; ++ex) {
+            const float2 t = tw[(fx * ex) % T];
+            s.x += x[ex].x * t.x - x[ex].y * t.y;
+            s.y += x[ex].x * t.y + x[ex].y * t.x;
+        }
+        Gt[(((int64_t)fx * H + fy) * n2 + ab) * chp + uv] = s;
+    }
+}
+
+__device__ __forceinline__ uint32_t pk_rn(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __floats2half2_rn(a, b));
+}
+
+// Forward operand of slot `slot`: row uv, K = (slot, ab) at k = slot kslot + ab.
+__global__ void k_tf_sf(const float2 *__restrict__ Gt, const float *__restrict__ qscale, int nbin, int n2, int chp,
+                        int kslot, int slot, int KS, int RB, tu32x4 *__restrict__ Gf)
+{
+    const int nq = kslot / 4;
+    const int64_t total = (int64_t)nbin * nq * chp;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int uv = (int)(e % chp);
+        const int64_t t = e / chp;
+        const int kq = (int)(t % nq), bin = (int)(t / nq);
+        const int kg = slot * nq + kq;   // global quad: k = 4 kg .. 4 kg + 3
+        const int ks = kg >> 2, gq = kg & 3;
+        tu32x4 u;
+#pragma unroll
+        for (int jp = 0; jp < 4; ++jp) {
+            const int ab = 4 * kq + jp;
+            uint32_t w = 0u;
+            if (ab < n2 && uv < n2) {
+                const float2 v = Gt[((int64_t)bin * n2 + ab) * chp + uv];
+                const float s = 1.0f / qscale[slot * n2 + ab];
+                w = pk_rn(v.x * s, v.y * s);
+            }
+            u[jp] = w;
+        }
+        Gf[(((int64_t)bin * KS + ks) * RB + (uv >> 4)) * 64 + (uv & 15) + 16 * gq] = u;
+    }
+}
+
+// Backward operand of slot `slot`: row ab, K = uv (kTfKsB steps, zero past
+// n2), conj(G) / s_q.
+__global__ void k_tf_sb(const float2 *__restrict__ Gt, const float *__restrict__ qscale, int nbin, int n2, int chp,
+                        int nslot, int slot, int RB, tu32x4 *__restrict__ Gb)
+{
+    const int nq = kTfKsB * 4;
+    const int64_t total = (int64_t)nbin * nq * chp;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int ab = (int)(e % chp);
+        const int64_t t = e / chp;
+        const int uq = (int)(t % nq), bin = (int)(t / nq);
+        const int ks = uq >> 2, gq = uq & 3;
+        tu32x4 u;
+        const float s = ab < n2 ? 1.0f / qscale[slot * n2 + ab] : 0.f;
+#pragma unroll
+        for (int jp = 0; jp < 4; ++jp) {
+            const int uv = 4 * uq + jp;
+            uint32_t w = 0u;
+            if (ab < n2 && uv < n2) {
+                const float2 v = Gt[((int64_t)bin * n2 + ab) * chp + uv];
+                w = pk_rn(v.x * s, -v.y * s);
+            }
+            u[jp] = w;
+        }
+        Gb[((((int64_t)bin * nslot + slot) * kTfKsB + ks) * RB + (ab >> 4)) * 64 + (ab & 15) + 16 * gq] = u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+// FFT lengths with instantiated transforms.
+static const int kTfLens[] = {16, 24, 32, 36, 40, 48, 64};
+
+template <int T>
+static int tf_set_lds()
+{
+    constexpr size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
+    if (lds > 65536) {
+        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_in<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_out<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    return SID_OK;
+}
+
+template <int T>
+static int tf_spectra(sid_psf *p, hipStream_t st, float2 *X1, float2 *Gt)
+{
+    FftState &f = p->plan.fft;
+    const int n = (int)p->n, n2 = n * n, P = (int)f.planes.size();
+    const int RB = f.chp / 16, KSf = f.kpad / 16;
+    for (int sl = 0; sl < P; ++sl) {
+        const Plane &pl = p->planes[f.planes[sl]];
+        hipLaunchKernelGGL(k_tf_g1<T>, dim3(n2 * f.tK), dim3(256), 0, st, p->H, p->M, n, f.planes[sl], pl.lo, pl.hi,
+                           pl.off_f, f.tDlo, f.tK, f.chp, f.ft.tw_fwd, X1);
+        SID_TRY_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_tf_g2<T>, dim3(n2 * f.tH), dim3(256), 0, st, X1, n2, f.tK, f.chp, f.ft.tw_fwd, Gt);
+        SID_TRY_HIP(hipGetLastError());
+        const int64_t tf_ = (int64_t)f.nbin * (f.kslot / 4) * f.chp, tb_ = (int64_t)f.nbin * (kTfKsB * 4) * f.chp;
+        hipLaunchKernelGGL(k_tf_sf, dim3((unsigned)std::min<int64_t>(65536, (tf_ + 255) / 256)), dim3(256), 0, st, Gt,
+                           f.qscale, f.nbin, n2, f.chp, f.kslot, sl, KSf, RB, (tu32x4 *)f.G);
+        SID_TRY_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_tf_sb, dim3((unsigned)std::min<int64_t>(65536, (tb_ + 255) / 256)), dim3(256), 0, st, Gt,
+                           f.qscale, f.nbin, n2, f.chp, P, sl, RB, (tu32x4 *)f.Gb);
+        SID_TRY_HIP(hipGetLastError());
+    }
+    return SID_OK;
+}
+
+template <int T>
+static int tf_dispatch_build(sid_psf *p, hipStream_t st, float2 *X1, float2 *Gt)
+{
+    SID_TRY(tf_set_lds<T>());
+    return tf_spectra<T>(p, st, X1, Gt);
+}
+
+static int tf_build_for(int T, sid_psf *p, hipStream_t st, float2 *X1, float2 *Gt)
+{
+    switch (T) {
+    case 16: return tf_dispatch_build<16>(p, st, X1, Gt);
+    case 24: return tf_dispatch_build<24>(p, st, X1, Gt);
+    case 32: return tf_dispatch_build<32>(p, st, X1, Gt);
+    case 36: return tf_dispatch_build<36>(p, st, X1, Gt);
+    case 40: return tf_dispatch_build<40>(p, st, X1, Gt);
+    case 48: return tf_dispatch_build<48>(p, st, X1, Gt);
+    case 64: return tf_dispatch_build<64>(p, st, X1, Gt);
+    default: set_error("internal: no tiled transform of length %d", T); return SID_ERR_ARG;
+    }
+}
+
+// Tile length: the smallest modelled time of one direction, max(HBM bytes /
+// 6 TB/s, MFMA flops / 1.2 PFLOP/s) with the spectra (FP16), the input
+// spectra written and read (FP32) and the MFMA work over the padded tiles.
+struct TfChoice {
+    int T = 0, B = 0, ntx = 0, nty = 0;
+    double cost = 0;
+};
+
+static TfChoice tf_choose(int K, int Lx, int Ly, int nslot, int chp, int kslot)
+{
+    TfChoice best;
+    const int force = std::getenv("SID_TF_T") ? std::atoi(std::getenv("SID_TF_T")) : 0;
+    for (int T : kTfLens) {
+        if (force && T != force) continue;
+        const int B = T - K + 1;
+        if (B < 1) continue;
+        const int ntx = (Lx + B - 1) / B, nty = (Ly + B - 1) / B, ntile = ntx * nty;
+        const int ntb = (ntile + 15) / 16;
+        if (ntb > 4) continue;
+        const double nbin = (double)T * (T / 2 + 1);
+        const double bytes = nbin * nslot * ((double)kslot * chp * 4 + 2.0 * ntile * kslot * 8);
+        const double flops = nbin * 16.0 * ntb * chp * (double)kslot * nslot * 16;
+        const double cost = std::max(bytes / 6e12, flops / 1.2e15);
+        if (best.T == 0 || cost < best.cost) best = TfChoice{T, B, ntx, nty, cost};
+    }
+    return best;
+}
+
+void tf_free(FftState &f)
+{
+    fft_plan_free(f.ft);
+    if (f.tiled && f.G) {   // the forward spectra of the tiled form
+        (void)hipFree(f.G);
+        f.G = nullptr;
+    }
+    for (void *q : {(void *)f.Gb, (void *)f.vin, (void *)f.zin, (void *)f.yout, (void *)f.bout, (void *)f.vmax,
+                    (void *)f.tmax, (void *)f.tout})
+        if (q) (void)hipFree(q);
+    f.Gb = nullptr;
+    f.vin = f.zin = f.yout = f.bout = nullptr;
+    f.vmax = f.tmax = f.tout = nullptr;
+    f.tiled = false;
+}
+
+// Geometry, buffers and spectra of the tiled form for the planes in f.planes
+// (f.qscale, f.dfft and the plane tables are already built).
+int tf_build(sid_psf *p, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    FftState &f = sp.fft;
+    const int n = (int)p->n, n2 = n * n, P = (int)f.planes.size();
+    if (n2 > 256 || P == 0) return kTfNotApplicable;
+    int Dlo = INT_MAX, Dhi = INT_MIN;
+    for (int c : f.planes) {
+        Dlo = std::min(Dlo, p->planes[c].tf_lo);
+        Dhi = std::max(Dhi, p->planes[c].tf_hi);
+    }
+    const int K = Dhi - Dlo + 1;
+    const int chp = (n2 + 15) / 16 * 16, kslot = (n2 + 3) / 4 * 4;
+    const TfChoice tc = tf_choose(K, (int)sp.Lx, (int)sp.Ly, P, chp, kslot);
+    if (tc.T == 0) return kTfNotApplicable;
+    f.tT = tc.T;
+    f.tH = tc.T / 2 + 1;
+    f.tB = tc.B;
+    f.tDlo = Dlo;
+    f.tDhi = Dhi;
+    f.tK = K;
+    f.ntx = tc.ntx;
+    f.nty = tc.nty;
+    f.ntile = tc.ntx * tc.nty;
+    f.nbin = f.tT * f.tH;
+    f.chp = chp;
+    f.kslot = kslot;
+    f.kpad = (P * kslot + 16 * kTfRing - 1) / (16 * kTfRing) * (16 * kTfRing);   // whole ring trips
+    f.ngrp = (n2 + kTfCG - 1) / kTfCG;
+    f.tiled = true;
+    SID_TRY(fft_plan_make(f.ft, f.tT));
+    const int RB = chp / 16, KSf = f.kpad / 16;
+    const int64_t gf = (int64_t)f.nbin * KSf * RB * 64 * 16, gb = (int64_t)f.nbin * P * kTfKsB * RB * 64 * 16;
+    SID_TRY_HIP(hipMalloc(&f.G, gf));
+    SID_TRY_HIP(hipMalloc(&f.Gb, gb));
+    SID_TRY_HIP(hipMemsetAsync(f.G, 0, gf, st));
+    SID_TRY_HIP(hipMemsetAsync(f.Gb, 0, gb, st));
+    f.g_bytes = gf + gb;
+    const int ntp = (f.ntile + 15) / 16 * 16;
+    const int64_t nvin = (int64_t)f.nbin * f.ntile * f.kpad, nzin = (int64_t)f.nbin * f.ntile * kTfKsB * 16;
+    const int64_t nout = (int64_t)f.nbin * ntp * chp;
+    SID_TRY_HIP(hipMalloc(&f.vin, sizeof(float2) * nvin));
+    SID_TRY_HIP(hipMemsetAsync(f.vin, 0, sizeof(float2) * nvin, st));   // K padding stays zero
+    SID_TRY_HIP(hipMalloc(&f.zin, sizeof(float2) * nzin));
+    SID_TRY_HIP(hipMemsetAsync(f.zin, 0, sizeof(float2) * nzin, st));
+    SID_TRY_HIP(hipMalloc(&f.yout, sizeof(float2) * nout));
+    SID_TRY_HIP(hipMalloc(&f.bout, sizeof(float2) * nout * P));
+    SID_TRY_HIP(hipMalloc(&f.vmax, sizeof(float) * (int64_t)f.ntile * P * f.ngrp));
+    SID_TRY_HIP(hipMalloc(&f.tmax, sizeof(float) * f.ntile));
+    SID_TRY_HIP(hipMalloc(&f.tout, sizeof(float) * sp.Lx * sp.Ly * p->NP));
+    SID_TRY_HIP(hipMemsetAsync(f.tout, 0, sizeof(float) * sp.Lx * sp.Ly * p->NP, st));
+    // spectra, slot by slot through two scratch buffers
+    float2 *X1 = nullptr, *Gt = nullptr;
+    SID_TRY_HIP(hipMalloc(&X1, sizeof(float2) * (int64_t)n2 * K * f.tH * chp));
+    if (hipMalloc(&Gt, sizeof(float2) * (int64_t)f.nbin * n2 * chp) != hipSuccess) {
+        (void)hipFree(X1);
+        set_error("hipMalloc of the tiled spectra scratch failed");
+        return SID_ERR_OOM;
+    }
+    int rc = tf_build_for(f.tT, p, st, X1, Gt);
+    const hipError_t e = hipStreamSynchronize(st);
+    (void)hipFree(X1);
+    (void)hipFree(Gt);
+    if (rc != SID_OK) return rc;
+    if (e != hipSuccess) {
+        set_error("tiled spectra build failed: %s", hipGetErrorString(e));
+        return SID_ERR_HIP;
+    }
+    return SID_OK;
+}
+
+static TfGeom tf_geom(const sid_psf *p, const FftState &f)
+{
+    const SensorPlan &sp = p->plan;
+    TfGeom g;
+    g.n2 = (int)(p->n * p->n);
+    g.chp = f.chp;
+    g.Lx = (int)sp.Lx;
+    g.Ly = (int)sp.Ly;
+    g.B = f.tB;
+    g.nty = f.nty;
+    g.ntile = f.ntile;
+    g.ntp = (f.ntile + 15) / 16 * 16;
+    g.nslot = (int)f.planes.size();
+    g.ngrp = f.ngrp;
+    g.nz_in = (int)sp.nz;
+    g.pad = (int)p->T;
+    g.S = (int)p->S;
+    g.XP = sp.XP;
+    g.YP = sp.YP;
+    g.off = 0;
+    g.kslot = f.kslot;
+    g.kdim = f.kpad;
+    return g;
+}
+
+template <int T>
+static int tf_in(const float *src, const int *slot_plane, const float *qscale, const TfGeom &g, const float2 *tw,
+                 float2 *out, float *vmax, hipStream_t st)
+{
+    const size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
+    hipLaunchKernelGGL(k_tf_in<T>, dim3((unsigned)(g.nslot * g.ntile * g.ngrp)), dim3(256), lds, st, src, slot_plane,
+                       qscale, g, tw, out, vmax);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+template <int T>
+static int tf_out(const float2 *spec, const int *slot_plane, const float *qscale, const TfGeom &g, const float2 *tw,
+                  float scale, float *dst, hipStream_t st)
+{
+    const size_t lds = sizeof(float2) * kTfCG * g.B * (T / 2 + 1);
+    hipLaunchKernelGGL(k_tf_out<T>, dim3((unsigned)(g.nslot * g.ntile * g.ngrp)), dim3(256), lds, st, spec, slot_plane,
+                       qscale, g, tw, scale, dst);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+#define SID_TF_SWITCH(T_, CALL)                          \
+    switch (T_) {                                        \
+    case 16: return CALL(16);                            \
+    case 24: return CALL(24);                            \
+    case 32: return CALL(32);                            \
+    case 36: return CALL(36);                            \
+    case 40: return CALL(40);                            \
+    case 48: return CALL(48);                            \
+    case 64: return CALL(64);                            \
+    default: set_error("internal: tiled length %d", T_); \
+        return SID_ERR_ARG;                              \
+    }
+
+static int tf_in_any(int T, const float *src, const int *slot_plane, const float *qscale, const TfGeom &g,
+                     const float2 *tw, float2 *out, float *vmax, hipStream_t st)
+{
+#define SID_TF_IN(L) tf_in<L>(src, slot_plane, qscale, g, tw, out, vmax, st)
+    SID_TF_SWITCH(T, SID_TF_IN)
+#undef SID_TF_IN
+}
+
+static int tf_out_any(int T, const float2 *spec, const int *slot_plane, const float *qscale, const TfGeom &g,
+                      const float2 *tw, float scale, float *dst, hipStream_t st)
+{
+#define SID_TF_OUT(L) tf_out<L>(spec, slot_plane, qscale, g, tw, scale, dst, st)
+    SID_TF_SWITCH(T, SID_TF_OUT)
+#undef SID_TF_OUT
+}
+
+template <bool BWD>
+static int tf_gemm(const FftState &f, const void *A, const float2 *Bm, int nslot, int KS, int kB, float2 *out,
+                   hipStream_t st)
+{
+    const int RB = f.chp / 16, ntb = (f.ntile + 15) / 16, ntp = ntb * 16;
+    const dim3 grid((unsigned)(8 * ((f.nbin + 7) / 8)));
+    const tu32x4 *a = (const tu32x4 *)A;
+#define SID_TF_GEMM(NT) \
+    hipLaunchKernelGGL((k_tf_gemm<NT, BWD>), grid, dim3(256), 0, st, a, Bm, f.tmax, f.nbin, nslot, f.ntile, ntp, KS, kB, \
+                       RB, f.chp, out)
+    switch (ntb) {
+    case 1: SID_TF_GEMM(1); break;
+    case 2: SID_TF_GEMM(2); break;
+    case 3: SID_TF_GEMM(3); break;
+    default: SID_TF_GEMM(4); break;
+    }
+#undef SID_TF_GEMM
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+static int tf_tmax(const FftState &f, int rowlen, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tf_tmax, dim3((unsigned)((f.ntile + 3) / 4)), dim3(256), 0, st, f.vmax, f.ntile, rowlen,
+                       f.tmax);
+    SID_TRY_HIP(hipGetLastError());
+    return SID_OK;
+}
+
+int tf_forward(sid_psf *p, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    FftState &f = sp.fft;
+    const int P = (int)f.planes.size();
+    TfGeom g = tf_geom(p, f);
+    g.off = -f.tDhi;
+    SID_TRY(tf_in_any(f.tT, sp.vol, f.d_slot_plane, f.qscale, g, f.ft.tw_fwd, f.vin, f.vmax, st));
+    SID_TRY(tf_tmax(f, P * f.ngrp, st));
+    fft_mark(f, st, 0, true);
+    SID_TRY(tf_gemm<false>(f, f.G, f.vin, 1, f.kpad / 16, f.kpad, f.yout, st));
+    fft_mark(f, st, 0, false);
+    TfGeom go = g;
+    go.nslot = 1;
+    go.off = f.tK - 1;
+    go.pad = 0;
+    go.S = (int)p->NP;
+    go.XP = sp.Lx;
+    go.YP = sp.Ly;
+    return tf_out_any(f.tT, f.yout, nullptr, nullptr, go, f.ft.tw_inv, 1.0f / ((float)f.tT * (float)f.tT), f.tout, st);
+}
+
+int tf_backward(sid_psf *p, hipStream_t st)
+{
+    SensorPlan &sp = p->plan;
+    FftState &f = sp.fft;
+    const int P = (int)f.planes.size();
+    TfGeom gi = tf_geom(p, f);
+    gi.nslot = 1;
+    gi.nz_in = 1;
+    gi.off = f.tDlo;
+    gi.kslot = 0;
+    gi.kdim = kTfKsB * 16;
+    SID_TRY(tf_in_any(f.tT, sp.img, nullptr, nullptr, gi, f.ft.tw_fwd, f.zin, f.vmax, st));
+    SID_TRY(tf_tmax(f, f.ngrp, st));
+    fft_mark(f, st, 1, true);
+    SID_TRY(tf_gemm<true>(f, f.Gb, f.zin, P, kTfKsB, kTfKsB * 16, f.bout, st));
+    fft_mark(f, st, 1, false);
+    TfGeom go = tf_geom(p, f);
+    go.off = 0;
+    return tf_out_any(f.tT, f.bout, f.d_slot_plane, f.qscale, go, f.ft.tw_inv, 1.0f / ((float)f.tT * (float)f.tT),
+                      f.dfft, st);
+}
+
+}  // namespace sid

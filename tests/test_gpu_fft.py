@@ -91,12 +91,14 @@ def test_fft_and_direct_agree_on_workload(env):
 
 @pytest.mark.parametrize("case", [CASES[1], CASES[3], CASES[5]],
                          ids=lambda c: "n%d_X%d_Y%d_M%d_Nz%d_%s" % (c[:5] + (c[5] or "full",)))
-def test_fft_fp16_spectra_vs_oracle(env, case):
-    """BF16X2 mode keeps the FFT planes' spectra in FP16, row q scaled by
+def test_fft_fp16_spectra_vs_oracle(env, case, monkeypatch):
+    """BF16X2 mode with spectra of the whole padded sensor (SID_FFT_UNTILED;
+    the tiled form has its own tests) keeps them in FP16, row q scaled by
     1/sum|h_q|.  One stored value carries a relative rounding error of at most
     2^-11 (4.9e-4); the bar is that bound, and every output sums 225 * Nz
     such values so the measured error sits well below it."""
     torch, sid_amd = env
+    monkeypatch.setenv("SID_FFT_UNTILED", "1")
     n, X, Y, M, Nz, crop = case
     print("case: fft fp16 spectra", case, flush=True)
     rng = np.random.default_rng(zlib.crc32(repr(case).encode()) + 1)
