@@ -357,6 +357,9 @@ def main():
                     help="skip the host-buffer leg (sid_deconv_rl with PCIe transfers, one 8-iteration call)")
     ap.add_argument("--no-direct", action="store_true",
                     help="skip the direct-only leg (every plane on the MFMA GEMM, one step per precision)")
+    ap.add_argument("--clean-exit", action="store_true",
+                    help="end through the interpreter's normal teardown (profilers such as rocprofv3 write their "
+                         "results there); by default the process leaves right after the result line")
     ap.add_argument("--fft-planes", type=int, default=-1,
                     help="pin the K widest planes to the FFT path and the rest to the direct GEMM (reproduces an "
                          "autotuned split in profiler runs, whose serialised kernels would mislead the autotuner)")
@@ -378,7 +381,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.grad_only:
         print(json.dumps({"gradient": grad_bench(dev)}))
-        return
+        return args
     if args.profile_copy:
         a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -689,10 +692,12 @@ def main():
         log(f"CPU baseline {out['cpu_baseline']['value']:.5f} RL iter/s on {out['cpu_baseline']['cores']} threads "
             f"({out['cpu_baseline']['cpu_model']}); rl_small {out['cpu_baseline']['rl_small']['iter_per_s']} RL iter/s")
     print(json.dumps(out), flush=True)
+    return args
 
 
 if __name__ == "__main__":
-    main()
+    if main().clean_exit:
+        sys.exit(0)
     # The result is printed and the device state released; leave without the
     # interpreter's and the runtimes' teardown, so no process of the run is
     # still winding down after its last line (nothing here starts a child

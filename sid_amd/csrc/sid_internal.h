@@ -144,11 +144,13 @@ struct FftState {
     int tT = 0, tH = 0, tB = 0;        // FFT length, stored half length tT/2 + 1, valid outputs per tile
     int tDlo = 0, tDhi = 0, tK = 0;    // common lenslet tap range of the planes and its width
     int ntx = 0, nty = 0, ntile = 0, nbin = 0, chp = 0, kslot = 0, kpad = 0, ngrp = 0;
+    int tsplit_f = 1, tchunk_f = 0;    // forward GEMM: K-step runs per bin and their length
+    int tsplit_b = 1, tchunk_b = 0;    // backward GEMM: slot runs per bin and their length
     FftPlan1D ft;                      // twiddles of length tT
     void *Gb = nullptr;                // backward spectra (G holds the forward ones), fragment order
     float2 *vin = nullptr;             // forward input spectra [nbin][ntile][kpad]
     float2 *zin = nullptr;             // backward input (image) spectra [nbin][ntile][chp]
-    float2 *yout = nullptr;            // forward GEMM result [nbin][ntile][chp]
+    float2 *yout = nullptr;            // forward GEMM partial results [tsplit_f][nbin][ntp][chp]
     float2 *bout = nullptr;            // backward GEMM result [nbin][slot][ntile][chp]
     float *vmax = nullptr;             // per tile and transform block: bound on its spectra
     float *tmax = nullptr;             // per tile: bound over all its blocks

@@ -52,6 +52,18 @@ typedef _Float16 th8 __attribute__((ext_vector_type(8)));
 
 constexpr int kTfCG = 8;   // channels per transform block (4 real pairs)
 
+// Work item of this block when the grid is 8 * ceil(n / 8) blocks: the
+// hardware deals blocks b, b + 8, ... to one XCD, so each XCD takes a
+// contiguous range of items (neighbouring items share L2 lines: the channel
+// groups of one tile read and write the same 128-byte lines).  -1 past n.
+__device__ __forceinline__ int xcd_item(int n)
+{
+    const int per = (n + 7) >> 3;
+    const int it = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    return it < n ? it : -1;
+}
+static inline unsigned xcd_grid(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
+
 // ---------------------------------------------------------------------------
 // Register FFT of compile-time length N (2/3/5-smooth): Stockham stages with
 // radices 4, then 2, 3, 5; tw[t] = exp(sgn 2 pi i t / N).  All indices are
@@ -129,7 +141,9 @@ __global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, co
     constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2;
     extern __shared__ float2 tsm[];   // [CG][T][H]
     __shared__ unsigned int rmax[CG];   // per channel: largest row sum of |w| (float bits, >= 0)
-    const int grp = blockIdx.x % g.ngrp, rest = blockIdx.x / g.ngrp;
+    const int item = xcd_item(g.nslot * g.ntile * g.ngrp);
+    if (item < 0) return;
+    const int grp = item % g.ngrp, rest = item / g.ngrp;
     const int tile = rest % g.ntile, slot = rest / g.ntile;
     const int tx = tile / g.nty, ty = tile - tx * g.nty;
     const int x0 = tx * g.B + g.off, y0 = ty * g.B + g.off;
@@ -197,19 +211,23 @@ __global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, co
     }
 }
 
-// Inverse: spec[bin][slot][ntp][chp] -> along x (one stored fy per thread;
-// only the tile's valid rows kept in LDS), then along y in real pairs with the
-// Hermitian completion; the valid B x B block, times scale * s_q (qscale, when
-// given), goes to the destination layout.
-template <int T>
-__global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec, const int *__restrict__ slot_plane,
-                                                const float *__restrict__ qscale, TfGeom g,
-                                                const float2 *__restrict__ tw, float scale, float *__restrict__ dst)
+// Inverse: spec[bin][slot][ntp][chp] (the sum of nsum partial spectra
+// pstride apart) -> along x (one stored fy per thread; only the tile's valid
+// rows kept in LDS), then along y in real pairs with the Hermitian completion;
+// the valid B x B block, times scale * s_q (qscale, when given), goes to the
+// destination layout.
+template <int T, bool SUM>
+__global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec, int nsum, int64_t pstride,
+                                                const int *__restrict__ slot_plane, const float *__restrict__ qscale,
+                                                TfGeom g, const float2 *__restrict__ tw, float scale,
+                                                float *__restrict__ dst)
 {
     constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2;
     extern __shared__ float2 tsm[];   // [CG][B][H]
     const int B = g.B;
-    const int grp = blockIdx.x % g.ngrp, rest = blockIdx.x / g.ngrp;
+    const int item = xcd_item(g.nslot * g.ntile * g.ngrp);
+    if (item < 0) return;
+    const int grp = item % g.ngrp, rest = item / g.ngrp;
     const int tile = rest % g.ntile, slot = rest / g.ntile;
     const int tx = tile / g.nty, ty = tile - tx * g.nty;
     const int c = slot_plane ? slot_plane[slot] : 0;
@@ -223,6 +241,17 @@ __global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec,
 #pragma unroll
         for (int fx = 0; fx < T; ++fx)
             w[fx] = chg < g.n2 ? s[(int64_t)(fx * H + k) * bstride] : make_float2(0.f, 0.f);
+        if constexpr (SUM) {
+            for (int q = 1; q < nsum; ++q) {   // partial sums of a split GEMM, in a fixed order
+                const float2 *sq = s + q * pstride;
+#pragma unroll
+                for (int fx = 0; fx < T; ++fx) {
+                    const float2 v = chg < g.n2 ? sq[(int64_t)(fx * H + k) * bstride] : make_float2(0.f, 0.f);
+                    w[fx].x += v.x;
+                    w[fx].y += v.y;
+                }
+            }
+        }
         reg_fft<T>(w, tw, 1.0f);
 #pragma unroll
         for (int x = 0; x < T; ++x) {
@@ -282,10 +311,14 @@ __global__ __launch_bounds__(256) void k_tf_tmax(const float *__restrict__ vmax,
 }
 
 // ---------------------------------------------------------------------------
-// Spectra GEMM.  Block = one frequency bin; over all tiles it computes
+// Spectra GEMM.  Block = one frequency bin and one part of its work; over all
+// tiles it computes
 //     C[row][tile] = sum_k A[row][k] * Bm[tile][k]        (complex)
-// forward: rows uv, K = (plane, ab), one slot; backward: for each of the nslot
-// planes in turn, rows ab, K = uv (16 K steps of 16 per slot, uv padded).
+// forward: rows uv, K = (plane, ab), one slot, the K steps cut into nsplit
+// runs of `chunk` steps, part q writing its partial sum at out + q pstride;
+// backward: for each of the planes in turn, rows ab, K = uv (16 K steps of 16
+// per slot, uv padded), the slots cut into nsplit runs of `chunk` slots.  (684
+// bins on 512 resident blocks would leave a second round two-thirds empty.)
 // A: FP16 (re, im) pairs in fragment order, the 16 bytes of lane (l16, g) of
 // flattened K step s (slot * KS + ks) and row block rb at A[((bin nslot KS + s)
 // RB + rb) 64 + lane]: row rb*16 + l16, complex k = ks*16 + 4g .. 4g + 3.
@@ -339,24 +372,33 @@ constexpr int kTfKsB = 16;    // backward K steps per slot (uv padded to 256)
 template <int NTB, bool BWD>
 __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A, const float2 *__restrict__ Bm,
                                                     const float *__restrict__ tmax, int nbin, int nslot, int ntile,
-                                                    int ntp, int KS, int kB, int RB, int chp,
-                                                    float2 *__restrict__ out)
+                                                    int ntp, int KS, int kB, int RB, int chp, int nsplit, int chunk,
+                                                    int64_t pstride, float2 *__restrict__ out)
 {
     constexpr int D = kTfRing;
-    constexpr int U = BWD ? kTfKsB : D;   // steps per loop trip (a backward trip is one slot)
-    static_assert(U % D == 0 && U % 2 == 0, "ring and LDS buffers repeat within a trip");
+    constexpr int U = D;   // steps per loop trip
+    static_assert(U % 2 == 0 && kTfKsB % U == 0, "LDS buffers repeat within a trip, slots end at trip ends");
     __shared__ tu32x4 bop[2][4][4][64];   // [buffer][operand rh, rl, ih, il][tile block][lane]
-    // blocks b, b + 8, ... run on one XCD: each XCD takes a contiguous range of bins
-    const int per = (nbin + 7) >> 3;
-    const int idx = blockIdx.x >> 3, bin = (blockIdx.x & 7) * per + idx;
-    if (idx >= per || bin >= nbin) return;
+    const int item = xcd_item(nbin * nsplit);
+    if (item < 0) return;
+    const int bin = item / nsplit, part = item - bin * nsplit;
+    // this block's steps: forward a run of K steps, backward a run of slots
+    int s_lo, nsteps, slot0 = 0;
+    if constexpr (BWD) {
+        slot0 = part * chunk;
+        s_lo = slot0 * KS;
+        nsteps = min(chunk, nslot - slot0) * KS;
+    } else {
+        s_lo = part * chunk;
+        nsteps = min(chunk, nslot * KS - s_lo);
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, g = lane >> 4;
     // this thread's Bm chunk of every step: tile tt, complex 4 qt .. 4 qt + 3
     const int tt = threadIdx.x >> 2, qt = threadIdx.x & 3;
     const bool tvalid = tt < ntile;
     const float mt = tvalid ? tmax[tt] : 0.f;
     const float sc = mt > 0.f ? 1.0f / mt : 0.f;
-    const float2 *bsrc = Bm + ((int64_t)bin * ntile + (tvalid ? tt : 0)) * kB + 4 * qt;
+    const float2 *bsrc = Bm + ((int64_t)bin * ntile + (tvalid ? tt : 0)) * kB + 4 * qt + (BWD ? 0 : (int64_t)s_lo * 16);
     tu32x4 *bdst = &bop[0][0][tt >> 4][(tt & 15) + 16 * qt];
     const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
     float mx[NTB];
@@ -366,8 +408,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
 #pragma unroll
     for (int i = 0; i < 4; ++i) rbw[i] = min(wave + 4 * i, RB - 1);
     const int64_t astep = (int64_t)RB * 64;
-    const tu32x4 *abase = A + (int64_t)bin * nslot * KS * astep + lane;
-    const int nsteps = nslot * KS;
+    const tu32x4 *abase = A + ((int64_t)bin * nslot * KS + s_lo) * astep + lane;
     tu32x4 a[D][4];
     float4 b0[D], b1[D];
 #pragma unroll
@@ -422,9 +463,9 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
             for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
             __builtin_amdgcn_sched_barrier(0);   // keep the steps apart (register pressure)
         }
-        if constexpr (BWD) {
-            // slot s0 / 16 is complete: write it (repeated row blocks store the same values twice)
-            float2 *ob = out + ((int64_t)bin * nslot + s0 / kTfKsB) * ntp * chp + 4 * g;
+        if (BWD && ((s0 + U) & (kTfKsB - 1)) == 0) {
+            // slot slot0 + s0 / 16 is complete: write it (repeated row blocks store the same values twice)
+            float2 *ob = out + ((int64_t)bin * nslot + slot0 + s0 / kTfKsB) * ntp * chp + 4 * g;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -439,7 +480,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
         }
     }
     if constexpr (!BWD) {
-        float2 *ob = out + (int64_t)bin * ntp * chp + 4 * g;
+        float2 *ob = out + part * pstride + (int64_t)bin * ntp * chp + 4 * g;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -590,7 +631,8 @@ static int tf_set_lds()
     constexpr size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
     if (lds > 65536) {
         SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_in<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_out<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_out<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_out<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     return SID_OK;
 }
@@ -727,11 +769,41 @@ int tf_build(sid_psf *p, hipStream_t st)
     const int ntp = (f.ntile + 15) / 16 * 16;
     const int64_t nvin = (int64_t)f.nbin * f.ntile * f.kpad, nzin = (int64_t)f.nbin * f.ntile * kTfKsB * 16;
     const int64_t nout = (int64_t)f.nbin * ntp * chp;
+    {
+        // Work splits of the two GEMMs: bins x parts dealt over the resident
+        // blocks (two per CU); the cost of a split is its rounds of blocks
+        // times the steps of one block, ties to fewer parts.  SID_TF_SPLIT_F /
+        // SID_TF_SPLIT_B force the number of parts (measurement).
+        int cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) cus = prop.multiProcessorCount;
+        const int64_t res = 2 * (int64_t)cus;
+        auto pick = [&](int units, int gran, int maxs, const char *env, int &nsplit, int &chunk) {
+            int force = 0;
+            if (const char *e = std::getenv(env)) force = std::max(1, std::atoi(e));
+            double best = 0;
+            nsplit = 1;
+            chunk = units;
+            for (int ns = 1; ns <= maxs; ++ns) {
+                if (force && ns != force) continue;
+                const int ch = ((units + ns - 1) / ns + gran - 1) / gran * gran;
+                const int parts = (units + ch - 1) / ch;
+                const double cost = (double)((f.nbin * (int64_t)parts + res - 1) / res) * ch;
+                if (best == 0 || cost < best * 0.999) {
+                    best = cost;
+                    nsplit = parts;
+                    chunk = ch;
+                }
+            }
+        };
+        pick(f.kpad / 16, kTfRing, 8, "SID_TF_SPLIT_F", f.tsplit_f, f.tchunk_f);
+        pick(P, 1, std::min(P, 8), "SID_TF_SPLIT_B", f.tsplit_b, f.tchunk_b);
+    }
     SID_TRY_HIP(hipMalloc(&f.vin, sizeof(float2) * nvin));
     SID_TRY_HIP(hipMemsetAsync(f.vin, 0, sizeof(float2) * nvin, st));   // K padding stays zero
     SID_TRY_HIP(hipMalloc(&f.zin, sizeof(float2) * nzin));
     SID_TRY_HIP(hipMemsetAsync(f.zin, 0, sizeof(float2) * nzin, st));
-    SID_TRY_HIP(hipMalloc(&f.yout, sizeof(float2) * nout));
+    SID_TRY_HIP(hipMalloc(&f.yout, sizeof(float2) * nout * f.tsplit_f));
     SID_TRY_HIP(hipMalloc(&f.bout, sizeof(float2) * nout * P));
     SID_TRY_HIP(hipMalloc(&f.vmax, sizeof(float) * (int64_t)f.ntile * P * f.ngrp));
     SID_TRY_HIP(hipMalloc(&f.tmax, sizeof(float) * f.ntile));
@@ -787,19 +859,24 @@ static int tf_in(const float *src, const int *slot_plane, const float *qscale, c
                  float2 *out, float *vmax, hipStream_t st)
 {
     const size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
-    hipLaunchKernelGGL(k_tf_in<T>, dim3((unsigned)(g.nslot * g.ntile * g.ngrp)), dim3(256), lds, st, src, slot_plane,
+    hipLaunchKernelGGL(k_tf_in<T>, dim3(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), dim3(256), lds, st, src, slot_plane,
                        qscale, g, tw, out, vmax);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
 template <int T>
-static int tf_out(const float2 *spec, const int *slot_plane, const float *qscale, const TfGeom &g, const float2 *tw,
-                  float scale, float *dst, hipStream_t st)
+static int tf_out(const float2 *spec, int nsum, int64_t pstride, const int *slot_plane, const float *qscale,
+                  const TfGeom &g, const float2 *tw, float scale, float *dst, hipStream_t st)
 {
     const size_t lds = sizeof(float2) * kTfCG * g.B * (T / 2 + 1);
-    hipLaunchKernelGGL(k_tf_out<T>, dim3((unsigned)(g.nslot * g.ntile * g.ngrp)), dim3(256), lds, st, spec, slot_plane,
-                       qscale, g, tw, scale, dst);
+    const dim3 grid(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp));
+    if (nsum > 1)
+        hipLaunchKernelGGL((k_tf_out<T, true>), grid, dim3(256), lds, st, spec, nsum, pstride, slot_plane, qscale, g, tw,
+                           scale, dst);
+    else
+        hipLaunchKernelGGL((k_tf_out<T, false>), grid, dim3(256), lds, st, spec, nsum, pstride, slot_plane, qscale, g,
+                           tw, scale, dst);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -825,10 +902,10 @@ static int tf_in_any(int T, const float *src, const int *slot_plane, const float
 #undef SID_TF_IN
 }
 
-static int tf_out_any(int T, const float2 *spec, const int *slot_plane, const float *qscale, const TfGeom &g,
-                      const float2 *tw, float scale, float *dst, hipStream_t st)
+static int tf_out_any(int T, const float2 *spec, int nsum, int64_t pstride, const int *slot_plane,
+                      const float *qscale, const TfGeom &g, const float2 *tw, float scale, float *dst, hipStream_t st)
 {
-#define SID_TF_OUT(L) tf_out<L>(spec, slot_plane, qscale, g, tw, scale, dst, st)
+#define SID_TF_OUT(L) tf_out<L>(spec, nsum, pstride, slot_plane, qscale, g, tw, scale, dst, st)
     SID_TF_SWITCH(T, SID_TF_OUT)
 #undef SID_TF_OUT
 }
@@ -838,11 +915,13 @@ static int tf_gemm(const FftState &f, const void *A, const float2 *Bm, int nslot
                    hipStream_t st)
 {
     const int RB = f.chp / 16, ntb = (f.ntile + 15) / 16, ntp = ntb * 16;
-    const dim3 grid((unsigned)(8 * ((f.nbin + 7) / 8)));
+    const int nsplit = BWD ? f.tsplit_b : f.tsplit_f, chunk = BWD ? f.tchunk_b : f.tchunk_f;
+    const int64_t pstride = (int64_t)f.nbin * ntp * f.chp;
+    const dim3 grid(xcd_grid((int64_t)f.nbin * nsplit));
     const tu32x4 *a = (const tu32x4 *)A;
 #define SID_TF_GEMM(NT) \
     hipLaunchKernelGGL((k_tf_gemm<NT, BWD>), grid, dim3(256), 0, st, a, Bm, f.tmax, f.nbin, nslot, f.ntile, ntp, KS, kB, \
-                       RB, f.chp, out)
+                       RB, f.chp, nsplit, chunk, pstride, out)
     switch (ntb) {
     case 1: SID_TF_GEMM(1); break;
     case 2: SID_TF_GEMM(2); break;
@@ -881,7 +960,8 @@ int tf_forward(sid_psf *p, hipStream_t st)
     go.S = (int)p->NP;
     go.XP = sp.Lx;
     go.YP = sp.Ly;
-    return tf_out_any(f.tT, f.yout, nullptr, nullptr, go, f.ft.tw_inv, 1.0f / ((float)f.tT * (float)f.tT), f.tout, st);
+    return tf_out_any(f.tT, f.yout, f.tsplit_f, (int64_t)f.nbin * go.ntp * f.chp, nullptr, nullptr, go, f.ft.tw_inv,
+                      1.0f / ((float)f.tT * (float)f.tT), f.tout, st);
 }
 
 int tf_backward(sid_psf *p, hipStream_t st)
@@ -902,8 +982,8 @@ int tf_backward(sid_psf *p, hipStream_t st)
     fft_mark(f, st, 1, false);
     TfGeom go = tf_geom(p, f);
     go.off = 0;
-    return tf_out_any(f.tT, f.bout, f.d_slot_plane, f.qscale, go, f.ft.tw_inv, 1.0f / ((float)f.tT * (float)f.tT),
-                      f.dfft, st);
+    return tf_out_any(f.tT, f.bout, 1, 0, f.d_slot_plane, f.qscale, go, f.ft.tw_inv,
+                      1.0f / ((float)f.tT * (float)f.tT), f.dfft, st);
 }
 
 }  // namespace sid
