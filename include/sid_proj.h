@@ -126,8 +126,12 @@ typedef struct {
     int32_t plane_method[512]; /* per plane in the current sensor plan: SID_METHOD_DIRECT / _FFT, 0 = empty */
     double autotune_ms[4];     /* last autotune: direct ms, FFT ms (all eligible planes), #eligible, #chosen */
     int64_t spectra_bytes;     /* device bytes of FFT-path spectra in the current plan */
-    int32_t fft_tile[4];       /* FFT path in tiles: FFT length T, valid lenslets per tile side, tiles,
-                                  common lenslet taps K; all 0 for spectra of the whole padded sensor */
+    int32_t fft_tile[4];       /* FFT path in tiles, its widest plane group: FFT length T, valid lenslets
+                                  per tile side, tiles, common lenslet taps K; all 0 for spectra of the
+                                  whole padded sensor */
+    int32_t fft_groups;        /* plane groups of the tiled FFT path (0 when not tiled) */
+    int32_t fft_group_planes[8]; /* planes per group, narrowest taps first (first 8 groups) */
+    int32_t fft_group_T[8];    /* FFT length per group */
 } sid_psf_info;
 int sid_psf_get_info(const sid_psf *psf, sid_psf_info *info);
 

@@ -143,7 +143,8 @@ class PSF:
                     support=[(inf.support_lo[c], inf.support_hi[c]) for c in range(inf.Nz)],
                     plane_method=[names.get(inf.plane_method[c], "?") for c in range(inf.Nz)],
                     autotune_ms=list(inf.autotune_ms), spectra_bytes=inf.spectra_bytes,
-                    fft_tile=list(inf.fft_tile))
+                    fft_tile=list(inf.fft_tile),
+                    fft_groups=[(inf.fft_group_planes[g], inf.fft_group_T[g]) for g in range(min(inf.fft_groups, 8))])
 
     def timing(self) -> list:
         buf = (ctypes.c_double * 12)()

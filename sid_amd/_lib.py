@@ -84,7 +84,10 @@ class PsfInfo(ctypes.Structure):
                 ("support_lo", ctypes.c_int64 * 512), ("support_hi", ctypes.c_int64 * 512),
                 ("plane_method", ctypes.c_int32 * 512), ("autotune_ms", ctypes.c_double * 4),
                 ("spectra_bytes", ctypes.c_int64),
-                ("fft_tile", ctypes.c_int32 * 4)]
+                ("fft_tile", ctypes.c_int32 * 4),
+                ("fft_groups", ctypes.c_int32),
+                ("fft_group_planes", ctypes.c_int32 * 8),
+                ("fft_group_T", ctypes.c_int32 * 8)]
 
 
 class LsOpts(ctypes.Structure):

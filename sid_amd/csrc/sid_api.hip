@@ -470,11 +470,18 @@ int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
     info->spectra_bytes = p->plan.fft.g_bytes;
     {
         const FftState &f = p->plan.fft;
-        const bool t = f.active && f.tiled;
-        info->fft_tile[0] = t ? f.tT : 0;
-        info->fft_tile[1] = t ? f.tB : 0;
-        info->fft_tile[2] = t ? f.ntile : 0;
-        info->fft_tile[3] = t ? f.tK : 0;
+        const bool t = f.active && f.tiled && !f.tg.empty();
+        const TfGroup *w = t ? &f.tg.back() : nullptr;   // the widest group
+        info->fft_tile[0] = t ? w->T : 0;
+        info->fft_tile[1] = t ? w->B : 0;
+        info->fft_tile[2] = t ? w->ntile : 0;
+        info->fft_tile[3] = t ? w->K : 0;
+        info->fft_groups = t ? (int32_t)f.tg.size() : 0;
+        for (int gi = 0; gi < 8; ++gi) {
+            const bool h = t && gi < (int)f.tg.size();
+            info->fft_group_planes[gi] = h ? f.tg[gi].ns : 0;
+            info->fft_group_T[gi] = h ? f.tg[gi].T : 0;
+        }
     }
     return SID_OK;
 }
@@ -606,7 +613,8 @@ static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
 // overlap.  autotune_methods prices each plane alone (direct GEMM time against
 // FFT-path time), which minimises the serial sum; overlapped, a projection
 // costs about the longer half plus their interference, which favours giving
-// the GEMM more planes.  Starting from that choice (k FFT planes), time the
+// the GEMM more planes.  Starting from that choice (k FFT planes) or, when it
+// is 1 % faster, from every eligible plane on the FFT path, time the
 // overlapped forward + backward pair with the narrowest FFT planes moved to the
 // GEMM s at a time (s about a ninth of k) while that gains 1 % over the
 // incumbent timed before and after it, else with s more FFT planes at a time
@@ -644,6 +652,18 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
     };
     int best_k = k0;
     double best_t = eval(k0);
+    // every eligible plane on the FFT path (nothing beside it) is a candidate
+    // of its own: with the tiled form, narrow planes cost little there
+    if (k0 < ne && rc == SID_OK) {
+        const double t = eval(ne);
+        if (rc == SID_OK && t < 0.99 * best_t) {
+            const double t_again = eval(best_k);
+            if (rc == SID_OK && t < 0.99 * t_again) {
+                best_t = t;
+                best_k = ne;
+            }
+        }
+    }
     for (int dir : {-s, s}) {
         bool moved = false;
         for (int k = best_k + dir; rc == SID_OK && k >= 0 && k <= ne; k += dir) {
@@ -670,7 +690,7 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
     // blocks per CU are tried on the same terms as the split (at 1005^2 x 51,
     // BF16X2, 26 FFT planes: 1.25 per CU 115.9 RL iter/s against 114.1).
     p->gemm_blocks_f = 0;
-    if (rc == SID_OK && !p->gemm_blocks_env && best_k > 0) {
+    if (rc == SID_OK && !p->gemm_blocks_env && best_k > 0 && best_k < ne) {
         auto timed = [&](int cap) {
             p->gemm_blocks_f = cap;
             const double t = time_projection_pair(p, st, &rc);

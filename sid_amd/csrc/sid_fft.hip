@@ -1034,6 +1034,14 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
     f.n2 = n2;
     // FP16 spectra in the BF16X2 mode, row q scaled by 1/qscale[q]
     f.half = p->precision == SID_PREC_BF16X2;
+    const bool tiled = f.half && std::getenv("SID_FFT_UNTILED") == nullptr;
+    if (tiled) {
+        // the tiled form groups slots of similar lenslet tap range: ascending width
+        std::stable_sort(f.planes.begin(), f.planes.end(), [&](int a, int b) {
+            return p->planes[a].tf_hi - p->planes[a].tf_lo < p->planes[b].tf_hi - p->planes[b].tf_lo;
+        });
+        for (size_t sl = 0; sl < f.planes.size(); ++sl) f.slot_of[f.planes[sl]] = (int)sl;
+    }
     std::vector<SpecJob> jobs;
     for (size_t sl = 0; sl < f.planes.size(); ++sl) {
         const Plane &pl = p->planes[f.planes[sl]];
@@ -1074,7 +1082,7 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
         SID_TRY_HIP(hipMemcpy(f.d_plane_list, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
     }
     // BF16X2: the tiled form (SID_FFT_UNTILED=1 keeps the whole-sensor spectra)
-    if (f.half && std::getenv("SID_FFT_UNTILED") == nullptr) {
+    if (tiled) {
         const int rc = tf_build(p, st);
         if (rc == SID_OK) {
             (void)hipFree(d_jobs);

@@ -110,6 +110,22 @@ struct FftPlan1D {
 };
 
 // FFT path state for one sensor geometry (sid_fft.hip).
+// One group of the tiled FFT path (sid_tfft.hip).
+struct TfGroup {
+    int s0 = 0, ns = 0;                // slots [s0, s0 + ns) of FftState::planes
+    int T = 0, H = 0, B = 0;           // FFT length, stored half length T / 2 + 1, valid outputs per tile side
+    int Dlo = 0, Dhi = 0, K = 0;       // common lenslet tap range of the group's planes and its width
+    int ntx = 0, nty = 0, ntile = 0, nbin = 0, kpad = 0;
+    int tsplit_f = 1, tchunk_f = 0;    // forward GEMM: K-step runs per bin and their length
+    int tsplit_b = 1, tchunk_b = 0;    // backward GEMM: slot runs per bin and their length
+    FftPlan1D ft;                      // twiddles of length T (spectra build)
+    void *G = nullptr, *Gb = nullptr;  // forward / backward spectra, MFMA fragment order
+    float2 *vin = nullptr;             // forward input spectra [nbin][ntile][kpad] (K padding stays zero)
+    float2 *zin = nullptr;             // backward input (image) spectra [nbin][ntile][256] (uv padding zero)
+    float *vmax = nullptr;             // per tile and transform block: bound on its spectra
+    float *tmax = nullptr;             // per tile: bound over all its blocks
+};
+
 struct FftState {
     bool active = false;
     std::vector<int> planes;       // FFT planes, in slot order
@@ -137,23 +153,16 @@ struct FftState {
     int *d_plane_list = nullptr;   // planes of H on the direct GEMM, then those on the FFT path
     int n_list_direct = 0, n_list_fft = 0;
     int64_t g_bytes = 0;
-    // Tiled form (BF16X2, sid_tfft.hip): overlap-save tiles of tT x tT lenslets
-    // over the lenslet sub-images; per frequency bin one complex GEMM over all
-    // tiles (FP16 spectra x FP16 hi + lo inputs on MFMA, FP32 accumulate).
+    // Tiled form (BF16X2, sid_tfft.hip): the FFT planes in groups of similar
+    // lenslet tap range (slots in ascending tap width); per group overlap-save
+    // tiles of T x T lenslets over the lenslet sub-images and, per frequency
+    // bin, one complex GEMM over all tiles (FP16 spectra x FP16 hi + lo inputs
+    // on MFMA, FP32 accumulate).
     bool tiled = false;
-    int tT = 0, tH = 0, tB = 0;        // FFT length, stored half length tT/2 + 1, valid outputs per tile
-    int tDlo = 0, tDhi = 0, tK = 0;    // common lenslet tap range of the planes and its width
-    int ntx = 0, nty = 0, ntile = 0, nbin = 0, chp = 0, kslot = 0, kpad = 0, ngrp = 0;
-    int tsplit_f = 1, tchunk_f = 0;    // forward GEMM: K-step runs per bin and their length
-    int tsplit_b = 1, tchunk_b = 0;    // backward GEMM: slot runs per bin and their length
-    FftPlan1D ft;                      // twiddles of length tT
-    void *Gb = nullptr;                // backward spectra (G holds the forward ones), fragment order
-    float2 *vin = nullptr;             // forward input spectra [nbin][ntile][kpad]
-    float2 *zin = nullptr;             // backward input (image) spectra [nbin][ntile][chp]
-    float2 *yout = nullptr;            // forward GEMM partial results [tsplit_f][nbin][ntp][chp]
-    float2 *bout = nullptr;            // backward GEMM result [nbin][slot][ntile][chp]
-    float *vmax = nullptr;             // per tile and transform block: bound on its spectra
-    float *tmax = nullptr;             // per tile: bound over all its blocks
+    std::vector<TfGroup> tg;
+    int chp = 0, kslot = 0, ngrp = 0;  // GEMM rows (n^2 padded to 16), K per slot (n^2 padded to 4), channel groups
+    float2 *yout = nullptr;            // forward GEMM partial results, largest group [tsplit_f][nbin][ntp][chp]
+    float2 *bout = nullptr;            // backward GEMM result, largest group [nbin][slot][ntp][chp]
     float *tout = nullptr;             // forward result of the FFT planes, lenslet image [nrows][NP]
     std::vector<hipEvent_t> ev;     // event pairs bracketing the GEMVs of the last call
     std::vector<int> ev_kind;        // 0 forward GEMV, 1 backward GEMV (one entry per pair)

@@ -66,11 +66,46 @@ static inline unsigned xcd_grid(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)
 
 // ---------------------------------------------------------------------------
 // Register FFT of compile-time length N (2/3/5-smooth): Stockham stages with
-// radices 4, then 2, 3, 5; tw[t] = exp(sgn 2 pi i t / N).  All indices are
-// compile-time constants after unrolling, so the sequence stays in registers.
+// radices 4, then 2, 3, 5.  All indices are compile-time constants after
+// unrolling, so the sequence stays in registers, and the twiddles
+// exp(sgn 2 pi i t / N) come from a table evaluated by the compiler (a
+// 24-term Taylor series in double on [-pi, pi], multiples of pi/2 exact),
+// which folds into literal operands: no table loads, no scalar registers.
 // ---------------------------------------------------------------------------
-template <int N, int Ns, int R>
-__device__ __forceinline__ void rf_stage(float2 (&x)[N], const float2 *__restrict__ tw, float sgn)
+constexpr double kCxPi = 3.14159265358979323846264338327950288;
+constexpr double cx_taylor(double x, bool sine)
+{
+    double term = sine ? x : 1.0, sum = term;
+    for (int k = 1; k < 24; ++k) {
+        term *= -x * x / (sine ? (2.0 * k) * (2.0 * k + 1) : (2.0 * k - 1) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+template <int N, int SGN>
+struct TwTable {
+    float re[N], im[N];
+    constexpr TwTable() : re(), im()
+    {
+        for (int t = 0; t < N; ++t) {
+            if ((4 * t) % N == 0) {   // quarter turns: exact
+                const int q = 4 * t / N;
+                re[t] = q == 0 ? 1.f : q == 2 ? -1.f : 0.f;
+                im[t] = q == 1 ? (float)SGN : q == 3 ? (float)-SGN : 0.f;
+                continue;
+            }
+            double x = 2.0 * kCxPi * t / N;
+            if (x > kCxPi) x -= 2.0 * kCxPi;
+            re[t] = (float)cx_taylor(x, false);
+            im[t] = (float)(SGN * cx_taylor(x, true));
+        }
+    }
+};
+template <int N, int SGN>
+inline constexpr TwTable<N, SGN> kTw{};
+
+template <int N, int Ns, int R, int SGN>
+__device__ __forceinline__ void rf_stage(float2 (&x)[N])
 {
     float2 y[N];
 #pragma unroll
@@ -80,9 +115,10 @@ __device__ __forceinline__ void rf_stage(float2 (&x)[N], const float2 *__restric
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float2 a = x[j + r * (N / R)];
-            v[r] = (r == 0 || k == 0) ? a : cmul(a, tw[k * r * (N / (Ns * R))]);
+            const int t = k * r * (N / (Ns * R));
+            v[r] = (r == 0 || k == 0) ? a : cmul(a, make_float2(kTw<N, SGN>.re[t], kTw<N, SGN>.im[t]));
         }
-        small_dft<R>(v, sgn);
+        small_dft<R>(v, (float)SGN);
         const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) y[base + r * Ns] = v[r];
@@ -91,24 +127,33 @@ __device__ __forceinline__ void rf_stage(float2 (&x)[N], const float2 *__restric
     for (int i = 0; i < N; ++i) x[i] = y[i];
 }
 
-template <int N, int Ns, int Rem>
+template <int N, int Ns, int Rem, int SGN>
 struct RegFft {
-    static __device__ __forceinline__ void run(float2 (&x)[N], const float2 *__restrict__ tw, float sgn)
+    static __device__ __forceinline__ void run(float2 (&x)[N])
     {
         constexpr int R = Rem % 4 == 0 ? 4 : Rem % 2 == 0 ? 2 : Rem % 3 == 0 ? 3 : 5;
         static_assert(Rem % R == 0, "register FFT lengths are 2/3/5-smooth");
-        rf_stage<N, Ns, R>(x, tw, sgn);
-        RegFft<N, Ns * R, Rem / R>::run(x, tw, sgn);
+        rf_stage<N, Ns, R, SGN>(x);
+        RegFft<N, Ns * R, Rem / R, SGN>::run(x);
     }
 };
-template <int N, int Ns>
-struct RegFft<N, Ns, 1> {
-    static __device__ __forceinline__ void run(float2 (&)[N], const float2 *__restrict__, float) {}
+template <int N, int Ns, int SGN>
+struct RegFft<N, Ns, 1, SGN> {
+    static __device__ __forceinline__ void run(float2 (&)[N]) {}
 };
-template <int N>
-__device__ __forceinline__ void reg_fft(float2 (&x)[N], const float2 *__restrict__ tw, float sgn)
+// Unscaled DFT in place: SGN = -1 forward, +1 inverse.
+template <int N, int SGN>
+__device__ __forceinline__ void reg_fft(float2 (&x)[N])
 {
-    RegFft<N, 1, N>::run(x, tw, sgn);
+    RegFft<N, 1, N, SGN>::run(x);
+}
+
+// Threads of a transform block: both phases in one pass where they fit.
+template <int T>
+constexpr int tf_threads()
+{
+    const int need = std::max(kTfCG / 2 * T, kTfCG * (T / 2 + 1));
+    return std::min(256, (need + 63) / 64 * 64);
 }
 
 // Geometry of one transform launch.  Lenslet (x, y) of plane c of the source
@@ -133,12 +178,12 @@ struct TfGeom {
 // least the window's L1 norm; a maximum, so independent of thread timing).
 // ---------------------------------------------------------------------------
 template <int T>
-__global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, const int *__restrict__ slot_plane,
-                                               const float *__restrict__ qscale, TfGeom g,
-                                               const float2 *__restrict__ tw, float2 *__restrict__ out,
-                                               float *__restrict__ vmax)
+__global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restrict__ src,
+                                                          const int *__restrict__ slot_plane,
+                                                          const float *__restrict__ qscale, TfGeom g,
+                                                          float2 *__restrict__ out, float *__restrict__ vmax)
 {
-    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2;
+    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2, NT = tf_threads<T>();
     extern __shared__ float2 tsm[];   // [CG][T][H]
     __shared__ unsigned int rmax[CG];   // per channel: largest row sum of |w| (float bits, >= 0)
     const int item = xcd_item(g.nslot * g.ntile * g.ngrp);
@@ -150,31 +195,34 @@ __global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, co
     const int c = slot_plane ? slot_plane[slot] : 0;
     const bool live = c < g.nz_in;
     const int ch0 = grp * CG;
+    // the window's columns inside the sensor: y in [ylo, yhi) (the same for the whole block)
+    const int ylo = max(0, -y0), yhi = min(T, g.Ly - y0);
     if (threadIdx.x < CG) rmax[threadIdx.x] = 0u;
     __syncthreads();
     const float *base = src + (int64_t)c * g.XP * g.YP * g.S;
-    for (int it = threadIdx.x; it < NPAIR * T; it += blockDim.x) {
+    for (int it = threadIdx.x; it < NPAIR * T; it += NT) {
         const int pair = it % NPAIR, x = it / NPAIR;
         const int ch = ch0 + 2 * pair;
         const int X = x0 + x;
         const bool rowin = live && X >= 0 && X < g.Lx && ch < g.n2;
         const bool two = ch + 1 < g.n2;
+        // rows outside the sensor read a row inside it and are zeroed; the
+        // second channel of a pair may be the record's padding (masked)
+        const int Xc = min(max(X, 0), g.Lx - 1);
+        const float *prow = base + ((int64_t)(Xc + g.pad) * g.YP + (y0 + g.pad)) * g.S + min(ch, g.n2 - 1);
         float2 z[T];
         float sa = 0.f, sb = 0.f;
 #pragma unroll
         for (int y = 0; y < T; ++y) {
-            const int Y = y0 + y;
             float2 v = make_float2(0.f, 0.f);
-            if (rowin && Y >= 0 && Y < g.Ly) {
-                const float *p = base + (((int64_t)X + g.pad) * g.YP + (Y + g.pad)) * g.S + ch;
-                if (two) v = ld_f2_a4(p);
-                else v.x = p[0];
-            }
+            if (y >= ylo && y < yhi) v = ld_f2_a4(prow + (int64_t)y * g.S);
+            v.x = rowin ? v.x : 0.f;
+            v.y = rowin && two ? v.y : 0.f;
             z[y] = v;
             sa += fabsf(v.x);
             sb += fabsf(v.y);
         }
-        reg_fft<T>(z, tw, -1.0f);
+        reg_fft<T, -1>(z);
         float2 *dA = tsm + ((2 * pair) * T + x) * H, *dB = tsm + ((2 * pair + 1) * T + x) * H;
 #pragma unroll
         for (int k = 0; k < H; ++k) {
@@ -186,20 +234,20 @@ __global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, co
         atomicMax(&rmax[2 * pair + 1], __float_as_uint(sb));
     }
     __syncthreads();
-    const int64_t bstride = (int64_t)g.ntile * g.kdim;
-    for (int it = threadIdx.x; it < CG * H; it += blockDim.x) {
+    const int64_t bstride = (int64_t)g.ntile * g.kdim, fstep = (int64_t)H * bstride;
+    for (int it = threadIdx.x; it < CG * H; it += NT) {
         const int ch = it % CG, k = it / CG;
         const int chg = ch0 + ch;
         float2 w[T];
         const float2 *s = tsm + ch * T * H + k;
 #pragma unroll
         for (int x = 0; x < T; ++x) w[x] = s[x * H];
-        reg_fft<T>(w, tw, -1.0f);
+        reg_fft<T, -1>(w);
         if (chg < g.n2) {
             const float q = qscale ? qscale[slot * g.n2 + chg] : 1.0f;
-            float2 *o = out + (int64_t)tile * g.kdim + slot * g.kslot + chg;
+            float2 *o = out + k * bstride + (int64_t)tile * g.kdim + slot * g.kslot + chg;
 #pragma unroll
-            for (int fx = 0; fx < T; ++fx) o[(int64_t)(fx * H + k) * bstride] = make_float2(w[fx].x * q, w[fx].y * q);
+            for (int fx = 0; fx < T; ++fx) o[fx * fstep] = make_float2(w[fx].x * q, w[fx].y * q);
         }
     }
     if (threadIdx.x == 0) {
@@ -215,14 +263,14 @@ __global__ __launch_bounds__(256) void k_tf_in(const float *__restrict__ src, co
 // pstride apart) -> along x (one stored fy per thread; only the tile's valid
 // rows kept in LDS), then along y in real pairs with the Hermitian completion;
 // the valid B x B block, times scale * s_q (qscale, when given), goes to the
-// destination layout.
+// destination layout (added to what is there when acc is set).
 template <int T, bool SUM>
-__global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec, int nsum, int64_t pstride,
-                                                const int *__restrict__ slot_plane, const float *__restrict__ qscale,
-                                                TfGeom g, const float2 *__restrict__ tw, float scale,
-                                                float *__restrict__ dst)
+__global__ __launch_bounds__(tf_threads<T>()) void k_tf_out(const float2 *__restrict__ spec, int nsum,
+                                                           int64_t pstride, const int *__restrict__ slot_plane,
+                                                           const float *__restrict__ qscale, TfGeom g, float scale,
+                                                           int acc, float *__restrict__ dst)
 {
-    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2;
+    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2, NT = tf_threads<T>();
     extern __shared__ float2 tsm[];   // [CG][B][H]
     const int B = g.B;
     const int item = xcd_item(g.nslot * g.ntile * g.ngrp);
@@ -232,27 +280,30 @@ __global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec,
     const int tx = tile / g.nty, ty = tile - tx * g.nty;
     const int c = slot_plane ? slot_plane[slot] : 0;
     const int ch0 = grp * CG;
-    const int64_t bstride = (int64_t)g.nslot * g.ntp * g.chp;
-    for (int it = threadIdx.x; it < CG * H; it += blockDim.x) {
+    const int64_t bstride = (int64_t)g.nslot * g.ntp * g.chp, fstep = (int64_t)H * bstride;
+    for (int it = threadIdx.x; it < CG * H; it += NT) {
         const int ch = it % CG, k = it / CG;
         const int chg = ch0 + ch;
+        const bool cin = chg < g.n2;   // channels up to the group's end are inside the chp padding
         float2 w[T];
-        const float2 *s = spec + ((int64_t)slot * g.ntp + tile) * g.chp + chg;
+        const float2 *s = spec + k * bstride + ((int64_t)slot * g.ntp + tile) * g.chp + chg;
 #pragma unroll
-        for (int fx = 0; fx < T; ++fx)
-            w[fx] = chg < g.n2 ? s[(int64_t)(fx * H + k) * bstride] : make_float2(0.f, 0.f);
+        for (int fx = 0; fx < T; ++fx) {
+            const float2 v = s[fx * fstep];
+            w[fx] = cin ? v : make_float2(0.f, 0.f);
+        }
         if constexpr (SUM) {
             for (int q = 1; q < nsum; ++q) {   // partial sums of a split GEMM, in a fixed order
                 const float2 *sq = s + q * pstride;
 #pragma unroll
                 for (int fx = 0; fx < T; ++fx) {
-                    const float2 v = chg < g.n2 ? sq[(int64_t)(fx * H + k) * bstride] : make_float2(0.f, 0.f);
-                    w[fx].x += v.x;
-                    w[fx].y += v.y;
+                    const float2 v = sq[fx * fstep];
+                    w[fx].x += cin ? v.x : 0.f;
+                    w[fx].y += cin ? v.y : 0.f;
                 }
             }
         }
-        reg_fft<T>(w, tw, 1.0f);
+        reg_fft<T, 1>(w);
 #pragma unroll
         for (int x = 0; x < T; ++x) {
             const int xr = x - g.off;
@@ -260,7 +311,7 @@ __global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec,
         }
     }
     __syncthreads();
-    for (int it = threadIdx.x; it < NPAIR * B; it += blockDim.x) {
+    for (int it = threadIdx.x; it < NPAIR * B; it += NT) {
         const int pair = it % NPAIR, xr = it / NPAIR;
         const int Px = tx * B + xr;
         const int ch = ch0 + 2 * pair;
@@ -280,7 +331,7 @@ __global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec,
             }
             z[fy] = make_float2(a.x - b.y, a.y + b.x);
         }
-        reg_fft<T>(z, tw, 1.0f);
+        reg_fft<T, 1>(z);
         const bool two = ch + 1 < g.n2;
         const float qa = scale * (qscale ? qscale[slot * g.n2 + ch] : 1.0f);
         const float qb = two ? scale * (qscale ? qscale[slot * g.n2 + ch + 1] : 1.0f) : 0.f;
@@ -290,8 +341,17 @@ __global__ __launch_bounds__(256) void k_tf_out(const float2 *__restrict__ spec,
             const int yr = y - g.off, Py = ty * B + yr;
             if (yr >= 0 && yr < B && Py < g.Ly) {
                 float *q = o + (int64_t)Py * g.S;
-                if (two) st_f2_a4(q, make_float2(z[y].x * qa, z[y].y * qb));
-                else q[0] = z[y].x * qa;
+                if (two) {
+                    float2 v = make_float2(z[y].x * qa, z[y].y * qb);
+                    if (acc) {
+                        const float2 u = ld_f2_a4(q);
+                        v.x += u.x;
+                        v.y += u.y;
+                    }
+                    st_f2_a4(q, v);
+                } else {
+                    q[0] = acc ? q[0] + z[y].x * qa : z[y].x * qa;
+                }
             }
         }
     }
@@ -375,7 +435,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
                                                     int ntp, int KS, int kB, int RB, int chp, int nsplit, int chunk,
                                                     int64_t pstride, float2 *__restrict__ out)
 {
-    constexpr int D = kTfRing;
+    constexpr int D = NTB >= 4 ? 2 : kTfRing;   // K steps in flight (four column blocks leave room for two)
     constexpr int U = D;   // steps per loop trip
     static_assert(U % 2 == 0 && kTfKsB % U == 0, "LDS buffers repeat within a trip, slots end at trip ends");
     __shared__ tu32x4 bop[2][4][4][64];   // [buffer][operand rh, rl, ih, il][tile block][lane]
@@ -638,59 +698,65 @@ static int tf_set_lds()
 }
 
 template <int T>
-static int tf_spectra(sid_psf *p, hipStream_t st, float2 *X1, float2 *Gt)
+static int tf_spectra(sid_psf *p, const TfGroup &g, hipStream_t st, float2 *X1, float2 *Gt)
 {
     FftState &f = p->plan.fft;
-    const int n = (int)p->n, n2 = n * n, P = (int)f.planes.size();
-    const int RB = f.chp / 16, KSf = f.kpad / 16;
-    for (int sl = 0; sl < P; ++sl) {
-        const Plane &pl = p->planes[f.planes[sl]];
-        hipLaunchKernelGGL(k_tf_g1<T>, dim3(n2 * f.tK), dim3(256), 0, st, p->H, p->M, n, f.planes[sl], pl.lo, pl.hi,
-                           pl.off_f, f.tDlo, f.tK, f.chp, f.ft.tw_fwd, X1);
+    const int n = (int)p->n, n2 = n * n;
+    const int RB = f.chp / 16, KSf = g.kpad / 16;
+    const float *qs = f.qscale + (int64_t)g.s0 * n2;
+    for (int sl = 0; sl < g.ns; ++sl) {
+        const int c = f.planes[g.s0 + sl];
+        const Plane &pl = p->planes[c];
+        hipLaunchKernelGGL(k_tf_g1<T>, dim3(n2 * g.K), dim3(256), 0, st, p->H, p->M, n, c, pl.lo, pl.hi, pl.off_f,
+                           g.Dlo, g.K, f.chp, g.ft.tw_fwd, X1);
         SID_TRY_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_tf_g2<T>, dim3(n2 * f.tH), dim3(256), 0, st, X1, n2, f.tK, f.chp, f.ft.tw_fwd, Gt);
+        hipLaunchKernelGGL(k_tf_g2<T>, dim3(n2 * g.H), dim3(256), 0, st, X1, n2, g.K, f.chp, g.ft.tw_fwd, Gt);
         SID_TRY_HIP(hipGetLastError());
-        const int64_t tf_ = (int64_t)f.nbin * (f.kslot / 4) * f.chp, tb_ = (int64_t)f.nbin * (kTfKsB * 4) * f.chp;
+        const int64_t tf_ = (int64_t)g.nbin * (f.kslot / 4) * f.chp, tb_ = (int64_t)g.nbin * (kTfKsB * 4) * f.chp;
         hipLaunchKernelGGL(k_tf_sf, dim3((unsigned)std::min<int64_t>(65536, (tf_ + 255) / 256)), dim3(256), 0, st, Gt,
-                           f.qscale, f.nbin, n2, f.chp, f.kslot, sl, KSf, RB, (tu32x4 *)f.G);
+                           qs, g.nbin, n2, f.chp, f.kslot, sl, KSf, RB, (tu32x4 *)g.G);
         SID_TRY_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_tf_sb, dim3((unsigned)std::min<int64_t>(65536, (tb_ + 255) / 256)), dim3(256), 0, st, Gt,
-                           f.qscale, f.nbin, n2, f.chp, P, sl, RB, (tu32x4 *)f.Gb);
+                           qs, g.nbin, n2, f.chp, g.ns, sl, RB, (tu32x4 *)g.Gb);
         SID_TRY_HIP(hipGetLastError());
     }
     return SID_OK;
 }
 
 template <int T>
-static int tf_dispatch_build(sid_psf *p, hipStream_t st, float2 *X1, float2 *Gt)
+static int tf_dispatch_build(sid_psf *p, const TfGroup &g, hipStream_t st, float2 *X1, float2 *Gt)
 {
     SID_TRY(tf_set_lds<T>());
-    return tf_spectra<T>(p, st, X1, Gt);
+    return tf_spectra<T>(p, g, st, X1, Gt);
 }
 
-static int tf_build_for(int T, sid_psf *p, hipStream_t st, float2 *X1, float2 *Gt)
+static int tf_build_for(sid_psf *p, const TfGroup &g, hipStream_t st, float2 *X1, float2 *Gt)
 {
-    switch (T) {
-    case 16: return tf_dispatch_build<16>(p, st, X1, Gt);
-    case 24: return tf_dispatch_build<24>(p, st, X1, Gt);
-    case 32: return tf_dispatch_build<32>(p, st, X1, Gt);
-    case 36: return tf_dispatch_build<36>(p, st, X1, Gt);
-    case 40: return tf_dispatch_build<40>(p, st, X1, Gt);
-    case 48: return tf_dispatch_build<48>(p, st, X1, Gt);
-    case 64: return tf_dispatch_build<64>(p, st, X1, Gt);
-    default: set_error("internal: no tiled transform of length %d", T); return SID_ERR_ARG;
+    switch (g.T) {
+    case 16: return tf_dispatch_build<16>(p, g, st, X1, Gt);
+    case 24: return tf_dispatch_build<24>(p, g, st, X1, Gt);
+    case 32: return tf_dispatch_build<32>(p, g, st, X1, Gt);
+    case 36: return tf_dispatch_build<36>(p, g, st, X1, Gt);
+    case 40: return tf_dispatch_build<40>(p, g, st, X1, Gt);
+    case 48: return tf_dispatch_build<48>(p, g, st, X1, Gt);
+    case 64: return tf_dispatch_build<64>(p, g, st, X1, Gt);
+    default: set_error("internal: no tiled transform of length %d", g.T); return SID_ERR_ARG;
     }
 }
 
-// Tile length: the smallest modelled time of one direction, max(HBM bytes /
-// 6 TB/s, MFMA flops / 1.2 PFLOP/s) with the spectra (FP16), the input
-// spectra written and read (FP32) and the MFMA work over the padded tiles.
+// Modelled time of one group of P planes with common tap width K at
+// transform length T, forward plus backward, in seconds: each GEMM the larger
+// of its HBM bytes (spectra once; the input spectra or the backward result
+// written and read) at 4.8 TB/s and its MFMA work at 1.2 PFLOP/s, the window
+// transforms in proportion to planes x tiles x T^2 log2 T (6.8e-10 s per unit
+// at T = 36, measured at 1005^2 x 51), and 40 us of launches.  The rates are
+// the ones the kernels reach on MI355X; only their ratios steer the choice.
 struct TfChoice {
     int T = 0, B = 0, ntx = 0, nty = 0;
     double cost = 0;
 };
 
-static TfChoice tf_choose(int K, int Lx, int Ly, int nslot, int chp, int kslot)
+static TfChoice tf_choose(int K, int Lx, int Ly, int P, int chp, int kslot)
 {
     TfChoice best;
     const int force = std::getenv("SID_TF_T") ? std::atoi(std::getenv("SID_TF_T")) : 0;
@@ -701,10 +767,14 @@ static TfChoice tf_choose(int K, int Lx, int Ly, int nslot, int chp, int kslot)
         const int ntx = (Lx + B - 1) / B, nty = (Ly + B - 1) / B, ntile = ntx * nty;
         const int ntb = (ntile + 15) / 16;
         if (ntb > 4) continue;
-        const double nbin = (double)T * (T / 2 + 1);
-        const double bytes = nbin * nslot * ((double)kslot * chp * 4 + 2.0 * ntile * kslot * 8);
-        const double flops = nbin * 16.0 * ntb * chp * (double)kslot * nslot * 16;
-        const double cost = std::max(bytes / 6e12, flops / 1.2e15);
+        const double nbin = (double)T * (T / 2 + 1), rows = chp;
+        const double kf = (double)P * kslot, kb = (double)P * kTfKsB * 16;
+        const double bytes_f = nbin * (rows * kf * 4 + 2.0 * ntile * kf * 8);
+        const double bytes_b = nbin * (rows * kb * 4 + 2.0 * ntb * 16 * rows * P * 8);
+        const double flops_f = nbin * 16.0 * ntb * rows * kf * 16, flops_b = nbin * 16.0 * ntb * rows * kb * 16;
+        const double xform = 6.8e-10 * P * ntile * (double)T * T * std::log2((double)T) / std::log2(36.0);
+        const double cost = std::max(bytes_f / 4.8e12, flops_f / 1.2e15) + std::max(bytes_b / 4.8e12, flops_b / 1.2e15) +
+                            xform + 40e-6;
         if (best.T == 0 || cost < best.cost) best = TfChoice{T, B, ntx, nty, cost};
     }
     return best;
@@ -712,112 +782,167 @@ static TfChoice tf_choose(int K, int Lx, int Ly, int nslot, int chp, int kslot)
 
 void tf_free(FftState &f)
 {
-    fft_plan_free(f.ft);
-    if (f.tiled && f.G) {   // the forward spectra of the tiled form
-        (void)hipFree(f.G);
-        f.G = nullptr;
+    for (TfGroup &g : f.tg) {
+        fft_plan_free(g.ft);
+        for (void *q : {g.G, g.Gb, (void *)g.vin, (void *)g.zin, (void *)g.vmax, (void *)g.tmax})
+            if (q) (void)hipFree(q);
     }
-    for (void *q : {(void *)f.Gb, (void *)f.vin, (void *)f.zin, (void *)f.yout, (void *)f.bout, (void *)f.vmax,
-                    (void *)f.tmax, (void *)f.tout})
+    f.tg.clear();
+    for (void *q : {(void *)f.yout, (void *)f.bout, (void *)f.tout})
         if (q) (void)hipFree(q);
-    f.Gb = nullptr;
-    f.vin = f.zin = f.yout = f.bout = nullptr;
-    f.vmax = f.tmax = f.tout = nullptr;
+    f.yout = f.bout = nullptr;
+    f.tout = nullptr;
     f.tiled = false;
 }
 
-// Geometry, buffers and spectra of the tiled form for the planes in f.planes
-// (f.qscale, f.dfft and the plane tables are already built).
+// Work splits of a group's two GEMMs: bins x parts dealt over the resident
+// blocks (two per CU); the cost of a split is its rounds of blocks times the
+// steps of one block, ties to fewer parts.  SID_TF_SPLIT_F / SID_TF_SPLIT_B
+// force the number of parts (measurement).
+static void tf_splits(TfGroup &g, int cus)
+{
+    const int64_t res = 2 * (int64_t)cus;
+    auto pick = [&](int units, int gran, int maxs, const char *env, int &nsplit, int &chunk) {
+        int force = 0;
+        if (const char *e = std::getenv(env)) force = std::max(1, std::atoi(e));
+        double best = 0;
+        nsplit = 1;
+        chunk = units;
+        for (int ns = 1; ns <= maxs; ++ns) {
+            if (force && ns != force) continue;
+            const int ch = ((units + ns - 1) / ns + gran - 1) / gran * gran;
+            const int parts = (units + ch - 1) / ch;
+            const double cost = (double)((g.nbin * (int64_t)parts + res - 1) / res) * ch;
+            if (best == 0 || cost < best * 0.999) {
+                best = cost;
+                nsplit = parts;
+                chunk = ch;
+            }
+        }
+    };
+    pick(g.kpad / 16, kTfRing, 8, "SID_TF_SPLIT_F", g.tsplit_f, g.tchunk_f);
+    pick(g.ns, 1, std::min(g.ns, 8), "SID_TF_SPLIT_B", g.tsplit_b, g.tchunk_b);
+}
+
+// Geometry, buffers and spectra of the tiled form for the planes in f.planes,
+// which fft_state_build orders by ascending tap width (f.qscale, f.dfft and the
+// plane tables are already built).  The slots are cut into at most
+// SID_TF_GROUPS (default 6) contiguous groups by dynamic programming over the
+// modelled group times: a narrow plane in a group of wide ones pays for the
+// wide group's transform length, a group of its own pays launches.
 int tf_build(sid_psf *p, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
     const int n = (int)p->n, n2 = n * n, P = (int)f.planes.size();
     if (n2 > 256 || P == 0) return kTfNotApplicable;
-    int Dlo = INT_MAX, Dhi = INT_MIN;
-    for (int c : f.planes) {
-        Dlo = std::min(Dlo, p->planes[c].tf_lo);
-        Dhi = std::max(Dhi, p->planes[c].tf_hi);
-    }
-    const int K = Dhi - Dlo + 1;
     const int chp = (n2 + 15) / 16 * 16, kslot = (n2 + 3) / 4 * 4;
-    const TfChoice tc = tf_choose(K, (int)sp.Lx, (int)sp.Ly, P, chp, kslot);
-    if (tc.T == 0) return kTfNotApplicable;
-    f.tT = tc.T;
-    f.tH = tc.T / 2 + 1;
-    f.tB = tc.B;
-    f.tDlo = Dlo;
-    f.tDhi = Dhi;
-    f.tK = K;
-    f.ntx = tc.ntx;
-    f.nty = tc.nty;
-    f.ntile = tc.ntx * tc.nty;
-    f.nbin = f.tT * f.tH;
-    f.chp = chp;
-    f.kslot = kslot;
-    f.kpad = (P * kslot + 16 * kTfRing - 1) / (16 * kTfRing) * (16 * kTfRing);   // whole ring trips
-    f.ngrp = (n2 + kTfCG - 1) / kTfCG;
-    f.tiled = true;
-    SID_TRY(fft_plan_make(f.ft, f.tT));
-    const int RB = chp / 16, KSf = f.kpad / 16;
-    const int64_t gf = (int64_t)f.nbin * KSf * RB * 64 * 16, gb = (int64_t)f.nbin * P * kTfKsB * RB * 64 * 16;
-    SID_TRY_HIP(hipMalloc(&f.G, gf));
-    SID_TRY_HIP(hipMalloc(&f.Gb, gb));
-    SID_TRY_HIP(hipMemsetAsync(f.G, 0, gf, st));
-    SID_TRY_HIP(hipMemsetAsync(f.Gb, 0, gb, st));
-    f.g_bytes = gf + gb;
-    const int ntp = (f.ntile + 15) / 16 * 16;
-    const int64_t nvin = (int64_t)f.nbin * f.ntile * f.kpad, nzin = (int64_t)f.nbin * f.ntile * kTfKsB * 16;
-    const int64_t nout = (int64_t)f.nbin * ntp * chp;
-    {
-        // Work splits of the two GEMMs: bins x parts dealt over the resident
-        // blocks (two per CU); the cost of a split is its rounds of blocks
-        // times the steps of one block, ties to fewer parts.  SID_TF_SPLIT_F /
-        // SID_TF_SPLIT_B force the number of parts (measurement).
-        int cus = 256;
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) cus = prop.multiProcessorCount;
-        const int64_t res = 2 * (int64_t)cus;
-        auto pick = [&](int units, int gran, int maxs, const char *env, int &nsplit, int &chunk) {
-            int force = 0;
-            if (const char *e = std::getenv(env)) force = std::max(1, std::atoi(e));
-            double best = 0;
-            nsplit = 1;
-            chunk = units;
-            for (int ns = 1; ns <= maxs; ++ns) {
-                if (force && ns != force) continue;
-                const int ch = ((units + ns - 1) / ns + gran - 1) / gran * gran;
-                const int parts = (units + ch - 1) / ch;
-                const double cost = (double)((f.nbin * (int64_t)parts + res - 1) / res) * ch;
-                if (best == 0 || cost < best * 0.999) {
-                    best = cost;
-                    nsplit = parts;
-                    chunk = ch;
+    const int Lx = (int)sp.Lx, Ly = (int)sp.Ly;
+    int maxg = 6;
+    if (const char *e = std::getenv("SID_TF_GROUPS")) maxg = std::max(1, std::atoi(e));
+    // groups of slots [i, j): common tap range and modelled cost
+    auto range = [&](int i, int j, int &lo, int &hi) {
+        lo = INT_MAX;
+        hi = INT_MIN;
+        for (int q = i; q < j; ++q) {
+            lo = std::min(lo, p->planes[f.planes[q]].tf_lo);
+            hi = std::max(hi, p->planes[f.planes[q]].tf_hi);
+        }
+    };
+    std::vector<std::vector<double>> gc(P + 1, std::vector<double>(P + 1, -1.0));
+    for (int i = 0; i < P; ++i)
+        for (int j = i + 1; j <= P; ++j) {
+            int lo, hi;
+            range(i, j, lo, hi);
+            const TfChoice tc = tf_choose(hi - lo + 1, Lx, Ly, j - i, chp, kslot);
+            gc[i][j] = tc.T ? tc.cost : -1.0;
+        }
+    // best[g][i]: cheapest cut of slots [i, P) into at most g groups
+    const double inf = 1e30;
+    std::vector<std::vector<double>> best(maxg + 1, std::vector<double>(P + 1, inf));
+    std::vector<std::vector<int>> cut(maxg + 1, std::vector<int>(P + 1, -1));
+    for (int g = 0; g <= maxg; ++g) best[g][P] = 0.0;
+    for (int g = 1; g <= maxg; ++g)
+        for (int i = P - 1; i >= 0; --i)
+            for (int j = i + 1; j <= P; ++j) {
+                if (gc[i][j] < 0 || best[g - 1][j] >= inf) continue;
+                const double c = gc[i][j] + best[g - 1][j];
+                if (c < best[g][i]) {
+                    best[g][i] = c;
+                    cut[g][i] = j;
                 }
             }
-        };
-        pick(f.kpad / 16, kTfRing, 8, "SID_TF_SPLIT_F", f.tsplit_f, f.tchunk_f);
-        pick(P, 1, std::min(P, 8), "SID_TF_SPLIT_B", f.tsplit_b, f.tchunk_b);
+    if (best[maxg][0] >= inf) return kTfNotApplicable;
+    int cus = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) cus = prop.multiProcessorCount;
     }
-    SID_TRY_HIP(hipMalloc(&f.vin, sizeof(float2) * nvin));
-    SID_TRY_HIP(hipMemsetAsync(f.vin, 0, sizeof(float2) * nvin, st));   // K padding stays zero
-    SID_TRY_HIP(hipMalloc(&f.zin, sizeof(float2) * nzin));
-    SID_TRY_HIP(hipMemsetAsync(f.zin, 0, sizeof(float2) * nzin, st));
-    SID_TRY_HIP(hipMalloc(&f.yout, sizeof(float2) * nout * f.tsplit_f));
-    SID_TRY_HIP(hipMalloc(&f.bout, sizeof(float2) * nout * P));
-    SID_TRY_HIP(hipMalloc(&f.vmax, sizeof(float) * (int64_t)f.ntile * P * f.ngrp));
-    SID_TRY_HIP(hipMalloc(&f.tmax, sizeof(float) * f.ntile));
+    f.chp = chp;
+    f.kslot = kslot;
+    f.ngrp = (n2 + kTfCG - 1) / kTfCG;
+    f.tiled = true;
+    int64_t nyout = 0, nbout = 0;
+    for (int g = maxg, i = 0; i < P; i = cut[g][i], --g) {
+        TfGroup t;
+        t.s0 = i;
+        t.ns = cut[g][i] - i;
+        range(i, cut[g][i], t.Dlo, t.Dhi);
+        t.K = t.Dhi - t.Dlo + 1;
+        const TfChoice tc = tf_choose(t.K, Lx, Ly, t.ns, chp, kslot);
+        t.T = tc.T;
+        t.H = tc.T / 2 + 1;
+        t.B = tc.B;
+        t.ntx = tc.ntx;
+        t.nty = tc.nty;
+        t.ntile = tc.ntx * tc.nty;
+        t.nbin = t.T * t.H;
+        t.kpad = (t.ns * kslot + 16 * kTfRing - 1) / (16 * kTfRing) * (16 * kTfRing);   // whole ring trips
+        tf_splits(t, cus);
+        f.tg.push_back(t);
+        const int64_t ntp = (t.ntile + 15) / 16 * 16;
+        nyout = std::max<int64_t>(nyout, (int64_t)t.nbin * ntp * chp * t.tsplit_f);
+        nbout = std::max<int64_t>(nbout, (int64_t)t.nbin * ntp * chp * t.ns);
+    }
+    SID_TRY_HIP(hipMalloc(&f.yout, sizeof(float2) * nyout));
+    SID_TRY_HIP(hipMalloc(&f.bout, sizeof(float2) * nbout));
     SID_TRY_HIP(hipMalloc(&f.tout, sizeof(float) * sp.Lx * sp.Ly * p->NP));
     SID_TRY_HIP(hipMemsetAsync(f.tout, 0, sizeof(float) * sp.Lx * sp.Ly * p->NP, st));
+    f.g_bytes = 0;
+    int64_t x1_elems = 0, gt_elems = 0;
+    const int RB = chp / 16;
+    for (TfGroup &t : f.tg) {
+        SID_TRY(fft_plan_make(t.ft, t.T));
+        const int64_t gf = (int64_t)t.nbin * (t.kpad / 16) * RB * 64 * 16,
+                      gb = (int64_t)t.nbin * t.ns * kTfKsB * RB * 64 * 16;
+        SID_TRY_HIP(hipMalloc(&t.G, gf));
+        SID_TRY_HIP(hipMalloc(&t.Gb, gb));
+        SID_TRY_HIP(hipMemsetAsync(t.G, 0, gf, st));
+        SID_TRY_HIP(hipMemsetAsync(t.Gb, 0, gb, st));
+        f.g_bytes += gf + gb;
+        const int64_t nvin = (int64_t)t.nbin * t.ntile * t.kpad, nzin = (int64_t)t.nbin * t.ntile * kTfKsB * 16;
+        SID_TRY_HIP(hipMalloc(&t.vin, sizeof(float2) * nvin));
+        SID_TRY_HIP(hipMemsetAsync(t.vin, 0, sizeof(float2) * nvin, st));   // K padding stays zero
+        SID_TRY_HIP(hipMalloc(&t.zin, sizeof(float2) * nzin));
+        SID_TRY_HIP(hipMemsetAsync(t.zin, 0, sizeof(float2) * nzin, st));   // uv padding stays zero
+        SID_TRY_HIP(hipMalloc(&t.vmax, sizeof(float) * (int64_t)t.ntile * t.ns * f.ngrp));
+        SID_TRY_HIP(hipMalloc(&t.tmax, sizeof(float) * t.ntile));
+        x1_elems = std::max<int64_t>(x1_elems, (int64_t)n2 * t.K * t.H * chp);
+        gt_elems = std::max<int64_t>(gt_elems, (int64_t)t.nbin * n2 * chp);
+    }
     // spectra, slot by slot through two scratch buffers
     float2 *X1 = nullptr, *Gt = nullptr;
-    SID_TRY_HIP(hipMalloc(&X1, sizeof(float2) * (int64_t)n2 * K * f.tH * chp));
-    if (hipMalloc(&Gt, sizeof(float2) * (int64_t)f.nbin * n2 * chp) != hipSuccess) {
+    SID_TRY_HIP(hipMalloc(&X1, sizeof(float2) * x1_elems));
+    if (hipMalloc(&Gt, sizeof(float2) * gt_elems) != hipSuccess) {
         (void)hipFree(X1);
         set_error("hipMalloc of the tiled spectra scratch failed");
         return SID_ERR_OOM;
     }
-    int rc = tf_build_for(f.tT, p, st, X1, Gt);
+    int rc = SID_OK;
+    for (const TfGroup &t : f.tg) {
+        rc = tf_build_for(p, t, st, X1, Gt);
+        if (rc != SID_OK) break;
+    }
     const hipError_t e = hipStreamSynchronize(st);
     (void)hipFree(X1);
     (void)hipFree(Gt);
@@ -829,7 +954,8 @@ int tf_build(sid_psf *p, hipStream_t st)
     return SID_OK;
 }
 
-static TfGeom tf_geom(const sid_psf *p, const FftState &f)
+// Transform geometry of group t: nslot planes from slot s0 on.
+static TfGeom tf_geom(const sid_psf *p, const FftState &f, const TfGroup &t)
 {
     const SensorPlan &sp = p->plan;
     TfGeom g;
@@ -837,11 +963,11 @@ static TfGeom tf_geom(const sid_psf *p, const FftState &f)
     g.chp = f.chp;
     g.Lx = (int)sp.Lx;
     g.Ly = (int)sp.Ly;
-    g.B = f.tB;
-    g.nty = f.nty;
-    g.ntile = f.ntile;
-    g.ntp = (f.ntile + 15) / 16 * 16;
-    g.nslot = (int)f.planes.size();
+    g.B = t.B;
+    g.nty = t.nty;
+    g.ntile = t.ntile;
+    g.ntp = (t.ntile + 15) / 16 * 16;
+    g.nslot = t.ns;
     g.ngrp = f.ngrp;
     g.nz_in = (int)sp.nz;
     g.pad = (int)p->T;
@@ -850,33 +976,33 @@ static TfGeom tf_geom(const sid_psf *p, const FftState &f)
     g.YP = sp.YP;
     g.off = 0;
     g.kslot = f.kslot;
-    g.kdim = f.kpad;
+    g.kdim = t.kpad;
     return g;
 }
 
 template <int T>
-static int tf_in(const float *src, const int *slot_plane, const float *qscale, const TfGeom &g, const float2 *tw,
-                 float2 *out, float *vmax, hipStream_t st)
+static int tf_in(const float *src, const int *slot_plane, const float *qscale, const TfGeom &g, float2 *out,
+                 float *vmax, hipStream_t st)
 {
     const size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
-    hipLaunchKernelGGL(k_tf_in<T>, dim3(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), dim3(256), lds, st, src, slot_plane,
-                       qscale, g, tw, out, vmax);
+    hipLaunchKernelGGL(k_tf_in<T>, dim3(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), dim3(tf_threads<T>()), lds, st,
+                       src, slot_plane, qscale, g, out, vmax);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
 template <int T>
 static int tf_out(const float2 *spec, int nsum, int64_t pstride, const int *slot_plane, const float *qscale,
-                  const TfGeom &g, const float2 *tw, float scale, float *dst, hipStream_t st)
+                  const TfGeom &g, float scale, int acc, float *dst, hipStream_t st)
 {
     const size_t lds = sizeof(float2) * kTfCG * g.B * (T / 2 + 1);
-    const dim3 grid(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp));
+    const dim3 grid(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), block(tf_threads<T>());
     if (nsum > 1)
-        hipLaunchKernelGGL((k_tf_out<T, true>), grid, dim3(256), lds, st, spec, nsum, pstride, slot_plane, qscale, g, tw,
-                           scale, dst);
+        hipLaunchKernelGGL((k_tf_out<T, true>), grid, block, lds, st, spec, nsum, pstride, slot_plane, qscale, g, scale,
+                           acc, dst);
     else
-        hipLaunchKernelGGL((k_tf_out<T, false>), grid, dim3(256), lds, st, spec, nsum, pstride, slot_plane, qscale, g,
-                           tw, scale, dst);
+        hipLaunchKernelGGL((k_tf_out<T, false>), grid, block, lds, st, spec, nsum, pstride, slot_plane, qscale, g, scale,
+                           acc, dst);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -895,32 +1021,32 @@ static int tf_out(const float2 *spec, int nsum, int64_t pstride, const int *slot
     }
 
 static int tf_in_any(int T, const float *src, const int *slot_plane, const float *qscale, const TfGeom &g,
-                     const float2 *tw, float2 *out, float *vmax, hipStream_t st)
+                     float2 *out, float *vmax, hipStream_t st)
 {
-#define SID_TF_IN(L) tf_in<L>(src, slot_plane, qscale, g, tw, out, vmax, st)
+#define SID_TF_IN(L) tf_in<L>(src, slot_plane, qscale, g, out, vmax, st)
     SID_TF_SWITCH(T, SID_TF_IN)
 #undef SID_TF_IN
 }
 
 static int tf_out_any(int T, const float2 *spec, int nsum, int64_t pstride, const int *slot_plane,
-                      const float *qscale, const TfGeom &g, const float2 *tw, float scale, float *dst, hipStream_t st)
+                      const float *qscale, const TfGeom &g, float scale, int acc, float *dst, hipStream_t st)
 {
-#define SID_TF_OUT(L) tf_out<L>(spec, nsum, pstride, slot_plane, qscale, g, tw, scale, dst, st)
+#define SID_TF_OUT(L) tf_out<L>(spec, nsum, pstride, slot_plane, qscale, g, scale, acc, dst, st)
     SID_TF_SWITCH(T, SID_TF_OUT)
 #undef SID_TF_OUT
 }
 
 template <bool BWD>
-static int tf_gemm(const FftState &f, const void *A, const float2 *Bm, int nslot, int KS, int kB, float2 *out,
-                   hipStream_t st)
+static int tf_gemm(const FftState &f, const TfGroup &t, const void *A, const float2 *Bm, int nslot, int KS, int kB,
+                   float2 *out, hipStream_t st)
 {
-    const int RB = f.chp / 16, ntb = (f.ntile + 15) / 16, ntp = ntb * 16;
-    const int nsplit = BWD ? f.tsplit_b : f.tsplit_f, chunk = BWD ? f.tchunk_b : f.tchunk_f;
-    const int64_t pstride = (int64_t)f.nbin * ntp * f.chp;
-    const dim3 grid(xcd_grid((int64_t)f.nbin * nsplit));
+    const int RB = f.chp / 16, ntb = (t.ntile + 15) / 16, ntp = ntb * 16;
+    const int nsplit = BWD ? t.tsplit_b : t.tsplit_f, chunk = BWD ? t.tchunk_b : t.tchunk_f;
+    const int64_t pstride = (int64_t)t.nbin * ntp * f.chp;
+    const dim3 grid(xcd_grid((int64_t)t.nbin * nsplit));
     const tu32x4 *a = (const tu32x4 *)A;
 #define SID_TF_GEMM(NT) \
-    hipLaunchKernelGGL((k_tf_gemm<NT, BWD>), grid, dim3(256), 0, st, a, Bm, f.tmax, f.nbin, nslot, f.ntile, ntp, KS, kB, \
+    hipLaunchKernelGGL((k_tf_gemm<NT, BWD>), grid, dim3(256), 0, st, a, Bm, t.tmax, t.nbin, nslot, t.ntile, ntp, KS, kB, \
                        RB, f.chp, nsplit, chunk, pstride, out)
     switch (ntb) {
     case 1: SID_TF_GEMM(1); break;
@@ -933,57 +1059,66 @@ static int tf_gemm(const FftState &f, const void *A, const float2 *Bm, int nslot
     return SID_OK;
 }
 
-static int tf_tmax(const FftState &f, int rowlen, hipStream_t st)
+static int tf_tmax(const TfGroup &t, int rowlen, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tf_tmax, dim3((unsigned)((f.ntile + 3) / 4)), dim3(256), 0, st, f.vmax, f.ntile, rowlen,
-                       f.tmax);
+    hipLaunchKernelGGL(k_tf_tmax, dim3((unsigned)((t.ntile + 3) / 4)), dim3(256), 0, st, t.vmax, t.ntile, rowlen,
+                       t.tmax);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
 
+// Forward image of the FFT planes into f.tout: group by group, the first
+// writing it and the others adding to it (a fixed order).
 int tf_forward(sid_psf *p, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
-    const int P = (int)f.planes.size();
-    TfGeom g = tf_geom(p, f);
-    g.off = -f.tDhi;
-    SID_TRY(tf_in_any(f.tT, sp.vol, f.d_slot_plane, f.qscale, g, f.ft.tw_fwd, f.vin, f.vmax, st));
-    SID_TRY(tf_tmax(f, P * f.ngrp, st));
-    fft_mark(f, st, 0, true);
-    SID_TRY(tf_gemm<false>(f, f.G, f.vin, 1, f.kpad / 16, f.kpad, f.yout, st));
-    fft_mark(f, st, 0, false);
-    TfGeom go = g;
-    go.nslot = 1;
-    go.off = f.tK - 1;
-    go.pad = 0;
-    go.S = (int)p->NP;
-    go.XP = sp.Lx;
-    go.YP = sp.Ly;
-    return tf_out_any(f.tT, f.yout, f.tsplit_f, (int64_t)f.nbin * go.ntp * f.chp, nullptr, nullptr, go, f.ft.tw_inv,
-                      1.0f / ((float)f.tT * (float)f.tT), f.tout, st);
+    const int n2 = (int)(p->n * p->n);
+    for (size_t gi = 0; gi < f.tg.size(); ++gi) {
+        const TfGroup &t = f.tg[gi];
+        TfGeom g = tf_geom(p, f, t);
+        g.off = -t.Dhi;
+        SID_TRY(tf_in_any(t.T, sp.vol, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, g, t.vin, t.vmax, st));
+        SID_TRY(tf_tmax(t, t.ns * f.ngrp, st));
+        fft_mark(f, st, 0, true);
+        SID_TRY(tf_gemm<false>(f, t, t.G, t.vin, 1, t.kpad / 16, t.kpad, f.yout, st));
+        fft_mark(f, st, 0, false);
+        TfGeom go = g;
+        go.nslot = 1;
+        go.off = t.K - 1;
+        go.pad = 0;
+        go.S = (int)p->NP;
+        go.XP = sp.Lx;
+        go.YP = sp.Ly;
+        SID_TRY(tf_out_any(t.T, f.yout, t.tsplit_f, (int64_t)t.nbin * go.ntp * f.chp, nullptr, nullptr, go,
+                           1.0f / ((float)t.T * (float)t.T), gi > 0, f.tout, st));
+    }
+    return SID_OK;
 }
 
+// Back-projection of the FFT planes into f.dfft (each group its own planes).
 int tf_backward(sid_psf *p, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
-    const int P = (int)f.planes.size();
-    TfGeom gi = tf_geom(p, f);
-    gi.nslot = 1;
-    gi.nz_in = 1;
-    gi.off = f.tDlo;
-    gi.kslot = 0;
-    gi.kdim = kTfKsB * 16;
-    SID_TRY(tf_in_any(f.tT, sp.img, nullptr, nullptr, gi, f.ft.tw_fwd, f.zin, f.vmax, st));
-    SID_TRY(tf_tmax(f, f.ngrp, st));
-    fft_mark(f, st, 1, true);
-    SID_TRY(tf_gemm<true>(f, f.Gb, f.zin, P, kTfKsB, kTfKsB * 16, f.bout, st));
-    fft_mark(f, st, 1, false);
-    TfGeom go = tf_geom(p, f);
-    go.off = 0;
-    return tf_out_any(f.tT, f.bout, 1, 0, f.d_slot_plane, f.qscale, go, f.ft.tw_inv,
-                      1.0f / ((float)f.tT * (float)f.tT), f.dfft, st);
+    const int n2 = (int)(p->n * p->n);
+    for (const TfGroup &t : f.tg) {
+        TfGeom gi = tf_geom(p, f, t);
+        gi.nslot = 1;
+        gi.nz_in = 1;
+        gi.off = t.Dlo;
+        gi.kslot = 0;
+        gi.kdim = kTfKsB * 16;
+        SID_TRY(tf_in_any(t.T, sp.img, nullptr, nullptr, gi, t.zin, t.vmax, st));
+        SID_TRY(tf_tmax(t, f.ngrp, st));
+        fft_mark(f, st, 1, true);
+        SID_TRY(tf_gemm<true>(f, t, t.Gb, t.zin, t.ns, kTfKsB, kTfKsB * 16, f.bout, st));
+        fft_mark(f, st, 1, false);
+        const TfGeom go = tf_geom(p, f, t);
+        SID_TRY(tf_out_any(t.T, f.bout, 1, 0, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, go,
+                           1.0f / ((float)t.T * (float)t.T), 0, f.dfft, st));
+    }
+    return SID_OK;
 }
 
 }  // namespace sid
