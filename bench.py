@@ -29,6 +29,7 @@ import numpy as np
 
 PEAK = {"fp32": 157.3e12, "bf16x2": 2.5e15, "bf16x3": 2.5e15, "fp64": 78.6e12}
 HBM_PEAK = 8.0e12
+PEAK_FP16 = 2.5e15    # dense FP16 MFMA (the tiled FFT path's spectra GEMMs)
 
 
 def log(*a):
@@ -334,6 +335,62 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     return out
 
 
+def pmc_summary(fetch_dir, write_dir, bench_out, dest):
+    """Summarise two rocprofv3 counter runs of the same bench command (FETCH_SIZE, then WRITE_SIZE, each with
+    --profile-copy) into the traffic summary the roofline reads: per kernel name the scaled bytes of its
+    last dispatches, and the tiled path's spectra GEMMs of the last forward + backward projection pair.
+    The scales come from the 1 GiB device copy (1 GiB read + 1 GiB written) in the same runs."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows += [r for r in csv.DictReader(open(f)) if r["Counter_Name"] == counter]
+        disp = {}
+        for r in rows:
+            k = int(r["Dispatch_Id"])
+            e = disp.setdefault(k, {"name": r["Kernel_Name"], "v": 0.0,
+                                    "ms": (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6})
+            e["v"] += float(r["Counter_Value"]) * 1024.0   # KiB per dispatch
+        return [disp[k] for k in sorted(disp)]
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy_f = [d for d in fetch if "copyBuffer" in d["name"]]
+    copy_w = [d for d in write if "copyBuffer" in d["name"]]
+    if not copy_f or not copy_w:
+        raise SystemExit("no 1 GiB copy dispatch in the counter runs (run the bench with --profile-copy)")
+    fs, ws = float(1 << 30) / copy_f[0]["v"], float(1 << 30) / copy_w[0]["v"]
+    line = [l for l in open(bench_out) if l.startswith("{")][-1]
+    res = json.loads(line)
+    head = res["config"]["method"]
+    hr = res["methods"][head]
+    kern = {}
+    for lst, key, sc in ((fetch, "read_bytes", fs), (write, "write_bytes", ws)):
+        names = {}
+        for d in lst:
+            names.setdefault(d["name"], []).append(d)
+        for nm, ds in names.items():
+            last = ds[-8:]
+            e = kern.setdefault(nm, {})
+            e[key] = sum(d["v"] for d in last) / len(last) * sc
+            e["ms"] = sum(d["ms"] for d in last) / len(last)
+    out = {"workload": res["config"]["workload"].split(":")[0], "precision": head, "method": res["config"]["projector"],
+           "spectra_gb": hr["spectra_gb"], "fetch_scale": fs, "write_scale": ws, "kernels": kern}
+    if hr.get("fft_tiled"):
+        ng = len(hr["fft_groups"])
+        gf = [d for d in fetch if "k_tf_gemm" in d["name"]][-2 * ng:]
+        gw = [d for d in write if "k_tf_gemm" in d["name"]][-2 * ng:]
+        out["fft_groups"] = hr["fft_groups"]
+        out["tf_gemm_read_bytes"] = [sum(d["v"] for d in gf[:ng]) * fs, sum(d["v"] for d in gf[ng:]) * fs]
+        out["tf_gemm_write_bytes"] = [sum(d["v"] for d in gw[:ng]) * ws, sum(d["v"] for d in gw[ng:]) * ws]
+        out["tf_gemm_bytes"] = sum(out["tf_gemm_read_bytes"]) + sum(out["tf_gemm_write_bytes"])
+        out["tf_gemm_algorithmic_bytes"] = [g * 1e9 for g in hr["tf_gemm_gb"]]
+        out["tf_gemm_ms"] = [sum(d["ms"] for d in gf[:ng]), sum(d["ms"] for d in gf[ng:])]
+    json.dump(out, open(dest, "w"), indent=1)
+    print(json.dumps({k: v for k, v in out.items() if k != "kernels"}, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,7 +420,14 @@ def main():
     ap.add_argument("--fft-planes", type=int, default=-1,
                     help="pin the K widest planes to the FFT path and the rest to the direct GEMM (reproduces an "
                          "autotuned split in profiler runs, whose serialised kernels would mislead the autotuner)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "BENCH_OUT"),
+                    help="summarise two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of one bench command and "
+                         "its output line into profiles/traffic_rl_full.json, then exit (no GPU use)")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                                    "traffic_rl_full.json"))
+        return args
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): the projector is measured on one "
               f"MI355X; multi-GPU sharding of this path is a design note, not a measured run.", file=sys.stderr)
@@ -455,10 +519,22 @@ def main():
                    planes_direct=sum(m == "direct" for m in meth), planes_fft=sum(m == "fft" for m in meth),
                    spectra_gb=inf["spectra_bytes"] / 1e9, autotune_ms=inf["autotune_ms"],
                    proj_per_s=2.0 / ((t_f + t_b + t_ff + t_fb + (tim[1] + tim[3]) / n_launch) * 1e-3))
-        if t_f + t_b > 0:
+        if t_f + t_b > 0 and flops_direct > 0:
             res["gemm_achieved_tflops"] = flops_direct / (0.5 * (t_f + t_b) * 1e-3) / 1e12
             res["gemm_frac_of_peak"] = res["gemm_achieved_tflops"] * 1e12 / PEAK[prec]
-        if t_gf + t_gb > 0:
+        res["fft_tiled"] = inf["fft_tile"][0] > 0
+        if t_gf + t_gb > 0 and res["fft_tiled"]:
+            # tiled FFT path: the spectra GEMMs of all plane groups, per projection (one launch per
+            # group); bytes they must move at least: FP16 spectra once, input spectra read, results written
+            gb_f, gb_b = inf["fft_gemm_bytes"]
+            fl_f, fl_b = inf["fft_gemm_flops"]
+            res["fft_groups"] = [list(g) for g in inf["fft_groups"]]
+            res["tf_gemm_gb"] = [gb_f / 1e9, gb_b / 1e9]
+            res["tf_gemm_stream_tbps"] = (gb_f + gb_b) / ((t_gf + t_gb) * 1e-3) / 1e12
+            res["tf_gemm_frac_of_hbm"] = res["tf_gemm_stream_tbps"] * 1e12 / HBM_PEAK
+            res["tf_gemm_mfma_tflops"] = (fl_f + fl_b) / ((t_gf + t_gb) * 1e-3) / 1e12
+            res["tf_gemm_mfma_frac"] = res["tf_gemm_mfma_tflops"] * 1e12 / PEAK_FP16
+        elif t_gf + t_gb > 0:
             # FFT path: each GEMV streams the spectra once (algorithmic bytes per launch)
             res["gemv_stream_tbps"] = inf["spectra_bytes"] / (0.5 * (t_gf + t_gb) * 1e-3) / 1e12
             res["gemv_frac_of_hbm"] = res["gemv_stream_tbps"] * 1e12 / HBM_PEAK
@@ -593,6 +669,25 @@ def main():
                 "unit": "TFLOP/s", "frac": round(hr["gemm_frac_of_peak"], 4), "traffic": traffic,
                 "kernel": "k_proj_gemm (forward and backward launches, direct planes)",
                 "algorithmic_per_launch": "2*X*Y*sum(M_c^2) over direct planes"}
+    elif hr.get("fft_tiled"):
+        # the spectra GEMMs of the tiled FFT path, all plane groups of one forward + one backward
+        # projection; the PMC traffic of the same GEMMs (counted in a run of the same plan)
+        traffic = None
+        if tj and tj.get("tf_gemm_bytes") and tj.get("fft_groups") == hr["fft_groups"]:
+            traffic = 0.5 * float(tj["tf_gemm_bytes"])
+        tb = sum(hr["tf_gemm_gb"]) * 1e9
+        roof = {"bound": "hbm", "achieved": round(hr["tf_gemm_stream_tbps"] * 1e3, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(hr["tf_gemm_frac_of_hbm"], 4), "traffic": traffic,
+                "traffic_source": "profiles/pmc_traffic_rl_full_bf16x2.md" if traffic else None,
+                "kernel": "k_tf_gemm (spectra GEMMs of the tiled FFT path, one launch per plane group and "
+                          "direction; FP16 MFMA)",
+                "algorithmic_per_launch": f"per projection {0.5 * tb / 1e9:.2f} GB (forward "
+                                          f"{hr['tf_gemm_gb'][0]:.2f} / backward {hr['tf_gemm_gb'][1]:.2f} GB: "
+                                          f"FP16 spectra once, input spectra read, results written)",
+                "ms_per_launch": round(0.5 * (hr["fwd_gemv_ms"] + hr["bwd_gemv_ms"]), 3),
+                "mfma_achieved_tflops": round(hr["tf_gemm_mfma_tflops"], 1),
+                "mfma_frac_of_fp16_peak": round(hr["tf_gemm_mfma_frac"], 4),
+                "plane_groups": hr["fft_groups"]}
     else:
         traffic = None
         if tj:
@@ -612,7 +707,7 @@ def main():
                 "algorithmic_per_launch": f"{hr['spectra_gb']:.2f} GB of spectra streamed once per launch"}
     # the MFMA half of the headline plan: the direct GEMM on its planes, beside the FFT path
     roof_mfma = None
-    if hr["fwd_gemm_ms"] + hr["bwd_gemm_ms"] > 0:
+    if hr["planes_direct"] > 0 and hr["fwd_gemm_ms"] + hr["bwd_gemm_ms"] > 0:
         prods = {"bf16x2": 2, "bf16x3": 3}.get(head, 1)
         t_launch = 0.5 * (hr["fwd_gemm_ms"] + hr["bwd_gemm_ms"]) * 1e-3
         alg = 2.0 * hr["gemm_macs_algorithmic"]
@@ -628,7 +723,15 @@ def main():
     fft_planes = hr["planes_fft"]
     dtype = head
     if head == "bf16x2":
-        dtype = "bf16x2+fp16-spectra" if fft_planes else "bf16x2"
+        # the reduced-precision split mode: direct planes multiply BF16 hi/lo volume values by the
+        # BF16 PSF; tiled FFT planes multiply FP16 hi/lo input spectra by FP16 spectra (FP32 accumulate)
+        tiled = hr.get("fft_tiled")
+        if fft_planes and not hr["planes_direct"]:
+            dtype = "fp16x2-split-mfma" if tiled else "fp16-spectra"
+        elif fft_planes:
+            dtype = "bf16x2-split-mfma+" + ("fp16x2-split-mfma" if tiled else "fp16-spectra")
+        else:
+            dtype = "bf16x2-split-mfma"
     out = {
         "metric": "RL iter/s @1005²×51z: BF16-split MFMA vs plain FP32; rel-L2 err vs FP64",
         "value": round(hr["iter_per_s"], 4),
@@ -646,7 +749,11 @@ def main():
                                             "bf16x3": "bf16 hi/lo split x bf16 hi/lo PSF, fp32 accumulate",
                                             "fp32": "fp32", "fp64": "fp64"}.get(head, head),
                       "fft_spectra": ("fp16 pairs, row-scaled" if head == "bf16x2" else "complex fp32")
-                      if fft_planes else None},
+                      if fft_planes else None,
+                      "fft_arithmetic": ("tiled: fp16 spectra x fp16 hi/lo input spectra on MFMA, fp32 accumulate"
+                                         if hr.get("fft_tiled") else "spectra GEMV, fp32 accumulate")
+                      if fft_planes else None,
+                      "fft_plane_groups": hr.get("fft_groups")},
         "data": "synthetic (seeded PSF, ball volume, Poisson image)",
         "config": {"workload": f"{wl.name}: {wl.X}x{wl.Y} px, Nnum={wl.n}, {wl.nz} planes z={wl.zs[0]}..{wl.zs[-1]}, "
                                f"M={wl.M}, {wl.n_balls} balls, {args.iters} ISRA iterations per step",

@@ -132,6 +132,11 @@ typedef struct {
     int32_t fft_groups;        /* plane groups of the tiled FFT path (0 when not tiled) */
     int32_t fft_group_planes[8]; /* planes per group, narrowest taps first (first 8 groups) */
     int32_t fft_group_T[8];    /* FFT length per group */
+    int64_t fft_gemm_bytes[2]; /* tiled path, all groups: bytes one forward / backward projection's spectra
+                                  GEMMs must move at least (FP16 spectra + input spectra read + results
+                                  written) */
+    double fft_gemm_flops[2];  /* the same GEMMs' MFMA work as issued (padded rows, tiles and K; FP16 hi
+                                  and lo products), flop */
 } sid_psf_info;
 int sid_psf_get_info(const sid_psf *psf, sid_psf_info *info);
 

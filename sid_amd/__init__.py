@@ -144,7 +144,8 @@ class PSF:
                     plane_method=[names.get(inf.plane_method[c], "?") for c in range(inf.Nz)],
                     autotune_ms=list(inf.autotune_ms), spectra_bytes=inf.spectra_bytes,
                     fft_tile=list(inf.fft_tile),
-                    fft_groups=[(inf.fft_group_planes[g], inf.fft_group_T[g]) for g in range(min(inf.fft_groups, 8))])
+                    fft_groups=[(inf.fft_group_planes[g], inf.fft_group_T[g]) for g in range(min(inf.fft_groups, 8))],
+                    fft_gemm_bytes=list(inf.fft_gemm_bytes), fft_gemm_flops=list(inf.fft_gemm_flops))
 
     def timing(self) -> list:
         buf = (ctypes.c_double * 12)()

@@ -87,7 +87,9 @@ class PsfInfo(ctypes.Structure):
                 ("fft_tile", ctypes.c_int32 * 4),
                 ("fft_groups", ctypes.c_int32),
                 ("fft_group_planes", ctypes.c_int32 * 8),
-                ("fft_group_T", ctypes.c_int32 * 8)]
+                ("fft_group_T", ctypes.c_int32 * 8),
+                ("fft_gemm_bytes", ctypes.c_int64 * 2),
+                ("fft_gemm_flops", ctypes.c_double * 2)]
 
 
 class LsOpts(ctypes.Structure):

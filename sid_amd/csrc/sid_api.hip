@@ -482,6 +482,7 @@ int sid_psf_get_info(const sid_psf *p, sid_psf_info *info)
             info->fft_group_planes[gi] = h ? f.tg[gi].ns : 0;
             info->fft_group_T[gi] = h ? f.tg[gi].T : 0;
         }
+        if (t) tf_gemm_work(f, info->fft_gemm_bytes, info->fft_gemm_flops);
     }
     return SID_OK;
 }
@@ -613,8 +614,9 @@ static double time_projection_pair(sid_psf *p, hipStream_t st, int *rc)
 // overlap.  autotune_methods prices each plane alone (direct GEMM time against
 // FFT-path time), which minimises the serial sum; overlapped, a projection
 // costs about the longer half plus their interference, which favours giving
-// the GEMM more planes.  Starting from that choice (k FFT planes) or, when it
-// is 1 % faster, from every eligible plane on the FFT path, time the
+// the GEMM more planes.  Starting from that choice (k FFT planes) or from
+// every eligible plane on the FFT path (when 1 % faster; in BF16X2 it is kept
+// unless 3 % slower, and then no other split is tried), time the
 // overlapped forward + backward pair with the narrowest FFT planes moved to the
 // GEMM s at a time (s about a ninth of k) while that gains 1 % over the
 // incumbent timed before and after it, else with s more FFT planes at a time
@@ -653,18 +655,27 @@ static int autotune(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t st
     int best_k = k0;
     double best_t = eval(k0);
     // every eligible plane on the FFT path (nothing beside it) is a candidate
-    // of its own: with the tiled form, narrow planes cost little there
+    // of its own: with the tiled form, narrow planes cost little there.  In
+    // the BF16X2 mode it is taken, and the search ends, unless it is 3 % slower
+    // than the incumbent (timed before and after it): its planes all carry FP16
+    // spectra, with a smaller error than the BF16-quantised PSF of direct
+    // planes.  Otherwise it must win by 1 %, like any other move.
+    bool settled = false;
     if (k0 < ne && rc == SID_OK) {
+        const bool pref = p->precision == SID_PREC_BF16X2;
+        const double lim = pref ? 1.03 : 0.99;
         const double t = eval(ne);
-        if (rc == SID_OK && t < 0.99 * best_t) {
+        if (rc == SID_OK && t < lim * best_t) {
             const double t_again = eval(best_k);
-            if (rc == SID_OK && t < 0.99 * t_again) {
+            if (rc == SID_OK && t < lim * t_again) {
                 best_t = t;
                 best_k = ne;
+                settled = pref;
             }
         }
     }
     for (int dir : {-s, s}) {
+        if (settled) break;
         bool moved = false;
         for (int k = best_k + dir; rc == SID_OK && k >= 0 && k <= ne; k += dir) {
             const double t = eval(k);

@@ -285,6 +285,8 @@ int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t s
 constexpr int kTfNotApplicable = 1000;
 int tf_build(sid_psf *p, hipStream_t st);
 void tf_free(FftState &f);
+// bytes (at least) and issued MFMA flop of one forward / backward projection's spectra GEMMs
+void tf_gemm_work(const FftState &f, int64_t bytes[2], double flops[2]);
 int tf_forward(sid_psf *p, hipStream_t st);    // FFT planes' forward image into f.tout
 int tf_backward(sid_psf *p, hipStream_t st);   // FFT planes' back-projection into f.dfft
 void fft_mark(FftState &f, hipStream_t st, int kind, bool begin);   // timing events (GEMV / GEMM)

@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
     constexpr int D = NTB >= 4 ? 2 : kTfRing;   // K steps in flight (four column blocks leave room for two)
     constexpr int U = D;   // steps per loop trip
     static_assert(U % 2 == 0 && kTfKsB % U == 0, "LDS buffers repeat within a trip, slots end at trip ends");
-    __shared__ tu32x4 bop[2][4][4][64];   // [buffer][operand rh, rl, ih, il][tile block][lane]
+    __shared__ tu32x4 bop[2][4][4][64];   // [buffer][operand rh, rl, ih, il][tile block][slot]
     const int item = xcd_item(nbin * nsplit);
     if (item < 0) return;
     const int bin = item / nsplit, part = item - bin * nsplit;
@@ -459,7 +459,9 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
     const float mt = tvalid ? tmax[tt] : 0.f;
     const float sc = mt > 0.f ? 1.0f / mt : 0.f;
     const float2 *bsrc = Bm + ((int64_t)bin * ntile + (tvalid ? tt : 0)) * kB + 4 * qt + (BWD ? 0 : (int64_t)s_lo * 16);
-    tu32x4 *bdst = &bop[0][0][tt >> 4][(tt & 15) + 16 * qt];
+    // 16-byte slot of (column c, k group g) in a tile block: 16 g + ((c + 4 g) & 15),
+    // so the 16 lanes of each quarter wave write and read 16 different bank groups
+    tu32x4 *bdst = &bop[0][0][tt >> 4][16 * qt + (((tt & 15) + 4 * qt) & 15)];
     const float4 zf = make_float4(0.f, 0.f, 0.f, 0.f);
     float mx[NTB];
 #pragma unroll
@@ -471,15 +473,20 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
     const tu32x4 *abase = A + ((int64_t)bin * nslot * KS + s_lo) * astep + lane;
     tu32x4 a[D][4];
     float4 b0[D], b1[D];
+    // the ring's first D steps, loaded in the loop's own order (B, then A):
+    // the loads pending at the loop's entry then match those pending on its
+    // back edge, so the compiler's waits at the top of a trip count the ring
+    // instead of draining every load in flight
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         const int s = min(r, nsteps - 1);
-        const tu32x4 *pa = abase + (int64_t)s * astep;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
         const float2 *pb = bsrc + (BWD ? (s & (kTfKsB - 1)) : s) * 16;
         b0[r] = *(const float4 *)pb;
         b1[r] = *(const float4 *)(pb + 2);
+        const tu32x4 *pa = abase + (int64_t)s * astep;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
+        __builtin_amdgcn_sched_barrier(0);   // keep that order
     }
     f4 cr[4][NTB], ci[4][NTB];
 #pragma unroll
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
                 b1[r] = *(const float4 *)(pb + 2);
             }
             __syncthreads();
-            const tu32x4 *bs = &bop[j & 1][0][0][lane];
+            const tu32x4 *bs = &bop[j & 1][0][0][16 * g + ((l16 + 4 * g) & 15)];
 #pragma unroll
             for (int tb = 0; tb < NTB; ++tb) {
                 const tu32x4 rh = bs[tb * 64], rl = bs[(4 + tb) * 64], ih = bs[(8 + tb) * 64], il = bs[(12 + tb) * 64];
@@ -760,6 +767,8 @@ static TfChoice tf_choose(int K, int Lx, int Ly, int P, int chp, int kslot)
 {
     TfChoice best;
     const int force = std::getenv("SID_TF_T") ? std::atoi(std::getenv("SID_TF_T")) : 0;
+    // SID_TF_GROUP_US replaces the 40 us of launches per group (tests: groups on small problems)
+    const double launch = std::getenv("SID_TF_GROUP_US") ? 1e-6 * std::atof(std::getenv("SID_TF_GROUP_US")) : 40e-6;
     for (int T : kTfLens) {
         if (force && T != force) continue;
         const int B = T - K + 1;
@@ -774,7 +783,7 @@ static TfChoice tf_choose(int K, int Lx, int Ly, int P, int chp, int kslot)
         const double flops_f = nbin * 16.0 * ntb * rows * kf * 16, flops_b = nbin * 16.0 * ntb * rows * kb * 16;
         const double xform = 6.8e-10 * P * ntile * (double)T * T * std::log2((double)T) / std::log2(36.0);
         const double cost = std::max(bytes_f / 4.8e12, flops_f / 1.2e15) + std::max(bytes_b / 4.8e12, flops_b / 1.2e15) +
-                            xform + 40e-6;
+                            xform + launch;
         if (best.T == 0 || cost < best.cost) best = TfChoice{T, B, ntx, nty, cost};
     }
     return best;
@@ -795,33 +804,26 @@ void tf_free(FftState &f)
     f.tiled = false;
 }
 
-// Work splits of a group's two GEMMs: bins x parts dealt over the resident
-// blocks (two per CU); the cost of a split is its rounds of blocks times the
-// steps of one block, ties to fewer parts.  SID_TF_SPLIT_F / SID_TF_SPLIT_B
-// force the number of parts (measurement).
+// Work splits of a group's two GEMMs.  One block per bin fills the GPU when
+// there are at least as many bins as resident blocks (two per CU); a smaller
+// group cuts each bin's work into parts so that bins x parts fills it once:
+// the forward into runs of K steps (partial sums, added in a fixed order by the
+// inverse transform), the backward into runs of slots.  More parts than that
+// gained nothing at 1005^2 x 51 (the GEMMs stream HBM; a last round of blocks
+// two-thirds empty runs faster per block) and the forward's partial sums cost
+// their traffic.  SID_TF_SPLIT_F / SID_TF_SPLIT_B force the number of parts.
 static void tf_splits(TfGroup &g, int cus)
 {
     const int64_t res = 2 * (int64_t)cus;
-    auto pick = [&](int units, int gran, int maxs, const char *env, int &nsplit, int &chunk) {
-        int force = 0;
-        if (const char *e = std::getenv(env)) force = std::max(1, std::atoi(e));
-        double best = 0;
-        nsplit = 1;
-        chunk = units;
-        for (int ns = 1; ns <= maxs; ++ns) {
-            if (force && ns != force) continue;
-            const int ch = ((units + ns - 1) / ns + gran - 1) / gran * gran;
-            const int parts = (units + ch - 1) / ch;
-            const double cost = (double)((g.nbin * (int64_t)parts + res - 1) / res) * ch;
-            if (best == 0 || cost < best * 0.999) {
-                best = cost;
-                nsplit = parts;
-                chunk = ch;
-            }
-        }
+    auto pick = [&](int units, int gran, const char *env, int &nsplit, int &chunk) {
+        int want = (int)((res + g.nbin - 1) / g.nbin);
+        if (const char *e = std::getenv(env)) want = std::max(1, std::atoi(e));
+        want = std::max(1, std::min(want, std::max(1, units / gran)));
+        chunk = ((units + want - 1) / want + gran - 1) / gran * gran;
+        nsplit = (units + chunk - 1) / chunk;
     };
-    pick(g.kpad / 16, kTfRing, 8, "SID_TF_SPLIT_F", g.tsplit_f, g.tchunk_f);
-    pick(g.ns, 1, std::min(g.ns, 8), "SID_TF_SPLIT_B", g.tsplit_b, g.tchunk_b);
+    pick(g.kpad / 16, kTfRing, "SID_TF_SPLIT_F", g.tsplit_f, g.tchunk_f);
+    pick(g.ns, 1, "SID_TF_SPLIT_B", g.tsplit_b, g.tchunk_b);
 }
 
 // Geometry, buffers and spectra of the tiled form for the planes in f.planes,
@@ -952,6 +954,25 @@ int tf_build(sid_psf *p, hipStream_t st)
         return SID_ERR_HIP;
     }
     return SID_OK;
+}
+
+void tf_gemm_work(const FftState &f, int64_t bytes[2], double flops[2])
+{
+    bytes[0] = bytes[1] = 0;
+    flops[0] = flops[1] = 0.0;
+    const int RB = f.chp / 16;
+    for (const TfGroup &t : f.tg) {
+        const int64_t ntb = (t.ntile + 15) / 16, ntp = 16 * ntb, rows = (int64_t)RB * 16;
+        const int64_t kf = (int64_t)t.kpad, kb = (int64_t)t.ns * kTfKsB * 16;   // complex K per direction
+        // spectra (FP16 complex, 4 B) once, input spectra (FP32 complex) once, results written
+        bytes[0] += t.nbin * (rows * kf * 4 + (int64_t)t.ntile * kf * 8 + t.tsplit_f * ntp * rows * 8);
+        bytes[1] += t.nbin * (rows * kb * 4 + (int64_t)t.ntile * kTfKsB * 16 * 8 + (int64_t)t.ns * ntp * rows * 8);
+        // per complex MAC eight real FP16 products (re and im of the result, two terms each, hi and
+        // lo), 2 flop each.  The four waves issue 16 row blocks (one repeats when RB = 15).
+        const double mf = (double)t.nbin * 16 * 16 * ntp * 16.0;
+        flops[0] += mf * kf;
+        flops[1] += mf * kb;
+    }
 }
 
 // Transform geometry of group t: nslot planes from slot s0 on.

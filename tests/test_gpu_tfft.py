@@ -176,3 +176,43 @@ def test_tiled_deterministic(env, monkeypatch):
     d = psf.backward(W)
     assert np.array_equal(a, b) and np.array_equal(c, d)
     psf.close()
+
+
+def test_tiled_plane_groups(env, monkeypatch):
+    """Planes of very different tap ranges split into groups, each with its own
+    transform length; forward, backward and ISRA against the oracle, and equal
+    (to the FP16 rounding) to the same planes in one group."""
+    torch, sid_amd = env
+    monkeypatch.delenv("SID_TF_T", raising=False)
+    monkeypatch.delenv("SID_TF_GROUPS", raising=False)
+    monkeypatch.setenv("SID_TF_GROUP_US", "0")   # groups pay no launch cost in the model
+    n, X, Y, M, Nz = 5, 200, 190, 61, 6
+    print("case: tiled plane groups", flush=True)
+    rng = np.random.default_rng(47)
+    H = random_psf(rng, M, n, Nz)
+    m = M // 2
+    radii = [2, 4, 9, 14, 22, 30]   # lenslet taps 3 .. 13 per axis
+    CA = np.array([[m + 1 - r, m + 1 + r] for r in radii], dtype=np.int32)
+    V = rng.random((X, Y, Nz))
+    W = rng.random((X, Y))
+    psf = sid_amd.PSF(H.astype(np.float32), CA, precision="bf16x2", method="fft")
+    O = psf.forward(V.astype(np.float32))
+    groups = psf.info()["fft_groups"]
+    print("groups (planes, T)", groups, flush=True)
+    assert len(groups) >= 2 and sum(g[0] for g in groups) == Nz
+    assert len(set(g[1] for g in groups)) >= 2
+    B = psf.backward(W.astype(np.float32))
+    assert rel_l2(O, c_forward(H, V, CA)) < TOL and rel_l2(B, c_backward(H, W, CA)) < TOL
+    htf = c_backward(H, c_forward(H, V, CA), CA)
+    got = psf.deconv_rl(htf.astype(np.float32), 3, htf.astype(np.float32))
+    e = rel_l2(got, c_isra(H, htf, htf, 3, CA))
+    print("isra rel-L2", e, flush=True)
+    assert e < 4 * TOL
+    psf.close()
+    monkeypatch.setenv("SID_TF_GROUPS", "1")
+    one = sid_amd.PSF(H.astype(np.float32), CA, precision="bf16x2", method="fft")
+    O1 = one.forward(V.astype(np.float32))
+    assert len(one.info()["fft_groups"]) == 1
+    B1 = one.backward(W.astype(np.float32))
+    assert rel_l2(O1, O) < TOL and rel_l2(B1, B) < TOL
+    one.close()
