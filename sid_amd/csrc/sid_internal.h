@@ -120,8 +120,8 @@ struct TfGroup {
     int tsplit_b = 1, tchunk_b = 0;    // backward GEMM: slot runs per bin and their length
     FftPlan1D ft;                      // twiddles of length T (spectra build)
     void *G = nullptr, *Gb = nullptr;  // forward / backward spectra, MFMA fragment order
-    float2 *vin = nullptr;             // forward input spectra [nbin][ntile][kpad] (K padding stays zero)
-    float2 *zin = nullptr;             // backward input (image) spectra [nbin][ntile][256] (uv padding zero)
+    float2 *vin = nullptr;             // forward input spectra [nbin][kpad / 16][ntile][16] (K padding stays zero)
+    float2 *zin = nullptr;             // backward input (image) spectra [nbin][16][ntile][16] (uv padding zero)
     float *vmax = nullptr;             // per tile and transform block: bound on its spectra
     float *tmax = nullptr;             // per tile: bound over all its blocks
 };

@@ -172,7 +172,7 @@ struct TfGeom {
 // channels are transformed in real pairs a + i b (Z = DFT(a + i b) gives
 // A(k) = (Z(k) + conj Z(-k)) / 2, B(k) = (Z(k) - conj Z(-k)) / (2i)): along y
 // in registers (one row per thread), the stored half spectrum through LDS,
-// then along x (one column per thread).  Output: out[bin][tile][k], scaled by
+// then along x (one column per thread).  Output: out[bin][k / 16][tile][k % 16], scaled by
 // s_q (qscale, forward) and vmax[tile][slot ngrp + grp] = max over the block's
 // channels of s_q T max_x sum_y |w[x][y]|, a bound on every |out| it wrote (at
 // least the window's L1 norm; a maximum, so independent of thread timing).
@@ -245,7 +245,8 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restri
         reg_fft<T, -1>(w);
         if (chg < g.n2) {
             const float q = qscale ? qscale[slot * g.n2 + chg] : 1.0f;
-            float2 *o = out + k * bstride + (int64_t)tile * g.kdim + slot * g.kslot + chg;
+            const int kk = slot * g.kslot + chg;   // GEMM column, stored by K steps of 16 (all tiles of a step together)
+            float2 *o = out + k * bstride + ((int64_t)(kk >> 4) * g.ntile + tile) * 16 + (kk & 15);
 #pragma unroll
             for (int fx = 0; fx < T; ++fx) o[fx * fstep] = make_float2(w[fx].x * q, w[fx].y * q);
         }
@@ -382,7 +383,8 @@ __global__ __launch_bounds__(256) void k_tf_tmax(const float *__restrict__ vmax,
 // A: FP16 (re, im) pairs in fragment order, the 16 bytes of lane (l16, g) of
 // flattened K step s (slot * KS + ks) and row block rb at A[((bin nslot KS + s)
 // RB + rb) 64 + lane]: row rb*16 + l16, complex k = ks*16 + 4g .. 4g + 3.
-// Bm: FP32 complex [bin][tile][kB], scaled per tile by 1/tmax and split into
+// Bm: FP32 complex [bin][kB / 16][tile][16] (a K step of every tile is one
+// contiguous run), scaled per tile by 1/tmax and split into
 // FP16 hi + lo; Re C = A . (Br, -Bi), Im C = A . (Bi, Br) over the interleaved
 // real K (v_mfma_f32_16x16x32_f16: lane (l16, g) feeds A[row l16][k 8g..8g+7],
 // B[k 8g..8g+7][col l16] and holds C[4g + i][l16]).
@@ -458,7 +460,8 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
     const bool tvalid = tt < ntile;
     const float mt = tvalid ? tmax[tt] : 0.f;
     const float sc = mt > 0.f ? 1.0f / mt : 0.f;
-    const float2 *bsrc = Bm + ((int64_t)bin * ntile + (tvalid ? tt : 0)) * kB + 4 * qt + (BWD ? 0 : (int64_t)s_lo * 16);
+    const int64_t bstep = (int64_t)ntile * 16;   // one K step of every tile
+    const float2 *bsrc = Bm + (int64_t)bin * ntile * kB + (tvalid ? tt : 0) * 16 + 4 * qt + (BWD ? 0 : s_lo * bstep);
     // 16-byte slot of (column c, k group g) in a tile block: 16 g + ((c + 4 g) & 15),
     // so the 16 lanes of each quarter wave write and read 16 different bank groups
     tu32x4 *bdst = &bop[0][0][tt >> 4][16 * qt + (((tt & 15) + 4 * qt) & 15)];
@@ -480,12 +483,12 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         const int s = min(r, nsteps - 1);
-        const float2 *pb = bsrc + (BWD ? (s & (kTfKsB - 1)) : s) * 16;
+        const float2 *pb = bsrc + (BWD ? (s & (kTfKsB - 1)) : s) * bstep;
         b0[r] = *(const float4 *)pb;
         b1[r] = *(const float4 *)(pb + 2);
         const tu32x4 *pa = abase + (int64_t)s * astep;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
+        for (int i = 0; i < 4; ++i) a[r][i] = __builtin_nontemporal_load(pa + rbw[i] * 64);
         __builtin_amdgcn_sched_barrier(0);   // keep that order
     }
     f4 cr[4][NTB], ci[4][NTB];
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
                 d[3 * 4 * 64] = il;
             }
             {
-                const float2 *pb = bsrc + (BWD ? (sn & (kTfKsB - 1)) : sn) * 16;
+                const float2 *pb = bsrc + (BWD ? (sn & (kTfKsB - 1)) : sn) * bstep;
                 b0[r] = *(const float4 *)pb;
                 b1[r] = *(const float4 *)(pb + 2);
             }
@@ -527,16 +530,17 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
             }
             const tu32x4 *pa = abase + (int64_t)sn * astep;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[r][i] = pa[rbw[i] * 64];
+            for (int i = 0; i < 4; ++i) a[r][i] = __builtin_nontemporal_load(pa + rbw[i] * 64);
             __builtin_amdgcn_sched_barrier(0);   // keep the steps apart (register pressure)
         }
         if (BWD && ((s0 + U) & (kTfKsB - 1)) == 0) {
-            // slot slot0 + s0 / 16 is complete: write it (repeated row blocks store the same values twice)
+            // slot slot0 + s0 / 16 is complete: write it
             float2 *ob = out + ((int64_t)bin * nslot + slot0 + s0 / kTfKsB) * ntp * chp + 4 * g;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int tb = 0; tb < NTB; ++tb) {
+                    if (wave + 4 * i >= RB) continue;   // a repeated row block (RB = 15)
                     float2 *o = ob + (int64_t)(tb * 16 + l16) * chp + rbw[i] * 16;
                     const float m = mx[tb];
                     *(float4 *)o = make_float4(cr[i][tb][0] * m, ci[i][tb][0] * m, cr[i][tb][1] * m, ci[i][tb][1] * m);
@@ -552,6 +556,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int tb = 0; tb < NTB; ++tb) {
+                if (wave + 4 * i >= RB) continue;   // a repeated row block (RB = 15)
                 float2 *o = ob + (int64_t)(tb * 16 + l16) * chp + rbw[i] * 16;
                 const float m = mx[tb];
                 *(float4 *)o = make_float4(cr[i][tb][0] * m, ci[i][tb][0] * m, cr[i][tb][1] * m, ci[i][tb][1] * m);
