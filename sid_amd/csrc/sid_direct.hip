@@ -1438,11 +1438,19 @@ int launch_gemm(sid_psf *p, int dir, hipStream_t st, int max_blocks)
     }
 }
 
+// The BF16 hi/lo shadows of the volume and the image feed only the direct BF16
+// GEMM: a plan with every nonempty plane on the FFT path writes none.
+static bool bf16_shadows(const sid_psf *p)
+{
+    if (p->precision != SID_PREC_BF16X2 && p->precision != SID_PREC_BF16X3) return false;
+    return !(p->plan.fft.active && p->plan.fft.all_fft);
+}
+
 int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long long check, unsigned long long mark)
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
-    const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
+    const bool sh = bf16_shadows(p);
     const int64_t total = sp.nrows * p->n * p->n;
     const int64_t stride = sp.mpad * p->NP;
     if (sp.fwd.nslabs == 0) {  // no contributing plane: the image is zero
@@ -1472,7 +1480,7 @@ int launch_bwd_reduce(sid_psf *p, float *B_col, int rl_update, hipStream_t st, u
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
-    const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
+    const bool sh = bf16_shadows(p);
     const int64_t stride = sp.mpad * p->NP;
     const int *planes = nullptr;
     int64_t np = p->nzH;
@@ -1506,7 +1514,7 @@ int launch_vol_in(sid_psf *p, const float *V_col, int64_t nz, int which, hipStre
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
-    const bool sh = which == 0 && (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3);
+    const bool sh = which == 0 && bf16_shadows(p);
     const int64_t total = sp.X * sp.Y * nz;
     if (total == 0) return SID_OK;
     const dim3 grid((unsigned)sp.Ly, (unsigned)((sp.Lx + kVinPx - 1) / kVinPx), (unsigned)nz);
@@ -1526,7 +1534,7 @@ int launch_img_in(sid_psf *p, const float *O_col, hipStream_t st, unsigned long 
 {
     SensorPlan &sp = p->plan;
     const bool f64 = p->precision == SID_PREC_FP64;
-    const bool sh = p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3;
+    const bool sh = bf16_shadows(p);
     const int64_t total = sp.X * sp.Y;
     if (f64)
         hipLaunchKernelGGL(k_vol_in<double>, dim3(grid_for(total)), dim3(256), 0, st, O_col, (int)sp.X, (int)sp.Y, 1,

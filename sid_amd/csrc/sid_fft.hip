@@ -1078,6 +1078,9 @@ int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st)
         for (int64_t c = 0; c < p->nzH; ++c)
             if (pf[c]) lst.push_back((int)c);
         f.n_list_fft = (int)lst.size() - f.n_list_direct;
+        f.all_fft = true;
+        for (int64_t c = 0; c < p->nzH; ++c)
+            if (!pf[c] && !p->planes[c].empty) f.all_fft = false;
         SID_TRY_HIP(hipMalloc(&f.d_plane_list, sizeof(int) * std::max<size_t>(lst.size(), 1)));
         SID_TRY_HIP(hipMemcpy(f.d_plane_list, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
     }

@@ -152,6 +152,7 @@ struct FftState {
     int *d_plane_fft = nullptr;    // per plane of H: 1 when on the FFT path
     int *d_plane_list = nullptr;   // planes of H on the direct GEMM, then those on the FFT path
     int n_list_direct = 0, n_list_fft = 0;
+    bool all_fft = false;           // every nonempty plane of H is on the FFT path
     int64_t g_bytes = 0;
     // Tiled form (BF16X2, sid_tfft.hip): the FFT planes in groups of similar
     // lenslet tap range (slots in ascending tap width); per group overlap-save

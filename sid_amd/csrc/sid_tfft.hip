@@ -540,7 +540,8 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int tb = 0; tb < NTB; ++tb) {
-                    if (wave + 4 * i >= RB) continue;   // a repeated row block (RB = 15)
+                    // a repeated row block (RB = 15), or a padding tile column: nothing to store
+                    if (wave + 4 * i >= RB || tb * 16 + l16 >= ntile) continue;
                     float2 *o = ob + (int64_t)(tb * 16 + l16) * chp + rbw[i] * 16;
                     const float m = mx[tb];
                     *(float4 *)o = make_float4(cr[i][tb][0] * m, ci[i][tb][0] * m, cr[i][tb][1] * m, ci[i][tb][1] * m);
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(256, 2) void k_tf_gemm(const tu32x4 *__restrict__ A
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int tb = 0; tb < NTB; ++tb) {
-                if (wave + 4 * i >= RB) continue;   // a repeated row block (RB = 15)
+                if (wave + 4 * i >= RB || tb * 16 + l16 >= ntile) continue;   // as in the backward
                 float2 *o = ob + (int64_t)(tb * 16 + l16) * chp + rbw[i] * 16;
                 const float m = mx[tb];
                 *(float4 *)o = make_float4(cr[i][tb][0] * m, ci[i][tb][0] * m, cr[i][tb][1] * m, ci[i][tb][1] * m);
