@@ -506,15 +506,6 @@ static int check_vdims(sid_psf *p, const int64_t vdims[3], bool rl)
     return SID_OK;
 }
 
-static int ensure_aux(sid_psf *p)
-{
-    if (p->aux) return SID_OK;
-    SID_TRY_HIP(hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking));
-    SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
-    SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
-    return SID_OK;
-}
-
 // The two halves of one projection (dir 0 forward, 1 backward): the direct
 // GEMM over the direct planes (MFMA-bound) and the FFT path over the others
 // (bound by the spectra stream from HBM).  They read the same staged volume or

@@ -638,7 +638,8 @@ __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t sla
                              int n, int Ly, int YP, int T_, int S, int X, int Y, T *__restrict__ img,
                              u16 *__restrict__ hi, u16 *__restrict__ lo, float *__restrict__ O_col,
                              const float2 *__restrict__ fft_img, int fft_ny, const float *__restrict__ tout,
-                             unsigned long long *__restrict__ nf, unsigned long long check, unsigned long long mark)
+                             const float *__restrict__ tout2, unsigned long long *__restrict__ nf,
+                             unsigned long long check, unsigned long long mark)
 {
     const int nn2 = n * n;
     const int64_t total = (int64_t)nrows * nn2;
@@ -654,6 +655,7 @@ __global__ void k_fwd_reduce(const T *__restrict__ part, int nslabs, int64_t sla
         const int x = Px * n + uv % n, y = Py * n + uv / n;
         if (fft_img) s += (T)fft_img[(int64_t)x * fft_ny + y].x;   // planes on the FFT path
         if (tout) s += (T)tout[(int64_t)r * NP + uv];                 // ... in the tiled form
+        if (tout2) s += (T)tout2[(int64_t)r * NP + uv];               // ... its second chain
         if (poison) s = (T)NAN;
         if (x >= X || y >= Y) s = 0;
         bad = bad || !isfinite(s);
@@ -1462,13 +1464,15 @@ int launch_fwd_reduce(sid_psf *p, float *O_col, hipStream_t st, unsigned long lo
         hipLaunchKernelGGL(k_fwd_reduce<double>, dim3(grid_for(total)), dim3(256), 0, st, sp.part_d, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
                            (int)sp.X, (int)sp.Y, sp.img_d, (u16 *)nullptr, (u16 *)nullptr, O_col,
-                           (const float2 *)nullptr, 0, (const float *)nullptr, p->nf, check, mark);
+                           (const float2 *)nullptr, 0, (const float *)nullptr, (const float *)nullptr, p->nf, check,
+                           mark);
     else
         hipLaunchKernelGGL(k_fwd_reduce<float>, dim3(grid_for(total)), dim3(256), 0, st, sp.part, ns, stride,
                            (int)p->NP, (int)sp.nrows, (int)p->n, (int)sp.Ly, (int)sp.YP, (int)p->T, (int)p->S,
                            (int)sp.X, (int)sp.Y, sp.img, sh ? sp.img_hi : nullptr, sh ? sp.img_lo : nullptr, O_col,
                            sp.fft.active && !sp.fft.tiled ? (const float2 *)sp.fft.grid : nullptr, sp.fft.Ny,
-                           sp.fft.active && sp.fft.tiled ? sp.fft.tout : nullptr, p->nf, check, mark);
+                           sp.fft.active && sp.fft.tiled ? sp.fft.tout : nullptr,
+                           sp.fft.active && sp.fft.tiled ? sp.fft.tout2 : nullptr, p->nf, check, mark);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }

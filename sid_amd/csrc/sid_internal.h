@@ -124,6 +124,8 @@ struct TfGroup {
     float2 *zin = nullptr;             // backward input (image) spectra [nbin][16][ntile][16] (uv padding zero)
     float *vmax = nullptr;             // per tile and transform block: bound on its spectra
     float *tmax = nullptr;             // per tile: bound over all its blocks
+    int chain = 0;                     // 0: the caller's stream, 1: the handle's second stream
+    double cost = 0;                   // modelled time (s) of one forward + backward
 };
 
 struct FftState {
@@ -165,6 +167,13 @@ struct FftState {
     float2 *yout = nullptr;            // forward GEMM partial results, largest group [tsplit_f][nbin][ntp][chp]
     float2 *bout = nullptr;            // backward GEMM result, largest group [nbin][slot][ntp][chp]
     float *tout = nullptr;             // forward result of the FFT planes, lenslet image [nrows][NP]
+    // With two chains (every plane on the FFT path, two or more groups), the
+    // groups of chain 1 run on the handle's second stream beside those of chain
+    // 0, each chain with its own GEMM results and forward image.
+    int nchain = 1;
+    float2 *yout2 = nullptr, *bout2 = nullptr;
+    float *tout2 = nullptr;
+    hipEvent_t tev_fork = nullptr, tev_join = nullptr;
     std::vector<hipEvent_t> ev;     // event pairs bracketing the GEMVs of the last call
     std::vector<int> ev_kind;        // 0 forward GEMV, 1 backward GEMV (one entry per pair)
 };
@@ -286,6 +295,8 @@ int autotune_methods(sid_psf *p, int64_t X, int64_t Y, int64_t nz, hipStream_t s
 constexpr int kTfNotApplicable = 1000;
 int tf_build(sid_psf *p, hipStream_t st);
 void tf_free(FftState &f);
+// sid_api.hip: the handle's second stream and its fork / join events (created on first use)
+int ensure_aux(sid_psf *p);
 // bytes (at least) and issued MFMA flop of one forward / backward projection's spectra GEMMs
 void tf_gemm_work(const FftState &f, int64_t bytes[2], double flops[2]);
 int tf_forward(sid_psf *p, hipStream_t st);    // FFT planes' forward image into f.tout

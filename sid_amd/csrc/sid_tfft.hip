@@ -795,6 +795,15 @@ static TfChoice tf_choose(int K, int Lx, int Ly, int P, int chp, int kslot)
     return best;
 }
 
+int ensure_aux(sid_psf *p)
+{
+    if (p->aux) return SID_OK;
+    SID_TRY_HIP(hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking));
+    SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    SID_TRY_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    return SID_OK;
+}
+
 void tf_free(FftState &f)
 {
     for (TfGroup &g : f.tg) {
@@ -803,10 +812,14 @@ void tf_free(FftState &f)
             if (q) (void)hipFree(q);
     }
     f.tg.clear();
-    for (void *q : {(void *)f.yout, (void *)f.bout, (void *)f.tout})
+    for (void *q : {(void *)f.yout, (void *)f.bout, (void *)f.tout, (void *)f.yout2, (void *)f.bout2, (void *)f.tout2})
         if (q) (void)hipFree(q);
-    f.yout = f.bout = nullptr;
-    f.tout = nullptr;
+    f.yout = f.bout = f.yout2 = f.bout2 = nullptr;
+    f.tout = f.tout2 = nullptr;
+    for (hipEvent_t e : {f.tev_fork, f.tev_join})
+        if (e) (void)hipEventDestroy(e);
+    f.tev_fork = f.tev_join = nullptr;
+    f.nchain = 1;
     f.tiled = false;
 }
 
@@ -907,15 +920,42 @@ int tf_build(sid_psf *p, hipStream_t st)
         t.nbin = t.T * t.H;
         t.kpad = (t.ns * kslot + 16 * kTfRing - 1) / (16 * kTfRing) * (16 * kTfRing);   // whole ring trips
         tf_splits(t, cus);
+        t.cost = tc.cost;
         f.tg.push_back(t);
-        const int64_t ntp = (t.ntile + 15) / 16 * 16;
-        nyout = std::max<int64_t>(nyout, (int64_t)t.nbin * ntp * chp * t.tsplit_f);
-        nbout = std::max<int64_t>(nbout, (int64_t)t.nbin * ntp * chp * t.ns);
     }
+    // Two chains when every plane is on this path and there are two groups or
+    // more: the groups, in order, go to the chain with the smaller modelled
+    // time so far; the chains run side by side on the caller's stream and the
+    // handle's second stream, filling each other's launch gaps and tails.
+    f.nchain = (p->overlap && f.all_fft && f.tg.size() >= 2) ? 2 : 1;
+    {
+        double load[2] = {0.0, 0.0};
+        for (TfGroup &t : f.tg) {
+            t.chain = f.nchain == 2 && load[1] < load[0] ? 1 : 0;
+            load[t.chain] += t.cost;
+        }
+    }
+    int64_t nyout2 = 0, nbout2 = 0;
+    for (const TfGroup &t : f.tg) {
+        const int64_t ntp = (t.ntile + 15) / 16 * 16;
+        const int64_t y = (int64_t)t.nbin * ntp * chp * t.tsplit_f, b = (int64_t)t.nbin * ntp * chp * t.ns;
+        (t.chain ? nyout2 : nyout) = std::max(t.chain ? nyout2 : nyout, y);
+        (t.chain ? nbout2 : nbout) = std::max(t.chain ? nbout2 : nbout, b);
+    }
+    const int64_t nimg = sp.Lx * sp.Ly * p->NP;
     SID_TRY_HIP(hipMalloc(&f.yout, sizeof(float2) * nyout));
     SID_TRY_HIP(hipMalloc(&f.bout, sizeof(float2) * nbout));
-    SID_TRY_HIP(hipMalloc(&f.tout, sizeof(float) * sp.Lx * sp.Ly * p->NP));
-    SID_TRY_HIP(hipMemsetAsync(f.tout, 0, sizeof(float) * sp.Lx * sp.Ly * p->NP, st));
+    SID_TRY_HIP(hipMalloc(&f.tout, sizeof(float) * nimg));
+    SID_TRY_HIP(hipMemsetAsync(f.tout, 0, sizeof(float) * nimg, st));
+    if (f.nchain == 2) {
+        SID_TRY(ensure_aux(p));
+        SID_TRY_HIP(hipMalloc(&f.yout2, sizeof(float2) * nyout2));
+        SID_TRY_HIP(hipMalloc(&f.bout2, sizeof(float2) * nbout2));
+        SID_TRY_HIP(hipMalloc(&f.tout2, sizeof(float) * nimg));
+        SID_TRY_HIP(hipMemsetAsync(f.tout2, 0, sizeof(float) * nimg, st));
+        SID_TRY_HIP(hipEventCreateWithFlags(&f.tev_fork, hipEventDisableTiming));
+        SID_TRY_HIP(hipEventCreateWithFlags(&f.tev_join, hipEventDisableTiming));
+    }
     f.g_bytes = 0;
     int64_t x1_elems = 0, gt_elems = 0;
     const int RB = chp / 16;
@@ -1094,22 +1134,34 @@ static int tf_tmax(const TfGroup &t, int rowlen, hipStream_t st)
     return SID_OK;
 }
 
-// Forward image of the FFT planes into f.tout: group by group, the first
-// writing it and the others adding to it (a fixed order).
+// Forward image of the FFT planes into f.tout (and f.tout2 for chain 1): in
+// each chain group by group, the first writing it and the others adding to
+// it (a fixed order); the forward reduction adds the two images.
 int tf_forward(sid_psf *p, hipStream_t st)
 {
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
     const int n2 = (int)(p->n * p->n);
-    for (size_t gi = 0; gi < f.tg.size(); ++gi) {
+    if (f.nchain == 2) {
+        SID_TRY_HIP(hipEventRecord(f.tev_fork, st));
+        SID_TRY_HIP(hipStreamWaitEvent(p->aux, f.tev_fork, 0));
+    }
+    int rc = SID_OK;
+    bool first[2] = {true, true};
+    for (size_t gi = 0; gi < f.tg.size() && rc == SID_OK; ++gi) {
         const TfGroup &t = f.tg[gi];
+        const hipStream_t s = t.chain ? p->aux : st;
+        float2 *yout = t.chain ? f.yout2 : f.yout;
+        float *tout = t.chain ? f.tout2 : f.tout;
         TfGeom g = tf_geom(p, f, t);
         g.off = -t.Dhi;
-        SID_TRY(tf_in_any(t.T, sp.vol, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, g, t.vin, t.vmax, st));
-        SID_TRY(tf_tmax(t, t.ns * f.ngrp, st));
-        fft_mark(f, st, 0, true);
-        SID_TRY(tf_gemm<false>(f, t, t.G, t.vin, 1, t.kpad / 16, t.kpad, f.yout, st));
-        fft_mark(f, st, 0, false);
+        rc = tf_in_any(t.T, sp.vol, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, g, t.vin, t.vmax, s);
+        if (rc == SID_OK) rc = tf_tmax(t, t.ns * f.ngrp, s);
+        if (rc != SID_OK) break;
+        fft_mark(f, s, 0, true);
+        rc = tf_gemm<false>(f, t, t.G, t.vin, 1, t.kpad / 16, t.kpad, yout, s);
+        fft_mark(f, s, 0, false);
+        if (rc != SID_OK) break;
         TfGeom go = g;
         go.nslot = 1;
         go.off = t.K - 1;
@@ -1117,10 +1169,16 @@ int tf_forward(sid_psf *p, hipStream_t st)
         go.S = (int)p->NP;
         go.XP = sp.Lx;
         go.YP = sp.Ly;
-        SID_TRY(tf_out_any(t.T, f.yout, t.tsplit_f, (int64_t)t.nbin * go.ntp * f.chp, nullptr, nullptr, go,
-                           1.0f / ((float)t.T * (float)t.T), gi > 0, f.tout, st));
+        rc = tf_out_any(t.T, yout, t.tsplit_f, (int64_t)t.nbin * go.ntp * f.chp, nullptr, nullptr, go,
+                        1.0f / ((float)t.T * (float)t.T), first[t.chain] ? 0 : 1, tout, s);
+        first[t.chain] = false;
     }
-    return SID_OK;
+    if (f.nchain == 2) {
+        // join even after a failed launch, so that st never runs ahead of work left on the second stream
+        SID_TRY_HIP(hipEventRecord(f.tev_join, p->aux));
+        SID_TRY_HIP(hipStreamWaitEvent(st, f.tev_join, 0));
+    }
+    return rc;
 }
 
 // Back-projection of the FFT planes into f.dfft (each group its own planes).
@@ -1129,23 +1187,37 @@ int tf_backward(sid_psf *p, hipStream_t st)
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
     const int n2 = (int)(p->n * p->n);
-    for (const TfGroup &t : f.tg) {
-        TfGeom gi = tf_geom(p, f, t);
-        gi.nslot = 1;
-        gi.nz_in = 1;
-        gi.off = t.Dlo;
-        gi.kslot = 0;
-        gi.kdim = kTfKsB * 16;
-        SID_TRY(tf_in_any(t.T, sp.img, nullptr, nullptr, gi, t.zin, t.vmax, st));
-        SID_TRY(tf_tmax(t, f.ngrp, st));
-        fft_mark(f, st, 1, true);
-        SID_TRY(tf_gemm<true>(f, t, t.Gb, t.zin, t.ns, kTfKsB, kTfKsB * 16, f.bout, st));
-        fft_mark(f, st, 1, false);
-        const TfGeom go = tf_geom(p, f, t);
-        SID_TRY(tf_out_any(t.T, f.bout, 1, 0, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, go,
-                           1.0f / ((float)t.T * (float)t.T), 0, f.dfft, st));
+    if (f.nchain == 2) {
+        SID_TRY_HIP(hipEventRecord(f.tev_fork, st));
+        SID_TRY_HIP(hipStreamWaitEvent(p->aux, f.tev_fork, 0));
     }
-    return SID_OK;
+    int rc = SID_OK;
+    for (size_t gi = 0; gi < f.tg.size() && rc == SID_OK; ++gi) {
+        const TfGroup &t = f.tg[gi];
+        const hipStream_t s = t.chain ? p->aux : st;
+        float2 *bout = t.chain ? f.bout2 : f.bout;
+        TfGeom gi_ = tf_geom(p, f, t);
+        gi_.nslot = 1;
+        gi_.nz_in = 1;
+        gi_.off = t.Dlo;
+        gi_.kslot = 0;
+        gi_.kdim = kTfKsB * 16;
+        rc = tf_in_any(t.T, sp.img, nullptr, nullptr, gi_, t.zin, t.vmax, s);
+        if (rc == SID_OK) rc = tf_tmax(t, f.ngrp, s);
+        if (rc != SID_OK) break;
+        fft_mark(f, s, 1, true);
+        rc = tf_gemm<true>(f, t, t.Gb, t.zin, t.ns, kTfKsB, kTfKsB * 16, bout, s);
+        fft_mark(f, s, 1, false);
+        if (rc != SID_OK) break;
+        const TfGeom go = tf_geom(p, f, t);
+        rc = tf_out_any(t.T, bout, 1, 0, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, go,
+                        1.0f / ((float)t.T * (float)t.T), 0, f.dfft, s);
+    }
+    if (f.nchain == 2) {
+        SID_TRY_HIP(hipEventRecord(f.tev_join, p->aux));
+        SID_TRY_HIP(hipStreamWaitEvent(st, f.tev_join, 0));
+    }
+    return rc;
 }
 
 }  // namespace sid
