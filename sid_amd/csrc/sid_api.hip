@@ -517,22 +517,20 @@ static int check_vdims(sid_psf *p, const int64_t vdims[3], bool rl)
 // launch_bwd_reduce).  With both halves present its direct planes are reduced
 // on the GEMM's stream as soon as the GEMM is done, beside the rest of the FFT
 // path; the caller then reduces the FFT planes only.
-struct BwdTail {
-    float *B_col;
-    int rl_update;
-    unsigned long long check, mark;
-};
-
+// When every plane is on the tiled FFT path the FFT half applies the whole
+// reduction itself (*tail_all_done); the caller then launches none.
 static int project_halves(sid_psf *p, int dir, hipStream_t st, Timer &tm, int ph_gemm, int ph_fft,
-                          const BwdTail *tail = nullptr, bool *tail_direct_done = nullptr)
+                          const BwdTail *tail = nullptr, bool *tail_direct_done = nullptr,
+                          bool *tail_all_done = nullptr)
 {
     if (tail_direct_done) *tail_direct_done = false;
+    if (tail_all_done) *tail_all_done = false;
     const GemmPlan &g = dir == 0 ? p->plan.fwd : p->plan.bwd;
     if (!(p->overlap && p->plan.fft.active && !g.items.empty())) {
         const int a = tm.mark();
         SID_TRY(launch_gemm(p, dir, st));
         const int b = tm.mark();
-        SID_TRY(dir == 0 ? fft_forward(p, st) : fft_backward(p, st));
+        SID_TRY(dir == 0 ? fft_forward(p, st) : fft_backward(p, st, tail_all_done ? tail : nullptr, tail_all_done));
         const int c = tm.mark();
         tm.span(ph_gemm, a, b);
         tm.span(ph_fft, b, c);
@@ -757,10 +755,10 @@ static int backward_raw(sid_psf *p, const float *O_dev, const int64_t odims[2], 
     const unsigned long long stamp = next_stamp(p);
     SID_TRY(launch_img_in(p, O_dev, st, stamp));
     const BwdTail tail{B_dev, 0, stamp, 0};
-    bool direct_done = false;
-    SID_TRY(project_halves(p, 1, st, tm, 2, 6, &tail, &direct_done));
+    bool direct_done = false, all_done = false;
+    SID_TRY(project_halves(p, 1, st, tm, 2, 6, &tail, &direct_done, &all_done));
     const int e2 = tm.mark();
-    SID_TRY(launch_bwd_reduce(p, B_dev, 0, st, stamp, 0, direct_done ? 2 : 0));
+    if (!all_done) SID_TRY(launch_bwd_reduce(p, B_dev, 0, st, stamp, 0, direct_done ? 2 : 0));
     const int e3 = tm.mark();
     tm.span(3, e2, e3);
     if (!sync) return SID_OK;
@@ -910,10 +908,10 @@ static int rl_impl(sid_psf *p, const float *Htf, float *X, const int64_t vdims[3
         const int c = tm.mark();
         s_vol = next_stamp(p);
         const BwdTail tail{hist ? hist + (int64_t)it * nv : nullptr, 1, s_img, s_vol};
-        bool direct_done = false;
-        SID_TRY(project_halves(p, 1, st, tm, 2, 6, &tail, &direct_done));
+        bool direct_done = false, all_done = false;
+        SID_TRY(project_halves(p, 1, st, tm, 2, 6, &tail, &direct_done, &all_done));
         const int d = tm.mark();
-        SID_TRY(launch_bwd_reduce(p, tail.B_col, 1, st, s_img, s_vol, direct_done ? 2 : 0));
+        if (!all_done) SID_TRY(launch_bwd_reduce(p, tail.B_col, 1, st, s_img, s_vol, direct_done ? 2 : 0));
         const int e = tm.mark();
         tm.span(1, b, c);
         tm.span(3, d, e);

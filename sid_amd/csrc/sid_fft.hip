@@ -1260,12 +1260,13 @@ int fft_forward(sid_psf *p, hipStream_t st)
 }
 
 // Backward FFT planes from the image in sp.img: writes f.dfft (padded volume layout).
-int fft_backward(sid_psf *p, hipStream_t st)
+int fft_backward(sid_psf *p, hipStream_t st, const BwdTail *tail, bool *tail_done)
 {
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
+    if (tail_done) *tail_done = false;
     if (!f.active) return SID_OK;
-    if (f.tiled) return tf_backward(p, st);
+    if (f.tiled) return tf_backward(p, st, tail, tail_done);
     const int n = (int)p->n;
     hipLaunchKernelGGL(k_fft_img_grid, dim3(grid_n((int64_t)f.Nx * f.Ny)), dim3(256), 0, st, sp.img, f.Nx, f.Ny,
                        (int)(sp.Lx * n), (int)(sp.Ly * n), n, sp.YP, (int)p->T, (int)p->S, f.grid);

@@ -285,7 +285,15 @@ void fft_state_free(FftState &f);
 bool fft_eligible(const sid_psf *p, int64_t c);
 int fft_state_build(sid_psf *p, const std::vector<char> &use, hipStream_t st);
 int fft_forward(sid_psf *p, hipStream_t st);
-int fft_backward(sid_psf *p, hipStream_t st);
+// The backward reduction that follows a backward projection (the arguments
+// of launch_bwd_reduce): the column-major output (or null), the ISRA update
+// flag, and the non-finite stamps to check and to set.
+struct BwdTail {
+    float *B_col;
+    int rl_update;
+    unsigned long long check, mark;
+};
+int fft_backward(sid_psf *p, hipStream_t st, const BwdTail *tail = nullptr, bool *tail_done = nullptr);
 void fft_timing_collect(sid_psf *p, double *fwd_ms, double *bwd_ms);
 int next_smooth235(int L);
 int fft_test(int L, int64_t nseq, float2 *data, int dir, hipStream_t st);
@@ -300,7 +308,8 @@ int ensure_aux(sid_psf *p);
 // bytes (at least) and issued MFMA flop of one forward / backward projection's spectra GEMMs
 void tf_gemm_work(const FftState &f, int64_t bytes[2], double flops[2]);
 int tf_forward(sid_psf *p, hipStream_t st);    // FFT planes' forward image into f.tout
-int tf_backward(sid_psf *p, hipStream_t st);   // FFT planes' back-projection into f.dfft
+// FFT planes' back-projection into f.dfft, or through the reduction's tail (see BwdTail)
+int tf_backward(sid_psf *p, hipStream_t st, const BwdTail *tail = nullptr, bool *tail_done = nullptr);
 void fft_mark(FftState &f, hipStream_t st, int kind, bool begin);   // timing events (GEMV / GEMM)
 // sid_library.hip
 void patched_rect(int64_t n, int64_t M, int64_t X, int64_t Y, const int64_t box[6], int64_t rect[4]);

@@ -260,16 +260,58 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restri
     }
 }
 
+// The reduction that follows a backward projection, applied by the inverse
+// transform itself when every plane of H is on the tiled path (no direct
+// slabs to add): mode 1 writes the back-projection to the column-major output,
+// mode 2 is the ISRA update of deconvRL.m:52,85-87 (r = Htf / d, Inf -> 0,
+// X = X r, NaN -> 0) on the padded iterate, with the iterate optionally copied
+// to the column-major output (the history).  A non-finite image stamp
+// (check) makes d NaN everywhere; a non-finite new iterate sets the next stamp
+// (mark).  The same arithmetic as k_bwd_reduce.
+struct TfTail {
+    int mode = 0;                       // 0: write the destination (no tail)
+    int X = 0, Y = 0, n = 0;
+    const float *htf = nullptr;         // padded layout, as the destination
+    float *vol = nullptr;
+    float *B_col = nullptr;             // column-major [X][Y][Nz], or null
+    unsigned long long *nf = nullptr;
+    unsigned long long check = 0, mark = 0;
+};
+
+// One element: plane c, lenslet (Px, Py), channel ab at padded offset o; true
+// when the new iterate is not finite.
+__device__ __forceinline__ bool tf_tail_elem(const TfTail &tl, int64_t o, int c, int Px, int Py, int ab, float d)
+{
+    const int x = Px * tl.n + ab % tl.n, y = Py * tl.n + ab / tl.n;
+    const bool real = x < tl.X && y < tl.Y;
+    const int64_t col = (int64_t)x + (int64_t)tl.X * ((int64_t)y + (int64_t)tl.Y * c);
+    if (tl.mode == 1) {
+        if (real) tl.B_col[col] = d;
+        return false;
+    }
+    float v = 0.f;
+    if (real) {
+        float ratio = tl.htf[o] / d;
+        if (isinf(ratio)) ratio = 0.f;
+        v = tl.vol[o] * ratio;
+        if (isnan(v)) v = 0.f;
+    }
+    tl.vol[o] = v;
+    if (tl.B_col && real) tl.B_col[col] = v;
+    return !isfinite(v);
+}
+
 // Inverse: spec[bin][slot][ntp][chp] (the sum of nsum partial spectra
 // pstride apart) -> along x (one stored fy per thread; only the tile's valid
 // rows kept in LDS), then along y in real pairs with the Hermitian completion;
 // the valid B x B block, times scale * s_q (qscale, when given), goes to the
-// destination layout (added to what is there when acc is set).
+// destination layout (added to what is there when acc is set), or goes
+// through the backward reduction's tail (tl.mode != 0).
 template <int T, bool SUM>
 __global__ __launch_bounds__(tf_threads<T>()) void k_tf_out(const float2 *__restrict__ spec, int nsum,
                                                            int64_t pstride, const int *__restrict__ slot_plane,
                                                            const float *__restrict__ qscale, TfGeom g, float scale,
-                                                           int acc, float *__restrict__ dst)
+                                                           int acc, float *__restrict__ dst, TfTail tl)
 {
     constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2, NT = tf_threads<T>();
     extern __shared__ float2 tsm[];   // [CG][B][H]
@@ -336,12 +378,28 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_out(const float2 *__rest
         const bool two = ch + 1 < g.n2;
         const float qa = scale * (qscale ? qscale[slot * g.n2 + ch] : 1.0f);
         const float qb = two ? scale * (qscale ? qscale[slot * g.n2 + ch + 1] : 1.0f) : 0.f;
-        float *o = dst + (((int64_t)c * g.XP + Px + g.pad) * g.YP + g.pad) * g.S + ch;
+        const int64_t o = (((int64_t)c * g.XP + Px + g.pad) * g.YP + g.pad) * g.S + ch;
+        if (tl.mode != 0) {
+            const bool poison = tl.check != 0 && tl.nf[1] == tl.check;   // non-finite image: NaN everywhere
+            bool bad = false;
+#pragma unroll
+            for (int y = 0; y < T; ++y) {
+                const int yr = y - g.off, Py = ty * B + yr;
+                if (yr >= 0 && yr < B && Py < g.Ly) {
+                    const int64_t q = o + (int64_t)Py * g.S;
+                    const float da = poison ? NAN : z[y].x * qa;
+                    bad |= tf_tail_elem(tl, q, c, Px, Py, ch, da);
+                    if (two) bad |= tf_tail_elem(tl, q + 1, c, Px, Py, ch + 1, poison ? NAN : z[y].y * qb);
+                }
+            }
+            if (bad && tl.mark) tl.nf[0] = tl.mark;   // an Inf iterate poisons the next forward
+            continue;
+        }
 #pragma unroll
         for (int y = 0; y < T; ++y) {
             const int yr = y - g.off, Py = ty * B + yr;
             if (yr >= 0 && yr < B && Py < g.Ly) {
-                float *q = o + (int64_t)Py * g.S;
+                float *q = dst + o + (int64_t)Py * g.S;
                 if (two) {
                     float2 v = make_float2(z[y].x * qa, z[y].y * qb);
                     if (acc) {
@@ -1060,16 +1118,16 @@ static int tf_in(const float *src, const int *slot_plane, const float *qscale, c
 
 template <int T>
 static int tf_out(const float2 *spec, int nsum, int64_t pstride, const int *slot_plane, const float *qscale,
-                  const TfGeom &g, float scale, int acc, float *dst, hipStream_t st)
+                  const TfGeom &g, float scale, int acc, float *dst, const TfTail &tl, hipStream_t st)
 {
     const size_t lds = sizeof(float2) * kTfCG * g.B * (T / 2 + 1);
     const dim3 grid(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), block(tf_threads<T>());
     if (nsum > 1)
         hipLaunchKernelGGL((k_tf_out<T, true>), grid, block, lds, st, spec, nsum, pstride, slot_plane, qscale, g, scale,
-                           acc, dst);
+                           acc, dst, tl);
     else
         hipLaunchKernelGGL((k_tf_out<T, false>), grid, block, lds, st, spec, nsum, pstride, slot_plane, qscale, g, scale,
-                           acc, dst);
+                           acc, dst, tl);
     SID_TRY_HIP(hipGetLastError());
     return SID_OK;
 }
@@ -1096,9 +1154,10 @@ static int tf_in_any(int T, const float *src, const int *slot_plane, const float
 }
 
 static int tf_out_any(int T, const float2 *spec, int nsum, int64_t pstride, const int *slot_plane,
-                      const float *qscale, const TfGeom &g, float scale, int acc, float *dst, hipStream_t st)
+                      const float *qscale, const TfGeom &g, float scale, int acc, float *dst, const TfTail &tl,
+                      hipStream_t st)
 {
-#define SID_TF_OUT(L) tf_out<L>(spec, nsum, pstride, slot_plane, qscale, g, scale, acc, dst, st)
+#define SID_TF_OUT(L) tf_out<L>(spec, nsum, pstride, slot_plane, qscale, g, scale, acc, dst, tl, st)
     SID_TF_SWITCH(T, SID_TF_OUT)
 #undef SID_TF_OUT
 }
@@ -1170,7 +1229,7 @@ int tf_forward(sid_psf *p, hipStream_t st)
         go.XP = sp.Lx;
         go.YP = sp.Ly;
         rc = tf_out_any(t.T, yout, t.tsplit_f, (int64_t)t.nbin * go.ntp * f.chp, nullptr, nullptr, go,
-                        1.0f / ((float)t.T * (float)t.T), first[t.chain] ? 0 : 1, tout, s);
+                        1.0f / ((float)t.T * (float)t.T), first[t.chain] ? 0 : 1, tout, TfTail(), s);
         first[t.chain] = false;
     }
     if (f.nchain == 2) {
@@ -1182,11 +1241,28 @@ int tf_forward(sid_psf *p, hipStream_t st)
 }
 
 // Back-projection of the FFT planes into f.dfft (each group its own planes).
-int tf_backward(sid_psf *p, hipStream_t st)
+// With a tail and every plane of H on this path, the inverse transforms apply
+// the backward reduction themselves (*tail_done is set; no dfft round trip).
+int tf_backward(sid_psf *p, hipStream_t st, const BwdTail *tail, bool *tail_done)
 {
     SensorPlan &sp = p->plan;
     FftState &f = sp.fft;
     const int n2 = (int)(p->n * p->n);
+    TfTail tl;
+    if (tail && f.n_list_fft == p->nzH && std::getenv("SID_TF_NOFUSE") == nullptr) {
+        tl.mode = tail->rl_update ? 2 : 1;
+        tl.X = (int)sp.X;
+        tl.Y = (int)sp.Y;
+        tl.n = (int)p->n;
+        tl.htf = sp.htf;
+        tl.vol = sp.vol;
+        tl.B_col = tail->B_col;
+        tl.nf = p->nf;
+        tl.check = tail->check;
+        tl.mark = tail->mark;
+        if (tl.mode == 1 && !tl.B_col) tl.mode = 0;
+    }
+    if (tail_done) *tail_done = tl.mode != 0;
     if (f.nchain == 2) {
         SID_TRY_HIP(hipEventRecord(f.tev_fork, st));
         SID_TRY_HIP(hipStreamWaitEvent(p->aux, f.tev_fork, 0));
@@ -1211,7 +1287,7 @@ int tf_backward(sid_psf *p, hipStream_t st)
         if (rc != SID_OK) break;
         const TfGeom go = tf_geom(p, f, t);
         rc = tf_out_any(t.T, bout, 1, 0, f.d_slot_plane + t.s0, f.qscale + (int64_t)t.s0 * n2, go,
-                        1.0f / ((float)t.T * (float)t.T), 0, f.dfft, s);
+                        1.0f / ((float)t.T * (float)t.T), 0, f.dfft, tl, s);
     }
     if (f.nchain == 2) {
         SID_TRY_HIP(hipEventRecord(f.tev_join, p->aux));
