@@ -550,6 +550,35 @@ def main():
             f"fwd {t_f:.2f} / bwd {t_b:.2f} ms on {res['planes_direct']} planes, FFT path fwd {t_ff:.2f} / "
             f"bwd {t_fb:.2f} ms on {res['planes_fft']} planes ({res['spectra_gb']:.1f} GB spectra)")
 
+    # The tiled FFT path runs its plane groups in two stream chains side by side, so each spectra
+    # GEMM shares the GPU with the other chain's kernels and its event time is not its own rate.
+    # One more step of the headline precision with one chain gives the GEMMs' alone-rate for the
+    # roofline (the headline value stays the two-chain run's).
+    head0 = args.precisions.split(",")[-1]
+    if results.get(head0, {}).get("fft_tiled"):
+        psf.set_precision(head0)
+        psf.set_overlap(False)
+        psf.deconv_rl(htfs[head0], args.iters, htfs[head0])
+        torch.cuda.synchronize()
+        psf.timing()
+        tim1 = np.zeros(12)
+        reps = max(1, min(args.steps, 3))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            psf.deconv_rl(htfs[head0], args.iters, htfs[head0])
+            tim1 += np.array(psf.timing())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        n1 = reps * args.iters
+        hr0 = results[head0]
+        gb = sum(hr0["tf_gemm_gb"]) * 1e9
+        hr0["one_chain"] = {"iter_per_s": n1 / (t1 - t0), "fwd_gemm_ms": tim1[7] / n1, "bwd_gemm_ms": tim1[8] / n1,
+                            "tf_gemm_stream_tbps": gb / ((tim1[7] + tim1[8]) / n1 * 1e-3) / 1e12}
+        psf.set_overlap(True)
+        log(f"{head0} with one stream chain: {hr0['one_chain']['iter_per_s']:.2f} RL iter/s; spectra GEMMs "
+            f"{hr0['one_chain']['fwd_gemm_ms']:.3f} / {hr0['one_chain']['bwd_gemm_ms']:.3f} ms per projection "
+            f"= {hr0['one_chain']['tf_gemm_stream_tbps']:.2f} TB/s")
+
     # errors against the FP64 device path (FP64 MFMA, float I/O), per output and per iteration
     rel = {}
     ref_name = None
@@ -676,17 +705,23 @@ def main():
         if tj and tj.get("tf_gemm_bytes") and tj.get("fft_groups") == hr["fft_groups"]:
             traffic = 0.5 * float(tj["tf_gemm_bytes"])
         tb = sum(hr["tf_gemm_gb"]) * 1e9
-        roof = {"bound": "hbm", "achieved": round(hr["tf_gemm_stream_tbps"] * 1e3, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(hr["tf_gemm_frac_of_hbm"], 4), "traffic": traffic,
+        oc = hr.get("one_chain")
+        ach = oc["tf_gemm_stream_tbps"] if oc else hr["tf_gemm_stream_tbps"]
+        ms_l = 0.5 * (oc["fwd_gemm_ms"] + oc["bwd_gemm_ms"]) if oc else 0.5 * (hr["fwd_gemv_ms"] + hr["bwd_gemv_ms"])
+        roof = {"bound": "hbm", "achieved": round(ach * 1e3, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(ach * 1e12 / HBM_PEAK, 4), "traffic": traffic,
                 "traffic_source": "profiles/pmc_traffic_rl_full_bf16x2.md" if traffic else None,
                 "kernel": "k_tf_gemm (spectra GEMMs of the tiled FFT path, one launch per plane group and "
                           "direction; FP16 MFMA)",
                 "algorithmic_per_launch": f"per projection {0.5 * tb / 1e9:.2f} GB (forward "
                                           f"{hr['tf_gemm_gb'][0]:.2f} / backward {hr['tf_gemm_gb'][1]:.2f} GB: "
                                           f"FP16 spectra once, input spectra read, results written)",
-                "ms_per_launch": round(0.5 * (hr["fwd_gemv_ms"] + hr["bwd_gemv_ms"]), 3),
-                "mfma_achieved_tflops": round(hr["tf_gemm_mfma_tflops"], 1),
-                "mfma_frac_of_fp16_peak": round(hr["tf_gemm_mfma_frac"], 4),
+                "ms_per_launch": round(ms_l, 3),
+                "timing": "HIP events around each group's GEMM launch on its stream, one stream chain"
+                          if oc else "HIP events around each group's GEMM launch on its stream",
+                "achieved_two_chains": round(hr["tf_gemm_stream_tbps"] * 1e3, 1),
+                "mfma_achieved_tflops": round(hr["tf_gemm_mfma_tflops"] * (ach / hr["tf_gemm_stream_tbps"]), 1),
+                "mfma_frac_of_fp16_peak": round(hr["tf_gemm_mfma_frac"] * (ach / hr["tf_gemm_stream_tbps"]), 4),
                 "plane_groups": hr["fft_groups"]}
     else:
         traffic = None

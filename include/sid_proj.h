@@ -108,6 +108,12 @@ int sid_psf_set_precision(sid_psf *psf, int precision);
 /* Force a method for every plane (SID_METHOD_DIRECT / _FFT) or restore the
  * autotuned per-plane choice (SID_METHOD_AUTO). */
 int sid_psf_set_method(sid_psf *psf, int method);
+/* Use the handle's second stream (on != 0, the default unless SID_NO_OVERLAP=1
+ * was set at creation) or only the caller's stream: the direct GEMM beside the
+ * FFT path, and the two chains of tiled FFT plane groups.  Drops the sensor
+ * plan (the next projection builds it again).  No counterpart in the
+ * reference; a measurement control (kernel times without concurrency). */
+int sid_psf_set_overlap(sid_psf *psf, int on);
 /* Pin the method of single planes under SID_METHOD_AUTO: methods[c] is
  * SID_METHOD_DIRECT or SID_METHOD_FFT for plane c, or SID_METHOD_AUTO to leave
  * that plane to the autotuner; n = size(H,5).  methods = NULL clears every pin.

@@ -126,6 +126,11 @@ class PSF:
     def set_method(self, method: str):
         check(lib().sid_psf_set_method(self._h, METHOD[method]))
 
+    def set_overlap(self, on: bool):
+        """Use the handle's second stream beside the caller's (default) or not;
+        the sensor plan is rebuilt by the next projection."""
+        check(lib().sid_psf_set_overlap(self._h, 1 if on else 0))
+
     def set_plane_methods(self, methods=None):
         """Pin planes to "direct" / "fft" ("auto" leaves a plane to the
         autotuner); one entry per plane of H, None clears every pin."""

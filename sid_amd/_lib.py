@@ -37,6 +37,7 @@ SIGNATURES = {
     "sid_backward_ones_device": (_I, [_P, ctypes.c_int64, ctypes.c_int64, _P, _P]),
     "sid_psf_set_precision": (_I, [_P, _I]),
     "sid_psf_set_method": (_I, [_P, _I]),
+    "sid_psf_set_overlap": (_I, [_P, _I]),
     "sid_psf_set_plane_methods": (_I, [_P, _I32P, ctypes.c_int64]),
     "sid_psf_get_info": (_I, [_P, _P]),
     "sid_forward": (_I, [_P, _P, _I64P, _P]),

@@ -421,6 +421,19 @@ int sid_psf_set_method(sid_psf *p, int method)
     return SID_OK;
 }
 
+int sid_psf_set_overlap(sid_psf *p, int on)
+{
+    if (!p) { set_error("null handle"); return SID_ERR_ARG; }
+    const bool v = on != 0;
+    if (v == p->overlap) return SID_OK;
+    if (p->plan.vol) SID_TRY_HIP(hipStreamSynchronize(p->stream));
+    if (p->aux) SID_TRY_HIP(hipStreamSynchronize(p->aux));
+    p->overlap = v;
+    fft_state_free(p->plan.fft);   // the sensor plan is built again by the next projection
+    free_plan_buffers(p->plan);
+    return SID_OK;
+}
+
 int sid_psf_set_plane_methods(sid_psf *p, const int32_t *methods, int64_t n)
 {
     if (!p) { set_error("null handle"); return SID_ERR_ARG; }
