@@ -216,3 +216,50 @@ def test_tiled_plane_groups(env, monkeypatch):
     B1 = one.backward(W.astype(np.float32))
     assert rel_l2(O1, O) < TOL and rel_l2(B1, B) < TOL
     one.close()
+
+
+def test_tiled_two_chains_and_fused_tail(env, monkeypatch):
+    """Every plane on the tiled path with several groups: the groups run in two
+    stream chains and the backward's inverse transforms apply the ISRA update
+    and the column-major outputs themselves.  Repeated calls are bit-identical;
+    one chain (sid_psf_set_overlap(0)) and the unfused reduction
+    (SID_TF_NOFUSE=1) give the same iterates (bit-identical for the reduction,
+    FP32 rounding of the forward image's summation order for the chains); the
+    history equals the iterates of separate calls; all match the oracle."""
+    torch, sid_amd = env
+    monkeypatch.delenv("SID_TF_T", raising=False)
+    monkeypatch.delenv("SID_TF_GROUPS", raising=False)
+    monkeypatch.delenv("SID_TF_NOFUSE", raising=False)
+    monkeypatch.setenv("SID_TF_GROUP_US", "0")
+    n, X, Y, M, Nz = 5, 200, 190, 61, 6
+    print("case: tiled two chains, fused tail", flush=True)
+    rng = np.random.default_rng(48)
+    H = random_psf(rng, M, n, Nz)
+    m = M // 2
+    CA = np.array([[m + 1 - r, m + 1 + r] for r in [2, 4, 9, 14, 22, 30]], dtype=np.int32)
+    V = rng.random((X, Y, Nz))
+    htf = c_backward(H, c_forward(H, V, CA), CA).astype(np.float32)
+    x0 = (htf + 0.05).astype(np.float32)
+    psf = sid_amd.PSF(H.astype(np.float32), CA, precision="bf16x2", method="fft")
+    a, hist = psf.deconv_rl(htf, 3, x0, history=True)
+    assert len(psf.info()["fft_groups"]) >= 2
+    b = psf.deconv_rl(htf, 3, x0)
+    assert np.array_equal(a, b)
+    assert np.array_equal(hist[-1], a)
+    assert np.array_equal(hist[0], psf.deconv_rl(htf, 1, x0))
+    W = rng.random((X, Y)).astype(np.float32)
+    B2 = psf.backward(W)
+    want = c_isra(H, htf.astype(np.float64), x0.astype(np.float64), 3, CA)
+    e = rel_l2(a, want)
+    print("isra rel-L2", e, flush=True)
+    assert e < 4 * TOL and rel_l2(B2, c_backward(H, W, CA)) < TOL
+    psf.set_overlap(False)
+    c = psf.deconv_rl(htf, 3, x0)
+    assert rel_l2(c, a) < 1e-6
+    psf.close()
+    monkeypatch.setenv("SID_TF_NOFUSE", "1")
+    unf = sid_amd.PSF(H.astype(np.float32), CA, precision="bf16x2", method="fft")
+    d = unf.deconv_rl(htf, 3, x0)
+    assert np.array_equal(d, a)
+    assert np.array_equal(unf.backward(W), B2)
+    unf.close()
