@@ -298,6 +298,20 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
         sid_amd.convn_same(vol, kt, precision="fp32")
     torch.cuda.synchronize()
     tc32 = (time.perf_counter() - t0) / reps
+    # the BF16 handles' transform form against their MFMA form (SID_CONVN_FFT=0)
+    tcm = None
+    if cprec != "fp32":
+        os.environ["SID_CONVN_FFT"] = "0"
+        try:
+            sid_amd.convn_same(vol, kt, precision=cprec)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                sid_amd.convn_same(vol, kt, precision=cprec)
+            torch.cuda.synchronize()
+            tcm = (time.perf_counter() - t0) / reps
+        finally:
+            del os.environ["SID_CONVN_FFT"]
     taps = int(np.prod(k.shape))
     fl = 2.0 * wl.X * wl.Y * wl.nz * taps
     psf.set_kernel(k)
@@ -308,13 +322,18 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     torch.cuda.synchronize()
     dtw = time.perf_counter() - t0
     psf.set_kernel(None)
-    out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_ms": round(tc * 1e3, 3),
+    form = "transform" if cprec != "fp32" and wl.X + k.shape[0] // 2 <= 1024 and wl.Y + k.shape[1] // 2 <= 1024 \
+        and k.shape[2] <= 8 else ("mfma" if cprec != "fp32" else "direct")
+    out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_form": form,
+                            "convn_mfma_ms": round(tcm * 1e3, 3) if tcm else None,
+                            "convn_ms": round(tc * 1e3, 3),
                             "convn_tflops": round(fl / tc / 1e12, 2), "convn_fp32_ms": round(tc32 * 1e3, 3),
                             "convn_fp32_tflops": round(fl / tc32 / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
                             "convn_frac_of_peak": round(fl / tc / PEAK[cprec], 4),
                             "convn_fp32_frac_of_vector_peak": round(fl / tc32 / 78.6e12, 4),
                             "rl_iter_per_s_wrapped": round(iters / dtw, 3)}
-    log(f"neuron kernel {k.shape}: convn {cprec} {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s algorithmic), "
+    log(f"neuron kernel {k.shape}: convn {cprec} {form} {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s algorithmic, "
+        f"MFMA form {tcm * 1e3 if tcm else float('nan'):.2f} ms), "
         f"FP32 {tc32 * 1e3:.2f} ms; wrapped RL {iters / dtw:.2f} iter/s")
     del vol
     P, T = wl.X * wl.Y, 1000

@@ -636,6 +636,8 @@ def convn_same(V, k):
         if w == 0:
             continue
         sh = [off[d] - t[d] for d in range(3)]
+        if any(abs(sh[d]) >= V.shape[d] for d in range(3)):
+            continue   # shifted past the volume: no overlap
         src = tuple(slice(max(0, s), V.shape[d] + min(0, s)) for d, s in enumerate(sh))
         dst = tuple(slice(max(0, -s), V.shape[d] - max(0, s)) for d, s in enumerate(sh))
         out[dst] += w * V[src]

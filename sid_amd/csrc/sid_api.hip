@@ -154,6 +154,10 @@ static void destroy(sid_psf *p)
                     (void *)p->wf_d, (void *)p->wb_d, (void *)p->kern, (void *)p->kwork, (void *)p->kfrag,
                     (void *)p->hbuf[0], (void *)p->hbuf[1], (void *)p->nf})
         if (q) (void)hipFree(q);
+    if (p->kfft) {
+        convn_fft_free(*p->kfft);
+        delete p->kfft;
+    }
     if (p->aux) {
         (void)hipStreamSynchronize(p->aux);
         (void)hipStreamDestroy(p->aux);
@@ -792,11 +796,19 @@ static int kwork_reserve(sid_psf *p, int64_t n)
     return SID_OK;
 }
 
-// convn(in, k, 'same') in the handle's arithmetic: BF16-split MFMA for BF16X2 / BF16X3, FP32 otherwise
+// convn(in, k, 'same') for the handle's precision: BF16X2 / BF16X3 take the FP32
+// transform form when it fits (kernel spectra kept on the handle), else the
+// BF16-split MFMA form; FP32 / FP64 take the exact FP32 direct sum
 static int handle_convn(sid_psf *p, const float *in, const int64_t vdims[3], float *out, hipStream_t st)
 {
-    if (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3)
+    if (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3) {
+        if (convn_fft_enabled() && convn_fft_fits(vdims, p->kdims)) {
+            if (!p->kfft) p->kfft = new ConvnFft();
+            SID_TRY(convn_fft_prepare(*p->kfft, vdims, p->kern, p->kdims, st));
+            return convn_fft_run(*p->kfft, in, out, st);
+        }
         return convn_mfma(in, vdims, p->kfrag, p->kdims, p->precision, out, st);
+    }
     return convn_same(in, vdims, p->kern, p->kdims, out, st);
 }
 
@@ -1132,8 +1144,13 @@ int sid_psf_set_kernel(sid_psf *p, const float *k, const int64_t kdims[3])
     if (p->stream) SID_TRY_HIP(hipStreamSynchronize(p->stream));
     if (p->kern) (void)hipFree(p->kern);
     if (p->kfrag) (void)hipFree(p->kfrag);
+    if (p->kfft) {
+        convn_fft_free(*p->kfft);
+        delete p->kfft;
+    }
     p->kern = nullptr;
     p->kfrag = nullptr;
+    p->kfft = nullptr;
     p->kdims[0] = p->kdims[1] = p->kdims[2] = 0;
     if (!k) return SID_OK;
     if (!kdims) { set_error("sid_psf_set_kernel: null kdims"); return SID_ERR_ARG; }
@@ -1154,11 +1171,17 @@ int sid_psf_set_kernel(sid_psf *p, const float *k, const int64_t kdims[3])
     return SID_OK;
 }
 
-// convn on the device in FP32 (precision FP32 / FP64) or the BF16-split MFMA form (BF16X2 / BF16X3)
+// convn on the device: exact FP32 (precision FP32 / FP64); BF16X2 / BF16X3 the FP32
+// transform form where it fits, else the BF16-split MFMA form
 static int convn_dispatch(const float *V, const int64_t vdims[3], const float *k_dev, const int kd[3], int precision,
                           float *out, hipStream_t st)
 {
     if (precision == SID_PREC_BF16X2 || precision == SID_PREC_BF16X3) {
+        if (convn_fft_enabled() && convn_fft_fits(vdims, kd)) {
+            int dev = 0;
+            SID_TRY_HIP(hipGetDevice(&dev));
+            return convn_fft_oneshot(dev, V, vdims, k_dev, kd, out, st);
+        }
         u16 *frag = nullptr;
         int rc = convn_build_frag(k_dev, kd, &frag, st);
         if (rc == SID_OK) rc = convn_mfma(V, vdims, frag, kd, precision, out, st);

@@ -46,10 +46,6 @@ namespace sid {
 // data[s * seq_stride + t * elem_stride].  dir = -1 forward, +1 inverse
 // (unnormalised).  tw[t] = exp(dir * 2 pi i t / L) for t in [0, L).
 // ---------------------------------------------------------------------------
-struct FftDesc {
-    int L, nst, spb;
-    int rad[12];
-};
 
 template <int R>
 __device__ __forceinline__ void stockham_stage(const float2 *__restrict__ in, float2 *__restrict__ out, int L, int Ns,
@@ -563,59 +559,6 @@ __global__ void k_fft_sub_scatter(const float2 *__restrict__ subs, const int *__
 // transformed together as z = a + i b (two real 2-D DFTs for the price of one
 // complex one); the Nx' x Ny' grid and its ping-pong copy live in LDS.
 // ---------------------------------------------------------------------------
-
-// One Stockham stage over nseq sequences of length L in LDS, element t of
-// sequence s at s * ss + t * es.  Consecutive threads take consecutive
-// sequences when the sequences are interleaved (es > 1), consecutive elements
-// otherwise, so LDS accesses stay on distinct banks.
-template <int R>
-__device__ __forceinline__ void stockham_stage_2d(const float2 *__restrict__ in, float2 *__restrict__ out, int L,
-                                                  int Ns, const float2 *__restrict__ tw, float sgn, int nseq, int ss,
-                                                  int es)
-{
-    const int nb = L / R;
-    const int tot = nseq * nb;
-    const int step = L / (Ns * R);   // step * k * r < L: no wrap of the twiddle index
-    const float rns = 1.0f / (float)Ns, rseq = 1.0f / (float)nseq;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        // neighbouring lanes take neighbouring lines: with an odd line pitch (or
-        // unit line stride) every read and write of a stage is bank-conflict free
-        const int j = qdiv(e, nseq, rseq), sq = e - j * nseq;
-        const float2 *src = in + sq * ss;
-        float2 *dst = out + sq * ss;
-        const int jq = qdiv(j, Ns, rns), k = j - jq * Ns;
-        const int tstep = step * k;
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float2 x = src[(j + r * nb) * es];
-            v[r] = r == 0 ? x : cmul(x, tw[tstep * r]);
-        }
-        small_dft<R>(v, sgn);
-        const int base = jq * Ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) dst[(base + r * Ns) * es] = v[r];
-    }
-}
-
-// All stages of one direction; returns the buffer holding the result.
-__device__ float2 *fft_pass_2d(float2 *a, float2 *b, const FftDesc &d, const float2 *__restrict__ tw, float sgn,
-                               int nseq, int ss, int es)
-{
-    int Ns = 1;
-    float2 *in = a, *out = b;
-    for (int st = 0; st < d.nst; ++st) {
-        const int R = d.rad[st];
-        if (R == 4) stockham_stage_2d<4>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
-        else if (R == 2) stockham_stage_2d<2>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
-        else if (R == 3) stockham_stage_2d<3>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
-        else stockham_stage_2d<5>(in, out, d.L, Ns, tw, sgn, nseq, ss, es);
-        lds_barrier();
-        Ns *= R;
-        float2 *t = in; in = out; out = t;
-    }
-    return in;
-}
 
 struct SubGeom {
     int nq, n2, Nxp, Nyp, Hy, Lx, Ly, T, S, nz_in;

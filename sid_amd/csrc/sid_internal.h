@@ -194,6 +194,8 @@ struct SensorPlan {
     std::vector<char> fft_use;       // per plane of H: on the FFT path in this plan
 };
 
+struct ConvnFft;   // sid_convn_fft.hip
+
 }  // namespace sid
 
 struct sid_psf {
@@ -228,6 +230,7 @@ struct sid_psf {
     void *lib = nullptr;              // cached library plan (sid_library.hip)
     float *kern = nullptr;            // neuron kernel (device, [kx][ky][kz]); null = none
     sid::u16 *kfrag = nullptr;        // its Toeplitz B fragments for the BF16 MFMA convn
+    sid::ConvnFft *kfft = nullptr;    // its transform form (BF16 handles), built at the first convn
     int kdims[3] = {0, 0, 0};
     float *kwork = nullptr;           // convn staging volume (column-major), kwork_elems floats
     float *hbuf[2] = {nullptr, nullptr};   // device copies of the host-buffer entries' arrays
@@ -348,6 +351,30 @@ int convn_same(const float *in, const int64_t vdims[3], const float *k_dev, cons
 int convn_build_frag(const float *k_dev, const int kd[3], u16 **frag, hipStream_t st);
 int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const int kd[3], int precision, float *out,
                hipStream_t st);
+// sid_convn_fft.hip: the same convolution by 2-D transforms in (x, y) and a
+// direct sum along z between them (FP32 arithmetic; BF16X2 / BF16X3 handles)
+struct ConvnFft {
+    int X = 0, Y = 0, Z = 0, kx = 0, ky = 0, kz = 0;
+    int Nx = 0, Ny = 0, Hx = 0;           // transform lengths, stored x bins Nx / 2 + 1
+    int Yp = 0, Pc = 0;                   // rows the x passes touch (Y up to 16), column pitch
+    FftPlan1D px, py;
+    float2 *kf = nullptr;                 // kernel spectra [kz][Hx][Ny], scaled by 1 / (Nx Ny)
+    float2 *s1 = nullptr, *s2 = nullptr;  // spectra by column, in / out [Z][Hx][Pc]
+    size_t s_elems = 0;                   // capacity of s1 and s2 (complex values each)
+    bool kf_ready = false;
+};
+// the transform form applies to this volume and kernel (sizes, kz <= 8)
+bool convn_fft_fits(const int64_t vdims[3], const int kd[3]);
+// (re)size c for the geometry and transform the kernel (when kf_ready is false)
+int convn_fft_prepare(ConvnFft &c, const int64_t vdims[3], const float *k_dev, const int kd[3], hipStream_t st);
+int convn_fft_run(ConvnFft &c, const float *in, float *out, hipStream_t st);
+void convn_fft_free(ConvnFft &c);
+// SID_CONVN_FFT=0 keeps the BF16 handles on the MFMA form
+bool convn_fft_enabled();
+// handle-free convn on `device` with its per-device work arrays (kept for the
+// process); synchronises st
+int convn_fft_oneshot(int device, const float *in, const int64_t vdims[3], const float *k_dev, const int kd[3],
+                      float *out, hipStream_t st);
 // sid_movie.hip: movie background (rank 1) and std image
 // work: mv_work_bytes(P, T, nullptr, nullptr) bytes of device memory
 struct MvWork;

@@ -52,101 +52,10 @@ typedef _Float16 th8 __attribute__((ext_vector_type(8)));
 
 constexpr int kTfCG = 8;   // channels per transform block (4 real pairs)
 
-// Work item of this block when the grid is 8 * ceil(n / 8) blocks: the
-// hardware deals blocks b, b + 8, ... to one XCD, so each XCD takes a
-// contiguous range of items (neighbouring items share L2 lines: the channel
-// groups of one tile read and write the same 128-byte lines).  -1 past n.
-__device__ __forceinline__ int xcd_item(int n)
-{
-    const int per = (n + 7) >> 3;
-    const int it = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    return it < n ? it : -1;
-}
-static inline unsigned xcd_grid(int64_t n) { return (unsigned)(8 * ((n + 7) / 8)); }
+// xcd_item / xcd_grid (sid_fft_core.h): the channel groups of one tile read
+// and write the same 128-byte lines, so neighbouring items share an XCD.
 
-// ---------------------------------------------------------------------------
-// Register FFT of compile-time length N (2/3/5-smooth): Stockham stages with
-// radices 4, then 2, 3, 5.  All indices are compile-time constants after
-// unrolling, so the sequence stays in registers, and the twiddles
-// exp(sgn 2 pi i t / N) come from a table evaluated by the compiler (a
-// 24-term Taylor series in double on [-pi, pi], multiples of pi/2 exact),
-// which folds into literal operands: no table loads, no scalar registers.
-// ---------------------------------------------------------------------------
-constexpr double kCxPi = 3.14159265358979323846264338327950288;
-constexpr double cx_taylor(double x, bool sine)
-{
-    double term = sine ? x : 1.0, sum = term;
-    for (int k = 1; k < 24; ++k) {
-        term *= -x * x / (sine ? (2.0 * k) * (2.0 * k + 1) : (2.0 * k - 1) * (2.0 * k));
-        sum += term;
-    }
-    return sum;
-}
-template <int N, int SGN>
-struct TwTable {
-    float re[N], im[N];
-    constexpr TwTable() : re(), im()
-    {
-        for (int t = 0; t < N; ++t) {
-            if ((4 * t) % N == 0) {   // quarter turns: exact
-                const int q = 4 * t / N;
-                re[t] = q == 0 ? 1.f : q == 2 ? -1.f : 0.f;
-                im[t] = q == 1 ? (float)SGN : q == 3 ? (float)-SGN : 0.f;
-                continue;
-            }
-            double x = 2.0 * kCxPi * t / N;
-            if (x > kCxPi) x -= 2.0 * kCxPi;
-            re[t] = (float)cx_taylor(x, false);
-            im[t] = (float)(SGN * cx_taylor(x, true));
-        }
-    }
-};
-template <int N, int SGN>
-inline constexpr TwTable<N, SGN> kTw{};
-
-template <int N, int Ns, int R, int SGN>
-__device__ __forceinline__ void rf_stage(float2 (&x)[N])
-{
-    float2 y[N];
-#pragma unroll
-    for (int j = 0; j < N / R; ++j) {
-        const int k = j % Ns;
-        float2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float2 a = x[j + r * (N / R)];
-            const int t = k * r * (N / (Ns * R));
-            v[r] = (r == 0 || k == 0) ? a : cmul(a, make_float2(kTw<N, SGN>.re[t], kTw<N, SGN>.im[t]));
-        }
-        small_dft<R>(v, (float)SGN);
-        const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) y[base + r * Ns] = v[r];
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = y[i];
-}
-
-template <int N, int Ns, int Rem, int SGN>
-struct RegFft {
-    static __device__ __forceinline__ void run(float2 (&x)[N])
-    {
-        constexpr int R = Rem % 4 == 0 ? 4 : Rem % 2 == 0 ? 2 : Rem % 3 == 0 ? 3 : 5;
-        static_assert(Rem % R == 0, "register FFT lengths are 2/3/5-smooth");
-        rf_stage<N, Ns, R, SGN>(x);
-        RegFft<N, Ns * R, Rem / R, SGN>::run(x);
-    }
-};
-template <int N, int Ns, int SGN>
-struct RegFft<N, Ns, 1, SGN> {
-    static __device__ __forceinline__ void run(float2 (&)[N]) {}
-};
-// Unscaled DFT in place: SGN = -1 forward, +1 inverse.
-template <int N, int SGN>
-__device__ __forceinline__ void reg_fft(float2 (&x)[N])
-{
-    RegFft<N, 1, N, SGN>::run(x);
-}
+// Register FFTs (reg_fft, sid_fft_core.h) transform the windows.
 
 // Threads of a transform block: both phases in one pass where they fit.
 template <int T>

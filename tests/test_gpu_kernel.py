@@ -51,10 +51,12 @@ def test_convn_same_vs_oracle(sid, vd, kd):
 
 @pytest.mark.parametrize("prec,bar", [("bf16x2", 2.0 ** -9), ("bf16x3", 2.0 ** -16)])
 @pytest.mark.parametrize("vd,kd", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
-def test_convn_mfma_vs_oracle(sid, vd, kd, prec, bar):
-    """BF16-split MFMA convn (Toeplitz form): the volume is carried as hi + lo
-    BF16 (2^-17 relative), the kernel as BF16 (2^-9 per tap) in BF16X2 or hi + lo
-    in BF16X3; the bars are those per-element bounds."""
+def test_convn_mfma_vs_oracle(sid, vd, kd, prec, bar, monkeypatch):
+    """BF16-split MFMA convn (Toeplitz form, forced by SID_CONVN_FFT=0): the
+    volume is carried as hi + lo BF16 (2^-17 relative), the kernel as BF16
+    (2^-9 per tap) in BF16X2 or hi + lo in BF16X3; the bars are those
+    per-element bounds."""
+    monkeypatch.setenv("SID_CONVN_FFT", "0")
     print("case: convn mfma", prec, vd, kd, flush=True)
     rng = np.random.default_rng(sum(vd) * 11 + sum(kd))
     V = rng.random(vd).astype(np.float32)
@@ -66,9 +68,55 @@ def test_convn_mfma_vs_oracle(sid, vd, kd, prec, bar):
     assert e < bar
 
 
-def test_wrapped_projectors_bf16x2(sid):
-    """A BF16X2 handle with a kernel: projections within the mode's budget."""
-    print("case: wrapped projectors bf16x2", flush=True)
+FFT_CASES = CONV_CASES + [
+    ((3, 2, 2), (65, 1, 1)),         # kernel far wider than the volume
+    ((300, 260, 7), (17, 17, 5)),    # 512-point transforms, the full-size neuron kernel
+    ((1000, 40, 3), (9, 5, 3)),      # 1024-point x transform (32 lanes x 32 points)
+    ((30, 1000, 3), (3, 9, 3)),      # 1024-point y transform
+    ((700, 30, 2), (5, 5, 1)),       # 768-point x transform (16 lanes x 48 points)
+    ((50, 40, 10), (5, 5, 8)),       # the deepest kernel of the transform form
+    ((45, 37, 9), (6, 4, 7)),        # even kernel sides, 128-point transforms
+]
+
+
+@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
+@pytest.mark.parametrize("vd,kd", FFT_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_convn_fft_vs_oracle(sid, vd, kd, prec):
+    """Transform form of convn (the default for BF16X2 / BF16X3 where it fits,
+    transform lengths up to 1024): FP32 2-D DFTs in (x, y) with the z sum
+    between them.  FP32 rounding in the
+    transforms (log2 N stages) stays below 2e-6 rel-L2 of the FP64 direct sum."""
+    print("case: convn fft", prec, vd, kd, flush=True)
+    rng = np.random.default_rng(sum(vd) * 13 + sum(kd))
+    V = rng.random(vd).astype(np.float32)
+    k = rng.random(kd).astype(np.float32)
+    ref = O.convn_same(V.astype(np.float64), k.astype(np.float64))
+    got = sid.convn_same(V, k, precision=prec)
+    e = rel_l2(got, ref)
+    print("convn fft", prec, "rel-L2", e, flush=True)
+    assert e < 2e-6
+
+
+def test_convn_fft_reuse(sid):
+    """The handle-free entry's work arrays are reused across calls: another
+    kernel on the same volume, then a smaller volume, each against the oracle."""
+    print("case: convn fft reuse", flush=True)
+    rng = np.random.default_rng(3)
+    for vd, kd in [((80, 70, 6), (9, 9, 3)), ((80, 70, 6), (5, 3, 5)), ((31, 17, 2), (9, 9, 3))]:
+        V = rng.random(vd).astype(np.float32)
+        k = rng.random(kd).astype(np.float32)
+        ref = O.convn_same(V.astype(np.float64), k.astype(np.float64))
+        assert rel_l2(sid.convn_same(V, k, precision="bf16x2"), ref) < 2e-6
+
+
+@pytest.mark.parametrize("form", ["fft", "mfma"])
+def test_wrapped_projectors_bf16x2(sid, form, monkeypatch):
+    """A BF16X2 handle with a kernel (convn in the transform or the MFMA form):
+    projections within the mode's budget; a second kernel replaces the first
+    (the handle's kernel spectra are rebuilt)."""
+    if form == "mfma":
+        monkeypatch.setenv("SID_CONVN_FFT", "0")
+    print("case: wrapped projectors bf16x2", form, flush=True)
     rng, H, k, F, B = _wrapped_case(seed=6)
     psf = sid.PSF(H.astype(np.float32), precision="bf16x2")
     psf.set_kernel(k.astype(np.float32))
@@ -78,6 +126,12 @@ def test_wrapped_projectors_bf16x2(sid):
     e_b = rel_l2(psf.backward(W.astype(np.float32)), B(W))
     print("wrapped bf16x2 forward", e_f, "backward", e_b, flush=True)
     assert e_f < 2e-3 and e_b < 2e-3
+    k2 = rng.random((3, 7, 2))
+    k2 = (k2 / np.linalg.norm(k2)).astype(np.float32).astype(np.float64)
+    psf.set_kernel(k2.astype(np.float32))
+    e2 = rel_l2(psf.forward(V.astype(np.float32)), c_forward(H, O.convn_same(V, k2)))
+    print("second kernel forward", e2, flush=True)
+    assert e2 < 2e-3
 
 
 def _wrapped_case(seed=5, n=5, X=40, Y=35, M=15, nz=4, kd=(5, 5, 3)):

@@ -698,6 +698,14 @@ def test_convn_same_is_matlab_convn():
     o = [s // 2 for s in k.shape]
     np.testing.assert_allclose(O.convn_same(V, k), full[o[0]:o[0] + 9, o[1]:o[1] + 8, o[2]:o[2] + 5],
                                rtol=1e-12, atol=1e-12)
+    # kernels more than twice the volume's side (taps shifted past the volume)
+    W = rng.random((3, 2, 2))
+    for ks in [(65, 1, 1), (7, 9, 5), (4, 6, 8)]:
+        k = rng.random(ks)
+        full = convolve(W, k, mode="full")
+        o = [s // 2 for s in ks]
+        np.testing.assert_allclose(O.convn_same(W, k), full[o[0]:o[0] + 3, o[1]:o[1] + 2, o[2]:o[2] + 2],
+                                   rtol=1e-12, atol=1e-12)
 
 
 def test_generate_kernel_known_answers():
