@@ -86,14 +86,22 @@ struct TfGeom {
 // channels of s_q T max_x sum_y |w[x][y]|, a bound on every |out| it wrote (at
 // least the window's L1 norm; a maximum, so independent of thread timing).
 // ---------------------------------------------------------------------------
+// Channel stride (float2) of the transform kernels' LDS spectra, n = rows x H
+// padded to 2 mod 8.  A 64-bit LDS access serves 16 lanes per cycle over 16
+// eight-byte slots; the 16 lanes of a row pass are 4 channel pairs x 4 rows
+// (offsets 2 p CS + r H, all distinct mod 16 for odd H when 2 CS = 4 mod 16)
+// and those of a column pass 8 channels x 2 bins (offsets ch CS + k, distinct
+// when CS = 2 mod 8).
+__host__ __device__ constexpr int tf_lds_cs(int n) { return n + ((2 - n % 8) + 8) % 8; }
+
 template <int T>
 __global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restrict__ src,
                                                           const int *__restrict__ slot_plane,
                                                           const float *__restrict__ qscale, TfGeom g,
                                                           float2 *__restrict__ out, float *__restrict__ vmax)
 {
-    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2, NT = tf_threads<T>();
-    extern __shared__ float2 tsm[];   // [CG][T][H]
+    constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2, NT = tf_threads<T>(), CS = tf_lds_cs(T * H);
+    extern __shared__ float2 tsm[];   // [CG][T][H], channel stride CS
     __shared__ unsigned int rmax[CG];   // per channel: largest row sum of |w| (float bits, >= 0)
     const int item = xcd_item(g.nslot * g.ntile * g.ngrp);
     if (item < 0) return;
@@ -109,6 +117,9 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restri
     if (threadIdx.x < CG) rmax[threadIdx.x] = 0u;
     __syncthreads();
     const float *base = src + (int64_t)c * g.XP * g.YP * g.S;
+    // this thread's largest row sums of |w| (channels 2 pair, 2 pair + 1) as float bits: the
+    // unsigned order of non-negative floats, with NaN above Inf (a NaN row poisons the bound)
+    unsigned int ma = 0u, mb = 0u;
     for (int it = threadIdx.x; it < NPAIR * T; it += NT) {
         const int pair = it % NPAIR, x = it / NPAIR;
         const int ch = ch0 + 2 * pair;
@@ -132,15 +143,28 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restri
             sb += fabsf(v.y);
         }
         reg_fft<T, -1>(z);
-        float2 *dA = tsm + ((2 * pair) * T + x) * H, *dB = tsm + ((2 * pair + 1) * T + x) * H;
+        float2 *dA = tsm + (2 * pair) * CS + x * H, *dB = tsm + (2 * pair + 1) * CS + x * H;
 #pragma unroll
         for (int k = 0; k < H; ++k) {
             const float2 zk = z[k], zm = z[(T - k) % T];
             dA[k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             dB[k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
         }
-        atomicMax(&rmax[2 * pair], __float_as_uint(sa));
-        atomicMax(&rmax[2 * pair + 1], __float_as_uint(sb));
+        ma = max(ma, __float_as_uint(sa));
+        mb = max(mb, __float_as_uint(sb));
+    }
+    // lanes with equal lane % NPAIR hold the same pair (NT and 64 are multiples
+    // of NPAIR): reduce over the wave, then one LDS atomic per pair and wave
+    static_assert(NPAIR == 4, "the row-sum reduction assumes 4 pairs per block");
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) {
+        ma = max(ma, (unsigned int)__shfl_xor((int)ma, o));
+        mb = max(mb, (unsigned int)__shfl_xor((int)mb, o));
+    }
+    if ((threadIdx.x & 63) < NPAIR) {
+        const int pair = threadIdx.x & 3;
+        atomicMax(&rmax[2 * pair], ma);
+        atomicMax(&rmax[2 * pair + 1], mb);
     }
     __syncthreads();
     const int64_t bstride = (int64_t)g.ntile * g.kdim, fstep = (int64_t)H * bstride;
@@ -148,7 +172,7 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_in(const float *__restri
         const int ch = it % CG, k = it / CG;
         const int chg = ch0 + ch;
         float2 w[T];
-        const float2 *s = tsm + ch * T * H + k;
+        const float2 *s = tsm + ch * CS + k;
 #pragma unroll
         for (int x = 0; x < T; ++x) w[x] = s[x * H];
         reg_fft<T, -1>(w);
@@ -223,8 +247,8 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_out(const float2 *__rest
                                                            int acc, float *__restrict__ dst, TfTail tl)
 {
     constexpr int H = T / 2 + 1, CG = kTfCG, NPAIR = CG / 2, NT = tf_threads<T>();
-    extern __shared__ float2 tsm[];   // [CG][B][H]
-    const int B = g.B;
+    extern __shared__ float2 tsm[];   // [CG][B][H], channel stride CS
+    const int B = g.B, CS = tf_lds_cs(B * H);
     const int item = xcd_item(g.nslot * g.ntile * g.ngrp);
     if (item < 0) return;
     const int grp = item % g.ngrp, rest = item / g.ngrp;
@@ -259,7 +283,7 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_out(const float2 *__rest
 #pragma unroll
         for (int x = 0; x < T; ++x) {
             const int xr = x - g.off;
-            if (xr >= 0 && xr < B) tsm[(ch * B + xr) * H + k] = w[x];
+            if (xr >= 0 && xr < B) tsm[ch * CS + xr * H + k] = w[x];
         }
     }
     __syncthreads();
@@ -268,7 +292,7 @@ __global__ __launch_bounds__(tf_threads<T>()) void k_tf_out(const float2 *__rest
         const int Px = tx * B + xr;
         const int ch = ch0 + 2 * pair;
         if (Px >= g.Lx || ch >= g.n2) continue;
-        const float2 *sA = tsm + ((2 * pair) * B + xr) * H, *sB = tsm + ((2 * pair + 1) * B + xr) * H;
+        const float2 *sA = tsm + (2 * pair) * CS + xr * H, *sB = tsm + (2 * pair + 1) * CS + xr * H;
         float2 z[T];
 #pragma unroll
         for (int fy = 0; fy < T; ++fy) {
@@ -668,7 +692,7 @@ static const int kTfLens[] = {16, 24, 32, 36, 40, 48, 64};
 template <int T>
 static int tf_set_lds()
 {
-    constexpr size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
+    constexpr size_t lds = sizeof(float2) * kTfCG * tf_lds_cs(T * (T / 2 + 1));   // bounds both kernels' areas
     if (lds > 65536) {
         SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_in<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         SID_TRY_HIP(hipFuncSetAttribute((const void *)k_tf_out<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1018,7 +1042,7 @@ template <int T>
 static int tf_in(const float *src, const int *slot_plane, const float *qscale, const TfGeom &g, float2 *out,
                  float *vmax, hipStream_t st)
 {
-    const size_t lds = sizeof(float2) * kTfCG * T * (T / 2 + 1);
+    const size_t lds = sizeof(float2) * kTfCG * tf_lds_cs(T * (T / 2 + 1));
     hipLaunchKernelGGL(k_tf_in<T>, dim3(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), dim3(tf_threads<T>()), lds, st,
                        src, slot_plane, qscale, g, out, vmax);
     SID_TRY_HIP(hipGetLastError());
@@ -1029,7 +1053,7 @@ template <int T>
 static int tf_out(const float2 *spec, int nsum, int64_t pstride, const int *slot_plane, const float *qscale,
                   const TfGeom &g, float scale, int acc, float *dst, const TfTail &tl, hipStream_t st)
 {
-    const size_t lds = sizeof(float2) * kTfCG * g.B * (T / 2 + 1);
+    const size_t lds = sizeof(float2) * kTfCG * tf_lds_cs(g.B * (T / 2 + 1));
     const dim3 grid(xcd_grid((int64_t)g.nslot * g.ntile * g.ngrp)), block(tf_threads<T>());
     if (nsum > 1)
         hipLaunchKernelGGL((k_tf_out<T, true>), grid, block, lds, st, spec, nsum, pstride, slot_plane, qscale, g, scale,
