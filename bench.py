@@ -322,7 +322,7 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     torch.cuda.synchronize()
     dtw = time.perf_counter() - t0
     psf.set_kernel(None)
-    form = "transform" if cprec != "fp32" and wl.X + k.shape[0] // 2 <= 1024 and wl.Y + k.shape[1] // 2 <= 1024 \
+    form = "transform" if cprec != "fp32" and wl.X + k.shape[0] // 2 <= 2048 and wl.Y + k.shape[1] // 2 <= 2048 \
         and k.shape[2] <= 8 else ("mfma" if cprec != "fp32" else "direct")
     out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_form": form,
                             "convn_mfma_ms": round(tcm * 1e3, 3) if tcm else None,

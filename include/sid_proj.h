@@ -273,7 +273,7 @@ int sid_psf_set_kernel(sid_psf *psf, const float *k, const int64_t kdims[3]);
  * floor(size(k)/2) per dimension, zeros outside V).  precision SID_PREC_FP32 /
  * SID_PREC_FP64: exact FP32 direct sum.  SID_PREC_BF16X2 / SID_PREC_BF16X3:
  * FP32 2-D transforms in (x, y) with the z sum per frequency where
- * X + floor(kx/2) and Y + floor(ky/2) are at most 1024 and kz <= 8 (rel-L2
+ * X + floor(kx/2) and Y + floor(ky/2) are at most 2048 and kz <= 8 (rel-L2
  * about 1e-7); otherwise (or with SID_CONVN_FFT=0 in the environment) BF16-split
  * MFMA: volume hi + lo, kernel BF16 (BF16X2) or hi + lo (BF16X3).  A handle
  * with a kernel uses its own precision the same way.  _device: V, k, out device

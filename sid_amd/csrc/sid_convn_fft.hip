@@ -339,8 +339,8 @@ __global__ void k_cf_kgrid(const float *__restrict__ k, CfDims g, const float2 *
 // Host side
 // ---------------------------------------------------------------------------
 
-// Transform lengths of the group FFT: N = N1 N2 with N2 | N1 <= 48.
-static const int kCfLengths[] = {128, 256, 512, 768, 1024};
+// Transform lengths of the group FFT: N = N1 N2 with N2 | N1 <= 64.
+static const int kCfLengths[] = {128, 256, 512, 768, 1024, 2048};
 
 static int cf_length(int n)
 {
@@ -522,6 +522,7 @@ static int cf_x(const ConvnFft &c, const CfDims &g, bool fwd, const float *in, f
     case 512: return cf_launch_x<32, 16>(c, g, fwd, in, out, st);
     case 768: return cf_launch_x<48, 16>(c, g, fwd, in, out, st);
     case 1024: return cf_launch_x<32, 32>(c, g, fwd, in, out, st);
+    case 2048: return cf_launch_x<64, 32>(c, g, fwd, in, out, st);
     }
     set_error("convn: transform length %d", c.Nx);
     return SID_ERR_DIMS;
@@ -537,6 +538,7 @@ static int cf_y(const ConvnFft &c, const CfDims &g, bool fwd, float2 *buf, hipSt
     case 512: return cf_launch_y<32, 16>(g, tw, fwd, buf, st);
     case 768: return cf_launch_y<48, 16>(g, tw, fwd, buf, st);
     case 1024: return cf_launch_y<32, 32>(g, tw, fwd, buf, st);
+    case 2048: return cf_launch_y<64, 32>(g, tw, fwd, buf, st);
     }
     set_error("convn: transform length %d", c.Ny);
     return SID_ERR_DIMS;

@@ -74,6 +74,8 @@ FFT_CASES = CONV_CASES + [
     ((1000, 40, 3), (9, 5, 3)),      # 1024-point x transform (32 lanes x 32 points)
     ((30, 1000, 3), (3, 9, 3)),      # 1024-point y transform
     ((700, 30, 2), (5, 5, 1)),       # 768-point x transform (16 lanes x 48 points)
+    ((1100, 20, 2), (9, 5, 3)),      # 2048-point x transform (32 lanes x 64 points)
+    ((20, 1100, 2), (3, 9, 2)),      # 2048-point y transform
     ((50, 40, 10), (5, 5, 8)),       # the deepest kernel of the transform form
     ((45, 37, 9), (6, 4, 7)),        # even kernel sides, 128-point transforms
 ]
@@ -83,7 +85,7 @@ FFT_CASES = CONV_CASES + [
 @pytest.mark.parametrize("vd,kd", FFT_CASES, ids=lambda c: "x".join(map(str, c)))
 def test_convn_fft_vs_oracle(sid, vd, kd, prec):
     """Transform form of convn (the default for BF16X2 / BF16X3 where it fits,
-    transform lengths up to 1024): FP32 2-D DFTs in (x, y) with the z sum
+    transform lengths up to 2048): FP32 2-D DFTs in (x, y) with the z sum
     between them.  FP32 rounding in the
     transforms (log2 N stages) stays below 2e-6 rel-L2 of the FP64 direct sum."""
     print("case: convn fft", prec, vd, kd, flush=True)
