@@ -413,7 +413,7 @@ def pmc_summary(fetch_dir, write_dir, bench_out, dest):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--iters", type=int, default=8, help="RL iterations per step")
     ap.add_argument("--workload", default="rl_full")
@@ -516,10 +516,13 @@ def main():
             psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
         torch.cuda.synchronize()
         tim = np.zeros(12)
+        t_steps = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            psf.deconv_rl(htfs[prec], args.iters, htfs[prec])
+            ts = time.perf_counter()
+            psf.deconv_rl(htfs[prec], args.iters, htfs[prec])   # returns after its last iteration's timing events
             tim += np.array(psf.timing())
+            t_steps.append(time.perf_counter() - ts)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         ms_step = (t1 - t0) * 1e3 / args.steps
@@ -532,6 +535,7 @@ def main():
         sup = inf["support"]
         flops_direct = 2.0 * wl.X * wl.Y * sum((hi - lo + 1) ** 2 for (lo, hi), m in zip(sup, meth) if m == "direct")
         res = dict(iter_per_s=args.steps * args.iters / (t1 - t0), ms_per_step=ms_step,
+                   median_step_iter_per_s=args.iters / float(np.median(t_steps)),
                    fwd_gemm_ms=t_f, bwd_gemm_ms=t_b, fwd_fft_ms=t_ff, bwd_fft_ms=t_fb,
                    fwd_gemv_ms=t_gf, bwd_gemv_ms=t_gb,
                    fwd_reduce_ms=tim[1] / n_launch, bwd_reduce_ms=tim[3] / n_launch,
