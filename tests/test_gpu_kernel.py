@@ -180,6 +180,26 @@ def test_wrapped_isra_and_ls_vs_oracle(sid):
     assert e_r < 2e-5 and e_l < 1e-4
 
 
+@pytest.mark.parametrize("form", ["fft", "mfma"])
+def test_wrapped_isra_bf16x2_vs_oracle(sid, form, monkeypatch):
+    """deconvRL on a BF16X2 handle with a kernel (convn in the transform or the
+    MFMA form), against the oracle's FP64 loop composed the same way; the bar is
+    the BF16X2 projector's budget over 4 iterations."""
+    if form == "mfma":
+        monkeypatch.setenv("SID_CONVN_FFT", "0")
+    print("case: wrapped ISRA bf16x2", form, flush=True)
+    rng, H, k, F, B = _wrapped_case(seed=9)
+    psf = sid.PSF(H.astype(np.float32), precision="bf16x2")
+    psf.set_kernel(k.astype(np.float32))
+    y = F(rng.random((40, 35, 4)))
+    htf = B(y).astype(np.float32).astype(np.float64)
+    X_ref = O.deconv_rl(F, B, htf, 4, htf)
+    X = psf.deconv_rl(htf.astype(np.float32), 4, htf.astype(np.float32))
+    e = rel_l2(X, X_ref)
+    print("wrapped isra bf16x2", form, e, flush=True)
+    assert e < 3e-3
+
+
 def test_reconstruct_with_kernel_and_solvers(sid):
     """reconstruct_S GPU branch: Htf = B(img/nrm) (wrapped), lamb_L1 clamp to
     the 1 - 1e-4 quantile of Htf, ISRA or fast_nnls, times nrm."""
