@@ -244,6 +244,7 @@ __global__ __launch_bounds__(256) void k_cf_zsum(CfDims g, const float2 *__restr
         float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (int l = 0; l < KZM; ++l) {
+            if (l >= g.kz) break;   // (a zero weight times a non-finite input would still be NaN)
             const float4 k = kr[l], v = in[j + l];
             o.x += k.x * v.x - k.y * v.y;
             o.y += k.x * v.y + k.y * v.x;

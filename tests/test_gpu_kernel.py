@@ -99,6 +99,23 @@ def test_convn_fft_vs_oracle(sid, vd, kd, prec):
     assert e < 2e-6
 
 
+def test_convn_fft_nonfinite(sid):
+    """Transform form with one NaN voxel: every (x, y) of the output planes the
+    kernel's depth reaches is NaN (an FFT convolution spreads it along the
+    transformed axes); the other planes are finite and match the oracle."""
+    print("case: convn fft nonfinite", flush=True)
+    rng = np.random.default_rng(21)
+    V = rng.random((40, 35, 9)).astype(np.float32)
+    k = rng.random((3, 5, 3)).astype(np.float32)
+    V[5, 7, 4] = np.nan
+    got = sid.convn_same(V, k, precision="bf16x2")
+    assert np.isnan(got[:, :, 3:6]).all()
+    ref = O.convn_same(np.nan_to_num(V, nan=0.0).astype(np.float64), k.astype(np.float64))
+    far = [0, 1, 2, 6, 7, 8]
+    assert np.isfinite(got[:, :, far]).all()
+    assert rel_l2(got[:, :, far], ref[:, :, far]) < 2e-6
+
+
 def test_convn_fft_reuse(sid):
     """The handle-free entry's work arrays are reused across calls: another
     kernel on the same volume, then a smaller volume, each against the oracle."""
