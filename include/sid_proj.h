@@ -280,8 +280,8 @@ int sid_psf_set_kernel(sid_psf *psf, const float *k, const int64_t kdims[3]);
  * pointers on `stream`; the plain variant takes host arrays (device 0).  The
  * transform form keeps its work arrays per device for the process (calls on
  * one device take turns), and a non-finite input value makes NaN every (x, y)
- * of the output planes within the kernel's depth (the direct forms spread it
- * over the kernel's footprint only). */
+ * of the output planes within the kernel's depth (the FP32 direct sum spreads
+ * it over the kernel's footprint only). */
 int sid_convn_same_device(const float *V_dev, const int64_t vdims[3], const float *k_dev, const int64_t kdims[3],
                           int precision, float *out_dev, void *stream);
 int sid_convn_same(const float *V, const int64_t vdims[3], const float *k, const int64_t kdims[3], int precision,
