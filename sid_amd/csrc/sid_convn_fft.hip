@@ -93,13 +93,14 @@ __device__ __forceinline__ void cf_load_tw(float2 *tw, const float2 *__restrict_
 }
 
 // LDS of the x passes: twiddles, then one area used first for the groups'
-// transposes and then for the 8 row pairs' spectra.  Pitch N + 8 float2
-// (8 mod 32): the 8 pairs x 8 bins a wave touches at once fall on 64
-// distinct 8-byte bank slots, two per cycle.
+// transposes and then for the row pairs' spectra.  A 64-bit LDS access serves
+// 16 lanes per cycle over 16 eight-byte slots; the 16 lanes of the split loop
+// are kCfPairs pairs x 16 / kCfPairs bins, so a pitch of N + 16 / kCfPairs
+// (N a multiple of 16) puts them on distinct slots.
 template <int N1, int N2>
 constexpr int cf_x_pitch()
 {
-    return N1 * N2 + 8;
+    return N1 * N2 + 16 / kCfPairs;
 }
 template <int N1, int N2>
 constexpr size_t cf_x_lds()
