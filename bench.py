@@ -216,6 +216,20 @@ def grad_bench(dev, reps=3):
     return res
 
 
+def convn_transform_bytes(X, Y, Z, kshape):
+    """HBM bytes of one transform-form convn (the five passes of sid_convn_fft.hip:
+    x forward, y forward, z sum, y inverse, x inverse) at its transform lengths:
+    the smallest of 128/256/512/768/1024/2048 covering X + kx//2 and Y + ky//2."""
+    kx, ky, kz = kshape
+    ln = [L for L in (128, 256, 512, 768, 1024, 2048)]
+    Nx = next(L for L in ln if L >= max(X + kx // 2, kx))
+    Ny = next(L for L in ln if L >= max(Y + ky // 2, ky))
+    Hx, Yp = Nx // 2 + 1, -(-Y // 16) * 16
+    vol, col = 4 * X * Y * Z, 8 * Z * Hx
+    return (vol + col * Yp) + (col * Y + col * Ny) + (2 * col * Ny + 8 * kz * Hx * Ny) + (col * Ny + col * Yp) \
+        + (col * Yp + vol)
+
+
 def library_bench(psf, wl_lib=None, steps=3):
     """Batched library workload: 2500 seeds (lateral box 7-21 px, depth 3-9
     planes, Gaussian blob sigma 2 px) in the 1005 x 1005 x 51 geometry, one
@@ -324,7 +338,11 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     psf.set_kernel(None)
     form = "transform" if cprec != "fp32" and wl.X + k.shape[0] // 2 <= 2048 and wl.Y + k.shape[1] // 2 <= 2048 \
         and k.shape[2] <= 8 else ("mfma" if cprec != "fp32" else "direct")
+    hbm = convn_transform_bytes(wl.X, wl.Y, wl.nz, k.shape) if form == "transform" else None
     out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_form": form,
+                            "convn_hbm_gb": round(hbm / 1e9, 3) if hbm else None,
+                            "convn_hbm_tbps": round(hbm / tc / 1e12, 3) if hbm else None,
+                            "convn_frac_of_hbm": round(hbm / tc / 8e12, 4) if hbm else None,
                             "convn_mfma_ms": round(tcm * 1e3, 3) if tcm else None,
                             "convn_ms": round(tc * 1e3, 3),
                             "convn_tflops": round(fl / tc / 1e12, 2), "convn_fp32_ms": round(tc32 * 1e3, 3),
@@ -332,8 +350,9 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
                             "convn_frac_of_peak": round(fl / tc / PEAK[cprec], 4),
                             "convn_fp32_frac_of_vector_peak": round(fl / tc32 / 78.6e12, 4),
                             "rl_iter_per_s_wrapped": round(iters / dtw, 3)}
+    rate = f"{hbm / tc / 1e12:.2f} TB/s over its {hbm / 1e9:.2f} GB of passes, " if hbm else ""
     log(f"neuron kernel {k.shape}: convn {cprec} {form} {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s algorithmic, "
-        f"MFMA form {tcm * 1e3 if tcm else float('nan'):.2f} ms), "
+        f"{rate}MFMA form {tcm * 1e3 if tcm else float('nan'):.2f} ms), "
         f"FP32 {tc32 * 1e3:.2f} ms; wrapped RL {iters / dtw:.2f} iter/s")
     del vol
     P, T = wl.X * wl.Y, 1000
