@@ -290,7 +290,7 @@ static int enqueue(sid_psf *p, LibPlan &L, hipStream_t st)
 int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *vals, const int64_t *val_off, int64_t X,
                     int64_t Y, float *out, hipStream_t st, int use_graph)
 {
-    const int64_t n = p->n, S = p->S, NP = p->NP, T = p->T;
+    const int64_t n = p->n, S = p->S, NP = p->NP;
     LibPlan *L = (LibPlan *)p->lib;
     const bool same = L && L->X == X && L->Y == Y && L->vals == vals && L->out == out && L->n_seeds == ns &&
                       L->boxes.size() == (size_t)ns * 6 &&
@@ -315,8 +315,16 @@ int library_forward(sid_psf *p, int64_t ns, const int64_t *boxes, const float *v
             sd.z0 = (int)b[4] - 1; sd.z1 = (int)b[5] - 1;
             sd.sx0 = (int)rect[0] - 1; sd.sx1 = (int)rect[1] - 1; sd.sy0 = (int)rect[2] - 1; sd.sy1 = (int)rect[3] - 1;
             const int qx0 = sd.x0 / (int)n, qx1 = sd.x1 / (int)n, qy0 = sd.y0 / (int)n, qy1 = sd.y1 / (int)n;
-            sd.ax0 = qx0 - (int)T; sd.ay0 = qy0 - (int)T;
-            sd.wx = qx1 - qx0 + 1 + 2 * (int)T; sd.wy = qy1 - qy0 + 1 + 2 * (int)T;
+            // the accumulator spans the taps of the seed's own planes only (output
+            // lenslet Q + D, D in [tf_lo, tf_hi] of a plane), not the widest plane's
+            int t_lo = 0, t_hi = 0;
+            for (int c = std::max(sd.z0, 0); c <= sd.z1 && c < (int)p->nzH; ++c)
+                if (!p->planes[c].empty) {
+                    t_lo = std::min(t_lo, p->planes[c].tf_lo);
+                    t_hi = std::max(t_hi, p->planes[c].tf_hi);
+                }
+            sd.ax0 = qx0 + t_lo; sd.ay0 = qy0 + t_lo;
+            sd.wx = qx1 - qx0 + 1 + t_hi - t_lo; sd.wy = qy1 - qy0 + 1 + t_hi - t_lo;
             sd.val_off = val_off[s];
             sd.acc_off = acc;
             acc += (int64_t)sd.wx * sd.wy * S;
