@@ -78,6 +78,9 @@ FFT_CASES = CONV_CASES + [
     ((20, 1100, 2), (3, 9, 2)),      # 2048-point y transform
     ((50, 40, 10), (5, 5, 8)),       # the deepest kernel of the transform form
     ((45, 37, 9), (6, 4, 7)),        # even kernel sides, 128-point transforms
+    ((1, 1, 1), (1, 1, 1)),          # one voxel, one tap
+    ((7, 1, 3), (3, 1, 3)),          # one row: 15 of the 16 rows of an x block are padding
+    ((5, 3, 2), (2, 2, 2)),          # even 2-tap kernel on a tiny volume
 ]
 
 
