@@ -11,6 +11,7 @@
  *   mex -DSID_MEX_LIBGPU   -output generate_LFM_library_GPU sid_mex_gateway.c -lsidproj
  *   mex -DSID_MEX_LIBCPU   -output generate_LFM_library_CPU sid_mex_gateway.c -lsidproj
  *   mex -DSID_MEX_PSF      -output sidPSF                  sid_mex_gateway.c -lsidproj
+ *     (h = sidPSF('create', H [, CAindex]); 'precision', 'kernel' and 'destroy' act on h)
  *
  * Replaces: external/LFrecon_util/forwardProjectGPU.m:1 (O = forwardProjectGPU(H, realspace)),
  * backwardProjectGPU_new.m:1 (B = backwardProjectGPU_new(H, projection)), forwardProjectACC.m:1 and
@@ -626,11 +627,14 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
     }
 #elif defined(SID_MEX_PSF)
     {
-        /* h = sidPSF('create', H [, CAindex]); sidPSF('destroy', h); sidPSF('precision', h, name) */
+        /* h = sidPSF('create', H [, CAindex]); sidPSF('destroy', h); sidPSF('precision', h, name);
+         * sidPSF('kernel', h, k): the neuron-kernel wrapping of reconstruct_S.m:122-128,184-190 on the
+         * handle (forward F(convn(X, k, 'same')), backward convn(B(y), k, 'same')); k = [] removes it */
         char cmd[32], prec[16];
         (void)nlhs;
         if (nrhs < 1 || !mxIsChar(prhs[0]) || mxGetString(prhs[0], cmd, sizeof cmd) != 0)
-            arg_error("usage: h = sidPSF('create', H [, CAindex]) | sidPSF('destroy', h) | sidPSF('precision', h, name)");
+            arg_error("usage: h = sidPSF('create', H [, CAindex]) | sidPSF('destroy', h) | "
+                      "sidPSF('precision', h, name) | sidPSF('kernel', h, k)");
         if (strcmp(cmd, "create") == 0) {
             int64_t dims[5];
             int owned, rc;
@@ -666,6 +670,24 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[])
             else arg_error("precision must be 'fp32', 'bf16x2', 'bf16x3' or 'fp64'");
             rc = sid_psf_set_precision(handle_arg(prhs[1]), pc);
             if (rc != SID_OK) fail(rc, "sidPSF precision");
+        } else if (strcmp(cmd, "kernel") == 0) {
+            sid_psf *p;
+            int rc, owned = 0;
+            if (nrhs != 3 || !is_real_numeric(prhs[2]))
+                arg_error("usage: sidPSF('kernel', h, k) with k a real [kx, ky, kz] array, or [] to remove it");
+            p = handle_arg(prhs[1]);
+            if (mxGetNumberOfElements(prhs[2]) == 0) {
+                rc = sid_psf_set_kernel(p, NULL, NULL);
+            } else {
+                int64_t kd[3];
+                float *k;
+                if (mxGetNumberOfDimensions(prhs[2]) > 3) arg_error("the kernel must be [kx, ky, kz]");
+                dims_of(prhs[2], kd, 3);
+                k = as_single(prhs[2], &owned);
+                rc = sid_psf_set_kernel(p, k, kd);
+                if (owned) mxFree(k);
+            }
+            if (rc != SID_OK) fail(rc, "sidPSF kernel");
         } else {
             arg_error("sidPSF: unknown command");
         }

@@ -104,6 +104,31 @@ def test_explicit_handles_and_ls_deconv(env):
     assert e.value.ident == "sid:arg"
 
 
+def test_handle_kernel_wrapping(env):
+    """sidPSF('kernel', h, k): the handle's projectors and ISRA become
+    F(convn(X, k, 'same')) / convn(B(y), k, 'same') (reconstruct_S.m:122-128,
+    184-190); k = [] restores the plain projectors."""
+    print("case: MEX sidPSF kernel", flush=True)
+    rng, H, V = _case(6)
+    H64 = H.astype(np.float64)
+    k = rng.random((5, 3, 3))
+    k = (k / np.linalg.norm(k)).astype(np.float32)
+    psf_mex = Mex("sidPSF")
+    _, h = psf_mex("create", H)
+    psf_mex("kernel", h, k, nlhs=0)
+    Vs = V.astype(np.float32)
+    _, O_ = Mex("forwardProjectGPU")(h, Vs)
+    want = c_forward(H64, O.convn_same(Vs.astype(np.float64), k.astype(np.float64)))
+    assert rel_l2(O_, want) < 5e-6
+    img = rng.random((40, 35)).astype(np.float32)
+    _, B_ = Mex("backwardProjectGPU_new")(h, img)
+    assert rel_l2(B_, O.convn_same(c_backward(H64, img), k.astype(np.float64))) < 5e-6
+    psf_mex("kernel", h, np.zeros((0, 0)), nlhs=0)
+    _, O2 = Mex("forwardProjectGPU")(h, Vs)
+    assert rel_l2(O2, c_forward(H64, Vs)) < 2e-6
+    psf_mex("destroy", h, nlhs=0)
+
+
 def test_library_gpu_gateway(env):
     print("case: MEX generate_LFM_library_GPU", flush=True)
     rng = np.random.default_rng(5)

@@ -55,6 +55,10 @@ def test_argument_errors_before_any_device_call():
     assert e.ident == "sid:arg" and "live" in e.msg
     e = _err(Mex("sidPSF"), "resize")
     assert e.ident == "sid:arg"
+    e = _err(Mex("sidPSF"), "kernel", np.array([12345], dtype=np.uint64), np.ones((3, 3, 1)))
+    assert e.ident == "sid:arg" and "live" in e.msg
+    e = _err(Mex("sidPSF"), "kernel", np.array([12345], dtype=np.uint64))
+    assert e.ident == "sid:arg" and "usage" in e.msg
     e = _err(Mex("forwardProjectGPU"), np.array([7], dtype=np.uint64), np.ones((9, 9)))
     assert e.ident == "sid:arg" and "live" in e.msg
     e = _err(Mex("forwardProjectACC"), H, np.ones((9, 9)), np.ones((3, 2)))
