@@ -312,11 +312,12 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
         sid_amd.convn_same(vol, kt, precision="fp32")
     torch.cuda.synchronize()
     tc32 = (time.perf_counter() - t0) / reps
-    # the BF16 handles' transform form against their MFMA form (SID_CONVN_FFT=0)
+    # the transform form against the direct forms (SID_CONVN_FFT=0): the BF16 MFMA
+    # form and the FP32 direct sum
     tcm = None
-    if cprec != "fp32":
-        os.environ["SID_CONVN_FFT"] = "0"
-        try:
+    os.environ["SID_CONVN_FFT"] = "0"
+    try:
+        if cprec != "fp32":
             sid_amd.convn_same(vol, kt, precision=cprec)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -324,8 +325,15 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
                 sid_amd.convn_same(vol, kt, precision=cprec)
             torch.cuda.synchronize()
             tcm = (time.perf_counter() - t0) / reps
-        finally:
-            del os.environ["SID_CONVN_FFT"]
+        sid_amd.convn_same(vol, kt, precision="fp32")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sid_amd.convn_same(vol, kt, precision="fp32")
+        torch.cuda.synchronize()
+        tcd = (time.perf_counter() - t0) / reps
+    finally:
+        del os.environ["SID_CONVN_FFT"]
     taps = int(np.prod(k.shape))
     fl = 2.0 * wl.X * wl.Y * wl.nz * taps
     psf.set_kernel(k)
@@ -336,8 +344,8 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
     torch.cuda.synchronize()
     dtw = time.perf_counter() - t0
     psf.set_kernel(None)
-    form = "transform" if cprec != "fp32" and wl.X + k.shape[0] // 2 <= 2048 and wl.Y + k.shape[1] // 2 <= 2048 \
-        and k.shape[2] <= 8 else ("mfma" if cprec != "fp32" else "direct")
+    form = "transform" if wl.X + k.shape[0] // 2 <= 2048 and wl.Y + k.shape[1] // 2 <= 2048 and k.shape[2] <= 8 \
+        else ("mfma" if cprec != "fp32" else "direct")
     hbm = convn_transform_bytes(wl.X, wl.Y, wl.nz, k.shape) if form == "transform" else None
     out["neuron_kernel"] = {"kernel": list(k.shape), "precision": cprec, "convn_form": form,
                             "convn_hbm_gb": round(hbm / 1e9, 3) if hbm else None,
@@ -346,14 +354,15 @@ def rows_bench(psf, wl, htf, prec, iters, dev):
                             "convn_mfma_ms": round(tcm * 1e3, 3) if tcm else None,
                             "convn_ms": round(tc * 1e3, 3),
                             "convn_tflops": round(fl / tc / 1e12, 2), "convn_fp32_ms": round(tc32 * 1e3, 3),
-                            "convn_fp32_tflops": round(fl / tc32 / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
+                            "convn_fp32_direct_ms": round(tcd * 1e3, 3),
+                            "convn_fp32_direct_tflops": round(fl / tcd / 1e12, 2), "fp32_vector_peak_tflops": 78.6,
                             "convn_frac_of_peak": round(fl / tc / PEAK[cprec], 4),
-                            "convn_fp32_frac_of_vector_peak": round(fl / tc32 / 78.6e12, 4),
+                            "convn_fp32_direct_frac_of_vector_peak": round(fl / tcd / 78.6e12, 4),
                             "rl_iter_per_s_wrapped": round(iters / dtw, 3)}
     rate = f"{hbm / tc / 1e12:.2f} TB/s over its {hbm / 1e9:.2f} GB of passes, " if hbm else ""
     log(f"neuron kernel {k.shape}: convn {cprec} {form} {tc * 1e3:.2f} ms ({fl / tc / 1e12:.1f} TFLOP/s algorithmic, "
         f"{rate}MFMA form {tcm * 1e3 if tcm else float('nan'):.2f} ms), "
-        f"FP32 {tc32 * 1e3:.2f} ms; wrapped RL {iters / dtw:.2f} iter/s")
+        f"FP32 {tc32 * 1e3:.2f} ms (direct sum {tcd * 1e3:.2f} ms); wrapped RL {iters / dtw:.2f} iter/s")
     del vol
     P, T = wl.X * wl.Y, 1000
     Y = torch.rand((T, P), device=dev)

@@ -270,13 +270,13 @@ int sid_library_nsf(sid_psf *psf, int64_t n_seeds, const double *centers, double
  * handle with a kernel (convolve seed volumes first with sid_convn_same_device). */
 int sid_psf_set_kernel(sid_psf *psf, const float *k, const int64_t kdims[3]);
 /* convn(V, k, 'same') on column-major volumes (MATLAB's window: offset
- * floor(size(k)/2) per dimension, zeros outside V).  precision SID_PREC_FP32 /
- * SID_PREC_FP64: exact FP32 direct sum.  SID_PREC_BF16X2 / SID_PREC_BF16X3:
- * FP32 2-D transforms in (x, y) with the z sum per frequency where
- * X + floor(kx/2) and Y + floor(ky/2) are at most 2048 and kz <= 8 (rel-L2
- * about 1e-7); otherwise (or with SID_CONVN_FFT=0 in the environment) BF16-split
- * MFMA: volume hi + lo, kernel BF16 (BF16X2) or hi + lo (BF16X3).  A handle
- * with a kernel uses its own precision the same way.  _device: V, k, out device
+ * floor(size(k)/2) per dimension, zeros outside V).  Where X + floor(kx/2) and
+ * Y + floor(ky/2) are at most 2048 and kz <= 8: FP32 2-D transforms in (x, y)
+ * with the z sum per frequency, at every precision (rel-L2 about 1e-7, below
+ * the FP32 direct sum's rounding).  Otherwise, or with SID_CONVN_FFT=0 in the
+ * environment: SID_PREC_FP32 / SID_PREC_FP64 the FP32 direct sum,
+ * SID_PREC_BF16X2 / SID_PREC_BF16X3 BF16-split MFMA (volume hi + lo, kernel
+ * BF16, or hi + lo for BF16X3).  A handle with a kernel does the same.  _device: V, k, out device
  * pointers on `stream`; the plain variant takes host arrays (device 0).  The
  * transform form keeps its work arrays per device for the process (calls on
  * one device take turns), and a non-finite input value makes NaN every (x, y)

@@ -796,19 +796,18 @@ static int kwork_reserve(sid_psf *p, int64_t n)
     return SID_OK;
 }
 
-// convn(in, k, 'same') for the handle's precision: BF16X2 / BF16X3 take the FP32
-// transform form when it fits (kernel spectra kept on the handle), else the
-// BF16-split MFMA form; FP32 / FP64 take the exact FP32 direct sum
+// convn(in, k, 'same') on a handle: the FP32 transform form where it fits (kernel
+// spectra kept on the handle; its rounding is below the FP32 direct sum's), else
+// the BF16-split MFMA form for BF16X2 / BF16X3 and the FP32 direct sum otherwise
 static int handle_convn(sid_psf *p, const float *in, const int64_t vdims[3], float *out, hipStream_t st)
 {
-    if (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3) {
-        if (convn_fft_enabled() && convn_fft_fits(vdims, p->kdims)) {
-            if (!p->kfft) p->kfft = new ConvnFft();
-            SID_TRY(convn_fft_prepare(*p->kfft, vdims, p->kern, p->kdims, st));
-            return convn_fft_run(*p->kfft, in, out, st);
-        }
-        return convn_mfma(in, vdims, p->kfrag, p->kdims, p->precision, out, st);
+    if (convn_fft_enabled() && convn_fft_fits(vdims, p->kdims)) {
+        if (!p->kfft) p->kfft = new ConvnFft();
+        SID_TRY(convn_fft_prepare(*p->kfft, vdims, p->kern, p->kdims, st));
+        return convn_fft_run(*p->kfft, in, out, st);
     }
+    if (p->precision == SID_PREC_BF16X2 || p->precision == SID_PREC_BF16X3)
+        return convn_mfma(in, vdims, p->kfrag, p->kdims, p->precision, out, st);
     return convn_same(in, vdims, p->kern, p->kdims, out, st);
 }
 
@@ -1171,17 +1170,22 @@ int sid_psf_set_kernel(sid_psf *p, const float *k, const int64_t kdims[3])
     return SID_OK;
 }
 
-// convn on the device: exact FP32 (precision FP32 / FP64); BF16X2 / BF16X3 the FP32
-// transform form where it fits, else the BF16-split MFMA form
+// convn on the device: the FP32 transform form where it fits, else the FP32
+// direct sum (precision FP32 / FP64) or the BF16-split MFMA form (BF16X2 / BF16X3)
 static int convn_dispatch(const float *V, const int64_t vdims[3], const float *k_dev, const int kd[3], int precision,
                           float *out, hipStream_t st)
 {
+    if (precision != SID_PREC_FP32 && precision != SID_PREC_FP64 && precision != SID_PREC_BF16X2 &&
+        precision != SID_PREC_BF16X3) {
+        set_error("unknown precision %d", precision);
+        return SID_ERR_ARG;
+    }
+    if (convn_fft_enabled() && convn_fft_fits(vdims, kd)) {
+        int dev = 0;
+        SID_TRY_HIP(hipGetDevice(&dev));
+        return convn_fft_oneshot(dev, V, vdims, k_dev, kd, out, st);
+    }
     if (precision == SID_PREC_BF16X2 || precision == SID_PREC_BF16X3) {
-        if (convn_fft_enabled() && convn_fft_fits(vdims, kd)) {
-            int dev = 0;
-            SID_TRY_HIP(hipGetDevice(&dev));
-            return convn_fft_oneshot(dev, V, vdims, k_dev, kd, out, st);
-        }
         u16 *frag = nullptr;
         int rc = convn_build_frag(k_dev, kd, &frag, st);
         if (rc == SID_OK) rc = convn_mfma(V, vdims, frag, kd, precision, out, st);
@@ -1191,7 +1195,6 @@ static int convn_dispatch(const float *V, const int64_t vdims[3], const float *k
         if (e != hipSuccess) { set_error("convn: %s", hipGetErrorString(e)); return SID_ERR_HIP; }
         return SID_OK;
     }
-    if (precision != SID_PREC_FP32 && precision != SID_PREC_FP64) { set_error("unknown precision %d", precision); return SID_ERR_ARG; }
     SID_TRY(convn_same(V, vdims, k_dev, kd, out, st));
     SID_TRY_HIP(hipStreamSynchronize(st));
     return SID_OK;

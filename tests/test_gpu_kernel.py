@@ -30,10 +30,15 @@ CONV_CASES = [
 ]
 
 
+@pytest.mark.parametrize("form", ["fft", "direct"])
 @pytest.mark.parametrize("vd,kd", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
-def test_convn_same_vs_oracle(sid, vd, kd):
+def test_convn_same_vs_oracle(sid, vd, kd, form, monkeypatch):
+    """FP32 convn: the transform form (default where it fits) and the direct
+    FP32 sum (SID_CONVN_FFT=0), each within 1e-6 rel-L2 of the FP64 oracle."""
     import torch
-    print("case: convn", vd, kd, flush=True)
+    if form == "direct":
+        monkeypatch.setenv("SID_CONVN_FFT", "0")
+    print("case: convn", form, vd, kd, flush=True)
     rng = np.random.default_rng(sum(vd) * 7 + sum(kd))
     V = rng.random(vd).astype(np.float32)
     k = rng.random(kd).astype(np.float32)
@@ -84,10 +89,10 @@ FFT_CASES = CONV_CASES + [
 ]
 
 
-@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2", "bf16x3"])
 @pytest.mark.parametrize("vd,kd", FFT_CASES, ids=lambda c: "x".join(map(str, c)))
 def test_convn_fft_vs_oracle(sid, vd, kd, prec):
-    """Transform form of convn (the default for BF16X2 / BF16X3 where it fits,
+    """Transform form of convn (the default at every precision where it fits,
     transform lengths up to 2048): FP32 2-D DFTs in (x, y) with the z sum
     between them.  FP32 rounding in the
     transforms (log2 N stages) stays below 2e-6 rel-L2 of the FP64 direct sum."""
@@ -166,8 +171,11 @@ def _wrapped_case(seed=5, n=5, X=40, Y=35, M=15, nz=4, kd=(5, 5, 3)):
     return rng, H, k, F, B
 
 
-def test_wrapped_projectors_vs_oracle(sid):
-    print("case: wrapped projectors n5 X40 Y35 M15 Nz4 k5x5x3", flush=True)
+@pytest.mark.parametrize("form", ["fft", "direct"])
+def test_wrapped_projectors_vs_oracle(sid, form, monkeypatch):
+    if form == "direct":
+        monkeypatch.setenv("SID_CONVN_FFT", "0")
+    print("case: wrapped projectors n5 X40 Y35 M15 Nz4 k5x5x3", form, flush=True)
     rng, H, k, F, B = _wrapped_case()
     psf = sid.PSF(H.astype(np.float32))
     psf.set_kernel(k.astype(np.float32))
