@@ -190,6 +190,19 @@ def test_wrapped_projectors_vs_oracle(sid, form, monkeypatch):
     assert rel_l2(psf.forward(V.astype(np.float32)), c_forward(H, V)) < 5e-6
 
 
+def test_wrapped_projectors_new_volume_size(sid):
+    """One handle with a kernel, two volume sizes in turn (the transform form
+    re-plans its lengths and keeps its larger arrays), then the first again."""
+    print("case: wrapped projectors, volume size changes", flush=True)
+    rng, H, k, F, B = _wrapped_case(seed=12)
+    psf = sid.PSF(H.astype(np.float32))
+    psf.set_kernel(k.astype(np.float32))
+    for shape in [(40, 35, 4), (30, 26, 4), (40, 35, 4)]:
+        V = rng.random(shape)
+        got = psf.forward(V.astype(np.float32))
+        assert rel_l2(got, c_forward(H, O.convn_same(V, k))) < 5e-6, shape
+
+
 def test_wrapped_isra_and_ls_vs_oracle(sid):
     print("case: wrapped ISRA / LS n5 X40 Y35 M15 Nz4 k5x5x3", flush=True)
     rng, H, k, F, B = _wrapped_case(seed=8)
