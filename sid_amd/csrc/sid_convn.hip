@@ -7,8 +7,9 @@
 //   out[x, y, z] = sum_{i,j,l} V[x + ox - i, y + oy - j, z + oz - l] k[i, j, l],
 //   (ox, oy, oz) = floor((kx, ky, kz) / 2), V = 0 outside the volume.
 //
-// FP32 / FP64 handles: k_convn_same, exact FP32.  One block computes a 64 x 16
-// (x, y) output tile of one plane.  For
+// The direct forms here run where the transform form (sid_convn_fft.hip) does
+// not apply or SID_CONVN_FFT=0.  FP32 / FP64: k_convn_same, the FP32 direct
+// sum.  One block computes a 64 x 16 (x, y) output tile of one plane.  For
 // every kernel slice l the (64 + kx - 1) x (16 + ky - 1) input tile of plane
 // z + oz - l is staged in LDS; a thread owns 4 consecutive x outputs and walks
 // a 4-wide register window along the kernel row, so each LDS read feeds four
@@ -87,7 +88,8 @@ __global__ __launch_bounds__(256) void k_convn_same(const float *__restrict__ V,
 }
 
 // ---------------------------------------------------------------------------
-// BF16X2 / BF16X3 handles: the same sum on MFMA (v_mfma_f32_16x16x32_bf16).
+// BF16X2 / BF16X3 beyond the transform form's lengths (sid_convn_fft.hip) or
+// with SID_CONVN_FFT=0: the same sum on MFMA (v_mfma_f32_16x16x32_bf16).
 // For one kernel row (j, l) the convolution along x of 16 output columns is a
 // GEMM with a 32 x 16 Toeplitz matrix of that row:
 //   out[x0 + m, y, z] += sum_t In[x0 + ox - (kx - 1) + t, y + oy - j, z + oz - l] * B[t][m],

@@ -1,5 +1,5 @@
-// convn(V, k, 'same') by transforms, for the BF16X2 / BF16X3 handles: the
-// neuron-kernel wrapping of reconstruct_S.m:122-128,184-190 (forwardFUN =
+// convn(V, k, 'same') by transforms, at every precision where the lengths fit:
+// the neuron-kernel wrapping of reconstruct_S.m:122-128,184-190 (forwardFUN =
 // F(convn(X, k, 'same')), backwardFUN = convn(B(y), k, 'same')) with the
 // 'same' window of MATLAB's convn:
 //   out[x, y, z] = sum_{i,j,l} V[x + ox - i, y + oy - j, z + oz - l] k[i, j, l],
@@ -12,8 +12,9 @@
 // output rows are aligned.  The z sum (kz <= 8 taps) is done directly in the
 // frequency domain before the inverse transforms:
 //   O_z(fx, fy) = sum_l K_l(fx, fy) V^_{z + oz - l}(fx, fy).
-// Everything is FP32 (rel-L2 error about 1e-7 against an exact convolution,
-// where the BF16-split MFMA form has the kernel's 2^-9 or 2^-17 quantisation).
+// Everything is FP32 (rel-L2 error about 1e-7 against an exact convolution:
+// below the FP32 direct sum's rounding, where the BF16-split MFMA form has the
+// kernel's 2^-9 or 2^-17 quantisation).
 //
 // Every transform is a "group FFT": N = N1 N2 points held by N2 lanes of a
 // wave, N1 per lane (element g + N2 m in lane g), an N1-point register FFT,

@@ -230,7 +230,7 @@ struct sid_psf {
     void *lib = nullptr;              // cached library plan (sid_library.hip)
     float *kern = nullptr;            // neuron kernel (device, [kx][ky][kz]); null = none
     sid::u16 *kfrag = nullptr;        // its Toeplitz B fragments for the BF16 MFMA convn
-    sid::ConvnFft *kfft = nullptr;    // its transform form (BF16 handles), built at the first convn
+    sid::ConvnFft *kfft = nullptr;    // its transform form, built at the first convn
     int kdims[3] = {0, 0, 0};
     float *kwork = nullptr;           // convn staging volume (column-major), kwork_elems floats
     float *hbuf[2] = {nullptr, nullptr};   // device copies of the host-buffer entries' arrays
@@ -352,7 +352,7 @@ int convn_build_frag(const float *k_dev, const int kd[3], u16 **frag, hipStream_
 int convn_mfma(const float *in, const int64_t vdims[3], const u16 *frag, const int kd[3], int precision, float *out,
                hipStream_t st);
 // sid_convn_fft.hip: the same convolution by 2-D transforms in (x, y) and a
-// direct sum along z between them (FP32 arithmetic; BF16X2 / BF16X3 handles)
+// direct sum along z between them (FP32 arithmetic, every precision where it fits)
 struct ConvnFft {
     int X = 0, Y = 0, Z = 0, kx = 0, ky = 0, kz = 0;
     int Nx = 0, Ny = 0, Hx = 0;           // transform lengths, stored x bins Nx / 2 + 1
@@ -369,7 +369,7 @@ bool convn_fft_fits(const int64_t vdims[3], const int kd[3]);
 int convn_fft_prepare(ConvnFft &c, const int64_t vdims[3], const float *k_dev, const int kd[3], hipStream_t st);
 int convn_fft_run(ConvnFft &c, const float *in, float *out, hipStream_t st);
 void convn_fft_free(ConvnFft &c);
-// SID_CONVN_FFT=0 keeps the BF16 handles on the MFMA form
+// SID_CONVN_FFT=0 keeps convn on the direct forms (FP32 sum, BF16 MFMA)
 bool convn_fft_enabled();
 // handle-free convn on `device` with its per-device work arrays (kept for the
 // process); synchronises st
