@@ -129,21 +129,42 @@ __global__ __launch_bounds__(256) void k_lib_gemm(const LibItem *__restrict__ it
             f4 acc[NT];
 #pragma unroll
             for (int ct = 0; ct < NT; ++ct) acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
-            for (int kc = 0; kc < LIB_KROW; kc += KC) {
-                if (!((live >> (kc / KC)) & 1u)) continue;
-                const float4 a0 = *(const float4 *)(Arow + kc + 8 * g);
-                const float4 a1 = *(const float4 *)(Arow + kc + 8 * g + 4);
-                __syncthreads();
-                for (int e = threadIdx.x; e < NP * (KC / 4); e += 256) {
+            // the weights of a live chunk: loaded into registers one chunk ahead
+            // (their loads land under the previous chunk's MFMAs), then staged
+            constexpr int WPT = (NP * (KC / 4) + 255) / 256;   // float4 per thread and chunk
+            float4 wreg[WPT];
+            auto wload = [&](int kc) {
+#pragma unroll
+                for (int q = 0; q < WPT; ++q) {
+                    const int e = (int)threadIdx.x + 256 * q;
                     const int col = e / (KC / 4), kq = e - col * (KC / 4);
                     const int kl = kc + 4 * kq;  // local k within the dy block
                     float4 w = {0.f, 0.f, 0.f, 0.f};
-                    if (kl < S) {
+                    if (e < NP * (KC / 4) && kl < S) {
                         const int k = kb + kl;   // multiple of 4: one float4 never crosses a chunk
                         w = *(const float4 *)(Wp + (int64_t)(k / KC) * NP * KC + (int64_t)col * KC + (k % KC));
                     }
-                    *(float4 *)(Bs + col * LDK + 4 * kq) = w;
+                    wreg[q] = w;
                 }
+            };
+            int kc = 0;
+            while (kc < LIB_KROW && !((live >> (kc / KC)) & 1u)) kc += KC;
+            wload(kc);
+            for (; kc < LIB_KROW;) {
+                int kn = kc + KC;   // the next live chunk of this tap
+                while (kn < LIB_KROW && !((live >> (kn / KC)) & 1u)) kn += KC;
+                const float4 a0 = *(const float4 *)(Arow + kc + 8 * g);
+                const float4 a1 = *(const float4 *)(Arow + kc + 8 * g + 4);
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < WPT; ++q) {
+                    const int e = (int)threadIdx.x + 256 * q;
+                    if (e < NP * (KC / 4)) {
+                        const int col = e / (KC / 4), kq = e - col * (KC / 4);
+                        *(float4 *)(Bs + col * LDK + 4 * kq) = wreg[q];
+                    }
+                }
+                if (kn < LIB_KROW) wload(kn);
                 __syncthreads();
 #pragma unroll
                 for (int ct = 0; ct < NT; ++ct) {
@@ -160,6 +181,7 @@ __global__ __launch_bounds__(256) void k_lib_gemm(const LibItem *__restrict__ it
                     c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, b1.w, c, 0, 0, 0);
                     acc[ct] = c;
                 }
+                kc = kn;
             }
             // scatter-add: lane holds C[4g + i][ct*16 + l16]
 #pragma unroll
